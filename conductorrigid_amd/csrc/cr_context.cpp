@@ -1,0 +1,858 @@
+// cr_context.cpp -- host side of the MI355X ConductorRigid force compute: the C ABI
+// declared in include/conductorrigid.h.
+//
+// Plays the role of the reference's host class ConductorRigid
+// (cppmodule/ConductorRigid.cc) below the HOOMD interface: SetParams' parameter
+// and table construction (:116-379), the GMRES driver MatrixVectorSolve
+// (cppmodule/ConductorRigid.cu:1051-1151) and the force pass of gpu_ComputeForce
+// (:1153-1286).  Everything runs on the caller's stream; the solver keeps its basis,
+// Hessenberg matrix and rotations on the device and reads back one scalar per
+// Arnoldi step to decide convergence.
+#include "../../include/conductorrigid.h"
+#include "cr_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace cr;
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+thread_local std::string g_error;
+
+int set_error(int code, const std::string& msg)
+{
+    g_error = msg;
+    return code;
+}
+
+#define CR_HIP(call)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return set_error(CR_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+#define CR_RET(call)                    \
+    do {                                \
+        int r_ = (call);                \
+        if (r_ != CR_OK) return r_;     \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t b)
+    {
+        if (b <= bytes && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, b > 0 ? b : 16);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+bool is_5smooth(int n)
+{
+    if (n < 1) return false;
+    for (int f : {2, 3, 5})
+        while (n % f == 0) n /= f;
+    return n == 1;
+}
+
+std::vector<int> radices(int n)
+{
+    std::vector<int> r;
+    while (n % 4 == 0) { r.push_back(4); n /= 4; }
+    while (n % 2 == 0) { r.push_back(2); n /= 2; }
+    while (n % 3 == 0) { r.push_back(3); n /= 3; }
+    while (n % 5 == 0) { r.push_back(5); n /= 5; }
+    return r;
+}
+
+// 5-smooth integers in [8, 4096] (cppmodule/ConductorRigid.cc:129-158)
+std::vector<int> smooth_list()
+{
+    std::vector<int> v;
+    for (long a = 1; a <= 4096; a *= 2)
+        for (long b = 1; a * b <= 4096; b *= 3)
+            for (long c = 1; a * b * c <= 4096; c *= 5)
+                if (a * b * c >= 8) v.push_back((int)(a * b * c));
+    std::sort(v.begin(), v.end());
+    return v;
+}
+
+double phiq_exact(double dist, double xi)
+{
+    // potential/charge real-space kernel (cppmodule/ConductorRigid.cc:299-327)
+    const double xi2 = xi * xi, dist2 = dist * dist, p15 = std::pow(kPi, 1.5);
+    const double expp = std::exp(-(dist + 2) * (dist + 2) * xi2), expm = std::exp(-(dist - 2) * (dist - 2) * xi2),
+                 exp0 = std::exp(-dist2 * xi2);
+    const double erfp = std::erfc((dist + 2) * xi), erfm = std::erfc((dist - 2) * xi), erf0 = std::erfc(dist * xi);
+    const double exppolyp = -(dist + 2.) / (32. * p15 * xi * dist);
+    const double exppolym = -(dist - 2.) / (32. * p15 * xi * dist);
+    const double exppoly0 = 1. / (16. * p15 * xi);
+    const double erfpolyp = 1. / (64. * kPi * xi2 * dist) * (2. * xi2 * (dist + 2.) * (dist + 2.) + 1.);
+    const double erfpolym = 1. / (64. * kPi * xi2 * dist) * (2. * xi2 * (dist - 2.) * (dist - 2.) + 1.);
+    const double erfpoly0 = -(2. * xi2 * dist2 + 1.) / (32. * kPi * xi2 * dist);
+    const double reg = (dist < 2) ? (-1. / (4. * kPi * dist) + (4. - dist) / (16. * kPi)) : 0.;
+    return exppolyp * expp + exppolym * expm + exppoly0 * exp0 + erfpolyp * erfp + erfpolym * erfm +
+           erfpoly0 * erf0 + reg;
+}
+
+double gradphiq_exact(double dist, double xi)
+{
+    // potential/charge gradient kernel (cppmodule/ConductorRigid.cc:329-345)
+    const double xi2 = xi * xi, dist2 = dist * dist, p15 = std::pow(kPi, 1.5);
+    const double expp = std::exp(-(dist + 2) * (dist + 2) * xi2), expm = std::exp(-(dist - 2) * (dist - 2) * xi2),
+                 exp0 = std::exp(-dist2 * xi2);
+    const double erfp = std::erfc((dist + 2) * xi), erfm = std::erfc((dist - 2) * xi), erf0 = std::erfc(dist * xi);
+    const double exppolyp = -(dist - 2.) / (32. * p15 * xi * dist2);
+    const double exppolym = -(dist + 2.) / (32. * p15 * xi * dist2);
+    const double exppoly0 = 1. / (16. * p15 * xi * dist);
+    const double erfpoly = (2. * xi2 * dist2 - 8. * xi2 - 1.) / (64. * kPi * xi2 * dist2);
+    const double erfpoly0 = -(2. * xi2 * dist2 - 1.) / (32. * kPi * xi2 * dist2);
+    const double reg = (dist < 2) ? (1. / (4. * kPi * dist2) - 1. / (16. * kPi)) : 0.;
+    return exppolyp * expp + exppolym * expm + exppoly0 * exp0 + erfpoly * erfp + erfpoly * erfm + erfpoly0 * erf0 +
+           reg;
+}
+
+}  // namespace
+
+struct cr_context {
+    int device = 0;
+    bool ready = false;
+    cr_params params{};
+    cr_resolved res{};
+    MeshParams mp{};
+    RealParams rp{};
+    int Nzh = 0;
+    long G = 0, Gh = 0;
+    AxisPlan plan_d[3]{}, plan_f[3]{};
+    DevBuf tw_d[3], tw_f[3];
+    DevBuf ktab[3], etab[3];
+    ScaleTables sc{};
+    DevBuf tab_d, gtab_d, tab_f, gtab_f;
+    DevBuf grid_real, grid_cplx;
+    DevBuf phi_recip, iso;
+    // Krylov workspace
+    int R = 10;
+    long ld = 0;
+    int n_cap = 0;
+    DevBuf V, w, b, Mb, x, partial, scal;
+    GmresState* g_dev = nullptr;
+    GmresState* g_host = nullptr;
+    double E0[3] = {0, 0, 0};
+
+    ~cr_context()
+    {
+        if (g_dev) (void)hipFree(g_dev);
+        if (g_host) (void)hipHostFree(g_host);
+    }
+
+    bool fp32() const { return params.precision == CR_PRECISION_FP32_BULK; }
+};
+
+namespace {
+
+SysView make_view(cr_context* c, const cr_system* s)
+{
+    SysView v;
+    v.N = s->N;
+    v.N_rigid = s->N_rigid;
+    v.N_isolated = s->N_isolated;
+    v.Nmax = s->Nmax > 0 ? s->Nmax : 1;
+    v.pos = reinterpret_cast<const double4*>(s->pos);
+    v.iso = c->iso.as<unsigned>();
+    v.body_charge = s->body_charge;
+    v.rigid_membership = s->rigid_membership;
+    v.rigid_members = s->rigid_members;
+    v.rigid_size = s->rigid_size;
+    v.rigid_com = reinterpret_cast<const double4*>(s->rigid_com);
+    v.nlist = s->nlist;
+    v.head_list = s->head_list;
+    v.n_neigh = s->n_neigh;
+    return v;
+}
+
+// Validate the system view, size per-bead buffers and refresh the body-tag map.
+int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out)
+{
+    if (!c) return set_error(CR_ERR_INVALID, "null context");
+    if (!c->ready) return set_error(CR_ERR_STATE, "cr_set_params has not been called");
+    if (!s) return set_error(CR_ERR_INVALID, "null system");
+    if (s->N < 0 || s->N_rigid < 0 || s->N_isolated < 0 || s->N_isolated > s->N + s->N_rigid)
+        return set_error(CR_ERR_INVALID, "inconsistent system sizes");
+    if (s->N > 0 && (!s->pos || !s->tag || !s->body_of_tag || !s->rigid_membership || !s->nlist ||
+                     !s->head_list || !s->n_neigh))
+        return set_error(CR_ERR_INVALID, "missing per-bead array");
+    if (s->N_rigid > 0 && (!s->rigid_members || !s->rigid_size || !s->rigid_com || s->Nmax <= 0))
+        return set_error(CR_ERR_INVALID, "missing rigid-body array");
+    if (reinterpret_cast<uintptr_t>(s->pos) % 16 != 0) return set_error(CR_ERR_INVALID, "pos must be 16-byte aligned");
+    const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
+    CR_HIP(c->phi_recip.ensure(Tsz * (size_t)std::max(1, s->N)));
+    CR_HIP(c->iso.ensure(sizeof(unsigned) * (size_t)std::max(1, s->N)));
+    CR_HIP(launch_membership(s->N, s->tag, s->body_of_tag, c->iso.as<unsigned>(), st));
+    *out = make_view(c, s);
+    return CR_OK;
+}
+
+template <typename T>
+const T* table(const cr_context* c, bool grad)
+{
+    if constexpr (sizeof(T) == sizeof(double))
+        return grad ? c->gtab_d.as<T>() : c->tab_d.as<T>();
+    else
+        return grad ? c->gtab_f.as<T>() : c->tab_f.as<T>();
+}
+
+template <typename T>
+const AxisPlan* plans(const cr_context* c)
+{
+    if constexpr (sizeof(T) == sizeof(double))
+        return c->plan_d;
+    else
+        return c->plan_f;
+}
+
+// grid <- Re IFFT(S FFT(grid)) on the context mesh (cppmodule/ConductorRigid.cu:927-933).
+template <typename T>
+int recip_impl(cr_context* c, hipStream_t st)
+{
+    const AxisPlan* pl = plans<T>(c);
+    const int Nx = c->mp.Nx, Ny = c->mp.Ny;
+    T* real = c->grid_real.as<T>();
+    T* cp = c->grid_cplx.as<T>();
+    CR_HIP(launch_fft_z_r2c<T>(pl[2], Nx, Ny, real, cp, st));
+    CR_HIP(launch_fft_yx<T>(pl[1], 1, Nx, Ny, c->Nzh, cp, -1, st));
+    CR_HIP(launch_fft_x_scale<T>(pl[0], Nx, Ny, c->Nzh, cp, c->sc, st));
+    CR_HIP(launch_fft_yx<T>(pl[1], 1, Nx, Ny, c->Nzh, cp, +1, st));
+    CR_HIP(launch_fft_z_c2r<T>(pl[2], Nx, Ny, cp, real, st));
+    return CR_OK;
+}
+
+template <typename T>
+int spread_impl(cr_context* c, const SysView& v, const double* q, hipStream_t st)
+{
+    CR_HIP(launch_fill<T>(c->grid_real.as<T>(), (T)0, c->G, st));
+    CR_HIP(launch_spread<T>(c->mp, v, q, c->grid_real.as<T>(), st));
+    return CR_OK;
+}
+
+// y = A x, or y = D A x with precond (PotentialChargeMultiply + sum_charges/set_charge,
+// cppmodule/ConductorRigid.cu:862-1020; D from precondition, :816-859).
+template <typename T>
+int matvec_impl(cr_context* c, const SysView& v, const double* x, double* y, int precond, hipStream_t st)
+{
+    CR_RET(spread_impl<T>(c, v, x, st));
+    CR_RET(recip_impl<T>(c, st));
+    CR_HIP(launch_contract<T>(c->mp, v, c->grid_real.as<T>(), c->phi_recip.as<T>(), st));
+    CR_HIP(launch_real_space<T>(c->rp, v, x, c->phi_recip.as<T>(), table<T>(c, false), y, precond, st));
+    CR_HIP(launch_constraint_rows(v, x, y + v.N, precond, st));
+    return CR_OK;
+}
+
+int matvec_any(cr_context* c, const SysView& v, const double* x, double* y, int precond, hipStream_t st)
+{
+    return c->fp32() ? matvec_impl<float>(c, v, x, y, precond, st) : matvec_impl<double>(c, v, x, y, precond, st);
+}
+
+// ||w||^2 into dst (device)
+int norm2(cr_context* c, const double* w, long n, double* dst, hipStream_t st)
+{
+    CR_HIP(launch_multidot(w, n, 1, w, n, c->partial.as<double>(), st));
+    CR_HIP(launch_reduce_partials(c->partial.as<double>(), krylov_blocks(n), 1, dst, st));
+    return CR_OK;
+}
+
+int read_scalar(cr_context* c, const double* dev, double* host, hipStream_t st)
+{
+    CR_HIP(hipMemcpyAsync(host, dev, sizeof(double), hipMemcpyDeviceToHost, st));
+    CR_HIP(hipStreamSynchronize(st));
+    return CR_OK;
+}
+
+// Restarted, left-preconditioned GMRES(R) following CUSP's krylov::gmres structure
+// (cppmodule/ConductorRigid.cu:1134-1136): the cycle starts from w = D(Ax - b), stops
+// when the Givens estimate |s_{i+1}| <= tol ||b||_2, x += V y at the end of each cycle.
+int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st)
+{
+    const long n = (long)v.N + v.N_isolated;
+    const int R = c->R, maxiter = c->res.max_iter;
+    double* V = c->V.as<double>();
+    double* w = c->w.as<double>();
+    double* x = c->x.as<double>();
+    double* Mb = c->Mb.as<double>();
+    double* partial = c->partial.as<double>();
+    double* scal = c->scal.as<double>();
+    GmresState* g = c->g_dev;
+    const int nblk = krylov_blocks(n);
+
+    CR_RET(norm2(c, c->b.as<double>(), n, scal, st));
+    double bnorm2 = 0;
+    CR_RET(read_scalar(c, scal, &bnorm2, st));
+    const double tolerance = c->res.gmres_tol * std::sqrt(bnorm2);
+
+    int iter = 0, matvecs = 0, restarts = 0, converged = 0;
+    double resid = 0;
+    for (;;) {
+        CR_RET(matvec_any(c, v, x, w, 1, st));
+        ++matvecs;
+        CR_HIP(launch_axpby(-1.0, Mb, 1.0, w, n, st));
+        CR_RET(norm2(c, w, n, scal, st));
+        CR_HIP(launch_cycle_init(g, scal, st));
+        CR_RET(read_scalar(c, &g->beta, &resid, st));
+        if (!std::isfinite(resid)) return set_error(CR_ERR_NOT_CONVERGED, "non-finite residual in GMRES");
+        if (resid <= tolerance) {
+            converged = 1;
+            break;
+        }
+        if (iter >= maxiter) break;
+        ++restarts;
+        CR_HIP(launch_scale_copy(w, &g->inv_beta, -1.0, V, n, st));
+        int i = -1;
+        do {
+            ++i;
+            ++iter;
+            double* vi = V + (long)i * c->ld;
+            CR_RET(matvec_any(c, v, vi, w, 1, st));
+            ++matvecs;
+            // CGS2: two projections onto V[:,0:i+1]
+            CR_HIP(launch_multidot(V, c->ld, i + 1, w, n, partial, st));
+            CR_HIP(launch_reduce_partials(partial, nblk, i + 1, g->h1, st));
+            CR_HIP(launch_gs_update(V, c->ld, i + 1, g->h1, w, n, 0, partial, st));
+            CR_HIP(launch_multidot(V, c->ld, i + 1, w, n, partial, st));
+            CR_HIP(launch_reduce_partials(partial, nblk, i + 1, g->h2, st));
+            CR_HIP(launch_gs_update(V, c->ld, i + 1, g->h2, w, n, 2, partial, st));
+            CR_HIP(launch_reduce_partials(partial, nblk, 1, &g->wnorm2, st));
+            CR_HIP(launch_givens(g, i, R, st));
+            CR_HIP(launch_scale_copy(w, &g->inv_norm, 1.0, V + (long)(i + 1) * c->ld, n, st));
+            CR_RET(read_scalar(c, &g->resid, &resid, st));
+            if (!std::isfinite(resid)) return set_error(CR_ERR_NOT_CONVERGED, "non-finite residual in GMRES");
+            if (resid <= tolerance) {
+                converged = 1;
+                break;
+            }
+        } while (i + 1 < R && iter + 1 <= maxiter);
+        CR_HIP(launch_backsub(g, R, st));
+        CR_HIP(launch_combine(V, c->ld, g->y, &g->ncols, x, n, st));
+        if (converged || iter >= maxiter) break;
+    }
+    if (stats) {
+        stats->iterations = iter;
+        stats->matvecs = matvecs;
+        stats->restarts = restarts;
+        stats->converged = converged;
+        stats->resid = resid;
+        stats->tolerance = tolerance;
+        stats->true_resid = -1.0;
+    }
+    return CR_OK;
+}
+
+template <typename T>
+int force_pass(cr_context* c, const SysView& v, const double* q, double* force, hipStream_t st)
+{
+    CR_RET(spread_impl<T>(c, v, q, st));
+    CR_RET(recip_impl<T>(c, st));
+    CR_HIP(launch_contract_force<T>(c->mp, v, q, c->grid_real.as<T>(), force, st));
+    CR_HIP(launch_real_space_force<T>(c->rp, v, q, table<T>(c, true), c->E0, force, st));
+    return CR_OK;
+}
+
+template <typename T>
+hipError_t upload_twiddles(DevBuf& buf, int n)
+{
+    std::vector<T> h(2 * (size_t)n);
+    for (int e = 0; e < n; ++e) {
+        const double a = -2.0 * kPi * (double)e / (double)n;
+        h[2 * e] = (T)std::cos(a);
+        h[2 * e + 1] = (T)std::sin(a);
+    }
+    hipError_t err = buf.ensure(sizeof(T) * h.size());
+    if (err != hipSuccess) return err;
+    return hipMemcpy(buf.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+}
+
+template <typename T>
+hipError_t upload(DevBuf& buf, const std::vector<double>& h)
+{
+    std::vector<T> t(h.begin(), h.end());
+    hipError_t err = buf.ensure(sizeof(T) * std::max<size_t>(1, t.size()));
+    if (err != hipSuccess) return err;
+    return hipMemcpy(buf.p, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cr_last_error(void) { return g_error.c_str(); }
+
+int cr_create(cr_context** out, int device)
+{
+    if (!out) return set_error(CR_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    CR_HIP(hipSetDevice(device));
+    cr_context* c = new cr_context();
+    c->device = device;
+    if (hipMalloc((void**)&c->g_dev, sizeof(GmresState)) != hipSuccess ||
+        hipHostMalloc((void**)&c->g_host, sizeof(GmresState), hipHostMallocDefault) != hipSuccess) {
+        delete c;
+        return set_error(CR_ERR_DEVICE, "allocation of solver state failed");
+    }
+    *out = c;
+    return CR_OK;
+}
+
+void cr_destroy(cr_context* ctx) { delete ctx; }
+
+int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated)
+{
+    if (!c || !p) return set_error(CR_ERR_INVALID, "null argument");
+    if (!(p->xi > 0) || !(p->errortol > 0 && p->errortol < 1)) return set_error(CR_ERR_INVALID, "xi > 0 and 0 < errortol < 1 required");
+    for (int d = 0; d < 3; ++d)
+        if (!(p->L[d] > 0)) return set_error(CR_ERR_INVALID, "box lengths must be positive");
+    if (p->precision != CR_PRECISION_FP64 && p->precision != CR_PRECISION_FP32_BULK)
+        return set_error(CR_ERR_INVALID, "unknown precision");
+    if (n_beads < 0 || n_isolated < 0) return set_error(CR_ERR_INVALID, "negative sizes");
+    CR_HIP(hipSetDevice(c->device));
+    c->ready = false;
+    c->params = *p;
+    cr_resolved& r = c->res;
+    std::memset(&r, 0, sizeof(r));
+    const double xi = p->xi, tol = p->errortol;
+    // ConductorRigid.cc:123-207
+    r.rc = (p->rc > 0) ? p->rc : std::sqrt(-std::log(tol)) / xi;
+    r.kcut = 2.0 * xi * std::sqrt(-std::log(tol));
+    const std::vector<int> sm = smooth_list();
+    for (int d = 0; d < 3; ++d) {
+        if (p->mesh[d] > 0) {
+            r.mesh[d] = p->mesh[d];
+        } else {
+            int m = (int)std::ceil(1.0 + p->L[d] * r.kcut / kPi);
+            for (int s : sm)
+                if (m <= s) {
+                    m = s;
+                    break;
+                }
+            r.mesh[d] = m;
+        }
+        if (r.mesh[d] > 4096) return set_error(CR_ERR_INVALID, "mesh exceeds 4096 nodes per dimension");
+        if (!is_5smooth(r.mesh[d])) return set_error(CR_ERR_INVALID, "mesh sizes must be 5-smooth (2^a 3^b 5^c)");
+        r.h[d] = p->L[d] / r.mesh[d];
+    }
+    int P = (p->P > 0) ? p->P : (int)std::ceil(-2.0 * std::log(tol) / kPi);
+    for (int d = 0; d < 3; ++d) r.eta[d] = P * r.h[d] * r.h[d] * xi * xi / kPi;
+    P = std::min(P, std::min(r.mesh[0], std::min(r.mesh[1], r.mesh[2])));
+    r.P = P;
+    r.drtable = 0.001;
+    r.N_table = (int)(r.rc / r.drtable - 1);
+    if (r.N_table < 2) return set_error(CR_ERR_INVALID, "real-space cutoff too small");
+    r.xiterm = 2.0 * xi * xi;
+    r.prefac = r.xiterm * xi / kPi * std::sqrt(2.0 / (kPi * r.eta[0] * r.eta[1] * r.eta[2]));
+    r.selfterm = (1.0 - std::exp(-4.0 * xi * xi)) / (8.0 * kPi * std::sqrt(kPi) * xi) + std::erfc(2.0 * xi) / (4.0 * kPi);
+    r.quadW = r.h[0] * r.h[1] * r.h[2];
+    r.precision = p->precision;
+    r.restart = (p->restart > 0) ? p->restart : 10;
+    r.max_iter = (p->max_iter > 0) ? p->max_iter : 100;
+    r.gmres_tol = (p->gmres_tol > 0) ? p->gmres_tol : tol;
+    if (r.restart > 32) return set_error(CR_ERR_INVALID, "restart must be <= 32");
+    const int max_n = (p->precision == CR_PRECISION_FP64) ? 2048 : 4096;
+    for (int d = 0; d < 3; ++d)
+        if (r.mesh[d] > max_n) return set_error(CR_ERR_INVALID, "mesh too large for the LDS-resident FFT in this precision");
+
+    MeshParams& mp = c->mp;
+    mp.Nx = r.mesh[0];
+    mp.Ny = r.mesh[1];
+    mp.Nz = r.mesh[2];
+    mp.P = r.P;
+    for (int d = 0; d < 3; ++d) {
+        mp.L[d] = p->L[d];
+        mp.h[d] = r.h[d];
+        mp.eta[d] = r.eta[d];
+    }
+    mp.prefac = r.prefac;
+    mp.xiterm = r.xiterm;
+    mp.quadW = r.quadW;
+    RealParams& rp = c->rp;
+    for (int d = 0; d < 3; ++d) rp.L[d] = p->L[d];
+    rp.rc = r.rc;
+    rp.rc2 = r.rc * r.rc;
+    rp.drtable = r.drtable;
+    rp.rmin2 = r.drtable * r.drtable;
+    rp.selfterm = r.selfterm;
+    rp.N_table = r.N_table;
+
+    // real-space tables (ConductorRigid.cc:276-354)
+    const int nt = r.N_table;
+    std::vector<double> tab(2 * (size_t)(nt + 1), 0.0), gtab(2 * (size_t)(nt + 1), 0.0);
+    for (int i = 0; i <= nt; ++i) {
+        const double dist = (i + 1) * r.drtable;
+        tab[2 * i] = phiq_exact(dist, xi);
+        gtab[2 * i] = gradphiq_exact(dist, xi);
+    }
+    for (int i = 0; i < nt; ++i) {
+        tab[2 * i + 1] = tab[2 * (i + 1)];
+        gtab[2 * i + 1] = gtab[2 * (i + 1)];
+    }
+    CR_HIP(upload<double>(c->tab_d, tab));
+    CR_HIP(upload<double>(c->gtab_d, gtab));
+    CR_HIP(upload<float>(c->tab_f, tab));
+    CR_HIP(upload<float>(c->gtab_f, gtab));
+
+    // FFT plans and k-space factor tables (ConductorRigid.cc:245-274)
+    c->Nzh = mp.Nz / 2 + 1;
+    c->G = (long)mp.Nx * mp.Ny * mp.Nz;
+    c->Gh = (long)mp.Nx * mp.Ny * c->Nzh;
+    const int Nd[3] = {mp.Nx, mp.Ny, mp.Nz};
+    for (int d = 0; d < 3; ++d) {
+        std::vector<int> rad = radices(Nd[d]);
+        if (rad.size() > 24) return set_error(CR_ERR_INVALID, "too many FFT stages");
+        CR_HIP(upload_twiddles<double>(c->tw_d[d], Nd[d]));
+        CR_HIP(upload_twiddles<float>(c->tw_f[d], Nd[d]));
+        for (AxisPlan* ap : {&c->plan_d[d], &c->plan_f[d]}) {
+            ap->n = Nd[d];
+            ap->nstages = (int)rad.size();
+            for (size_t s = 0; s < rad.size(); ++s) ap->radix[s] = rad[s];
+        }
+        c->plan_d[d].tw = c->tw_d[d].p;
+        c->plan_f[d].tw = c->tw_f[d].p;
+        std::vector<double> kv(Nd[d]), ev(Nd[d]);
+        for (int i = 0; i < Nd[d]; ++i) {
+            const double k = ((i < (Nd[d] + 1) / 2) ? i : i - Nd[d]) * 2.0 * kPi / p->L[d];
+            kv[i] = k;
+            ev[i] = std::exp(-(1.0 - r.eta[d]) * k * k / (4.0 * xi * xi));
+        }
+        CR_HIP(upload<double>(c->ktab[d], kv));
+        CR_HIP(upload<double>(c->etab[d], ev));
+        c->sc.k[d] = c->ktab[d].as<double>();
+        c->sc.e[d] = c->etab[d].as<double>();
+    }
+    c->sc.invG = 1.0 / (double)c->G;
+    const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
+    CR_HIP(c->grid_real.ensure(Tsz * (size_t)c->G));
+    CR_HIP(c->grid_cplx.ensure(2 * Tsz * (size_t)c->Gh));
+
+    // Krylov workspace: basis of R+1 columns of n = N + N_isolated doubles
+    c->R = r.restart;
+    const long n = (long)n_beads + n_isolated;
+    c->n_cap = (int)n;
+    c->ld = ((std::max<long>(n, 1) + 63) / 64) * 64;
+    CR_HIP(c->V.ensure(sizeof(double) * (size_t)c->ld * (c->R + 1)));
+    CR_HIP(c->w.ensure(sizeof(double) * (size_t)c->ld));
+    CR_HIP(c->b.ensure(sizeof(double) * (size_t)c->ld));
+    CR_HIP(c->Mb.ensure(sizeof(double) * (size_t)c->ld));
+    CR_HIP(c->x.ensure(sizeof(double) * (size_t)c->ld));
+    CR_HIP(c->partial.ensure(sizeof(double) * 32 * 1024));
+    CR_HIP(c->scal.ensure(sizeof(double) * 64));
+    c->ready = true;
+    return CR_OK;
+}
+
+int cr_get_resolved(const cr_context* c, cr_resolved* out)
+{
+    if (!c || !out) return set_error(CR_ERR_INVALID, "null argument");
+    if (!c->ready) return set_error(CR_ERR_STATE, "cr_set_params has not been called");
+    *out = c->res;
+    return CR_OK;
+}
+
+int cr_update_field(cr_context* c, const double E0[3])
+{
+    if (!c || !E0) return set_error(CR_ERR_INVALID, "null argument");
+    for (int d = 0; d < 3; ++d) c->E0[d] = E0[d];
+    return CR_OK;
+}
+
+int cr_zero_force(cr_context* c, int N, double* force, void* stream)
+{
+    if (!c || (N > 0 && !force)) return set_error(CR_ERR_INVALID, "null argument");
+    CR_HIP(launch_zero_force(N, force, (hipStream_t)stream));
+    return CR_OK;
+}
+
+int cr_objective(cr_context* c, const cr_system* s, double* b, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    SysView v;
+    CR_RET(prepare(c, s, st, &v));
+    CR_HIP(launch_objective(v, c->mp.L, c->E0, b, st));
+    return CR_OK;
+}
+
+int cr_precondition(cr_context* c, const cr_system* s, const double* x, double* y, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    SysView v;
+    CR_RET(prepare(c, s, st, &v));
+    CR_HIP(launch_precondition(v, x, y, st));
+    return CR_OK;
+}
+
+int cr_matvec(cr_context* c, const cr_system* s, const double* x, double* y, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    SysView v;
+    CR_RET(prepare(c, s, st, &v));
+    return matvec_any(c, v, x, y, 0, st);
+}
+
+int cr_compute_force(cr_context* c, const cr_system* s, double* q_inout, double* phi_inout, double* force,
+                     int x0_mode, cr_solve_stats* stats, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    SysView v;
+    CR_RET(prepare(c, s, st, &v));
+    const long n = (long)v.N + v.N_isolated;
+    if (n > c->n_cap) return set_error(CR_ERR_INVALID, "system larger than the size given to cr_set_params");
+    if ((v.N > 0 && (!q_inout || !force)) || (v.N_isolated > 0 && !phi_inout))
+        return set_error(CR_ERR_INVALID, "null output array");
+    double* x = c->x.as<double>();
+    // objective and its preconditioned copy (MatrixVectorSolve, cppmodule/ConductorRigid.cu:1099-1107)
+    CR_HIP(launch_objective(v, c->mp.L, c->E0, c->b.as<double>(), st));
+    CR_HIP(launch_precondition(v, c->b.as<double>(), c->Mb.as<double>(), st));
+    if (x0_mode == CR_X0_ZERO) {
+        CR_HIP(launch_fill<double>(x, 0.0, n, st));
+    } else if (x0_mode == CR_X0_PREVIOUS) {
+        CR_HIP(hipMemcpyAsync(x, q_inout, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
+        CR_HIP(hipMemcpyAsync(x + v.N, phi_inout, sizeof(double) * v.N_isolated, hipMemcpyDeviceToDevice, st));
+    } else {
+        return set_error(CR_ERR_INVALID, "unknown x0 mode");
+    }
+    CR_RET(gmres(c, v, stats, st));
+    CR_HIP(hipMemcpyAsync(q_inout, x, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
+    CR_HIP(hipMemcpyAsync(phi_inout, x + v.N, sizeof(double) * v.N_isolated, hipMemcpyDeviceToDevice, st));
+    if (stats) {
+        // FP64 judge of the converged state: ||D (b - A x)||_2
+        double* w = c->w.as<double>();
+        CR_RET(matvec_any(c, v, x, w, 1, st));
+        CR_HIP(launch_axpby(-1.0, c->Mb.as<double>(), 1.0, w, n, st));
+        CR_RET(norm2(c, w, n, c->scal.as<double>(), st));
+        double r2 = 0;
+        CR_RET(read_scalar(c, c->scal.as<double>(), &r2, st));
+        stats->true_resid = std::sqrt(r2);
+    }
+    if (c->fp32())
+        CR_RET(force_pass<float>(c, v, x, force, st));
+    else
+        CR_RET(force_pass<double>(c, v, x, force, st));
+    return CR_OK;
+}
+
+int cr_spread(cr_context* c, const cr_system* s, const double* q, double* grid_out, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    SysView v;
+    CR_RET(prepare(c, s, st, &v));
+    if (c->fp32()) {
+        CR_RET(spread_impl<float>(c, v, q, st));
+        if (grid_out) CR_HIP((launch_convert<float, double>(c->grid_real.as<float>(), grid_out, c->G, st)));
+    } else {
+        CR_RET(spread_impl<double>(c, v, q, st));
+        if (grid_out) CR_HIP((launch_convert<double, double>(c->grid_real.as<double>(), grid_out, c->G, st)));
+    }
+    return CR_OK;
+}
+
+int cr_recip_grid(cr_context* c, const double* grid_in, double* grid_out, void* stream)
+{
+    if (!c || !c->ready) return set_error(CR_ERR_STATE, "cr_set_params has not been called");
+    hipStream_t st = (hipStream_t)stream;
+    if (c->fp32()) {
+        if (grid_in) CR_HIP((launch_convert<double, float>(grid_in, c->grid_real.as<float>(), c->G, st)));
+        CR_RET(recip_impl<float>(c, st));
+        if (grid_out) CR_HIP((launch_convert<float, double>(c->grid_real.as<float>(), grid_out, c->G, st)));
+    } else {
+        if (grid_in) CR_HIP((launch_convert<double, double>(grid_in, c->grid_real.as<double>(), c->G, st)));
+        CR_RET(recip_impl<double>(c, st));
+        if (grid_out) CR_HIP((launch_convert<double, double>(c->grid_real.as<double>(), grid_out, c->G, st)));
+    }
+    return CR_OK;
+}
+
+int cr_fft_c2c(cr_context* c, const double* in, double* out, int sign, void* stream)
+{
+    if (!c || !c->ready) return set_error(CR_ERR_STATE, "cr_set_params has not been called");
+    if (!in || !out || (sign != 1 && sign != -1)) return set_error(CR_ERR_INVALID, "bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    const long n2 = 2 * c->G;
+    const int Nx = c->mp.Nx, Ny = c->mp.Ny, Nz = c->mp.Nz;
+    DevBuf tmp;
+    if (c->fp32()) {
+        CR_HIP(tmp.ensure(sizeof(float) * (size_t)n2));
+        float* t = tmp.as<float>();
+        CR_HIP((launch_convert<double, float>(in, t, n2, st)));
+        CR_HIP(launch_fft_z_c2c<float>(c->plan_f[2], Nx, Ny, t, sign, st));
+        CR_HIP(launch_fft_yx<float>(c->plan_f[1], 1, Nx, Ny, Nz, t, sign, st));
+        CR_HIP(launch_fft_yx<float>(c->plan_f[0], 0, Nx, Ny, Nz, t, sign, st));
+        CR_HIP((launch_convert<float, double>(t, out, n2, st)));
+    } else {
+        CR_HIP(tmp.ensure(sizeof(double) * (size_t)n2));
+        double* t = tmp.as<double>();
+        CR_HIP((launch_convert<double, double>(in, t, n2, st)));
+        CR_HIP(launch_fft_z_c2c<double>(c->plan_d[2], Nx, Ny, t, sign, st));
+        CR_HIP(launch_fft_yx<double>(c->plan_d[1], 1, Nx, Ny, Nz, t, sign, st));
+        CR_HIP(launch_fft_yx<double>(c->plan_d[0], 0, Nx, Ny, Nz, t, sign, st));
+        CR_HIP((launch_convert<double, double>(t, out, n2, st)));
+    }
+    CR_HIP(hipStreamSynchronize(st));
+    return CR_OK;
+}
+
+int cr_contract(cr_context* c, const cr_system* s, double* phi, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    SysView v;
+    CR_RET(prepare(c, s, st, &v));
+    if (c->fp32()) {
+        CR_HIP(launch_contract<float>(c->mp, v, c->grid_real.as<float>(), c->phi_recip.as<float>(), st));
+        CR_HIP((launch_convert<float, double>(c->phi_recip.as<float>(), phi, v.N, st)));
+    } else {
+        CR_HIP(launch_contract<double>(c->mp, v, c->grid_real.as<double>(), phi, st));
+    }
+    return CR_OK;
+}
+
+int cr_contract_force(cr_context* c, const cr_system* s, const double* q, double* force, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    SysView v;
+    CR_RET(prepare(c, s, st, &v));
+    if (c->fp32())
+        CR_HIP(launch_contract_force<float>(c->mp, v, q, c->grid_real.as<float>(), force, st));
+    else
+        CR_HIP(launch_contract_force<double>(c->mp, v, q, c->grid_real.as<double>(), force, st));
+    return CR_OK;
+}
+
+int cr_real_space(cr_context* c, const cr_system* s, const double* x, double* y, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    SysView v;
+    CR_RET(prepare(c, s, st, &v));
+    if (c->fp32()) {
+        CR_HIP((launch_convert<double, float>(y, c->phi_recip.as<float>(), v.N, st)));
+        CR_HIP(launch_real_space<float>(c->rp, v, x, c->phi_recip.as<float>(), table<float>(c, false), y, 0, st));
+    } else {
+        CR_HIP((launch_convert<double, double>(y, c->phi_recip.as<double>(), v.N, st)));
+        CR_HIP(launch_real_space<double>(c->rp, v, x, c->phi_recip.as<double>(), table<double>(c, false), y, 0, st));
+    }
+    return CR_OK;
+}
+
+int cr_real_space_force(cr_context* c, const cr_system* s, const double* q, double* force, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    SysView v;
+    CR_RET(prepare(c, s, st, &v));
+    if (c->fp32())
+        CR_HIP(launch_real_space_force<float>(c->rp, v, q, table<float>(c, true), c->E0, force, st));
+    else
+        CR_HIP(launch_real_space_force<double>(c->rp, v, q, table<double>(c, true), c->E0, force, st));
+    return CR_OK;
+}
+
+int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int reps, double* stage_ms,
+                      int max_stages, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    SysView v;
+    CR_RET(prepare(c, s, st, &v));
+    if (reps < 1 || !stage_ms || !x) return set_error(CR_ERR_INVALID, "bad argument");
+    const int nst = CR_NUM_STAGES;
+    hipEvent_t ev[nst + 1];
+    for (auto& e : ev) CR_HIP(hipEventCreate(&e));
+    double acc[nst] = {0};
+    double* y = c->w.as<double>();
+    for (int r = 0; r < reps; ++r) {
+        CR_HIP(hipEventRecord(ev[0], st));
+        if (c->fp32()) {
+            using T = float;
+            const AxisPlan* pl = plans<T>(c);
+            T* real = c->grid_real.as<T>();
+            T* cp = c->grid_cplx.as<T>();
+            CR_HIP(launch_fill<T>(real, (T)0, c->G, st));
+            CR_HIP(hipEventRecord(ev[1], st));
+            CR_HIP(launch_spread<T>(c->mp, v, x, real, st));
+            CR_HIP(hipEventRecord(ev[2], st));
+            CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st));
+            CR_HIP(hipEventRecord(ev[3], st));
+            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, -1, st));
+            CR_HIP(hipEventRecord(ev[4], st));
+            CR_HIP(launch_fft_x_scale<T>(pl[0], c->mp.Nx, c->mp.Ny, c->Nzh, cp, c->sc, st));
+            CR_HIP(hipEventRecord(ev[5], st));
+            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, +1, st));
+            CR_HIP(hipEventRecord(ev[6], st));
+            CR_HIP(launch_fft_z_c2r<T>(pl[2], c->mp.Nx, c->mp.Ny, cp, real, st));
+            CR_HIP(hipEventRecord(ev[7], st));
+            CR_HIP(launch_contract<T>(c->mp, v, real, c->phi_recip.as<T>(), st));
+            CR_HIP(hipEventRecord(ev[8], st));
+            CR_HIP(launch_real_space<T>(c->rp, v, x, c->phi_recip.as<T>(), table<T>(c, false), y, 1, st));
+        } else {
+            using T = double;
+            const AxisPlan* pl = plans<T>(c);
+            T* real = c->grid_real.as<T>();
+            T* cp = c->grid_cplx.as<T>();
+            CR_HIP(launch_fill<T>(real, (T)0, c->G, st));
+            CR_HIP(hipEventRecord(ev[1], st));
+            CR_HIP(launch_spread<T>(c->mp, v, x, real, st));
+            CR_HIP(hipEventRecord(ev[2], st));
+            CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st));
+            CR_HIP(hipEventRecord(ev[3], st));
+            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, -1, st));
+            CR_HIP(hipEventRecord(ev[4], st));
+            CR_HIP(launch_fft_x_scale<T>(pl[0], c->mp.Nx, c->mp.Ny, c->Nzh, cp, c->sc, st));
+            CR_HIP(hipEventRecord(ev[5], st));
+            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, +1, st));
+            CR_HIP(hipEventRecord(ev[6], st));
+            CR_HIP(launch_fft_z_c2r<T>(pl[2], c->mp.Nx, c->mp.Ny, cp, real, st));
+            CR_HIP(hipEventRecord(ev[7], st));
+            CR_HIP(launch_contract<T>(c->mp, v, real, c->phi_recip.as<T>(), st));
+            CR_HIP(hipEventRecord(ev[8], st));
+            CR_HIP(launch_real_space<T>(c->rp, v, x, c->phi_recip.as<T>(), table<T>(c, false), y, 1, st));
+        }
+        CR_HIP(hipEventRecord(ev[9], st));
+        CR_HIP(launch_constraint_rows(v, x, y + v.N, 1, st));
+        CR_HIP(hipEventRecord(ev[10], st));
+        CR_HIP(hipEventSynchronize(ev[10]));
+        for (int k = 0; k < nst; ++k) {
+            float ms = 0;
+            CR_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            acc[k] += ms;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < nst && k < max_stages; ++k) stage_ms[k] = acc[k] / reps;
+    return CR_OK;
+}
+
+int cr_build_nlist(cr_context* c, int N, const double* pos, const double L[3], double rlist, unsigned* head_list,
+                   unsigned* n_neigh, unsigned* nlist, long capacity, long* total, void* stream)
+{
+    if (!c || !L || !total || !(rlist > 0)) return set_error(CR_ERR_INVALID, "bad argument");
+    if (N > 0 && (!pos || !head_list || !n_neigh || (capacity > 0 && !nlist)))
+        return set_error(CR_ERR_INVALID, "null array");
+    for (int d = 0; d < 3; ++d)
+        if (!(L[d] > 2 * rlist) && !(L[d] > 0)) return set_error(CR_ERR_INVALID, "bad box");
+    CR_HIP(build_nlist(N, reinterpret_cast<const double4*>(pos), L, rlist, head_list, n_neigh, nlist, capacity, total,
+                       nullptr, (hipStream_t)stream));
+    return CR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+// cr_kernels.h -- internal launcher interface between the host context
+// (cr_context.cpp) and the HIP kernels (k_*.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cr {
+
+constexpr unsigned NOT_MEMBER = 0xffffffffu;
+
+// Spectral-Ewald mesh parameters (SetParams, cppmodule/ConductorRigid.cc:199-207;
+// kernel constants, cppmodule/ConductorRigid.cu:913-917).
+struct MeshParams {
+    int Nx, Ny, Nz, P;
+    double L[3];
+    double h[3];
+    double eta[3];
+    double prefac;   // (2 xi^2/pi)^{3/2} / sqrt(eta_x eta_y eta_z)
+    double xiterm;   // 2 xi^2
+    double quadW;    // h_x h_y h_z
+};
+
+// Real-space table parameters (cppmodule/ConductorRigid.cc:278-279, .cu:540-554).
+struct RealParams {
+    double L[3];
+    double rc, rc2, rmin2, drtable, selfterm;
+    int N_table;
+};
+
+// One batched 1-D FFT stage plan along one mesh axis.
+struct AxisPlan {
+    int n;            // transform length
+    int nstages;
+    int radix[24];
+    const void* tw;   // n twiddles e^{-2 pi i e/n} (float2 or double2)
+};
+
+// Device views of the system (borrowed pointers).
+struct SysView {
+    int N, N_rigid, N_isolated, Nmax;
+    const double4* pos;
+    const unsigned* iso;             // isolated-body tag per global index (context-owned)
+    const double* body_charge;
+    const unsigned* rigid_membership;
+    const unsigned* rigid_members;
+    const unsigned* rigid_size;
+    const double4* rigid_com;
+    const unsigned* nlist;
+    const unsigned* head_list;
+    const unsigned* n_neigh;
+};
+
+// ---- k_se.hip ----
+template <typename T>
+hipError_t launch_spread(const MeshParams& mp, const SysView& s, const double* q, T* grid, hipStream_t st);
+template <typename T>
+hipError_t launch_contract(const MeshParams& mp, const SysView& s, const T* grid, T* phi, hipStream_t st);
+template <typename T>
+hipError_t launch_contract_force(const MeshParams& mp, const SysView& s, const double* q, const T* grid,
+                                 double* force, hipStream_t st);
+template <typename T>
+hipError_t launch_fill(T* p, T v, long n, hipStream_t st);
+template <typename Tin, typename Tout>
+hipError_t launch_convert(const Tin* in, Tout* out, long n, hipStream_t st);
+
+// ---- k_fft.hip ----
+// Real mesh [Nx][Ny][Nz] <-> half spectrum [Nx][Ny][Nz/2+1] (interleaved complex).
+template <typename T>
+hipError_t launch_fft_z_r2c(const AxisPlan& pz, int Nx, int Ny, const T* real, T* cplx, hipStream_t st);
+template <typename T>
+hipError_t launch_fft_z_c2r(const AxisPlan& pz, int Nx, int Ny, const T* cplx, T* real, hipStream_t st);
+// C2C along y (axis 1) or x (axis 0) of a complex mesh [Nx][Ny][Zc]; sign -1 / +1.
+template <typename T>
+hipError_t launch_fft_yx(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, T* cplx, int sign, hipStream_t st);
+// C2C along z of a full complex mesh [Nx][Ny][Nz].
+template <typename T>
+hipError_t launch_fft_z_c2c(const AxisPlan& pz, int Nx, int Ny, T* cplx, int sign, hipStream_t st);
+// Forward x FFT, multiply by the Green's function scale S(k), inverse x FFT.
+struct ScaleTables {
+    const double* k[3];    // wavenumbers per axis
+    const double* e[3];    // exp(-(1-eta_d) k_d^2 / (4 xi^2)) per axis
+    double invG;
+};
+template <typename T>
+hipError_t launch_fft_x_scale(const AxisPlan& px, int Nx, int Ny, int Zc, T* cplx, const ScaleTables& sc,
+                              hipStream_t st);
+
+// ---- k_real.hip ----
+template <typename T>
+hipError_t launch_real_space(const RealParams& rp, const SysView& s, const double* x, const T* phi_recip,
+                             const T* table, double* y, int precond, hipStream_t st);
+template <typename T>
+hipError_t launch_real_space_force(const RealParams& rp, const SysView& s, const double* q, const T* gtable,
+                                   const double E0[3], double* force, hipStream_t st);
+hipError_t launch_constraint_rows(const SysView& s, const double* q, double* y_bottom, int precond, hipStream_t st);
+hipError_t launch_precondition(const SysView& s, const double* x, double* y, hipStream_t st);
+hipError_t launch_objective(const SysView& s, const double L[3], const double E0[3], double* b, hipStream_t st);
+hipError_t launch_membership(int N, const unsigned* tag, const unsigned* body_of_tag, unsigned* iso, hipStream_t st);
+hipError_t launch_zero_force(int N, double* force, hipStream_t st);
+
+// ---- k_nlist.hip ----
+hipError_t build_nlist(int N, const double4* pos, const double L[3], double rlist, unsigned* head,
+                       unsigned* nn, unsigned* nlist, long capacity, long* total, void* scratch_pool,
+                       hipStream_t st);
+
+// ---- k_krylov.hip ----
+// Partial dot products of w with columns 0..ncols-1 of V (column pitch ld):
+// partial[b * 32 + k]; nblocks returned by krylov_blocks(n).
+int krylov_blocks(long n);
+hipError_t launch_multidot(const double* V, long ld, int ncols, const double* w, long n, double* partial,
+                           hipStream_t st);
+// out[k] = sum over blocks of partial[b*32+k] (+ out_add[k] if non-null), k < ncols
+hipError_t launch_reduce_partials(const double* partial, int nblocks, int ncols, double* out, hipStream_t st);
+// w -= V[:,0:ncols] h; then (mode 1) partial dots V^T w, (mode 2) partial ||w||^2, (mode 0) nothing
+hipError_t launch_gs_update(const double* V, long ld, int ncols, const double* h, double* w, long n, int mode,
+                            double* partial, hipStream_t st);
+// y = a*x + b*y elementwise helpers
+hipError_t launch_axpby(double a, const double* x, double b, double* y, long n, hipStream_t st);
+hipError_t launch_scale_copy(const double* x, const double* s_ptr, double sign, double* y, long n, hipStream_t st);
+// x += V[:,0:k] c  (c on device)
+hipError_t launch_combine(const double* V, long ld, const double* c, const int* kcount, double* x, long n,
+                          hipStream_t st);
+
+// Arnoldi step bookkeeping on device (single thread): see k_krylov.hip.
+struct GmresState {
+    double H[33 * 32];     // column-major (R+1) x R, R <= 32
+    double cs[32], sn[32], s[33];
+    double h1[32], h2[32];
+    double wnorm2;
+    double inv_norm;       // 1 / H(i+1,i)
+    double beta, inv_beta;
+    double y[32];
+    int i;                 // current Arnoldi column
+    int ncols;             // columns used in the last cycle (i+1)
+    double resid;
+};
+hipError_t launch_givens(GmresState* g, int i, int R, hipStream_t st);
+hipError_t launch_cycle_init(GmresState* g, const double* norm2, hipStream_t st);
+hipError_t launch_backsub(GmresState* g, int R, hipStream_t st);
+
+}  // namespace cr

@@ -1,0 +1,265 @@
+// k_krylov.hip -- FP64 restarted GMRES building blocks, device resident.
+//
+// The reference runs CUSP's krylov::gmres (cppmodule/ConductorRigid.cu:1134-1136):
+// modified Gram-Schmidt Arnoldi on the device with one device->host sync per dot
+// product, Hessenberg matrix and Givens rotations on the host in float.  Here the
+// basis, Hessenberg matrix, rotations and least-squares right-hand side stay on the
+// device in double; orthogonalisation is classical Gram-Schmidt with one
+// re-orthogonalisation pass (CGS2), so each pass over the basis is one kernel that
+// produces all (j+1) projections at once, reduced deterministically (fixed block
+// order, no floating-point atomics).
+#include "cr_kernels.h"
+
+namespace cr {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kChunk = 8;       // basis columns per multidot launch row
+constexpr int kMaxBlocks = 1024;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block-reduce kChunk values held per thread; thread 0 writes them.
+__device__ __forceinline__ void block_reduce_chunk(double (&acc)[kChunk], double* out)
+{
+    __shared__ double red[kBlock / 64][kChunk];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) {
+        const double v = wave_sum(acc[k]);
+        if (lane == 0) red[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kChunk) {
+        double s = 0;
+        for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+// partial[b*32 + c0 + k] = sum_{i in block b} V[c0+k][i] w[i]
+__global__ __launch_bounds__(kBlock) void k_multidot(const double* __restrict__ V, long ld, int ncols,
+                                                     const double* __restrict__ w, long n,
+                                                     double* __restrict__ partial)
+{
+    const int c0 = blockIdx.y * kChunk;
+    double acc[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) acc[k] = 0.0;
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+        const double wi = w[i];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k)
+            if (c0 + k < ncols) acc[k] += V[(long)(c0 + k) * ld + i] * wi;
+    }
+    __shared__ double res[kChunk];
+    block_reduce_chunk(acc, res);
+    if (threadIdx.x < kChunk && c0 + (int)threadIdx.x < ncols)
+        partial[(long)blockIdx.x * 32 + c0 + threadIdx.x] = res[threadIdx.x];
+}
+
+__global__ void k_reduce_partials(const double* __restrict__ partial, int nblocks, int ncols, double* __restrict__ out)
+{
+    const int k = threadIdx.x;
+    if (k >= ncols) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += partial[(long)b * 32 + k];
+    out[k] = s;
+}
+
+// w -= V[:, 0:ncols] h ; mode 2 also accumulates ||w||^2 partials into partial[b*32]
+__global__ __launch_bounds__(kBlock) void k_gs_update(const double* __restrict__ V, long ld, int ncols,
+                                                      const double* __restrict__ h, double* __restrict__ w, long n,
+                                                      int mode, double* __restrict__ partial)
+{
+    __shared__ double hs[32];
+    if (threadIdx.x < 32) hs[threadIdx.x] = (threadIdx.x < (unsigned)ncols) ? h[threadIdx.x] : 0.0;
+    __syncthreads();
+    double acc[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) acc[k] = 0.0;
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+        double wi = w[i];
+        for (int k = 0; k < ncols; ++k) wi -= hs[k] * V[(long)k * ld + i];
+        w[i] = wi;
+        acc[0] += wi * wi;
+    }
+    if (mode == 2) {
+        __shared__ double res[kChunk];
+        block_reduce_chunk(acc, res);
+        if (threadIdx.x == 0) partial[(long)blockIdx.x * 32] = res[0];
+    }
+}
+
+__global__ void k_axpby(double a, const double* __restrict__ x, double b, double* __restrict__ y, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = a * x[i] + b * y[i];
+}
+
+__global__ void k_scale_copy(const double* __restrict__ x, const double* __restrict__ s, double sign,
+                             double* __restrict__ y, long n)
+{
+    const double f = sign * (*s);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = f * x[i];
+}
+
+__global__ void k_combine(const double* __restrict__ V, long ld, const double* __restrict__ c,
+                          const int* __restrict__ kcount, double* __restrict__ x, long n)
+{
+    const int K = *kcount;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double xi = x[i];
+        for (int k = 0; k < K; ++k) xi += c[k] * V[(long)k * ld + i];
+        x[i] = xi;
+    }
+}
+
+// Givens plane rotation helpers (Saad, Iterative Methods for Sparse Linear Systems,
+// Alg. 6.9; the same rotation CUSP's GMRES applies).
+__device__ __forceinline__ void apply_rot(double& dx, double& dy, double cs, double sn)
+{
+    const double t = cs * dx + sn * dy;
+    dy = -sn * dx + cs * dy;
+    dx = t;
+}
+
+__device__ __forceinline__ void gen_rot(double dx, double dy, double& cs, double& sn)
+{
+    if (dy == 0.0) {
+        cs = 1.0;
+        sn = 0.0;
+    } else if (fabs(dy) > fabs(dx)) {
+        const double t = dx / dy;
+        sn = 1.0 / sqrt(1.0 + t * t);
+        cs = t * sn;
+    } else {
+        const double t = dy / dx;
+        cs = 1.0 / sqrt(1.0 + t * t);
+        sn = t * cs;
+    }
+}
+
+// Column i of H from the two CGS projections and ||w||, then the rotations.
+__global__ void k_givens(GmresState* g, int i, int R)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int ld = R + 1;
+    double* H = g->H;
+    for (int k = 0; k <= i; ++k) H[i * ld + k] = g->h1[k] + g->h2[k];
+    const double hn = sqrt(g->wnorm2);
+    H[i * ld + i + 1] = hn;
+    g->inv_norm = (hn != 0.0) ? 1.0 / hn : 0.0;
+    for (int k = 0; k < i; ++k) apply_rot(H[i * ld + k], H[i * ld + k + 1], g->cs[k], g->sn[k]);
+    gen_rot(H[i * ld + i], H[i * ld + i + 1], g->cs[i], g->sn[i]);
+    apply_rot(H[i * ld + i], H[i * ld + i + 1], g->cs[i], g->sn[i]);
+    apply_rot(g->s[i], g->s[i + 1], g->cs[i], g->sn[i]);
+    g->resid = fabs(g->s[i + 1]);
+    g->i = i;
+    g->ncols = i + 1;
+}
+
+__global__ void k_cycle_init(GmresState* g, const double* norm2)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double beta = sqrt(*norm2);
+    g->beta = beta;
+    g->inv_beta = (beta != 0.0) ? 1.0 / beta : 0.0;
+    for (int k = 0; k < 33; ++k) g->s[k] = 0.0;
+    g->s[0] = beta;
+    g->resid = beta;
+    g->ncols = 0;
+}
+
+// Back substitution H(0:i,0:i) y = s(0:i) (R upper triangular after the rotations).
+__global__ void k_backsub(GmresState* g, int R)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int ld = R + 1;
+    const int m = g->ncols;
+    for (int k = 0; k < m; ++k) g->y[k] = g->s[k];
+    for (int j = m - 1; j >= 0; --j) {
+        g->y[j] /= g->H[j * ld + j];
+        for (int k = j - 1; k >= 0; --k) g->y[k] -= g->H[j * ld + k] * g->y[j];
+    }
+}
+
+inline unsigned vblocks(long n)
+{
+    long b = (n + kBlock - 1) / kBlock;
+    if (b > kMaxBlocks) b = kMaxBlocks;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+}  // namespace
+
+int krylov_blocks(long n) { return (int)vblocks(n); }
+
+hipError_t launch_multidot(const double* V, long ld, int ncols, const double* w, long n, double* partial,
+                           hipStream_t st)
+{
+    const unsigned nb = vblocks(n);
+    const unsigned nchunk = (unsigned)((ncols + kChunk - 1) / kChunk);
+    hipLaunchKernelGGL(k_multidot, dim3(nb, nchunk), dim3(kBlock), 0, st, V, ld, ncols, w, n, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_partials(const double* partial, int nblocks, int ncols, double* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(32), 0, st, partial, nblocks, ncols, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gs_update(const double* V, long ld, int ncols, const double* h, double* w, long n, int mode,
+                            double* partial, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_gs_update, dim3(vblocks(n)), dim3(kBlock), 0, st, V, ld, ncols, h, w, n, mode, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpby(double a, const double* x, double b, double* y, long n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_axpby, dim3(vblocks(n)), dim3(kBlock), 0, st, a, x, b, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_copy(const double* x, const double* s_ptr, double sign, double* y, long n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_scale_copy, dim3(vblocks(n)), dim3(kBlock), 0, st, x, s_ptr, sign, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_combine(const double* V, long ld, const double* c, const int* kcount, double* x, long n,
+                          hipStream_t st)
+{
+    hipLaunchKernelGGL(k_combine, dim3(vblocks(n)), dim3(kBlock), 0, st, V, ld, c, kcount, x, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_givens(GmresState* g, int i, int R, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, st, g, i, R);
+    return hipGetLastError();
+}
+
+hipError_t launch_cycle_init(GmresState* g, const double* norm2, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cycle_init, dim3(1), dim3(64), 0, st, g, norm2);
+    return hipGetLastError();
+}
+
+hipError_t launch_backsub(GmresState* g, int R, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_backsub, dim3(1), dim3(64), 0, st, g, R);
+    return hipGetLastError();
+}
+
+}  // namespace cr

@@ -1,0 +1,240 @@
+// k_real.hip -- real-space pair sums, constraint rows, objective, preconditioner and
+// membership of the ConductorRigid saddle-point operator.
+//
+// Reference kernels: real_space (cppmodule/ConductorRigid.cu:469-567),
+// real_space_force (:570-664), sum_charges / set_charge (:667-761),
+// compute_objective (:764-813), precondition (:816-859),
+// initialize/update_membership (:92-141), zeroforce (:56-66).
+#include "cr_kernels.h"
+
+namespace cr {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+__device__ __forceinline__ double min_image(double d, double L) { return d - L * rint(d / L); }
+
+// Linear interpolation in a (value, next value) table (cppmodule/ConductorRigid.cu:549-554),
+// index clamped to [0, N_table-1].
+template <typename T>
+__device__ __forceinline__ T table_lookup(const RealParams& rp, const T* __restrict__ tab, double d)
+{
+    int ti = (int)floor(rp.N_table * (d - rp.drtable) / (rp.rc - rp.drtable));
+    ti = ti < 0 ? 0 : (ti > rp.N_table - 1 ? rp.N_table - 1 : ti);
+    const double lin = d / rp.drtable - ti - 1.0;
+    const T a = tab[2 * ti], b = tab[2 * ti + 1];
+    return a + (b - a) * (T)lin;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_real_space(RealParams rp, SysView s, const double* __restrict__ x,
+                                                    const T* __restrict__ phi_recip, const T* __restrict__ table,
+                                                    double* __restrict__ y, int precond)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    const double* q = x;
+    const double* Phi = x + s.N;
+    const double4 ri = s.pos[i];
+    T acc = phi_recip[i] + (T)rp.selfterm * (T)q[i];
+    const unsigned head = s.head_list[i], nn = s.n_neigh[i];
+    for (unsigned jj = 0; jj < nn; ++jj) {
+        const unsigned j = s.nlist[head + jj];
+        const double4 rj = s.pos[j];
+        const double dx = min_image(ri.x - rj.x, rp.L[0]);
+        const double dy = min_image(ri.y - rj.y, rp.L[1]);
+        const double dz = min_image(ri.z - rj.z, rp.L[2]);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < rp.rc2 && d2 >= rp.rmin2) acc += table_lookup<T>(rp, table, sqrt(d2)) * (T)q[j];
+    }
+    const double out = Phi[s.iso[i]] - (double)acc;
+    y[i] = precond ? -4.0 * kPi * out : out;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_real_space_force(RealParams rp, SysView s, const double* __restrict__ q,
+                                                          const T* __restrict__ gtable, double E0x, double E0y,
+                                                          double E0z, double* __restrict__ force)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    const double qi = q[i];
+    double Fx = force[4 * i] + qi * E0x;
+    double Fy = force[4 * i + 1] + qi * E0y;
+    double Fz = force[4 * i + 2] + qi * E0z;
+    T fx = 0, fy = 0, fz = 0;
+    const double4 ri = s.pos[i];
+    const unsigned head = s.head_list[i], nn = s.n_neigh[i];
+    for (unsigned jj = 0; jj < nn; ++jj) {
+        const unsigned j = s.nlist[head + jj];
+        const double4 rj = s.pos[j];
+        const double dx = min_image(ri.x - rj.x, rp.L[0]);
+        const double dy = min_image(ri.y - rj.y, rp.L[1]);
+        const double dz = min_image(ri.z - rj.z, rp.L[2]);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < rp.rc2 && d2 >= rp.rmin2) {
+            const double d = sqrt(d2);
+            const T A = table_lookup<T>(rp, gtable, d);
+            const T c = -A * (T)(qi * q[j] / d);
+            fx += c * (T)dx;
+            fy += c * (T)dy;
+            fz += c * (T)dz;
+        }
+    }
+    force[4 * i] = Fx + (double)fx;
+    force[4 * i + 1] = Fy + (double)fy;
+    force[4 * i + 2] = Fz + (double)fz;
+    force[4 * i + 3] = 0.0;
+}
+
+// sum_charges: y_bottom[tag(r)] = sum of bead charges of rigid body r (FP64), optionally
+// scaled by the preconditioner's 1/size.
+__global__ void k_sum_charges(SysView s, const double* __restrict__ q, double* __restrict__ yb, int precond)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= s.N_rigid) return;
+    const unsigned size = s.rigid_size[r];
+    if (size == 0) return;
+    const unsigned* mem = s.rigid_members + (long)r * s.Nmax;
+    double acc = 0.0;
+    for (unsigned k = 0; k < size; ++k) acc += q[mem[k]];
+    const unsigned tag = s.iso[mem[0]];
+    yb[tag] = precond ? acc / (double)size : acc;
+}
+
+// set_charge: isolated beads carry their own body's charge row.
+__global__ void k_set_charge(SysView s, const double* __restrict__ q, double* __restrict__ yb, int precond)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    if (s.rigid_membership[i] != NOT_MEMBER) return;
+    yb[s.iso[i]] = precond ? q[i] / (4.0 * kPi) : q[i];
+}
+
+__global__ void k_precond_diag(int N, int B, const double* __restrict__ x, double* __restrict__ y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) y[i] = -4.0 * kPi * x[i];
+    else if (i < N + B) y[i] = x[i] / (4.0 * kPi);
+}
+
+__global__ void k_precond_rigid(SysView s, const double* __restrict__ x, double* __restrict__ y)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= s.N_rigid) return;
+    const unsigned size = s.rigid_size[r];
+    if (size == 0) return;
+    const unsigned tag = s.iso[s.rigid_members[(long)r * s.Nmax]];
+    y[s.N + tag] = x[s.N + tag] / (double)size;
+}
+
+// compute_objective with zero_array folded in: rigid beads get -(r_i - R_b).E0 (minimum
+// image), beads outside rigid bodies get 0, rows N.. get the body charges.
+__global__ void k_objective(SysView s, double Lx, double Ly, double Lz, double E0x, double E0y, double E0z,
+                            double* __restrict__ b)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nslot = (long)s.N_rigid * s.Nmax;
+    if (t < nslot) {
+        const unsigned idx = s.rigid_members[t];
+        if (idx != NOT_MEMBER) {
+            const double4 p = s.pos[idx];
+            const double4 c = s.rigid_com[t / s.Nmax];
+            const double rx = min_image(p.x - c.x, Lx), ry = min_image(p.y - c.y, Ly), rz = min_image(p.z - c.z, Lz);
+            b[idx] = -(rx * E0x + ry * E0y + rz * E0z);
+        }
+    }
+    if (t < s.N && s.rigid_membership[t] == NOT_MEMBER) b[t] = 0.0;
+    if (t < s.N_isolated) b[s.N + t] = s.body_charge[t];
+}
+
+__global__ void k_membership(int N, const unsigned* __restrict__ tag, const unsigned* __restrict__ body_of_tag,
+                             unsigned* __restrict__ iso)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) iso[i] = body_of_tag[tag[i]];
+}
+
+__global__ void k_zero_force(int N, double* f)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 4 * N) f[i] = 0.0;
+}
+
+inline unsigned blocks(long n, int b) { return (unsigned)((n + b - 1) / b > 0 ? (n + b - 1) / b : 1); }
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_real_space(const RealParams& rp, const SysView& s, const double* x, const T* phi_recip,
+                             const T* table, double* y, int precond, hipStream_t st)
+{
+    if (s.N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_real_space<T>, dim3(blocks(s.N, 256)), dim3(256), 0, st, rp, s, x, phi_recip, table, y,
+                       precond);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_real_space_force(const RealParams& rp, const SysView& s, const double* q, const T* gtable,
+                                   const double E0[3], double* force, hipStream_t st)
+{
+    if (s.N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_real_space_force<T>, dim3(blocks(s.N, 256)), dim3(256), 0, st, rp, s, q, gtable, E0[0],
+                       E0[1], E0[2], force);
+    return hipGetLastError();
+}
+
+hipError_t launch_constraint_rows(const SysView& s, const double* q, double* yb, int precond, hipStream_t st)
+{
+    if (s.N_rigid > 0)
+        hipLaunchKernelGGL(k_sum_charges, dim3(blocks(s.N_rigid, 256)), dim3(256), 0, st, s, q, yb, precond);
+    if (s.N > 0) hipLaunchKernelGGL(k_set_charge, dim3(blocks(s.N, 256)), dim3(256), 0, st, s, q, yb, precond);
+    return hipGetLastError();
+}
+
+hipError_t launch_precondition(const SysView& s, const double* x, double* y, hipStream_t st)
+{
+    const long n = (long)s.N + s.N_isolated;
+    if (n > 0) hipLaunchKernelGGL(k_precond_diag, dim3(blocks(n, 256)), dim3(256), 0, st, s.N, s.N_isolated, x, y);
+    if (s.N_rigid > 0)
+        hipLaunchKernelGGL(k_precond_rigid, dim3(blocks(s.N_rigid, 256)), dim3(256), 0, st, s, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_objective(const SysView& s, const double L[3], const double E0[3], double* b, hipStream_t st)
+{
+    long n = (long)s.N_rigid * s.Nmax;
+    if (s.N > n) n = s.N;
+    if (s.N_isolated > n) n = s.N_isolated;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_objective, dim3(blocks(n, 256)), dim3(256), 0, st, s, L[0], L[1], L[2], E0[0], E0[1], E0[2],
+                       b);
+    return hipGetLastError();
+}
+
+hipError_t launch_membership(int N, const unsigned* tag, const unsigned* body_of_tag, unsigned* iso, hipStream_t st)
+{
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_membership, dim3(blocks(N, 256)), dim3(256), 0, st, N, tag, body_of_tag, iso);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_force(int N, double* force, hipStream_t st)
+{
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_force, dim3(blocks(4L * N, 256)), dim3(256), 0, st, N, force);
+    return hipGetLastError();
+}
+
+template hipError_t launch_real_space<float>(const RealParams&, const SysView&, const double*, const float*,
+                                             const float*, double*, int, hipStream_t);
+template hipError_t launch_real_space<double>(const RealParams&, const SysView&, const double*, const double*,
+                                              const double*, double*, int, hipStream_t);
+template hipError_t launch_real_space_force<float>(const RealParams&, const SysView&, const double*, const float*,
+                                                   const double*, double*, hipStream_t);
+template hipError_t launch_real_space_force<double>(const RealParams&, const SysView&, const double*, const double*,
+                                                    const double*, double*, hipStream_t);
+
+}  // namespace cr

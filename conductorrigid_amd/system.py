@@ -1,0 +1,310 @@
+"""HOOMD-shaped system data for the ConductorRigid force compute, and a seeded
+generator of rigid conductor clusters.
+
+The reference plugin reads its inputs from HOOMD 1.x (cppmodule/ConductorRigid.cc:420-487):
+particle positions/types by global index, particle tags, the user's per-tag
+isolated-body map ``body`` and per-body net ``charge`` lists
+(cppmodule/ConductorRigid.cc:93-105), and HOOMD's RigidData (centre of mass, body
+size, member indices with pitch Nmax, per-particle rigid index).  HOOMD is not
+part of this package, so :class:`SystemDefinition` holds exactly those arrays
+with the same meaning and layout.
+
+Isolated-body tags follow the reference's convention: every rigid body and every
+bead that belongs to no rigid body is one "isolated body" with its own tag and
+its own net charge; rigid body ``r`` has tag ``r`` and isolated beads take the
+tags after the rigid bodies.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+NOT_MEMBER = 0xFFFFFFFF
+
+
+@dataclass
+class SystemDefinition:
+    L: np.ndarray                 # (3,) box lengths; box spans [-L/2, L/2)
+    pos: np.ndarray               # (N, 4) float64  x, y, z, type  by global index
+    tag: np.ndarray               # (N,) uint32     particle tag of each global index
+    body_of_tag: np.ndarray       # (N,) uint32     isolated-body tag per particle tag
+    body_charge: np.ndarray       # (N_isolated,) float64 net charge per isolated body
+    rigid_membership: np.ndarray  # (N,) uint32     rigid index per global index or NOT_MEMBER
+    rigid_members: np.ndarray     # (N_rigid*Nmax,) uint32 global indices, NOT_MEMBER padded
+    rigid_size: np.ndarray        # (N_rigid,) uint32
+    rigid_com: np.ndarray         # (N_rigid, 4) float64 centre of mass (wrapped)
+    Nmax: int
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def N(self) -> int:
+        return int(self.pos.shape[0])
+
+    @property
+    def N_rigid(self) -> int:
+        return int(self.rigid_size.shape[0])
+
+    @property
+    def N_isolated(self) -> int:
+        return int(self.body_charge.shape[0])
+
+    def iso_membership(self) -> np.ndarray:
+        """Isolated-body tag of each bead by global index (update_membership,
+        cppmodule/ConductorRigid.cu:117-141)."""
+        return self.body_of_tag[self.tag]
+
+    def body_counts(self) -> np.ndarray:
+        return np.bincount(self.iso_membership(), minlength=self.N_isolated)
+
+    def validate(self) -> None:
+        N = self.N
+        if self.pos.shape != (N, 4) or self.pos.dtype != np.float64:
+            raise ValueError("pos must be float64 (N,4)")
+        if sorted(self.tag.tolist()) != list(range(N)):
+            raise ValueError("tag must be a permutation of 0..N-1")
+        iso = self.iso_membership()
+        if iso.size and iso.max() >= self.N_isolated:
+            raise ValueError("body tag out of range")
+        for r in range(self.N_rigid):
+            n = int(self.rigid_size[r])
+            mem = self.rigid_members[r * self.Nmax:r * self.Nmax + n]
+            if np.any(mem == NOT_MEMBER) or np.any(self.rigid_membership[mem] != r):
+                raise ValueError(f"rigid body {r} members inconsistent")
+        half = 0.5 * self.L
+        if np.any(self.pos[:, :3] < -half) or np.any(self.pos[:, :3] >= half):
+            raise ValueError("positions must be wrapped into [-L/2, L/2)")
+
+
+def box_length(n_beads: int, phi: float, radius: float = 1.0) -> float:
+    """Cubic box edge for ``n_beads`` spheres of ``radius`` at volume fraction ``phi``."""
+    return (n_beads * 4.0 / 3.0 * math.pi * radius ** 3 / phi) ** (1.0 / 3.0)
+
+
+def _grow_cluster(rng: np.random.Generator, n: int, n_candidates: int = 24) -> np.ndarray:
+    """Random compact aggregate of ``n`` touching unit spheres.  Each new bead
+    touches (centre distance exactly 2) a randomly chosen existing bead and
+    overlaps none; of ``n_candidates`` random non-overlapping candidates the
+    one closest to the current centre is kept, which keeps the cluster compact.
+    Returns centre positions relative to the cluster's centre of mass."""
+    pts = np.zeros((n, 3))
+    for k in range(1, n):
+        for _attempt in range(1000):
+            base = pts[rng.integers(0, k, size=n_candidates)]
+            d = rng.normal(size=(n_candidates, 3))
+            d /= np.linalg.norm(d, axis=1, keepdims=True)
+            cand = base + 2.0 * d
+            dist = np.linalg.norm(cand[:, None, :] - pts[None, :k, :], axis=2)
+            good = np.all(dist >= 2.0 - 1e-12, axis=1)
+            if np.any(good):
+                c = cand[good]
+                com = pts[:k].mean(axis=0)
+                pts[k] = c[np.argmin(np.linalg.norm(c - com, axis=1))]
+                break
+        else:
+            raise RuntimeError("cluster growth stalled")
+    return pts - pts.mean(axis=0)
+
+
+def _random_rotations(rng: np.random.Generator, m: int) -> np.ndarray:
+    q = rng.normal(size=(m, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.empty((m, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    R[:, 0, 1] = 2 * (x * y - z * w)
+    R[:, 0, 2] = 2 * (x * z + y * w)
+    R[:, 1, 0] = 2 * (x * y + z * w)
+    R[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    R[:, 1, 2] = 2 * (y * z - x * w)
+    R[:, 2, 0] = 2 * (x * z - y * w)
+    R[:, 2, 1] = 2 * (y * z + x * w)
+    R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def _wrap(x: np.ndarray, L: np.ndarray) -> np.ndarray:
+    y = x - L * np.floor((x + 0.5 * L) / L)
+    # guard the half-open interval against rounding up to +L/2
+    y = np.where(y >= 0.5 * L, y - L, y)
+    return y
+
+
+def _body_means(x: np.ndarray, offsets: np.ndarray, bodies: np.ndarray, count: np.ndarray) -> np.ndarray:
+    """Mean of the first count[b] rows of each body's slice of x."""
+    k = count[bodies]
+    kmax = int(k.max())
+    cols = np.arange(kmax)
+    rows = offsets[bodies, None] + np.minimum(cols[None, :], k[:, None] - 1)
+    mask = (cols[None, :] < k[:, None]).astype(np.float64)
+    return (x[rows] * mask[:, :, None]).sum(axis=1) / k[:, None]
+
+
+def make_system(n_bodies: int, body_sizes, phi: float, seed: int, charges: str = "alternating",
+                n_isolated_beads: int = 0, spatial_sort: bool = True,
+                n_candidates: int = 16) -> SystemDefinition:
+    """Seeded random system of rigid conductor clusters (plus optional isolated beads).
+
+    ``body_sizes`` is an int (all bodies that size) or a sequence to draw from.
+    Every body starts as one seed bead on a random site of an FCC lattice (with
+    jitter) and all bodies then grow together, one bead per body per round:
+    a new bead touches (centre distance 2) a random bead of its own body,
+    overlaps no bead of any body (periodic minimum image), and of
+    ``n_candidates`` proposals the one closest to the body's centre is kept,
+    which makes compact random aggregates.  ``charges`` is "alternating"
+    (+1/-1 by body), "uniform" (U[-2, 2] shifted to total 0) or "zero".
+    """
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    if np.isscalar(body_sizes):
+        sizes = np.full(n_bodies, int(body_sizes), dtype=np.int64)
+    else:
+        sizes = rng.choice(np.asarray(body_sizes, dtype=np.int64), size=n_bodies)
+    n_beads = int(sizes.sum()) + n_isolated_beads
+    Lval = box_length(n_beads, phi)
+    L = np.array([Lval] * 3)
+
+    n_sites_needed = n_bodies + n_isolated_beads
+    m = max(1, int(math.ceil((n_sites_needed / 4.0) ** (1.0 / 3.0))))
+    a = Lval / m
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    cells = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1).reshape(-1, 3)
+    sites = ((cells[:, None, :] + basis[None, :, :]).reshape(-1, 3) + 0.25) * a - 0.5 * Lval
+    centres = sites[rng.permutation(sites.shape[0])[:n_sites_needed]]
+    centres = centres + rng.uniform(-0.05, 0.05, size=centres.shape) * a
+
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    owner = np.concatenate([np.repeat(np.arange(n_bodies), sizes), n_bodies + np.arange(n_isolated_beads)])
+    unwrapped = np.zeros((n_beads, 3))
+    placed = np.zeros(n_beads, dtype=bool)
+    unwrapped[offsets[:-1]] = centres[:n_bodies]
+    placed[offsets[:-1]] = True
+    if n_isolated_beads:
+        unwrapped[offsets[-1]:] = centres[n_bodies:]
+        placed[offsets[-1]:] = True
+    count = np.ones(n_bodies, dtype=np.int64)
+    tol = 1e-9
+    stall = 0
+    restarts = 0
+    while np.any(count < sizes):
+        active = np.nonzero(count < sizes)[0]
+        idx_placed = np.nonzero(placed)[0]
+        tree = cKDTree(_wrap(unwrapped[idx_placed], L) + 0.5 * L, boxsize=Lval)
+        A, K = active.size, n_candidates
+        base_k = (rng.random((A, K)) * count[active, None]).astype(np.int64)
+        base = unwrapped[offsets[active, None] + base_k]
+        d = rng.normal(size=(A, K, 3))
+        d /= np.linalg.norm(d, axis=2, keepdims=True)
+        cand = base + 2.0 * d
+        dist, _ = tree.query((_wrap(cand.reshape(-1, 3), L) + 0.5 * L), k=1)
+        valid = (dist.reshape(A, K) >= 2.0 - tol)
+        com = np.stack([unwrapped[offsets[b]:offsets[b] + count[b]].mean(axis=0) for b in active]) \
+            if A < 2000 else _body_means(unwrapped, offsets, active, count)
+        score = np.linalg.norm(cand - com[:, None, :], axis=2)
+        score[~valid] = np.inf
+        best = np.argmin(score, axis=1)
+        ok = np.isfinite(score[np.arange(A), best])
+        chosen_b = active[ok]
+        chosen = cand[np.arange(A)[ok], best[ok]]
+        if chosen_b.size:
+            t2 = cKDTree(_wrap(chosen, L) + 0.5 * L, boxsize=Lval)
+            pr = t2.query_pairs(2.0 - tol, output_type="ndarray")
+            drop = np.zeros(chosen_b.size, dtype=bool)
+            if pr.size:
+                drop[np.maximum(pr[:, 0], pr[:, 1])] = True
+            chosen_b, chosen = chosen_b[~drop], chosen[~drop]
+            slot = offsets[chosen_b] + count[chosen_b]
+            unwrapped[slot] = chosen
+            placed[slot] = True
+            count[chosen_b] += 1
+        stall = 0 if chosen_b.size else stall + 1
+        if stall > 8:
+            # every remaining body is caged in: re-seed each at a random free spot
+            restarts += 1
+            if restarts > 200:
+                raise RuntimeError("cluster growth stalled; lower the volume fraction")
+            for b in np.nonzero(count < sizes)[0]:
+                placed[offsets[b]:offsets[b + 1]] = False
+                idx_placed = np.nonzero(placed)[0]
+                tree = cKDTree(_wrap(unwrapped[idx_placed], L) + 0.5 * L, boxsize=Lval)
+                trial = rng.uniform(-0.5, 0.5, size=(4096, 3)) * Lval
+                room = tree.query(trial + 0.5 * L, k=1)[0]
+                if room.max() < 2.0:
+                    raise RuntimeError("no free space to re-seed a body; lower the volume fraction")
+                p0 = trial[np.argmax(room)]
+                unwrapped[offsets[b]] = p0
+                placed[offsets[b]] = True
+                count[b] = 1
+            stall = 0
+
+    com_unwrapped = np.array([unwrapped[offsets[b]:offsets[b + 1]].mean(axis=0) for b in range(n_bodies)]) \
+        if n_bodies else np.zeros((0, 3))
+    pos_tagorder = _wrap(unwrapped, L)
+
+    # global index order: spatial (cell) sort like HOOMD's particle sorter
+    if spatial_sort:
+        nb = max(1, int(Lval // 2.0))
+        cell = np.floor((pos_tagorder + 0.5 * L) / L * nb).astype(np.int64).clip(0, nb - 1)
+        key = (cell[:, 0] * nb + cell[:, 1]) * nb + cell[:, 2]
+        order = np.argsort(key, kind="stable")          # order[idx] = tag
+    else:
+        order = np.arange(n_beads)
+    tag = order.astype(np.uint32)
+    rtag = np.empty(n_beads, dtype=np.int64)
+    rtag[order] = np.arange(n_beads)
+
+    pos = np.zeros((n_beads, 4))
+    pos[:, :3] = pos_tagorder[order]
+
+    N_iso = n_bodies + n_isolated_beads
+    body_of_tag = owner.astype(np.uint32)          # rigid body r -> tag r; isolated beads after
+    Nmax = int(sizes.max()) if n_bodies else 1
+    rigid_members = np.full(n_bodies * Nmax, NOT_MEMBER, dtype=np.uint32)
+    rigid_membership = np.full(n_beads, NOT_MEMBER, dtype=np.uint32)
+    for b in range(n_bodies):
+        idxs = rtag[offsets[b]:offsets[b + 1]]
+        rigid_members[b * Nmax:b * Nmax + sizes[b]] = idxs
+        rigid_membership[idxs] = b
+    rigid_com = np.zeros((n_bodies, 4))
+    rigid_com[:, :3] = _wrap(com_unwrapped, L) if n_bodies else 0
+
+    if charges == "alternating":
+        body_charge = np.where(np.arange(N_iso) % 2 == 0, 1.0, -1.0)
+        if N_iso % 2:
+            body_charge[-1] = 0.0
+    elif charges == "uniform":
+        body_charge = rng.uniform(-2.0, 2.0, size=N_iso)
+        body_charge -= body_charge.mean()
+    elif charges == "zero":
+        body_charge = np.zeros(N_iso)
+    else:
+        raise ValueError(charges)
+
+    sysdef = SystemDefinition(L=L, pos=pos, tag=tag, body_of_tag=body_of_tag, body_charge=body_charge,
+                              rigid_membership=rigid_membership, rigid_members=rigid_members,
+                              rigid_size=sizes.astype(np.uint32), rigid_com=rigid_com, Nmax=Nmax,
+                              meta=dict(seed=seed, phi=phi, n_bodies=n_bodies, charges=charges))
+    return sysdef
+
+
+# Workloads used by the parity tests and the benchmark.
+WORKLOADS = {
+    # N=1000, 50 bodies x 20 beads, phi=0.2, Q=+-1 alternating, mesh 64^3, P=8, rc=4
+    "n1k": dict(n_bodies=50, body_sizes=20, phi=0.2, charges="alternating", seed=1,
+                xi=0.929, errortol=1e-6, mesh=(64, 64, 64), P=8, rc=4.0, restart=10, tol=1e-6),
+    # N=100k, 5000 x 20, phi=0.3, Q~U[-2,2] neutral, mesh 128^3, P=8, rc=4, restart 30
+    "n100k": dict(n_bodies=5000, body_sizes=20, phi=0.3, charges="uniform", seed=2,
+                  xi=0.6, errortol=1e-6, mesh=(128, 128, 128), P=8, rc=4.0, restart=30, tol=1e-6),
+    # N=1M, 50k x 20, phi=0.3, mesh 256^3, P=8, rc=4, restart 30
+    "n1m": dict(n_bodies=50000, body_sizes=20, phi=0.3, charges="uniform", seed=3,
+                xi=0.6, errortol=1e-6, mesh=(256, 256, 256), P=8, rc=4.0, restart=30, tol=1e-6),
+}
+
+
+def make_workload(name: str) -> tuple[SystemDefinition, dict]:
+    w = dict(WORKLOADS[name])
+    sysdef = make_system(w.pop("n_bodies"), w.pop("body_sizes"), w.pop("phi"), seed=w.pop("seed"),
+                         charges=w.pop("charges"))
+    return sysdef, w

@@ -1,0 +1,197 @@
+/*
+ * conductorrigid.h -- C ABI of the MI355X-native ConductorRigid force compute.
+ *
+ * Drop-in boundary for the reference's device ABI (cppmodule/ConductorRigid.cuh:19-129):
+ * the reference's host class ConductorRigid (cppmodule/ConductorRigid.cc) calls four
+ * host-callable functions that take raw device pointers -- gpu_ZeroForce,
+ * gpu_ComputeForce, MatrixVectorMultiply and Precondition -- and owns a cuFFT plan,
+ * the k-space scale table and the real-space tables built by SetParams.  Here the
+ * same roles are served by an opaque context that owns the mesh buffers, FFT
+ * twiddles, tables and Krylov workspace; every compute entry point takes plain
+ * device pointers and sizes plus an explicit HIP stream (passed as void*), and
+ * returns an int status (CR_OK == 0).  No torch or HIP types appear here.
+ *
+ * Precision: positions, charges, potentials and forces cross the boundary as
+ * IEEE double (HOOMD's Scalar in double precision).  CR_PRECISION_FP64 computes
+ * everything in double ("plain method"); CR_PRECISION_FP32_BULK runs spreading,
+ * FFTs, contraction and the pair sum in float and keeps the Krylov solver, the
+ * residual check and the per-body constraint sums in double.
+ */
+#ifndef CONDUCTORRIGID_H
+#define CONDUCTORRIGID_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CR_OK 0
+#define CR_ERR_INVALID 1     /* bad argument or inconsistent sizes */
+#define CR_ERR_DEVICE 2      /* HIP runtime error */
+#define CR_ERR_NOT_CONVERGED 3
+#define CR_ERR_STATE 4       /* call out of order (e.g. compute before set_params) */
+
+#define CR_NOT_MEMBER 0xffffffffu
+
+#define CR_PRECISION_FP64 0
+#define CR_PRECISION_FP32_BULK 1
+
+/* x0 modes for the solve (the reference always uses the previous step's (q, Phi),
+ * cppmodule/ConductorRigid.cu:1125-1127) */
+#define CR_X0_PREVIOUS 0     /* start from the caller's q_inout / phi_inout */
+#define CR_X0_ZERO 1         /* start from zero */
+
+typedef struct cr_context cr_context;
+
+/* Numerical parameters: the reference derives all of them from (xi, errortol) in
+ * SetParams (cppmodule/ConductorRigid.cc:116-207); any field > 0 overrides the derived
+ * value.  GMRES defaults follow the reference (restart 10, 100 iterations, tolerance
+ * errortol, cppmodule/ConductorRigid.cu:1134-1136). */
+typedef struct {
+    double L[3];          /* orthorhombic box lengths; box spans [-L/2, L/2) */
+    double xi;            /* Ewald splitting parameter */
+    double errortol;      /* error tolerance */
+    double rc;            /* real-space cutoff override (0: sqrt(-ln tol)/xi) */
+    int mesh[3];          /* mesh override (0: derived, rounded up to 5-smooth) */
+    int P;                /* window support override (0: ceil(-2 ln tol / pi)) */
+    int precision;        /* CR_PRECISION_* */
+    int restart;          /* GMRES restart (0: 10) */
+    int max_iter;         /* GMRES iteration limit (0: 100) */
+    double gmres_tol;     /* relative GMRES tolerance (0: errortol) */
+} cr_params;
+
+/* Parameters as resolved by cr_set_params (SetParams' derived quantities). */
+typedef struct {
+    double rc, kcut;
+    int mesh[3];
+    int P;
+    double h[3], eta[3];
+    int N_table;
+    double drtable, selfterm, prefac, xiterm, quadW;
+    int precision, restart, max_iter;
+    double gmres_tol;
+} cr_resolved;
+
+/* HOOMD-shaped system view: all pointers are DEVICE pointers (borrowed for the call).
+ * Mirrors the arrays ConductorRigid::computeForces hands to gpu_ComputeForce
+ * (cppmodule/ConductorRigid.cc:420-544).  The active group is every particle. */
+typedef struct {
+    int N;                              /* beads */
+    const double* pos;                  /* [4N] x, y, z, type by global index */
+    const unsigned* tag;                /* [N] tag of each global index */
+    const unsigned* body_of_tag;        /* [N] isolated-body tag per particle tag (user `body`) */
+    const double* body_charge;          /* [N_isolated] net charge per isolated body (user `charge`) */
+    int N_rigid;                        /* rigid bodies */
+    int N_isolated;                     /* rigid bodies + isolated beads */
+    int Nmax;                           /* pitch of rigid_members */
+    const unsigned* rigid_membership;   /* [N] rigid index per global index or CR_NOT_MEMBER */
+    const unsigned* rigid_members;      /* [N_rigid*Nmax] global indices, CR_NOT_MEMBER padded */
+    const unsigned* rigid_size;         /* [N_rigid] */
+    const double* rigid_com;            /* [4*N_rigid] centres of mass */
+    const unsigned* nlist;              /* full neighbour list */
+    const unsigned* head_list;          /* [N] first neighbour of each bead */
+    const unsigned* n_neigh;            /* [N] neighbour count of each bead */
+} cr_system;
+
+typedef struct {
+    int iterations;        /* Arnoldi steps */
+    int matvecs;           /* operator applications incl. restart residuals */
+    int restarts;
+    int converged;         /* 1 when the residual estimate met tol * ||b|| */
+    double resid;          /* final preconditioned residual estimate */
+    double tolerance;      /* tol * ||b||_2 */
+    double true_resid;     /* ||M (b - A x)||_2 recomputed in FP64 after the solve */
+} cr_solve_stats;
+
+/* ---- lifetime ---------------------------------------------------------- */
+int  cr_create(cr_context** out, int device);
+void cr_destroy(cr_context* ctx);
+const char* cr_last_error(void);
+
+/* SetParams (cppmodule/ConductorRigid.cc:116-379): resolve parameters, build the
+ * real-space tables, FFT plan/twiddles and size the mesh workspace.  n_beads and
+ * n_isolated size the Krylov workspace. */
+int cr_set_params(cr_context* ctx, const cr_params* params, int n_beads, int n_isolated);
+int cr_get_resolved(const cr_context* ctx, cr_resolved* out);
+
+/* UpdateField (cppmodule/ConductorRigid.cc:382-387). */
+int cr_update_field(cr_context* ctx, const double E0[3]);
+
+/* ---- the reference's device ABI ---------------------------------------- */
+
+/* gpu_ZeroForce (cppmodule/ConductorRigid.cuh:19-21). force: device double[4N]. */
+int cr_zero_force(cr_context* ctx, int N, double* force, void* stream);
+
+/* gpu_ComputeForce (cppmodule/ConductorRigid.cuh:23-73): solve for the bead charges
+ * and body potentials (GMRES, x0 per x0_mode), write them back to q_inout [N] and
+ * phi_inout [N_isolated], and overwrite force [4N] with
+ * F = q E0 + reciprocal + real-space forces.  stats may be NULL. */
+int cr_compute_force(cr_context* ctx, const cr_system* sys, double* q_inout, double* phi_inout,
+                     double* force, int x0_mode, cr_solve_stats* stats, void* stream);
+
+/* MatrixVectorMultiply (cppmodule/ConductorRigid.cuh:76-114): y = A x with
+ * x = [q (N) ; Phi (N_isolated)], y = [Phi_b(i) - (M q)_i ; sum_{i in b} q_i]. */
+int cr_matvec(cr_context* ctx, const cr_system* sys, const double* x, double* y, void* stream);
+
+/* Precondition (cppmodule/ConductorRigid.cuh:117-129): y = D x, D diagonal. */
+int cr_precondition(cr_context* ctx, const cr_system* sys, const double* x, double* y, void* stream);
+
+/* compute_objective (cppmodule/ConductorRigid.cu:764-813): b = [-(r_i - R_b).E0 ; Q_b]. */
+int cr_objective(cr_context* ctx, const cr_system* sys, double* b, void* stream);
+
+/* ---- stages of the matvec, exposed for parity tests and kernel timing ---- */
+
+/* spread (cppmodule/ConductorRigid.cu:144-234) of the charges q [N] onto the real
+ * mesh; grid_out (may be NULL) receives a double copy [Nx*Ny*Nz], z fastest. */
+int cr_spread(cr_context* ctx, const cr_system* sys, const double* q, double* grid_out, void* stream);
+
+/* grid <- Re IFFT(S * FFT(grid)) on the context's real mesh (cufftExecC2C forward,
+ * scale, inverse: cppmodule/ConductorRigid.cu:927-933).  grid_in/grid_out (double,
+ * may be NULL) load / read back the real mesh. */
+int cr_recip_grid(cr_context* ctx, const double* grid_in, double* grid_out, void* stream);
+
+/* Forward / inverse 3-D FFT of a full complex double mesh (interleaved re,im),
+ * unnormalised, sign -1 / +1 (cuFFT C2C conventions).  For tests of the FFT kernels. */
+int cr_fft_c2c(cr_context* ctx, const double* in, double* out, int sign, void* stream);
+
+/* contract (cppmodule/ConductorRigid.cu:260-360): phi [N] from the context's mesh. */
+int cr_contract(cr_context* ctx, const cr_system* sys, double* phi, void* stream);
+
+/* contract_force (cppmodule/ConductorRigid.cu:363-466): force [4N] (overwritten). */
+int cr_contract_force(cr_context* ctx, const cr_system* sys, const double* q, double* force,
+                      void* stream);
+
+/* real_space (cppmodule/ConductorRigid.cu:469-567): y[i] = Phi[b(i)] - (y[i] + self q_i
+ * + sum_j A(r_ij) q_j) for x = [q ; Phi]. */
+int cr_real_space(cr_context* ctx, const cr_system* sys, const double* x, double* y, void* stream);
+
+/* real_space_force (cppmodule/ConductorRigid.cu:570-664): force += q E0 + pair forces. */
+int cr_real_space_force(cr_context* ctx, const cr_system* sys, const double* q, double* force,
+                        void* stream);
+
+/* ---- kernel timing ------------------------------------------------------
+ * Runs each stage of the matvec y = A x `reps` times on `stream`, bracketing every
+ * stage with HIP events, and writes the average milliseconds per launch into
+ * stage_ms[k] for k < min(max_stages, CR_NUM_STAGES), in this order:
+ *   0 mesh clear, 1 spread, 2 FFT z R2C, 3 FFT y forward, 4 FFT x forward+scale+inverse,
+ *   5 FFT y inverse, 6 FFT z C2R, 7 contract, 8 real space, 9 constraint rows.
+ * No reference counterpart: the reference has no per-kernel timing (HOOMD's
+ * Profiler wraps the whole step, cppmodule/ConductorRigid.cc:415-416). */
+#define CR_NUM_STAGES 10
+int cr_profile_matvec(cr_context* ctx, const cr_system* sys, const double* x, int reps, double* stage_ms,
+                      int max_stages, void* stream);
+
+/* ---- neighbour list (the HOOMD NeighborListGPUBinned the reference's compute.py
+ * creates, pymodule/compute.py:38-44): full list, r < rlist, intra-body pairs kept.
+ * head_list/n_neigh: device uint[N]; nlist: device uint[capacity].  Returns the
+ * total pair count in *total; if it exceeds capacity nothing past capacity is
+ * written and the caller retries with a larger buffer. */
+int cr_build_nlist(cr_context* ctx, int N, const double* pos, const double L[3], double rlist,
+                   unsigned* head_list, unsigned* n_neigh, unsigned* nlist, long capacity,
+                   long* total, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
