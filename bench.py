@@ -101,7 +101,7 @@ def main() -> int:
         f"({time.perf_counter() - t0:.1f} s)")
     prm = dict(xi=w["xi"], errortol=w["errortol"], rc=w["rc"], mesh=w["mesh"], P=w["P"])
     common = dict(system=sysdef, field=(0.0, 0.0, 1.0), restart=w["restart"], gmres_tol=w["tol"], max_iter=1000,
-                  x0="zero", **prm)
+                  x0="zero", preconditioner="block", **prm)
 
     def time_steps(cr, steps, warmup):
         for _ in range(warmup):

@@ -17,6 +17,8 @@ CR_PRECISION_FP64 = 0
 CR_PRECISION_FP32_BULK = 1
 CR_X0_PREVIOUS = 0
 CR_X0_ZERO = 1
+CR_PRECOND_DIAGONAL = 0
+CR_PRECOND_NEARFIELD_BLOCK = 1
 CR_NUM_STAGES = 10
 STAGE_NAMES = ["mesh_clear", "spread", "fft_z_r2c", "fft_y_fwd", "fft_x_scale", "fft_y_inv", "fft_z_c2r",
                "contract", "real_space", "constraint_rows"]
@@ -27,7 +29,7 @@ vp = C.c_void_p
 class CrParams(C.Structure):
     _fields_ = [("L", C.c_double * 3), ("xi", C.c_double), ("errortol", C.c_double), ("rc", C.c_double),
                 ("mesh", C.c_int * 3), ("P", C.c_int), ("precision", C.c_int), ("restart", C.c_int),
-                ("max_iter", C.c_int), ("gmres_tol", C.c_double)]
+                ("max_iter", C.c_int), ("gmres_tol", C.c_double), ("preconditioner", C.c_int)]
 
 
 class CrResolved(C.Structure):
@@ -53,7 +55,8 @@ class CrSystem(C.Structure):
 
 class CrSolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("matvecs", C.c_int), ("restarts", C.c_int), ("converged", C.c_int),
-                ("resid", C.c_double), ("tolerance", C.c_double), ("true_resid", C.c_double)]
+                ("resid", C.c_double), ("tolerance", C.c_double), ("true_resid", C.c_double),
+                ("true_relres", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

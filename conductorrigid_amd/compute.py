@@ -26,6 +26,7 @@ from . import _lib
 from .system import NOT_MEMBER, SystemDefinition
 
 _PRECISIONS = {"fp64": _lib.CR_PRECISION_FP64, "fp32": _lib.CR_PRECISION_FP32_BULK}
+_PRECONDITIONERS = {"diagonal": _lib.CR_PRECOND_DIAGONAL, "block": _lib.CR_PRECOND_NEARFIELD_BLOCK}
 
 
 def _torch():
@@ -112,7 +113,8 @@ class ConductorRigid:
 
     def __init__(self, group=None, body=None, charge=None, field=(0.0, 0.0, 0.0), xi=0.5, errortol=1e-3,
                  fileprefix="", period=0, *, system: SystemDefinition, mesh=None, P=0, rc=0.0, restart=0,
-                 max_iter=0, gmres_tol=0.0, precision="fp64", x0="previous", skin=0.4, device=0, t0=0):
+                 max_iter=0, gmres_tol=0.0, precision="fp64", x0="previous", skin=0.4, device=0, t0=0,
+                 preconditioner="diagonal"):
         if group not in (None, "all"):
             raise ValueError("only the group of all particles is supported")
         torch = _torch()
@@ -136,7 +138,7 @@ class ConductorRigid:
         self.t0 = int(t0)
         self.enabled = True
         self._opts = dict(mesh=mesh, P=P, rc=rc, restart=restart, max_iter=max_iter, gmres_tol=gmres_tol,
-                          precision=precision)
+                          precision=precision, preconditioner=preconditioner)
         self.x0_mode = {"previous": _lib.CR_X0_PREVIOUS, "zero": _lib.CR_X0_ZERO}[x0]
         ctx = C.c_void_p()
         _lib.check(self.lib.cr_create(C.byref(ctx), int(device)))
@@ -158,7 +160,8 @@ class ConductorRigid:
         mesh = o["mesh"] or (0, 0, 0)
         p = _lib.CrParams((C.c_double * 3)(*[float(x) for x in self.system.L]), self.xi, self.errortol,
                           float(o["rc"]), (C.c_int * 3)(*[int(m) for m in mesh]), int(o["P"]),
-                          _PRECISIONS[o["precision"]], int(o["restart"]), int(o["max_iter"]), float(o["gmres_tol"]))
+                          _PRECISIONS[o["precision"]], int(o["restart"]), int(o["max_iter"]), float(o["gmres_tol"]),
+                          _PRECONDITIONERS[o["preconditioner"]])
         _lib.check(self.lib.cr_set_params(self.ctx, C.byref(p), self.system.N, self.system.N_isolated))
         r = _lib.CrResolved()
         _lib.check(self.lib.cr_get_resolved(self.ctx, C.byref(r)))

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -150,11 +151,21 @@ struct cr_context {
     DevBuf tab_d, gtab_d, tab_f, gtab_f;
     DevBuf grid_real, grid_cplx;
     DevBuf phi_recip, iso;
+    // real-space operator assembled per force evaluation (CSR over the neighbour list rows)
+    bool csr_ready = false;
+    long rs_ldn = 0;
+    DevBuf rs_col, rs_val, rs_len;
+    // bead binning for the owner-computes spread
+    TileGeom tg{};
+    DevBuf bead_tile, tile_count, tile_start, tile_cursor, sorted, scan_sums, halo;
     // Krylov workspace
     int R = 10;
     long ld = 0;
     int n_cap = 0;
-    DevBuf V, w, b, Mb, x, partial, scal;
+    DevBuf V, w, b, Mb, x, partial, scal, yraw;
+    // near-field block preconditioner: beads grouped by body tag, per-body inverses
+    int block_nmax = 0;
+    DevBuf body_count, body_start, body_cursor, body_beads, ginv, uvec, inv_s;
     GmresState* g_dev = nullptr;
     GmresState* g_host = nullptr;
     double E0[3] = {0, 0, 0};
@@ -209,6 +220,20 @@ int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out)
     CR_HIP(c->iso.ensure(sizeof(unsigned) * (size_t)std::max(1, s->N)));
     CR_HIP(launch_membership(s->N, s->tag, s->body_of_tag, c->iso.as<unsigned>(), st));
     *out = make_view(c, s);
+    c->csr_ready = false;  // positions or neighbours may have changed
+    if (c->tg.valid) {
+        // positions may have changed since the last call: rebin (cheap, O(N))
+        const size_t nt = (size_t)c->tg.ntiles;
+        CR_HIP(c->bead_tile.ensure(sizeof(unsigned) * (size_t)std::max(1, s->N)));
+        CR_HIP(c->sorted.ensure(sizeof(unsigned) * (size_t)std::max(1, s->N)));
+        CR_HIP(c->tile_count.ensure(sizeof(unsigned) * (nt + 1)));
+        CR_HIP(c->tile_start.ensure(sizeof(unsigned) * (nt + 1)));
+        CR_HIP(c->tile_cursor.ensure(sizeof(unsigned) * nt));
+        CR_HIP(c->scan_sums.ensure(sizeof(unsigned long long) * ((nt + 1) / 1024 + 4)));
+        CR_HIP(launch_bin_beads(c->mp, c->tg, *out, c->bead_tile.as<unsigned>(), c->tile_count.as<unsigned>(),
+                                c->tile_start.as<unsigned>(), c->tile_cursor.as<unsigned>(), c->sorted.as<unsigned>(),
+                                c->scan_sums.as<unsigned long long>(), st));
+    }
     return CR_OK;
 }
 
@@ -246,11 +271,84 @@ int recip_impl(cr_context* c, hipStream_t st)
     return CR_OK;
 }
 
+bool contract_tiled_ok(const cr_context* c, size_t elem)
+{
+    return c->tg.valid && tiled_lds(c->mp, c->tg, elem).chunk >= 4;
+}
+
+// phi <- contraction of the context mesh (binned LDS version when the tile + halo fits)
+template <typename T>
+int contract_impl(cr_context* c, const SysView& v, T* phi, hipStream_t st)
+{
+    if (contract_tiled_ok(c, sizeof(T)))
+        CR_HIP(launch_contract_tiled<T>(c->mp, c->tg, v, c->tile_start.as<unsigned>(), c->sorted.as<unsigned>(),
+                                        nullptr, c->grid_real.as<T>(), phi, nullptr, st));
+    else
+        CR_HIP(launch_contract<T>(c->mp, v, c->grid_real.as<T>(), phi, st));
+    return CR_OK;
+}
+
+template <typename T>
+int contract_force_impl(cr_context* c, const SysView& v, const double* q, double* force, hipStream_t st)
+{
+    if (contract_tiled_ok(c, sizeof(T)))
+        CR_HIP(launch_contract_tiled<T>(c->mp, c->tg, v, c->tile_start.as<unsigned>(), c->sorted.as<unsigned>(), q,
+                                        c->grid_real.as<T>(), nullptr, force, st));
+    else
+        CR_HIP(launch_contract_force<T>(c->mp, v, q, c->grid_real.as<T>(), force, st));
+    return CR_OK;
+}
+
+// Assemble the real-space operator for the current positions (valid until the next
+// prepare()).  The pair arrays share the neighbour list's row offsets.
+int ensure_csr(cr_context* c, const SysView& v, hipStream_t st)
+{
+    if (c->csr_ready || v.N <= 0) return CR_OK;
+    // ELL width: the longest neighbour row bounds the longest in-cutoff row
+    unsigned* dmax = reinterpret_cast<unsigned*>(c->scal.as<double>() + 32);
+    CR_HIP(launch_fill<unsigned>(dmax, 0u, 1, st));
+    CR_HIP(launch_max_u32(v.n_neigh, v.N, dmax, st));
+    unsigned width = 0;
+    CR_HIP(hipMemcpyAsync(&width, dmax, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    CR_HIP(hipStreamSynchronize(st));
+    c->rs_ldn = ((long)v.N + 63) / 64 * 64;
+    const size_t cells = std::max<size_t>(1, (size_t)width * (size_t)c->rs_ldn);
+    const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
+    CR_HIP(c->rs_col.ensure(sizeof(unsigned) * cells));
+    CR_HIP(c->rs_val.ensure(Tsz * cells));
+    CR_HIP(c->rs_len.ensure(sizeof(unsigned) * (size_t)v.N));
+    if (c->fp32())
+        CR_HIP(launch_real_csr_build<float>(c->rp, v, table<float>(c, false), c->rs_ldn, c->rs_col.as<unsigned>(),
+                                            c->rs_val.as<float>(), c->rs_len.as<unsigned>(), st));
+    else
+        CR_HIP(launch_real_csr_build<double>(c->rp, v, table<double>(c, false), c->rs_ldn, c->rs_col.as<unsigned>(),
+                                             c->rs_val.as<double>(), c->rs_len.as<unsigned>(), st));
+    c->csr_ready = true;
+    return CR_OK;
+}
+
+// the spread kernel alone (the atomic fallback expects a cleared mesh)
+template <typename T>
+int spread_only(cr_context* c, const SysView& v, const double* q, hipStream_t st)
+{
+    if (c->tg.valid)
+        CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, v, c->tile_start.as<unsigned>(), c->sorted.as<unsigned>(), q,
+                                      c->grid_real.as<T>(), c->halo.as<T>(), st));
+    else
+        CR_HIP(launch_spread<T>(c->mp, v, q, c->grid_real.as<T>(), st));
+    return CR_OK;
+}
+
 template <typename T>
 int spread_impl(cr_context* c, const SysView& v, const double* q, hipStream_t st)
 {
-    CR_HIP(launch_fill<T>(c->grid_real.as<T>(), (T)0, c->G, st));
-    CR_HIP(launch_spread<T>(c->mp, v, q, c->grid_real.as<T>(), st));
+    if (c->tg.valid) {
+        CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, v, c->tile_start.as<unsigned>(), c->sorted.as<unsigned>(), q,
+                                      c->grid_real.as<T>(), c->halo.as<T>(), st));
+    } else {
+        CR_HIP(launch_fill<T>(c->grid_real.as<T>(), (T)0, c->G, st));
+        CR_HIP(launch_spread<T>(c->mp, v, q, c->grid_real.as<T>(), st));
+    }
     return CR_OK;
 }
 
@@ -261,8 +359,13 @@ int matvec_impl(cr_context* c, const SysView& v, const double* x, double* y, int
 {
     CR_RET(spread_impl<T>(c, v, x, st));
     CR_RET(recip_impl<T>(c, st));
-    CR_HIP(launch_contract<T>(c->mp, v, c->grid_real.as<T>(), c->phi_recip.as<T>(), st));
-    CR_HIP(launch_real_space<T>(c->rp, v, x, c->phi_recip.as<T>(), table<T>(c, false), y, precond, st));
+    CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
+    if (c->csr_ready)
+        CR_HIP(launch_real_space_csr<T>(c->rp, v, x, c->phi_recip.as<T>(), c->rs_ldn, c->rs_col.as<unsigned>(),
+                                        c->rs_val.as<T>(),
+                                        c->rs_len.as<unsigned>(), y, precond, st));
+    else
+        CR_HIP(launch_real_space<T>(c->rp, v, x, c->phi_recip.as<T>(), table<T>(c, false), y, precond, st));
     CR_HIP(launch_constraint_rows(v, x, y + v.N, precond, st));
     return CR_OK;
 }
@@ -287,6 +390,55 @@ int read_scalar(cr_context* c, const double* dev, double* host, hipStream_t st)
     return CR_OK;
 }
 
+bool block_precond(const cr_context* c) { return c->params.preconditioner == CR_PRECOND_NEARFIELD_BLOCK; }
+
+// Build the per-body inverses of the near-field block preconditioner for the current
+// positions (once per force evaluation).
+int block_setup(cr_context* c, const SysView& v, hipStream_t st)
+{
+    const int B = v.N_isolated;
+    const int nmax = std::max(1, v.Nmax);
+    if (nmax > 64) return set_error(CR_ERR_INVALID, "near-field block preconditioner supports bodies of <= 64 beads");
+    c->block_nmax = nmax;
+    const size_t b1 = (size_t)std::max(1, B);
+    CR_HIP(c->body_count.ensure(sizeof(unsigned) * (b1 + 1)));
+    CR_HIP(c->body_start.ensure(sizeof(unsigned) * (b1 + 1)));
+    CR_HIP(c->body_cursor.ensure(sizeof(unsigned) * b1));
+    CR_HIP(c->body_beads.ensure(sizeof(unsigned) * (size_t)std::max(1, v.N)));
+    CR_HIP(c->ginv.ensure(sizeof(float) * b1 * nmax * nmax));
+    CR_HIP(c->uvec.ensure(sizeof(float) * b1 * nmax));
+    CR_HIP(c->inv_s.ensure(sizeof(double) * b1));
+    CR_HIP(c->scan_sums.ensure(sizeof(unsigned long long) * std::max<size_t>((b1 + 1) / 1024 + 4,
+                                                                             c->scan_sums.bytes / 8)));
+    CR_HIP(launch_block_setup(v, c->mp.L, nmax, c->body_count.as<unsigned>(), c->body_start.as<unsigned>(),
+                              c->body_cursor.as<unsigned>(), c->body_beads.as<unsigned>(),
+                              c->scan_sums.as<unsigned long long>(), c->ginv.as<float>(), c->uvec.as<float>(),
+                              c->inv_s.as<double>(), st));
+    return CR_OK;
+}
+
+// out = D in with the configured preconditioner (diagonal: the reference's; block: near field)
+int apply_precond(cr_context* c, const SysView& v, const double* in, double* out, hipStream_t st)
+{
+    if (block_precond(c)) {
+        CR_HIP(launch_block_apply(v, c->block_nmax, c->body_start.as<unsigned>(), c->body_beads.as<unsigned>(),
+                                  c->ginv.as<float>(), c->uvec.as<float>(), c->inv_s.as<double>(), in, out, st));
+    } else {
+        CR_HIP(launch_precondition(v, in, out, st));
+    }
+    return CR_OK;
+}
+
+// out = D A x (diagonal D is fused into the matvec's last kernels)
+int matvec_prec(cr_context* c, const SysView& v, const double* x, double* out, hipStream_t st)
+{
+    if (block_precond(c)) {
+        CR_RET(matvec_any(c, v, x, c->yraw.as<double>(), 0, st));
+        return apply_precond(c, v, c->yraw.as<double>(), out, st);
+    }
+    return matvec_any(c, v, x, out, 1, st);
+}
+
 // Restarted, left-preconditioned GMRES(R) following CUSP's krylov::gmres structure
 // (cppmodule/ConductorRigid.cu:1134-1136): the cycle starts from w = D(Ax - b), stops
 // when the Givens estimate |s_{i+1}| <= tol ||b||_2, x += V y at the end of each cycle.
@@ -303,7 +455,9 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
     GmresState* g = c->g_dev;
     const int nblk = krylov_blocks(n);
 
-    CR_RET(norm2(c, c->b.as<double>(), n, scal, st));
+    // stopping scale: ||b|| for the reference's diagonal preconditioner (CUSP's
+    // default_monitor), ||D b|| for the block preconditioner
+    CR_RET(norm2(c, block_precond(c) ? Mb : c->b.as<double>(), n, scal, st));
     double bnorm2 = 0;
     CR_RET(read_scalar(c, scal, &bnorm2, st));
     const double tolerance = c->res.gmres_tol * std::sqrt(bnorm2);
@@ -311,7 +465,7 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
     int iter = 0, matvecs = 0, restarts = 0, converged = 0;
     double resid = 0;
     for (;;) {
-        CR_RET(matvec_any(c, v, x, w, 1, st));
+        CR_RET(matvec_prec(c, v, x, w, st));
         ++matvecs;
         CR_HIP(launch_axpby(-1.0, Mb, 1.0, w, n, st));
         CR_RET(norm2(c, w, n, scal, st));
@@ -330,7 +484,7 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
             ++i;
             ++iter;
             double* vi = V + (long)i * c->ld;
-            CR_RET(matvec_any(c, v, vi, w, 1, st));
+            CR_RET(matvec_prec(c, v, vi, w, st));
             ++matvecs;
             // CGS2: two projections onto V[:,0:i+1]
             CR_HIP(launch_multidot(V, c->ld, i + 1, w, n, partial, st));
@@ -361,6 +515,7 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
         stats->resid = resid;
         stats->tolerance = tolerance;
         stats->true_resid = -1.0;
+        stats->true_relres = -1.0;
     }
     return CR_OK;
 }
@@ -370,7 +525,7 @@ int force_pass(cr_context* c, const SysView& v, const double* q, double* force, 
 {
     CR_RET(spread_impl<T>(c, v, q, st));
     CR_RET(recip_impl<T>(c, st));
-    CR_HIP(launch_contract_force<T>(c->mp, v, q, c->grid_real.as<T>(), force, st));
+    CR_RET(contract_force_impl<T>(c, v, q, force, st));
     CR_HIP(launch_real_space_force<T>(c->rp, v, q, table<T>(c, true), c->E0, force, st));
     return CR_OK;
 }
@@ -430,6 +585,8 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
         if (!(p->L[d] > 0)) return set_error(CR_ERR_INVALID, "box lengths must be positive");
     if (p->precision != CR_PRECISION_FP64 && p->precision != CR_PRECISION_FP32_BULK)
         return set_error(CR_ERR_INVALID, "unknown precision");
+    if (p->preconditioner != CR_PRECOND_DIAGONAL && p->preconditioner != CR_PRECOND_NEARFIELD_BLOCK)
+        return set_error(CR_ERR_INVALID, "unknown preconditioner");
     if (n_beads < 0 || n_isolated < 0) return set_error(CR_ERR_INVALID, "negative sizes");
     CR_HIP(hipSetDevice(c->device));
     c->ready = false;
@@ -473,7 +630,8 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
     r.max_iter = (p->max_iter > 0) ? p->max_iter : 100;
     r.gmres_tol = (p->gmres_tol > 0) ? p->gmres_tol : tol;
     if (r.restart > 32) return set_error(CR_ERR_INVALID, "restart must be <= 32");
-    const int max_n = (p->precision == CR_PRECISION_FP64) ? 2048 : 4096;
+    // one LDS-resident pencil plus the twiddle table must fit 64 KiB
+    const int max_n = (p->precision == CR_PRECISION_FP64) ? 1024 : 2048;
     for (int d = 0; d < 3; ++d)
         if (r.mesh[d] > max_n) return set_error(CR_ERR_INVALID, "mesh too large for the LDS-resident FFT in this precision");
 
@@ -545,7 +703,69 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
         c->sc.e[d] = c->etab[d].as<double>();
     }
     c->sc.invG = 1.0 / (double)c->G;
+    // binned spread/contract tiling: per axis the largest divisor of the mesh size in
+    // [max(P-1, 4), 16], else the smallest one in (16, 32]; the tile extended by P-1
+    // nodes per axis must fit 64 KiB of LDS, shrinking z, then y, then x if needed
     const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
+    {
+        TileGeom& tg = c->tg;
+        tg.valid = 1;
+        const int lo = std::max(r.P - 1, 4);
+        std::vector<int> opts[3];
+        for (int d = 0; d < 3; ++d) {
+            const int n = Nd[d];
+            for (int t = std::min(n, 32); t >= lo; --t)
+                if (n % t == 0) opts[d].push_back(t);      // descending
+            int pick = -1;
+            for (size_t i = 0; i < opts[d].size(); ++i)
+                if (opts[d][i] <= 16) {
+                    pick = (int)i;
+                    break;
+                }
+            if (pick < 0 && !opts[d].empty()) pick = (int)opts[d].size() - 1;
+            if (pick < 0) {
+                tg.valid = 0;
+                tg.T[d] = n;
+            } else {
+                tg.T[d] = opts[d][pick];
+                opts[d].erase(opts[d].begin(), opts[d].begin() + pick);  // remaining: smaller choices
+            }
+        }
+        auto ext_bytes = [&]() {
+            return (size_t)(tg.T[0] + r.P - 1) * (tg.T[1] + r.P - 1) * (tg.T[2] + r.P - 1) * Tsz;
+        };
+        // leave about a quarter of the LDS budget for the per-bead staging tables
+        const size_t ext_cap = kTiledLdsBudget * 3 / 4;
+        for (int d = 2; tg.valid && ext_bytes() > ext_cap && d >= 0;) {
+            if (opts[d].size() > 1) {
+                opts[d].erase(opts[d].begin());
+                tg.T[d] = opts[d][0];
+            } else {
+                --d;
+            }
+        }
+        if (ext_bytes() > ext_cap) tg.valid = 0;
+        // tuning override: CR_TILE="tx,ty,tz" (used when each edge divides the mesh, is
+        // >= P-1 and the extended tile fits the LDS budget)
+        if (const char* env = std::getenv("CR_TILE")) {
+            int t3[3];
+            if (std::sscanf(env, "%d,%d,%d", &t3[0], &t3[1], &t3[2]) == 3) {
+                bool ok = true;
+                for (int d = 0; d < 3; ++d) ok = ok && t3[d] >= std::max(r.P - 1, 1) && Nd[d] % t3[d] == 0;
+                const size_t eb = (size_t)(t3[0] + r.P - 1) * (t3[1] + r.P - 1) * (t3[2] + r.P - 1) * Tsz;
+                if (ok && eb <= ext_cap) {
+                    for (int d = 0; d < 3; ++d) tg.T[d] = t3[d];
+                    tg.valid = 1;
+                }
+            }
+        }
+        tg.ntiles = 1;
+        for (int d = 0; d < 3; ++d) {
+            tg.nt[d] = Nd[d] / tg.T[d];
+            tg.ntiles *= tg.nt[d];
+        }
+        if (tg.valid) CR_HIP(c->halo.ensure(ext_bytes() * (size_t)tg.ntiles));
+    }
     CR_HIP(c->grid_real.ensure(Tsz * (size_t)c->G));
     CR_HIP(c->grid_cplx.ensure(2 * Tsz * (size_t)c->Gh));
 
@@ -559,6 +779,7 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
     CR_HIP(c->b.ensure(sizeof(double) * (size_t)c->ld));
     CR_HIP(c->Mb.ensure(sizeof(double) * (size_t)c->ld));
     CR_HIP(c->x.ensure(sizeof(double) * (size_t)c->ld));
+    CR_HIP(c->yraw.ensure(sizeof(double) * (size_t)c->ld));
     CR_HIP(c->partial.ensure(sizeof(double) * 32 * 1024));
     CR_HIP(c->scal.ensure(sizeof(double) * 64));
     c->ready = true;
@@ -601,8 +822,8 @@ int cr_precondition(cr_context* c, const cr_system* s, const double* x, double* 
     hipStream_t st = (hipStream_t)stream;
     SysView v;
     CR_RET(prepare(c, s, st, &v));
-    CR_HIP(launch_precondition(v, x, y, st));
-    return CR_OK;
+    if (block_precond(c)) CR_RET(block_setup(c, v, st));
+    return apply_precond(c, v, x, y, st);
 }
 
 int cr_matvec(cr_context* c, const cr_system* s, const double* x, double* y, void* stream)
@@ -610,6 +831,7 @@ int cr_matvec(cr_context* c, const cr_system* s, const double* x, double* y, voi
     hipStream_t st = (hipStream_t)stream;
     SysView v;
     CR_RET(prepare(c, s, st, &v));
+    CR_RET(ensure_csr(c, v, st));
     return matvec_any(c, v, x, y, 0, st);
 }
 
@@ -626,7 +848,8 @@ int cr_compute_force(cr_context* c, const cr_system* s, double* q_inout, double*
     double* x = c->x.as<double>();
     // objective and its preconditioned copy (MatrixVectorSolve, cppmodule/ConductorRigid.cu:1099-1107)
     CR_HIP(launch_objective(v, c->mp.L, c->E0, c->b.as<double>(), st));
-    CR_HIP(launch_precondition(v, c->b.as<double>(), c->Mb.as<double>(), st));
+    if (block_precond(c)) CR_RET(block_setup(c, v, st));
+    CR_RET(apply_precond(c, v, c->b.as<double>(), c->Mb.as<double>(), st));
     if (x0_mode == CR_X0_ZERO) {
         CR_HIP(launch_fill<double>(x, 0.0, n, st));
     } else if (x0_mode == CR_X0_PREVIOUS) {
@@ -635,18 +858,28 @@ int cr_compute_force(cr_context* c, const cr_system* s, double* q_inout, double*
     } else {
         return set_error(CR_ERR_INVALID, "unknown x0 mode");
     }
+    CR_RET(ensure_csr(c, v, st));
     CR_RET(gmres(c, v, stats, st));
     CR_HIP(hipMemcpyAsync(q_inout, x, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
     CR_HIP(hipMemcpyAsync(phi_inout, x + v.N, sizeof(double) * v.N_isolated, hipMemcpyDeviceToDevice, st));
     if (stats) {
         // FP64 judge of the converged state: ||D (b - A x)||_2
         double* w = c->w.as<double>();
-        CR_RET(matvec_any(c, v, x, w, 1, st));
+        CR_RET(matvec_prec(c, v, x, w, st));
         CR_HIP(launch_axpby(-1.0, c->Mb.as<double>(), 1.0, w, n, st));
         CR_RET(norm2(c, w, n, c->scal.as<double>(), st));
         double r2 = 0;
         CR_RET(read_scalar(c, c->scal.as<double>(), &r2, st));
         stats->true_resid = std::sqrt(r2);
+        // unpreconditioned relative residual ||b - A x|| / ||b||
+        CR_RET(matvec_any(c, v, x, w, 0, st));
+        CR_HIP(launch_axpby(-1.0, c->b.as<double>(), 1.0, w, n, st));
+        CR_RET(norm2(c, w, n, c->scal.as<double>(), st));
+        CR_RET(norm2(c, c->b.as<double>(), n, c->scal.as<double>() + 1, st));
+        double rb[2] = {0, 0};
+        CR_HIP(hipMemcpyAsync(rb, c->scal.as<double>(), 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        CR_HIP(hipStreamSynchronize(st));
+        stats->true_relres = rb[1] > 0 ? std::sqrt(rb[0] / rb[1]) : std::sqrt(rb[0]);
     }
     if (c->fp32())
         CR_RET(force_pass<float>(c, v, x, force, st));
@@ -721,10 +954,10 @@ int cr_contract(cr_context* c, const cr_system* s, double* phi, void* stream)
     SysView v;
     CR_RET(prepare(c, s, st, &v));
     if (c->fp32()) {
-        CR_HIP(launch_contract<float>(c->mp, v, c->grid_real.as<float>(), c->phi_recip.as<float>(), st));
+        CR_RET(contract_impl<float>(c, v, c->phi_recip.as<float>(), st));
         CR_HIP((launch_convert<float, double>(c->phi_recip.as<float>(), phi, v.N, st)));
     } else {
-        CR_HIP(launch_contract<double>(c->mp, v, c->grid_real.as<double>(), phi, st));
+        CR_RET(contract_impl<double>(c, v, phi, st));
     }
     return CR_OK;
 }
@@ -735,9 +968,9 @@ int cr_contract_force(cr_context* c, const cr_system* s, const double* q, double
     SysView v;
     CR_RET(prepare(c, s, st, &v));
     if (c->fp32())
-        CR_HIP(launch_contract_force<float>(c->mp, v, q, c->grid_real.as<float>(), force, st));
+        CR_RET(contract_force_impl<float>(c, v, q, force, st));
     else
-        CR_HIP(launch_contract_force<double>(c->mp, v, q, c->grid_real.as<double>(), force, st));
+        CR_RET(contract_force_impl<double>(c, v, q, force, st));
     return CR_OK;
 }
 
@@ -775,6 +1008,7 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
     SysView v;
     CR_RET(prepare(c, s, st, &v));
     if (reps < 1 || !stage_ms || !x) return set_error(CR_ERR_INVALID, "bad argument");
+    CR_RET(ensure_csr(c, v, st));
     const int nst = CR_NUM_STAGES;
     hipEvent_t ev[nst + 1];
     for (auto& e : ev) CR_HIP(hipEventCreate(&e));
@@ -787,9 +1021,9 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
             const AxisPlan* pl = plans<T>(c);
             T* real = c->grid_real.as<T>();
             T* cp = c->grid_cplx.as<T>();
-            CR_HIP(launch_fill<T>(real, (T)0, c->G, st));
+            if (!c->tg.valid) CR_HIP(launch_fill<T>(real, (T)0, c->G, st));
             CR_HIP(hipEventRecord(ev[1], st));
-            CR_HIP(launch_spread<T>(c->mp, v, x, real, st));
+            CR_RET(spread_only<T>(c, v, x, st));
             CR_HIP(hipEventRecord(ev[2], st));
             CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st));
             CR_HIP(hipEventRecord(ev[3], st));
@@ -801,17 +1035,18 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
             CR_HIP(hipEventRecord(ev[6], st));
             CR_HIP(launch_fft_z_c2r<T>(pl[2], c->mp.Nx, c->mp.Ny, cp, real, st));
             CR_HIP(hipEventRecord(ev[7], st));
-            CR_HIP(launch_contract<T>(c->mp, v, real, c->phi_recip.as<T>(), st));
+            CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
             CR_HIP(hipEventRecord(ev[8], st));
-            CR_HIP(launch_real_space<T>(c->rp, v, x, c->phi_recip.as<T>(), table<T>(c, false), y, 1, st));
+            CR_HIP(launch_real_space_csr<T>(c->rp, v, x, c->phi_recip.as<T>(), c->rs_ldn, c->rs_col.as<unsigned>(),
+                                            c->rs_val.as<T>(), c->rs_len.as<unsigned>(), y, 1, st));
         } else {
             using T = double;
             const AxisPlan* pl = plans<T>(c);
             T* real = c->grid_real.as<T>();
             T* cp = c->grid_cplx.as<T>();
-            CR_HIP(launch_fill<T>(real, (T)0, c->G, st));
+            if (!c->tg.valid) CR_HIP(launch_fill<T>(real, (T)0, c->G, st));
             CR_HIP(hipEventRecord(ev[1], st));
-            CR_HIP(launch_spread<T>(c->mp, v, x, real, st));
+            CR_RET(spread_only<T>(c, v, x, st));
             CR_HIP(hipEventRecord(ev[2], st));
             CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st));
             CR_HIP(hipEventRecord(ev[3], st));
@@ -823,9 +1058,10 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
             CR_HIP(hipEventRecord(ev[6], st));
             CR_HIP(launch_fft_z_c2r<T>(pl[2], c->mp.Nx, c->mp.Ny, cp, real, st));
             CR_HIP(hipEventRecord(ev[7], st));
-            CR_HIP(launch_contract<T>(c->mp, v, real, c->phi_recip.as<T>(), st));
+            CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
             CR_HIP(hipEventRecord(ev[8], st));
-            CR_HIP(launch_real_space<T>(c->rp, v, x, c->phi_recip.as<T>(), table<T>(c, false), y, 1, st));
+            CR_HIP(launch_real_space_csr<T>(c->rp, v, x, c->phi_recip.as<T>(), c->rs_ldn, c->rs_col.as<unsigned>(),
+                                            c->rs_val.as<T>(), c->rs_len.as<unsigned>(), y, 1, st));
         }
         CR_HIP(hipEventRecord(ev[9], st));
         CR_HIP(launch_constraint_rows(v, x, y + v.N, 1, st));

@@ -52,11 +52,44 @@ struct SysView {
     const unsigned* n_neigh;
 };
 
+// Mesh tiling for the owner-computes spread: tile edges divide the mesh and are at
+// least P-1 nodes, so a bead's window touches its own tile and the next one per axis.
+struct TileGeom {
+    int T[3];      // tile edge per axis (nodes)
+    int nt[3];     // tiles per axis
+    int ntiles;
+    int valid;     // 0: no admissible tiling, use the atomic spread
+};
+
 // ---- k_se.hip ----
 template <typename T>
 hipError_t launch_spread(const MeshParams& mp, const SysView& s, const double* q, T* grid, hipStream_t st);
+// Bin beads by the tile holding their window's first node (counting sort):
+// tile_count[ntiles+1] (zeroed here), tile_start[ntiles+1], cursor[ntiles], sorted[N].
+hipError_t launch_bin_beads(const MeshParams& mp, const TileGeom& tg, const SysView& s, unsigned* bead_tile,
+                            unsigned* tile_count, unsigned* tile_start, unsigned* cursor, unsigned* sorted,
+                            unsigned long long* scan_sums, hipStream_t st);
+// halo: per-tile scratch of (T+P-1)^3 elements per tile (the +side extensions)
+template <typename T>
+hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const SysView& s, const unsigned* tile_start,
+                               const unsigned* sorted, const double* q, T* grid, T* halo, hipStream_t st);
 template <typename T>
 hipError_t launch_contract(const MeshParams& mp, const SysView& s, const T* grid, T* phi, hipStream_t st);
+// LDS plan of the binned kernels (k_tiled.hip): the tile extended by P-1 nodes per axis
+// plus per-bead staging tables for `chunk` beads, within kTiledLdsBudget bytes.
+constexpr size_t kTiledLdsBudget = 64 * 1024;
+struct TiledLds {
+    size_t ext_bytes;
+    size_t total_bytes;
+    int chunk;
+};
+TiledLds tiled_lds(const MeshParams& mp, const TileGeom& tg, size_t elem);
+// Binned contraction from an LDS copy of the tile and its +side halo: potential (force == nullptr)
+// or reciprocal force (phi unused).
+template <typename T>
+hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const SysView& s, const unsigned* tile_start,
+                                 const unsigned* sorted, const double* q, const T* grid, T* phi, double* force,
+                                 hipStream_t st);
 template <typename T>
 hipError_t launch_contract_force(const MeshParams& mp, const SysView& s, const double* q, const T* grid,
                                  double* force, hipStream_t st);
@@ -91,6 +124,17 @@ hipError_t launch_fft_x_scale(const AxisPlan& px, int Nx, int Ny, int Zc, T* cpl
 template <typename T>
 hipError_t launch_real_space(const RealParams& rp, const SysView& s, const double* x, const T* phi_recip,
                              const T* table, double* y, int precond, hipStream_t st);
+// Real-space operator assembled once per force evaluation (column-major ELL, row stride
+// ldn >= N, width >= max n_neigh) and applied per matvec.
+template <typename T>
+hipError_t launch_real_csr_build(const RealParams& rp, const SysView& s, const T* table, long ldn, unsigned* col, T* val,
+                                 unsigned* row_len, hipStream_t st);
+template <typename T>
+hipError_t launch_real_space_csr(const RealParams& rp, const SysView& s, const double* x, const T* phi_recip,
+                                 long ldn, const unsigned* col, const T* val, const unsigned* row_len, double* y,
+                                 int precond, hipStream_t st);
+// *out = max(*out, a[i]) over i < n (out must be initialised)
+hipError_t launch_max_u32(const unsigned* a, int n, unsigned* out, hipStream_t st);
 template <typename T>
 hipError_t launch_real_space_force(const RealParams& rp, const SysView& s, const double* q, const T* gtable,
                                    const double E0[3], double* force, hipStream_t st);
@@ -100,7 +144,18 @@ hipError_t launch_objective(const SysView& s, const double L[3], const double E0
 hipError_t launch_membership(int N, const unsigned* tag, const unsigned* body_of_tag, unsigned* iso, hipStream_t st);
 hipError_t launch_zero_force(int N, double* force, hipStream_t st);
 
+// ---- k_precond.hip ----
+// Near-field block preconditioner: per isolated body, the inverse of its own saddle block.
+hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, unsigned* body_count,
+                              unsigned* body_start, unsigned* cursor, unsigned* body_beads,
+                              unsigned long long* scan_sums, float* ginv, float* uvec, double* inv_s, hipStream_t st);
+hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_start, const unsigned* body_beads,
+                              const float* ginv, const float* uvec, const double* inv_s, const double* y, double* w,
+                              hipStream_t st);
+
 // ---- k_nlist.hip ----
+// out = exclusive scan of in[0:n]; sums needs n/1024 + 2 entries; total lands in sums[nblocks].
+hipError_t exclusive_scan_u32(const unsigned* in, unsigned* out, long n, unsigned long long* sums, hipStream_t st);
 hipError_t build_nlist(int N, const double4* pos, const double L[3], double rlist, unsigned* head,
                        unsigned* nn, unsigned* nlist, long capacity, long* total, void* scratch_pool,
                        hipStream_t st);

@@ -8,9 +8,11 @@
 // with the scale (cppmodule/ConductorRigid.cu:237-257) fused between the forward
 // and inverse x transforms so the spectrum makes one round trip through HBM.
 //
-// Each workgroup loads a batch of pencils into LDS (coalesced: along the pencil for
-// z, across neighbouring z columns for y and x), runs Stockham autosort stages
-// (Govindaraju et al., SC'08) ping-ponging between two LDS buffers, and writes back.
+// Each workgroup loads a batch of pencils into one LDS buffer (coalesced: along the
+// pencil for z; across 16 (FP32) or 8 (FP64) neighbouring z columns -- 128-byte rows --
+// for y and x), copies the axis' twiddle table into LDS, and runs the Stockham
+// autosort stages (Govindaraju et al., SC'08) in place: each stage reads its
+// butterfly inputs into registers, synchronises, and writes the outputs back.
 #include "cr_kernels.h"
 
 namespace cr {
@@ -42,76 +44,103 @@ __device__ __forceinline__ T2 twiddle(const T2* tw, int e, int sign)
 }
 
 constexpr int kThreads = 256;
+constexpr int kMaxElems = 4096;  // cols * n per workgroup: <= 16 elements per thread
 
-// Stockham stages over `cols` pencils of length n held at pitch `pitch` in src.
-// Returns the buffer holding the result.
+// One Stockham stage of radix R over `cols` pencils of length n (pitch `pitch`), in place.
+template <typename T2, int R, int MAXB>
+__device__ __forceinline__ void stage(T2* buf, int n, int pitch, int cols, int Ns, const T2* tw, int sign)
+{
+    const int nb = n / R;
+    const int total = cols * nb;
+    const int step = n / (Ns * R);
+    T2 v[MAXB][R];
+    int cj[MAXB];
+#pragma unroll
+    for (int k = 0; k < MAXB; ++k) {
+        const int b = threadIdx.x + k * kThreads;
+        cj[k] = -1;
+        if (b < total) {
+            const int c = b / nb, j = b - c * nb;
+            cj[k] = c * pitch + j;
+            const int kk = j % Ns;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                T2 a = buf[c * pitch + j + r * nb];
+                if (r > 0) a = cmul(a, twiddle(tw, r * kk * step, sign));
+                v[k][r] = a;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < MAXB; ++k) {
+        if (cj[k] < 0) continue;
+        const int b = threadIdx.x + k * kThreads;
+        const int c = b / nb, j = b - c * nb;
+        const int kk = j % Ns;
+        T2* dp = buf + c * pitch + (j / Ns) * Ns * R + kk;
+        if (R == 2) {
+            T2 o0, o1;
+            o0.x = v[k][0].x + v[k][1].x; o0.y = v[k][0].y + v[k][1].y;
+            o1.x = v[k][0].x - v[k][1].x; o1.y = v[k][0].y - v[k][1].y;
+            dp[0] = o0;
+            dp[Ns] = o1;
+        } else if (R == 4) {
+            T2 a0, a1, a2, a3, b3;
+            a0.x = v[k][0].x + v[k][2].x; a0.y = v[k][0].y + v[k][2].y;
+            a1.x = v[k][0].x - v[k][2].x; a1.y = v[k][0].y - v[k][2].y;
+            a2.x = v[k][1].x + v[k][3].x; a2.y = v[k][1].y + v[k][3].y;
+            a3.x = v[k][1].x - v[k][3].x; a3.y = v[k][1].y - v[k][3].y;
+            // a3 * (sign * i): W4 = -i forward, +i inverse
+            if (sign < 0) { b3.x = a3.y; b3.y = -a3.x; }
+            else { b3.x = -a3.y; b3.y = a3.x; }
+            T2 o;
+            o.x = a0.x + a2.x; o.y = a0.y + a2.y; dp[0] = o;
+            o.x = a1.x + b3.x; o.y = a1.y + b3.y; dp[Ns] = o;
+            o.x = a0.x - a2.x; o.y = a0.y - a2.y; dp[2 * Ns] = o;
+            o.x = a1.x - b3.x; o.y = a1.y - b3.y; dp[3 * Ns] = o;
+        } else {
+            const int rstep = n / R;
+#pragma unroll
+            for (int o = 0; o < R; ++o) {
+                T2 acc = v[k][0];
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const T2 t = cmul(v[k][r], twiddle(tw, ((r * o) % R) * rstep, sign));
+                    acc.x += t.x;
+                    acc.y += t.y;
+                }
+                dp[o * Ns] = acc;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// All stages of the axis plan, in place in LDS.
 template <typename T2>
-__device__ T2* fft_lds(T2* src, T2* dst, int n, int pitch, int cols, const AxisPlan& plan, const T2* tw, int sign)
+__device__ void fft_lds(T2* buf, int n, int pitch, int cols, const AxisPlan& plan, const T2* tw, int sign)
 {
     int Ns = 1;
     for (int s = 0; s < plan.nstages; ++s) {
         const int R = plan.radix[s];
-        const int nb = n / R;
-        const int step = n / (Ns * R);
-        for (int work = threadIdx.x; work < cols * nb; work += blockDim.x) {
-            const int c = work / nb;
-            const int j = work - c * nb;
-            const T2* sp = src + c * pitch;
-            T2* dp = dst + c * pitch;
-            const int k = j % Ns;
-            T2 v[5];
-            for (int r = 0; r < R; ++r) {
-                T2 a = sp[j + r * nb];
-                if (r > 0) a = cmul(a, twiddle(tw, (r * k * step) % n, sign));
-                v[r] = a;
-            }
-            const int idxD = (j / Ns) * Ns * R + k;
-            if (R == 2) {
-                T2 o0, o1;
-                o0.x = v[0].x + v[1].x; o0.y = v[0].y + v[1].y;
-                o1.x = v[0].x - v[1].x; o1.y = v[0].y - v[1].y;
-                dp[idxD] = o0;
-                dp[idxD + Ns] = o1;
-            } else if (R == 4) {
-                // W4 = e^{sign i pi/2}: forward (-i), inverse (+i)
-                T2 a0, a1, a2, a3;
-                a0.x = v[0].x + v[2].x; a0.y = v[0].y + v[2].y;
-                a1.x = v[0].x - v[2].x; a1.y = v[0].y - v[2].y;
-                a2.x = v[1].x + v[3].x; a2.y = v[1].y + v[3].y;
-                a3.x = v[1].x - v[3].x; a3.y = v[1].y - v[3].y;
-                // a3 * (sign * i)
-                T2 b3;
-                if (sign < 0) { b3.x = a3.y; b3.y = -a3.x; }
-                else { b3.x = -a3.y; b3.y = a3.x; }
-                T2 o0, o1, o2, o3;
-                o0.x = a0.x + a2.x; o0.y = a0.y + a2.y;
-                o2.x = a0.x - a2.x; o2.y = a0.y - a2.y;
-                o1.x = a1.x + b3.x; o1.y = a1.y + b3.y;
-                o3.x = a1.x - b3.x; o3.y = a1.y - b3.y;
-                dp[idxD] = o0;
-                dp[idxD + Ns] = o1;
-                dp[idxD + 2 * Ns] = o2;
-                dp[idxD + 3 * Ns] = o3;
-            } else {
-                const int rstep = n / R;
-                for (int o = 0; o < R; ++o) {
-                    T2 acc = v[0];
-                    for (int r = 1; r < R; ++r) {
-                        T2 t = cmul(v[r], twiddle(tw, ((r * o) % R) * rstep, sign));
-                        acc.x += t.x;
-                        acc.y += t.y;
-                    }
-                    dp[idxD + o * Ns] = acc;
-                }
-            }
-        }
-        __syncthreads();
-        T2* t = src;
-        src = dst;
-        dst = t;
+        if (R == 4) stage<T2, 4, kMaxElems / kThreads / 4>(buf, n, pitch, cols, Ns, tw, sign);
+        else if (R == 2) stage<T2, 2, kMaxElems / kThreads / 2>(buf, n, pitch, cols, Ns, tw, sign);
+        else if (R == 3) stage<T2, 3, (kMaxElems / kThreads + 2) / 3>(buf, n, pitch, cols, Ns, tw, sign);
+        else stage<T2, 5, (kMaxElems / kThreads + 4) / 5>(buf, n, pitch, cols, Ns, tw, sign);
         Ns *= R;
     }
-    return src;
+}
+
+// LDS layout: [twiddles n][cols x pitch pencils]
+template <typename T2>
+__device__ __forceinline__ T2* lds_setup(char* smem_raw, const AxisPlan& plan, T2** tw_out)
+{
+    T2* tw = reinterpret_cast<T2*>(smem_raw);
+    const T2* g = reinterpret_cast<const T2*>(plan.tw);
+    for (int e = threadIdx.x; e < plan.n; e += blockDim.x) tw[e] = g[e];
+    *tw_out = tw;
+    return tw + plan.n;
 }
 
 // ---- z: real rows -> half-spectrum rows ----
@@ -122,8 +151,8 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_r2c(AxisPlan plan, long nrow
     using T2 = typename Cplx<T>::type;
     extern __shared__ char smem_raw[];
     const int n = plan.n, nh = n / 2 + 1, pitch = n + 1;
-    T2* A = (T2*)smem_raw;
-    T2* B = A + cols * pitch;
+    T2* tw;
+    T2* A = lds_setup<T2>(smem_raw, plan, &tw);
     const long row0 = (long)blockIdx.x * cols;
     for (int idx = threadIdx.x; idx < cols * n; idx += blockDim.x) {
         const int c = idx / n, t = idx - c * n;
@@ -134,11 +163,11 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_r2c(AxisPlan plan, long nrow
         A[c * pitch + t] = v;
     }
     __syncthreads();
-    T2* res = fft_lds<T2>(A, B, n, pitch, cols, plan, (const T2*)plan.tw, -1);
+    fft_lds<T2>(A, n, pitch, cols, plan, tw, -1);
     for (int idx = threadIdx.x; idx < cols * nh; idx += blockDim.x) {
         const int c = idx / nh, t = idx - c * nh;
         const long row = row0 + c;
-        if (row < nrows) cplx[row * nh + t] = res[c * pitch + t];
+        if (row < nrows) cplx[row * nh + t] = A[c * pitch + t];
     }
 }
 
@@ -151,8 +180,8 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_c2r(AxisPlan plan, long nrow
     using T2 = typename Cplx<T>::type;
     extern __shared__ char smem_raw[];
     const int n = plan.n, nh = n / 2 + 1, pitch = n + 1;
-    T2* A = (T2*)smem_raw;
-    T2* B = A + cols * pitch;
+    T2* tw;
+    T2* A = lds_setup<T2>(smem_raw, plan, &tw);
     const long row0 = (long)blockIdx.x * cols;
     for (int idx = threadIdx.x; idx < cols * n; idx += blockDim.x) {
         const int c = idx / n, t = idx - c * n;
@@ -171,11 +200,11 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_c2r(AxisPlan plan, long nrow
         A[c * pitch + t] = v;
     }
     __syncthreads();
-    T2* res = fft_lds<T2>(A, B, n, pitch, cols, plan, (const T2*)plan.tw, +1);
+    fft_lds<T2>(A, n, pitch, cols, plan, tw, +1);
     for (int idx = threadIdx.x; idx < cols * n; idx += blockDim.x) {
         const int c = idx / n, t = idx - c * n;
         const long row = row0 + c;
-        if (row < nrows) real[row * n + t] = res[c * pitch + t].x;
+        if (row < nrows) real[row * n + t] = A[c * pitch + t].x;
     }
 }
 
@@ -187,8 +216,8 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_c2c(AxisPlan plan, long nrow
     using T2 = typename Cplx<T>::type;
     extern __shared__ char smem_raw[];
     const int n = plan.n, pitch = n + 1;
-    T2* A = (T2*)smem_raw;
-    T2* B = A + cols * pitch;
+    T2* tw;
+    T2* A = lds_setup<T2>(smem_raw, plan, &tw);
     const long row0 = (long)blockIdx.x * cols;
     for (int idx = threadIdx.x; idx < cols * n; idx += blockDim.x) {
         const int c = idx / n, t = idx - c * n;
@@ -200,11 +229,11 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_c2c(AxisPlan plan, long nrow
         A[c * pitch + t] = v;
     }
     __syncthreads();
-    T2* res = fft_lds<T2>(A, B, n, pitch, cols, plan, (const T2*)plan.tw, sign);
+    fft_lds<T2>(A, n, pitch, cols, plan, tw, sign);
     for (int idx = threadIdx.x; idx < cols * n; idx += blockDim.x) {
         const int c = idx / n, t = idx - c * n;
         const long row = row0 + c;
-        if (row < nrows) data[row * n + t] = res[c * pitch + t];
+        if (row < nrows) data[row * n + t] = A[c * pitch + t];
     }
 }
 
@@ -214,22 +243,22 @@ struct StridedLayout {
     long base;    // offset of column z0
     long stride;  // between consecutive pencil elements
     int z0;
+    int line;
 };
 
-__device__ __forceinline__ StridedLayout strided_layout(int axis, int Nx, int Ny, int Zc, int cols)
+__device__ __forceinline__ StridedLayout strided_layout(int axis, int Ny, int Zc, int cols)
 {
     StridedLayout s;
     const int zblocks = (Zc + cols - 1) / cols;
-    const int line = blockIdx.x / zblocks;
-    s.z0 = (blockIdx.x - line * zblocks) * cols;
+    s.line = blockIdx.x / zblocks;
+    s.z0 = (blockIdx.x - s.line * zblocks) * cols;
     if (axis == 1) {  // pencil over y at fixed x = line
-        s.base = (long)line * Ny * Zc + s.z0;
+        s.base = (long)s.line * Ny * Zc + s.z0;
         s.stride = Zc;
     } else {          // pencil over x at fixed y = line
-        s.base = (long)line * Zc + s.z0;
+        s.base = (long)s.line * Zc + s.z0;
         s.stride = (long)Ny * Zc;
     }
-    (void)Nx;
     return s;
 }
 
@@ -258,19 +287,19 @@ __device__ __forceinline__ void store_strided(T2* __restrict__ data, const T2* A
 }
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_fft_strided(AxisPlan plan, int axis, int Nx, int Ny, int Zc, int cols,
+__global__ __launch_bounds__(kThreads) void k_fft_strided(AxisPlan plan, int axis, int Ny, int Zc, int cols,
                                                           typename Cplx<T>::type* __restrict__ data, int sign)
 {
     using T2 = typename Cplx<T>::type;
     extern __shared__ char smem_raw[];
     const int n = plan.n, pitch = n + 1;
-    T2* A = (T2*)smem_raw;
-    T2* B = A + cols * pitch;
-    const StridedLayout L = strided_layout(axis, Nx, Ny, Zc, cols);
+    T2* tw;
+    T2* A = lds_setup<T2>(smem_raw, plan, &tw);
+    const StridedLayout L = strided_layout(axis, Ny, Zc, cols);
     load_strided(data, A, n, pitch, cols, Zc, L);
     __syncthreads();
-    T2* res = fft_lds<T2>(A, B, n, pitch, cols, plan, (const T2*)plan.tw, sign);
-    store_strided(data, res, n, pitch, cols, Zc, L);
+    fft_lds<T2>(A, n, pitch, cols, plan, tw, sign);
+    store_strided(data, A, n, pitch, cols, Zc, L);
 }
 
 template <typename T>
@@ -283,21 +312,19 @@ __device__ __forceinline__ double sin_t<double>(double x) { return sin(x); }
 // Forward x FFT, multiply by S(k) = j0(|k|)^2 exp(-sum_d (1-eta_d) k_d^2/4xi^2)/|k|^2/G
 // (cppmodule/ConductorRigid.cc:253-268; S(0) = 0), inverse x FFT.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_fft_x_scale(AxisPlan plan, int Nx, int Ny, int Zc, int cols,
+__global__ __launch_bounds__(kThreads) void k_fft_x_scale(AxisPlan plan, int Ny, int Zc, int cols,
                                                           typename Cplx<T>::type* __restrict__ data, ScaleTables sc)
 {
     using T2 = typename Cplx<T>::type;
     extern __shared__ char smem_raw[];
     const int n = plan.n, pitch = n + 1;
-    T2* A = (T2*)smem_raw;
-    T2* B = A + cols * pitch;
-    const StridedLayout L = strided_layout(0, Nx, Ny, Zc, cols);
-    const int zblocks = (Zc + cols - 1) / cols;
-    const int y = blockIdx.x / zblocks;
+    T2* tw;
+    T2* A = lds_setup<T2>(smem_raw, plan, &tw);
+    const StridedLayout L = strided_layout(0, Ny, Zc, cols);
+    const int y = L.line;
     load_strided(data, A, n, pitch, cols, Zc, L);
     __syncthreads();
-    T2* res = fft_lds<T2>(A, B, n, pitch, cols, plan, (const T2*)plan.tw, -1);
-    T2* other = (res == A) ? B : A;
+    fft_lds<T2>(A, n, pitch, cols, plan, tw, -1);
     const double ky = sc.k[1][y], ey = sc.e[1][y];
     for (int idx = threadIdx.x; idx < cols * n; idx += blockDim.x) {
         const int c = idx / n, x = idx - c * n;
@@ -311,31 +338,36 @@ __global__ __launch_bounds__(kThreads) void k_fft_x_scale(AxisPlan plan, int Nx,
             const T j0 = sin_t<T>(kk) / kk;
             S = j0 * j0 * (T)(sc.e[0][x] * ey * sc.e[2][z]) / k2 * (T)sc.invG;
         }
-        T2 v = res[c * pitch + x];
+        T2 v = A[c * pitch + x];
         v.x *= S;
         v.y *= S;
-        res[c * pitch + x] = v;
+        A[c * pitch + x] = v;
     }
     __syncthreads();
-    T2* out = fft_lds<T2>(res, other, n, pitch, cols, plan, (const T2*)plan.tw, +1);
-    store_strided(data, out, n, pitch, cols, Zc, L);
+    fft_lds<T2>(A, n, pitch, cols, plan, tw, +1);
+    store_strided(data, A, n, pitch, cols, Zc, L);
 }
 
-// Pencils per workgroup: keep the two LDS buffers within 64 KiB.
+// Pencils per workgroup: 128-byte rows for the strided passes (16 float2 / 8 double2
+// columns), capped so that cols * n <= 4096 elements (16 per thread in registers).
 template <typename T>
-int pick_cols(int n, int max_cols)
+size_t lds_bytes(int n, int cols)
 {
-    const long per = 2L * (n + 1) * 2 * (long)sizeof(T);
-    int cols = (int)((64L * 1024) / per);
-    if (cols > max_cols) cols = max_cols;
-    if (cols < 1) cols = 1;
+    return ((size_t)(n + 1) * (size_t)cols + (size_t)n) * 2 * sizeof(T);
+}
+
+template <typename T>
+int pick_cols(int n)
+{
+    int cols = (sizeof(T) == 4) ? 16 : 8;
+    while (cols > 1 && (cols * n > kMaxElems || lds_bytes<T>(n, cols) > 64 * 1024)) cols >>= 1;
     return cols;
 }
 
 template <typename T>
-size_t lds_bytes(int n, int cols)
+bool fits(int n)
 {
-    return 2UL * (size_t)(n + 1) * (size_t)cols * 2 * sizeof(T);
+    return n <= kMaxElems && lds_bytes<T>(n, pick_cols<T>(n)) <= 64 * 1024;
 }
 
 }  // namespace
@@ -344,12 +376,11 @@ template <typename T>
 hipError_t launch_fft_z_r2c(const AxisPlan& pz, int Nx, int Ny, const T* real, T* cplx, hipStream_t st)
 {
     using T2 = typename Cplx<T>::type;
+    if (!fits<T>(pz.n)) return hipErrorInvalidValue;
     const long nrows = (long)Nx * Ny;
-    const int cols = pick_cols<T>(pz.n, 16);
-    const size_t lds = lds_bytes<T>(pz.n, cols);
-    if (lds > 64 * 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_fft_z_r2c<T>, dim3((unsigned)((nrows + cols - 1) / cols)), dim3(kThreads), lds, st, pz,
-                       nrows, cols, real, (T2*)cplx);
+    const int cols = pick_cols<T>(pz.n);
+    hipLaunchKernelGGL(k_fft_z_r2c<T>, dim3((unsigned)((nrows + cols - 1) / cols)), dim3(kThreads),
+                       lds_bytes<T>(pz.n, cols), st, pz, nrows, cols, real, (T2*)cplx);
     return hipGetLastError();
 }
 
@@ -357,12 +388,11 @@ template <typename T>
 hipError_t launch_fft_z_c2r(const AxisPlan& pz, int Nx, int Ny, const T* cplx, T* real, hipStream_t st)
 {
     using T2 = typename Cplx<T>::type;
+    if (!fits<T>(pz.n)) return hipErrorInvalidValue;
     const long nrows = (long)Nx * Ny;
-    const int cols = pick_cols<T>(pz.n, 16);
-    const size_t lds = lds_bytes<T>(pz.n, cols);
-    if (lds > 64 * 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_fft_z_c2r<T>, dim3((unsigned)((nrows + cols - 1) / cols)), dim3(kThreads), lds, st, pz,
-                       nrows, cols, (const T2*)cplx, real);
+    const int cols = pick_cols<T>(pz.n);
+    hipLaunchKernelGGL(k_fft_z_c2r<T>, dim3((unsigned)((nrows + cols - 1) / cols)), dim3(kThreads),
+                       lds_bytes<T>(pz.n, cols), st, pz, nrows, cols, (const T2*)cplx, real);
     return hipGetLastError();
 }
 
@@ -370,12 +400,11 @@ template <typename T>
 hipError_t launch_fft_z_c2c(const AxisPlan& pz, int Nx, int Ny, T* cplx, int sign, hipStream_t st)
 {
     using T2 = typename Cplx<T>::type;
+    if (!fits<T>(pz.n)) return hipErrorInvalidValue;
     const long nrows = (long)Nx * Ny;
-    const int cols = pick_cols<T>(pz.n, 16);
-    const size_t lds = lds_bytes<T>(pz.n, cols);
-    if (lds > 64 * 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_fft_z_c2c<T>, dim3((unsigned)((nrows + cols - 1) / cols)), dim3(kThreads), lds, st, pz,
-                       nrows, cols, (T2*)cplx, sign);
+    const int cols = pick_cols<T>(pz.n);
+    hipLaunchKernelGGL(k_fft_z_c2c<T>, dim3((unsigned)((nrows + cols - 1) / cols)), dim3(kThreads),
+                       lds_bytes<T>(pz.n, cols), st, pz, nrows, cols, (T2*)cplx, sign);
     return hipGetLastError();
 }
 
@@ -383,13 +412,12 @@ template <typename T>
 hipError_t launch_fft_yx(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, T* cplx, int sign, hipStream_t st)
 {
     using T2 = typename Cplx<T>::type;
-    const int cols = pick_cols<T>(pa.n, 16);
-    const size_t lds = lds_bytes<T>(pa.n, cols);
-    if (lds > 64 * 1024) return hipErrorInvalidValue;
+    if (!fits<T>(pa.n)) return hipErrorInvalidValue;
+    const int cols = pick_cols<T>(pa.n);
     const int lines = (axis == 1) ? Nx : Ny;
     const int zblocks = (Zc + cols - 1) / cols;
-    hipLaunchKernelGGL(k_fft_strided<T>, dim3((unsigned)(lines * zblocks)), dim3(kThreads), lds, st, pa, axis, Nx, Ny,
-                       Zc, cols, (T2*)cplx, sign);
+    hipLaunchKernelGGL(k_fft_strided<T>, dim3((unsigned)(lines * zblocks)), dim3(kThreads), lds_bytes<T>(pa.n, cols),
+                       st, pa, axis, Ny, Zc, cols, (T2*)cplx, sign);
     return hipGetLastError();
 }
 
@@ -398,12 +426,12 @@ hipError_t launch_fft_x_scale(const AxisPlan& px, int Nx, int Ny, int Zc, T* cpl
                               hipStream_t st)
 {
     using T2 = typename Cplx<T>::type;
-    const int cols = pick_cols<T>(px.n, 16);
-    const size_t lds = lds_bytes<T>(px.n, cols);
-    if (lds > 64 * 1024) return hipErrorInvalidValue;
+    (void)Nx;
+    if (!fits<T>(px.n)) return hipErrorInvalidValue;
+    const int cols = pick_cols<T>(px.n);
     const int zblocks = (Zc + cols - 1) / cols;
-    hipLaunchKernelGGL(k_fft_x_scale<T>, dim3((unsigned)(Ny * zblocks)), dim3(kThreads), lds, st, px, Nx, Ny, Zc, cols,
-                       (T2*)cplx, sc);
+    hipLaunchKernelGGL(k_fft_x_scale<T>, dim3((unsigned)(Ny * zblocks)), dim3(kThreads), lds_bytes<T>(px.n, cols), st,
+                       px, Ny, Zc, cols, (T2*)cplx, sc);
     return hipGetLastError();
 }
 

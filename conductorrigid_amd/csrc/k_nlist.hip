@@ -168,8 +168,10 @@ __global__ void k_scan_add(unsigned* __restrict__ out, long n, const unsigned lo
 
 inline unsigned blocks(long n, int b) { return (unsigned)((n + b - 1) / b > 0 ? (n + b - 1) / b : 1); }
 
-// out = exclusive scan of in; *total (device) = sum.  sums: >= n/kScanBlock + 2 entries.
-hipError_t exclusive_scan(const unsigned* in, unsigned* out, long n, unsigned long long* sums, hipStream_t st)
+}  // namespace
+
+// out = exclusive scan of in; total (device) lands in sums[nblocks].  sums: >= n/kScanBlock + 2 entries.
+hipError_t exclusive_scan_u32(const unsigned* in, unsigned* out, long n, unsigned long long* sums, hipStream_t st)
 {
     const long nb = (n + kScanBlock - 1) / kScanBlock;
     if (nb > 0) hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(kScanBlock), 0, st, in, out, n, sums);
@@ -177,8 +179,6 @@ hipError_t exclusive_scan(const unsigned* in, unsigned* out, long n, unsigned lo
     if (nb > 0) hipLaunchKernelGGL(k_scan_add, dim3(blocks(n, 256)), dim3(256), 0, st, out, n, sums);
     return hipGetLastError();
 }
-
-}  // namespace
 
 hipError_t build_nlist(int N, const double4* pos, const double L[3], double rlist, unsigned* head, unsigned* nn,
                        unsigned* nlist, long capacity, long* total, void* /*scratch_pool*/, hipStream_t st)
@@ -220,12 +220,12 @@ hipError_t build_nlist(int N, const double4* pos, const double L[3], double rlis
     NL_TRY(hipMemsetAsync(count, 0, sizeof(unsigned) * ncell, st));
     NL_TRY(hipMemsetAsync(cursor, 0, sizeof(unsigned) * ncell, st));
     hipLaunchKernelGGL(k_cell_of, dim3(blocks(N, 256)), dim3(256), 0, st, N, pos, cg, cell, count);
-    NL_TRY(exclusive_scan(count, start, ncell, sums, st));
+    NL_TRY(exclusive_scan_u32(count, start, ncell, sums, st));
     hipLaunchKernelGGL(k_cell_fill, dim3(blocks(N, 256)), dim3(256), 0, st, N, cell, start, cursor, members);
     hipLaunchKernelGGL(k_cell_sort, dim3(blocks(ncell, 256)), dim3(256), 0, st, ncell, start, count, members);
     hipLaunchKernelGGL((k_pairs<false>), dim3(blocks(N, 128)), dim3(128), 0, st, N, pos, cg, cell, start, count,
                        members, nn, (const unsigned*)nullptr, (unsigned*)nullptr, 0L);
-    NL_TRY(exclusive_scan(nn, head, N, sums, st));
+    NL_TRY(exclusive_scan_u32(nn, head, N, sums, st));
     hipLaunchKernelGGL((k_pairs<true>), dim3(blocks(N, 128)), dim3(128), 0, st, N, pos, cg, cell, start, count,
                        members, (unsigned*)nullptr, head, nlist, capacity);
     {

@@ -1,0 +1,215 @@
+// k_precond.hip -- near-field block preconditioner for the conductor saddle-point system.
+//
+// The reference's "PreconditionWrapper" (cppmodule/PreconditionWrapper.cuh:23-75 ->
+// precondition, cppmodule/ConductorRigid.cu:816-859) is a diagonal scaling.  At large N
+// the bead-bead coupling inside each body (beads touch) dominates the conditioning, and
+// diagonally preconditioned GMRES(30) stagnates.  This preconditioner inverts, per body,
+// the body's own block of the saddle-point operator
+//
+//     A_b = [ -G_bb   1 ]       with G_ij the free-space interaction of two unit
+//           [  1^T    0 ]       spherical shells: 1/4pi (i = j), (4 - r)/16pi (r < 2),
+//                                1/4pi r (r >= 2)
+//
+// in closed form: u = G^{-1} 1, s = 1^T u,
+//     A_b^{-1} = [ -G^{-1} + u u^T / s   u / s ]
+//                [  u^T / s              1 / s ].
+// G is symmetric positive definite (a Gram matrix of the Coulomb operator), so it is
+// inverted by Gauss-Jordan without pivoting, in FP64, one workgroup per body; the
+// inverse is stored in FP32 (it only steers convergence, never the converged answer).
+#include "cr_kernels.h"
+
+namespace cr {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+__device__ __forceinline__ double min_image(double d, double L) { return d - L * rint(d / L); }
+
+__device__ __forceinline__ double shell_shell(double r)
+{
+    if (r == 0.0) return 1.0 / (4.0 * kPi);
+    if (r < 2.0) return (4.0 - r) / (16.0 * kPi);
+    return 1.0 / (4.0 * kPi * r);
+}
+
+__global__ void k_body_count(int N, const unsigned* __restrict__ iso, unsigned* __restrict__ count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) atomicAdd(&count[iso[i]], 1u);
+}
+
+__global__ void k_body_fill(int N, const unsigned* __restrict__ iso, const unsigned* __restrict__ start,
+                            unsigned* __restrict__ cursor, unsigned* __restrict__ beads)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const unsigned t = iso[i];
+    beads[start[t] + atomicAdd(&cursor[t], 1u)] = (unsigned)i;
+}
+
+// members of each body in increasing bead index (deterministic block layout)
+__global__ void k_body_sort(int B, const unsigned* __restrict__ start, unsigned* __restrict__ beads)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B) return;
+    unsigned* m = beads + start[t];
+    const unsigned n = start[t + 1] - start[t];
+    for (unsigned a = 1; a < n; ++a) {
+        const unsigned v = m[a];
+        unsigned b = a;
+        while (b > 0 && m[b - 1] > v) {
+            m[b] = m[b - 1];
+            --b;
+        }
+        m[b] = v;
+    }
+}
+
+// One 64-thread workgroup per body; thread i owns row i of [G | I] in LDS.
+__global__ __launch_bounds__(64) void k_block_setup(int nmax, const unsigned* __restrict__ start,
+                                                    const unsigned* __restrict__ beads,
+                                                    const double4* __restrict__ pos, double Lx, double Ly,
+                                                    double Lz, float* __restrict__ ginv, float* __restrict__ uvec,
+                                                    double* __restrict__ inv_s)
+{
+    extern __shared__ double aug[];
+    __shared__ double red[64];
+    const int t = blockIdx.x;
+    const int b0 = start[t];
+    const int n = start[t + 1] - b0;
+    const int i = threadIdx.x;
+    const int pitch = 2 * nmax;
+    if (n > nmax || n > 64) {  // body larger than the workspace: poison the solve
+        if (i == 0) inv_s[t] = __builtin_nan("");
+        return;
+    }
+    if (i < n) {
+        const double4 pi = pos[beads[b0 + i]];
+        for (int j = 0; j < n; ++j) {
+            double g;
+            if (j == i) {
+                g = shell_shell(0.0);
+            } else {
+                const double4 pj = pos[beads[b0 + j]];
+                const double dx = min_image(pi.x - pj.x, Lx), dy = min_image(pi.y - pj.y, Ly),
+                             dz = min_image(pi.z - pj.z, Lz);
+                g = shell_shell(sqrt(dx * dx + dy * dy + dz * dz));
+            }
+            aug[i * pitch + j] = g;
+            aug[i * pitch + n + j] = (i == j) ? 1.0 : 0.0;
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        if (i == k) {
+            const double inv = 1.0 / aug[k * pitch + k];
+            for (int j = 0; j < 2 * n; ++j) aug[k * pitch + j] *= inv;
+        }
+        __syncthreads();
+        if (i < n && i != k) {
+            const double f = aug[i * pitch + k];
+            for (int j = 0; j < 2 * n; ++j) aug[i * pitch + j] -= f * aug[k * pitch + j];
+        }
+        __syncthreads();
+    }
+    double u = 0.0;
+    if (i < n) {
+        float* row = ginv + ((long)t * nmax + i) * nmax;
+        for (int j = 0; j < n; ++j) {
+            const double v = aug[i * pitch + n + j];
+            row[j] = (float)v;
+            u += v;
+        }
+        uvec[(long)t * nmax + i] = (float)u;
+    }
+    red[i] = u;
+    __syncthreads();
+    if (i == 0) {
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) s += red[j];
+        inv_s[t] = 1.0 / s;
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// w = A_b^{-1} y per body: bead rows and the body's constraint row.
+__global__ __launch_bounds__(64) void k_block_apply(int N, int nmax, const unsigned* __restrict__ start,
+                                                    const unsigned* __restrict__ beads,
+                                                    const float* __restrict__ ginv, const float* __restrict__ uvec,
+                                                    const double* __restrict__ inv_s, const double* __restrict__ y,
+                                                    double* __restrict__ w)
+{
+    __shared__ double ys[64];
+    const int t = blockIdx.x;
+    const int b0 = start[t];
+    const int n = start[t + 1] - b0;
+    const int i = threadIdx.x;
+    if (n > nmax || n > 64) {
+        if (i == 0) w[N + t] = __builtin_nan("");
+        return;
+    }
+    const unsigned bead = (i < n) ? beads[b0 + i] : 0u;
+    const double yi = (i < n) ? y[bead] : 0.0;
+    ys[i] = yi;
+    __syncthreads();
+    double gy = 0.0, ui = 0.0;
+    if (i < n) {
+        const float* row = ginv + ((long)t * nmax + i) * nmax;
+        for (int j = 0; j < n; ++j) gy += (double)row[j] * ys[j];
+        ui = (double)uvec[(long)t * nmax + i];
+    }
+    const double dot = wave_sum(ui * yi);
+    const double c = (dot + y[N + t]) * inv_s[t];
+    if (i < n) w[bead] = -gy + ui * c;
+    if (i == 0) w[N + t] = c;
+}
+
+inline unsigned blocks(long n, int b) { return (unsigned)((n + b - 1) / b > 0 ? (n + b - 1) / b : 1); }
+
+__global__ void k_zero_u32(unsigned* p, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+}  // namespace
+
+hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, unsigned* body_count,
+                              unsigned* body_start, unsigned* cursor, unsigned* body_beads,
+                              unsigned long long* scan_sums, float* ginv, float* uvec, double* inv_s, hipStream_t st)
+{
+    const int B = s.N_isolated;
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_u32, dim3(blocks(B + 1, 256)), dim3(256), 0, st, body_count, (long)B + 1);
+    hipLaunchKernelGGL(k_zero_u32, dim3(blocks(B, 256)), dim3(256), 0, st, cursor, (long)B);
+    if (s.N > 0) hipLaunchKernelGGL(k_body_count, dim3(blocks(s.N, 256)), dim3(256), 0, st, s.N, s.iso, body_count);
+    hipError_t e = exclusive_scan_u32(body_count, body_start, (long)B + 1, scan_sums, st);
+    if (e != hipSuccess) return e;
+    if (s.N > 0)
+        hipLaunchKernelGGL(k_body_fill, dim3(blocks(s.N, 256)), dim3(256), 0, st, s.N, s.iso, body_start, cursor,
+                           body_beads);
+    hipLaunchKernelGGL(k_body_sort, dim3(blocks(B, 256)), dim3(256), 0, st, B, body_start, body_beads);
+    const size_t lds = sizeof(double) * (size_t)nmax * 2 * nmax;
+    hipLaunchKernelGGL(k_block_setup, dim3(B), dim3(64), lds, st, nmax, body_start, body_beads, s.pos, L[0], L[1],
+                       L[2], ginv, uvec, inv_s);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_start, const unsigned* body_beads,
+                              const float* ginv, const float* uvec, const double* inv_s, const double* y, double* w,
+                              hipStream_t st)
+{
+    if (s.N_isolated <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_block_apply, dim3(s.N_isolated), dim3(64), 0, st, s.N, nmax, body_start, body_beads, ginv,
+                       uvec, inv_s, y, w);
+    return hipGetLastError();
+}
+
+}  // namespace cr

@@ -52,6 +52,66 @@ __global__ __launch_bounds__(256) void k_real_space(RealParams rp, SysView s, co
     y[i] = precond ? -4.0 * kPi * out : out;
 }
 
+// The real-space part of the operator is constant while GMRES runs (positions are
+// fixed), so it is assembled once per force evaluation: the entries of bead i are the
+// pairs inside [dr, rc) with the same table interpolation as real_space.  Storage is
+// column-major ELL -- entry k of row i at k*ldn + i, row_len[i] entries per row -- so
+// that a wavefront's 64 consecutive rows read 64 consecutive words per step.
+__global__ void k_max_u32(const unsigned* __restrict__ a, int n, unsigned* __restrict__ out)
+{
+    unsigned v = 0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        v = max(v, a[i]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, v);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_real_csr_build(RealParams rp, SysView s, const T* __restrict__ table,
+                                                        long ldn, unsigned* __restrict__ col, T* __restrict__ val,
+                                                        unsigned* __restrict__ row_len)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    const double4 ri = s.pos[i];
+    const unsigned head = s.head_list[i], nn = s.n_neigh[i];
+    unsigned cnt = 0;
+    for (unsigned jj = 0; jj < nn; ++jj) {
+        const unsigned j = s.nlist[head + jj];
+        const double4 rj = s.pos[j];
+        const double dx = min_image(ri.x - rj.x, rp.L[0]);
+        const double dy = min_image(ri.y - rj.y, rp.L[1]);
+        const double dz = min_image(ri.z - rj.z, rp.L[2]);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < rp.rc2 && d2 >= rp.rmin2) {
+            col[cnt * ldn + i] = j;
+            val[cnt * ldn + i] = table_lookup<T>(rp, table, sqrt(d2));
+            ++cnt;
+        }
+    }
+    row_len[i] = cnt;
+}
+
+// real_space on the assembled rows: y_i = Phi_{b(i)} - (phi_i + self q_i + sum_j A_ij q_j)
+template <typename T>
+__global__ __launch_bounds__(256) void k_real_space_csr(RealParams rp, SysView s, const double* __restrict__ x,
+                                                        const T* __restrict__ phi_recip, long ldn,
+                                                        const unsigned* __restrict__ col, const T* __restrict__ val,
+                                                        const unsigned* __restrict__ row_len, double* __restrict__ y,
+                                                        int precond)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    const double* q = x;
+    const double* Phi = x + s.N;
+    T acc = phi_recip[i] + (T)rp.selfterm * (T)q[i];
+    const unsigned len = row_len[i];
+    for (unsigned k = 0; k < len; ++k) acc += val[k * ldn + i] * (T)q[col[k * ldn + i]];
+    const double out = Phi[s.iso[i]] - (double)acc;
+    y[i] = precond ? -4.0 * kPi * out : out;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_real_space_force(RealParams rp, SysView s, const double* __restrict__ q,
                                                           const T* __restrict__ gtable, double E0x, double E0y,
@@ -177,6 +237,27 @@ hipError_t launch_real_space(const RealParams& rp, const SysView& s, const doubl
 }
 
 template <typename T>
+hipError_t launch_real_csr_build(const RealParams& rp, const SysView& s, const T* table, long ldn, unsigned* col, T* val,
+                                 unsigned* row_len, hipStream_t st)
+{
+    if (s.N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_real_csr_build<T>, dim3(blocks(s.N, 256)), dim3(256), 0, st, rp, s, table, ldn, col, val,
+                       row_len);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_real_space_csr(const RealParams& rp, const SysView& s, const double* x, const T* phi_recip,
+                                 long ldn, const unsigned* col, const T* val, const unsigned* row_len, double* y,
+                                 int precond, hipStream_t st)
+{
+    if (s.N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_real_space_csr<T>, dim3(blocks(s.N, 256)), dim3(256), 0, st, rp, s, x, phi_recip, ldn, col,
+                       val, row_len, y, precond);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_real_space_force(const RealParams& rp, const SysView& s, const double* q, const T* gtable,
                                    const double E0[3], double* force, hipStream_t st)
 {
@@ -232,6 +313,22 @@ template hipError_t launch_real_space<float>(const RealParams&, const SysView&, 
                                              const float*, double*, int, hipStream_t);
 template hipError_t launch_real_space<double>(const RealParams&, const SysView&, const double*, const double*,
                                               const double*, double*, int, hipStream_t);
+template hipError_t launch_real_csr_build<float>(const RealParams&, const SysView&, const float*, long, unsigned*,
+                                                 float*, unsigned*, hipStream_t);
+template hipError_t launch_real_csr_build<double>(const RealParams&, const SysView&, const double*, long, unsigned*,
+                                                  double*, unsigned*, hipStream_t);
+template hipError_t launch_real_space_csr<float>(const RealParams&, const SysView&, const double*, const float*, long,
+                                                 const unsigned*, const float*, const unsigned*, double*, int,
+                                                 hipStream_t);
+template hipError_t launch_real_space_csr<double>(const RealParams&, const SysView&, const double*, const double*, long,
+                                                  const unsigned*, const double*, const unsigned*, double*, int,
+                                                  hipStream_t);
+hipError_t launch_max_u32(const unsigned* a, int n, unsigned* out, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_max_u32, dim3(std::min(blocks(n, 256), 1024u)), dim3(256), 0, st, a, n, out);
+    return hipGetLastError();
+}
 template hipError_t launch_real_space_force<float>(const RealParams&, const SysView&, const double*, const float*,
                                                    const double*, double*, hipStream_t);
 template hipError_t launch_real_space_force<double>(const RealParams&, const SysView&, const double*, const double*,

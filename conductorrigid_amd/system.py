@@ -188,17 +188,28 @@ def make_system(n_bodies: int, body_sizes, phi: float, seed: int, charges: str =
     tol = 1e-9
     stall = 0
     restarts = 0
+    # placed beads live in a large tree rebuilt only after many insertions, plus a
+    # small tree of the beads placed since (rebuilt every round)
+    big_tree = None
+    recent: list[np.ndarray] = []
+    n_recent = 0
     while np.any(count < sizes):
         active = np.nonzero(count < sizes)[0]
-        idx_placed = np.nonzero(placed)[0]
-        tree = cKDTree(_wrap(unwrapped[idx_placed], L) + 0.5 * L, boxsize=Lval)
+        if big_tree is None or n_recent > 20000:
+            idx_placed = np.nonzero(placed)[0]
+            big_tree = cKDTree(_wrap(unwrapped[idx_placed], L) + 0.5 * L, boxsize=Lval)
+            recent, n_recent = [], 0
+        small_tree = cKDTree(_wrap(np.concatenate(recent), L) + 0.5 * L, boxsize=Lval) if recent else None
         A, K = active.size, n_candidates
         base_k = (rng.random((A, K)) * count[active, None]).astype(np.int64)
         base = unwrapped[offsets[active, None] + base_k]
         d = rng.normal(size=(A, K, 3))
         d /= np.linalg.norm(d, axis=2, keepdims=True)
         cand = base + 2.0 * d
-        dist, _ = tree.query((_wrap(cand.reshape(-1, 3), L) + 0.5 * L), k=1)
+        cq = _wrap(cand.reshape(-1, 3), L) + 0.5 * L
+        dist, _ = big_tree.query(cq, k=1)
+        if small_tree is not None:
+            dist = np.minimum(dist, small_tree.query(cq, k=1)[0])
         valid = (dist.reshape(A, K) >= 2.0 - tol)
         com = np.stack([unwrapped[offsets[b]:offsets[b] + count[b]].mean(axis=0) for b in active]) \
             if A < 2000 else _body_means(unwrapped, offsets, active, count)
@@ -219,24 +230,36 @@ def make_system(n_bodies: int, body_sizes, phi: float, seed: int, charges: str =
             unwrapped[slot] = chosen
             placed[slot] = True
             count[chosen_b] += 1
+            recent.append(chosen.copy())
+            n_recent += chosen.shape[0]
         stall = 0 if chosen_b.size else stall + 1
         if stall > 8:
             # every remaining body is caged in: re-seed each at a random free spot
             restarts += 1
             if restarts > 200:
                 raise RuntimeError("cluster growth stalled; lower the volume fraction")
-            for b in np.nonzero(count < sizes)[0]:
+            caged = np.nonzero(count < sizes)[0]
+            for b in caged:
                 placed[offsets[b]:offsets[b + 1]] = False
-                idx_placed = np.nonzero(placed)[0]
-                tree = cKDTree(_wrap(unwrapped[idx_placed], L) + 0.5 * L, boxsize=Lval)
-                trial = rng.uniform(-0.5, 0.5, size=(4096, 3)) * Lval
-                room = tree.query(trial + 0.5 * L, k=1)[0]
-                if room.max() < 2.0:
-                    raise RuntimeError("no free space to re-seed a body; lower the volume fraction")
-                p0 = trial[np.argmax(room)]
+            idx_placed = np.nonzero(placed)[0]
+            tree = cKDTree(_wrap(unwrapped[idx_placed], L) + 0.5 * L, boxsize=Lval)
+            trial = rng.uniform(-0.5, 0.5, size=(max(4096, 64 * caged.size), 3)) * Lval
+            room = tree.query(trial + 0.5 * L, k=1)[0]
+            order = np.argsort(-room)
+            chosen = []
+            for t in order:
+                if room[t] < 2.0 or len(chosen) == caged.size:
+                    break
+                p0 = trial[t]
+                if all(np.linalg.norm(_wrap(p0 - c, L)) >= 2.0 for c in chosen):
+                    chosen.append(p0)
+            if len(chosen) < caged.size:
+                raise RuntimeError("no free space to re-seed caged bodies; lower the volume fraction")
+            for b, p0 in zip(caged, chosen):
                 unwrapped[offsets[b]] = p0
                 placed[offsets[b]] = True
                 count[b] = 1
+            big_tree = None
             stall = 0
 
     com_unwrapped = np.array([unwrapped[offsets[b]:offsets[b + 1]].mean(axis=0) for b in range(n_bodies)]) \

@@ -59,7 +59,15 @@ typedef struct {
     int restart;          /* GMRES restart (0: 10) */
     int max_iter;         /* GMRES iteration limit (0: 100) */
     double gmres_tol;     /* relative GMRES tolerance (0: errortol) */
+    int preconditioner;   /* CR_PRECOND_* */
 } cr_params;
+
+/* Preconditioners.  DIAGONAL is the reference's (cppmodule/ConductorRigid.cu:816-859) and
+ * keeps CUSP's stopping rule |s_{i+1}| <= tol ||b||_2.  NEARFIELD_BLOCK inverts each body's
+ * own saddle block (free-space shell-shell interactions of its beads plus its constraint
+ * row) and stops at ||D r|| <= tol ||D b||_2; it has no counterpart in the reference. */
+#define CR_PRECOND_DIAGONAL 0
+#define CR_PRECOND_NEARFIELD_BLOCK 1
 
 /* Parameters as resolved by cr_set_params (SetParams' derived quantities). */
 typedef struct {
@@ -101,7 +109,8 @@ typedef struct {
     int converged;         /* 1 when the residual estimate met tol * ||b|| */
     double resid;          /* final preconditioned residual estimate */
     double tolerance;      /* tol * ||b||_2 */
-    double true_resid;     /* ||M (b - A x)||_2 recomputed in FP64 after the solve */
+    double true_resid;     /* ||D (b - A x)||_2 recomputed in FP64 after the solve */
+    double true_relres;    /* ||b - A x||_2 / ||b||_2 (unpreconditioned), FP64 */
 } cr_solve_stats;
 
 /* ---- lifetime ---------------------------------------------------------- */

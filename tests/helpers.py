@@ -32,12 +32,12 @@ class GpuCase:
     """A HIP context + device system for one (system, parameter set)."""
 
     def __init__(self, sysdef, xi, errortol, rc=0.0, mesh=None, P=0, precision="fp64", restart=0,
-                 max_iter=0, gmres_tol=0.0, skin=0.4):
+                 max_iter=0, gmres_tol=0.0, skin=0.4, preconditioner="diagonal"):
         from conductorrigid_amd.compute import ConductorRigid
 
         self.cr = ConductorRigid(system=sysdef, xi=xi, errortol=errortol, rc=rc, mesh=mesh, P=P,
                                  precision=precision, restart=restart, max_iter=max_iter, gmres_tol=gmres_tol,
-                                 skin=skin, field=(0.0, 0.0, 1.0))
+                                 skin=skin, field=(0.0, 0.0, 1.0), preconditioner=preconditioner)
         self.torch = self.cr.torch
         self.lib = self.cr.lib
         self.ctx = self.cr.ctx

@@ -1,0 +1,103 @@
+"""GPU solver tests beyond stage parity: the near-field block preconditioner,
+warm starts, and size-independent properties at 100k beads."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import GpuCase, n1k_system, rel, small_system
+
+pytestmark = pytest.mark.gpu
+
+N1K = dict(xi=0.929, errortol=1e-6, rc=4.0, mesh=(64, 64, 64), P=8)
+SMALL = dict(xi=0.8, errortol=1e-5, rc=3.5, mesh=(30, 30, 30), P=7)
+
+
+def _shell_shell(r):
+    if r == 0.0:
+        return 1 / (4 * math.pi)
+    return (4 - r) / (16 * math.pi) if r < 2 else 1 / (4 * math.pi * r)
+
+
+def _block_apply_reference(s, y):
+    """Per body: w = A_b^{-1} y with A_b = [[-G, 1], [1^T, 0]], solved densely."""
+    iso = s.iso_membership()
+    w = np.zeros_like(y)
+    for t in range(s.N_isolated):
+        beads = np.nonzero(iso == t)[0]
+        n = beads.size
+        G = np.zeros((n, n))
+        for a in range(n):
+            for b in range(n):
+                d = s.pos[beads[a], :3] - s.pos[beads[b], :3]
+                d -= s.L * np.round(d / s.L)
+                G[a, b] = _shell_shell(0.0 if a == b else float(np.linalg.norm(d)))
+        A = np.zeros((n + 1, n + 1))
+        A[:n, :n] = -G
+        A[:n, n] = 1
+        A[n, :n] = 1
+        sol = np.linalg.solve(A, np.concatenate([y[beads], [y[s.N + t]]]))
+        w[beads] = sol[:n]
+        w[s.N + t] = sol[n]
+    return w
+
+
+def test_block_preconditioner_is_blockwise_inverse(require_gpu):
+    s = small_system(seed=21)
+    g = GpuCase(s, preconditioner="block", **SMALL)
+    y = np.random.default_rng(1).normal(size=g.n)
+    v = rel(g.precondition(y), _block_apply_reference(s, y))
+    print(f"[block] apply vs dense per-body solve: {v:.2e}", flush=True)
+    assert v < 1e-5   # inverses stored in FP32
+
+
+@pytest.mark.parametrize("which", ["n1k", "small"])
+def test_block_preconditioner_same_solution_fewer_iterations(which, require_gpu):
+    s, prm = (n1k_system(), N1K) if which == "n1k" else (small_system(), SMALL)
+    gd = GpuCase(s, gmres_tol=1e-12, restart=30, max_iter=800, **prm)
+    qd, Pd, Fd, sd = gd.solve("zero")
+    gb = GpuCase(s, gmres_tol=1e-12, restart=30, max_iter=800, preconditioner="block", **prm)
+    qb, Pb, Fb, sb = gb.solve("zero")
+    print(f"[block] {which}: iters diag={sd['iterations']} block={sb['iterations']} "
+          f"relres={sb['true_relres']:.2e}", flush=True)
+    assert sd["converged"] and sb["converged"]
+    assert rel(qb, qd) < 1e-9 and rel(Pb, Pd) < 1e-9 and rel(Fb[:, :3], Fd[:, :3]) < 1e-9
+    assert sb["iterations"] < sd["iterations"]
+
+
+def test_warm_start_from_solution_takes_no_iterations(require_gpu):
+    s = small_system(seed=22)
+    g = GpuCase(s, gmres_tol=1e-8, restart=30, max_iter=500, preconditioner="block", **SMALL)
+    q, P, F, st = g.solve("zero")
+    assert st["converged"] and st["iterations"] > 0
+    q2, P2, F2, st2 = g.solve("previous")
+    assert st2["converged"] and st2["iterations"] == 0
+    assert rel(F2[:, :3], F[:, :3]) < 1e-12
+
+
+def test_100k_properties(require_gpu):
+    from conductorrigid_amd.system import make_workload
+
+    s, w = make_workload("n100k")
+    prm = dict(xi=w["xi"], errortol=w["errortol"], rc=w["rc"], mesh=w["mesh"], P=w["P"])
+    g = GpuCase(s, gmres_tol=1e-8, restart=30, max_iter=600, preconditioner="block", **prm)
+    # symmetry of the operator at full size (size-independent property)
+    rng = np.random.default_rng(5)
+    x, y = rng.normal(size=g.n), rng.normal(size=g.n)
+    a, b = g.matvec(x) @ y, x @ g.matvec(y)
+    print(f"[100k] symmetry {abs(a - b) / abs(a):.2e}", flush=True)
+    assert abs(a - b) < 1e-11 * abs(a)
+    q, P, F, st = g.solve("zero")
+    print(f"[100k] iters={st['iterations']} relres={st['true_relres']:.2e} conv={st['converged']}", flush=True)
+    assert st["converged"] and st["true_relres"] < 1e-7
+    sums = np.bincount(s.iso_membership(), weights=q, minlength=s.N_isolated)
+    assert np.max(np.abs(sums - s.body_charge)) < 1e-8
+    # neutral box: total force ~ 0 relative to the force scale (SE is not exactly momentum conserving)
+    assert np.linalg.norm(F[:, :3].sum(0)) < 1e-3 * np.abs(F[:, :3]).sum() / np.sqrt(s.N)
+    # FP32 bulk vs FP64 at the same parameters: the 1e-5 relative force-error gate
+    g32 = GpuCase(s, gmres_tol=1e-8, restart=30, max_iter=600, preconditioner="block", precision="fp32", **prm)
+    q32, P32, F32, st32 = g32.solve("zero")
+    e = rel(F32[:, :3], F[:, :3])
+    print(f"[100k] fp32 iters={st32['iterations']} force err vs fp64 {e:.2e}", flush=True)
+    assert e < 1e-5

@@ -1,0 +1,328 @@
+"""Pins the CPU FP64 oracle (oracle/cr_oracle.c) to what the mathematics fixes.
+
+The reference ships no tests, fixtures or golden vectors (it cannot run here: CUDA,
+HOOMD 1.x, CUSP and boost are absent), so each oracle function is checked against
+closed forms, invariants, textbook routines or brute force, chosen so that a
+dropped term, a wrong sign or index, or a transposed operand fails a check.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+import oracle as O
+from conductorrigid_amd.system import make_system
+from helpers import rel
+
+E0 = np.array([0.0, 0.0, 1.0])
+
+
+# ---------------------------------------------------------------- parameters
+def test_smooth_list_is_all_5smooth_integers_in_range():
+    got = O.smooth_list()
+    brute = []
+    for n in range(8, 4097):
+        m = n
+        for f in (2, 3, 5):
+            while m % f == 0:
+                m //= f
+        if m == 1:
+            brute.append(n)
+    assert got == brute and len(got) == 131
+
+
+@pytest.mark.parametrize("xi,tol,L,rc,mesh,P", [
+    (0.5, 1e-3, 27.565, 5.257, 25, 5),
+    (0.929, 1e-6, 27.565, 4.001, 64, 9),
+    (0.929, 1e-6, 111.769, 4.001, 250, 9),
+    (0.929, 1e-6, 240.8, 4.001, 540, 9),
+])
+def test_params_golden(xi, tol, L, rc, mesh, P):
+    p = O.make_params([L] * 3, xi, tol)
+    assert round(p.rc, 3) == rc and list(p.N) == [mesh] * 3 and p.P == P
+    assert p.N_table == int(p.rc / 0.001 - 1)
+    # eta is computed with P before the clamp to the mesh (SetParams order)
+    q = O.make_params([3.0] * 3, 0.5, 1e-6, mesh=(8, 8, 8))
+    assert q.P == 8 and abs(q.eta[0] - 9 * (3.0 / 8) ** 2 * 0.25 / math.pi) < 1e-15
+
+
+# ------------------------------------------------------ real-space kernel
+def _recip_continuum(r, xi):
+    """(1/2 pi^2) int_0^inf j0(k)^2 e^{-k^2/4xi^2} sin(kr)/(kr) dk: the k-space part of
+    the shell-shell interaction in infinite space (Green's function 1/k^2)."""
+    f = lambda k: (math.sin(k) / k) ** 2 * math.exp(-k * k / (4 * xi * xi)) * (
+        math.sin(k * r) / (k * r) if r > 0 else 1.0)
+    val, _ = integrate.quad(f, 0, 60 * xi, limit=400, epsabs=1e-14, epsrel=1e-12)
+    return val / (2 * math.pi ** 2)
+
+
+@pytest.mark.parametrize("xi", [0.6, 0.929])
+@pytest.mark.parametrize("r", [0.3, 1.0, 1.7, 2.0, 2.5, 3.3, 5.0])
+def test_real_plus_recip_is_shell_shell_potential(xi, r):
+    """Ewald split must reassemble the exact interaction of two unit spherical shells
+    (1/4 pi r for r >= 2, (4 - r)/16 pi for overlapping shells)."""
+    exact = 1 / (4 * math.pi * r) if r >= 2 else (4 - r) / (16 * math.pi)
+    total = O.phiq_exact(r, xi) + _recip_continuum(r, xi)
+    assert abs(total - exact) < 1e-10 * abs(exact)
+
+
+@pytest.mark.parametrize("xi", [0.5, 0.929, 1.3])
+def test_self_term_is_zero_distance_limit(xi):
+    p = O.make_params([20.0] * 3, xi, 1e-6)
+    # linear extrapolation to r = 0 (the kernel has a finite slope there)
+    h = 1e-4
+    assert abs(2 * O.phiq_exact(h, xi) - O.phiq_exact(2 * h, xi) - p.selfterm) < 1e-9
+    # and self + continuum k-part = potential of a unit shell on itself, 1/4 pi
+    assert abs(p.selfterm + _recip_continuum(0.0, xi) - 1 / (4 * math.pi)) < 1e-10
+
+
+@pytest.mark.parametrize("xi", [0.6, 0.929])
+def test_gradient_kernel_is_derivative(xi):
+    for r in np.linspace(0.05, 6.0, 97):
+        h = 1e-5
+        fd = (O.phiq_exact(r + h, xi) - O.phiq_exact(r - h, xi)) / (2 * h)
+        g = O.gradphiq_exact(r, xi)
+        assert abs(fd - g) <= 1e-7 * max(1e-3, abs(g)), r
+    # continuity of the kernel across the overlap boundary r = 2
+    assert abs(O.phiq_exact(2 - 1e-12, xi) - O.phiq_exact(2 + 1e-12, xi)) < 2e-11
+
+
+def test_table_interpolation():
+    sysdef = make_system(4, 8, 0.1, seed=2)
+    o = O.Oracle(sysdef, 0.929, 1e-6, rc=4.0)
+    tab, gtab = o.tables()
+    p = o.p
+    # node values are exact, .y holds the next node, last .y is zero
+    for i in (0, 1, 1000, p.N_table - 1):
+        assert tab[i, 0] == O.phiq_exact((i + 1) * 0.001, p.xi)
+        assert tab[i, 1] == tab[i + 1, 0]
+    assert tab[p.N_table, 1] == 0.0
+    # linear interpolation error bound dr^2/8 max|phi''| away from the kink
+    for r in np.linspace(0.01, 3.99, 211):
+        v = O.lib().cro_table_lookup(O.C.byref(p), O._d(np.ascontiguousarray(tab.ravel())), float(r))
+        assert abs(v - O.phiq_exact(r, p.xi)) < 2e-6 * abs(O.phiq_exact(r, p.xi)) + 1e-9, r
+
+
+# ----------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("shape", [(8, 9, 10), (5, 6, 8), (16, 12, 15), (7, 11, 13), (32, 30, 25)])
+def test_fft_matches_numpy(shape):
+    rng = np.random.default_rng(0)
+    a, b = rng.normal(size=shape), rng.normal(size=shape)
+    r, i = O.fft3d(a, b, -1)
+    assert rel(r + 1j * i, np.fft.fftn(a + 1j * b)) < 1e-14
+    r, i = O.fft3d(a, b, +1)
+    assert rel(r + 1j * i, np.fft.ifftn(a + 1j * b) * a.size) < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(8, 9, 10), (5, 6, 8)])
+def test_fft_matches_brute_force_dft(shape):
+    rng = np.random.default_rng(1)
+    a, b = rng.normal(size=shape), rng.normal(size=shape)
+    r, i = O.fft3d(a, b, -1)
+    br, bi = O.dft3d_brute(a, b, -1)
+    assert rel(r + 1j * i, br + 1j * bi) < 1e-13
+
+
+# ------------------------------------------------------- spread / contract
+@pytest.fixture(scope="module")
+def n1k():
+    s = make_system(50, 20, 0.2, seed=1, charges="alternating")
+    return s, O.Oracle(s, 0.929, 1e-6, rc=4.0, mesh=(64, 64, 64), P=8)
+
+
+def test_spread_contract_adjoint(n1k):
+    s, o = n1k
+    rng = np.random.default_rng(2)
+    q = rng.normal(size=s.N)
+    g = rng.normal(size=tuple(o.p.N))
+    lhs = o.contract(g) @ q
+    rhs = o.p.quadW * np.sum(g * o.spread(q))
+    assert abs(lhs - rhs) < 1e-12 * abs(rhs)
+
+
+def test_spread_conserves_charge_with_wide_window():
+    s = make_system(6, 8, 0.2, seed=3)
+    q = np.random.default_rng(3).normal(size=s.N)
+    for P, bound in ((8, 2e-5), (12, 1e-8)):
+        o = O.Oracle(s, 0.8, 1e-6, rc=3.0, mesh=(40, 40, 40), P=P)
+        tot = o.p.quadW * o.spread(q).sum()
+        assert abs(tot - q.sum()) < bound * np.abs(q).sum()
+
+
+def test_spread_window_holds_nearest_nodes():
+    """One bead: the window values equal the periodic Gaussian at those nodes, and
+    along each axis the window is the set of the P nodes nearest the bead."""
+    s = make_system(1, 1, 0.05, seed=4)
+    for P in (7, 8):
+        o = O.Oracle(s, 0.9, 1e-6, rc=2.5, mesh=(24, 24, 24), P=P)
+        g = o.spread(np.array([1.0]))
+        assert np.count_nonzero(g) == P ** 3
+        node = o.p.h[0] * np.arange(24) - 0.5 * s.L[0]
+        w = []
+        for ax in range(3):
+            d = node - s.pos[0, ax]
+            d -= s.L[ax] * np.round(d / s.L[ax])
+            nearest = set(np.argsort(np.abs(d))[:P].tolist())
+            used = set(np.nonzero(np.any(g != 0, axis=tuple(a for a in range(3) if a != ax)))[0].tolist())
+            assert used == nearest, (P, ax)
+            w.append(np.exp(-o.p.xiterm * d ** 2 / o.p.eta[ax]))
+        full = o.p.prefac * w[0][:, None, None] * w[1][None, :, None] * w[2][None, None, :]
+        inside = g != 0
+        assert np.allclose(g[inside], full[inside], rtol=1e-13, atol=0)
+
+
+def test_se_recip_matches_direct_ksum():
+    """Mesh k-space part (spread, FFT, scale, IFFT, contract) against the explicit k-sum:
+    pins the 1/G, h^3 and Green's-function normalisation."""
+    s = make_system(5, 8, 0.15, seed=5, charges="alternating")
+    q = np.random.default_rng(5).normal(size=s.N)
+    q -= q.mean()
+    o = O.Oracle(s, 0.7, 1e-8, rc=3.0, mesh=(48, 48, 48), P=14)
+    se = o.contract(o.recip_grid(o.spread(q)))
+    # direct k-sum only (rc_real tiny -> no real-space pairs; remove the self term)
+    dk = o.direct_matvec(q, rc_real=1e-9, kmax=2 * math.pi * 24 / s.L[0]) - o.p.selfterm * q
+    assert rel(se, dk) < 1e-6
+
+
+# ----------------------------------------------------------------- operator
+def test_matvec_symmetric(n1k):
+    s, o = n1k
+    x = np.random.default_rng(6).normal(size=o.n)
+    y = np.random.default_rng(7).normal(size=o.n)
+    a, b = o.matvec(x) @ y, x @ o.matvec(y)
+    assert abs(a - b) < 1e-12 * abs(a)
+
+
+def test_constraint_rows_and_objective():
+    s = make_system(8, [8, 20], 0.2, seed=8, charges="uniform", n_isolated_beads=4)
+    o = O.Oracle(s, 0.8, 1e-5, rc=3.5, mesh=(30, 30, 30), P=7)
+    q = np.random.default_rng(8).normal(size=s.N)
+    x = np.concatenate([q, np.zeros(s.N_isolated)])
+    y = o.matvec(x)
+    assert np.allclose(y[s.N:], np.bincount(s.iso_membership(), weights=q, minlength=s.N_isolated), atol=1e-13)
+    b = o.objective(E0)
+    iso = s.iso_membership()
+    for i in range(s.N):
+        r = s.rigid_membership[i]
+        if r == 0xFFFFFFFF:
+            assert b[i] == 0.0
+        else:
+            d = s.pos[i, :3] - s.rigid_com[r, :3]
+            d -= s.L * np.round(d / s.L)
+            assert abs(b[i] + d @ E0) < 1e-13
+    assert np.array_equal(b[s.N:], s.body_charge)
+    # preconditioner: bead rows x(-4 pi), rigid rows / size, isolated-bead rows / 4 pi
+    z = np.random.default_rng(9).normal(size=o.n)
+    pz = o.precondition(z)
+    assert np.allclose(pz[:s.N], -4 * math.pi * z[:s.N], rtol=1e-15)
+    sizes = np.bincount(iso, minlength=s.N_isolated)
+    for t in range(s.N_isolated):
+        expect = z[s.N + t] / sizes[t] if t < s.N_rigid else z[s.N + t] / (4 * math.pi)
+        assert abs(pz[s.N + t] - expect) < 1e-15 * max(1, abs(expect))
+
+
+def test_gmres_matches_dense_lu():
+    s = make_system(6, 8, 0.2, seed=10, charges="alternating", n_isolated_beads=3)
+    o = O.Oracle(s, 0.8, 1e-5, rc=3.5, mesh=(24, 24, 24), P=6)
+    A = np.stack([o.matvec(e) for e in np.eye(o.n)], axis=1)
+    b = o.objective(E0)
+    x_lu = np.linalg.solve(A, b)
+    x, info = o.gmres(b, restart=30, maxiter=2000, tol=1e-13)
+    assert info["converged"]
+    assert rel(x, x_lu) < 1e-9
+    # charge constraints hold
+    assert np.allclose(np.bincount(s.iso_membership(), weights=x[:s.N], minlength=s.N_isolated), s.body_charge,
+                       atol=1e-10)
+
+
+def test_gmres_iteration_limit_and_full_gmres():
+    s = make_system(4, 8, 0.2, seed=11, charges="alternating")
+    o = O.Oracle(s, 0.8, 1e-5, rc=3.5, mesh=(20, 20, 20), P=6)
+    b = o.objective(E0)
+    _, info = o.gmres(b, restart=10, maxiter=5, tol=1e-14)
+    assert info["iters"] == 5 and not info["converged"]
+    # restart >= n: converges within n steps (up to rounding)
+    _, info = o.gmres(b, restart=o.n if o.n <= 64 else 64, maxiter=o.n + 5, tol=1e-10)
+    assert info["converged"]
+    # exact initial guess: converged with zero iterations
+    x, _ = o.gmres(b, restart=30, maxiter=500, tol=1e-13)
+    _, info = o.gmres(b, x0=x, restart=30, maxiter=500, tol=1e-9)
+    assert info["iters"] == 0 and info["converged"]
+
+
+# ------------------------------------------------------------- direct Ewald
+@pytest.fixture(scope="module")
+def tiny():
+    return make_system(3, 8, 0.12, seed=12, charges="alternating", n_isolated_beads=2)
+
+
+def test_direct_ewald_xi_invariance(tiny):
+    s = tiny
+    q = np.random.default_rng(12).normal(size=s.N)
+    q -= q.mean()
+    vals = []
+    for xi in (0.6, 0.9, 1.2):
+        o = O.Oracle(s, xi, 1e-6, rc=2.5)
+        vals.append(o.direct_matvec(q, rc_real=2 + 7.0 / xi, kmax=2 * xi * 7.5))
+    assert rel(vals[0], vals[2]) < 1e-9 and rel(vals[1], vals[2]) < 1e-9
+
+
+def test_se_matvec_close_to_direct_ewald(tiny):
+    s = tiny
+    q = np.random.default_rng(13).normal(size=s.N)
+    q -= q.mean()
+    o = O.Oracle(s, 0.9, 1e-8, rc=4.0, mesh=(48, 48, 48), P=12)
+    x = np.concatenate([q, np.zeros(s.N_isolated)])
+    se = -o.matvec(x)[:s.N]
+    direct = o.direct_matvec(q, rc_real=4.0, kmax=2 * math.pi * 24 / s.L[0])
+    assert rel(se, direct) < 1e-6
+
+
+def _direct_solve(s, xi, rc_real, kmax):
+    o = O.Oracle(s, xi, 1e-6, rc=2.5)
+    N, B = s.N, s.N_isolated
+    M = np.stack([o.direct_matvec(e, rc_real, kmax) for e in np.eye(N)], axis=1)
+    iso = s.iso_membership()
+    Sg = np.zeros((B, N))
+    Sg[iso, np.arange(N)] = 1.0
+    A = np.block([[-M, Sg.T], [Sg, np.zeros((B, B))]])
+    b = o.objective(E0)
+    x = np.linalg.solve(A, b)
+    q = x[:N]
+    # L* = q.Mq/2 - sum_i q_i E0.(r_i - R_b): the field term exactly as the objective
+    # states it (minimum image to the body centre), so bodies straddling the box
+    # boundary need no unwrapping
+    U = 0.5 * q @ M @ q + q @ b[:N]
+    return o, q, U
+
+
+def test_direct_forces_are_energy_gradient(tiny):
+    """Envelope theorem: F_i = -dU*/dr_i with U = q.Mq/2 - sum q_i E0.r_i at the solved q."""
+    s = tiny
+    xi, rcr, kmax = 0.9, 2 + 7.0 / 0.9, 2 * 0.9 * 7.5
+    o, q, U = _direct_solve(s, xi, rcr, kmax)
+    F = o.direct_force(q, E0, rcr, kmax)
+    h = 1e-5
+    for i in (0, 5, s.N - 1):
+        for d in range(3):
+            fd = []
+            for sign in (+1, -1):
+                s2 = make_system(3, 8, 0.12, seed=12, charges="alternating", n_isolated_beads=2)
+                s2.pos[i, d] += sign * h
+                fd.append(_direct_solve(s2, xi, rcr, kmax)[2])
+            grad = (fd[0] - fd[1]) / (2 * h)
+            assert abs(-grad - F[i, d]) < 1e-6 * np.abs(F[:, :3]).max(), (i, d)
+
+
+# ---------------------------------------------------------- neighbour list
+def test_nlist_is_all_pairs_within_cutoff():
+    s = make_system(40, 20, 0.25, seed=14, charges="uniform")
+    rl = 4.4
+    head, nn, nl = O.build_nlist(s.pos, s.L, rl)
+    d = s.pos[:, None, :3] - s.pos[None, :, :3]
+    d -= s.L * np.round(d / s.L)
+    r2 = (d ** 2).sum(-1)
+    for i in range(0, s.N, 37):
+        expect = set(np.nonzero((r2[i] < rl * rl) & (np.arange(s.N) != i))[0].tolist())
+        assert set(nl[head[i]:head[i] + nn[i]].tolist()) == expect
