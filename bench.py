@@ -4,14 +4,19 @@ One step = one force evaluation (computeForces): positions -> GMRES solve for th
 bead charges and body potentials -> bead forces.  The configuration is static and
 its neighbour list is built once, outside the timed region.  Workload: N = 1,000,000
 beads (50,000 rigid bodies x 20 beads, volume fraction 0.3, seeded), mesh 256^3,
-P = 8, rc = 4, xi = 0.6, GMRES restart 30, tolerance 1e-6, E0 = (0,0,1).
+P = 8, rc = 4, xi = 0.6, GMRES restart 30, tolerance 1e-6, E0 = (0,0,1), near-field
+block preconditioner.
 
 Headline value: FP32-bulk mode (spread / FFT / contract / pair sum in float, Krylov
-and constraint sums in double), solved from x0 = 0 every step.  The plain FP64 GPU
-solve on the same system is timed beside it, with the relative force error of the
-FP32-bulk result against it.  roofline: the dominant matvec kernel's algorithmic
-bytes per launch over its HIP-event-timed average duration.  cpu_baseline: the CPU
-FP64 restatement (oracle/) timed on this host on a bounded sample.
+and constraint sums in double), solved from x0 = 0 every step.  Beside it:
+  * the plain FP64 GPU solve on the same system, with the relative force and charge
+    error of the FP32-bulk result against it;
+  * GMRES iterations from x0 = 0 and from the previous step's solution along a short
+    run of rigid random body translations;
+  * roofline: the dominant matvec kernel's algorithmic bytes per launch over its
+    average launch duration, timed with HIP events on the library's stream;
+  * cpu_baseline: the CPU FP64 restatement (oracle/) timed on this host on a
+    bounded sample (one full-size matvec, scaled by the solve's matvec count).
 """
 from __future__ import annotations
 
@@ -26,7 +31,23 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "force evals/sec at N=1M beads, rel err≤1e-5; GMRES iters plain vs warm-start"
 HBM_PEAK_GBS = 8000.0
+
+# matvec stage -> kernel it times (FP32-bulk, P = 8)
+STAGE_KERNEL = {
+    "mesh_clear": "k_fill<float> (unbinned fallback only)",
+    "spread": "k_spread_tiled<float, double, 8>",
+    "spread_halo": "k_spread_halo_add<float>",
+    "fft_z_r2c": "k_fft_z_r2c<float>",
+    "fft_y_fwd": "k_fft_strided<float>",
+    "fft_x_scale": "k_fft_x_scale<float>",
+    "fft_y_inv": "k_fft_strided<float>",
+    "fft_z_c2r": "k_fft_z_c2r<float>",
+    "contract": "k_contract_tiled<float, false, 8>",
+    "real_space": "k_real_space_csr<float>",
+    "constraint_rows": "k_sum_charges + k_set_charge",
+}
 
 
 def log(msg: str) -> None:
@@ -35,11 +56,14 @@ def log(msg: str) -> None:
 
 def algorithmic_bytes(stage: str, N: int, G: int, Gh: int, nn: float, B: int, t: int) -> float:
     """Minimal HBM bytes a stage must move: its inputs once and its outputs once.
-    t = bytes per bulk scalar (4 FP32-bulk, 8 FP64); bead record = 4 scalars (x,y,z,q)."""
+    t = bytes per bulk scalar (4 FP32-bulk, 8 FP64); a bead record is 4 scalars
+    (x, y, z, q).  The halo add of the binned spread has no algorithmic bytes: it
+    exists only because of the tiling, so its time is pure overhead."""
     rec = 4 * t
     return {
         "mesh_clear": t * G,
         "spread": rec * N + t * G,
+        "spread_halo": 0.0,
         "fft_z_r2c": t * G + 2 * t * Gh,
         "fft_y_fwd": 4 * t * Gh,
         "fft_x_scale": 4 * t * Gh,
@@ -61,14 +85,17 @@ def cpu_baseline(sysdef, prm, matvecs_per_eval: float, nlist) -> dict:
     o = O.Oracle(sysdef, prm["xi"], prm["errortol"], rc=prm["rc"], mesh=prm["mesh"], P=prm["P"], nlist=nlist)
     x = np.random.default_rng(0).normal(size=o.n)
     o.matvec(x)  # warm (page in)
-    t0 = time.perf_counter()
-    o.matvec(x)
-    dt = time.perf_counter() - t0
+    reps = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        o.matvec(x)
+        reps.append(time.perf_counter() - t0)
+    dt = float(np.median(reps))
     per_eval = dt * (matvecs_per_eval + 1.0)
     return {"value": 1.0 / per_eval, "unit": "force evals/sec", "cores": int(O.lib().cro_num_threads()),
             "kind": "port",
-            "sample": f"one FP64 matvec of the CPU restatement on the full N={sysdef.N} system "
-                      f"({dt:.2f} s), scaled by {matvecs_per_eval:.0f}+1 matvecs per force evaluation "
+            "sample": f"median of 3 FP64 matvecs of the CPU restatement on the full N={sysdef.N} system "
+                      f"({dt:.3f} s each), scaled by {matvecs_per_eval:.0f}+1 matvecs per force evaluation "
                       f"(the FP64 GPU solve's count)"}
 
 
@@ -78,6 +105,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="n1m")
+    ap.add_argument("--warm-steps", type=int, default=4, help="steps of the x0=previous run (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=5)
@@ -92,15 +120,15 @@ def main() -> int:
     import torch
 
     from conductorrigid_amd import _lib
-    from conductorrigid_amd.compute import ConductorRigid
-    from conductorrigid_amd.system import make_workload
+    from conductorrigid_amd.compute import ConductorRigid, update_positions
+    from conductorrigid_amd.system import make_workload, translate_bodies
 
     t0 = time.perf_counter()
     sysdef, w = make_workload(args.workload)
     log(f"system {args.workload}: N={sysdef.N} bodies={sysdef.N_isolated} L={sysdef.L[0]:.3f} "
         f"({time.perf_counter() - t0:.1f} s)")
     prm = dict(xi=w["xi"], errortol=w["errortol"], rc=w["rc"], mesh=w["mesh"], P=w["P"])
-    common = dict(system=sysdef, field=(0.0, 0.0, 1.0), restart=w["restart"], gmres_tol=w["tol"], max_iter=1000,
+    common = dict(field=(0.0, 0.0, 1.0), restart=w["restart"], gmres_tol=w["tol"], max_iter=1000,
                   x0="zero", preconditioner="block", **prm)
 
     def time_steps(cr, steps, warmup):
@@ -115,11 +143,13 @@ def main() -> int:
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / steps, iters, st
 
-    cr32 = ConductorRigid(precision="fp32", **common)
+    cr32 = ConductorRigid(system=sysdef, precision="fp32", **common)
     log(f"params {cr32.params['mesh']} P={cr32.params['P']} rc={cr32.params['rc']} pairs={cr32.dsys.n_pairs}")
     dt32, it32, st32 = time_steps(cr32, args.steps, args.warmup)
     F32 = cr32.get_forces()
-    log(f"fp32-bulk: {dt32 * 1e3:.2f} ms/eval iters={it32} conv={st32['converged']} true_resid={st32['true_resid']:.3e} tol={st32['tolerance']:.3e}")
+    q32 = cr32.get_charges()
+    log(f"fp32-bulk: {dt32 * 1e3:.2f} ms/eval iters={it32} conv={st32['converged']} "
+        f"true_relres={st32['true_relres']:.3e}")
 
     # per-stage kernel timing (HIP events on the library's stream)
     lib = _lib.load()
@@ -135,20 +165,48 @@ def main() -> int:
     stages = {}
     for k, name in enumerate(_lib.STAGE_NAMES):
         b = algorithmic_bytes(name, sysdef.N, G, Gh, nn, sysdef.N_isolated, 4)
-        stages[name] = {"ms": ms[k], "GBs": b / (ms[k] * 1e-3) / 1e9 if ms[k] > 0 else None, "bytes": b}
+        gbs = b / (ms[k] * 1e-3) / 1e9 if ms[k] > 0 and b > 0 else None
+        stages[name] = {"ms": ms[k], "GBs": gbs, "bytes": b, "kernel": STAGE_KERNEL[name]}
     dom = max(stages, key=lambda k: stages[k]["ms"])
     log("stages " + ", ".join(f"{k}={v['ms'] * 1e3:.1f}us" for k, v in stages.items()))
-    roof = {"bound": "hbm", "kernel": dom, "achieved": stages[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": stages[dom]["GBs"] / HBM_PEAK_GBS, "traffic": None,
-            "bytes_per_launch": stages[dom]["bytes"], "avg_launch_ms": stages[dom]["ms"]}
+    d = stages[dom]
+    achieved = d["GBs"] or 0.0
+    roof = {"bound": "hbm", "kernel": d["kernel"], "stage": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "bytes_per_launch": d["bytes"], "avg_launch_ms": d["ms"]}
+    sc_bytes = stages["spread"]["bytes"] + stages["contract"]["bytes"]
+    sc_ms = stages["spread"]["ms"] + stages["spread_halo"]["ms"] + stages["contract"]["ms"]
+    spread_contract = {"GBs": sc_bytes / (sc_ms * 1e-3) / 1e9, "frac": sc_bytes / (sc_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    matvec_ms = sum(v["ms"] for v in stages.values())
+
+    # warm start from the previous step along rigid random translations
+    warm = None
+    if args.warm_steps > 0:
+        pos0, com0 = sysdef.pos.copy(), sysdef.rigid_com.copy()
+        cr32.computeForces(0, x0="zero")
+        its, tms = [], []
+        for k in range(args.warm_steps):
+            pos, com = translate_bodies(cr32.system, 0.02, seed=1000 + k)
+            update_positions(cr32, pos, com)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            st = cr32.computeForces(0, x0="previous")
+            torch.cuda.synchronize()
+            tms.append(time.perf_counter() - t)
+            its.append(st["iterations"])
+        warm = {"iters": its, "ms_per_eval": 1e3 * float(np.mean(tms)), "step": 0.02,
+                "moves": "rigid random body translations, uniform in [-0.02, 0.02]^3 per step"}
+        log(f"x0=previous: iters={its} {warm['ms_per_eval']:.2f} ms/eval")
+        update_positions(cr32, pos0, com0)  # restore the timed configuration
 
     extra = {}
     if not args.no_fp64:
-        cr64 = ConductorRigid(precision="fp64", **common)
+        sys64, _ = make_workload(args.workload)
+        cr64 = ConductorRigid(system=sys64, precision="fp64", **common)
         dt64, it64, st64 = time_steps(cr64, max(1, min(args.steps, 3)), 1)
         F64 = cr64.get_forces()
         errF = float(np.linalg.norm(F32[:, :3] - F64[:, :3]) / np.linalg.norm(F64[:, :3]))
-        errq = float(np.linalg.norm(cr32.get_charges() - cr64.get_charges()) / np.linalg.norm(cr64.get_charges()))
+        errq = float(np.linalg.norm(q32 - cr64.get_charges()) / np.linalg.norm(cr64.get_charges()))
         log(f"fp64: {dt64 * 1e3:.2f} ms/eval iters={it64}; fp32 rel force err {errF:.2e} charge err {errq:.2e}")
         extra.update(fp64_evals_per_sec=1.0 / dt64, fp64_ms_per_eval=dt64 * 1e3, fp64_iters=it64[-1],
                      fp64_matvecs=st64["matvecs"], speedup_vs_fp64=dt64 / dt32, rel_force_err_vs_fp64=errF,
@@ -167,7 +225,7 @@ def main() -> int:
         log(f"cpu baseline {cpu['value']:.4f} evals/s ({cpu['cores']} threads)")
 
     out = {
-        "metric": "force evals/sec at N=1M beads, rel err<=1e-5; GMRES iters plain vs warm-start",
+        "metric": METRIC,
         "value": 1.0 / dt32,
         "unit": "force evals/sec",
         "n_gpus": 1,
@@ -181,11 +239,18 @@ def main() -> int:
         "data": "synthetic (seeded rigid-cluster system)",
         "config": {"workload": f"{args.workload}: N={sysdef.N} beads, {sysdef.N_rigid} rigid bodies x 20, phi=0.3, "
                                f"mesh {mesh[0]}^3, P={cr32.params['P']}, rc={cr32.params['rc']}, xi={prm['xi']}, "
-                               f"GMRES restart {w['restart']} tol {w['tol']}, x0=0",
-                   "beads": sysdef.N, "mesh": mesh, "gmres_iters": it32[-1], "matvecs": st32["matvecs"]},
+                               f"GMRES restart {w['restart']} tol {w['tol']}, near-field block preconditioner, x0=0",
+                   "beads": sysdef.N, "mesh": mesh},
+        "gmres_iters": {"zero": it32[-1], "previous": warm["iters"] if warm else None, "learned": None},
+        "matvecs_per_eval": st32["matvecs"],
+        "true_relres": st32["true_relres"],
         "roofline": roof,
+        "spread_contract_hbm": spread_contract,
         "cpu_baseline": cpu,
+        "matvec_ms": matvec_ms,
         "stages_ms": {k: v["ms"] for k, v in stages.items()},
+        "stages_GBs": {k: v["GBs"] for k, v in stages.items()},
+        "warm_start_previous": warm,
         **extra,
     }
     print(json.dumps(out), flush=True)

@@ -158,6 +158,7 @@ struct cr_context {
     // bead binning for the owner-computes spread
     TileGeom tg{};
     DevBuf bead_tile, tile_count, tile_start, tile_cursor, sorted, scan_sums, halo;
+    DevBuf rec_off, rec_w;  // per-bead window records in binned order
     // Krylov workspace
     int R = 10;
     long ld = 0;
@@ -233,6 +234,16 @@ int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out)
         CR_HIP(launch_bin_beads(c->mp, c->tg, *out, c->bead_tile.as<unsigned>(), c->tile_count.as<unsigned>(),
                                 c->tile_start.as<unsigned>(), c->tile_cursor.as<unsigned>(), c->sorted.as<unsigned>(),
                                 c->scan_sums.as<unsigned long long>(), st));
+        const size_t nb = (size_t)std::max(1, s->N);
+        CR_HIP(c->rec_off.ensure(sizeof(int) * nb));
+        CR_HIP(c->rec_w.ensure(Tsz * 3 * (size_t)c->mp.P * nb));
+        if (c->fp32())
+            CR_HIP(launch_bead_records<float>(c->mp, c->tg, *out, c->sorted.as<unsigned>(), c->bead_tile.as<unsigned>(),
+                                              c->rec_off.as<int>(), c->rec_w.as<float>(), st));
+        else
+            CR_HIP(launch_bead_records<double>(c->mp, c->tg, *out, c->sorted.as<unsigned>(),
+                                               c->bead_tile.as<unsigned>(), c->rec_off.as<int>(),
+                                               c->rec_w.as<double>(), st));
     }
     return CR_OK;
 }
@@ -271,9 +282,19 @@ int recip_impl(cr_context* c, hipStream_t st)
     return CR_OK;
 }
 
+BinView bin_view(const cr_context* c)
+{
+    BinView b;
+    b.tile_start = c->tile_start.as<unsigned>();
+    b.sorted = c->sorted.as<unsigned>();
+    b.rec_off = c->rec_off.as<int>();
+    b.rec_w = c->rec_w.p;
+    return b;
+}
+
 bool contract_tiled_ok(const cr_context* c, size_t elem)
 {
-    return c->tg.valid && tiled_lds(c->mp, c->tg, elem).chunk >= 4;
+    return c->tg.valid && tiled_lds(c->mp, c->tg, elem, elem).chunk >= 4;
 }
 
 // phi <- contraction of the context mesh (binned LDS version when the tile + halo fits)
@@ -281,8 +302,8 @@ template <typename T>
 int contract_impl(cr_context* c, const SysView& v, T* phi, hipStream_t st)
 {
     if (contract_tiled_ok(c, sizeof(T)))
-        CR_HIP(launch_contract_tiled<T>(c->mp, c->tg, v, c->tile_start.as<unsigned>(), c->sorted.as<unsigned>(),
-                                        nullptr, c->grid_real.as<T>(), phi, nullptr, st));
+        CR_HIP(launch_contract_tiled<T>(c->mp, c->tg, v, bin_view(c), nullptr, c->grid_real.as<T>(), phi, nullptr,
+                                        st));
     else
         CR_HIP(launch_contract<T>(c->mp, v, c->grid_real.as<T>(), phi, st));
     return CR_OK;
@@ -292,8 +313,7 @@ template <typename T>
 int contract_force_impl(cr_context* c, const SysView& v, const double* q, double* force, hipStream_t st)
 {
     if (contract_tiled_ok(c, sizeof(T)))
-        CR_HIP(launch_contract_tiled<T>(c->mp, c->tg, v, c->tile_start.as<unsigned>(), c->sorted.as<unsigned>(), q,
-                                        c->grid_real.as<T>(), nullptr, force, st));
+        CR_HIP(launch_contract_tiled<T>(c->mp, c->tg, v, bin_view(c), q, c->grid_real.as<T>(), nullptr, force, st));
     else
         CR_HIP(launch_contract_force<T>(c->mp, v, q, c->grid_real.as<T>(), force, st));
     return CR_OK;
@@ -329,12 +349,11 @@ int ensure_csr(cr_context* c, const SysView& v, hipStream_t st)
 
 // the spread kernel alone (the atomic fallback expects a cleared mesh)
 template <typename T>
-int spread_only(cr_context* c, const SysView& v, const double* q, hipStream_t st)
+int spread_only(cr_context* c, const SysView& v, const double* q, hipStream_t st, int parts = 3)
 {
     if (c->tg.valid)
-        CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, v, c->tile_start.as<unsigned>(), c->sorted.as<unsigned>(), q,
-                                      c->grid_real.as<T>(), c->halo.as<T>(), st));
-    else
+        CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, bin_view(c), q, c->grid_real.as<T>(), c->halo.as<T>(), st, parts));
+    else if (parts & 1)
         CR_HIP(launch_spread<T>(c->mp, v, q, c->grid_real.as<T>(), st));
     return CR_OK;
 }
@@ -343,8 +362,7 @@ template <typename T>
 int spread_impl(cr_context* c, const SysView& v, const double* q, hipStream_t st)
 {
     if (c->tg.valid) {
-        CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, v, c->tile_start.as<unsigned>(), c->sorted.as<unsigned>(), q,
-                                      c->grid_real.as<T>(), c->halo.as<T>(), st));
+        CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, bin_view(c), q, c->grid_real.as<T>(), c->halo.as<T>(), st));
     } else {
         CR_HIP(launch_fill<T>(c->grid_real.as<T>(), (T)0, c->G, st));
         CR_HIP(launch_spread<T>(c->mp, v, q, c->grid_real.as<T>(), st));
@@ -705,7 +723,8 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
     c->sc.invG = 1.0 / (double)c->G;
     // binned spread/contract tiling: per axis the largest divisor of the mesh size in
     // [max(P-1, 4), 16], else the smallest one in (16, 32]; the tile extended by P-1
-    // nodes per axis must fit 64 KiB of LDS, shrinking z, then y, then x if needed
+    // nodes per axis, held in the spread's double accumulator, must fit 3/4 of the
+    // 64 KiB LDS budget, shrinking z, then y, then x if needed
     const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
     {
         TileGeom& tg = c->tg;
@@ -732,7 +751,7 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
             }
         }
         auto ext_bytes = [&]() {
-            return (size_t)(tg.T[0] + r.P - 1) * (tg.T[1] + r.P - 1) * (tg.T[2] + r.P - 1) * Tsz;
+            return (size_t)(tg.T[0] + r.P - 1) * (tg.T[1] + r.P - 1) * (tg.T[2] + r.P - 1) * sizeof(SpreadAcc);
         };
         // leave about a quarter of the LDS budget for the per-bead staging tables
         const size_t ext_cap = kTiledLdsBudget * 3 / 4;
@@ -752,7 +771,8 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
             if (std::sscanf(env, "%d,%d,%d", &t3[0], &t3[1], &t3[2]) == 3) {
                 bool ok = true;
                 for (int d = 0; d < 3; ++d) ok = ok && t3[d] >= std::max(r.P - 1, 1) && Nd[d] % t3[d] == 0;
-                const size_t eb = (size_t)(t3[0] + r.P - 1) * (t3[1] + r.P - 1) * (t3[2] + r.P - 1) * Tsz;
+                const size_t eb =
+                    (size_t)(t3[0] + r.P - 1) * (t3[1] + r.P - 1) * (t3[2] + r.P - 1) * sizeof(SpreadAcc);
                 if (ok && eb <= ext_cap) {
                     for (int d = 0; d < 3; ++d) tg.T[d] = t3[d];
                     tg.valid = 1;
@@ -764,7 +784,7 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
             tg.nt[d] = Nd[d] / tg.T[d];
             tg.ntiles *= tg.nt[d];
         }
-        if (tg.valid) CR_HIP(c->halo.ensure(ext_bytes() * (size_t)tg.ntiles));
+        if (tg.valid) CR_HIP(c->halo.ensure(ext_bytes() / sizeof(SpreadAcc) * Tsz * (size_t)tg.ntiles));
     }
     CR_HIP(c->grid_real.ensure(Tsz * (size_t)c->G));
     CR_HIP(c->grid_cplx.ensure(2 * Tsz * (size_t)c->Gh));
@@ -1023,20 +1043,22 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
             T* cp = c->grid_cplx.as<T>();
             if (!c->tg.valid) CR_HIP(launch_fill<T>(real, (T)0, c->G, st));
             CR_HIP(hipEventRecord(ev[1], st));
-            CR_RET(spread_only<T>(c, v, x, st));
+            CR_RET(spread_only<T>(c, v, x, st, 1));
             CR_HIP(hipEventRecord(ev[2], st));
-            CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st));
+            CR_RET(spread_only<T>(c, v, x, st, 2));
             CR_HIP(hipEventRecord(ev[3], st));
-            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, -1, st));
+            CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st));
             CR_HIP(hipEventRecord(ev[4], st));
-            CR_HIP(launch_fft_x_scale<T>(pl[0], c->mp.Nx, c->mp.Ny, c->Nzh, cp, c->sc, st));
+            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, -1, st));
             CR_HIP(hipEventRecord(ev[5], st));
-            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, +1, st));
+            CR_HIP(launch_fft_x_scale<T>(pl[0], c->mp.Nx, c->mp.Ny, c->Nzh, cp, c->sc, st));
             CR_HIP(hipEventRecord(ev[6], st));
-            CR_HIP(launch_fft_z_c2r<T>(pl[2], c->mp.Nx, c->mp.Ny, cp, real, st));
+            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, +1, st));
             CR_HIP(hipEventRecord(ev[7], st));
-            CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
+            CR_HIP(launch_fft_z_c2r<T>(pl[2], c->mp.Nx, c->mp.Ny, cp, real, st));
             CR_HIP(hipEventRecord(ev[8], st));
+            CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
+            CR_HIP(hipEventRecord(ev[9], st));
             CR_HIP(launch_real_space_csr<T>(c->rp, v, x, c->phi_recip.as<T>(), c->rs_ldn, c->rs_col.as<unsigned>(),
                                             c->rs_val.as<T>(), c->rs_len.as<unsigned>(), y, 1, st));
         } else {
@@ -1046,27 +1068,29 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
             T* cp = c->grid_cplx.as<T>();
             if (!c->tg.valid) CR_HIP(launch_fill<T>(real, (T)0, c->G, st));
             CR_HIP(hipEventRecord(ev[1], st));
-            CR_RET(spread_only<T>(c, v, x, st));
+            CR_RET(spread_only<T>(c, v, x, st, 1));
             CR_HIP(hipEventRecord(ev[2], st));
-            CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st));
+            CR_RET(spread_only<T>(c, v, x, st, 2));
             CR_HIP(hipEventRecord(ev[3], st));
-            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, -1, st));
+            CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st));
             CR_HIP(hipEventRecord(ev[4], st));
-            CR_HIP(launch_fft_x_scale<T>(pl[0], c->mp.Nx, c->mp.Ny, c->Nzh, cp, c->sc, st));
+            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, -1, st));
             CR_HIP(hipEventRecord(ev[5], st));
-            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, +1, st));
+            CR_HIP(launch_fft_x_scale<T>(pl[0], c->mp.Nx, c->mp.Ny, c->Nzh, cp, c->sc, st));
             CR_HIP(hipEventRecord(ev[6], st));
-            CR_HIP(launch_fft_z_c2r<T>(pl[2], c->mp.Nx, c->mp.Ny, cp, real, st));
+            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, +1, st));
             CR_HIP(hipEventRecord(ev[7], st));
-            CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
+            CR_HIP(launch_fft_z_c2r<T>(pl[2], c->mp.Nx, c->mp.Ny, cp, real, st));
             CR_HIP(hipEventRecord(ev[8], st));
+            CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
+            CR_HIP(hipEventRecord(ev[9], st));
             CR_HIP(launch_real_space_csr<T>(c->rp, v, x, c->phi_recip.as<T>(), c->rs_ldn, c->rs_col.as<unsigned>(),
                                             c->rs_val.as<T>(), c->rs_len.as<unsigned>(), y, 1, st));
         }
-        CR_HIP(hipEventRecord(ev[9], st));
-        CR_HIP(launch_constraint_rows(v, x, y + v.N, 1, st));
         CR_HIP(hipEventRecord(ev[10], st));
-        CR_HIP(hipEventSynchronize(ev[10]));
+        CR_HIP(launch_constraint_rows(v, x, y + v.N, 1, st));
+        CR_HIP(hipEventRecord(ev[11], st));
+        CR_HIP(hipEventSynchronize(ev[11]));
         for (int k = 0; k < nst; ++k) {
             float ms = 0;
             CR_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));

@@ -64,32 +64,49 @@ struct TileGeom {
 // ---- k_se.hip ----
 template <typename T>
 hipError_t launch_spread(const MeshParams& mp, const SysView& s, const double* q, T* grid, hipStream_t st);
+
+// ---- k_tiled.hip ----
 // Bin beads by the tile holding their window's first node (counting sort):
 // tile_count[ntiles+1] (zeroed here), tile_start[ntiles+1], cursor[ntiles], sorted[N].
 hipError_t launch_bin_beads(const MeshParams& mp, const TileGeom& tg, const SysView& s, unsigned* bead_tile,
                             unsigned* tile_count, unsigned* tile_start, unsigned* cursor, unsigned* sorted,
                             unsigned long long* scan_sums, hipStream_t st);
-// halo: per-tile scratch of (T+P-1)^3 elements per tile (the +side extensions)
+// Binned bead order and per-bead window records (k_bead_records): rec_off packs the
+// window's first node relative to its tile origin (x | y << 10 | z << 20), rec_w holds
+// the 3P separable axis weights (x, then y, then z) in the mesh precision.
+struct BinView {
+    const unsigned* tile_start;   // [ntiles + 1]
+    const unsigned* sorted;       // [N] bead index per binned slot
+    const int* rec_off;           // [N]
+    const void* rec_w;            // [N * 3P]
+};
 template <typename T>
-hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const SysView& s, const unsigned* tile_start,
-                               const unsigned* sorted, const double* q, T* grid, T* halo, hipStream_t st);
+hipError_t launch_bead_records(const MeshParams& mp, const TileGeom& tg, const SysView& s, const unsigned* sorted,
+                               const unsigned* bead_tile, int* rec_off, T* rec_w, hipStream_t st);
+// halo: per-tile scratch of (T+P-1)^3 elements per tile (the +side extensions).
+// parts: bit 0 launches the tile kernel, bit 1 the halo add (3 = the whole spread).
 template <typename T>
-hipError_t launch_contract(const MeshParams& mp, const SysView& s, const T* grid, T* phi, hipStream_t st);
-// LDS plan of the binned kernels (k_tiled.hip): the tile extended by P-1 nodes per axis
-// plus per-bead staging tables for `chunk` beads, within kTiledLdsBudget bytes.
+hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const BinView& bv, const double* q, T* grid,
+                               T* halo, hipStream_t st, int parts = 3);
+// LDS plan of the binned kernels: the tile extended by P-1 nodes per axis (ext_elem bytes
+// per node) plus per-bead staging tables (stage_elem bytes per value) for `chunk` beads,
+// within kTiledLdsBudget bytes.  The spread accumulates its tile in SpreadAcc.
 constexpr size_t kTiledLdsBudget = 64 * 1024;
+using SpreadAcc = double;
 struct TiledLds {
     size_t ext_bytes;
     size_t total_bytes;
     int chunk;
 };
-TiledLds tiled_lds(const MeshParams& mp, const TileGeom& tg, size_t elem);
+TiledLds tiled_lds(const MeshParams& mp, const TileGeom& tg, size_t ext_elem, size_t stage_elem);
 // Binned contraction from an LDS copy of the tile and its +side halo: potential (force == nullptr)
 // or reciprocal force (phi unused).
 template <typename T>
-hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const SysView& s, const unsigned* tile_start,
-                                 const unsigned* sorted, const double* q, const T* grid, T* phi, double* force,
-                                 hipStream_t st);
+hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const SysView& s, const BinView& bv,
+                                 const double* q, const T* grid, T* phi, double* force, hipStream_t st);
+// ---- k_se.hip (unbinned fallbacks) ----
+template <typename T>
+hipError_t launch_contract(const MeshParams& mp, const SysView& s, const T* grid, T* phi, hipStream_t st);
 template <typename T>
 hipError_t launch_contract_force(const MeshParams& mp, const SysView& s, const double* q, const T* grid,
                                  double* force, hipStream_t st);
@@ -166,7 +183,7 @@ hipError_t build_nlist(int N, const double4* pos, const double L[3], double rlis
 int krylov_blocks(long n);
 hipError_t launch_multidot(const double* V, long ld, int ncols, const double* w, long n, double* partial,
                            hipStream_t st);
-// out[k] = sum over blocks of partial[b*32+k] (+ out_add[k] if non-null), k < ncols
+// out[k] = sum over blocks of partial[b*32+k], k < ncols (fixed summation order)
 hipError_t launch_reduce_partials(const double* partial, int nblocks, int ncols, double* out, hipStream_t st);
 // w -= V[:,0:ncols] h; then (mode 1) partial dots V^T w, (mode 2) partial ||w||^2, (mode 0) nothing
 hipError_t launch_gs_update(const double* V, long ld, int ncols, const double* h, double* w, long n, int mode,

@@ -64,13 +64,24 @@ __global__ __launch_bounds__(kBlock) void k_multidot(const double* __restrict__ 
         partial[(long)blockIdx.x * 32 + c0 + threadIdx.x] = res[threadIdx.x];
 }
 
-__global__ void k_reduce_partials(const double* __restrict__ partial, int nblocks, int ncols, double* __restrict__ out)
+// out[k] = sum_b partial[b*32 + k]: one workgroup per column; each thread sums a
+// strided subset of the blocks (independent loads in flight), then a fixed-order
+// tree over the workgroup, so the result is deterministic.
+__global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __restrict__ partial, int nblocks,
+                                                            double* __restrict__ out)
 {
-    const int k = threadIdx.x;
-    if (k >= ncols) return;
+    const int k = blockIdx.x;
     double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += partial[(long)b * 32 + k];
-    out[k] = s;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partial[(long)b * 32 + k];
+    __shared__ double red[kBlock / 64];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+        out[k] = t;
+    }
 }
 
 // w -= V[:, 0:ncols] h ; mode 2 also accumulates ||w||^2 partials into partial[b*32]
@@ -214,7 +225,8 @@ hipError_t launch_multidot(const double* V, long ld, int ncols, const double* w,
 
 hipError_t launch_reduce_partials(const double* partial, int nblocks, int ncols, double* out, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(32), 0, st, partial, nblocks, ncols, out);
+    if (ncols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(ncols), dim3(kBlock), 0, st, partial, nblocks, out);
     return hipGetLastError();
 }
 

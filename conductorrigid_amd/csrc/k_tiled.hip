@@ -84,10 +84,9 @@ __device__ __forceinline__ TileCoords tile_coords(const MeshParams& mp, const Ti
     return c;
 }
 
-// LDS: [ext Ex*Ey*Ez T][w ch*3P T][u ch*3 T][qs ch T][off ch int][ids ch uint]
+// LDS: [ext Ex*Ey*Ez (accumulator type)][w ch*3P T][u ch*3 T][qs ch T][off ch int][ids ch uint]
 template <typename T>
 struct Stage {
-    T* ext;
     T* w;
     T* u;
     T* qs;
@@ -96,11 +95,10 @@ struct Stage {
 };
 
 template <typename T>
-__device__ __forceinline__ Stage<T> stage_layout(char* smem, int esz, int ch, int P)
+__device__ __forceinline__ Stage<T> stage_layout(char* smem, size_t ext_bytes, int ch, int P)
 {
     Stage<T> s;
-    s.ext = reinterpret_cast<T*>(smem);
-    s.w = s.ext + esz;
+    s.w = reinterpret_cast<T*>(smem + ext_bytes);
     s.u = s.w + ch * 3 * P;
     s.qs = s.u + ch * 3;
     s.off = reinterpret_cast<int*>(s.qs + ch);
@@ -108,9 +106,30 @@ __device__ __forceinline__ Stage<T> stage_layout(char* smem, int esz, int ch, in
     return s;
 }
 
-// Load beads [k0, k0+cnt) of the bin and fill the chunk's LDS tables.  u holds each
-// bead's position relative to its window start in node units (node s+p minus bead =
-// (p - u) h), so the axis weight is exp(-2 xi^2 h^2 (p - u)^2 / eta).
+// A bead's window: packed offset of its first node in the tile (x | y << 10 | z << 20)
+// and u, its position relative to the window start in node units (node s+p minus bead
+// = (p - u) h), so the axis weight is exp(-2 xi^2 h^2 (p - u)^2 / eta).
+__device__ __forceinline__ int bead_window(const MeshParams& mp, const double4& r, int x0, int y0, int z0,
+                                           double u[3])
+{
+    const int P = mp.P;
+    const int sx = window_start(r.x, mp.L[0], mp.Nx, P);
+    const int sy = window_start(r.y, mp.L[1], mp.Ny, P);
+    const int sz = window_start(r.z, mp.L[2], mp.Nz, P);
+    u[0] = (r.x + 0.5 * mp.L[0]) / mp.h[0] - sx;
+    u[1] = (r.y + 0.5 * mp.L[1]) / mp.h[1] - sy;
+    u[2] = (r.z + 0.5 * mp.L[2]) / mp.h[2] - sz;
+    const int ox = wrap_idx(sx, mp.Nx) - x0, oy = wrap_idx(sy, mp.Ny) - y0, oz = wrap_idx(sz, mp.Nz) - z0;
+    return ox | (oy << 10) | (oz << 20);
+}
+
+template <typename T>
+__device__ __forceinline__ T axis_coef(const MeshParams& mp, int d)
+{
+    return (T)(mp.xiterm * mp.h[d] * mp.h[d] / mp.eta[d]);
+}
+
+// Load beads [k0, k0+cnt) of the bin and fill the chunk's LDS tables from positions.
 template <typename T>
 __device__ __forceinline__ void stage_chunk(const MeshParams& mp, const TileCoords& tc, const unsigned* __restrict__ sorted,
                                             unsigned k0, int cnt, const double4* __restrict__ pos,
@@ -119,23 +138,17 @@ __device__ __forceinline__ void stage_chunk(const MeshParams& mp, const TileCoor
     const int P = mp.P;
     for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
         const unsigned bead = sorted[k0 + i];
-        const double4 r = pos[bead];
-        const int sx = window_start(r.x, mp.L[0], mp.Nx, P);
-        const int sy = window_start(r.y, mp.L[1], mp.Ny, P);
-        const int sz = window_start(r.z, mp.L[2], mp.Nz, P);
-        const int ox = wrap_idx(sx, mp.Nx) - tc.x0, oy = wrap_idx(sy, mp.Ny) - tc.y0, oz = wrap_idx(sz, mp.Nz) - tc.z0;
-        s.off[i] = ox | (oy << 10) | (oz << 20);
+        double u[3];
+        s.off[i] = bead_window(mp, pos[bead], tc.x0, tc.y0, tc.z0, u);
         s.ids[i] = bead;
-        s.u[3 * i + 0] = (T)((r.x + 0.5 * mp.L[0]) / mp.h[0] - sx);
-        s.u[3 * i + 1] = (T)((r.y + 0.5 * mp.L[1]) / mp.h[1] - sy);
-        s.u[3 * i + 2] = (T)((r.z + 0.5 * mp.L[2]) / mp.h[2] - sz);
+        s.u[3 * i + 0] = (T)u[0];
+        s.u[3 * i + 1] = (T)u[1];
+        s.u[3 * i + 2] = (T)u[2];
         s.qs[i] = q ? qscale * (T)q[bead] : (T)0;
     }
     __syncthreads();
     const int per = 3 * P;
-    const T c0 = (T)(mp.xiterm * mp.h[0] * mp.h[0] / mp.eta[0]);
-    const T c1 = (T)(mp.xiterm * mp.h[1] * mp.h[1] / mp.eta[1]);
-    const T c2 = (T)(mp.xiterm * mp.h[2] * mp.h[2] / mp.eta[2]);
+    const T c0 = axis_coef<T>(mp, 0), c1 = axis_coef<T>(mp, 1), c2 = axis_coef<T>(mp, 2);
     for (int j = threadIdx.x; j < cnt * per; j += blockDim.x) {
         const int i = j / per, rem = j - i * per;
         const int d = rem / P, p = rem - d * P;
@@ -146,28 +159,104 @@ __device__ __forceinline__ void stage_chunk(const MeshParams& mp, const TileCoor
     __syncthreads();
 }
 
+// Window records of every bead in binned order, built once per bead configuration so
+// that each operator application streams them instead of re-deriving windows.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bead_records(MeshParams mp, TileGeom tg, int N,
+                                                      const unsigned* __restrict__ sorted,
+                                                      const unsigned* __restrict__ bead_tile,
+                                                      const double4* __restrict__ pos, int* __restrict__ rec_off,
+                                                      T* __restrict__ rec_w)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const unsigned bead = sorted[k];
+    const int t = (int)bead_tile[bead];
+    const int tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
+    double u[3];
+    rec_off[k] = bead_window(mp, pos[bead], tx * tg.T[0], ty * tg.T[1], tz * tg.T[2], u);
+    const int P = mp.P;
+    T* w = rec_w + (long)k * 3 * P;
+    for (int d = 0; d < 3; ++d) {
+        const T cd = axis_coef<T>(mp, d), ud = (T)u[d];
+        for (int p = 0; p < P; ++p) {
+            const T dd = (T)p - ud;
+            w[d * P + p] = gexp<T>(-cd * dd * dd);
+        }
+    }
+}
+
+// Fill the chunk's LDS tables for beads [k0, k0+cnt) from their window records.
+template <typename T>
+__device__ __forceinline__ void stage_records(int P, const unsigned* __restrict__ sorted, unsigned k0, int cnt,
+                                              const int* __restrict__ rec_off, const T* __restrict__ rec_w,
+                                              const double* __restrict__ q, T qscale, Stage<T>& s)
+{
+    const int per = 3 * P;
+    const T* src = rec_w + (long)k0 * per;
+    for (int j = threadIdx.x; j < cnt * per; j += blockDim.x) s.w[j] = src[j];
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const unsigned bead = sorted[k0 + i];
+        s.off[i] = rec_off[k0 + i];
+        s.ids[i] = bead;
+        if (q) s.qs[i] = qscale * (T)q[bead];
+    }
+    __syncthreads();
+}
+
+// Walks a thread's nodes k = tid, tid + stride, ... of an Ex x Ey x Ez box in
+// row-major order, keeping (lx, ly, lz) by carries instead of divisions.
+struct BoxWalk {
+    int lx, ly, lz, dx, dy, dz, Ey, Ez;
+    __device__ __forceinline__ BoxWalk(int tid, int stride, int Ey_, int Ez_) : Ey(Ey_), Ez(Ez_)
+    {
+        lz = tid % Ez;
+        const int r = tid / Ez;
+        ly = r % Ey;
+        lx = r / Ey;
+        dz = stride % Ez;
+        const int q = stride / Ez;
+        dy = q % Ey;
+        dx = q / Ey;
+    }
+    __device__ __forceinline__ void next()
+    {
+        lz += dz;
+        int c = lz >= Ez;
+        lz -= c ? Ez : 0;
+        ly += dy + c;
+        c = ly >= Ey;
+        ly -= c ? Ey : 0;
+        lx += dx + c;
+    }
+};
+
 // PT > 0: window support fixed at compile time (loops unroll, P divisions become
 // constants, the P node updates of a lane issue back to back); PT == 0: runtime P <= 16.
-template <typename T, int PT>
+// The LDS tile accumulates in A = double for both mesh precisions: ds_add_f64 measures
+// several times the throughput of ds_add_f32 on gfx950, and the sums gain accuracy.
+template <typename T, typename A, int PT>
 __global__ __launch_bounds__(256) void k_spread_tiled(MeshParams mp, TileGeom tg, int ch,
                                                       const unsigned* __restrict__ tstart,
                                                       const unsigned* __restrict__ sorted,
-                                                      const double4* __restrict__ pos, const double* __restrict__ q,
-                                                      T* __restrict__ grid, T* __restrict__ halo)
+                                                      const int* __restrict__ rec_off, const T* __restrict__ rec_w,
+                                                      const double* __restrict__ q, T* __restrict__ grid,
+                                                      T* __restrict__ halo)
 {
     extern __shared__ char smem_raw[];
     const TileCoords tc = tile_coords(mp, tg);
     constexpr int PM = PT > 0 ? PT : 16;
     const int P = PT > 0 ? PT : mp.P;
     const int esz = tc.Ex * tc.Ey * tc.Ez;
-    Stage<T> s = stage_layout<T>(smem_raw, esz, ch, P);
-    for (int k = threadIdx.x; k < esz; k += blockDim.x) s.ext[k] = (T)0;
+    A* ext = reinterpret_cast<A*>(smem_raw);
+    Stage<T> s = stage_layout<T>(smem_raw, (size_t)esz * sizeof(A), ch, P);
+    for (int k = threadIdx.x; k < esz; k += blockDim.x) ext[k] = (A)0;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     const int xs = tc.Ey * tc.Ez, ys = tc.Ez, per = 3 * P;
     const unsigned kb = tstart[tc.t], ke = tstart[tc.t + 1];
     for (unsigned k0 = kb; k0 < ke; k0 += ch) {
         const int cnt = (int)min((unsigned)ch, ke - k0);
-        stage_chunk<T>(mp, tc, sorted, k0, cnt, pos, q, (T)mp.prefac, s);
+        stage_records<T>(P, sorted, k0, cnt, rec_off, rec_w, q, (T)mp.prefac, s);
         for (int i = wave; i < cnt; i += nwave) {
             const int o = s.off[i];
             const int ox = o & 1023, oy = (o >> 10) & 1023, oz = o >> 20;
@@ -179,10 +268,10 @@ __global__ __launch_bounds__(256) void k_spread_tiled(MeshParams mp, TileGeom tg
             for (int pair = lane; pair < P * P; pair += kWave) {
                 const int py = pair / P, pz = pair - py * P;
                 const T c = qs * wi[P + py] * wi[2 * P + pz];
-                T* col = s.ext + ox * xs + (oy + py) * ys + (oz + pz);
+                A* col = ext + ox * xs + (oy + py) * ys + (oz + pz);
 #pragma unroll
                 for (int px = 0; px < PM; ++px)
-                    if (px < P) atomic_add(col + px * xs, c * wx[px]);
+                    if (px < P) atomic_add(col + px * xs, (A)(c * wx[px]));
             }
         }
         __syncthreads();
@@ -190,53 +279,57 @@ __global__ __launch_bounds__(256) void k_spread_tiled(MeshParams mp, TileGeom tg
     const long plane = (long)mp.Ny * mp.Nz;
     T* hb = halo + (long)tc.t * esz;
     const int Tx = tg.T[0], Ty = tg.T[1], Tz = tg.T[2];
-    for (int k = threadIdx.x; k < esz; k += blockDim.x) {
-        const int lz = k % tc.Ez, ly = (k / tc.Ez) % tc.Ey, lx = k / (tc.Ez * tc.Ey);
-        if (lx < Tx && ly < Ty && lz < Tz)
-            grid[(long)(tc.x0 + lx) * plane + (long)(tc.y0 + ly) * mp.Nz + (tc.z0 + lz)] = s.ext[k];
+    BoxWalk bw(threadIdx.x, blockDim.x, tc.Ey, tc.Ez);
+    for (int k = threadIdx.x; k < esz; k += blockDim.x, bw.next()) {
+        if (bw.lx < Tx && bw.ly < Ty && bw.lz < Tz)
+            grid[(long)(tc.x0 + bw.lx) * plane + (long)(tc.y0 + bw.ly) * mp.Nz + (tc.z0 + bw.lz)] = (T)ext[k];
         else
-            hb[k] = s.ext[k];
+            hb[k] = (T)ext[k];
     }
 }
 
 // Adds the extensions of the (up to 7) tiles below each node's tile into the mesh.
+// Block (y, x) covers one z row, threads walk z: no 64-bit index arithmetic.
 template <typename T>
 __global__ __launch_bounds__(256) void k_spread_halo_add(MeshParams mp, TileGeom tg, const T* __restrict__ halo,
                                                          T* __restrict__ grid)
 {
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long G = (long)mp.Nx * mp.Ny * mp.Nz;
-    if (g >= G) return;
-    const int z = (int)(g % mp.Nz), y = (int)((g / mp.Nz) % mp.Ny), x = (int)(g / ((long)mp.Nz * mp.Ny));
+    const int y = blockIdx.x, x = blockIdx.y;
     const int P = mp.P;
-    const int lx = x % tg.T[0], ly = y % tg.T[1], lz = z % tg.T[2];
-    if (lx >= P - 1 && ly >= P - 1 && lz >= P - 1) return;
-    const int tx = x / tg.T[0], ty = y / tg.T[1], tz = z / tg.T[2];
+    const int lx = x % tg.T[0], ly = y % tg.T[1];
+    const int tx = x / tg.T[0], ty = y / tg.T[1];
     const int Ey = tg.T[1] + P - 1, Ez = tg.T[2] + P - 1;
     const long esz = (long)(tg.T[0] + P - 1) * Ey * Ez;
-    T acc = 0;
-    for (int a = 0; a < 2; ++a) {
-        if (a && lx >= P - 1) continue;
-        for (int b = 0; b < 2; ++b) {
-            if (b && ly >= P - 1) continue;
-            for (int c = 0; c < 2; ++c) {
-                if ((c && lz >= P - 1) || (a + b + c == 0)) continue;
-                const int sx = (tx - a + tg.nt[0]) % tg.nt[0];
-                const int sy = (ty - b + tg.nt[1]) % tg.nt[1];
-                const int sz = (tz - c + tg.nt[2]) % tg.nt[2];
-                const long src = ((long)sx * tg.nt[1] + sy) * tg.nt[2] + sz;
-                const int ex = lx + a * tg.T[0], ey = ly + b * tg.T[1], ez = lz + c * tg.T[2];
-                acc += halo[src * esz + ((long)ex * Ey + ey) * Ez + ez];
+    T* row = grid + ((long)x * mp.Ny + y) * mp.Nz;
+    const bool xy_inner = lx >= P - 1 && ly >= P - 1;
+    for (int z = threadIdx.x; z < mp.Nz; z += blockDim.x) {
+        const int lz = z % tg.T[2], tz = z / tg.T[2];
+        if (xy_inner && lz >= P - 1) continue;
+        T acc = 0;
+        for (int a = 0; a < 2; ++a) {
+            if (a && lx >= P - 1) continue;
+            for (int b = 0; b < 2; ++b) {
+                if (b && ly >= P - 1) continue;
+                for (int c = 0; c < 2; ++c) {
+                    if ((c && lz >= P - 1) || (a + b + c == 0)) continue;
+                    const int sx = (tx - a + tg.nt[0]) % tg.nt[0];
+                    const int sy = (ty - b + tg.nt[1]) % tg.nt[1];
+                    const int sz = (tz - c + tg.nt[2]) % tg.nt[2];
+                    const long src = ((long)sx * tg.nt[1] + sy) * tg.nt[2] + sz;
+                    const int ex = lx + a * tg.T[0], ey = ly + b * tg.T[1], ez = lz + c * tg.T[2];
+                    acc += halo[src * esz + ((long)ex * Ey + ey) * Ez + ez];
+                }
             }
         }
+        row[z] += acc;
     }
-    grid[g] += acc;
 }
 
 template <typename T, bool FORCE, int PT>
 __global__ __launch_bounds__(256) void k_contract_tiled(MeshParams mp, TileGeom tg, int ch,
                                                         const unsigned* __restrict__ tstart,
                                                         const unsigned* __restrict__ sorted,
+                                                        const int* __restrict__ rec_off, const T* __restrict__ rec_w,
                                                         const double4* __restrict__ pos, const double* __restrict__ q,
                                                         const T* __restrict__ grid, T* __restrict__ phi,
                                                         double* __restrict__ force)
@@ -246,12 +339,16 @@ __global__ __launch_bounds__(256) void k_contract_tiled(MeshParams mp, TileGeom 
     constexpr int PM = PT > 0 ? PT : 16;
     const int P = PT > 0 ? PT : mp.P;
     const int esz = tc.Ex * tc.Ey * tc.Ez;
-    Stage<T> s = stage_layout<T>(smem_raw, esz, ch, P);
+    T* ext = reinterpret_cast<T*>(smem_raw);
+    Stage<T> s = stage_layout<T>(smem_raw, (size_t)esz * sizeof(T), ch, P);
     const long plane = (long)mp.Ny * mp.Nz;
-    for (int k = threadIdx.x; k < esz; k += blockDim.x) {
-        const int lz = k % tc.Ez, ly = (k / tc.Ez) % tc.Ey, lx = k / (tc.Ez * tc.Ey);
-        const int gx = (tc.x0 + lx) % mp.Nx, gy = (tc.y0 + ly) % mp.Ny, gz = (tc.z0 + lz) % mp.Nz;
-        s.ext[k] = grid[(long)gx * plane + (long)gy * mp.Nz + gz];
+    BoxWalk bw(threadIdx.x, blockDim.x, tc.Ey, tc.Ez);
+    for (int k = threadIdx.x; k < esz; k += blockDim.x, bw.next()) {
+        int gx = tc.x0 + bw.lx, gy = tc.y0 + bw.ly, gz = tc.z0 + bw.lz;
+        gx -= gx >= mp.Nx ? mp.Nx : 0;
+        gy -= gy >= mp.Ny ? mp.Ny : 0;
+        gz -= gz >= mp.Nz ? mp.Nz : 0;
+        ext[k] = grid[(long)gx * plane + (long)gy * mp.Nz + gz];
     }
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     const int xs = tc.Ey * tc.Ez, ys = tc.Ez, per = 3 * P;
@@ -259,12 +356,16 @@ __global__ __launch_bounds__(256) void k_contract_tiled(MeshParams mp, TileGeom 
     const unsigned kb = tstart[tc.t], ke = tstart[tc.t + 1];
     for (unsigned k0 = kb; k0 < ke; k0 += ch) {
         const int cnt = (int)min((unsigned)ch, ke - k0);
-        stage_chunk<T>(mp, tc, sorted, k0, cnt, pos, FORCE ? q : nullptr, (T)1, s);
+        if (FORCE)
+            stage_chunk<T>(mp, tc, sorted, k0, cnt, pos, q, (T)1, s);
+        else
+            stage_records<T>(P, sorted, k0, cnt, rec_off, rec_w, nullptr, (T)1, s);
         for (int i = wave; i < cnt; i += nwave) {
             const int o = s.off[i];
             const int ox = o & 1023, oy = (o >> 10) & 1023, oz = o >> 20;
             const T* wi = s.w + i * per;
-            const T ux = s.u[3 * i], uy = s.u[3 * i + 1], uz = s.u[3 * i + 2];
+            const T ux = FORCE ? s.u[3 * i] : (T)0, uy = FORCE ? s.u[3 * i + 1] : (T)0;
+            const T uz = FORCE ? s.u[3 * i + 2] : (T)0;
             T wx[PM];
 #pragma unroll
             for (int px = 0; px < PM; ++px) wx[px] = (px < P) ? wi[px] : (T)0;
@@ -272,7 +373,7 @@ __global__ __launch_bounds__(256) void k_contract_tiled(MeshParams mp, TileGeom 
             for (int pair = lane; pair < P * P; pair += kWave) {
                 const int py = pair / P, pz = pair - py * P;
                 const T wyz = wi[P + py] * wi[2 * P + pz];
-                const T* col = s.ext + ox * xs + (oy + py) * ys + (oz + pz);
+                const T* col = ext + ox * xs + (oy + py) * ys + (oz + pz);
                 T g[PM];
 #pragma unroll
                 for (int px = 0; px < PM; ++px) g[px] = (px < P) ? col[px * xs] : (T)0;
@@ -292,12 +393,12 @@ __global__ __launch_bounds__(256) void k_contract_tiled(MeshParams mp, TileGeom 
             }
             const unsigned bead = s.ids[i];
             if (!FORCE) {
-                acc = wave_sum(acc);
+                acc = wave_total(acc);
                 if (lane == 0) phi[bead] = acc * pc;
             } else {
-                fx = wave_sum(fx);
-                fy = wave_sum(fy);
-                fz = wave_sum(fz);
+                fx = wave_total(fx);
+                fy = wave_total(fy);
+                fz = wave_total(fz);
                 if (lane == 0) {
                     // F = 2 (h^3 prefac) (2 xi^2) q_i sum phi_g e^{...} (r/eta), r = bead - node = (u - p) h
                     // (cppmodule/ConductorRigid.cu:449)
@@ -321,11 +422,11 @@ inline unsigned blocks(long n, int b)
 
 }  // namespace
 
-TiledLds tiled_lds(const MeshParams& mp, const TileGeom& tg, size_t elem)
+TiledLds tiled_lds(const MeshParams& mp, const TileGeom& tg, size_t ext_elem, size_t stage_elem)
 {
     TiledLds l;
-    l.ext_bytes = (size_t)(tg.T[0] + mp.P - 1) * (tg.T[1] + mp.P - 1) * (tg.T[2] + mp.P - 1) * elem;
-    const size_t per_bead = (size_t)(3 * mp.P + 3 + 1) * elem + 2 * sizeof(int);
+    l.ext_bytes = (size_t)(tg.T[0] + mp.P - 1) * (tg.T[1] + mp.P - 1) * (tg.T[2] + mp.P - 1) * ext_elem;
+    const size_t per_bead = (size_t)(3 * mp.P + 3 + 1) * stage_elem + 2 * sizeof(int);
     long ch = l.ext_bytes < kTiledLdsBudget ? (long)((kTiledLdsBudget - l.ext_bytes) / per_bead) : 0;
     if (ch > 256) ch = 256;
     ch -= ch % 4;
@@ -353,46 +454,63 @@ hipError_t launch_bin_beads(const MeshParams& mp, const TileGeom& tg, const SysV
 }
 
 template <typename T>
-hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const SysView& s, const unsigned* tile_start,
-                               const unsigned* sorted, const double* q, T* grid, T* halo, hipStream_t st)
+hipError_t launch_bead_records(const MeshParams& mp, const TileGeom& tg, const SysView& s, const unsigned* sorted,
+                               const unsigned* bead_tile, int* rec_off, T* rec_w, hipStream_t st)
 {
-    const TiledLds l = tiled_lds(mp, tg, sizeof(T));
-    if (l.chunk < 4 || mp.P > 16) return hipErrorInvalidValue;
-#define CR_SPREAD_P(PV)                                                                                          \
-    hipLaunchKernelGGL((k_spread_tiled<T, PV>), dim3(tg.ntiles), dim3(256), l.total_bytes, st, mp, tg, l.chunk,  \
-                       tile_start, sorted, s.pos, q, grid, halo)
-    switch (mp.P) {
-        case 4: CR_SPREAD_P(4); break;
-        case 5: CR_SPREAD_P(5); break;
-        case 6: CR_SPREAD_P(6); break;
-        case 7: CR_SPREAD_P(7); break;
-        case 8: CR_SPREAD_P(8); break;
-        case 9: CR_SPREAD_P(9); break;
-        case 10: CR_SPREAD_P(10); break;
-        default: CR_SPREAD_P(0); break;
-    }
-#undef CR_SPREAD_P
-    const long G = (long)mp.Nx * mp.Ny * mp.Nz;
-    hipLaunchKernelGGL(k_spread_halo_add<T>, dim3(blocks(G, 256)), dim3(256), 0, st, mp, tg, (const T*)halo, grid);
+    if (s.N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bead_records<T>, dim3(blocks(s.N, 256)), dim3(256), 0, st, mp, tg, s.N, sorted, bead_tile,
+                       s.pos, rec_off, rec_w);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const SysView& s, const unsigned* tile_start,
-                                 const unsigned* sorted, const double* q, const T* grid, T* phi, double* force,
-                                 hipStream_t st)
+hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const BinView& bv, const double* q, T* grid,
+                               T* halo, hipStream_t st, int parts)
 {
-    const TiledLds l = tiled_lds(mp, tg, sizeof(T));
+    const TiledLds l = tiled_lds(mp, tg, sizeof(SpreadAcc), sizeof(T));
     if (l.chunk < 4 || mp.P > 16) return hipErrorInvalidValue;
+    const T* rec_w = static_cast<const T*>(bv.rec_w);
+#define CR_SPREAD_P(PV)                                                                                        \
+    hipLaunchKernelGGL((k_spread_tiled<T, SpreadAcc, PV>), dim3(tg.ntiles), dim3(256), l.total_bytes, st, mp, \
+                       tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, q, grid, halo)
+    if (parts & 1) {
+        switch (mp.P) {
+            case 4: CR_SPREAD_P(4); break;
+            case 5: CR_SPREAD_P(5); break;
+            case 6: CR_SPREAD_P(6); break;
+            case 7: CR_SPREAD_P(7); break;
+            case 8: CR_SPREAD_P(8); break;
+            case 9: CR_SPREAD_P(9); break;
+            case 10: CR_SPREAD_P(10); break;
+            default: CR_SPREAD_P(0); break;
+        }
+    }
+#undef CR_SPREAD_P
+    if (parts & 2) {
+        const int zthreads = mp.Nz >= 256 ? 256 : ((mp.Nz + 63) / 64) * 64;
+        hipLaunchKernelGGL(k_spread_halo_add<T>, dim3(mp.Ny, mp.Nx), dim3(zthreads), 0, st, mp, tg, (const T*)halo,
+                           grid);
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const SysView& s, const BinView& bv,
+                                 const double* q, const T* grid, T* phi, double* force, hipStream_t st)
+{
+    const TiledLds l = tiled_lds(mp, tg, sizeof(T), sizeof(T));
+    if (l.chunk < 4 || mp.P > 16) return hipErrorInvalidValue;
+    const T* rec_w = static_cast<const T*>(bv.rec_w);
 #define CR_CONTRACT_P(PV)                                                                                       \
     do {                                                                                                        \
         if (force)                                                                                              \
             hipLaunchKernelGGL((k_contract_tiled<T, true, PV>), dim3(tg.ntiles), dim3(256), l.total_bytes, st,  \
-                               mp, tg, l.chunk, tile_start, sorted, s.pos, q, grid, (T*)nullptr, force);        \
+                               mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, s.pos, q, grid,    \
+                               (T*)nullptr, force);                                                             \
         else                                                                                                    \
             hipLaunchKernelGGL((k_contract_tiled<T, false, PV>), dim3(tg.ntiles), dim3(256), l.total_bytes, st, \
-                               mp, tg, l.chunk, tile_start, sorted, s.pos, (const double*)nullptr, grid, phi,   \
-                               (double*)nullptr);                                                               \
+                               mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, s.pos,             \
+                               (const double*)nullptr, grid, phi, (double*)nullptr);                            \
     } while (0)
     switch (mp.P) {
         case 4: CR_CONTRACT_P(4); break;
@@ -408,15 +526,15 @@ hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const
     return hipGetLastError();
 }
 
-template hipError_t launch_spread_tiled<float>(const MeshParams&, const TileGeom&, const SysView&, const unsigned*,
-                                               const unsigned*, const double*, float*, float*, hipStream_t);
-template hipError_t launch_spread_tiled<double>(const MeshParams&, const TileGeom&, const SysView&, const unsigned*,
-                                                const unsigned*, const double*, double*, double*, hipStream_t);
-template hipError_t launch_contract_tiled<float>(const MeshParams&, const TileGeom&, const SysView&, const unsigned*,
-                                                 const unsigned*, const double*, const float*, float*, double*,
-                                                 hipStream_t);
-template hipError_t launch_contract_tiled<double>(const MeshParams&, const TileGeom&, const SysView&, const unsigned*,
-                                                  const unsigned*, const double*, const double*, double*, double*,
-                                                  hipStream_t);
+#define CR_INST(T)                                                                                               \
+    template hipError_t launch_bead_records<T>(const MeshParams&, const TileGeom&, const SysView&, const unsigned*, \
+                                               const unsigned*, int*, T*, hipStream_t);                         \
+    template hipError_t launch_spread_tiled<T>(const MeshParams&, const TileGeom&, const BinView&, const double*, \
+                                               T*, T*, hipStream_t, int);                                          \
+    template hipError_t launch_contract_tiled<T>(const MeshParams&, const TileGeom&, const SysView&,              \
+                                                 const BinView&, const double*, const T*, T*, double*, hipStream_t);
+CR_INST(float)
+CR_INST(double)
+#undef CR_INST
 
 }  // namespace cr

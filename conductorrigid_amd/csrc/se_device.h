@@ -28,6 +28,44 @@ __device__ __forceinline__ T wave_sum(T v)
     return v;
 }
 
+// Sum over the 64 lanes of a fully active wavefront, returned uniformly.  DPP row shifts
+// (row_shr 1, 2, 4, 8; lanes shifted in from outside the row read 0) leave each row of
+// 16 lanes' total in its lane 15; the four row totals are then read out as scalars.
+// Stays in the VALU: no LDS round trips, unlike ds_bpermute-based shuffles.
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ float lane_value(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <typename T>
+__device__ __forceinline__ T wave_total(T v)
+{
+    v += dpp_shift<0x111>(v);
+    v += dpp_shift<0x112>(v);
+    v += dpp_shift<0x114>(v);
+    v += dpp_shift<0x118>(v);
+    return (lane_value(v, 15) + lane_value(v, 31)) + (lane_value(v, 47) + lane_value(v, 63));
+}
+
 template <typename T>
 __device__ __forceinline__ T gexp(T a);
 template <>

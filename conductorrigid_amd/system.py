@@ -82,31 +82,6 @@ def box_length(n_beads: int, phi: float, radius: float = 1.0) -> float:
     return (n_beads * 4.0 / 3.0 * math.pi * radius ** 3 / phi) ** (1.0 / 3.0)
 
 
-def _grow_cluster(rng: np.random.Generator, n: int, n_candidates: int = 24) -> np.ndarray:
-    """Random compact aggregate of ``n`` touching unit spheres.  Each new bead
-    touches (centre distance exactly 2) a randomly chosen existing bead and
-    overlaps none; of ``n_candidates`` random non-overlapping candidates the
-    one closest to the current centre is kept, which keeps the cluster compact.
-    Returns centre positions relative to the cluster's centre of mass."""
-    pts = np.zeros((n, 3))
-    for k in range(1, n):
-        for _attempt in range(1000):
-            base = pts[rng.integers(0, k, size=n_candidates)]
-            d = rng.normal(size=(n_candidates, 3))
-            d /= np.linalg.norm(d, axis=1, keepdims=True)
-            cand = base + 2.0 * d
-            dist = np.linalg.norm(cand[:, None, :] - pts[None, :k, :], axis=2)
-            good = np.all(dist >= 2.0 - 1e-12, axis=1)
-            if np.any(good):
-                c = cand[good]
-                com = pts[:k].mean(axis=0)
-                pts[k] = c[np.argmin(np.linalg.norm(c - com, axis=1))]
-                break
-        else:
-            raise RuntimeError("cluster growth stalled")
-    return pts - pts.mean(axis=0)
-
-
 def _random_rotations(rng: np.random.Generator, m: int) -> np.ndarray:
     q = rng.normal(size=(m, 4))
     q /= np.linalg.norm(q, axis=1, keepdims=True)
@@ -324,6 +299,21 @@ WORKLOADS = {
     "n1m": dict(n_bodies=50000, body_sizes=20, phi=0.3, charges="uniform", seed=3,
                 xi=0.6, errortol=1e-6, mesh=(256, 256, 256), P=8, rc=4.0, restart=30, tol=1e-6),
 }
+
+
+def translate_bodies(s: SystemDefinition, step: float, seed: int) -> tuple[np.ndarray, np.ndarray]:
+    """Rigid random translation of every isolated body by a displacement uniform in
+    [-step, step]^3 (a stand-in for one dynamics step between force evaluations).
+    Returns wrapped bead positions (N, 4) and rigid centres of mass (N_rigid, 4)."""
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(-step, step, size=(s.N_isolated, 3))
+    pos = s.pos.copy()
+    pos[:, :3] = _wrap(pos[:, :3] + d[s.iso_membership()], s.L)
+    com = s.rigid_com.copy()
+    if s.N_rigid:
+        # rigid body r carries isolated-body tag r
+        com[:, :3] = _wrap(com[:, :3] + d[:s.N_rigid], s.L)
+    return pos, com
 
 
 def make_workload(name: str) -> tuple[SystemDefinition, dict]:
