@@ -274,6 +274,34 @@ class Oracle:
         lib().cro_direct_force(C.byref(self.p), C.byref(self.sys), float(rc_real), float(kmax), _d(q), _d(E), _d(out))
         return out
 
+    def direct_solve(self, E0, rc_real, kmax, tol=1e-12, restart=60, maxiter=2000):
+        """Direct-Ewald force evaluation: the saddle-point system of the reference
+        (cppmodule/ConductorRigid.cu:862-1020) with M from direct_matvec (closed-form
+        real space over images within rc_real, explicit k-sum to kmax) solved by
+        GMRES to `tol` with the reference's diagonal preconditioner, then the direct
+        forces.  Returns (q, Phi, F, iterations)."""
+        import scipy.sparse.linalg as spla
+
+        N, B = self.sysdef.N, self.sysdef.N_isolated
+        iso = self.sysdef.iso_membership().astype(np.int64)
+
+        def A(x):
+            q, Phi = x[:N], x[N:]
+            top = -self.direct_matvec(q, rc_real, kmax) + Phi[iso]
+            return np.concatenate([top, np.bincount(iso, weights=q, minlength=B)])
+
+        n = N + B
+        Aop = spla.LinearOperator((n, n), matvec=A, dtype=np.float64)
+        Mop = spla.LinearOperator((n, n), matvec=self.precondition, dtype=np.float64)
+        b = self.objective(E0)
+        count = [0]
+        x, info = spla.gmres(Aop, b, M=Mop, rtol=tol, atol=0.0, restart=restart, maxiter=maxiter,
+                             callback=lambda _r: count.__setitem__(0, count[0] + 1), callback_type="pr_norm")
+        if info != 0:
+            raise RuntimeError(f"direct-Ewald GMRES did not converge (info={info})")
+        q, Phi = x[:N], x[N:]
+        return q, Phi, self.direct_force(q, E0, rc_real, kmax), count[0]
+
 
 def fft3d(re, im, sign):
     n = (C.c_int * 3)(*re.shape)
