@@ -124,7 +124,7 @@ const char* cr_last_error(void);
 int cr_set_params(cr_context* ctx, const cr_params* params, int n_beads, int n_isolated);
 int cr_get_resolved(const cr_context* ctx, cr_resolved* out);
 
-/* UpdateField (cppmodule/ConductorRigid.cc:382-387). */
+/* UpdateField (cppmodule/ConductorRigid.cc:381-386). */
 int cr_update_field(cr_context* ctx, const double E0[3]);
 
 /* ---- the reference's device ABI ---------------------------------------- */
