@@ -105,7 +105,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="n1m")
-    ap.add_argument("--warm-steps", type=int, default=4, help="steps of the x0=previous run (0: skip)")
+    ap.add_argument("--warm-steps", type=int, default=4, help="evaluated steps of the warm-start comparison (0: skip)")
+    ap.add_argument("--train-steps", type=int, default=24, help="trajectory steps the warm-start network trains on")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=5)
@@ -148,6 +149,7 @@ def main() -> int:
     dt32, it32, st32 = time_steps(cr32, args.steps, args.warmup)
     F32 = cr32.get_forces()
     q32 = cr32.get_charges()
+    nl32 = cr32.dsys.nlist_host() if args.no_fp64 else None
     log(f"fp32-bulk: {dt32 * 1e3:.2f} ms/eval iters={it32} conv={st32['converged']} "
         f"true_relres={st32['true_relres']:.3e}")
 
@@ -159,6 +161,7 @@ def main() -> int:
     _lib.check(lib.cr_profile_matvec(cr32.ctx, C.byref(sv), x.data_ptr(), args.profile_reps, ms,
                                      _lib.CR_NUM_STAGES, cr32.stream()))
     mesh = cr32.params["mesh"]
+    P_used, rc_used = cr32.params["P"], cr32.params["rc"]
     G = int(np.prod(mesh))
     Gh = mesh[0] * mesh[1] * (mesh[2] // 2 + 1)
     nn = cr32.dsys.n_pairs / sysdef.N
@@ -179,25 +182,37 @@ def main() -> int:
     spread_contract = {"GBs": sc_bytes / (sc_ms * 1e-3) / 1e9, "frac": sc_bytes / (sc_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     matvec_ms = sum(v["ms"] for v in stages.values())
 
-    # warm start from the previous step along rigid random translations
+    del cr32
+    # Warm starts along a Brownian trajectory with |E0| uniform in [0.5, 2] per step: the
+    # network is trained on the GPU on solver outputs from the first steps, then each
+    # later step is solved from zero, from the previous solution, from the prediction and
+    # from the previous solution plus the predicted response to the field change.
     warm = None
     if args.warm_steps > 0:
-        pos0, com0 = sysdef.pos.copy(), sysdef.rigid_com.copy()
-        cr32.computeForces(0, x0="zero")
-        its, tms = [], []
-        for k in range(args.warm_steps):
-            pos, com = translate_bodies(cr32.system, 0.02, seed=1000 + k)
-            update_positions(cr32, pos, com)
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            st = cr32.computeForces(0, x0="previous")
-            torch.cuda.synchronize()
-            tms.append(time.perf_counter() - t)
-            its.append(st["iterations"])
-        warm = {"iters": its, "ms_per_eval": 1e3 * float(np.mean(tms)), "step": 0.02,
-                "moves": "rigid random body translations, uniform in [-0.02, 0.02]^3 per step"}
-        log(f"x0=previous: iters={its} {warm['ms_per_eval']:.2f} ms/eval")
-        update_positions(cr32, pos0, com0)  # restore the timed configuration
+        import copy
+
+        import conductorrigid_amd.warmstart as WS
+
+        crw = ConductorRigid(system=copy.deepcopy(sysdef), precision="fp32", **common)
+        mlp = WS.WarmStartMLP(device=0, seed=0)
+        t0 = time.perf_counter()
+        _, pairs = WS.run_trajectory(crw, args.train_steps, 2e-4, collect=range(0, args.train_steps, 2), mlp=mlp,
+                                     seed=0)
+        hist = WS.fit(mlp, pairs, epochs=2, batch=16384, lr=2e-3)
+        t_train = time.perf_counter() - t0
+        del pairs
+        iters, wms = WS.compare_starts(crw, mlp, args.warm_steps, 2e-4, seed=1)
+        warm = {"protocol": f"Brownian trajectory (dt 2e-4, rigid bodies, contact repulsion), |E0| ~ U[0.5, 2] "
+                            f"per step; network trained on {len(range(0, args.train_steps, 2))} frames of the "
+                            f"first {args.train_steps} steps ({t_train:.1f} s incl. trajectory), evaluated on the "
+                            f"next {args.warm_steps}",
+                "iters": {k: float(np.mean(v)) for k, v in iters.items()},
+                "ms_per_eval": {k: float(np.mean(v)) for k, v in wms.items()},
+                "train_loss": [hist[0], hist[-1]],
+                "learned_speedup_vs_zero": float(np.mean(wms["zero"]) / np.mean(wms["learned"]))}
+        log("warm starts: " + ", ".join(f"{k} {warm['iters'][k]:.1f} it / {warm['ms_per_eval'][k]:.1f} ms"
+                                        for k in warm["iters"]))
+        del crw, mlp
 
     extra = {}
     if not args.no_fp64:
@@ -216,7 +231,7 @@ def main() -> int:
         del cr64
     else:
         mv = st32["matvecs"]
-        nl = cr32.dsys.nlist_host()
+        nl = nl32
     cpu = None
     if not args.no_cpu_baseline:
         torch.cuda.synchronize()
@@ -238,10 +253,14 @@ def main() -> int:
         "dtype": "f32 bulk + f64 krylov",
         "data": "synthetic (seeded rigid-cluster system)",
         "config": {"workload": f"{args.workload}: N={sysdef.N} beads, {sysdef.N_rigid} rigid bodies x 20, phi=0.3, "
-                               f"mesh {mesh[0]}^3, P={cr32.params['P']}, rc={cr32.params['rc']}, xi={prm['xi']}, "
+                               f"mesh {mesh[0]}^3, P={P_used}, rc={rc_used}, xi={prm['xi']}, "
                                f"GMRES restart {w['restart']} tol {w['tol']}, near-field block preconditioner, x0=0",
                    "beads": sysdef.N, "mesh": mesh},
-        "gmres_iters": {"zero": it32[-1], "previous": warm["iters"] if warm else None, "learned": None},
+        "gmres_iters": {"zero": it32[-1],
+                        "previous": warm["iters"]["previous"] if warm else None,
+                        "learned": warm["iters"]["learned"] if warm else None,
+                        "learned+previous": warm["iters"]["learned+previous"] if warm else None,
+                        "zero_on_trajectory": warm["iters"]["zero"] if warm else None},
         "matvecs_per_eval": st32["matvecs"],
         "true_relres": st32["true_relres"],
         "roofline": roof,
@@ -250,7 +269,7 @@ def main() -> int:
         "matvec_ms": matvec_ms,
         "stages_ms": {k: v["ms"] for k, v in stages.items()},
         "stages_GBs": {k: v["GBs"] for k, v in stages.items()},
-        "warm_start_previous": warm,
+        "warm_start": warm,
         **extra,
     }
     print(json.dumps(out), flush=True)

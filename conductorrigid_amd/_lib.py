@@ -17,6 +17,12 @@ CR_PRECISION_FP64 = 0
 CR_PRECISION_FP32_BULK = 1
 CR_X0_PREVIOUS = 0
 CR_X0_ZERO = 1
+CR_X0_CHARGES = 2
+CR_MLP_IN = 176
+CR_MLP_TRUNK = 168
+CR_MLP_HIDDEN = 128
+CR_MLP_OUT = 4
+CR_MLP_PARAMS = 39684
 CR_PRECOND_DIAGONAL = 0
 CR_PRECOND_NEARFIELD_BLOCK = 1
 CR_NUM_STAGES = 11
@@ -83,6 +89,18 @@ _SIGS = {
     "cr_real_space": (C.c_int, [vp, C.POINTER(CrSystem), vp, vp, vp]),
     "cr_real_space_force": (C.c_int, [vp, C.POINTER(CrSystem), vp, vp, vp]),
     "cr_profile_matvec": (C.c_int, [vp, C.POINTER(CrSystem), vp, C.c_int, C.POINTER(C.c_double), C.c_int, vp]),
+    "cr_bd_step": (C.c_int, [vp, C.POINTER(CrSystem), vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
+                             C.c_double, C.c_double, vp]),
+    "cr_mlp_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_ulonglong]),
+    "cr_mlp_destroy": (None, [vp]),
+    "cr_mlp_set_params": (C.c_int, [vp, vp]),
+    "cr_mlp_get_params": (C.c_int, [vp, vp]),
+    "cr_mlp_features": (C.c_int, [vp, C.POINTER(CrSystem), vp, vp]),
+    "cr_mlp_targets": (C.c_int, [vp, C.POINTER(CrSystem), vp, vp, vp]),
+    "cr_mlp_charges": (C.c_int, [vp, C.POINTER(CrSystem), vp, vp, vp]),
+    "cr_mlp_predict": (C.c_int, [vp, C.c_int, vp, vp, vp]),
+    "cr_mlp_gradient": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, vp]),
+    "cr_mlp_train_step": (C.c_int, [vp, C.c_int, vp, vp, C.c_float, vp, vp]),
     "cr_build_nlist": (C.c_int, [vp, C.c_int, vp, C.POINTER(C.c_double), C.c_double, vp, vp, vp, C.c_long,
                                  C.POINTER(C.c_long), vp]),
 }

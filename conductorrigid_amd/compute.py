@@ -27,6 +27,10 @@ from .system import NOT_MEMBER, SystemDefinition
 
 _PRECISIONS = {"fp64": _lib.CR_PRECISION_FP64, "fp32": _lib.CR_PRECISION_FP32_BULK}
 _PRECONDITIONERS = {"diagonal": _lib.CR_PRECOND_DIAGONAL, "block": _lib.CR_PRECOND_NEARFIELD_BLOCK}
+# x0: "previous" = the reference's warm start from the last solution; "zero"; "charges" =
+# from the charges currently in ``self.charges`` (e.g. a learned prediction), projected
+# onto the body charges, with fitted body potentials
+_X0_MODES = {"previous": _lib.CR_X0_PREVIOUS, "zero": _lib.CR_X0_ZERO, "charges": _lib.CR_X0_CHARGES}
 
 
 def _torch():
@@ -139,7 +143,7 @@ class ConductorRigid:
         self.enabled = True
         self._opts = dict(mesh=mesh, P=P, rc=rc, restart=restart, max_iter=max_iter, gmres_tol=gmres_tol,
                           precision=precision, preconditioner=preconditioner)
-        self.x0_mode = {"previous": _lib.CR_X0_PREVIOUS, "zero": _lib.CR_X0_ZERO}[x0]
+        self.x0_mode = _X0_MODES[x0]
         ctx = C.c_void_p()
         _lib.check(self.lib.cr_create(C.byref(ctx), int(device)))
         self.ctx = ctx
@@ -193,7 +197,7 @@ class ConductorRigid:
         return self.torch.cuda.current_stream(self.dsys.dev).cuda_stream
 
     def computeForces(self, timestep: int = 0, x0: str | None = None, stats: bool = True) -> dict | None:
-        mode = self.x0_mode if x0 is None else {"previous": _lib.CR_X0_PREVIOUS, "zero": _lib.CR_X0_ZERO}[x0]
+        mode = self.x0_mode if x0 is None else _X0_MODES[x0]
         st = _lib.CrSolveStats()
         sv = self.dsys.view()
         code = self.lib.cr_compute_force(self.ctx, C.byref(sv), self.charges.data_ptr(), self.potentials.data_ptr(),

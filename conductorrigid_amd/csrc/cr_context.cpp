@@ -9,6 +9,7 @@
 // Hessenberg matrix and rotations on the device and reads back one scalar per
 // Arnoldi step to decide convergence.
 #include "../../include/conductorrigid.h"
+#include "cr_host.h"
 #include "cr_kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -26,47 +27,6 @@ using namespace cr;
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
-thread_local std::string g_error;
-
-int set_error(int code, const std::string& msg)
-{
-    g_error = msg;
-    return code;
-}
-
-#define CR_HIP(call)                                                                                    \
-    do {                                                                                                \
-        hipError_t e_ = (call);                                                                         \
-        if (e_ != hipSuccess)                                                                           \
-            return set_error(CR_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));          \
-    } while (0)
-
-#define CR_RET(call)                    \
-    do {                                \
-        int r_ = (call);                \
-        if (r_ != CR_OK) return r_;     \
-    } while (0)
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t b)
-    {
-        if (b <= bytes && p) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        hipError_t e = hipMalloc(&p, b > 0 ? b : 16);
-        if (e == hipSuccess) bytes = b;
-        return e;
-    }
-    template <typename T>
-    T* as() const { return static_cast<T*>(p); }
-    ~DevBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
-};
 
 bool is_5smooth(int n)
 {
@@ -167,6 +127,9 @@ struct cr_context {
     // near-field block preconditioner: beads grouped by body tag, per-body inverses
     int block_nmax = 0;
     DevBuf body_count, body_start, body_cursor, body_beads, ginv, uvec, inv_s;
+    // per-body segment scratch: bead counts and double sums (warm starts, learned features)
+    DevBuf seg_count, seg_sum;
+    DevBuf bd_force;  // Brownian step: bead forces plus steric repulsion
     GmresState* g_dev = nullptr;
     GmresState* g_host = nullptr;
     double E0[3] = {0, 0, 0};
@@ -455,6 +418,18 @@ int matvec_prec(cr_context* c, const SysView& v, const double* x, double* out, h
         return apply_precond(c, v, c->yraw.as<double>(), out, st);
     }
     return matvec_any(c, v, x, out, 1, st);
+}
+
+// Per-body bead counts and a double scratch of one entry per body.
+int segments(cr_context* c, const SysView& v, hipStream_t st, unsigned** counts, double** sums)
+{
+    const size_t B = (size_t)std::max(1, v.N_isolated);
+    CR_HIP(c->seg_count.ensure(sizeof(unsigned) * B));
+    CR_HIP(c->seg_sum.ensure(sizeof(double) * B));
+    CR_HIP(launch_body_counts(v, c->seg_count.as<unsigned>(), st));
+    *counts = c->seg_count.as<unsigned>();
+    *sums = c->seg_sum.as<double>();
+    return CR_OK;
 }
 
 // Restarted, left-preconditioned GMRES(R) following CUSP's krylov::gmres structure
@@ -875,31 +850,42 @@ int cr_compute_force(cr_context* c, const cr_system* s, double* q_inout, double*
     } else if (x0_mode == CR_X0_PREVIOUS) {
         CR_HIP(hipMemcpyAsync(x, q_inout, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
         CR_HIP(hipMemcpyAsync(x + v.N, phi_inout, sizeof(double) * v.N_isolated, hipMemcpyDeviceToDevice, st));
-    } else {
+    } else if (x0_mode != CR_X0_CHARGES) {
         return set_error(CR_ERR_INVALID, "unknown x0 mode");
     }
     CR_RET(ensure_csr(c, v, st));
+    if (x0_mode == CR_X0_CHARGES) {
+        // q from the caller, shifted per body onto its net charge; Phi fitted to the
+        // equipotential rows given that q (one operator application)
+        unsigned* counts = nullptr;
+        double* sums = nullptr;
+        CR_RET(segments(c, v, st, &counts, &sums));
+        CR_HIP(hipMemcpyAsync(x, q_inout, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
+        CR_HIP(launch_project_charges(v, counts, sums, x, st));
+        CR_HIP(launch_fill<double>(x + v.N, 0.0, v.N_isolated, st));
+        double* w = c->w.as<double>();
+        CR_RET(matvec_any(c, v, x, w, 0, st));
+        CR_HIP(launch_fit_potentials(v, counts, c->b.as<double>(), w, sums, x + v.N, st));
+    }
     CR_RET(gmres(c, v, stats, st));
     CR_HIP(hipMemcpyAsync(q_inout, x, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
     CR_HIP(hipMemcpyAsync(phi_inout, x + v.N, sizeof(double) * v.N_isolated, hipMemcpyDeviceToDevice, st));
     if (stats) {
-        // FP64 judge of the converged state: ||D (b - A x)||_2
+        // FP64 judge of the converged state from one operator application r = A x - b:
+        // ||D r||_2 and the unpreconditioned ||r|| / ||b||
         double* w = c->w.as<double>();
-        CR_RET(matvec_prec(c, v, x, w, st));
-        CR_HIP(launch_axpby(-1.0, c->Mb.as<double>(), 1.0, w, n, st));
-        CR_RET(norm2(c, w, n, c->scal.as<double>(), st));
-        double r2 = 0;
-        CR_RET(read_scalar(c, c->scal.as<double>(), &r2, st));
-        stats->true_resid = std::sqrt(r2);
-        // unpreconditioned relative residual ||b - A x|| / ||b||
+        double* dw = c->yraw.as<double>();
         CR_RET(matvec_any(c, v, x, w, 0, st));
         CR_HIP(launch_axpby(-1.0, c->b.as<double>(), 1.0, w, n, st));
-        CR_RET(norm2(c, w, n, c->scal.as<double>(), st));
-        CR_RET(norm2(c, c->b.as<double>(), n, c->scal.as<double>() + 1, st));
-        double rb[2] = {0, 0};
-        CR_HIP(hipMemcpyAsync(rb, c->scal.as<double>(), 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        CR_RET(apply_precond(c, v, w, dw, st));
+        CR_RET(norm2(c, dw, n, c->scal.as<double>(), st));
+        CR_RET(norm2(c, w, n, c->scal.as<double>() + 1, st));
+        CR_RET(norm2(c, c->b.as<double>(), n, c->scal.as<double>() + 2, st));
+        double rb[3] = {0, 0, 0};
+        CR_HIP(hipMemcpyAsync(rb, c->scal.as<double>(), 3 * sizeof(double), hipMemcpyDeviceToHost, st));
         CR_HIP(hipStreamSynchronize(st));
-        stats->true_relres = rb[1] > 0 ? std::sqrt(rb[0] / rb[1]) : std::sqrt(rb[0]);
+        stats->true_resid = std::sqrt(rb[0]);
+        stats->true_relres = rb[2] > 0 ? std::sqrt(rb[1] / rb[2]) : std::sqrt(rb[1]);
     }
     if (c->fp32())
         CR_RET(force_pass<float>(c, v, x, force, st));
@@ -1112,6 +1098,74 @@ int cr_build_nlist(cr_context* c, int N, const double* pos, const double L[3], d
         if (!(L[d] > 2 * rlist) && !(L[d] > 0)) return set_error(CR_ERR_INVALID, "bad box");
     CR_HIP(build_nlist(N, reinterpret_cast<const double4*>(pos), L, rlist, head_list, n_neigh, nlist, capacity, total,
                        nullptr, (hipStream_t)stream));
+    return CR_OK;
+}
+
+// ---- learned warm start: features and the charge-share transform ----
+
+int cr_mlp_features(cr_context* c, const cr_system* s, unsigned short* X, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    SysView v;
+    CR_RET(prepare(c, s, st, &v));
+    if (v.N > 0 && !X) return set_error(CR_ERR_INVALID, "null feature array");
+    unsigned* counts = nullptr;
+    double* sums = nullptr;
+    CR_RET(segments(c, v, st, &counts, &sums));
+    CR_HIP(launch_mlp_features(v, c->mp.L, c->E0, counts, X, st));
+    return CR_OK;
+}
+
+int cr_mlp_targets(cr_context* c, const cr_system* s, const double* q, float* t, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    SysView v;
+    CR_RET(prepare(c, s, st, &v));
+    if (v.N > 0 && (!q || !t)) return set_error(CR_ERR_INVALID, "null array");
+    unsigned* counts = nullptr;
+    double* sums = nullptr;
+    CR_RET(segments(c, v, st, &counts, &sums));
+    CR_HIP(launch_mlp_share(v, counts, q, nullptr, t, nullptr, 0, st));
+    return CR_OK;
+}
+
+int cr_mlp_charges(cr_context* c, const cr_system* s, const float* y, double* q, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    SysView v;
+    CR_RET(prepare(c, s, st, &v));
+    if (v.N > 0 && (!q || !y)) return set_error(CR_ERR_INVALID, "null array");
+    unsigned* counts = nullptr;
+    double* sums = nullptr;
+    CR_RET(segments(c, v, st, &counts, &sums));
+    CR_HIP(launch_mlp_share(v, counts, nullptr, y, nullptr, q, 1, st));
+    return CR_OK;
+}
+
+// ---- Brownian rigid-body step (trajectory driver for warm-start runs) ----
+
+int cr_bd_step(cr_context* c, const cr_system* s, double* pos, double* rigid_com, const double* force,
+               const double* noise_rigid, const double* noise_beads, double dt, double kT, double gamma_t,
+               double gamma_r, double k_rep, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    SysView v;
+    CR_RET(prepare(c, s, st, &v));
+    if ((v.N > 0 && (!pos || !force)) || (v.N_rigid > 0 && (!rigid_com || !noise_rigid)))
+        return set_error(CR_ERR_INVALID, "null array");
+    if (!(dt > 0) || !(kT >= 0) || !(gamma_t > 0) || !(gamma_r > 0) || !(k_rep >= 0))
+        return set_error(CR_ERR_INVALID, "bad BD parameter");
+    CR_HIP(c->bd_force.ensure(4 * sizeof(double) * (size_t)std::max(1, v.N)));
+    BdParams p;
+    for (int d = 0; d < 3; ++d) p.L[d] = c->mp.L[d];
+    p.dt = dt;
+    p.kT = kT;
+    p.gamma_t = gamma_t;
+    p.gamma_r = gamma_r;
+    p.k_rep = k_rep;
+    CR_HIP(launch_bd_step(p, v, reinterpret_cast<double4*>(pos), reinterpret_cast<double4*>(rigid_com),
+                          reinterpret_cast<const double4*>(force), c->bd_force.as<double4>(), noise_rigid,
+                          noise_beads, st));
     return CR_OK;
 }
 

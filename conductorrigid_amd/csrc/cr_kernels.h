@@ -159,6 +159,15 @@ hipError_t launch_constraint_rows(const SysView& s, const double* q, double* y_b
 hipError_t launch_precondition(const SysView& s, const double* x, double* y, hipStream_t st);
 hipError_t launch_objective(const SysView& s, const double L[3], const double E0[3], double* b, hipStream_t st);
 hipError_t launch_membership(int N, const unsigned* tag, const unsigned* body_of_tag, unsigned* iso, hipStream_t st);
+// Per-body segments (isolated-body tag s.iso): counts[b] = beads of body b.
+hipError_t launch_body_counts(const SysView& s, unsigned* counts, hipStream_t st);
+// q_i += (Q_b - sum_{j in b} q_j) / n_b, so every body carries its net charge exactly.
+// sums: scratch [N_isolated] doubles.
+hipError_t launch_project_charges(const SysView& s, const unsigned* counts, double* sums, double* q, hipStream_t st);
+// Phi_b = mean over the beads of b of (b_i - y_i), the least-squares equipotential of the
+// rows Phi_b - (M q)_i = b_i given y_i = -(M q)_i.  sums: scratch [N_isolated] doubles.
+hipError_t launch_fit_potentials(const SysView& s, const unsigned* counts, const double* b, const double* y,
+                                 double* sums, double* phi, hipStream_t st);
 hipError_t launch_zero_force(int N, double* force, hipStream_t st);
 
 // ---- k_precond.hip ----
@@ -169,6 +178,74 @@ hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, uns
 hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_start, const unsigned* body_beads,
                               const float* ginv, const float* uvec, const double* inv_s, const double* y, double* w,
                               hipStream_t st);
+
+// ---- k_mlp.hip ----
+// Learned warm start: per-bead MLP trunk kMlpTrunk -> kMlpHidden -> kMlpHidden -> kMlpOut
+// (SiLU), output y = (E0, 1) . u (the solved charges are linear in E0 at fixed geometry).
+constexpr int kMlpIn = 176;       // feature columns per bead (row pitch)
+constexpr int kMlpTrunk = 168;    // 32 neighbours x 5, charge share, COM offset and |offset|, zero padded
+constexpr int kMlpGate = 168;     // E0 as bf16 pairs: hi at [168, 171), lo at [171, 174)
+constexpr int kMlpHidden = 128;
+constexpr int kMlpOut = 4;
+constexpr int kMlpNeigh = 32;
+constexpr double kMlpRadius = 3.0;
+// network outputs are charges in units of 4 pi (potential = q / (4 pi r) in these units)
+constexpr double kMlpChargeUnit = 12.566370614359172;
+// Flat FP32 parameter vector: W1 [kMlpIn][H] | b1 [H] | W2 [H][H] | b2 [H] | W3 [H][4] | b3 [4]
+// (W1 rows >= kMlpTrunk multiply the gate columns, which are masked, and stay unused)
+constexpr int kMlpOffW1 = 0;
+constexpr int kMlpOffB1 = kMlpOffW1 + kMlpIn * kMlpHidden;
+constexpr int kMlpOffW2 = kMlpOffB1 + kMlpHidden;
+constexpr int kMlpOffB2 = kMlpOffW2 + kMlpHidden * kMlpHidden;
+constexpr int kMlpOffW3 = kMlpOffB2 + kMlpHidden;
+constexpr int kMlpOffB3 = kMlpOffW3 + kMlpHidden * kMlpOut;
+constexpr int kMlpParams = kMlpOffB3 + kMlpOut;
+// bf16 compute copies (raw bits) and FP32 biases
+struct MlpDevWeights {
+    const unsigned short* W1t;   // [H][kMlpIn]
+    const unsigned short* W2t;   // [H][H], W2 transposed
+    const unsigned short* W2n;   // [H][H], W2 as stored
+    const unsigned short* W3;    // [H][kMlpOut]
+    const float* b1;
+    const float* b2;
+    const float* b3;
+};
+// activations and gradients of one training batch of M rows
+struct MlpTrainBufs {
+    float* z1;             // [M][H]
+    float* z2;             // [M][H]
+    float* du;             // [M][kMlpOut]
+    float* part;           // [slices][kMlpIn][H]
+    unsigned short* h1;    // [M][H]
+    unsigned short* h2;    // [M][H]
+    unsigned short* dz1;   // [M][H]
+    unsigned short* dz2;   // [M][H]
+    int slices;
+};
+hipError_t launch_mlp_features(const SysView& s, const double L[3], const double E0[3], const unsigned* body_count,
+                               unsigned short* X, hipStream_t st);
+hipError_t launch_mlp_pack(const float* theta, const MlpDevWeights& w, hipStream_t st);
+// y[m] for m < M; tb != nullptr also stores the activations for the backward pass
+hipError_t launch_mlp_forward(int M, const unsigned short* X, const MlpDevWeights& w, float* y, const MlpTrainBufs* tb,
+                              hipStream_t st);
+// grad (flat, kMlpParams) of mean_m (y_m - t_m)^2; y receives the forward outputs
+hipError_t launch_mlp_gradient(int M, const unsigned short* X, const float* t, const MlpDevWeights& w,
+                               const MlpTrainBufs& tb, float* y, float* grad, hipStream_t st);
+hipError_t launch_mlp_adam(float* theta, const float* g, float* m1, float* m2, float lr, int step, hipStream_t st);
+// t_i = (q_i - Q_b / n_b) / kMlpChargeUnit (to = 0) or q_i = Q_b / n_b + kMlpChargeUnit y_i (to = 1)
+hipError_t launch_mlp_share(const SysView& s, const unsigned* counts, const double* q, const float* y, float* t,
+                            double* q_out, int to_charges, hipStream_t st);
+
+// ---- k_dynamics.hip ----
+struct BdParams {
+    double L[3];
+    double dt, kT, gamma_t, gamma_r, k_rep;
+};
+// Brownian rigid-body step from bead forces plus steric repulsion (force_tmp: [N]
+// scratch); noise_rigid [6 N_rigid], noise_beads [3 N] (may be null when every bead is
+// in a rigid body).  pos/com are updated in place.
+hipError_t launch_bd_step(const BdParams& p, const SysView& s, double4* pos, double4* com, const double4* force_in,
+                          double4* force_tmp, const double* noise_rigid, const double* noise_beads, hipStream_t st);
 
 // ---- k_nlist.hip ----
 // out = exclusive scan of in[0:n]; sums needs n/1024 + 2 entries; total lands in sums[nblocks].

@@ -216,6 +216,45 @@ __global__ void k_membership(int N, const unsigned* __restrict__ tag, const unsi
     if (i < N) iso[i] = body_of_tag[tag[i]];
 }
 
+__global__ void k_zero_seg(int B, unsigned* counts, double* sums)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    if (counts) counts[b] = 0u;
+    if (sums) sums[b] = 0.0;
+}
+
+__global__ void k_count_beads(int N, const unsigned* __restrict__ iso, unsigned* __restrict__ counts)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) atomicAdd(&counts[iso[i]], 1u);
+}
+
+// sums[b] += v_i - (w ? w_i : 0) over the beads of b
+__global__ void k_seg_sum(int N, const unsigned* __restrict__ iso, const double* __restrict__ v,
+                          const double* __restrict__ w, double* __restrict__ sums)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) unsafeAtomicAdd(&sums[iso[i]], w ? v[i] - w[i] : v[i]);
+}
+
+__global__ void k_project(int N, const unsigned* __restrict__ iso, const double* __restrict__ Q,
+                          const unsigned* __restrict__ counts, const double* __restrict__ sums, double* __restrict__ q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) {
+        const unsigned b = iso[i];
+        q[i] += (Q[b] - sums[b]) / (double)counts[b];
+    }
+}
+
+__global__ void k_seg_mean(int B, const unsigned* __restrict__ counts, const double* __restrict__ sums,
+                           double* __restrict__ out)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < B) out[b] = counts[b] ? sums[b] / (double)counts[b] : 0.0;
+}
+
 __global__ void k_zero_force(int N, double* f)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -299,6 +338,38 @@ hipError_t launch_membership(int N, const unsigned* tag, const unsigned* body_of
 {
     if (N <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_membership, dim3(blocks(N, 256)), dim3(256), 0, st, N, tag, body_of_tag, iso);
+    return hipGetLastError();
+}
+
+hipError_t launch_body_counts(const SysView& s, unsigned* counts, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_zero_seg, dim3(blocks(s.N_isolated, 256)), dim3(256), 0, st, s.N_isolated, counts,
+                       (double*)nullptr);
+    if (s.N > 0) hipLaunchKernelGGL(k_count_beads, dim3(blocks(s.N, 256)), dim3(256), 0, st, s.N, s.iso, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_project_charges(const SysView& s, const unsigned* counts, double* sums, double* q, hipStream_t st)
+{
+    if (s.N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_seg, dim3(blocks(s.N_isolated, 256)), dim3(256), 0, st, s.N_isolated, (unsigned*)nullptr,
+                       sums);
+    hipLaunchKernelGGL(k_seg_sum, dim3(blocks(s.N, 256)), dim3(256), 0, st, s.N, s.iso, (const double*)q,
+                       (const double*)nullptr, sums);
+    hipLaunchKernelGGL(k_project, dim3(blocks(s.N, 256)), dim3(256), 0, st, s.N, s.iso, s.body_charge, counts,
+                       (const double*)sums, q);
+    return hipGetLastError();
+}
+
+hipError_t launch_fit_potentials(const SysView& s, const unsigned* counts, const double* b, const double* y,
+                                 double* sums, double* phi, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_zero_seg, dim3(blocks(s.N_isolated, 256)), dim3(256), 0, st, s.N_isolated, (unsigned*)nullptr,
+                       sums);
+    if (s.N > 0)
+        hipLaunchKernelGGL(k_seg_sum, dim3(blocks(s.N, 256)), dim3(256), 0, st, s.N, s.iso, b, y, sums);
+    hipLaunchKernelGGL(k_seg_mean, dim3(blocks(s.N_isolated, 256)), dim3(256), 0, st, s.N_isolated, counts,
+                       (const double*)sums, phi);
     return hipGetLastError();
 }
 

@@ -41,6 +41,10 @@ extern "C" {
  * cppmodule/ConductorRigid.cu:1125-1127) */
 #define CR_X0_PREVIOUS 0     /* start from the caller's q_inout / phi_inout */
 #define CR_X0_ZERO 1         /* start from zero */
+#define CR_X0_CHARGES 2      /* start from the caller's q_inout (e.g. a learned prediction), shifted per
+                                body onto its net charge, with the body potentials fitted to the
+                                equipotential rows for that q (one extra operator application);
+                                phi_inout is output only.  No reference counterpart. */
 
 typedef struct cr_context cr_context;
 
@@ -200,6 +204,58 @@ int cr_profile_matvec(cr_context* ctx, const cr_system* sys, const double* x, in
 int cr_build_nlist(cr_context* ctx, int N, const double* pos, const double L[3], double rlist,
                    unsigned* head_list, unsigned* n_neigh, unsigned* nlist, long capacity,
                    long* total, void* stream);
+
+/* ---- learned warm start -------------------------------------------------
+ * No reference counterpart: the reference starts each solve from the previous step's
+ * (q, Phi) (cppmodule/ConductorRigid.cu:1125-1127).  A per-bead MLP predicts each bead's
+ * charge relative to its body's uniform share Q_b / n_b, in units of 4 pi; the
+ * prediction enters the solve through CR_X0_CHARGES, so GMRES (to the same tolerance)
+ * still decides the answer.  At fixed geometry the charges are linear in E0, so the
+ * network outputs four geometric responses u = (a_x, a_y, a_z, c) and predicts
+ * y = E0 . a + c.  Trunk: CR_MLP_TRUNK bf16 features -> 128 -> 128 -> 4, SiLU, bf16 weights
+ * with FP32 accumulation on MFMA; FP32 master weights trained with Adam on the device.
+ * Feature row (CR_MLP_IN columns, bf16 bits): the 32 nearest neighbours within r < 3 as
+ * d = (dx, dy, dz)/3, a same-body flag (+1 / -1) and |d| (empty slots 0) [0, 160);
+ * Q_b / n_b [160]; the offset o from the rigid body's centre of mass / 4 and |o|
+ * [161, 165); zero [165, 168); E0 as bf16 pairs hi [168, 171), lo [171, 174); zero.
+ * Parameter vector (FP32, CR_MLP_PARAMS):
+ * W1 [CR_MLP_IN][128] | b1 [128] | W2 [128][128] | b2 [128] | W3 [128][4] | b3 [4]. */
+#define CR_MLP_IN 176
+#define CR_MLP_TRUNK 168
+#define CR_MLP_HIDDEN 128
+#define CR_MLP_OUT 4
+#define CR_MLP_PARAMS 39684
+typedef struct cr_mlp cr_mlp;
+int  cr_mlp_create(cr_mlp** out, int device, unsigned long long seed);
+void cr_mlp_destroy(cr_mlp* m);
+int  cr_mlp_set_params(cr_mlp* m, const float* theta);      /* host [CR_MLP_PARAMS]; resets Adam */
+int  cr_mlp_get_params(const cr_mlp* m, float* theta);      /* host [CR_MLP_PARAMS] */
+/* X: device bf16 bits [N * CR_MLP_IN] for the system's beads (field from the context) */
+int  cr_mlp_features(cr_context* ctx, const cr_system* sys, unsigned short* X, void* stream);
+/* t_i = (q_i - Q_b / n_b) / (4 pi)  (training targets from solved charges; device arrays) */
+int  cr_mlp_targets(cr_context* ctx, const cr_system* sys, const double* q, float* t, void* stream);
+/* q_i = Q_b / n_b + 4 pi y_i  (charges from network outputs; device arrays) */
+int  cr_mlp_charges(cr_context* ctx, const cr_system* sys, const float* y, double* q, void* stream);
+/* y [M] = network(X [M rows]) */
+int  cr_mlp_predict(cr_mlp* m, int M, const unsigned short* X, float* y, void* stream);
+/* grad [CR_MLP_PARAMS] (device FP32) of mean_m (y_m - t_m)^2, y [M] optional */
+int  cr_mlp_gradient(cr_mlp* m, int M, const unsigned short* X, const float* t, float* y, float* grad, void* stream);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss; y [M] optional */
+int  cr_mlp_train_step(cr_mlp* m, int M, const unsigned short* X, const float* t, float lr, float* y, void* stream);
+
+/* ---- Brownian rigid-body step -----------------------------------------------
+ * The force's downstream consumer (HOOMD integrates the bodies in the reference's
+ * setting); used to generate trajectories.  Per rigid body: F = sum of member forces,
+ * T = sum (r_i - R) x F_i; dR = F dt/gamma_t + sqrt(2 kT dt/gamma_t) n, rotation vector
+ * T dt/gamma_r + sqrt(2 kT dt/gamma_r) n' applied about R.  Beads outside rigid bodies
+ * translate.  Bead forces include a soft steric repulsion k_rep (2 - r) r_hat from beads of
+ * other bodies closer than contact (r < 2).
+ * noise_rigid: device [6 N_rigid] standard normals; noise_beads: [3 N] or NULL.
+ * pos [4N] and rigid_com [4 N_rigid] (the arrays sys points to) are updated in place;
+ * the neighbour list must be rebuilt by the caller. */
+int cr_bd_step(cr_context* ctx, const cr_system* sys, double* pos, double* rigid_com, const double* force,
+               const double* noise_rigid, const double* noise_beads, double dt, double kT, double gamma_t,
+               double gamma_r, double k_rep, void* stream);
 
 #ifdef __cplusplus
 }
