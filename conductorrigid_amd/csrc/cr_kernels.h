@@ -137,6 +137,19 @@ template <typename T>
 hipError_t launch_fft_x_scale(const AxisPlan& px, int Nx, int Ny, int Zc, T* cplx, const ScaleTables& sc,
                               hipStream_t st);
 
+// ---- k_fft_reg.hip: register four-step passes for n = R * R, R in {8, 16, 32} ----
+int fftr_radix(int n);  // R, or 0 when n has no fast path
+template <typename T>
+hipError_t launch_fftr_z_r2c(const AxisPlan& pz, long nrows, const T* real, T* cplx, hipStream_t st);
+template <typename T>
+hipError_t launch_fftr_z_c2r(const AxisPlan& pz, long nrows, const T* cplx, T* real, hipStream_t st);
+template <typename T>
+hipError_t launch_fftr_z_c2c(const AxisPlan& pz, long nrows, T* data, int sign, hipStream_t st);
+template <typename T>
+hipError_t launch_fftr_strided(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, T* data, int sign, hipStream_t st);
+template <typename T>
+hipError_t launch_fftr_x_scale(const AxisPlan& px, int Ny, int Zc, T* data, const ScaleTables& sc, hipStream_t st);
+
 // ---- k_real.hip ----
 template <typename T>
 hipError_t launch_real_space(const RealParams& rp, const SysView& s, const double* x, const T* phi_recip,

@@ -376,8 +376,9 @@ template <typename T>
 hipError_t launch_fft_z_r2c(const AxisPlan& pz, int Nx, int Ny, const T* real, T* cplx, hipStream_t st)
 {
     using T2 = typename Cplx<T>::type;
-    if (!fits<T>(pz.n)) return hipErrorInvalidValue;
     const long nrows = (long)Nx * Ny;
+    if (fftr_radix(pz.n)) return launch_fftr_z_r2c<T>(pz, nrows, real, cplx, st);
+    if (!fits<T>(pz.n)) return hipErrorInvalidValue;
     const int cols = pick_cols<T>(pz.n);
     hipLaunchKernelGGL(k_fft_z_r2c<T>, dim3((unsigned)((nrows + cols - 1) / cols)), dim3(kThreads),
                        lds_bytes<T>(pz.n, cols), st, pz, nrows, cols, real, (T2*)cplx);
@@ -388,8 +389,9 @@ template <typename T>
 hipError_t launch_fft_z_c2r(const AxisPlan& pz, int Nx, int Ny, const T* cplx, T* real, hipStream_t st)
 {
     using T2 = typename Cplx<T>::type;
-    if (!fits<T>(pz.n)) return hipErrorInvalidValue;
     const long nrows = (long)Nx * Ny;
+    if (fftr_radix(pz.n)) return launch_fftr_z_c2r<T>(pz, nrows, cplx, real, st);
+    if (!fits<T>(pz.n)) return hipErrorInvalidValue;
     const int cols = pick_cols<T>(pz.n);
     hipLaunchKernelGGL(k_fft_z_c2r<T>, dim3((unsigned)((nrows + cols - 1) / cols)), dim3(kThreads),
                        lds_bytes<T>(pz.n, cols), st, pz, nrows, cols, (const T2*)cplx, real);
@@ -400,8 +402,9 @@ template <typename T>
 hipError_t launch_fft_z_c2c(const AxisPlan& pz, int Nx, int Ny, T* cplx, int sign, hipStream_t st)
 {
     using T2 = typename Cplx<T>::type;
-    if (!fits<T>(pz.n)) return hipErrorInvalidValue;
     const long nrows = (long)Nx * Ny;
+    if (fftr_radix(pz.n)) return launch_fftr_z_c2c<T>(pz, nrows, cplx, sign, st);
+    if (!fits<T>(pz.n)) return hipErrorInvalidValue;
     const int cols = pick_cols<T>(pz.n);
     hipLaunchKernelGGL(k_fft_z_c2c<T>, dim3((unsigned)((nrows + cols - 1) / cols)), dim3(kThreads),
                        lds_bytes<T>(pz.n, cols), st, pz, nrows, cols, (T2*)cplx, sign);
@@ -412,6 +415,7 @@ template <typename T>
 hipError_t launch_fft_yx(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, T* cplx, int sign, hipStream_t st)
 {
     using T2 = typename Cplx<T>::type;
+    if (fftr_radix(pa.n)) return launch_fftr_strided<T>(pa, axis, Nx, Ny, Zc, cplx, sign, st);
     if (!fits<T>(pa.n)) return hipErrorInvalidValue;
     const int cols = pick_cols<T>(pa.n);
     const int lines = (axis == 1) ? Nx : Ny;
@@ -427,6 +431,7 @@ hipError_t launch_fft_x_scale(const AxisPlan& px, int Nx, int Ny, int Zc, T* cpl
 {
     using T2 = typename Cplx<T>::type;
     (void)Nx;
+    if (fftr_radix(px.n)) return launch_fftr_x_scale<T>(px, Ny, Zc, cplx, sc, st);
     if (!fits<T>(px.n)) return hipErrorInvalidValue;
     const int cols = pick_cols<T>(px.n);
     const int zblocks = (Zc + cols - 1) / cols;

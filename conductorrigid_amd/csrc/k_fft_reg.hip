@@ -1,0 +1,507 @@
+// k_fft_reg.hip -- register-resident FFT passes for square pencil lengths n = R * R
+// (R = 8, 16, 32: meshes of 64, 256 or 1024 nodes per axis).
+//
+// Four-step decomposition (Bailey): with j = j2 + R j1 and k = k1 + R k2,
+//   X[k1 + R k2] = sum_j2 W_R^{j2 k2} [ W_n^{j2 k1} sum_j1 x[j2 + R j1] W_R^{j1 k1} ].
+// R threads own one pencil.  Thread j2 loads x[j2 + R j1] (j1 = 0..R-1; for a fixed j1
+// the R threads read R consecutive elements), runs an R-point DFT in registers with
+// compile-time roots, multiplies by the pass twiddle W_n^{j2 k1}, and hands the values
+// over through one padded LDS transpose; thread k1 then runs the second R-point DFT
+// and holds X[k1 + R k2].  One barrier per pass instead of two per radix stage.
+// The inverse from that layout (used by the C2R pass and the fused x pass) runs the
+// same steps in the other order, ending with thread b holding x[b + R a].
+//
+// Unnormalised, forward e^{-ikx}, inverse e^{+ikx}, as the generic passes in k_fft.hip
+// (which replace the reference's cuFFT plan, cppmodule/ConductorRigid.cc:228).
+#include "cr_kernels.h"
+
+namespace cr {
+
+namespace {
+
+template <typename T>
+struct Cx;
+template <>
+struct Cx<float> { using t = float2; };
+template <>
+struct Cx<double> { using t = double2; };
+
+// cos(2 pi k / 32) for any integer k: cos(pi a / 16), a = 0..8, by symmetry
+__device__ __forceinline__ double cos32(int k)
+{
+    constexpr double c[9] = {1.0,
+                             0.98078528040323044913,
+                             0.92387953251128675613,
+                             0.83146961230254523708,
+                             0.70710678118654752440,
+                             0.55557023301960222474,
+                             0.38268343236508977173,
+                             0.19509032201612826785,
+                             0.0};
+    k &= 31;
+    const int a = k <= 16 ? k : 32 - k;
+    return a <= 8 ? c[a] : -c[16 - a];
+}
+__device__ __forceinline__ double sin32(int k) { return cos32(k - 8); }
+
+template <typename T2>
+__device__ __forceinline__ T2 cmul(T2 a, T2 b)
+{
+    T2 r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+
+template <int R>
+__device__ __forceinline__ constexpr int bitrev(int i)
+{
+    int r = 0;
+    for (int b = 1; b < R; b <<= 1) {
+        r = (r << 1) | (i & 1);
+        i >>= 1;
+    }
+    return r;
+}
+
+// In-register R-point DFT, natural order in and out, root e^{SIGN 2 pi i / R}
+// (radix-2 decimation in time; roots are compile-time constants after unrolling).
+template <int R, int SIGN, typename T2>
+__device__ __forceinline__ void dft(T2 (&v)[R])
+{
+    using T = decltype(v[0].x);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int j = bitrev<R>(i);
+        if (j > i) {
+            const T2 t = v[i];
+            v[i] = v[j];
+            v[j] = t;
+        }
+    }
+#pragma unroll
+    for (int len = 2; len <= R; len <<= 1) {
+#pragma unroll
+        for (int i = 0; i < R; i += len) {
+#pragma unroll
+            for (int k = 0; k < len / 2; ++k) {
+                const int m = k * (32 / len);
+                const T2 a = v[i + k];
+                T2 b = v[i + k + len / 2];
+                if (m == 8) {  // root = SIGN * i
+                    const T bx = b.x;
+                    b.x = -(T)SIGN * b.y;
+                    b.y = (T)SIGN * bx;
+                } else if (m != 0) {
+                    T2 w;
+                    w.x = (T)cos32(m);
+                    w.y = (T)(SIGN * sin32(m));
+                    b = cmul(b, w);
+                }
+                v[i + k].x = a.x + b.x;
+                v[i + k].y = a.y + b.y;
+                v[i + k + len / 2].x = a.x - b.x;
+                v[i + k + len / 2].y = a.y - b.y;
+            }
+        }
+    }
+}
+
+// pass twiddle W_n^{e} (table holds e^{-2 pi i e / n}); conjugated for SIGN = +1
+template <int SIGN, typename T2>
+__device__ __forceinline__ T2 ptw(const T2* __restrict__ tw, int e)
+{
+    T2 w = tw[e];
+    if (SIGN > 0) w.y = -w.y;
+    return w;
+}
+
+// Pencils (or columns) per workgroup: as many as fit ~40 KB of transpose buffer.
+template <typename T, int R>
+constexpr int lanes_per_wg()
+{
+    constexpr int pad = R + 1;
+    constexpr int fit = 40960 / (int)(R * pad * 2 * sizeof(T));
+    constexpr int cap = 256 / R;
+    int p = 1;
+    while (p * 2 <= fit && p * 2 <= cap) p *= 2;
+    return p;
+}
+
+// ---- contiguous pencils (z) ---------------------------------------------------
+// Forward steps from x layout (thread j2 holds x[j2 + R j1]) to X layout (thread k1
+// holds X[k1 + R k2]).  buf: this pencil's R x (R+1) transpose buffer.
+template <int R, int SIGN, typename T2>
+__device__ __forceinline__ void four_step(T2 (&v)[R], T2* buf, const T2* __restrict__ tw, int j2, int k1)
+{
+    dft<R, SIGN>(v);
+#pragma unroll
+    for (int q = 0; q < R; ++q) buf[q * (R + 1) + j2] = q == 0 ? v[q] : cmul(v[q], ptw<SIGN>(tw, j2 * q));
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = buf[k1 * (R + 1) + q];
+    dft<R, SIGN>(v);
+}
+
+// Inverse steps from X layout (thread k1 holds X[k1 + R k2]) to x layout (thread b
+// holds x[b + R a]).
+template <int R, typename T2>
+__device__ __forceinline__ void four_step_inv_from_X(T2 (&v)[R], T2* buf, const T2* __restrict__ tw, int k1, int b)
+{
+    dft<R, +1>(v);
+#pragma unroll
+    for (int q = 0; q < R; ++q) buf[q * (R + 1) + k1] = q == 0 ? v[q] : cmul(v[q], ptw<+1>(tw, q * k1));
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = buf[b * (R + 1) + q];
+    dft<R, +1>(v);
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__((R * lanes_per_wg<T, R>())) void k_fftr_z_r2c(long nrows, const T* __restrict__ real,
+                                                                         typename Cx<T>::t* __restrict__ cplx,
+                                                                         const typename Cx<T>::t* __restrict__ tw)
+{
+    using T2 = typename Cx<T>::t;
+    constexpr int n = R * R, nh = n / 2 + 1;
+    __shared__ T2 lds[lanes_per_wg<T, R>() * R * (R + 1)];
+    const int p = threadIdx.x / R, l = threadIdx.x % R;
+    const long row = (long)blockIdx.x * lanes_per_wg<T, R>() + p;
+    const bool ok = row < nrows;
+    T2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        v[q].x = ok ? real[row * n + l + R * q] : (T)0;
+        v[q].y = 0;
+    }
+    four_step<R, -1>(v, lds + p * R * (R + 1), tw, l, l);
+    if (!ok) return;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int k = l + R * q;
+        if (k < nh) cplx[row * nh + k] = v[q];
+    }
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__((R * lanes_per_wg<T, R>())) void k_fftr_z_c2r(long nrows,
+                                                                         const typename Cx<T>::t* __restrict__ cplx,
+                                                                         T* __restrict__ real,
+                                                                         const typename Cx<T>::t* __restrict__ tw)
+{
+    using T2 = typename Cx<T>::t;
+    constexpr int n = R * R, nh = n / 2 + 1;
+    __shared__ T2 lds[lanes_per_wg<T, R>() * R * (R + 1)];
+    const int p = threadIdx.x / R, l = threadIdx.x % R;
+    const long row = (long)blockIdx.x * lanes_per_wg<T, R>() + p;
+    const bool ok = row < nrows;
+    T2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int k = l + R * q;
+        T2 z;
+        z.x = 0;
+        z.y = 0;
+        if (ok) {
+            if (k < nh) {
+                z = cplx[row * nh + k];
+            } else {  // Hermitian extension
+                z = cplx[row * nh + (n - k)];
+                z.y = -z.y;
+            }
+        }
+        v[q] = z;
+    }
+    four_step_inv_from_X<R>(v, lds + p * R * (R + 1), tw, l, l);
+    if (!ok) return;
+#pragma unroll
+    for (int q = 0; q < R; ++q) real[row * n + l + R * q] = v[q].x;
+}
+
+template <typename T, int R, int SIGN>
+__global__ __launch_bounds__((R * lanes_per_wg<T, R>())) void k_fftr_z_c2c(long nrows,
+                                                                         typename Cx<T>::t* __restrict__ data,
+                                                                         const typename Cx<T>::t* __restrict__ tw)
+{
+    using T2 = typename Cx<T>::t;
+    constexpr int n = R * R;
+    __shared__ T2 lds[lanes_per_wg<T, R>() * R * (R + 1)];
+    const int p = threadIdx.x / R, l = threadIdx.x % R;
+    const long row = (long)blockIdx.x * lanes_per_wg<T, R>() + p;
+    const bool ok = row < nrows;
+    T2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        T2 z;
+        z.x = 0;
+        z.y = 0;
+        if (ok) z = data[row * n + l + R * q];
+        v[q] = z;
+    }
+    four_step<R, SIGN>(v, lds + p * R * (R + 1), tw, l, l);
+    if (!ok) return;
+#pragma unroll
+    for (int q = 0; q < R; ++q) data[row * n + l + R * q] = v[q];
+}
+
+// ---- strided pencils (y: axis 1, x: axis 0) of [Nx][Ny][Zc] ---------------------
+// A workgroup takes C neighbouring z columns of one line; thread t handles column
+// c = t % C (consecutive lanes read consecutive columns) and pencil position t / C.
+// Transpose buffer layout [q][l][c] keeps both phases free of bank conflicts.
+struct Lines {
+    long base, stride;
+    int z0, line;
+};
+
+template <int C>
+__device__ __forceinline__ Lines lines(int axis, int Ny, int Zc)
+{
+    Lines L;
+    const int zb = (Zc + C - 1) / C;
+    L.line = blockIdx.x / zb;
+    L.z0 = (blockIdx.x - L.line * zb) * C;
+    if (axis == 1) {
+        L.base = (long)L.line * Ny * Zc + L.z0;
+        L.stride = Zc;
+    } else {
+        L.base = (long)L.line * Zc + L.z0;
+        L.stride = (long)Ny * Zc;
+    }
+    return L;
+}
+
+template <int R, int C, int SIGN, typename T2>
+__device__ __forceinline__ void four_step_cols(T2 (&v)[R], T2* lds, const T2* __restrict__ tw, int c, int l)
+{
+    dft<R, SIGN>(v);
+#pragma unroll
+    for (int q = 0; q < R; ++q) lds[(q * R + l) * C + c] = q == 0 ? v[q] : cmul(v[q], ptw<SIGN>(tw, l * q));
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = lds[(l * R + q) * C + c];
+    dft<R, SIGN>(v);
+}
+
+template <int R, int C, typename T2>
+__device__ __forceinline__ void four_step_cols_inv_from_X(T2 (&v)[R], T2* lds, const T2* __restrict__ tw, int c, int l)
+{
+    dft<R, +1>(v);
+#pragma unroll
+    for (int q = 0; q < R; ++q) lds[(q * R + l) * C + c] = q == 0 ? v[q] : cmul(v[q], ptw<+1>(tw, q * l));
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = lds[(l * R + q) * C + c];
+    dft<R, +1>(v);
+}
+
+template <typename T, int R, int SIGN>
+__global__ __launch_bounds__((R * lanes_per_wg<T, R>())) void k_fftr_strided(int axis, int Ny, int Zc,
+                                                                           typename Cx<T>::t* __restrict__ data,
+                                                                           const typename Cx<T>::t* __restrict__ tw)
+{
+    using T2 = typename Cx<T>::t;
+    constexpr int C = lanes_per_wg<T, R>();
+    __shared__ T2 lds[C * R * R];
+    const Lines L = lines<C>(axis, Ny, Zc);
+    const int c = threadIdx.x % C, l = threadIdx.x / C;
+    const bool ok = L.z0 + c < Zc;
+    T2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        T2 z;
+        z.x = 0;
+        z.y = 0;
+        if (ok) z = data[L.base + c + (long)(l + R * q) * L.stride];
+        v[q] = z;
+    }
+    four_step_cols<R, C, SIGN>(v, lds, tw, c, l);
+    if (!ok) return;
+#pragma unroll
+    for (int q = 0; q < R; ++q) data[L.base + c + (long)(l + R * q) * L.stride] = v[q];
+}
+
+template <typename T>
+__device__ __forceinline__ T sin_t(T x);
+template <>
+__device__ __forceinline__ float sin_t<float>(float x) { return sinf(x); }
+template <>
+__device__ __forceinline__ double sin_t<double>(double x) { return sin(x); }
+
+// Forward x FFT, S(k) = j0(|k|)^2 exp(-sum_d (1-eta_d) k_d^2/4xi^2)/|k|^2/G (S(0) = 0;
+// cppmodule/ConductorRigid.cc:253-268), inverse x FFT, on the half spectrum.
+template <typename T, int R>
+__global__ __launch_bounds__((R * lanes_per_wg<T, R>())) void k_fftr_x_scale(int Ny, int Zc,
+                                                                           typename Cx<T>::t* __restrict__ data,
+                                                                           const typename Cx<T>::t* __restrict__ tw,
+                                                                           ScaleTables sc)
+{
+    using T2 = typename Cx<T>::t;
+    constexpr int C = lanes_per_wg<T, R>();
+    __shared__ T2 lds[C * R * R];
+    const Lines L = lines<C>(0, Ny, Zc);
+    const int c = threadIdx.x % C, l = threadIdx.x / C;
+    const int z = L.z0 + c;
+    const bool ok = z < Zc;
+    T2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        T2 u;
+        u.x = 0;
+        u.y = 0;
+        if (ok) u = data[L.base + c + (long)(l + R * q) * L.stride];
+        v[q] = u;
+    }
+    four_step_cols<R, C, -1>(v, lds, tw, c, l);
+    // thread (c, l) holds X[l + R q]: scale by S(kx, ky, kz)
+    const double ky = sc.k[1][L.line], ey = sc.e[1][L.line];
+    const double kz = ok ? sc.k[2][z] : 0.0, ez = ok ? sc.e[2][z] : 0.0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int x = l + R * q;
+        const T kx = (T)sc.k[0][x], kyt = (T)ky, kzt = (T)kz;
+        const T k2 = kx * kx + kyt * kyt + kzt * kzt;
+        T S = 0;
+        if (k2 > (T)0) {
+            const T kk = sqrt(k2);
+            const T j0 = sin_t<T>(kk) / kk;
+            S = j0 * j0 * (T)(sc.e[0][x] * ey * ez) / k2 * (T)sc.invG;
+        }
+        v[q].x *= S;
+        v[q].y *= S;
+    }
+    __syncthreads();  // the transpose buffer is reused
+    four_step_cols_inv_from_X<R, C>(v, lds, tw, c, l);
+    if (!ok) return;
+#pragma unroll
+    for (int q = 0; q < R; ++q) data[L.base + c + (long)(l + R * q) * L.stride] = v[q];
+}
+
+inline unsigned ublk(long n, int per) { return (unsigned)((n + per - 1) / per); }
+
+template <typename T, int R>
+hipError_t z_r2c(long nrows, const T* real, T* cplx, const void* tw, hipStream_t st)
+{
+    using T2 = typename Cx<T>::t;
+    constexpr int P = lanes_per_wg<T, R>();
+    hipLaunchKernelGGL((k_fftr_z_r2c<T, R>), dim3(ublk(nrows, P)), dim3(R * P), 0, st, nrows, real, (T2*)cplx,
+                       (const T2*)tw);
+    return hipGetLastError();
+}
+
+template <typename T, int R>
+hipError_t z_c2r(long nrows, const T* cplx, T* real, const void* tw, hipStream_t st)
+{
+    using T2 = typename Cx<T>::t;
+    constexpr int P = lanes_per_wg<T, R>();
+    hipLaunchKernelGGL((k_fftr_z_c2r<T, R>), dim3(ublk(nrows, P)), dim3(R * P), 0, st, nrows, (const T2*)cplx, real,
+                       (const T2*)tw);
+    return hipGetLastError();
+}
+
+template <typename T, int R>
+hipError_t z_c2c(long nrows, T* data, int sign, const void* tw, hipStream_t st)
+{
+    using T2 = typename Cx<T>::t;
+    constexpr int P = lanes_per_wg<T, R>();
+    if (sign < 0)
+        hipLaunchKernelGGL((k_fftr_z_c2c<T, R, -1>), dim3(ublk(nrows, P)), dim3(R * P), 0, st, nrows, (T2*)data,
+                           (const T2*)tw);
+    else
+        hipLaunchKernelGGL((k_fftr_z_c2c<T, R, +1>), dim3(ublk(nrows, P)), dim3(R * P), 0, st, nrows, (T2*)data,
+                           (const T2*)tw);
+    return hipGetLastError();
+}
+
+template <typename T, int R>
+hipError_t strided(int axis, int Nx, int Ny, int Zc, T* data, int sign, const void* tw, hipStream_t st)
+{
+    using T2 = typename Cx<T>::t;
+    constexpr int C = lanes_per_wg<T, R>();
+    const int lines_n = axis == 1 ? Nx : Ny;
+    const unsigned blocks = (unsigned)(lines_n * ((Zc + C - 1) / C));
+    if (sign < 0)
+        hipLaunchKernelGGL((k_fftr_strided<T, R, -1>), dim3(blocks), dim3(R * C), 0, st, axis, Ny, Zc, (T2*)data,
+                           (const T2*)tw);
+    else
+        hipLaunchKernelGGL((k_fftr_strided<T, R, +1>), dim3(blocks), dim3(R * C), 0, st, axis, Ny, Zc, (T2*)data,
+                           (const T2*)tw);
+    return hipGetLastError();
+}
+
+template <typename T, int R>
+hipError_t x_scale(int Ny, int Zc, T* data, const void* tw, const ScaleTables& sc, hipStream_t st)
+{
+    using T2 = typename Cx<T>::t;
+    constexpr int C = lanes_per_wg<T, R>();
+    const unsigned blocks = (unsigned)(Ny * ((Zc + C - 1) / C));
+    hipLaunchKernelGGL((k_fftr_x_scale<T, R>), dim3(blocks), dim3(R * C), 0, st, Ny, Zc, (T2*)data, (const T2*)tw, sc);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int fftr_radix(int n)
+{
+    for (int R : {8, 16, 32})
+        if (R * R == n) return R;
+    return 0;
+}
+
+#define CR_FFTR_DISPATCH(n, CALL)                       \
+    switch (fftr_radix(n)) {                            \
+        case 8: return CALL(8);                         \
+        case 16: return CALL(16);                       \
+        case 32: return CALL(32);                       \
+        default: return hipErrorInvalidValue;           \
+    }
+
+template <typename T>
+hipError_t launch_fftr_z_r2c(const AxisPlan& pz, long nrows, const T* real, T* cplx, hipStream_t st)
+{
+#define CALL(R) z_r2c<T, R>(nrows, real, cplx, pz.tw, st)
+    CR_FFTR_DISPATCH(pz.n, CALL)
+#undef CALL
+}
+
+template <typename T>
+hipError_t launch_fftr_z_c2r(const AxisPlan& pz, long nrows, const T* cplx, T* real, hipStream_t st)
+{
+#define CALL(R) z_c2r<T, R>(nrows, cplx, real, pz.tw, st)
+    CR_FFTR_DISPATCH(pz.n, CALL)
+#undef CALL
+}
+
+template <typename T>
+hipError_t launch_fftr_z_c2c(const AxisPlan& pz, long nrows, T* data, int sign, hipStream_t st)
+{
+#define CALL(R) z_c2c<T, R>(nrows, data, sign, pz.tw, st)
+    CR_FFTR_DISPATCH(pz.n, CALL)
+#undef CALL
+}
+
+template <typename T>
+hipError_t launch_fftr_strided(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, T* data, int sign, hipStream_t st)
+{
+#define CALL(R) strided<T, R>(axis, Nx, Ny, Zc, data, sign, pa.tw, st)
+    CR_FFTR_DISPATCH(pa.n, CALL)
+#undef CALL
+}
+
+template <typename T>
+hipError_t launch_fftr_x_scale(const AxisPlan& px, int Ny, int Zc, T* data, const ScaleTables& sc, hipStream_t st)
+{
+#define CALL(R) x_scale<T, R>(Ny, Zc, data, px.tw, sc, st)
+    CR_FFTR_DISPATCH(px.n, CALL)
+#undef CALL
+}
+
+#define CR_FFTR_INST(T)                                                                                          \
+    template hipError_t launch_fftr_z_r2c<T>(const AxisPlan&, long, const T*, T*, hipStream_t);                \
+    template hipError_t launch_fftr_z_c2r<T>(const AxisPlan&, long, const T*, T*, hipStream_t);                \
+    template hipError_t launch_fftr_z_c2c<T>(const AxisPlan&, long, T*, int, hipStream_t);                     \
+    template hipError_t launch_fftr_strided<T>(const AxisPlan&, int, int, int, int, T*, int, hipStream_t);     \
+    template hipError_t launch_fftr_x_scale<T>(const AxisPlan&, int, int, T*, const ScaleTables&, hipStream_t);
+CR_FFTR_INST(float)
+CR_FFTR_INST(double)
+
+}  // namespace cr
