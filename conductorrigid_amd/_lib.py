@@ -25,8 +25,8 @@ CR_MLP_OUT = 4
 CR_MLP_PARAMS = 39684
 CR_PRECOND_DIAGONAL = 0
 CR_PRECOND_NEARFIELD_BLOCK = 1
-CR_NUM_STAGES = 11
-STAGE_NAMES = ["mesh_clear", "spread", "spread_halo", "fft_z_r2c", "fft_y_fwd", "fft_x_scale", "fft_y_inv", "fft_z_c2r",
+CR_NUM_STAGES = 10
+STAGE_NAMES = ["mesh_clear", "spread", "fft_z_r2c", "fft_y_fwd", "fft_x_scale", "fft_y_inv", "fft_z_c2r",
                "contract", "real_space", "constraint_rows"]
 
 vp = C.c_void_p

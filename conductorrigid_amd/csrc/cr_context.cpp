@@ -117,7 +117,7 @@ struct cr_context {
     DevBuf rs_col, rs_val, rs_len;
     // bead binning for the owner-computes spread
     TileGeom tg{};
-    DevBuf bead_tile, tile_count, tile_start, tile_cursor, sorted, scan_sums, halo;
+    DevBuf bead_tile, tile_count, tile_start, tile_cursor, sorted, scan_sums;
     DevBuf rec_off, rec_w;  // per-bead window records in binned order
     // Krylov workspace
     int R = 10;
@@ -260,7 +260,7 @@ bool contract_tiled_ok(const cr_context* c, size_t elem)
     return c->tg.valid && tiled_lds(c->mp, c->tg, elem, elem).chunk >= 4;
 }
 
-// phi <- contraction of the context mesh (binned LDS version when the tile + halo fits)
+// phi <- contraction of the context mesh (binned LDS version when the extended tile fits)
 template <typename T>
 int contract_impl(cr_context* c, const SysView& v, T* phi, hipStream_t st)
 {
@@ -310,13 +310,13 @@ int ensure_csr(cr_context* c, const SysView& v, hipStream_t st)
     return CR_OK;
 }
 
-// the spread kernel alone (the atomic fallback expects a cleared mesh)
+// the spread kernel alone on a mesh the caller has cleared
 template <typename T>
-int spread_only(cr_context* c, const SysView& v, const double* q, hipStream_t st, int parts = 3)
+int spread_only(cr_context* c, const SysView& v, const double* q, hipStream_t st)
 {
     if (c->tg.valid)
-        CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, bin_view(c), q, c->grid_real.as<T>(), c->halo.as<T>(), st, parts));
-    else if (parts & 1)
+        CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, bin_view(c), q, c->grid_real.as<T>(), st, false));
+    else
         CR_HIP(launch_spread<T>(c->mp, v, q, c->grid_real.as<T>(), st));
     return CR_OK;
 }
@@ -325,7 +325,7 @@ template <typename T>
 int spread_impl(cr_context* c, const SysView& v, const double* q, hipStream_t st)
 {
     if (c->tg.valid) {
-        CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, bin_view(c), q, c->grid_real.as<T>(), c->halo.as<T>(), st));
+        CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, bin_view(c), q, c->grid_real.as<T>(), st));
     } else {
         CR_HIP(launch_fill<T>(c->grid_real.as<T>(), (T)0, c->G, st));
         CR_HIP(launch_spread<T>(c->mp, v, q, c->grid_real.as<T>(), st));
@@ -699,7 +699,7 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
     // binned spread/contract tiling: per axis the largest divisor of the mesh size in
     // [max(P-1, 4), 16], else the smallest one in (16, 32]; the tile extended by P-1
     // nodes per axis, held in the spread's double accumulator, must fit 3/4 of the
-    // 64 KiB LDS budget, shrinking z, then y, then x if needed
+    // 64 KiB LDS budget, shrinking x, then y, then z if needed
     const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
     {
         TileGeom& tg = c->tg;
@@ -730,12 +730,14 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
         };
         // leave about a quarter of the LDS budget for the per-bead staging tables
         const size_t ext_cap = kTiledLdsBudget * 3 / 4;
-        for (int d = 2; tg.valid && ext_bytes() > ext_cap && d >= 0;) {
+        // shrink x first: z stays the longest edge, so the tile's mesh rows are long
+        // contiguous runs for the write-out and the extended-tile load
+        for (int d = 0; tg.valid && ext_bytes() > ext_cap && d <= 2;) {
             if (opts[d].size() > 1) {
                 opts[d].erase(opts[d].begin());
                 tg.T[d] = opts[d][0];
             } else {
-                --d;
+                ++d;
             }
         }
         if (ext_bytes() > ext_cap) tg.valid = 0;
@@ -759,7 +761,6 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
             tg.nt[d] = Nd[d] / tg.T[d];
             tg.ntiles *= tg.nt[d];
         }
-        if (tg.valid) CR_HIP(c->halo.ensure(ext_bytes() / sizeof(SpreadAcc) * Tsz * (size_t)tg.ntiles));
     }
     CR_HIP(c->grid_real.ensure(Tsz * (size_t)c->G));
     CR_HIP(c->grid_cplx.ensure(2 * Tsz * (size_t)c->Gh));
@@ -1027,24 +1028,22 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
             const AxisPlan* pl = plans<T>(c);
             T* real = c->grid_real.as<T>();
             T* cp = c->grid_cplx.as<T>();
-            if (!c->tg.valid) CR_HIP(launch_fill<T>(real, (T)0, c->G, st));
+            CR_HIP(launch_fill<T>(real, (T)0, c->G, st));
             CR_HIP(hipEventRecord(ev[1], st));
-            CR_RET(spread_only<T>(c, v, x, st, 1));
+            CR_RET(spread_only<T>(c, v, x, st));
             CR_HIP(hipEventRecord(ev[2], st));
-            CR_RET(spread_only<T>(c, v, x, st, 2));
-            CR_HIP(hipEventRecord(ev[3], st));
             CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st));
-            CR_HIP(hipEventRecord(ev[4], st));
+            CR_HIP(hipEventRecord(ev[3], st));
             CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, -1, st));
-            CR_HIP(hipEventRecord(ev[5], st));
+            CR_HIP(hipEventRecord(ev[4], st));
             CR_HIP(launch_fft_x_scale<T>(pl[0], c->mp.Nx, c->mp.Ny, c->Nzh, cp, c->sc, st));
-            CR_HIP(hipEventRecord(ev[6], st));
+            CR_HIP(hipEventRecord(ev[5], st));
             CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, +1, st));
-            CR_HIP(hipEventRecord(ev[7], st));
+            CR_HIP(hipEventRecord(ev[6], st));
             CR_HIP(launch_fft_z_c2r<T>(pl[2], c->mp.Nx, c->mp.Ny, cp, real, st));
-            CR_HIP(hipEventRecord(ev[8], st));
+            CR_HIP(hipEventRecord(ev[7], st));
             CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
-            CR_HIP(hipEventRecord(ev[9], st));
+            CR_HIP(hipEventRecord(ev[8], st));
             CR_HIP(launch_real_space_csr<T>(c->rp, v, x, c->phi_recip.as<T>(), c->rs_ldn, c->rs_col.as<unsigned>(),
                                             c->rs_val.as<T>(), c->rs_len.as<unsigned>(), y, 1, st));
         } else {
@@ -1052,31 +1051,29 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
             const AxisPlan* pl = plans<T>(c);
             T* real = c->grid_real.as<T>();
             T* cp = c->grid_cplx.as<T>();
-            if (!c->tg.valid) CR_HIP(launch_fill<T>(real, (T)0, c->G, st));
+            CR_HIP(launch_fill<T>(real, (T)0, c->G, st));
             CR_HIP(hipEventRecord(ev[1], st));
-            CR_RET(spread_only<T>(c, v, x, st, 1));
+            CR_RET(spread_only<T>(c, v, x, st));
             CR_HIP(hipEventRecord(ev[2], st));
-            CR_RET(spread_only<T>(c, v, x, st, 2));
-            CR_HIP(hipEventRecord(ev[3], st));
             CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st));
-            CR_HIP(hipEventRecord(ev[4], st));
+            CR_HIP(hipEventRecord(ev[3], st));
             CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, -1, st));
-            CR_HIP(hipEventRecord(ev[5], st));
+            CR_HIP(hipEventRecord(ev[4], st));
             CR_HIP(launch_fft_x_scale<T>(pl[0], c->mp.Nx, c->mp.Ny, c->Nzh, cp, c->sc, st));
-            CR_HIP(hipEventRecord(ev[6], st));
+            CR_HIP(hipEventRecord(ev[5], st));
             CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, +1, st));
-            CR_HIP(hipEventRecord(ev[7], st));
+            CR_HIP(hipEventRecord(ev[6], st));
             CR_HIP(launch_fft_z_c2r<T>(pl[2], c->mp.Nx, c->mp.Ny, cp, real, st));
-            CR_HIP(hipEventRecord(ev[8], st));
+            CR_HIP(hipEventRecord(ev[7], st));
             CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
-            CR_HIP(hipEventRecord(ev[9], st));
+            CR_HIP(hipEventRecord(ev[8], st));
             CR_HIP(launch_real_space_csr<T>(c->rp, v, x, c->phi_recip.as<T>(), c->rs_ldn, c->rs_col.as<unsigned>(),
                                             c->rs_val.as<T>(), c->rs_len.as<unsigned>(), y, 1, st));
         }
-        CR_HIP(hipEventRecord(ev[10], st));
+        CR_HIP(hipEventRecord(ev[9], st));
         CR_HIP(launch_constraint_rows(v, x, y + v.N, 1, st));
-        CR_HIP(hipEventRecord(ev[11], st));
-        CR_HIP(hipEventSynchronize(ev[11]));
+        CR_HIP(hipEventRecord(ev[10], st));
+        CR_HIP(hipEventSynchronize(ev[10]));
         for (int k = 0; k < nst; ++k) {
             float ms = 0;
             CR_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));

@@ -83,15 +83,16 @@ struct BinView {
 template <typename T>
 hipError_t launch_bead_records(const MeshParams& mp, const TileGeom& tg, const SysView& s, const unsigned* sorted,
                                const unsigned* bead_tile, int* rec_off, T* rec_w, hipStream_t st);
-// halo: per-tile scratch of (T+P-1)^3 elements per tile (the +side extensions).
-// parts: bit 0 launches the tile kernel, bit 1 the halo add (3 = the whole spread).
+// grid = spread of q; clear = false leaves the mesh clear to the caller (it must be zero)
 template <typename T>
 hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const BinView& bv, const double* q, T* grid,
-                               T* halo, hipStream_t st, int parts = 3);
+                               hipStream_t st, bool clear = true);
 // LDS plan of the binned kernels: the tile extended by P-1 nodes per axis (ext_elem bytes
 // per node) plus per-bead staging tables (stage_elem bytes per value) for `chunk` beads,
 // within kTiledLdsBudget bytes.  The spread accumulates its tile in SpreadAcc.
 constexpr size_t kTiledLdsBudget = 64 * 1024;
+constexpr int kTiledThreads = 512;   // workgroup of the binned kernels
+constexpr int kTiledChunk = 64;      // beads staged per round (bins hold ~60 at 1M / 256^3)
 using SpreadAcc = double;
 struct TiledLds {
     size_t ext_bytes;
@@ -99,7 +100,7 @@ struct TiledLds {
     int chunk;
 };
 TiledLds tiled_lds(const MeshParams& mp, const TileGeom& tg, size_t ext_elem, size_t stage_elem);
-// Binned contraction from an LDS copy of the tile and its +side halo: potential (force == nullptr)
+// Binned contraction from an LDS copy of the tile extended by P-1 nodes on the + side: potential (force == nullptr)
 // or reciprocal force (phi unused).
 template <typename T>
 hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const SysView& s, const BinView& bv,

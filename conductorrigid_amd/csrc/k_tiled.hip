@@ -11,12 +11,14 @@
 // 3P separable axis weights of every bead are evaluated by all threads into an LDS
 // table, and then each wavefront takes one bead at a time, lanes over the (y, z)
 // window plane, walking x -- every operand of the inner loop comes from LDS.
-//   spread:   window products are added into an LDS copy of the extended tile; the
-//             interior is stored straight to the mesh (each node has one owner), the
-//             extension to a per-tile scratch block added in by k_spread_halo_add.
+//   spread:   window products are added into an LDS copy of the extended tile, which
+//             is then added into the (cleared) mesh with one global atomic per node:
+//             each node receives the sums of the at most 8 tiles whose extension covers
+//             it.  Measured against storing the interior and gathering the extensions
+//             from per-tile scratch in a second pass, this moves fewer bytes.
 //   contract: the extended tile is read into LDS once; each wavefront reduces its
 //             bead's window across lanes.
-// No global atomics and no mesh clear.
+// Per bead, all node updates stay in LDS.
 #include "se_device.h"
 
 namespace cr {
@@ -236,12 +238,11 @@ struct BoxWalk {
 // The LDS tile accumulates in A = double for both mesh precisions: ds_add_f64 measures
 // several times the throughput of ds_add_f32 on gfx950, and the sums gain accuracy.
 template <typename T, typename A, int PT>
-__global__ __launch_bounds__(256) void k_spread_tiled(MeshParams mp, TileGeom tg, int ch,
+__global__ __launch_bounds__(kTiledThreads) void k_spread_tiled(MeshParams mp, TileGeom tg, int ch,
                                                       const unsigned* __restrict__ tstart,
                                                       const unsigned* __restrict__ sorted,
                                                       const int* __restrict__ rec_off, const T* __restrict__ rec_w,
-                                                      const double* __restrict__ q, T* __restrict__ grid,
-                                                      T* __restrict__ halo)
+                                                      const double* __restrict__ q, T* __restrict__ grid)
 {
     extern __shared__ char smem_raw[];
     const TileCoords tc = tile_coords(mp, tg);
@@ -277,56 +278,18 @@ __global__ __launch_bounds__(256) void k_spread_tiled(MeshParams mp, TileGeom tg
         __syncthreads();
     }
     const long plane = (long)mp.Ny * mp.Nz;
-    T* hb = halo + (long)tc.t * esz;
-    const int Tx = tg.T[0], Ty = tg.T[1], Tz = tg.T[2];
     BoxWalk bw(threadIdx.x, blockDim.x, tc.Ey, tc.Ez);
     for (int k = threadIdx.x; k < esz; k += blockDim.x, bw.next()) {
-        if (bw.lx < Tx && bw.ly < Ty && bw.lz < Tz)
-            grid[(long)(tc.x0 + bw.lx) * plane + (long)(tc.y0 + bw.ly) * mp.Nz + (tc.z0 + bw.lz)] = (T)ext[k];
-        else
-            hb[k] = (T)ext[k];
-    }
-}
-
-// Adds the extensions of the (up to 7) tiles below each node's tile into the mesh.
-// Block (y, x) covers one z row, threads walk z: no 64-bit index arithmetic.
-template <typename T>
-__global__ __launch_bounds__(256) void k_spread_halo_add(MeshParams mp, TileGeom tg, const T* __restrict__ halo,
-                                                         T* __restrict__ grid)
-{
-    const int y = blockIdx.x, x = blockIdx.y;
-    const int P = mp.P;
-    const int lx = x % tg.T[0], ly = y % tg.T[1];
-    const int tx = x / tg.T[0], ty = y / tg.T[1];
-    const int Ey = tg.T[1] + P - 1, Ez = tg.T[2] + P - 1;
-    const long esz = (long)(tg.T[0] + P - 1) * Ey * Ez;
-    T* row = grid + ((long)x * mp.Ny + y) * mp.Nz;
-    const bool xy_inner = lx >= P - 1 && ly >= P - 1;
-    for (int z = threadIdx.x; z < mp.Nz; z += blockDim.x) {
-        const int lz = z % tg.T[2], tz = z / tg.T[2];
-        if (xy_inner && lz >= P - 1) continue;
-        T acc = 0;
-        for (int a = 0; a < 2; ++a) {
-            if (a && lx >= P - 1) continue;
-            for (int b = 0; b < 2; ++b) {
-                if (b && ly >= P - 1) continue;
-                for (int c = 0; c < 2; ++c) {
-                    if ((c && lz >= P - 1) || (a + b + c == 0)) continue;
-                    const int sx = (tx - a + tg.nt[0]) % tg.nt[0];
-                    const int sy = (ty - b + tg.nt[1]) % tg.nt[1];
-                    const int sz = (tz - c + tg.nt[2]) % tg.nt[2];
-                    const long src = ((long)sx * tg.nt[1] + sy) * tg.nt[2] + sz;
-                    const int ex = lx + a * tg.T[0], ey = ly + b * tg.T[1], ez = lz + c * tg.T[2];
-                    acc += halo[src * esz + ((long)ex * Ey + ey) * Ez + ez];
-                }
-            }
-        }
-        row[z] += acc;
+        int gx = tc.x0 + bw.lx, gy = tc.y0 + bw.ly, gz = tc.z0 + bw.lz;
+        gx -= gx >= mp.Nx ? mp.Nx : 0;
+        gy -= gy >= mp.Ny ? mp.Ny : 0;
+        gz -= gz >= mp.Nz ? mp.Nz : 0;
+        atomic_add(&grid[(long)gx * plane + (long)gy * mp.Nz + gz], (T)ext[k]);
     }
 }
 
 template <typename T, bool FORCE, int PT>
-__global__ __launch_bounds__(256) void k_contract_tiled(MeshParams mp, TileGeom tg, int ch,
+__global__ __launch_bounds__(kTiledThreads) void k_contract_tiled(MeshParams mp, TileGeom tg, int ch,
                                                         const unsigned* __restrict__ tstart,
                                                         const unsigned* __restrict__ sorted,
                                                         const int* __restrict__ rec_off, const T* __restrict__ rec_w,
@@ -428,7 +391,7 @@ TiledLds tiled_lds(const MeshParams& mp, const TileGeom& tg, size_t ext_elem, si
     l.ext_bytes = (size_t)(tg.T[0] + mp.P - 1) * (tg.T[1] + mp.P - 1) * (tg.T[2] + mp.P - 1) * ext_elem;
     const size_t per_bead = (size_t)(3 * mp.P + 3 + 1) * stage_elem + 2 * sizeof(int);
     long ch = l.ext_bytes < kTiledLdsBudget ? (long)((kTiledLdsBudget - l.ext_bytes) / per_bead) : 0;
-    if (ch > 256) ch = 256;
+    if (ch > kTiledChunk) ch = kTiledChunk;
     ch -= ch % 4;
     l.chunk = (int)ch;
     l.total_bytes = l.ext_bytes + (size_t)ch * per_bead;
@@ -465,32 +428,29 @@ hipError_t launch_bead_records(const MeshParams& mp, const TileGeom& tg, const S
 
 template <typename T>
 hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const BinView& bv, const double* q, T* grid,
-                               T* halo, hipStream_t st, int parts)
+                               hipStream_t st, bool clear)
 {
     const TiledLds l = tiled_lds(mp, tg, sizeof(SpreadAcc), sizeof(T));
     if (l.chunk < 4 || mp.P > 16) return hipErrorInvalidValue;
+    if (clear) {
+        hipError_t e = launch_fill<T>(grid, (T)0, (long)mp.Nx * mp.Ny * mp.Nz, st);
+        if (e != hipSuccess) return e;
+    }
     const T* rec_w = static_cast<const T*>(bv.rec_w);
 #define CR_SPREAD_P(PV)                                                                                        \
-    hipLaunchKernelGGL((k_spread_tiled<T, SpreadAcc, PV>), dim3(tg.ntiles), dim3(256), l.total_bytes, st, mp, \
-                       tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, q, grid, halo)
-    if (parts & 1) {
-        switch (mp.P) {
-            case 4: CR_SPREAD_P(4); break;
-            case 5: CR_SPREAD_P(5); break;
-            case 6: CR_SPREAD_P(6); break;
-            case 7: CR_SPREAD_P(7); break;
-            case 8: CR_SPREAD_P(8); break;
-            case 9: CR_SPREAD_P(9); break;
-            case 10: CR_SPREAD_P(10); break;
-            default: CR_SPREAD_P(0); break;
-        }
+    hipLaunchKernelGGL((k_spread_tiled<T, SpreadAcc, PV>), dim3(tg.ntiles), dim3(kTiledThreads), l.total_bytes, \
+                       st, mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, q, grid)
+    switch (mp.P) {
+        case 4: CR_SPREAD_P(4); break;
+        case 5: CR_SPREAD_P(5); break;
+        case 6: CR_SPREAD_P(6); break;
+        case 7: CR_SPREAD_P(7); break;
+        case 8: CR_SPREAD_P(8); break;
+        case 9: CR_SPREAD_P(9); break;
+        case 10: CR_SPREAD_P(10); break;
+        default: CR_SPREAD_P(0); break;
     }
 #undef CR_SPREAD_P
-    if (parts & 2) {
-        const int zthreads = mp.Nz >= 256 ? 256 : ((mp.Nz + 63) / 64) * 64;
-        hipLaunchKernelGGL(k_spread_halo_add<T>, dim3(mp.Ny, mp.Nx), dim3(zthreads), 0, st, mp, tg, (const T*)halo,
-                           grid);
-    }
     return hipGetLastError();
 }
 
@@ -504,11 +464,11 @@ hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const
 #define CR_CONTRACT_P(PV)                                                                                       \
     do {                                                                                                        \
         if (force)                                                                                              \
-            hipLaunchKernelGGL((k_contract_tiled<T, true, PV>), dim3(tg.ntiles), dim3(256), l.total_bytes, st,  \
+            hipLaunchKernelGGL((k_contract_tiled<T, true, PV>), dim3(tg.ntiles), dim3(kTiledThreads), l.total_bytes, st,  \
                                mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, s.pos, q, grid,    \
                                (T*)nullptr, force);                                                             \
         else                                                                                                    \
-            hipLaunchKernelGGL((k_contract_tiled<T, false, PV>), dim3(tg.ntiles), dim3(256), l.total_bytes, st, \
+            hipLaunchKernelGGL((k_contract_tiled<T, false, PV>), dim3(tg.ntiles), dim3(kTiledThreads), l.total_bytes, st, \
                                mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, s.pos,             \
                                (const double*)nullptr, grid, phi, (double*)nullptr);                            \
     } while (0)
@@ -530,7 +490,7 @@ hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const
     template hipError_t launch_bead_records<T>(const MeshParams&, const TileGeom&, const SysView&, const unsigned*, \
                                                const unsigned*, int*, T*, hipStream_t);                         \
     template hipError_t launch_spread_tiled<T>(const MeshParams&, const TileGeom&, const BinView&, const double*, \
-                                               T*, T*, hipStream_t, int);                                          \
+                                               T*, hipStream_t, bool);                                          \
     template hipError_t launch_contract_tiled<T>(const MeshParams&, const TileGeom&, const SysView&,              \
                                                  const BinView&, const double*, const T*, T*, double*, hipStream_t);
 CR_INST(float)
