@@ -144,6 +144,8 @@ def main() -> int:
 
     cr32 = ConductorRigid(system=sysdef, precision="fp32", **common)
     log(f"params {cr32.params['mesh']} P={cr32.params['P']} rc={cr32.params['rc']} pairs={cr32.dsys.n_pairs}")
+    tune32 = cr32.autotune()  # setup, untimed: the tile shape for this system
+    log(f"autotune fp32: {tune32}")
     dt32, it32, st32 = time_steps(cr32, args.steps, args.warmup)
     F32 = cr32.get_forces()
     q32 = cr32.get_charges()
@@ -192,6 +194,7 @@ def main() -> int:
         import conductorrigid_amd.warmstart as WS
 
         crw = ConductorRigid(system=copy.deepcopy(sysdef), precision="fp32", **common)
+        crw.set_tiles(tune32["tile"], tune32["lds_bytes"])
         mlp = WS.WarmStartMLP(device=0, seed=0)
         t0 = time.perf_counter()
         _, pairs = WS.run_trajectory(crw, args.train_steps, 2e-4, collect=range(0, args.train_steps, 2), mlp=mlp,
@@ -216,6 +219,9 @@ def main() -> int:
     if not args.no_fp64:
         sys64, _ = make_workload(args.workload)
         cr64 = ConductorRigid(system=sys64, precision="fp64", **common)
+        tune64 = cr64.autotune()
+        log(f"autotune fp64: {tune64}")
+        extra["autotune_fp64"] = tune64
         dt64, it64, st64 = time_steps(cr64, max(1, min(args.steps, 3)), 1)
         F64 = cr64.get_forces()
         errF = float(np.linalg.norm(F32[:, :3] - F64[:, :3]) / np.linalg.norm(F64[:, :3]))
@@ -266,6 +272,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "matvec_ms": matvec_ms,
         "stages_ms": {k: v["ms"] for k, v in stages.items()},
+        "autotune_fp32": tune32,
         "stages_GBs": {k: v["GBs"] for k, v in stages.items()},
         "warm_start": warm,
         **extra,

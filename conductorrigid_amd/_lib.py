@@ -59,6 +59,11 @@ class CrSystem(C.Structure):
                 ("n_neigh", vp)]
 
 
+class CrTuning(C.Structure):
+    _fields_ = [("tile", C.c_int * 3), ("lds_bytes", C.c_int), ("spread_contract_ms", C.c_double),
+                ("candidates", C.c_int)]
+
+
 class CrSolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("matvecs", C.c_int), ("restarts", C.c_int), ("converged", C.c_int),
                 ("resid", C.c_double), ("tolerance", C.c_double), ("true_resid", C.c_double),
@@ -91,6 +96,8 @@ _SIGS = {
     "cr_profile_matvec": (C.c_int, [vp, C.POINTER(CrSystem), vp, C.c_int, C.POINTER(C.c_double), C.c_int, vp]),
     "cr_bd_step": (C.c_int, [vp, C.POINTER(CrSystem), vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                              C.c_double, C.c_double, vp]),
+    "cr_autotune": (C.c_int, [vp, C.POINTER(CrSystem), C.POINTER(CrTuning), vp]),
+    "cr_set_tiles": (C.c_int, [vp, C.POINTER(C.c_int), C.c_int]),
     "cr_mlp_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_ulonglong]),
     "cr_mlp_destroy": (None, [vp]),
     "cr_mlp_set_params": (C.c_int, [vp, vp]),

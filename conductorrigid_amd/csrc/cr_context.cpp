@@ -728,8 +728,18 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
         auto ext_bytes = [&]() {
             return (size_t)(tg.T[0] + r.P - 1) * (tg.T[1] + r.P - 1) * (tg.T[2] + r.P - 1) * sizeof(SpreadAcc);
         };
-        // leave about a quarter of the LDS budget for the per-bead staging tables
-        const size_t ext_cap = kTiledLdsBudget * 3 / 4;
+        // LDS budget per binned workgroup: the default, or CR_LDS_KB (tuning override, up to
+        // the device's per-workgroup maximum); about a quarter is left for the per-bead
+        // staging tables
+        size_t budget = kTiledLdsBudget;
+        if (const char* env = std::getenv("CR_LDS_KB")) {
+            int dev_max = 0;
+            (void)hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+            const long kb = std::atol(env);
+            if (kb >= 16 && (size_t)kb * 1024 <= (size_t)dev_max) budget = (size_t)kb * 1024;
+        }
+        tg.lds = (int)budget;
+        const size_t ext_cap = budget * 3 / 4;
         // shrink x first: z stays the longest edge, so the tile's mesh rows are long
         // contiguous runs for the write-out and the extended-tile load
         for (int d = 0; tg.valid && ext_bytes() > ext_cap && d <= 2;) {
@@ -1163,6 +1173,113 @@ int cr_bd_step(cr_context* c, const cr_system* s, double* pos, double* rigid_com
     CR_HIP(launch_bd_step(p, v, reinterpret_cast<double4*>(pos), reinterpret_cast<double4*>(rigid_com),
                           reinterpret_cast<const double4*>(force), c->bd_force.as<double4>(), noise_rigid,
                           noise_beads, st));
+    return CR_OK;
+}
+
+// ---- tile autotuner ----
+
+int cr_set_tiles(cr_context* c, const int tile[3], int lds_bytes)
+{
+    if (!c || !c->ready || !tile) return set_error(CR_ERR_STATE, "cr_set_params has not been called");
+    const int P = c->mp.P, Nd[3] = {c->mp.Nx, c->mp.Ny, c->mp.Nz};
+    int dev_max = 0;
+    CR_HIP(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    if (lds_bytes < 16 * 1024 || lds_bytes > dev_max) return set_error(CR_ERR_INVALID, "LDS budget out of range");
+    for (int d = 0; d < 3; ++d)
+        if (tile[d] < std::max(P - 1, 1) || Nd[d] % tile[d] != 0)
+            return set_error(CR_ERR_INVALID, "tile edges must divide the mesh and be >= P - 1");
+    TileGeom tg{};
+    for (int d = 0; d < 3; ++d) tg.T[d] = tile[d];
+    tg.lds = lds_bytes;
+    tg.valid = 1;
+    tg.ntiles = 1;
+    for (int d = 0; d < 3; ++d) {
+        tg.nt[d] = Nd[d] / tg.T[d];
+        tg.ntiles *= tg.nt[d];
+    }
+    const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
+    if (tiled_lds(c->mp, tg, sizeof(SpreadAcc), Tsz).chunk < 16)
+        return set_error(CR_ERR_INVALID, "extended tile does not fit the LDS budget");
+    c->tg = tg;
+    return CR_OK;
+}
+
+int cr_autotune(cr_context* c, const cr_system* s, cr_tuning* out, void* stream)
+{
+    if (!c || !c->ready) return set_error(CR_ERR_STATE, "cr_set_params has not been called");
+    hipStream_t st = (hipStream_t)stream;
+    const int P = c->mp.P, Nd[3] = {c->mp.Nx, c->mp.Ny, c->mp.Nz};
+    int dev_max = 0;
+    CR_HIP(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
+    std::vector<int> opts[3];
+    for (int d = 0; d < 3; ++d)
+        for (int t = std::max(P - 1, 4); t <= std::min(Nd[d], 32); ++t)
+            if (Nd[d] % t == 0) opts[d].push_back(t);
+    const TileGeom keep = c->tg;
+    hipEvent_t e0, e1;
+    CR_HIP(hipEventCreate(&e0));
+    CR_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    TileGeom best_tg = keep;
+    int tried = 0;
+    const double* q = nullptr;
+    SysView v;
+    for (int tx : opts[0])
+        for (int ty : opts[1])
+            for (int tz : opts[2]) {
+                // the smallest budget (in 32 KiB steps from 64 KiB) that holds the tile and
+                // at least 32 staged beads: more LDS than that only lowers occupancy
+                const int T3[3] = {tx, ty, tz};
+                int budget = 0;
+                for (int b = 64 * 1024; b <= dev_max; b += 32 * 1024) {
+                    TileGeom g{};
+                    for (int d = 0; d < 3; ++d) g.T[d] = T3[d];
+                    g.lds = b;
+                    if (tiled_lds(c->mp, g, sizeof(SpreadAcc), Tsz).chunk >= 32) {
+                        budget = b;
+                        break;
+                    }
+                }
+                if (!budget || cr_set_tiles(c, T3, budget) != CR_OK) continue;
+                CR_RET(prepare(c, s, st, &v));
+                if (!q) {  // any charges will do for timing: the bead records
+                    CR_HIP(c->yraw.ensure(sizeof(double) * (size_t)std::max(1, v.N)));
+                    CR_HIP(launch_fill<double>(c->yraw.as<double>(), 1.0, v.N, st));
+                    q = c->yraw.as<double>();
+                }
+                auto run = [&]() -> int {
+                    if (c->fp32()) {
+                        CR_RET(spread_impl<float>(c, v, q, st));
+                        CR_RET(contract_impl<float>(c, v, c->phi_recip.as<float>(), st));
+                    } else {
+                        CR_RET(spread_impl<double>(c, v, q, st));
+                        CR_RET(contract_impl<double>(c, v, c->phi_recip.as<double>(), st));
+                    }
+                    return CR_OK;
+                };
+                CR_RET(run());  // warm
+                CR_HIP(hipEventRecord(e0, st));
+                for (int r = 0; r < 3; ++r) CR_RET(run());
+                CR_HIP(hipEventRecord(e1, st));
+                CR_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                CR_HIP(hipEventElapsedTime(&ms, e0, e1));
+                ++tried;
+                if (ms / 3.0 < best) {
+                    best = ms / 3.0;
+                    best_tg = c->tg;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->tg = tried ? best_tg : keep;
+    if (out) {
+        for (int d = 0; d < 3; ++d) out->tile[d] = c->tg.T[d];
+        out->lds_bytes = c->tg.lds;
+        out->spread_contract_ms = tried ? best : -1.0;
+        out->candidates = tried;
+    }
     return CR_OK;
 }
 

@@ -59,6 +59,7 @@ struct TileGeom {
     int nt[3];     // tiles per axis
     int ntiles;
     int valid;     // 0: no admissible tiling, use the atomic spread
+    int lds;       // LDS bytes a binned workgroup may use (extended tile + staging)
 };
 
 // ---- k_se.hip ----
@@ -90,7 +91,7 @@ hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const B
 // LDS plan of the binned kernels: the tile extended by P-1 nodes per axis (ext_elem bytes
 // per node) plus per-bead staging tables (stage_elem bytes per value) for `chunk` beads,
 // within kTiledLdsBudget bytes.  The spread accumulates its tile in SpreadAcc.
-constexpr size_t kTiledLdsBudget = 64 * 1024;
+constexpr size_t kTiledLdsBudget = 64 * 1024;   // default per-workgroup LDS budget
 constexpr int kTiledThreads = 512;   // workgroup of the binned kernels
 constexpr int kTiledChunk = 64;      // beads staged per round (bins hold ~60 at 1M / 256^3)
 using SpreadAcc = double;

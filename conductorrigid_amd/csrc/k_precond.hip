@@ -132,6 +132,87 @@ __global__ __launch_bounds__(64) void k_block_setup(int nmax, const unsigned* __
     }
 }
 
+__device__ __forceinline__ double lane_bcast(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Bodies of at most NB <= 32 beads: one wavefront per body, lane i holds row i of G in
+// registers and the inverse is formed in place by Gauss-Jordan (no pivoting: G is
+// symmetric positive definite), pivot rows broadcast with readlane -- no LDS, no
+// barriers.  Rows and columns n..NB-1 are padded with the identity, which the
+// elimination leaves untouched.  Four bodies per 256-thread workgroup.
+template <int NB>
+__global__ __launch_bounds__(256) void k_block_setup_wave(int B, int nmax, const unsigned* __restrict__ start,
+                                                          const unsigned* __restrict__ beads,
+                                                          const double4* __restrict__ pos, double Lx, double Ly,
+                                                          double Lz, float* __restrict__ ginv,
+                                                          float* __restrict__ uvec, double* __restrict__ inv_s)
+{
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= B) return;  // whole wavefront
+    const int i = threadIdx.x & 63;
+    const int b0 = start[t];
+    const int n = start[t + 1] - b0;
+    if (n > NB) {
+        if (i == 0) inv_s[t] = __builtin_nan("");
+        return;
+    }
+    double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (i < n) pi = pos[beads[b0 + i]];
+    double a[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const double xj = lane_bcast(pi.x, j), yj = lane_bcast(pi.y, j), zj = lane_bcast(pi.z, j);
+        double g;
+        if (i < n && j < n) {
+            if (i == j) {
+                g = shell_shell(0.0);
+            } else {
+                const double dx = min_image(pi.x - xj, Lx), dy = min_image(pi.y - yj, Ly),
+                             dz = min_image(pi.z - zj, Lz);
+                g = shell_shell(sqrt(dx * dx + dy * dy + dz * dz));
+            }
+        } else {
+            g = (i == j) ? 1.0 : 0.0;
+        }
+        a[j] = g;
+    }
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        double pk[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) pk[j] = lane_bcast(a[j], k);
+        const double inv = 1.0 / pk[k];
+        const double f = a[k];
+        if (i == k) {
+#pragma unroll
+            for (int j = 0; j < NB; ++j) a[j] = (j == k) ? inv : pk[j] * inv;
+        } else {
+#pragma unroll
+            for (int j = 0; j < NB; ++j) a[j] = (j == k) ? -f * inv : a[j] - f * inv * pk[j];
+        }
+    }
+    double u = 0.0;
+    if (i < n) {
+        float* row = ginv + ((long)t * nmax + i) * nmax;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            if (j < n) {
+                row[j] = (float)a[j];
+                u += a[j];
+            }
+        uvec[(long)t * nmax + i] = (float)u;
+    }
+    double ssum = 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) ssum += (j < n) ? lane_bcast(u, j) : 0.0;
+    if (i == 0) inv_s[t] = 1.0 / ssum;
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -196,9 +277,24 @@ hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, uns
         hipLaunchKernelGGL(k_body_fill, dim3(blocks(s.N, 256)), dim3(256), 0, st, s.N, s.iso, body_start, cursor,
                            body_beads);
     hipLaunchKernelGGL(k_body_sort, dim3(blocks(B, 256)), dim3(256), 0, st, B, body_start, body_beads);
-    const size_t lds = sizeof(double) * (size_t)nmax * 2 * nmax;
-    hipLaunchKernelGGL(k_block_setup, dim3(B), dim3(64), lds, st, nmax, body_start, body_beads, s.pos, L[0], L[1],
-                       L[2], ginv, uvec, inv_s);
+    const unsigned wb = (unsigned)((B + 3) / 4);
+#define CR_BLOCK_WAVE(NB)                                                                                   \
+    hipLaunchKernelGGL(k_block_setup_wave<NB>, dim3(wb), dim3(256), 0, st, B, nmax, body_start, body_beads, \
+                       s.pos, L[0], L[1], L[2], ginv, uvec, inv_s)
+    if (nmax <= 8) {
+        CR_BLOCK_WAVE(8);
+    } else if (nmax <= 16) {
+        CR_BLOCK_WAVE(16);
+    } else if (nmax <= 24) {
+        CR_BLOCK_WAVE(24);
+    } else if (nmax <= 32) {
+        CR_BLOCK_WAVE(32);
+    } else {
+        const size_t lds = sizeof(double) * (size_t)nmax * 2 * nmax;
+        hipLaunchKernelGGL(k_block_setup, dim3(B), dim3(64), lds, st, nmax, body_start, body_beads, s.pos, L[0],
+                           L[1], L[2], ginv, uvec, inv_s);
+    }
+#undef CR_BLOCK_WAVE
     return hipGetLastError();
 }
 

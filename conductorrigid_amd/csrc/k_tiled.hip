@@ -390,7 +390,8 @@ TiledLds tiled_lds(const MeshParams& mp, const TileGeom& tg, size_t ext_elem, si
     TiledLds l;
     l.ext_bytes = (size_t)(tg.T[0] + mp.P - 1) * (tg.T[1] + mp.P - 1) * (tg.T[2] + mp.P - 1) * ext_elem;
     const size_t per_bead = (size_t)(3 * mp.P + 3 + 1) * stage_elem + 2 * sizeof(int);
-    long ch = l.ext_bytes < kTiledLdsBudget ? (long)((kTiledLdsBudget - l.ext_bytes) / per_bead) : 0;
+    const size_t budget = tg.lds > 0 ? (size_t)tg.lds : kTiledLdsBudget;
+    long ch = l.ext_bytes < budget ? (long)((budget - l.ext_bytes) / per_bead) : 0;
     if (ch > kTiledChunk) ch = kTiledChunk;
     ch -= ch % 4;
     l.chunk = (int)ch;
@@ -438,8 +439,14 @@ hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const B
     }
     const T* rec_w = static_cast<const T*>(bv.rec_w);
 #define CR_SPREAD_P(PV)                                                                                        \
-    hipLaunchKernelGGL((k_spread_tiled<T, SpreadAcc, PV>), dim3(tg.ntiles), dim3(kTiledThreads), l.total_bytes, \
-                       st, mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, q, grid)
+    do {                                                                                                       \
+        if (l.total_bytes > 65536)                                                                             \
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_tiled<T, SpreadAcc, PV>),       \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total_bytes);         \
+        hipLaunchKernelGGL((k_spread_tiled<T, SpreadAcc, PV>), dim3(tg.ntiles), dim3(kTiledThreads),          \
+                           l.total_bytes, st, mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, q, \
+                           grid);                                                                              \
+    } while (0)
     switch (mp.P) {
         case 4: CR_SPREAD_P(4); break;
         case 5: CR_SPREAD_P(5); break;
@@ -463,6 +470,12 @@ hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const
     const T* rec_w = static_cast<const T*>(bv.rec_w);
 #define CR_CONTRACT_P(PV)                                                                                       \
     do {                                                                                                        \
+        if (l.total_bytes > 65536) {                                                                            \
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract_tiled<T, true, PV>),            \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total_bytes);          \
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract_tiled<T, false, PV>),           \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total_bytes);          \
+        }                                                                                                       \
         if (force)                                                                                              \
             hipLaunchKernelGGL((k_contract_tiled<T, true, PV>), dim3(tg.ntiles), dim3(kTiledThreads), l.total_bytes, st,  \
                                mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, s.pos, q, grid,    \

@@ -256,6 +256,22 @@ int cr_bd_step(cr_context* ctx, const cr_system* sys, double* pos, double* rigid
                const double* noise_rigid, const double* noise_beads, double dt, double kT, double gamma_t,
                double gamma_r, double k_rep, void* stream);
 
+/* ---- spread/contract tiling ---------------------------------------------
+ * No reference counterpart (the reference launches one thread block per bead).  The
+ * binned spread and contraction work on mesh tiles whose edges divide the mesh and are
+ * at least P - 1; each workgroup holds its tile extended by P - 1 nodes per axis in LDS.
+ * cr_set_params picks a tiling by rule; cr_autotune times the spread + contraction of
+ * the given system for every admissible tile shape (edges 4..32, each with the smallest
+ * LDS budget that holds it) and keeps the fastest; cr_set_tiles sets one explicitly. */
+typedef struct {
+    int tile[3];
+    int lds_bytes;
+    double spread_contract_ms;   /* best measured time of one spread + contraction */
+    int candidates;              /* tile shapes timed */
+} cr_tuning;
+int cr_autotune(cr_context* ctx, const cr_system* sys, cr_tuning* out, void* stream);
+int cr_set_tiles(cr_context* ctx, const int tile[3], int lds_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -169,3 +169,22 @@ def test_fp32_bulk_accuracy(mode, require_gpu):
     _report("fp32 charges vs fp64", vq)
     _report("fp32 forces vs fp64", vF)
     assert vF < 1e-5 and vq < 1e-5
+
+
+@pytest.mark.parametrize("tile", [(8, 8, 8), (16, 8, 8), (8, 16, 32)])
+def test_tiling_does_not_change_the_answer(case, tile):
+    """Every admissible tile shape (and the autotuner's choice) gives the oracle's
+    spread and contraction and the same tight solve."""
+    name, s, g, o = case
+    mesh = g.mesh
+    if any(m % t for m, t in zip(mesh, tile)):
+        pytest.skip("tile does not divide this mesh")
+    g.cr.set_tiles(tile, 160 * 1024)
+    q = np.random.default_rng(5).normal(size=s.N)
+    assert rel(g.spread(q), o.spread(q)) < 1e-12
+    tuned = g.cr.autotune()
+    assert tuned["candidates"] >= 1 and all(m % t == 0 for m, t in zip(mesh, tuned["tile"]))
+    assert rel(g.spread(q), o.spread(q)) < 1e-12
+    qs, Phi, F, st = g.solve("zero")
+    qo, Po, Fo, so = o.compute_forces([0.0, 0.0, 1.0], restart=30, maxiter=600, tol=1e-12)
+    assert rel(qs, qo) < 1e-10 and rel(F[:, :3], Fo[:, :3]) < 1e-10
