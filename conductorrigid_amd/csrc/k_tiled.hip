@@ -68,8 +68,15 @@ __device__ __forceinline__ int xcd_tile(int b, int ntiles)
     return (b % 8) * (ntiles / 8) + b / 8;
 }
 
+// LDS strides of the extended tile: the z stride is Ez rounded up to even, so that the
+// 64 lanes of one window update (8 y rows x 8 z nodes of doubles) fall on alternating
+// bank halves instead of overlapping ones.
+constexpr bool kExtPad = true;
+__host__ __device__ __forceinline__ int ext_ys(int Ez) { return kExtPad ? (Ez + 1) & ~1 : Ez; }
+
 struct TileCoords {
     int t, x0, y0, z0, Ex, Ey, Ez;
+    int ys, xs, esz;  // LDS z stride, x stride, elements of the (padded) extended tile
 };
 
 __device__ __forceinline__ TileCoords tile_coords(const MeshParams& mp, const TileGeom& tg)
@@ -83,6 +90,9 @@ __device__ __forceinline__ TileCoords tile_coords(const MeshParams& mp, const Ti
     c.Ex = tg.T[0] + mp.P - 1;
     c.Ey = tg.T[1] + mp.P - 1;
     c.Ez = tg.T[2] + mp.P - 1;
+    c.ys = ext_ys(c.Ez);
+    c.xs = c.Ey * c.ys;
+    c.esz = c.Ex * c.xs;
     return c;
 }
 
@@ -248,12 +258,12 @@ __global__ __launch_bounds__(kTiledThreads) void k_spread_tiled(MeshParams mp, T
     const TileCoords tc = tile_coords(mp, tg);
     constexpr int PM = PT > 0 ? PT : 16;
     const int P = PT > 0 ? PT : mp.P;
-    const int esz = tc.Ex * tc.Ey * tc.Ez;
+    const int esz = tc.esz;
     A* ext = reinterpret_cast<A*>(smem_raw);
     Stage<T> s = stage_layout<T>(smem_raw, (size_t)esz * sizeof(A), ch, P);
     for (int k = threadIdx.x; k < esz; k += blockDim.x) ext[k] = (A)0;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    const int xs = tc.Ey * tc.Ez, ys = tc.Ez, per = 3 * P;
+    const int xs = tc.xs, ys = tc.ys, per = 3 * P;
     const unsigned kb = tstart[tc.t], ke = tstart[tc.t + 1];
     for (unsigned k0 = kb; k0 < ke; k0 += ch) {
         const int cnt = (int)min((unsigned)ch, ke - k0);
@@ -278,13 +288,14 @@ __global__ __launch_bounds__(kTiledThreads) void k_spread_tiled(MeshParams mp, T
         __syncthreads();
     }
     const long plane = (long)mp.Ny * mp.Nz;
+    const int nodes = tc.Ex * tc.Ey * tc.Ez;
     BoxWalk bw(threadIdx.x, blockDim.x, tc.Ey, tc.Ez);
-    for (int k = threadIdx.x; k < esz; k += blockDim.x, bw.next()) {
+    for (int k = threadIdx.x; k < nodes; k += blockDim.x, bw.next()) {
         int gx = tc.x0 + bw.lx, gy = tc.y0 + bw.ly, gz = tc.z0 + bw.lz;
         gx -= gx >= mp.Nx ? mp.Nx : 0;
         gy -= gy >= mp.Ny ? mp.Ny : 0;
         gz -= gz >= mp.Nz ? mp.Nz : 0;
-        atomic_add(&grid[(long)gx * plane + (long)gy * mp.Nz + gz], (T)ext[k]);
+        atomic_add(&grid[(long)gx * plane + (long)gy * mp.Nz + gz], (T)ext[bw.lx * xs + bw.ly * ys + bw.lz]);
     }
 }
 
@@ -301,20 +312,21 @@ __global__ __launch_bounds__(kTiledThreads) void k_contract_tiled(MeshParams mp,
     const TileCoords tc = tile_coords(mp, tg);
     constexpr int PM = PT > 0 ? PT : 16;
     const int P = PT > 0 ? PT : mp.P;
-    const int esz = tc.Ex * tc.Ey * tc.Ez;
+    const int esz = tc.esz;
+    const int xs = tc.xs, ys = tc.ys, per = 3 * P;
     T* ext = reinterpret_cast<T*>(smem_raw);
     Stage<T> s = stage_layout<T>(smem_raw, (size_t)esz * sizeof(T), ch, P);
     const long plane = (long)mp.Ny * mp.Nz;
+    const int nodes = tc.Ex * tc.Ey * tc.Ez;
     BoxWalk bw(threadIdx.x, blockDim.x, tc.Ey, tc.Ez);
-    for (int k = threadIdx.x; k < esz; k += blockDim.x, bw.next()) {
+    for (int k = threadIdx.x; k < nodes; k += blockDim.x, bw.next()) {
         int gx = tc.x0 + bw.lx, gy = tc.y0 + bw.ly, gz = tc.z0 + bw.lz;
         gx -= gx >= mp.Nx ? mp.Nx : 0;
         gy -= gy >= mp.Ny ? mp.Ny : 0;
         gz -= gz >= mp.Nz ? mp.Nz : 0;
-        ext[k] = grid[(long)gx * plane + (long)gy * mp.Nz + gz];
+        ext[bw.lx * xs + bw.ly * ys + bw.lz] = grid[(long)gx * plane + (long)gy * mp.Nz + gz];
     }
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    const int xs = tc.Ey * tc.Ez, ys = tc.Ez, per = 3 * P;
     const T pc = (T)(mp.quadW * mp.prefac);
     const unsigned kb = tstart[tc.t], ke = tstart[tc.t + 1];
     for (unsigned k0 = kb; k0 < ke; k0 += ch) {
@@ -388,7 +400,7 @@ inline unsigned blocks(long n, int b)
 TiledLds tiled_lds(const MeshParams& mp, const TileGeom& tg, size_t ext_elem, size_t stage_elem)
 {
     TiledLds l;
-    l.ext_bytes = (size_t)(tg.T[0] + mp.P - 1) * (tg.T[1] + mp.P - 1) * (tg.T[2] + mp.P - 1) * ext_elem;
+    l.ext_bytes = (size_t)(tg.T[0] + mp.P - 1) * (tg.T[1] + mp.P - 1) * ext_ys(tg.T[2] + mp.P - 1) * ext_elem;
     const size_t per_bead = (size_t)(3 * mp.P + 3 + 1) * stage_elem + 2 * sizeof(int);
     const size_t budget = tg.lds > 0 ? (size_t)tg.lds : kTiledLdsBudget;
     long ch = l.ext_bytes < budget ? (long)((budget - l.ext_bytes) / per_bead) : 0;
