@@ -67,18 +67,53 @@ __global__ void k_max_u32(const unsigned* __restrict__ a, int n, unsigned* __res
     if ((threadIdx.x & 63) == 0) atomicMax(out, v);
 }
 
+// The neighbour rows of a workgroup's 256 beads usually form one contiguous span of the
+// list (head_list is a running sum); stage that span in LDS with coalesced loads so each
+// thread then walks its own row from LDS.  Rows outside the staged span (any other
+// head_list layout, or a span beyond kNlStage) are read from global memory.
+constexpr int kNlStage = 8192;
+
+struct NlSpan {
+    unsigned lo, hi;  // staged entries [lo, hi)
+};
+
+__device__ __forceinline__ NlSpan stage_nlist(const SysView& s, unsigned* sh)
+{
+    __shared__ unsigned bounds[2];
+    const int i0 = blockIdx.x * blockDim.x;
+    const int i1 = min(s.N, i0 + (int)blockDim.x) - 1;
+    if (threadIdx.x == 0) {
+        const unsigned lo = s.head_list[i0], hi = s.head_list[i1] + s.n_neigh[i1];
+        const bool ok = hi >= lo && hi - lo <= (unsigned)kNlStage;
+        bounds[0] = lo;
+        bounds[1] = ok ? hi : lo;
+    }
+    __syncthreads();
+    NlSpan sp{bounds[0], bounds[1]};
+    for (unsigned k = sp.lo + threadIdx.x; k < sp.hi; k += blockDim.x) sh[k - sp.lo] = s.nlist[k];
+    __syncthreads();
+    return sp;
+}
+
+__device__ __forceinline__ unsigned nl_at(const SysView& s, const unsigned* sh, const NlSpan& sp, unsigned k)
+{
+    return (k >= sp.lo && k < sp.hi) ? sh[k - sp.lo] : s.nlist[k];
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_real_csr_build(RealParams rp, SysView s, const T* __restrict__ table,
                                                         long ldn, unsigned* __restrict__ col, T* __restrict__ val,
                                                         unsigned* __restrict__ row_len)
 {
+    __shared__ unsigned sh[kNlStage];
+    const NlSpan sp = stage_nlist(s, sh);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     const double4 ri = s.pos[i];
     const unsigned head = s.head_list[i], nn = s.n_neigh[i];
     unsigned cnt = 0;
     for (unsigned jj = 0; jj < nn; ++jj) {
-        const unsigned j = s.nlist[head + jj];
+        const unsigned j = nl_at(s, sh, sp, head + jj);
         const double4 rj = s.pos[j];
         const double dx = min_image(ri.x - rj.x, rp.L[0]);
         const double dy = min_image(ri.y - rj.y, rp.L[1]);
@@ -117,6 +152,8 @@ __global__ __launch_bounds__(256) void k_real_space_force(RealParams rp, SysView
                                                           const T* __restrict__ gtable, double E0x, double E0y,
                                                           double E0z, double* __restrict__ force)
 {
+    __shared__ unsigned sh[kNlStage];
+    const NlSpan sp = stage_nlist(s, sh);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     const double qi = q[i];
@@ -127,7 +164,7 @@ __global__ __launch_bounds__(256) void k_real_space_force(RealParams rp, SysView
     const double4 ri = s.pos[i];
     const unsigned head = s.head_list[i], nn = s.n_neigh[i];
     for (unsigned jj = 0; jj < nn; ++jj) {
-        const unsigned j = s.nlist[head + jj];
+        const unsigned j = nl_at(s, sh, sp, head + jj);
         const double4 rj = s.pos[j];
         const double dx = min_image(ri.x - rj.x, rp.L[0]);
         const double dy = min_image(ri.y - rj.y, rp.L[1]);
