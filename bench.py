@@ -97,6 +97,88 @@ def cpu_baseline(sysdef, prm, matvecs_per_eval: float, nlist) -> dict:
                       f"(the FP64 GPU solve's count)"}
 
 
+def bench_batch(args) -> int:
+    """64 independent boxes (N log-uniform in [2k, 50k], bodies of 8/20/40 beads, volume
+    fraction 0.1-0.3, per-box mesh and xi for 1e-6 Ewald error, P = 9), FP32 bulk with
+    FP64 Krylov, near-field block preconditioner, restart 30, tolerance 1e-6, x0 = 0.
+    One step = one force evaluation of every box.  Timed three ways on the same boxes:
+    the lockstep batch with its Arnoldi step replayed from a HIP graph, the same batch
+    launched eagerly, and the boxes solved one after another by the single-box path."""
+    import torch
+
+    from conductorrigid_amd.compute import BatchConductorRigid, batch_boxes
+    from conductorrigid_amd.system import make_batch
+
+    t0 = time.perf_counter()
+    items = make_batch(64, seed=4, workers=8)
+    N = [s.N for s, _ in items]
+    meshes = [p["mesh"][0] for _, p in items]
+    log(f"batch: {len(items)} boxes, {sum(N)} beads (N {min(N)}..{max(N)}), meshes {sorted(set(meshes))} "
+        f"({time.perf_counter() - t0:.1f} s)")
+    boxes = batch_boxes(items, precision="fp32", preconditioner="block", restart=30, max_iter=1000, gmres_tol=1e-6)
+    t0 = time.perf_counter()
+    tunes = [b.autotune() for b in boxes]
+    t_tune = time.perf_counter() - t0
+    log(f"autotune of {len(boxes)} boxes: {t_tune:.1f} s")
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            st = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps, st
+
+    res = {}
+    for use_graph in (True, False):
+        batch = BatchConductorRigid(boxes, use_graph=use_graph)
+        dt, st = timed(lambda: batch.computeForces(x0="zero"), args.steps, args.warmup)
+        key = "batch_graph" if use_graph else "batch_eager"
+        res[key] = {"ms_per_eval": dt * 1e3, "iters_max": max(s["iterations"] for s in st),
+                    "iters_mean": float(np.mean([s["iterations"] for s in st])),
+                    "all_converged": all(s["converged"] == 1 for s in st),
+                    "max_true_relres": max(s["true_relres"] for s in st)}
+        log(f"{key}: {dt * 1e3:.2f} ms per batch eval; iters mean {res[key]['iters_mean']:.1f} "
+            f"max {res[key]['iters_max']}")
+        del batch
+
+    def sequential():
+        return [b.computeForces(x0="zero") for b in boxes]
+
+    dts, sts = timed(sequential, max(1, min(args.steps, 2)), 1)
+    res["sequential"] = {"ms_per_eval": dts * 1e3, "iters_mean": float(np.mean([s["iterations"] for s in sts]))}
+    log(f"sequential single-box solves: {dts * 1e3:.2f} ms for all boxes")
+    dt = res["batch_graph"]["ms_per_eval"] * 1e-3
+    out = {
+        "metric": "batched force evals/sec (64 boxes, all boxes per eval)",
+        "value": 1.0 / dt,
+        "unit": "batch evals/sec",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 bulk + f64 krylov",
+        "data": "synthetic (seeded rigid-cluster boxes)",
+        "config": {"workload": f"batch64: {len(items)} boxes, {sum(N)} beads, N log-uniform [2000, 50000], bodies "
+                               f"8/20/40, phi 0.1-0.3, meshes {min(meshes)}^3-{max(meshes)}^3, P=9, tol 1e-6, "
+                               f"restart 30, near-field block preconditioner, x0=0",
+                   "beads": sum(N), "boxes": len(items)},
+        "beads_per_sec": sum(N) / dt,
+        "speedup_vs_sequential": res["sequential"]["ms_per_eval"] / res["batch_graph"]["ms_per_eval"],
+        "graph_vs_eager": res["batch_eager"]["ms_per_eval"] / res["batch_graph"]["ms_per_eval"],
+        "modes": res,
+        "autotune_s": t_tune,
+        "tiles": [t["tile"] for t in tunes],
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +195,9 @@ def main() -> int:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1).", file=sys.stderr)
         return 2
+
+    if args.workload == "batch64":
+        return bench_batch(args)
 
     import ctypes as C
 

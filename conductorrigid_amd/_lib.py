@@ -110,6 +110,11 @@ _SIGS = {
     "cr_mlp_train_step": (C.c_int, [vp, C.c_int, vp, vp, C.c_float, vp, vp]),
     "cr_build_nlist": (C.c_int, [vp, C.c_int, vp, C.POINTER(C.c_double), C.c_double, vp, vp, vp, C.c_long,
                                  C.POINTER(C.c_long), vp]),
+    "cr_batch_create": (C.c_int, [C.POINTER(vp), C.c_int]),
+    "cr_batch_destroy": (None, [vp]),
+    "cr_batch_compute_force": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.POINTER(CrSystem), C.POINTER(vp),
+                                         C.POINTER(vp), C.POINTER(vp), C.c_int, C.c_int, C.POINTER(CrSolveStats),
+                                         vp]),
 }
 
 _lib = None

@@ -273,4 +273,67 @@ def update_positions(cr: ConductorRigid, pos: np.ndarray, rigid_com: np.ndarray 
     cr.dsys.build_nlist()
 
 
-__all__ = ["ConductorRigid", "DeviceSystem", "update_positions", "NOT_MEMBER"]
+
+
+class BatchConductorRigid:
+    """Independent periodic boxes solved together (C ABI ``cr_batch_compute_force``).
+
+    Each box is a :class:`ConductorRigid` with its own system, parameters and tiling; a
+    call gives every box the charges, potentials and forces its own ``computeForces``
+    would give, while all boxes share each spread / contraction launch and each
+    lockstep Arnoldi step (replayed from a captured HIP graph when ``use_graph``).  No
+    reference counterpart: the reference handles one box per plugin instance."""
+
+    def __init__(self, boxes: list, device: int = 0, use_graph: bool = True):
+        if not boxes:
+            raise ValueError("a batch needs at least one box")
+        self.boxes = list(boxes)
+        self.lib = _lib.load()
+        self.use_graph = bool(use_graph)
+        self.torch = self.boxes[0].torch
+        h = C.c_void_p()
+        _lib.check(self.lib.cr_batch_create(C.byref(h), int(device)))
+        self.handle = h
+        self.last_stats = None
+
+    def autotune(self) -> list[dict]:
+        return [b.autotune() for b in self.boxes]
+
+    def computeForces(self, x0: str = "previous", stats: bool = True) -> list[dict] | None:
+        n = len(self.boxes)
+        ctxs = (C.c_void_p * n)(*[b.ctx.value for b in self.boxes])
+        views = [b.dsys.view() for b in self.boxes]
+        systems = (_lib.CrSystem * n)(*views)
+        q = (C.c_void_p * n)(*[b.charges.data_ptr() for b in self.boxes])
+        phi = (C.c_void_p * n)(*[b.potentials.data_ptr() for b in self.boxes])
+        force = (C.c_void_p * n)(*[b.forces.data_ptr() for b in self.boxes])
+        st = (_lib.CrSolveStats * n)()
+        code = self.lib.cr_batch_compute_force(self.handle, n, ctxs, systems, q, phi, force, _X0_MODES[x0],
+                                               int(self.use_graph), st if stats else None, self.boxes[0].stream())
+        _lib.check(code)
+        self.last_stats = [st[i].as_dict() for i in range(n)] if stats else None
+        return self.last_stats
+
+    compute = computeForces
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                self.lib.cr_batch_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def batch_boxes(items, precision: str = "fp32", preconditioner: str = "block", restart: int = 30,
+                max_iter: int = 300, gmres_tol: float = 0.0, field=(0.0, 0.0, 1.0), device: int = 0) -> list:
+    """One :class:`ConductorRigid` per (system, params) item of ``system.make_batch``."""
+    out = []
+    for sysdef, p in items:
+        out.append(ConductorRigid(system=sysdef, xi=p["xi"], errortol=p["errortol"], rc=p["rc"], mesh=p["mesh"],
+                                  P=p["P"], precision=precision, restart=restart, max_iter=max_iter,
+                                  gmres_tol=gmres_tol, field=field, device=device, preconditioner=preconditioner))
+    return out
+
+
+__all__ = ["ConductorRigid", "BatchConductorRigid", "DeviceSystem", "batch_boxes", "update_positions", "NOT_MEMBER"]

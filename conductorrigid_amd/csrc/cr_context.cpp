@@ -9,8 +9,7 @@
 // Hessenberg matrix and rotations on the device and reads back one scalar per
 // Arnoldi step to decide convergence.
 #include "../../include/conductorrigid.h"
-#include "cr_host.h"
-#include "cr_kernels.h"
+#include "cr_context_impl.h"
 
 #include <hip/hip_runtime.h>
 
@@ -95,55 +94,8 @@ double gradphiq_exact(double dist, double xi)
 
 }  // namespace
 
-struct cr_context {
-    int device = 0;
-    bool ready = false;
-    cr_params params{};
-    cr_resolved res{};
-    MeshParams mp{};
-    RealParams rp{};
-    int Nzh = 0;
-    long G = 0, Gh = 0;
-    AxisPlan plan_d[3]{}, plan_f[3]{};
-    DevBuf tw_d[3], tw_f[3];
-    DevBuf ktab[3], etab[3];
-    ScaleTables sc{};
-    DevBuf tab_d, gtab_d, tab_f, gtab_f;
-    DevBuf grid_real, grid_cplx;
-    DevBuf phi_recip, iso;
-    // real-space operator assembled per force evaluation (CSR over the neighbour list rows)
-    bool csr_ready = false;
-    long rs_ldn = 0;
-    DevBuf rs_col, rs_val, rs_len;
-    // bead binning for the owner-computes spread
-    TileGeom tg{};
-    DevBuf bead_tile, tile_count, tile_start, tile_cursor, sorted, scan_sums;
-    DevBuf rec_off, rec_w;  // per-bead window records in binned order
-    // Krylov workspace
-    int R = 10;
-    long ld = 0;
-    int n_cap = 0;
-    DevBuf V, w, b, Mb, x, partial, scal, yraw;
-    // near-field block preconditioner: beads grouped by body tag, per-body inverses
-    int block_nmax = 0;
-    DevBuf body_count, body_start, body_cursor, body_beads, ginv, uvec, inv_s;
-    // per-body segment scratch: bead counts and double sums (warm starts, learned features)
-    DevBuf seg_count, seg_sum;
-    DevBuf bd_force;  // Brownian step: bead forces plus steric repulsion
-    GmresState* g_dev = nullptr;
-    GmresState* g_host = nullptr;
-    double E0[3] = {0, 0, 0};
 
-    ~cr_context()
-    {
-        if (g_dev) (void)hipFree(g_dev);
-        if (g_host) (void)hipHostFree(g_host);
-    }
-
-    bool fp32() const { return params.precision == CR_PRECISION_FP32_BULK; }
-};
-
-namespace {
+namespace cr::impl {
 
 SysView make_view(cr_context* c, const cr_system* s)
 {
@@ -384,7 +336,6 @@ int block_setup(cr_context* c, const SysView& v, hipStream_t st)
     const size_t b1 = (size_t)std::max(1, B);
     CR_HIP(c->body_count.ensure(sizeof(unsigned) * (b1 + 1)));
     CR_HIP(c->body_start.ensure(sizeof(unsigned) * (b1 + 1)));
-    CR_HIP(c->body_cursor.ensure(sizeof(unsigned) * b1));
     CR_HIP(c->body_beads.ensure(sizeof(unsigned) * (size_t)std::max(1, v.N)));
     CR_HIP(c->ginv.ensure(sizeof(float) * b1 * nmax * nmax));
     CR_HIP(c->uvec.ensure(sizeof(float) * b1 * nmax));
@@ -392,7 +343,7 @@ int block_setup(cr_context* c, const SysView& v, hipStream_t st)
     CR_HIP(c->scan_sums.ensure(sizeof(unsigned long long) * std::max<size_t>((b1 + 1) / 1024 + 4,
                                                                              c->scan_sums.bytes / 8)));
     CR_HIP(launch_block_setup(v, c->mp.L, nmax, c->body_count.as<unsigned>(), c->body_start.as<unsigned>(),
-                              c->body_cursor.as<unsigned>(), c->body_beads.as<unsigned>(),
+                              c->body_beads.as<unsigned>(),
                               c->scan_sums.as<unsigned long long>(), c->ginv.as<float>(), c->uvec.as<float>(),
                               c->inv_s.as<double>(), st));
     return CR_OK;
@@ -546,7 +497,66 @@ hipError_t upload(DevBuf& buf, const std::vector<double>& h)
     return hipMemcpy(buf.p, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice);
 }
 
-}  // namespace
+int recip_any(cr_context* c, hipStream_t st) { return c->fp32() ? recip_impl<float>(c, st) : recip_impl<double>(c, st); }
+
+const AxisPlan* axis_plans(const cr_context* c) { return c->fp32() ? plans<float>(c) : plans<double>(c); }
+
+int force_pass_any(cr_context* c, const SysView& v, const double* q, double* force, hipStream_t st)
+{
+    return c->fp32() ? force_pass<float>(c, v, q, force, st) : force_pass<double>(c, v, q, force, st);
+}
+
+int init_x0(cr_context* c, const SysView& v, int x0_mode, const double* q_in, const double* phi_in, double* x,
+            hipStream_t st)
+{
+    const long n = (long)v.N + v.N_isolated;
+    if (x0_mode == CR_X0_ZERO) {
+        CR_HIP(launch_fill<double>(x, 0.0, n, st));
+    } else if (x0_mode == CR_X0_PREVIOUS) {
+        CR_HIP(hipMemcpyAsync(x, q_in, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
+        CR_HIP(hipMemcpyAsync(x + v.N, phi_in, sizeof(double) * v.N_isolated, hipMemcpyDeviceToDevice, st));
+    } else if (x0_mode == CR_X0_CHARGES) {
+        // q from the caller, shifted per body onto its net charge; Phi fitted to the
+        // equipotential rows given that q (one operator application)
+        unsigned* counts = nullptr;
+        double* sums = nullptr;
+        CR_RET(segments(c, v, st, &counts, &sums));
+        CR_HIP(hipMemcpyAsync(x, q_in, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
+        CR_HIP(launch_project_charges(v, counts, sums, x, st));
+        CR_HIP(launch_fill<double>(x + v.N, 0.0, v.N_isolated, st));
+        double* w = c->w.as<double>();
+        CR_RET(matvec_any(c, v, x, w, 0, st));
+        CR_HIP(launch_fit_potentials(v, counts, c->b.as<double>(), w, sums, x + v.N, st));
+    } else {
+        return set_error(CR_ERR_INVALID, "unknown x0 mode");
+    }
+    return CR_OK;
+}
+
+int true_residual(cr_context* c, const SysView& v, const double* x, cr_solve_stats* stats, hipStream_t st)
+{
+    // FP64 judge of the converged state from one operator application r = A x - b:
+    // ||D r||_2 and the unpreconditioned ||r|| / ||b||
+    const long n = (long)v.N + v.N_isolated;
+    double* w = c->w.as<double>();
+    double* dw = c->yraw.as<double>();
+    CR_RET(matvec_any(c, v, x, w, 0, st));
+    CR_HIP(launch_axpby(-1.0, c->b.as<double>(), 1.0, w, n, st));
+    CR_RET(apply_precond(c, v, w, dw, st));
+    CR_RET(norm2(c, dw, n, c->scal.as<double>(), st));
+    CR_RET(norm2(c, w, n, c->scal.as<double>() + 1, st));
+    CR_RET(norm2(c, c->b.as<double>(), n, c->scal.as<double>() + 2, st));
+    double rb[3] = {0, 0, 0};
+    CR_HIP(hipMemcpyAsync(rb, c->scal.as<double>(), 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    CR_HIP(hipStreamSynchronize(st));
+    stats->true_resid = std::sqrt(rb[0]);
+    stats->true_relres = rb[2] > 0 ? std::sqrt(rb[1] / rb[2]) : std::sqrt(rb[1]);
+    return CR_OK;
+}
+
+}  // namespace cr::impl
+
+using namespace cr::impl;
 
 extern "C" {
 
@@ -789,6 +799,7 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
     CR_HIP(c->partial.ensure(sizeof(double) * 32 * 1024));
     CR_HIP(c->scal.ensure(sizeof(double) * 64));
     c->ready = true;
+    ++c->generation;
     return CR_OK;
 }
 
@@ -856,48 +867,14 @@ int cr_compute_force(cr_context* c, const cr_system* s, double* q_inout, double*
     CR_HIP(launch_objective(v, c->mp.L, c->E0, c->b.as<double>(), st));
     if (block_precond(c)) CR_RET(block_setup(c, v, st));
     CR_RET(apply_precond(c, v, c->b.as<double>(), c->Mb.as<double>(), st));
-    if (x0_mode == CR_X0_ZERO) {
-        CR_HIP(launch_fill<double>(x, 0.0, n, st));
-    } else if (x0_mode == CR_X0_PREVIOUS) {
-        CR_HIP(hipMemcpyAsync(x, q_inout, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
-        CR_HIP(hipMemcpyAsync(x + v.N, phi_inout, sizeof(double) * v.N_isolated, hipMemcpyDeviceToDevice, st));
-    } else if (x0_mode != CR_X0_CHARGES) {
+    if (x0_mode != CR_X0_ZERO && x0_mode != CR_X0_PREVIOUS && x0_mode != CR_X0_CHARGES)
         return set_error(CR_ERR_INVALID, "unknown x0 mode");
-    }
     CR_RET(ensure_csr(c, v, st));
-    if (x0_mode == CR_X0_CHARGES) {
-        // q from the caller, shifted per body onto its net charge; Phi fitted to the
-        // equipotential rows given that q (one operator application)
-        unsigned* counts = nullptr;
-        double* sums = nullptr;
-        CR_RET(segments(c, v, st, &counts, &sums));
-        CR_HIP(hipMemcpyAsync(x, q_inout, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
-        CR_HIP(launch_project_charges(v, counts, sums, x, st));
-        CR_HIP(launch_fill<double>(x + v.N, 0.0, v.N_isolated, st));
-        double* w = c->w.as<double>();
-        CR_RET(matvec_any(c, v, x, w, 0, st));
-        CR_HIP(launch_fit_potentials(v, counts, c->b.as<double>(), w, sums, x + v.N, st));
-    }
+    CR_RET(init_x0(c, v, x0_mode, q_inout, phi_inout, x, st));
     CR_RET(gmres(c, v, stats, st));
     CR_HIP(hipMemcpyAsync(q_inout, x, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
     CR_HIP(hipMemcpyAsync(phi_inout, x + v.N, sizeof(double) * v.N_isolated, hipMemcpyDeviceToDevice, st));
-    if (stats) {
-        // FP64 judge of the converged state from one operator application r = A x - b:
-        // ||D r||_2 and the unpreconditioned ||r|| / ||b||
-        double* w = c->w.as<double>();
-        double* dw = c->yraw.as<double>();
-        CR_RET(matvec_any(c, v, x, w, 0, st));
-        CR_HIP(launch_axpby(-1.0, c->b.as<double>(), 1.0, w, n, st));
-        CR_RET(apply_precond(c, v, w, dw, st));
-        CR_RET(norm2(c, dw, n, c->scal.as<double>(), st));
-        CR_RET(norm2(c, w, n, c->scal.as<double>() + 1, st));
-        CR_RET(norm2(c, c->b.as<double>(), n, c->scal.as<double>() + 2, st));
-        double rb[3] = {0, 0, 0};
-        CR_HIP(hipMemcpyAsync(rb, c->scal.as<double>(), 3 * sizeof(double), hipMemcpyDeviceToHost, st));
-        CR_HIP(hipStreamSynchronize(st));
-        stats->true_resid = std::sqrt(rb[0]);
-        stats->true_relres = rb[2] > 0 ? std::sqrt(rb[1] / rb[2]) : std::sqrt(rb[1]);
-    }
+    if (stats) CR_RET(true_residual(c, v, x, stats, st));
     if (c->fp32())
         CR_RET(force_pass<float>(c, v, x, force, st));
     else
@@ -1201,6 +1178,7 @@ int cr_set_tiles(cr_context* c, const int tile[3], int lds_bytes)
     if (tiled_lds(c->mp, tg, sizeof(SpreadAcc), Tsz).chunk < 16)
         return set_error(CR_ERR_INVALID, "extended tile does not fit the LDS budget");
     c->tg = tg;
+    ++c->generation;
     return CR_OK;
 }
 
@@ -1274,6 +1252,7 @@ int cr_autotune(cr_context* c, const cr_system* s, cr_tuning* out, void* stream)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->tg = tried ? best_tg : keep;
+    ++c->generation;
     if (out) {
         for (int d = 0; d < 3; ++d) out->tile[d] = c->tg.T[d];
         out->lds_bytes = c->tg.lds;

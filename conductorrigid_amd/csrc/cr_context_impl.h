@@ -1,0 +1,84 @@
+// cr_context_impl.h -- the context behind the C ABI (cr_context.cpp) and the internal
+// operations the batched solver (cr_batch.cpp) composes from it.  Not part of the ABI.
+#pragma once
+
+#include "cr_host.h"
+#include "cr_kernels.h"
+
+using namespace cr;
+
+struct cr_context {
+    int device = 0;
+    bool ready = false;
+    cr_params params{};
+    cr_resolved res{};
+    MeshParams mp{};
+    RealParams rp{};
+    int Nzh = 0;
+    long G = 0, Gh = 0;
+    AxisPlan plan_d[3]{}, plan_f[3]{};
+    DevBuf tw_d[3], tw_f[3];
+    DevBuf ktab[3], etab[3];
+    ScaleTables sc{};
+    DevBuf tab_d, gtab_d, tab_f, gtab_f;
+    DevBuf grid_real, grid_cplx;
+    DevBuf phi_recip, iso;
+    // real-space operator assembled per force evaluation (CSR over the neighbour list rows)
+    bool csr_ready = false;
+    long rs_ldn = 0;
+    DevBuf rs_col, rs_val, rs_len;
+    // bead binning for the owner-computes spread
+    TileGeom tg{};
+    DevBuf bead_tile, tile_count, tile_start, tile_cursor, sorted, scan_sums;
+    DevBuf rec_off, rec_w;  // per-bead window records in binned order
+    // Krylov workspace
+    int R = 10;
+    long ld = 0;
+    int n_cap = 0;
+    DevBuf V, w, b, Mb, x, partial, scal, yraw;
+    // near-field block preconditioner: beads grouped by body tag, per-body inverses
+    int block_nmax = 0;
+    DevBuf body_count, body_start, body_beads, ginv, uvec, inv_s;
+    // per-body segment scratch: bead counts and double sums (warm starts, learned features)
+    DevBuf seg_count, seg_sum;
+    DevBuf bd_force;  // Brownian step: bead forces plus steric repulsion
+    GmresState* g_dev = nullptr;
+    GmresState* g_host = nullptr;
+    double E0[3] = {0, 0, 0};
+    unsigned long generation = 0;  // bumped whenever parameters, tables or tiling change
+
+    ~cr_context()
+    {
+        if (g_dev) (void)hipFree(g_dev);
+        if (g_host) (void)hipHostFree(g_host);
+    }
+
+    bool fp32() const { return params.precision == CR_PRECISION_FP32_BULK; }
+};
+
+namespace cr::impl {
+
+SysView make_view(cr_context* c, const cr_system* s);
+// Validate the system, refresh the body-tag map, rebin beads and rebuild window records.
+int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out);
+BinView bin_view(const cr_context* c);
+bool contract_tiled_ok(const cr_context* c, size_t elem);
+// Real-space ELL operator for the current positions (valid until the next prepare()).
+int ensure_csr(cr_context* c, const SysView& v, hipStream_t st);
+// grid <- Re IFFT(S FFT(grid)) on the context's mesh, in its precision.
+int recip_any(cr_context* c, hipStream_t st);
+// FFT plans of the context's precision (x, y, z).
+const AxisPlan* axis_plans(const cr_context* c);
+int matvec_any(cr_context* c, const SysView& v, const double* x, double* y, int precond, hipStream_t st);
+bool block_precond(const cr_context* c);
+int block_setup(cr_context* c, const SysView& v, hipStream_t st);
+int apply_precond(cr_context* c, const SysView& v, const double* in, double* out, hipStream_t st);
+// x (length N + B) from the x0 mode; uses the context's w as scratch for CR_X0_CHARGES.
+int init_x0(cr_context* c, const SysView& v, int x0_mode, const double* q_in, const double* phi_in, double* x,
+            hipStream_t st);
+// FP64 residuals of x against the context's objective b: ||D r|| and ||r|| / ||b||.
+int true_residual(cr_context* c, const SysView& v, const double* x, cr_solve_stats* stats, hipStream_t st);
+// Final force pass for charges q (spread, k-space, contract_force, real-space force + q E0).
+int force_pass_any(cr_context* c, const SysView& v, const double* q, double* force, hipStream_t st);
+
+}  // namespace cr::impl

@@ -106,6 +106,32 @@ TiledLds tiled_lds(const MeshParams& mp, const TileGeom& tg, size_t ext_elem, si
 template <typename T>
 hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const SysView& s, const BinView& bv,
                                  const double* q, const T* grid, T* phi, double* force, hipStream_t st);
+// One box of a batched binned spread / contraction (device array, tile0 ascending): its
+// mesh, tiling and binned bead records, and the arrays it reads and writes.
+struct TiledBox {
+    MeshParams mp;
+    TileGeom tg;
+    int ch;                      // staging chunk within the launch's LDS size
+    int tile0;                   // first global tile of this box
+    int box;                     // index into the skip flags
+    long G;                      // mesh nodes
+    const unsigned* tile_start;  // [ntiles + 1]
+    const unsigned* sorted;      // [N]
+    const int* rec_off;          // [N]
+    const void* rec_w;           // [N * 3P] (T)
+    const double* q;             // spread input, bead order
+    void* grid;                  // real mesh (T)
+    void* phi;                   // contraction output (T), bead order
+};
+// skip (device, may be null): boxes with skip[box] != 0 are left untouched.
+template <typename T>
+hipError_t launch_fill_batch(const TiledBox* boxes, int nbox, long Gmax, const int* skip, hipStream_t st);
+template <typename T>
+hipError_t launch_spread_tiled_batch(const TiledBox* boxes, int nbox, int total_tiles, int P, size_t lds_bytes,
+                                     const int* skip, hipStream_t st);
+template <typename T>
+hipError_t launch_contract_tiled_batch(const TiledBox* boxes, int nbox, int total_tiles, int P, size_t lds_bytes,
+                                       const int* skip, hipStream_t st);
 // ---- k_se.hip (unbinned fallbacks) ----
 template <typename T>
 hipError_t launch_contract(const MeshParams& mp, const SysView& s, const T* grid, T* phi, hipStream_t st);
@@ -140,7 +166,7 @@ hipError_t launch_fft_x_scale(const AxisPlan& px, int Nx, int Ny, int Zc, T* cpl
                               hipStream_t st);
 
 // ---- k_fft_reg.hip: register four-step passes for n = R * R, R in {8, 16, 32} ----
-int fftr_radix(int n);  // R, or 0 when n has no fast path
+int fftr_radix(int n);  // n when the register-resident passes take length n, else 0
 template <typename T>
 hipError_t launch_fftr_z_r2c(const AxisPlan& pz, long nrows, const T* real, T* cplx, hipStream_t st);
 template <typename T>
@@ -151,6 +177,11 @@ template <typename T>
 hipError_t launch_fftr_strided(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, T* data, int sign, hipStream_t st);
 template <typename T>
 hipError_t launch_fftr_x_scale(const AxisPlan& px, int Ny, int Zc, T* data, const ScaleTables& sc, hipStream_t st);
+// x pass + scale for nbox meshes stored back to back ([box][Nx][Ny][Zc], Nx = px.n), each
+// with its own scale tables sct[box] (device array); register-path lengths only.
+template <typename T>
+hipError_t launch_fftr_x_scale_batch(const AxisPlan& px, int nbox, int Ny, int Zc, T* data, const ScaleTables* sct,
+                                     hipStream_t st);
 
 // ---- k_real.hip ----
 template <typename T>
@@ -188,7 +219,7 @@ hipError_t launch_zero_force(int N, double* force, hipStream_t st);
 // ---- k_precond.hip ----
 // Near-field block preconditioner: per isolated body, the inverse of its own saddle block.
 hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, unsigned* body_count,
-                              unsigned* body_start, unsigned* cursor, unsigned* body_beads,
+                              unsigned* body_start, unsigned* body_beads,
                               unsigned long long* scan_sums, float* ginv, float* uvec, double* inv_s, hipStream_t st);
 hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_start, const unsigned* body_beads,
                               const float* ginv, const float* uvec, const double* inv_s, const double* y, double* w,
@@ -303,5 +334,79 @@ struct GmresState {
 hipError_t launch_givens(GmresState* g, int i, int R, hipStream_t st);
 hipError_t launch_cycle_init(GmresState* g, const double* norm2, hipStream_t st);
 hipError_t launch_backsub(GmresState* g, int R, hipStream_t st);
+
+// ---- k_batch.hip: a batch of independent boxes solved in lockstep ----
+// The unknown vectors [q (N_b) ; Phi (B_b)] of all boxes are concatenated (box b at
+// offset xoff); segmented launches give each box a contiguous run of workgroups (blk0
+// ascending over boxes) so every dot product stays within one box.  skip[b] != 0 (a
+// box that has converged) makes a box's workgroups return at once.
+struct BatchSeg {
+    long xoff;       // offset of the box's unknowns in the batch vectors
+    int n;           // N + B
+    int blk0, nblk;  // the box's workgroups in segmented vector launches
+};
+// One box of the batched operator rows: real-space ELL rows (the reciprocal potential
+// already contracted into phi_recip) and the per-body charge rows.
+struct RowBox {
+    long xoff;
+    int N, N_rigid, N_isolated, Nmax;
+    int blk0;    // first bead-row workgroup
+    int rblk0;   // first rigid-body-row workgroup
+    const unsigned* iso;
+    const unsigned* rigid_membership;
+    const unsigned* rigid_members;
+    const unsigned* rigid_size;
+    const void* phi_recip;  // T[N]
+    long ldn;
+    const unsigned* col;
+    const void* val;        // T[width * ldn]
+    const unsigned* row_len;
+    double selfterm;
+};
+// Per-box Krylov state of the batched solver.
+struct BatchGmres {
+    GmresState g;
+    double tol;      // tol * ||b|| (or ||D b||)
+    double norm2;    // scratch: squared norm reduced by launch_breduce(..., kRedNorm)
+};
+enum BatchRed { kRedH1 = 0, kRedH2 = 1, kRedWnorm = 2, kRedNorm = 3 };
+constexpr int kBatchBlock = 256;
+// Workgroups a box of n unknowns gets in segmented launches.
+int batch_blocks(long n);
+template <typename T>
+hipError_t launch_brows(const RowBox* boxes, int nbox, int bead_blocks, int rigid_blocks, const double* x, double* y,
+                        int precond, const int* skip, hipStream_t st);
+// partial[k * 32 + c] = sum over block k's range of V[c] . w for c < ncols, with
+// ncols = fixed_cols if > 0 else *step + 1 (device step counter).
+hipError_t launch_bdot(const double* V, long ld, int fixed_cols, const int* step, const double* w,
+                       const BatchSeg* segs, int nbox, int total_blocks, double* partial, const int* skip,
+                       hipStream_t st);
+hipError_t launch_breduce(const double* partial, const BatchSeg* segs, int nbox, int fixed_cols, const int* step,
+                          BatchGmres* gs, int which, const int* skip, hipStream_t st);
+// w -= V[:, 0:*step+1] h (h1 or h2 by `which`); mode 2 adds ||w||^2 partials in partial[k*32]
+hipError_t launch_bgs_update(const double* V, long ld, const int* step, double* w, const BatchSeg* segs, int nbox,
+                             int total_blocks, const BatchGmres* gs, int which, int mode, double* partial,
+                             const int* skip, hipStream_t st);
+// Givens step i = *step for every box not skipped; a box whose estimate meets its tol sets skip.
+hipError_t launch_bgivens(BatchGmres* gs, int nbox, const int* step, int R, int* skip, hipStream_t st);
+// V[:, col] = sign * scale * w per box; col = *step + 1 with scale 1/||w|| (col_fixed < 0),
+// or col_fixed with scale 1/beta.
+hipError_t launch_bscale_copy(const double* w, const BatchSeg* segs, int nbox, int total_blocks,
+                              const BatchGmres* gs, const int* step, int col_fixed, double sign, double* V, long ld,
+                              const int* skip, hipStream_t st);
+hipError_t launch_bcopy_col(const double* V, long ld, const int* step, double* u, long n, hipStream_t st);
+// Cycle start from the reduced ||w||^2: beta, rotations reset; skip[b] = beta <= tol for
+// active boxes (finished[b] != 0 boxes keep skip = 1).
+hipError_t launch_bcycle_init(BatchGmres* gs, int nbox, const int* finished, int* skip, hipStream_t st);
+hipError_t launch_bbacksub(BatchGmres* gs, int nbox, int R, const int* finished, hipStream_t st);
+hipError_t launch_bcombine(const double* V, long ld, const BatchSeg* segs, int nbox, int total_blocks,
+                           const BatchGmres* gs, const int* finished, double* x, hipStream_t st);
+// tol_b = rel_b * sqrt(norm2_b) from the last kRedNorm reduction.
+hipError_t launch_bset_tol(BatchGmres* gs, int nbox, const double* rel, hipStream_t st);
+// Publish pub[3b] = resid, pub[3b+1] = skip, pub[3b+2] = tol for the host; inc != 0 ends a
+// lockstep step (++*step).
+hipError_t launch_bstep_end(int* step, const BatchGmres* gs, const int* skip, int nbox, double* pub, int inc,
+                            hipStream_t st);
+hipError_t launch_bset_step(int* step, int value, hipStream_t st);
 
 }  // namespace cr

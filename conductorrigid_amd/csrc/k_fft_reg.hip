@@ -1,15 +1,17 @@
-// k_fft_reg.hip -- register-resident FFT passes for square pencil lengths n = R * R
-// (R = 8, 16, 32: meshes of 64, 256 or 1024 nodes per axis).
+// k_fft_reg.hip -- register-resident FFT passes for pencil lengths n = A * B with
+// A = B or A = 2B (powers of two, A <= 32): meshes of 32, 64, 128, 256, 512 or 1024
+// nodes per axis.
 //
-// Four-step decomposition (Bailey): with j = j2 + R j1 and k = k1 + R k2,
-//   X[k1 + R k2] = sum_j2 W_R^{j2 k2} [ W_n^{j2 k1} sum_j1 x[j2 + R j1] W_R^{j1 k1} ].
-// R threads own one pencil.  Thread j2 loads x[j2 + R j1] (j1 = 0..R-1; for a fixed j1
-// the R threads read R consecutive elements), runs an R-point DFT in registers with
+// Four-step decomposition (Bailey): with j = j2 + B j1 and k = k1 + A k2,
+//   X[k1 + A k2] = sum_j2 W_B^{j2 k2} [ W_n^{j2 k1} sum_j1 x[j2 + B j1] W_A^{j1 k1} ].
+// B threads own one pencil.  Thread j2 loads x[j2 + B j1] (j1 = 0..A-1; for a fixed j1
+// the B threads read B consecutive elements), runs an A-point DFT in registers with
 // compile-time roots, multiplies by the pass twiddle W_n^{j2 k1}, and hands the values
-// over through one padded LDS transpose; thread k1 then runs the second R-point DFT
-// and holds X[k1 + R k2].  One barrier per pass instead of two per radix stage.
-// The inverse from that layout (used by the C2R pass and the fused x pass) runs the
-// same steps in the other order, ending with thread b holding x[b + R a].
+// over through one padded LDS transpose; thread t then runs the B-point DFTs of
+// k1 = t, t + B (A / B of them) and holds X[k1 + A k2].  One barrier per pass instead of
+// two per radix stage.  The inverse from that layout (used by the C2R pass and the fused
+// x pass) runs the same steps in the other order, ending with thread t holding
+// x[t + B j1].
 //
 // Unnormalised, forward e^{-ikx}, inverse e^{+ikx}, as the generic passes in k_fft.hip
 // (which replace the reference's cuFFT plan, cppmodule/ConductorRigid.cc:228).
@@ -116,132 +118,157 @@ __device__ __forceinline__ T2 ptw(const T2* __restrict__ tw, int e)
     return w;
 }
 
-// Pencils (or columns) per workgroup: as many as fit ~40 KB of transpose buffer.
-template <typename T, int R>
+// Pencils (or columns) per workgroup for a pencil of B threads and an A x (B+1)
+// transpose buffer: as many as fit ~40 KB of LDS, at most 256 threads.
+template <typename T, int A, int B>
 constexpr int lanes_per_wg()
 {
-    constexpr int pad = R + 1;
-    constexpr int fit = 40960 / (int)(R * pad * 2 * sizeof(T));
-    constexpr int cap = 256 / R;
+    constexpr int fit = 40960 / (int)(A * (B + 1) * 2 * sizeof(T));
+    constexpr int cap = 256 / B;
     int p = 1;
     while (p * 2 <= fit && p * 2 <= cap) p *= 2;
     return p;
 }
 
 // ---- contiguous pencils (z) ---------------------------------------------------
-// Forward steps from x layout (thread j2 holds x[j2 + R j1]) to X layout (thread k1
-// holds X[k1 + R k2]).  buf: this pencil's R x (R+1) transpose buffer.
-template <int R, int SIGN, typename T2>
-__device__ __forceinline__ void four_step(T2 (&v)[R], T2* buf, const T2* __restrict__ tw, int j2, int k1)
+// n = A * B with A = m * B.  Forward: thread j2 (of B) holds x[j2 + B j1], j1 = 0..A-1;
+// an A-point DFT over j1 gives k1, the pass twiddle W_n^{j2 k1} is applied, and the
+// values go through a padded LDS transpose; thread t then runs m B-point DFTs over j2,
+// for k1 = t + B s (s = 0..m-1), and holds v[s B + k2] = X[(t + B s) + A k2].
+// buf: this pencil's A x (B+1) transpose buffer.
+template <int A, int B, int SIGN, typename T2>
+__device__ __forceinline__ void four_step(T2 (&v)[A], T2* buf, const T2* __restrict__ tw, int t)
 {
-    dft<R, SIGN>(v);
+    constexpr int m = A / B;
+    dft<A, SIGN>(v);
 #pragma unroll
-    for (int q = 0; q < R; ++q) buf[q * (R + 1) + j2] = q == 0 ? v[q] : cmul(v[q], ptw<SIGN>(tw, j2 * q));
+    for (int q = 0; q < A; ++q) buf[q * (B + 1) + t] = q == 0 ? v[q] : cmul(v[q], ptw<SIGN>(tw, t * q));
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = buf[k1 * (R + 1) + q];
-    dft<R, SIGN>(v);
+    for (int s = 0; s < m; ++s) {
+        T2 u[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) u[j] = buf[(t + B * s) * (B + 1) + j];
+        dft<B, SIGN>(u);
+#pragma unroll
+        for (int j = 0; j < B; ++j) v[s * B + j] = u[j];
+    }
 }
 
-// Inverse steps from X layout (thread k1 holds X[k1 + R k2]) to x layout (thread b
-// holds x[b + R a]).
-template <int R, typename T2>
-__device__ __forceinline__ void four_step_inv_from_X(T2 (&v)[R], T2* buf, const T2* __restrict__ tw, int k1, int b)
+// Inverse from the X layout of four_step (thread t holds v[s B + k2] = X[(t + B s) + A k2])
+// to the x layout (thread t holds x[t + B j1] in v[j1]):
+//   x[j2 + B j1] = sum_k1 W_A^{j1 k1} W_n^{j2 k1} sum_k2 X[k1 + A k2] W_B^{j2 k2}  (+ roots).
+template <int A, int B, typename T2>
+__device__ __forceinline__ void four_step_inv_from_X(T2 (&v)[A], T2* buf, const T2* __restrict__ tw, int t)
 {
-    dft<R, +1>(v);
+    constexpr int m = A / B;
 #pragma unroll
-    for (int q = 0; q < R; ++q) buf[q * (R + 1) + k1] = q == 0 ? v[q] : cmul(v[q], ptw<+1>(tw, q * k1));
+    for (int s = 0; s < m; ++s) {
+        T2 u[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) u[j] = v[s * B + j];
+        dft<B, +1>(u);
+        const int k1 = t + B * s;
+#pragma unroll
+        for (int j = 0; j < B; ++j) buf[k1 * (B + 1) + j] = (j == 0 || k1 == 0) ? u[j] : cmul(u[j], ptw<+1>(tw, j * k1));
+    }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = buf[b * (R + 1) + q];
-    dft<R, +1>(v);
+    for (int q = 0; q < A; ++q) v[q] = buf[q * (B + 1) + t];
+    dft<A, +1>(v);
 }
 
-template <typename T, int R>
-__global__ __launch_bounds__((R * lanes_per_wg<T, R>())) void k_fftr_z_r2c(long nrows, const T* __restrict__ real,
-                                                                         typename Cx<T>::t* __restrict__ cplx,
-                                                                         const typename Cx<T>::t* __restrict__ tw)
+template <typename T, int A, int B>
+__global__ __launch_bounds__((B * lanes_per_wg<T, A, B>())) void k_fftr_z_r2c(long nrows, const T* __restrict__ real,
+                                                                            typename Cx<T>::t* __restrict__ cplx,
+                                                                            const typename Cx<T>::t* __restrict__ tw)
 {
     using T2 = typename Cx<T>::t;
-    constexpr int n = R * R, nh = n / 2 + 1;
-    __shared__ T2 lds[lanes_per_wg<T, R>() * R * (R + 1)];
-    const int p = threadIdx.x / R, l = threadIdx.x % R;
-    const long row = (long)blockIdx.x * lanes_per_wg<T, R>() + p;
+    constexpr int n = A * B, nh = n / 2 + 1, m = A / B;
+    __shared__ T2 lds[lanes_per_wg<T, A, B>() * A * (B + 1)];
+    const int p = threadIdx.x / B, l = threadIdx.x % B;
+    const long row = (long)blockIdx.x * lanes_per_wg<T, A, B>() + p;
     const bool ok = row < nrows;
-    T2 v[R];
+    T2 v[A];
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-        v[q].x = ok ? real[row * n + l + R * q] : (T)0;
+    for (int q = 0; q < A; ++q) {
+        v[q].x = ok ? real[row * n + l + B * q] : (T)0;
         v[q].y = 0;
     }
-    four_step<R, -1>(v, lds + p * R * (R + 1), tw, l, l);
+    four_step<A, B, -1>(v, lds + p * A * (B + 1), tw, l);
     if (!ok) return;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-        const int k = l + R * q;
-        if (k < nh) cplx[row * nh + k] = v[q];
-    }
-}
-
-template <typename T, int R>
-__global__ __launch_bounds__((R * lanes_per_wg<T, R>())) void k_fftr_z_c2r(long nrows,
-                                                                         const typename Cx<T>::t* __restrict__ cplx,
-                                                                         T* __restrict__ real,
-                                                                         const typename Cx<T>::t* __restrict__ tw)
-{
-    using T2 = typename Cx<T>::t;
-    constexpr int n = R * R, nh = n / 2 + 1;
-    __shared__ T2 lds[lanes_per_wg<T, R>() * R * (R + 1)];
-    const int p = threadIdx.x / R, l = threadIdx.x % R;
-    const long row = (long)blockIdx.x * lanes_per_wg<T, R>() + p;
-    const bool ok = row < nrows;
-    T2 v[R];
+    for (int s = 0; s < m; ++s)
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-        const int k = l + R * q;
-        T2 z;
-        z.x = 0;
-        z.y = 0;
-        if (ok) {
-            if (k < nh) {
-                z = cplx[row * nh + k];
-            } else {  // Hermitian extension
-                z = cplx[row * nh + (n - k)];
-                z.y = -z.y;
-            }
+        for (int j = 0; j < B; ++j) {
+            const int k = l + B * s + A * j;
+            if (k < nh) cplx[row * nh + k] = v[s * B + j];
         }
-        v[q] = z;
-    }
-    four_step_inv_from_X<R>(v, lds + p * R * (R + 1), tw, l, l);
-    if (!ok) return;
-#pragma unroll
-    for (int q = 0; q < R; ++q) real[row * n + l + R * q] = v[q].x;
 }
 
-template <typename T, int R, int SIGN>
-__global__ __launch_bounds__((R * lanes_per_wg<T, R>())) void k_fftr_z_c2c(long nrows,
-                                                                         typename Cx<T>::t* __restrict__ data,
-                                                                         const typename Cx<T>::t* __restrict__ tw)
+template <typename T, int A, int B>
+__global__ __launch_bounds__((B * lanes_per_wg<T, A, B>())) void k_fftr_z_c2r(long nrows,
+                                                                            const typename Cx<T>::t* __restrict__ cplx,
+                                                                            T* __restrict__ real,
+                                                                            const typename Cx<T>::t* __restrict__ tw)
 {
     using T2 = typename Cx<T>::t;
-    constexpr int n = R * R;
-    __shared__ T2 lds[lanes_per_wg<T, R>() * R * (R + 1)];
-    const int p = threadIdx.x / R, l = threadIdx.x % R;
-    const long row = (long)blockIdx.x * lanes_per_wg<T, R>() + p;
+    constexpr int n = A * B, nh = n / 2 + 1, m = A / B;
+    __shared__ T2 lds[lanes_per_wg<T, A, B>() * A * (B + 1)];
+    const int p = threadIdx.x / B, l = threadIdx.x % B;
+    const long row = (long)blockIdx.x * lanes_per_wg<T, A, B>() + p;
     const bool ok = row < nrows;
-    T2 v[R];
+    T2 v[A];
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
+    for (int s = 0; s < m; ++s)
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const int k = l + B * s + A * j;
+            T2 z;
+            z.x = 0;
+            z.y = 0;
+            if (ok) {
+                if (k < nh) {
+                    z = cplx[row * nh + k];
+                } else {  // Hermitian extension
+                    z = cplx[row * nh + (n - k)];
+                    z.y = -z.y;
+                }
+            }
+            v[s * B + j] = z;
+        }
+    four_step_inv_from_X<A, B>(v, lds + p * A * (B + 1), tw, l);
+    if (!ok) return;
+#pragma unroll
+    for (int q = 0; q < A; ++q) real[row * n + l + B * q] = v[q].x;
+}
+
+template <typename T, int A, int B, int SIGN>
+__global__ __launch_bounds__((B * lanes_per_wg<T, A, B>())) void k_fftr_z_c2c(long nrows,
+                                                                            typename Cx<T>::t* __restrict__ data,
+                                                                            const typename Cx<T>::t* __restrict__ tw)
+{
+    using T2 = typename Cx<T>::t;
+    constexpr int n = A * B, m = A / B;
+    __shared__ T2 lds[lanes_per_wg<T, A, B>() * A * (B + 1)];
+    const int p = threadIdx.x / B, l = threadIdx.x % B;
+    const long row = (long)blockIdx.x * lanes_per_wg<T, A, B>() + p;
+    const bool ok = row < nrows;
+    T2 v[A];
+#pragma unroll
+    for (int q = 0; q < A; ++q) {
         T2 z;
         z.x = 0;
         z.y = 0;
-        if (ok) z = data[row * n + l + R * q];
+        if (ok) z = data[row * n + l + B * q];
         v[q] = z;
     }
-    four_step<R, SIGN>(v, lds + p * R * (R + 1), tw, l, l);
+    four_step<A, B, SIGN>(v, lds + p * A * (B + 1), tw, l);
     if (!ok) return;
 #pragma unroll
-    for (int q = 0; q < R; ++q) data[row * n + l + R * q] = v[q];
+    for (int s = 0; s < m; ++s)
+#pragma unroll
+        for (int j = 0; j < B; ++j) data[row * n + l + B * s + A * j] = v[s * B + j];
 }
 
 // ---- strided pencils (y: axis 1, x: axis 0) of [Nx][Ny][Zc] ---------------------
@@ -270,54 +297,74 @@ __device__ __forceinline__ Lines lines(int axis, int Ny, int Zc)
     return L;
 }
 
-template <int R, int C, int SIGN, typename T2>
-__device__ __forceinline__ void four_step_cols(T2 (&v)[R], T2* lds, const T2* __restrict__ tw, int c, int l)
+// Column forms of four_step / four_step_inv_from_X: thread (c, l), l = 0..B-1, holds
+// x[l + B j1] in v[j1] (x layout) or X[(l + B s) + A k2] in v[s B + k2] (X layout).
+template <int A, int B, int C, int SIGN, typename T2>
+__device__ __forceinline__ void four_step_cols(T2 (&v)[A], T2* lds, const T2* __restrict__ tw, int c, int l)
 {
-    dft<R, SIGN>(v);
+    constexpr int m = A / B;
+    dft<A, SIGN>(v);
 #pragma unroll
-    for (int q = 0; q < R; ++q) lds[(q * R + l) * C + c] = q == 0 ? v[q] : cmul(v[q], ptw<SIGN>(tw, l * q));
+    for (int q = 0; q < A; ++q) lds[(q * B + l) * C + c] = q == 0 ? v[q] : cmul(v[q], ptw<SIGN>(tw, l * q));
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = lds[(l * R + q) * C + c];
-    dft<R, SIGN>(v);
+    for (int s = 0; s < m; ++s) {
+        T2 u[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) u[j] = lds[((l + B * s) * B + j) * C + c];
+        dft<B, SIGN>(u);
+#pragma unroll
+        for (int j = 0; j < B; ++j) v[s * B + j] = u[j];
+    }
 }
 
-template <int R, int C, typename T2>
-__device__ __forceinline__ void four_step_cols_inv_from_X(T2 (&v)[R], T2* lds, const T2* __restrict__ tw, int c, int l)
+template <int A, int B, int C, typename T2>
+__device__ __forceinline__ void four_step_cols_inv_from_X(T2 (&v)[A], T2* lds, const T2* __restrict__ tw, int c, int l)
 {
-    dft<R, +1>(v);
+    constexpr int m = A / B;
 #pragma unroll
-    for (int q = 0; q < R; ++q) lds[(q * R + l) * C + c] = q == 0 ? v[q] : cmul(v[q], ptw<+1>(tw, q * l));
+    for (int s = 0; s < m; ++s) {
+        T2 u[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) u[j] = v[s * B + j];
+        dft<B, +1>(u);
+        const int k1 = l + B * s;
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+            lds[(k1 * B + j) * C + c] = (j == 0 || k1 == 0) ? u[j] : cmul(u[j], ptw<+1>(tw, j * k1));
+    }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = lds[(l * R + q) * C + c];
-    dft<R, +1>(v);
+    for (int q = 0; q < A; ++q) v[q] = lds[(q * B + l) * C + c];
+    dft<A, +1>(v);
 }
 
-template <typename T, int R, int SIGN>
-__global__ __launch_bounds__((R * lanes_per_wg<T, R>())) void k_fftr_strided(int axis, int Ny, int Zc,
-                                                                           typename Cx<T>::t* __restrict__ data,
-                                                                           const typename Cx<T>::t* __restrict__ tw)
+template <typename T, int A, int B, int SIGN>
+__global__ __launch_bounds__((B * lanes_per_wg<T, A, B>())) void k_fftr_strided(int axis, int Ny, int Zc,
+                                                                              typename Cx<T>::t* __restrict__ data,
+                                                                              const typename Cx<T>::t* __restrict__ tw)
 {
     using T2 = typename Cx<T>::t;
-    constexpr int C = lanes_per_wg<T, R>();
-    __shared__ T2 lds[C * R * R];
+    constexpr int C = lanes_per_wg<T, A, B>(), m = A / B;
+    __shared__ T2 lds[C * A * B];
     const Lines L = lines<C>(axis, Ny, Zc);
     const int c = threadIdx.x % C, l = threadIdx.x / C;
     const bool ok = L.z0 + c < Zc;
-    T2 v[R];
+    T2 v[A];
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
+    for (int q = 0; q < A; ++q) {
         T2 z;
         z.x = 0;
         z.y = 0;
-        if (ok) z = data[L.base + c + (long)(l + R * q) * L.stride];
+        if (ok) z = data[L.base + c + (long)(l + B * q) * L.stride];
         v[q] = z;
     }
-    four_step_cols<R, C, SIGN>(v, lds, tw, c, l);
+    four_step_cols<A, B, C, SIGN>(v, lds, tw, c, l);
     if (!ok) return;
 #pragma unroll
-    for (int q = 0; q < R; ++q) data[L.base + c + (long)(l + R * q) * L.stride] = v[q];
+    for (int s = 0; s < m; ++s)
+#pragma unroll
+        for (int j = 0; j < B; ++j) data[L.base + c + (long)(l + B * s + A * j) * L.stride] = v[s * B + j];
 }
 
 template <typename T>
@@ -329,136 +376,173 @@ __device__ __forceinline__ double sin_t<double>(double x) { return sin(x); }
 
 // Forward x FFT, S(k) = j0(|k|)^2 exp(-sum_d (1-eta_d) k_d^2/4xi^2)/|k|^2/G (S(0) = 0;
 // cppmodule/ConductorRigid.cc:253-268), inverse x FFT, on the half spectrum.
-template <typename T, int R>
-__global__ __launch_bounds__((R * lanes_per_wg<T, R>())) void k_fftr_x_scale(int Ny, int Zc,
-                                                                           typename Cx<T>::t* __restrict__ data,
-                                                                           const typename Cx<T>::t* __restrict__ tw,
-                                                                           ScaleTables sc)
+template <typename T, int A, int B>
+__device__ __forceinline__ void x_scale_pass(int Ny, int Zc, typename Cx<T>::t* __restrict__ data,
+                                             const typename Cx<T>::t* __restrict__ tw, const ScaleTables& sc)
 {
     using T2 = typename Cx<T>::t;
-    constexpr int C = lanes_per_wg<T, R>();
-    __shared__ T2 lds[C * R * R];
+    constexpr int C = lanes_per_wg<T, A, B>(), m = A / B;
+    __shared__ T2 lds[C * A * B];
     const Lines L = lines<C>(0, Ny, Zc);
     const int c = threadIdx.x % C, l = threadIdx.x / C;
     const int z = L.z0 + c;
     const bool ok = z < Zc;
-    T2 v[R];
+    T2 v[A];
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
+    for (int q = 0; q < A; ++q) {
         T2 u;
         u.x = 0;
         u.y = 0;
-        if (ok) u = data[L.base + c + (long)(l + R * q) * L.stride];
+        if (ok) u = data[L.base + c + (long)(l + B * q) * L.stride];
         v[q] = u;
     }
-    four_step_cols<R, C, -1>(v, lds, tw, c, l);
-    // thread (c, l) holds X[l + R q]: scale by S(kx, ky, kz)
+    four_step_cols<A, B, C, -1>(v, lds, tw, c, l);
+    // thread (c, l) holds X[(l + B s) + A k2] in v[s B + k2]: scale by S(kx, ky, kz)
     const double ky = sc.k[1][L.line], ey = sc.e[1][L.line];
     const double kz = ok ? sc.k[2][z] : 0.0, ez = ok ? sc.e[2][z] : 0.0;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-        const int x = l + R * q;
-        const T kx = (T)sc.k[0][x], kyt = (T)ky, kzt = (T)kz;
-        const T k2 = kx * kx + kyt * kyt + kzt * kzt;
-        T S = 0;
-        if (k2 > (T)0) {
-            const T kk = sqrt(k2);
-            const T j0 = sin_t<T>(kk) / kk;
-            S = j0 * j0 * (T)(sc.e[0][x] * ey * ez) / k2 * (T)sc.invG;
+    for (int s = 0; s < m; ++s)
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const int x = l + B * s + A * j;
+            const T kx = (T)sc.k[0][x], kyt = (T)ky, kzt = (T)kz;
+            const T k2 = kx * kx + kyt * kyt + kzt * kzt;
+            T S = 0;
+            if (k2 > (T)0) {
+                const T kk = sqrt(k2);
+                const T j0 = sin_t<T>(kk) / kk;
+                S = j0 * j0 * (T)(sc.e[0][x] * ey * ez) / k2 * (T)sc.invG;
+            }
+            v[s * B + j].x *= S;
+            v[s * B + j].y *= S;
         }
-        v[q].x *= S;
-        v[q].y *= S;
-    }
     __syncthreads();  // the transpose buffer is reused
-    four_step_cols_inv_from_X<R, C>(v, lds, tw, c, l);
+    four_step_cols_inv_from_X<A, B, C>(v, lds, tw, c, l);
     if (!ok) return;
 #pragma unroll
-    for (int q = 0; q < R; ++q) data[L.base + c + (long)(l + R * q) * L.stride] = v[q];
+    for (int q = 0; q < A; ++q) data[L.base + c + (long)(l + B * q) * L.stride] = v[q];
+}
+
+template <typename T, int A, int B>
+__global__ __launch_bounds__((B * lanes_per_wg<T, A, B>())) void k_fftr_x_scale(int Ny, int Zc,
+                                                                              typename Cx<T>::t* __restrict__ data,
+                                                                              const typename Cx<T>::t* __restrict__ tw,
+                                                                              ScaleTables sc)
+{
+    x_scale_pass<T, A, B>(Ny, Zc, data, tw, sc);
+}
+
+// Boxes of one mesh stored back to back ([box][Nx][Ny][Zc]): blockIdx.y picks the box
+// and its own scale tables.
+template <typename T, int A, int B>
+__global__ __launch_bounds__((B * lanes_per_wg<T, A, B>())) void k_fftr_x_scale_batch(
+    int Ny, int Zc, long box_stride, typename Cx<T>::t* __restrict__ data, const typename Cx<T>::t* __restrict__ tw,
+    const ScaleTables* __restrict__ sct)
+{
+    x_scale_pass<T, A, B>(Ny, Zc, data + blockIdx.y * box_stride, tw, sct[blockIdx.y]);
 }
 
 inline unsigned ublk(long n, int per) { return (unsigned)((n + per - 1) / per); }
 
-template <typename T, int R>
+template <typename T, int A, int B>
 hipError_t z_r2c(long nrows, const T* real, T* cplx, const void* tw, hipStream_t st)
 {
     using T2 = typename Cx<T>::t;
-    constexpr int P = lanes_per_wg<T, R>();
-    hipLaunchKernelGGL((k_fftr_z_r2c<T, R>), dim3(ublk(nrows, P)), dim3(R * P), 0, st, nrows, real, (T2*)cplx,
+    constexpr int P = lanes_per_wg<T, A, B>();
+    hipLaunchKernelGGL((k_fftr_z_r2c<T, A, B>), dim3(ublk(nrows, P)), dim3(B * P), 0, st, nrows, real, (T2*)cplx,
                        (const T2*)tw);
     return hipGetLastError();
 }
 
-template <typename T, int R>
+template <typename T, int A, int B>
 hipError_t z_c2r(long nrows, const T* cplx, T* real, const void* tw, hipStream_t st)
 {
     using T2 = typename Cx<T>::t;
-    constexpr int P = lanes_per_wg<T, R>();
-    hipLaunchKernelGGL((k_fftr_z_c2r<T, R>), dim3(ublk(nrows, P)), dim3(R * P), 0, st, nrows, (const T2*)cplx, real,
-                       (const T2*)tw);
+    constexpr int P = lanes_per_wg<T, A, B>();
+    hipLaunchKernelGGL((k_fftr_z_c2r<T, A, B>), dim3(ublk(nrows, P)), dim3(B * P), 0, st, nrows, (const T2*)cplx,
+                       real, (const T2*)tw);
     return hipGetLastError();
 }
 
-template <typename T, int R>
+template <typename T, int A, int B>
 hipError_t z_c2c(long nrows, T* data, int sign, const void* tw, hipStream_t st)
 {
     using T2 = typename Cx<T>::t;
-    constexpr int P = lanes_per_wg<T, R>();
+    constexpr int P = lanes_per_wg<T, A, B>();
     if (sign < 0)
-        hipLaunchKernelGGL((k_fftr_z_c2c<T, R, -1>), dim3(ublk(nrows, P)), dim3(R * P), 0, st, nrows, (T2*)data,
+        hipLaunchKernelGGL((k_fftr_z_c2c<T, A, B, -1>), dim3(ublk(nrows, P)), dim3(B * P), 0, st, nrows, (T2*)data,
                            (const T2*)tw);
     else
-        hipLaunchKernelGGL((k_fftr_z_c2c<T, R, +1>), dim3(ublk(nrows, P)), dim3(R * P), 0, st, nrows, (T2*)data,
+        hipLaunchKernelGGL((k_fftr_z_c2c<T, A, B, +1>), dim3(ublk(nrows, P)), dim3(B * P), 0, st, nrows, (T2*)data,
                            (const T2*)tw);
     return hipGetLastError();
 }
 
-template <typename T, int R>
+template <typename T, int A, int B>
 hipError_t strided(int axis, int Nx, int Ny, int Zc, T* data, int sign, const void* tw, hipStream_t st)
 {
     using T2 = typename Cx<T>::t;
-    constexpr int C = lanes_per_wg<T, R>();
+    constexpr int C = lanes_per_wg<T, A, B>();
     const int lines_n = axis == 1 ? Nx : Ny;
     const unsigned blocks = (unsigned)(lines_n * ((Zc + C - 1) / C));
     if (sign < 0)
-        hipLaunchKernelGGL((k_fftr_strided<T, R, -1>), dim3(blocks), dim3(R * C), 0, st, axis, Ny, Zc, (T2*)data,
+        hipLaunchKernelGGL((k_fftr_strided<T, A, B, -1>), dim3(blocks), dim3(B * C), 0, st, axis, Ny, Zc, (T2*)data,
                            (const T2*)tw);
     else
-        hipLaunchKernelGGL((k_fftr_strided<T, R, +1>), dim3(blocks), dim3(R * C), 0, st, axis, Ny, Zc, (T2*)data,
+        hipLaunchKernelGGL((k_fftr_strided<T, A, B, +1>), dim3(blocks), dim3(B * C), 0, st, axis, Ny, Zc, (T2*)data,
                            (const T2*)tw);
     return hipGetLastError();
 }
 
-template <typename T, int R>
+template <typename T, int A, int B>
 hipError_t x_scale(int Ny, int Zc, T* data, const void* tw, const ScaleTables& sc, hipStream_t st)
 {
     using T2 = typename Cx<T>::t;
-    constexpr int C = lanes_per_wg<T, R>();
+    constexpr int C = lanes_per_wg<T, A, B>();
     const unsigned blocks = (unsigned)(Ny * ((Zc + C - 1) / C));
-    hipLaunchKernelGGL((k_fftr_x_scale<T, R>), dim3(blocks), dim3(R * C), 0, st, Ny, Zc, (T2*)data, (const T2*)tw, sc);
+    hipLaunchKernelGGL((k_fftr_x_scale<T, A, B>), dim3(blocks), dim3(B * C), 0, st, Ny, Zc, (T2*)data,
+                       (const T2*)tw, sc);
+    return hipGetLastError();
+}
+
+template <typename T, int A, int B>
+hipError_t x_scale_batch(int nbox, int Ny, int Zc, T* data, const void* tw, const ScaleTables* sct, hipStream_t st)
+{
+    using T2 = typename Cx<T>::t;
+    constexpr int C = lanes_per_wg<T, A, B>();
+    const unsigned blocks = (unsigned)(Ny * ((Zc + C - 1) / C));
+    const long stride = (long)(A * B) * Ny * Zc;
+    hipLaunchKernelGGL((k_fftr_x_scale_batch<T, A, B>), dim3(blocks, (unsigned)nbox), dim3(B * C), 0, st, Ny, Zc,
+                       stride, (T2*)data, (const T2*)tw, sct);
     return hipGetLastError();
 }
 
 }  // namespace
 
+// Register path lengths n = A * B (A = B or A = 2B, both powers of two, A <= 32):
+// 32 = 8 x 4, 64 = 8 x 8, 128 = 16 x 8, 256 = 16 x 16, 512 = 32 x 16, 1024 = 32 x 32.
 int fftr_radix(int n)
 {
-    for (int R : {8, 16, 32})
-        if (R * R == n) return R;
+    for (int v : {32, 64, 128, 256, 512, 1024})
+        if (v == n) return n;
     return 0;
 }
 
 #define CR_FFTR_DISPATCH(n, CALL)                       \
     switch (fftr_radix(n)) {                            \
-        case 8: return CALL(8);                         \
-        case 16: return CALL(16);                       \
-        case 32: return CALL(32);                       \
+        case 32: return CALL(8, 4);                     \
+        case 64: return CALL(8, 8);                     \
+        case 128: return CALL(16, 8);                   \
+        case 256: return CALL(16, 16);                  \
+        case 512: return CALL(32, 16);                  \
+        case 1024: return CALL(32, 32);                 \
         default: return hipErrorInvalidValue;           \
     }
 
 template <typename T>
 hipError_t launch_fftr_z_r2c(const AxisPlan& pz, long nrows, const T* real, T* cplx, hipStream_t st)
 {
-#define CALL(R) z_r2c<T, R>(nrows, real, cplx, pz.tw, st)
+#define CALL(A, B) z_r2c<T, A, B>(nrows, real, cplx, pz.tw, st)
     CR_FFTR_DISPATCH(pz.n, CALL)
 #undef CALL
 }
@@ -466,7 +550,7 @@ hipError_t launch_fftr_z_r2c(const AxisPlan& pz, long nrows, const T* real, T* c
 template <typename T>
 hipError_t launch_fftr_z_c2r(const AxisPlan& pz, long nrows, const T* cplx, T* real, hipStream_t st)
 {
-#define CALL(R) z_c2r<T, R>(nrows, cplx, real, pz.tw, st)
+#define CALL(A, B) z_c2r<T, A, B>(nrows, cplx, real, pz.tw, st)
     CR_FFTR_DISPATCH(pz.n, CALL)
 #undef CALL
 }
@@ -474,7 +558,7 @@ hipError_t launch_fftr_z_c2r(const AxisPlan& pz, long nrows, const T* cplx, T* r
 template <typename T>
 hipError_t launch_fftr_z_c2c(const AxisPlan& pz, long nrows, T* data, int sign, hipStream_t st)
 {
-#define CALL(R) z_c2c<T, R>(nrows, data, sign, pz.tw, st)
+#define CALL(A, B) z_c2c<T, A, B>(nrows, data, sign, pz.tw, st)
     CR_FFTR_DISPATCH(pz.n, CALL)
 #undef CALL
 }
@@ -482,7 +566,7 @@ hipError_t launch_fftr_z_c2c(const AxisPlan& pz, long nrows, T* data, int sign, 
 template <typename T>
 hipError_t launch_fftr_strided(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, T* data, int sign, hipStream_t st)
 {
-#define CALL(R) strided<T, R>(axis, Nx, Ny, Zc, data, sign, pa.tw, st)
+#define CALL(A, B) strided<T, A, B>(axis, Nx, Ny, Zc, data, sign, pa.tw, st)
     CR_FFTR_DISPATCH(pa.n, CALL)
 #undef CALL
 }
@@ -490,7 +574,16 @@ hipError_t launch_fftr_strided(const AxisPlan& pa, int axis, int Nx, int Ny, int
 template <typename T>
 hipError_t launch_fftr_x_scale(const AxisPlan& px, int Ny, int Zc, T* data, const ScaleTables& sc, hipStream_t st)
 {
-#define CALL(R) x_scale<T, R>(Ny, Zc, data, px.tw, sc, st)
+#define CALL(A, B) x_scale<T, A, B>(Ny, Zc, data, px.tw, sc, st)
+    CR_FFTR_DISPATCH(px.n, CALL)
+#undef CALL
+}
+
+template <typename T>
+hipError_t launch_fftr_x_scale_batch(const AxisPlan& px, int nbox, int Ny, int Zc, T* data, const ScaleTables* sct,
+                                     hipStream_t st)
+{
+#define CALL(A, B) x_scale_batch<T, A, B>(nbox, Ny, Zc, data, px.tw, sct, st)
     CR_FFTR_DISPATCH(px.n, CALL)
 #undef CALL
 }
@@ -500,7 +593,9 @@ hipError_t launch_fftr_x_scale(const AxisPlan& px, int Ny, int Zc, T* data, cons
     template hipError_t launch_fftr_z_c2r<T>(const AxisPlan&, long, const T*, T*, hipStream_t);                \
     template hipError_t launch_fftr_z_c2c<T>(const AxisPlan&, long, T*, int, hipStream_t);                     \
     template hipError_t launch_fftr_strided<T>(const AxisPlan&, int, int, int, int, T*, int, hipStream_t);     \
-    template hipError_t launch_fftr_x_scale<T>(const AxisPlan&, int, int, T*, const ScaleTables&, hipStream_t);
+    template hipError_t launch_fftr_x_scale<T>(const AxisPlan&, int, int, T*, const ScaleTables&, hipStream_t); \
+    template hipError_t launch_fftr_x_scale_batch<T>(const AxisPlan&, int, int, int, T*, const ScaleTables*,     \
+                                                     hipStream_t);
 CR_FFTR_INST(float)
 CR_FFTR_INST(double)
 

@@ -39,31 +39,23 @@ __global__ void k_body_count(int N, const unsigned* __restrict__ iso, unsigned* 
     if (i < N) atomicAdd(&count[iso[i]], 1u);
 }
 
-__global__ void k_body_fill(int N, const unsigned* __restrict__ iso, const unsigned* __restrict__ start,
-                            unsigned* __restrict__ cursor, unsigned* __restrict__ beads)
+// Members of each body in a fixed order, without atomics: a rigid body's beads in its
+// rigid_members order (the rigid body is one isolated body, tag iso[first member]), a
+// bead outside every rigid body alone in its own body.
+__global__ void k_body_fill_rigid(SysView s, const unsigned* __restrict__ start, unsigned* __restrict__ beads)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const unsigned t = iso[i];
-    beads[start[t] + atomicAdd(&cursor[t], 1u)] = (unsigned)i;
+    const long slot = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long r = slot / s.Nmax;
+    const int k = (int)(slot - r * s.Nmax);
+    if (r >= s.N_rigid || k >= (int)s.rigid_size[r]) return;
+    const unsigned* mem = s.rigid_members + r * s.Nmax;
+    beads[start[s.iso[mem[0]]] + k] = mem[k];
 }
 
-// members of each body in increasing bead index (deterministic block layout)
-__global__ void k_body_sort(int B, const unsigned* __restrict__ start, unsigned* __restrict__ beads)
+__global__ void k_body_fill_single(SysView s, const unsigned* __restrict__ start, unsigned* __restrict__ beads)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= B) return;
-    unsigned* m = beads + start[t];
-    const unsigned n = start[t + 1] - start[t];
-    for (unsigned a = 1; a < n; ++a) {
-        const unsigned v = m[a];
-        unsigned b = a;
-        while (b > 0 && m[b - 1] > v) {
-            m[b] = m[b - 1];
-            --b;
-        }
-        m[b] = v;
-    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < s.N && s.rigid_membership[i] == NOT_MEMBER) beads[start[s.iso[i]]] = (unsigned)i;
 }
 
 // One 64-thread workgroup per body; thread i owns row i of [G | I] in LDS.
@@ -140,11 +132,12 @@ __device__ __forceinline__ double lane_bcast(double v, int l)
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
-// Bodies of at most NB <= 32 beads: one wavefront per body, lane i holds row i of G in
+// Bodies of at most NB <= 64 beads: one wavefront per body, lane i holds row i of G in
 // registers and the inverse is formed in place by Gauss-Jordan (no pivoting: G is
-// symmetric positive definite), pivot rows broadcast with readlane -- no LDS, no
-// barriers.  Rows and columns n..NB-1 are padded with the identity, which the
-// elimination leaves untouched.  Four bodies per 256-thread workgroup.
+// symmetric positive definite), the pivot row broadcast element by element with
+// readlane -- no LDS, no barriers.  Rows and columns n..NB-1 are padded with the
+// identity, which the elimination leaves untouched.  Four bodies per 256-thread
+// workgroup.
 template <int NB>
 __global__ __launch_bounds__(256) void k_block_setup_wave(int B, int nmax, const unsigned* __restrict__ start,
                                                           const unsigned* __restrict__ beads,
@@ -183,17 +176,15 @@ __global__ __launch_bounds__(256) void k_block_setup_wave(int B, int nmax, const
     }
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        double pk[NB];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) pk[j] = lane_bcast(a[j], k);
-        const double inv = 1.0 / pk[k];
+        const double inv = 1.0 / lane_bcast(a[k], k);
         const double f = a[k];
-        if (i == k) {
+        const bool pivot = i == k;
 #pragma unroll
-            for (int j = 0; j < NB; ++j) a[j] = (j == k) ? inv : pk[j] * inv;
-        } else {
-#pragma unroll
-            for (int j = 0; j < NB; ++j) a[j] = (j == k) ? -f * inv : a[j] - f * inv * pk[j];
+        for (int j = 0; j < NB; ++j) {
+            // lane k's a[j] before this step (each j is read before it is overwritten)
+            const double pkj = lane_bcast(a[j], k);
+            if (j == k) a[j] = pivot ? inv : -f * inv;
+            else a[j] = pivot ? pkj * inv : a[j] - f * inv * pkj;
         }
     }
     double u = 0.0;
@@ -263,20 +254,20 @@ __global__ void k_zero_u32(unsigned* p, long n)
 }  // namespace
 
 hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, unsigned* body_count,
-                              unsigned* body_start, unsigned* cursor, unsigned* body_beads,
+                              unsigned* body_start, unsigned* body_beads,
                               unsigned long long* scan_sums, float* ginv, float* uvec, double* inv_s, hipStream_t st)
 {
     const int B = s.N_isolated;
     if (B <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_zero_u32, dim3(blocks(B + 1, 256)), dim3(256), 0, st, body_count, (long)B + 1);
-    hipLaunchKernelGGL(k_zero_u32, dim3(blocks(B, 256)), dim3(256), 0, st, cursor, (long)B);
     if (s.N > 0) hipLaunchKernelGGL(k_body_count, dim3(blocks(s.N, 256)), dim3(256), 0, st, s.N, s.iso, body_count);
     hipError_t e = exclusive_scan_u32(body_count, body_start, (long)B + 1, scan_sums, st);
     if (e != hipSuccess) return e;
+    if (s.N_rigid > 0)
+        hipLaunchKernelGGL(k_body_fill_rigid, dim3(blocks((long)s.N_rigid * s.Nmax, 256)), dim3(256), 0, st, s,
+                           body_start, body_beads);
     if (s.N > 0)
-        hipLaunchKernelGGL(k_body_fill, dim3(blocks(s.N, 256)), dim3(256), 0, st, s.N, s.iso, body_start, cursor,
-                           body_beads);
-    hipLaunchKernelGGL(k_body_sort, dim3(blocks(B, 256)), dim3(256), 0, st, B, body_start, body_beads);
+        hipLaunchKernelGGL(k_body_fill_single, dim3(blocks(s.N, 256)), dim3(256), 0, st, s, body_start, body_beads);
     const unsigned wb = (unsigned)((B + 3) / 4);
 #define CR_BLOCK_WAVE(NB)                                                                                   \
     hipLaunchKernelGGL(k_block_setup_wave<NB>, dim3(wb), dim3(256), 0, st, B, nmax, body_start, body_beads, \
@@ -289,6 +280,10 @@ hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, uns
         CR_BLOCK_WAVE(24);
     } else if (nmax <= 32) {
         CR_BLOCK_WAVE(32);
+    } else if (nmax <= 40) {
+        CR_BLOCK_WAVE(40);
+    } else if (nmax <= 64) {
+        CR_BLOCK_WAVE(64);
     } else {
         const size_t lds = sizeof(double) * (size_t)nmax * 2 * nmax;
         hipLaunchKernelGGL(k_block_setup, dim3(B), dim3(64), lds, st, nmax, body_start, body_beads, s.pos, L[0],

@@ -79,10 +79,10 @@ struct TileCoords {
     int ys, xs, esz;  // LDS z stride, x stride, elements of the (padded) extended tile
 };
 
-__device__ __forceinline__ TileCoords tile_coords(const MeshParams& mp, const TileGeom& tg)
+__device__ __forceinline__ TileCoords tile_coords(const MeshParams& mp, const TileGeom& tg, int t)
 {
     TileCoords c;
-    c.t = xcd_tile(blockIdx.x, tg.ntiles);
+    c.t = t;
     const int tz = c.t % tg.nt[2], ty = (c.t / tg.nt[2]) % tg.nt[1], tx = c.t / (tg.nt[2] * tg.nt[1]);
     c.x0 = tx * tg.T[0];
     c.y0 = ty * tg.T[1];
@@ -248,14 +248,13 @@ struct BoxWalk {
 // The LDS tile accumulates in A = double for both mesh precisions: ds_add_f64 measures
 // several times the throughput of ds_add_f32 on gfx950, and the sums gain accuracy.
 template <typename T, typename A, int PT>
-__global__ __launch_bounds__(kTiledThreads) void k_spread_tiled(MeshParams mp, TileGeom tg, int ch,
-                                                      const unsigned* __restrict__ tstart,
-                                                      const unsigned* __restrict__ sorted,
-                                                      const int* __restrict__ rec_off, const T* __restrict__ rec_w,
-                                                      const double* __restrict__ q, T* __restrict__ grid)
+__device__ __forceinline__ void spread_tile(const MeshParams& mp, const TileGeom& tg, int t, int ch,
+                                            const unsigned* __restrict__ tstart,
+                                            const unsigned* __restrict__ sorted, const int* __restrict__ rec_off,
+                                            const T* __restrict__ rec_w, const double* __restrict__ q,
+                                            T* __restrict__ grid, char* smem_raw)
 {
-    extern __shared__ char smem_raw[];
-    const TileCoords tc = tile_coords(mp, tg);
+    const TileCoords tc = tile_coords(mp, tg, t);
     constexpr int PM = PT > 0 ? PT : 16;
     const int P = PT > 0 ? PT : mp.P;
     const int esz = tc.esz;
@@ -299,17 +298,27 @@ __global__ __launch_bounds__(kTiledThreads) void k_spread_tiled(MeshParams mp, T
     }
 }
 
-template <typename T, bool FORCE, int PT>
-__global__ __launch_bounds__(kTiledThreads) void k_contract_tiled(MeshParams mp, TileGeom tg, int ch,
-                                                        const unsigned* __restrict__ tstart,
-                                                        const unsigned* __restrict__ sorted,
-                                                        const int* __restrict__ rec_off, const T* __restrict__ rec_w,
-                                                        const double4* __restrict__ pos, const double* __restrict__ q,
-                                                        const T* __restrict__ grid, T* __restrict__ phi,
-                                                        double* __restrict__ force)
+template <typename T, typename A, int PT>
+__global__ __launch_bounds__(kTiledThreads) void k_spread_tiled(MeshParams mp, TileGeom tg, int ch,
+                                                      const unsigned* __restrict__ tstart,
+                                                      const unsigned* __restrict__ sorted,
+                                                      const int* __restrict__ rec_off, const T* __restrict__ rec_w,
+                                                      const double* __restrict__ q, T* __restrict__ grid)
 {
     extern __shared__ char smem_raw[];
-    const TileCoords tc = tile_coords(mp, tg);
+    spread_tile<T, A, PT>(mp, tg, xcd_tile(blockIdx.x, tg.ntiles), ch, tstart, sorted, rec_off, rec_w, q, grid,
+                          smem_raw);
+}
+
+template <typename T, bool FORCE, int PT>
+__device__ __forceinline__ void contract_tile(const MeshParams& mp, const TileGeom& tg, int t, int ch,
+                                              const unsigned* __restrict__ tstart,
+                                              const unsigned* __restrict__ sorted, const int* __restrict__ rec_off,
+                                              const T* __restrict__ rec_w, const double4* __restrict__ pos,
+                                              const double* __restrict__ q, const T* __restrict__ grid,
+                                              T* __restrict__ phi, double* __restrict__ force, char* smem_raw)
+{
+    const TileCoords tc = tile_coords(mp, tg, t);
     constexpr int PM = PT > 0 ? PT : 16;
     const int P = PT > 0 ? PT : mp.P;
     const int esz = tc.esz;
@@ -387,6 +396,69 @@ __global__ __launch_bounds__(kTiledThreads) void k_contract_tiled(MeshParams mp,
         }
         __syncthreads();
     }
+}
+
+template <typename T, bool FORCE, int PT>
+__global__ __launch_bounds__(kTiledThreads) void k_contract_tiled(MeshParams mp, TileGeom tg, int ch,
+                                                        const unsigned* __restrict__ tstart,
+                                                        const unsigned* __restrict__ sorted,
+                                                        const int* __restrict__ rec_off, const T* __restrict__ rec_w,
+                                                        const double4* __restrict__ pos, const double* __restrict__ q,
+                                                        const T* __restrict__ grid, T* __restrict__ phi,
+                                                        double* __restrict__ force)
+{
+    extern __shared__ char smem_raw[];
+    contract_tile<T, FORCE, PT>(mp, tg, xcd_tile(blockIdx.x, tg.ntiles), ch, tstart, sorted, rec_off, rec_w, pos, q,
+                                grid, phi, force, smem_raw);
+}
+
+// Batched launches: one workgroup per tile of any box.  Boxes own consecutive ranges of
+// the global tile index (tile0 ascending); the XCD mapping runs over the global index, so
+// each XCD takes a contiguous run of tiles that stays within one box where it can.
+__device__ __forceinline__ int batch_box(const TiledBox* __restrict__ boxes, int nbox, int g)
+{
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (boxes[mid].tile0 <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T>
+__global__ void k_fill_batch(const TiledBox* __restrict__ boxes, const int* __restrict__ skip)
+{
+    const TiledBox& d = boxes[blockIdx.y];
+    if (skip && skip[d.box]) return;
+    T* g = static_cast<T*>(d.grid);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < d.G; i += (long)gridDim.x * blockDim.x)
+        g[i] = (T)0;
+}
+
+template <typename T, typename A, int PT>
+__global__ __launch_bounds__(kTiledThreads) void k_spread_tiled_batch(const TiledBox* __restrict__ boxes, int nbox,
+                                                                      int total, const int* __restrict__ skip)
+{
+    extern __shared__ char smem_raw[];
+    const int g = xcd_tile(blockIdx.x, total);
+    const TiledBox& d = boxes[batch_box(boxes, nbox, g)];
+    if (skip && skip[d.box]) return;
+    spread_tile<T, A, PT>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off,
+                          static_cast<const T*>(d.rec_w), d.q, static_cast<T*>(d.grid), smem_raw);
+}
+
+template <typename T, int PT>
+__global__ __launch_bounds__(kTiledThreads) void k_contract_tiled_batch(const TiledBox* __restrict__ boxes, int nbox,
+                                                                        int total, const int* __restrict__ skip)
+{
+    extern __shared__ char smem_raw[];
+    const int g = xcd_tile(blockIdx.x, total);
+    const TiledBox& d = boxes[batch_box(boxes, nbox, g)];
+    if (skip && skip[d.box]) return;
+    contract_tile<T, false, PT>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off,
+                                static_cast<const T*>(d.rec_w), nullptr, nullptr, static_cast<const T*>(d.grid),
+                                static_cast<T*>(d.phi), nullptr, smem_raw);
 }
 
 inline unsigned blocks(long n, int b)
@@ -511,13 +583,85 @@ hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_fill_batch(const TiledBox* boxes, int nbox, long Gmax, const int* skip, hipStream_t st)
+{
+    if (nbox <= 0) return hipSuccess;
+    long nb = (Gmax + 255) / 256;
+    if (nb > 2048) nb = 2048;
+    hipLaunchKernelGGL(k_fill_batch<T>, dim3((unsigned)std::max(1L, nb), (unsigned)nbox), dim3(256), 0, st, boxes,
+                       skip);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_spread_tiled_batch(const TiledBox* boxes, int nbox, int total_tiles, int P, size_t lds_bytes,
+                                     const int* skip, hipStream_t st)
+{
+    if (nbox <= 0 || total_tiles <= 0) return hipSuccess;
+    if (P > 16) return hipErrorInvalidValue;
+#define CR_SPREAD_B(PV)                                                                                         \
+    do {                                                                                                        \
+        if (lds_bytes > 65536)                                                                                  \
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_tiled_batch<T, SpreadAcc, PV>),  \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);              \
+        hipLaunchKernelGGL((k_spread_tiled_batch<T, SpreadAcc, PV>), dim3(total_tiles), dim3(kTiledThreads),    \
+                           lds_bytes, st, boxes, nbox, total_tiles, skip);                                      \
+    } while (0)
+    switch (P) {
+        case 4: CR_SPREAD_B(4); break;
+        case 5: CR_SPREAD_B(5); break;
+        case 6: CR_SPREAD_B(6); break;
+        case 7: CR_SPREAD_B(7); break;
+        case 8: CR_SPREAD_B(8); break;
+        case 9: CR_SPREAD_B(9); break;
+        case 10: CR_SPREAD_B(10); break;
+        default: CR_SPREAD_B(0); break;
+    }
+#undef CR_SPREAD_B
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_contract_tiled_batch(const TiledBox* boxes, int nbox, int total_tiles, int P, size_t lds_bytes,
+                                       const int* skip, hipStream_t st)
+{
+    if (nbox <= 0 || total_tiles <= 0) return hipSuccess;
+    if (P > 16) return hipErrorInvalidValue;
+#define CR_CONTRACT_B(PV)                                                                                       \
+    do {                                                                                                        \
+        if (lds_bytes > 65536)                                                                                  \
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract_tiled_batch<T, PV>),           \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);              \
+        hipLaunchKernelGGL((k_contract_tiled_batch<T, PV>), dim3(total_tiles), dim3(kTiledThreads), lds_bytes,  \
+                           st, boxes, nbox, total_tiles, skip);                                                 \
+    } while (0)
+    switch (P) {
+        case 4: CR_CONTRACT_B(4); break;
+        case 5: CR_CONTRACT_B(5); break;
+        case 6: CR_CONTRACT_B(6); break;
+        case 7: CR_CONTRACT_B(7); break;
+        case 8: CR_CONTRACT_B(8); break;
+        case 9: CR_CONTRACT_B(9); break;
+        case 10: CR_CONTRACT_B(10); break;
+        default: CR_CONTRACT_B(0); break;
+    }
+#undef CR_CONTRACT_B
+    return hipGetLastError();
+}
+
 #define CR_INST(T)                                                                                               \
     template hipError_t launch_bead_records<T>(const MeshParams&, const TileGeom&, const SysView&, const unsigned*, \
                                                const unsigned*, int*, T*, hipStream_t);                         \
     template hipError_t launch_spread_tiled<T>(const MeshParams&, const TileGeom&, const BinView&, const double*, \
                                                T*, hipStream_t, bool);                                          \
     template hipError_t launch_contract_tiled<T>(const MeshParams&, const TileGeom&, const SysView&,              \
-                                                 const BinView&, const double*, const T*, T*, double*, hipStream_t);
+                                                 const BinView&, const double*, const T*, T*, double*, hipStream_t); \
+    template hipError_t launch_fill_batch<T>(const TiledBox*, int, long, const int*, hipStream_t);             \
+    template hipError_t launch_spread_tiled_batch<T>(const TiledBox*, int, int, int, size_t, const int*,          \
+                                                     hipStream_t);                                              \
+    template hipError_t launch_contract_tiled_batch<T>(const TiledBox*, int, int, int, size_t, const int*,        \
+                                                       hipStream_t);
 CR_INST(float)
 CR_INST(double)
 #undef CR_INST

@@ -321,3 +321,67 @@ def make_workload(name: str) -> tuple[SystemDefinition, dict]:
     sysdef = make_system(w.pop("n_bodies"), w.pop("body_sizes"), w.pop("phi"), seed=w.pop("seed"),
                          charges=w.pop("charges"))
     return sysdef, w
+
+
+# Mesh sizes of batched boxes: the lengths the register-resident FFT passes take
+# (n = A * B, A = B or 2B), all with binned tilings of tile edge >= P - 1 = 8.
+BATCH_MESHES = (32, 64, 128)
+
+
+def batch_box_params(L: float, errortol: float = 1e-6, h_target: float = 1.0) -> dict:
+    """Ewald parameters of one batched box for ``errortol``: the mesh is the smallest
+    entry of BATCH_MESHES with spacing <= h_target (capped at 128); xi is the largest
+    splitting for which SetParams' mesh rule, N >= 1 + L kcut / pi with
+    kcut = 2 xi sqrt(-ln tol) (cppmodule/ConductorRigid.cc:124-129), still fits that
+    mesh, and rc = sqrt(-ln tol) / xi (:123).  P = ceil(-2 ln tol / pi) (:201); the
+    Gaussian width eta = P xi^2 h^2 / pi is then about 0.5 < 1 for every box."""
+    m = BATCH_MESHES[-1]
+    for cand in BATCH_MESHES:
+        if L / cand <= h_target:
+            m = cand
+            break
+    s = math.sqrt(-math.log(errortol))
+    xi = math.pi * (m - 1) / (L * 2.0 * s)
+    return dict(mesh=(m, m, m), xi=xi, rc=s / xi, errortol=errortol,
+                P=int(math.ceil(-2.0 * math.log(errortol) / math.pi)))
+
+
+def _batch_box(args) -> tuple[SystemDefinition, dict]:
+    b, n_bodies, body_sizes, phi, seed, errortol = args
+    for k in range(8):
+        # dense packings of large clusters can cage the growth: take the next seed
+        try:
+            sysdef = make_system(n_bodies, list(body_sizes), phi, seed=seed + k, charges="uniform")
+            break
+        except RuntimeError:
+            continue
+    else:
+        raise RuntimeError(f"box {b}: cluster growth stalled for 8 seeds at volume fraction {phi:.3f}")
+    p = batch_box_params(float(sysdef.L[0]), errortol)
+    p["phi"] = phi
+    return sysdef, p
+
+
+def make_batch(n_boxes: int = 64, seed: int = 4, n_range=(2000, 50000), body_sizes=(8, 20, 40),
+               phi_range=(0.1, 0.3), errortol: float = 1e-6, workers: int = 1) -> list[tuple[SystemDefinition, dict]]:
+    """Seeded set of independent periodic boxes of uneven size: N log-uniform in
+    n_range, body sizes drawn per body from body_sizes, volume fraction uniform in
+    phi_range, net charges U[-2, 2] shifted to a neutral box; Ewald parameters per box
+    from batch_box_params.  Returns [(system, params)] per box, the same for any number
+    of worker processes.  The cluster growth of make_system cages above a volume
+    fraction of about 0.33 for mixed 8/20/40-bead bodies, hence the default upper bound
+    of 0.3."""
+    rng = np.random.default_rng(seed)
+    mean_size = float(np.mean(body_sizes))
+    jobs = []
+    for b in range(n_boxes):
+        n_target = math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))
+        phi = float(rng.uniform(*phi_range))
+        n_bodies = max(2, int(round(n_target / mean_size)))
+        jobs.append((b, n_bodies, tuple(body_sizes), phi, int(rng.integers(1 << 30)), errortol))
+    if workers <= 1:
+        return [_batch_box(j) for j in jobs]
+    from concurrent.futures import ProcessPoolExecutor
+
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(_batch_box, jobs))

@@ -272,6 +272,29 @@ typedef struct {
 int cr_autotune(cr_context* ctx, const cr_system* sys, cr_tuning* out, void* stream);
 int cr_set_tiles(cr_context* ctx, const int tile[3], int lds_bytes);
 
+/* ---- batched boxes ------------------------------------------------------
+ * No reference counterpart: the reference computes one box per computeForces call
+ * (cppmodule/ConductorRigid.cc:403-557).  A batch solves many independent periodic boxes
+ * together.  Each box keeps its own context (cr_set_params with its own box, mesh and
+ * tolerance; cr_autotune for its tiling) and its own system; cr_batch_compute_force gives
+ * every box the result cr_compute_force would give it (same outputs, same per-box
+ * stopping rule and restart cycle), but the boxes advance through GMRES in lockstep: one
+ * binned spread and one binned contraction launch over the tiles of all boxes per operator
+ * application, segmented FP64 Krylov launches over all boxes per Arnoldi step, and one
+ * host sync per step for the whole batch.  use_graph != 0 captures the Arnoldi step once
+ * as a HIP graph (re-captured only when buffers or launch shapes change) and replays it.
+ * All boxes of a call must share the device, precision, window support P, restart,
+ * iteration limit and preconditioner, and have a binned tiling; a context may appear
+ * once per call.  q_inout / phi_inout / force / stats hold one entry per box (stats may
+ * be NULL).  Work runs on the batch's own streams, ordered after the caller's stream and
+ * before its later work. */
+typedef struct cr_batch cr_batch;
+int  cr_batch_create(cr_batch** out, int device);
+void cr_batch_destroy(cr_batch* batch);
+int  cr_batch_compute_force(cr_batch* batch, int nbox, cr_context* const* ctx, const cr_system* sys,
+                            double* const* q_inout, double* const* phi_inout, double* const* force, int x0_mode,
+                            int use_graph, cr_solve_stats* stats, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

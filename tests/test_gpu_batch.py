@@ -1,0 +1,110 @@
+"""Batched boxes: the lockstep batch solve gives every box what its own solve gives.
+
+The batch shares launches across boxes (binned spread / contraction over all tiles,
+segmented Krylov steps, a captured HIP graph for the step), so the checks compare each
+box against the single-box path (cr_compute_force) on the same inputs: same converged
+charges, potentials and forces to the solver tolerance, same iteration counts up to
+rounding, and graph replay against eager launches."""
+import copy
+
+import numpy as np
+import pytest
+
+from helpers import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(n_boxes=5, seed=11, n_range=(400, 3000), phi_range=(0.1, 0.3)):
+    from conductorrigid_amd.system import make_batch
+
+    return make_batch(n_boxes, seed=seed, n_range=n_range, phi_range=phi_range)
+
+
+def _boxes(items, **kw):
+    from conductorrigid_amd.compute import batch_boxes
+
+    return batch_boxes(items, **kw)
+
+
+def _state(cr):
+    return cr.get_charges(), cr.get_potentials(), cr.get_forces()[:, :3]
+
+
+def _single(items, x0="zero", **kw):
+    out = []
+    for cr in _boxes(items, **kw):
+        st = cr.computeForces(x0=x0)
+        out.append((_state(cr), st))
+    return out
+
+
+# FP64 solves are driven far below the discretisation error so that the two paths must
+# agree closely; FP32-bulk operators carry ~1e-7 rounding, so those solves stop at 1e-7.
+@pytest.mark.parametrize("precision,precond,gtol,tol,diters", [("fp64", "diagonal", 1e-10, 1e-7, 2),
+                                                               ("fp64", "block", 1e-10, 1e-7, 2),
+                                                               ("fp32", "block", 1e-7, 1e-4, 3)])
+def test_batch_matches_single_box_solves(precision, precond, gtol, tol, diters, require_gpu):
+    from conductorrigid_amd.compute import BatchConductorRigid
+
+    items = _batch()
+    kw = dict(precision=precision, preconditioner=precond, gmres_tol=gtol, restart=30, max_iter=400)
+    ref = _single(items, **kw)
+    boxes = _boxes(items, **kw)
+    batch = BatchConductorRigid(boxes, use_graph=True)
+    stats = batch.computeForces(x0="zero")
+    for b, cr in enumerate(boxes):
+        (q0, P0, F0), s0 = ref[b]
+        q, P, F = _state(cr)
+        print(f"[batch {precision}/{precond}] box {b} N={cr.system.N} mesh={cr.params['mesh'][0]} "
+              f"iters {stats[b]['iterations']} vs {s0['iterations']}: q {rel(q, q0):.1e} "
+              f"Phi {rel(P, P0):.1e} F {rel(F, F0):.1e} relres {stats[b]['true_relres']:.1e}", flush=True)
+        assert stats[b]["converged"] == 1
+        assert abs(stats[b]["iterations"] - s0["iterations"]) <= diters
+        assert rel(q, q0) < tol and rel(P, P0) < tol and rel(F, F0) < tol
+        assert abs(stats[b]["tolerance"] - s0["tolerance"]) <= 1e-9 * s0["tolerance"]
+
+
+def test_batch_graph_replay_matches_eager_over_calls(require_gpu):
+    """Two calls in a row (the second after the bodies move, with the previous charges as
+    x0): the captured step replayed from the graph and the same step launched eagerly
+    give the same answers and iteration counts."""
+    from conductorrigid_amd.compute import BatchConductorRigid, update_positions
+    from conductorrigid_amd.system import translate_bodies
+
+    items = _batch(n_boxes=4, seed=5)
+    kw = dict(precision="fp64", preconditioner="block", gmres_tol=1e-9, restart=10, max_iter=400)
+    runs = {}
+    for use_graph in (True, False):
+        boxes = _boxes(copy.deepcopy(items), **kw)  # update_positions below mutates the systems
+        batch = BatchConductorRigid(boxes, use_graph=use_graph)
+        s1 = batch.computeForces(x0="zero")
+        for k, cr in enumerate(boxes):
+            pos, com = translate_bodies(cr.system, 0.05, seed=100 + k)
+            update_positions(cr, pos, com)
+        s2 = batch.computeForces(x0="previous")
+        runs[use_graph] = ([_state(cr) for cr in boxes], s1, s2)
+    (sg, g1, g2), (se, e1, e2) = runs[True], runs[False]
+    for b in range(len(items)):
+        print(f"[graph] box {b}: iters graph {g1[b]['iterations']}/{g2[b]['iterations']} "
+              f"eager {e1[b]['iterations']}/{e2[b]['iterations']} restarts {g1[b]['restarts']}/{g2[b]['restarts']} "
+              f"q {rel(sg[b][0], se[b][0]):.1e} F {rel(sg[b][2], se[b][2]):.1e}", flush=True)
+        assert g1[b]["converged"] == 1 and g2[b]["converged"] == 1
+        assert abs(g1[b]["iterations"] - e1[b]["iterations"]) <= 1
+        assert abs(g2[b]["iterations"] - e2[b]["iterations"]) <= 1
+        for a, e in zip(sg[b], se[b]):
+            assert rel(a, e) < 1e-7
+    # restarted cycles happened (restart 10), so the replayed step crossed cycle boundaries
+    assert max(s["restarts"] for s in g1) >= 2
+
+
+def test_batch_rejects_mixed_precision(require_gpu):
+    from conductorrigid_amd import _lib
+    from conductorrigid_amd.compute import BatchConductorRigid
+
+    items = _batch(n_boxes=2, seed=3)
+    a = _boxes(items[:1], precision="fp64")
+    b = _boxes(items[1:], precision="fp32")
+    batch = BatchConductorRigid(a + b)
+    with pytest.raises(_lib.CrError, match="precision"):
+        batch.computeForces(x0="zero")

@@ -188,3 +188,28 @@ def test_tiling_does_not_change_the_answer(case, tile):
     qs, Phi, F, st = g.solve("zero")
     qo, Po, Fo, so = o.compute_forces([0.0, 0.0, 1.0], restart=30, maxiter=600, tol=1e-12)
     assert rel(qs, qo) < 1e-10 and rel(F[:, :3], Fo[:, :3]) < 1e-10
+
+
+# Register four-step FFT lengths n = A * B (32 = 8x4, 128 = 16x8, 512 = 32x16 beside the
+# square 64): every axis pass (z R2C / C2R / C2C, strided y, fused x + scale) against
+# numpy's FFT and against the oracle's k-space step.
+@pytest.mark.parametrize("mesh", [(32, 128, 64), (128, 32, 512), (512, 64, 32)])
+def test_register_fft_lengths(mesh, require_gpu):
+    s = n1k_system()
+    prm = dict(xi=0.929, errortol=1e-6, rc=4.0, mesh=mesh, P=8)
+    g = GpuCase(s, **prm)
+    rng = np.random.default_rng(9)
+    z = rng.normal(size=mesh) + 1j * rng.normal(size=mesh)
+    v1 = rel(g.fft(z, -1), np.fft.fftn(z))
+    v2 = rel(g.fft(z, +1), np.fft.ifftn(z) * z.size)
+    o = O.Oracle(s, prm["xi"], prm["errortol"], rc=prm["rc"], mesh=mesh, P=prm["P"])
+    grid = rng.normal(size=mesh)
+    og = o.recip_grid(grid)
+    v3 = rel(g.recip(grid), og)
+    g32 = GpuCase(s, precision="fp32", **prm)
+    v4 = rel(g32.recip(grid), og)
+    _report(f"register fft {mesh} fwd", v1)
+    _report(f"register fft {mesh} inv", v2)
+    _report(f"register recip {mesh} fp64", v3)
+    _report(f"register recip {mesh} fp32", v4)
+    assert v1 < 1e-13 and v2 < 1e-13 and v3 < 1e-12 and v4 < 1e-5
