@@ -59,7 +59,9 @@ struct cr_batch {
     size_t lds_spread = 0, lds_contract = 0;
     long Gmax = 0;
     DevBuf V, u, w, x, b, Mb, yraw, partial;
-    DevBuf d_spread, d_contract, d_rows, d_segs, d_gs, d_skip, d_finished, d_step, d_pub, d_rel;
+    DevBuf d_spread, d_contract, d_rows, d_segs, d_gs, d_skip, d_finished, d_step, d_pub, d_rel, d_blocks;
+    int total_bodies = 0;
+    std::vector<BlockBox> blocks;
     double* h_pub = nullptr;  // pinned: 3 doubles per box
     size_t h_pub_n = 0;
     std::vector<TiledBox> spread_boxes, contract_boxes;
@@ -111,17 +113,14 @@ int group_recip(MeshGroup& g, const std::vector<cr_context*>& ctx, hipStream_t s
     return CR_OK;
 }
 
-// y = A u (or D A u) for every box not skipped.  host_skip (may be null) also drops the
-// per-box FFT chains of boxes known to be frozen.
-int batch_matvec(cr_batch* B, const double* in, double* out, const int* skip, const std::vector<char>* host_skip,
-                 hipStream_t st)
+// Spread of u (the descriptors' input) and the k-space step for every box not skipped.
+// host_skip (may be null) also drops mesh groups whose boxes are all frozen.
+int batch_spread_recip(cr_batch* B, const int* skip, const std::vector<char>* host_skip, hipStream_t st)
 {
     cr_context* c0 = B->ctx[0];
     const bool f32 = c0->fp32();
-    const bool blk = block_precond(c0);
     const int P = c0->mp.P;
     const TiledBox* ds = B->d_spread.as<TiledBox>();
-    const TiledBox* dc = B->d_contract.as<TiledBox>();
     if (f32) {
         CR_HIP(launch_fill_batch<float>(ds, B->nbox, B->Gmax, skip, st));
         CR_HIP(launch_spread_tiled_batch<float>(ds, B->nbox, B->total_tiles, P, B->lds_spread, skip, st));
@@ -143,6 +142,20 @@ int batch_matvec(cr_batch* B, const double* in, double* out, const int* skip, co
         CR_HIP(hipEventRecord(B->join[k], B->side[k]));
         CR_HIP(hipStreamWaitEvent(st, B->join[k], 0));
     }
+    return CR_OK;
+}
+
+// out = A u, or D A u with precond, for every box not skipped (u = the batch vector the
+// spread descriptors read).
+int batch_matvec(cr_batch* B, const double* in, double* out, int precond, const int* skip,
+                 const std::vector<char>* host_skip, hipStream_t st)
+{
+    cr_context* c0 = B->ctx[0];
+    const bool f32 = c0->fp32();
+    const bool blk = precond && block_precond(c0);
+    const int P = c0->mp.P;
+    const TiledBox* dc = B->d_contract.as<TiledBox>();
+    CR_RET(batch_spread_recip(B, skip, host_skip, st));
     if (f32)
         CR_HIP(launch_contract_tiled_batch<float>(dc, B->nbox, B->total_tiles, P, B->lds_contract, skip, st));
     else
@@ -150,17 +163,13 @@ int batch_matvec(cr_batch* B, const double* in, double* out, const int* skip, co
     double* rows_out = blk ? B->yraw.as<double>() : out;
     if (f32)
         CR_HIP(launch_brows<float>(B->d_rows.as<RowBox>(), B->nbox, B->bead_blocks, B->rigid_blocks, in, rows_out,
-                                   blk ? 0 : 1, skip, st));
+                                   (precond && !blk) ? 1 : 0, skip, st));
     else
         CR_HIP(launch_brows<double>(B->d_rows.as<RowBox>(), B->nbox, B->bead_blocks, B->rigid_blocks, in, rows_out,
-                                    blk ? 0 : 1, skip, st));
-    if (blk) {
-        for (int b = 0; b < B->nbox; ++b) {
-            if (host_skip && (*host_skip)[b]) continue;
-            const long o = B->segs[b].xoff;
-            CR_RET(apply_precond(B->ctx[b], B->views[b], B->yraw.as<double>() + o, out + o, st));
-        }
-    }
+                                    (precond && !blk) ? 1 : 0, skip, st));
+    if (blk)
+        CR_HIP(launch_block_apply_batch(B->d_blocks.as<BlockBox>(), B->nbox, B->total_bodies, B->yraw.as<double>(), out,
+                                        skip, st));
     return CR_OK;
 }
 
@@ -177,7 +186,7 @@ int batch_step(cr_batch* B, const std::vector<char>* host_skip, hipStream_t st)
     const BatchSeg* sg = B->d_segs.as<BatchSeg>();
     BatchGmres* gs = B->d_gs.as<BatchGmres>();
     CR_HIP(launch_bcopy_col(V, B->ld, step, B->u.as<double>(), B->n_tot, st));
-    CR_RET(batch_matvec(B, B->u.as<double>(), w, skip, host_skip, st));
+    CR_RET(batch_matvec(B, B->u.as<double>(), w, 1, skip, host_skip, st));
     CR_HIP(launch_bdot(V, B->ld, 0, step, w, sg, nb, B->seg_blocks, part, skip, st));
     CR_HIP(launch_breduce(part, sg, nb, 0, step, gs, kRedH1, skip, st));
     CR_HIP(launch_bgs_update(V, B->ld, step, w, sg, nb, B->seg_blocks, gs, kRedH1, 0, part, skip, st));
@@ -197,9 +206,9 @@ std::vector<const void*> step_signature(const cr_batch* B)
 {
     std::vector<const void*> s = {B->V.p, B->u.p, B->w.p, B->yraw.p, B->partial.p, B->d_spread.p, B->d_contract.p,
                                   B->d_rows.p, B->d_segs.p, B->d_gs.p, B->d_skip.p, B->d_step.p, B->d_pub.p,
-                                  B->h_pub};
+                                  B->d_blocks.p, B->h_pub};
     const long shape[] = {B->nbox, B->R, B->n_tot, B->ld, B->seg_blocks, B->bead_blocks, B->rigid_blocks,
-                          B->total_tiles, (long)B->lds_spread, (long)B->lds_contract, B->Gmax};
+                          B->total_tiles, (long)B->lds_spread, (long)B->lds_contract, B->Gmax, B->total_bodies};
     for (long v : shape) s.push_back(reinterpret_cast<const void*>(v));
     for (const auto& g : B->groups) {
         s.push_back(g->real.p);
@@ -211,11 +220,6 @@ std::vector<const void*> step_signature(const cr_batch* B)
         s.push_back(c);
         s.push_back(reinterpret_cast<const void*>(c->generation));
         s.push_back(c->phi_recip.p);
-        s.push_back(c->body_start.p);
-        s.push_back(c->body_beads.p);
-        s.push_back(c->ginv.p);
-        s.push_back(c->uvec.p);
-        s.push_back(c->inv_s.p);
     }
     return s;
 }
@@ -319,10 +323,11 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     B->rows.resize(nbox);
     B->spread_boxes.resize(nbox);
     B->contract_boxes.resize(nbox);
+    B->blocks.resize(nbox);
 
     // per-box preparation (binning, records, ELL rows, objective, preconditioner setup)
     long xoff = 0;
-    int blk = 0, bblk = 0, rblk = 0, tiles = 0;
+    int blk = 0, bblk = 0, rblk = 0, tiles = 0, bodies = 0;
     size_t lds_s = 0, lds_c = 0;
     long Gmax = 0;
     for (int b = 0; b < nbox; ++b) {
@@ -337,6 +342,19 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
         if (block_precond(c)) CR_RET(block_setup(c, v, st));
         CR_RET(apply_precond(c, v, c->b.as<double>(), c->Mb.as<double>(), st));
         CR_RET(ensure_csr(c, v, st));
+
+        BlockBox& bb = B->blocks[b];
+        bb = BlockBox{};
+        bb.xoff = xoff;
+        bb.N = v.N;
+        bb.nmax = c->block_nmax;
+        bb.body0 = bodies;
+        bb.start = c->body_start.as<unsigned>();
+        bb.beads = c->body_beads.as<unsigned>();
+        bb.ginv = c->ginv.as<float>();
+        bb.uvec = c->uvec.as<float>();
+        bb.inv_s = c->inv_s.as<double>();
+        bodies += v.N_isolated;
 
         BatchSeg& s = B->segs[b];
         s.xoff = xoff;
@@ -398,6 +416,7 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     B->bead_blocks = bblk;
     B->rigid_blocks = rblk;
     B->total_tiles = tiles;
+    B->total_bodies = bodies;
     B->lds_spread = lds_s;
     B->lds_contract = lds_c;
     B->Gmax = Gmax;
@@ -466,6 +485,7 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     CR_HIP(B->d_step.ensure(sizeof(int)));
     CR_HIP(B->d_pub.ensure(sizeof(double) * 3 * nbox));
     CR_HIP(B->d_rel.ensure(sizeof(double) * nbox));
+    CR_HIP(B->d_blocks.ensure(sizeof(BlockBox) * nbox));
     if (B->h_pub_n < (size_t)(3 * nbox)) {
         if (B->h_pub) (void)hipHostFree(B->h_pub);
         B->h_pub = nullptr;
@@ -474,7 +494,12 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
         B->h_pub_n = (size_t)(3 * nbox);
     }
     double* u = B->u.as<double>();
-    for (int b = 0; b < nbox; ++b) B->spread_boxes[b].q = u + B->segs[b].xoff;
+    for (int b = 0; b < nbox; ++b) {
+        B->spread_boxes[b].q = u + B->segs[b].xoff;
+        B->contract_boxes[b].q = u + B->segs[b].xoff;
+        B->contract_boxes[b].pos = B->views[b].pos;
+        B->contract_boxes[b].force = force[b];
+    }
     std::vector<double> rel(nbox);
     for (int b = 0; b < nbox; ++b) rel[b] = ctx[b]->res.gmres_tol;
     CR_HIP(hipMemcpyAsync(B->d_spread.p, B->spread_boxes.data(), sizeof(TiledBox) * nbox, hipMemcpyHostToDevice, st));
@@ -483,6 +508,7 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     CR_HIP(hipMemcpyAsync(B->d_rows.p, B->rows.data(), sizeof(RowBox) * nbox, hipMemcpyHostToDevice, st));
     CR_HIP(hipMemcpyAsync(B->d_segs.p, B->segs.data(), sizeof(BatchSeg) * nbox, hipMemcpyHostToDevice, st));
     CR_HIP(hipMemcpyAsync(B->d_rel.p, rel.data(), sizeof(double) * nbox, hipMemcpyHostToDevice, st));
+    CR_HIP(hipMemcpyAsync(B->d_blocks.p, B->blocks.data(), sizeof(BlockBox) * nbox, hipMemcpyHostToDevice, st));
 
     // objective, preconditioned objective and x0 of every box into the batch vectors
     double* x = B->x.as<double>();
@@ -522,7 +548,7 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
         // cycle start: w = D (A x) - D b for every unfinished box
         CR_HIP(hipMemcpyAsync(u, x, sizeof(double) * B->n_tot, hipMemcpyDeviceToDevice, st));
         for (int b = 0; b < nb; ++b) host_skip[b] = (char)h_fin[b];
-        CR_RET(batch_matvec(B, u, w, finished, &host_skip, st));
+        CR_RET(batch_matvec(B, u, w, 1, finished, &host_skip, st));
         for (int b = 0; b < nb; ++b)
             if (!h_fin[b]) ++matvecs[b];
         CR_HIP(launch_axpby(-1.0, B->Mb.as<double>(), 1.0, w, B->n_tot, st));
@@ -589,14 +615,33 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
         if (!any_left || iter >= maxiter) break;
     }
 
-    // outputs, FP64 residual check and force pass per box
+    // outputs
     for (int b = 0; b < nb; ++b) {
-        cr_context* c = ctx[b];
         const SysView& v = B->views[b];
         const long o = B->segs[b].xoff;
         CR_HIP(hipMemcpyAsync(q_inout[b], x + o, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
         CR_HIP(hipMemcpyAsync(phi_inout[b], x + o + v.N, sizeof(double) * v.N_isolated, hipMemcpyDeviceToDevice, st));
-        if (stats) {
+    }
+    CR_HIP(hipMemcpyAsync(u, x, sizeof(double) * B->n_tot, hipMemcpyDeviceToDevice, st));
+    if (stats) {
+        // FP64 judge of every box from one batched operator application r = A x - b:
+        // ||D r||, ||r|| and ||b|| per box, read back together
+        double* dr = B->yraw.as<double>();
+        CR_RET(batch_matvec(B, u, w, 0, nullptr, nullptr, st));
+        CR_HIP(launch_axpby(-1.0, B->b.as<double>(), 1.0, w, B->n_tot, st));
+        if (block_precond(c0))
+            CR_HIP(launch_block_apply_batch(B->d_blocks.as<BlockBox>(), nb, B->total_bodies, w, dr, nullptr, st));
+        else
+            CR_HIP(launch_bprecond_diag(B->d_rows.as<RowBox>(), nb, B->bead_blocks, B->rigid_blocks, w, dr, st));
+        const double* vecs[3] = {dr, w, B->b.as<double>()};
+        for (int k = 0; k < 3; ++k) {
+            CR_HIP(launch_bdot(vecs[k], B->ld, 1, step, vecs[k], sg, nb, blk, part, nullptr, st));
+            CR_HIP(launch_breduce(part, sg, nb, 1, step, gs, kRedNorm, nullptr, st));
+            CR_HIP(launch_bcollect(gs, nb, B->d_pub.as<double>(), 3, k, st));
+        }
+        CR_HIP(hipMemcpyAsync(B->h_pub, B->d_pub.p, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, st));
+        CR_HIP(hipStreamSynchronize(st));
+        for (int b = 0; b < nb; ++b) {
             cr_solve_stats& S = stats[b];
             S.iterations = iters[b];
             S.matvecs = matvecs[b];
@@ -604,10 +649,22 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
             S.converged = conv[b];
             S.resid = resid[b];
             S.tolerance = tol[b];
-            CR_RET(true_residual(c, v, x + o, &S, st));
+            const double rD = B->h_pub[3 * b], r2 = B->h_pub[3 * b + 1], b2 = B->h_pub[3 * b + 2];
+            S.true_resid = std::sqrt(rD);
+            S.true_relres = b2 > 0 ? std::sqrt(r2 / b2) : std::sqrt(r2);
         }
-        CR_RET(force_pass_any(c, v, x + o, force[b], st));
     }
+    // force pass: spread of the converged charges, k-space step and reciprocal-space force
+    // contraction for all boxes, then the real-space pair forces + q E0 per box
+    CR_RET(batch_spread_recip(B, nullptr, nullptr, st));
+    if (c0->fp32())
+        CR_HIP(launch_contract_tiled_batch<float>(B->d_contract.as<TiledBox>(), nb, B->total_tiles, c0->mp.P,
+                                                  B->lds_contract, nullptr, st, true));
+    else
+        CR_HIP(launch_contract_tiled_batch<double>(B->d_contract.as<TiledBox>(), nb, B->total_tiles, c0->mp.P,
+                                                   B->lds_contract, nullptr, st, true));
+    for (int b = 0; b < nb; ++b)
+        CR_RET(real_space_force_any(ctx[b], B->views[b], x + B->segs[b].xoff, force[b], st));
     CR_HIP(hipEventRecord(B->done_ev, st));
     CR_HIP(hipStreamWaitEvent(caller, B->done_ev, 0));
     return CR_OK;

@@ -501,6 +501,15 @@ int recip_any(cr_context* c, hipStream_t st) { return c->fp32() ? recip_impl<flo
 
 const AxisPlan* axis_plans(const cr_context* c) { return c->fp32() ? plans<float>(c) : plans<double>(c); }
 
+int real_space_force_any(cr_context* c, const SysView& v, const double* q, double* force, hipStream_t st)
+{
+    if (c->fp32())
+        CR_HIP(launch_real_space_force<float>(c->rp, v, q, table<float>(c, true), c->E0, force, st));
+    else
+        CR_HIP(launch_real_space_force<double>(c->rp, v, q, table<double>(c, true), c->E0, force, st));
+    return CR_OK;
+}
+
 int force_pass_any(cr_context* c, const SysView& v, const double* q, double* force, hipStream_t st)
 {
     return c->fp32() ? force_pass<float>(c, v, q, force, st) : force_pass<double>(c, v, q, force, st);

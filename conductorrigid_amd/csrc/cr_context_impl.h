@@ -78,6 +78,8 @@ int init_x0(cr_context* c, const SysView& v, int x0_mode, const double* q_in, co
             hipStream_t st);
 // FP64 residuals of x against the context's objective b: ||D r|| and ||r|| / ||b||.
 int true_residual(cr_context* c, const SysView& v, const double* x, cr_solve_stats* stats, hipStream_t st);
+// force += q E0 + real-space pair forces (the last step of the force pass).
+int real_space_force_any(cr_context* c, const SysView& v, const double* q, double* force, hipStream_t st);
 // Final force pass for charges q (spread, k-space, contract_force, real-space force + q E0).
 int force_pass_any(cr_context* c, const SysView& v, const double* q, double* force, hipStream_t st);
 

@@ -119,9 +119,11 @@ struct TiledBox {
     const unsigned* sorted;      // [N]
     const int* rec_off;          // [N]
     const void* rec_w;           // [N * 3P] (T)
-    const double* q;             // spread input, bead order
+    const double* q;             // spread input / charges of the force contraction, bead order
     void* grid;                  // real mesh (T)
     void* phi;                   // contraction output (T), bead order
+    const double4* pos;          // bead positions (force contraction)
+    double* force;               // force contraction output [4N] (overwritten)
 };
 // skip (device, may be null): boxes with skip[box] != 0 are left untouched.
 template <typename T>
@@ -131,7 +133,7 @@ hipError_t launch_spread_tiled_batch(const TiledBox* boxes, int nbox, int total_
                                      const int* skip, hipStream_t st);
 template <typename T>
 hipError_t launch_contract_tiled_batch(const TiledBox* boxes, int nbox, int total_tiles, int P, size_t lds_bytes,
-                                       const int* skip, hipStream_t st);
+                                       const int* skip, hipStream_t st, bool force = false);
 // ---- k_se.hip (unbinned fallbacks) ----
 template <typename T>
 hipError_t launch_contract(const MeshParams& mp, const SysView& s, const T* grid, T* phi, hipStream_t st);
@@ -221,6 +223,19 @@ hipError_t launch_zero_force(int N, double* force, hipStream_t st);
 hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, unsigned* body_count,
                               unsigned* body_start, unsigned* body_beads,
                               unsigned long long* scan_sums, float* ginv, float* uvec, double* inv_s, hipStream_t st);
+// One box of a batched block-preconditioner application (device array, body0 ascending):
+// the box's per-body inverses and its offset in the batch vectors.
+struct BlockBox {
+    long xoff;
+    int N, nmax, body0;
+    const unsigned* start;
+    const unsigned* beads;
+    const float* ginv;
+    const float* uvec;
+    const double* inv_s;
+};
+hipError_t launch_block_apply_batch(const BlockBox* boxes, int nbox, int total_bodies, const double* y, double* w,
+                                    const int* skip, hipStream_t st);
 hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_start, const unsigned* body_beads,
                               const float* ginv, const float* uvec, const double* inv_s, const double* y, double* w,
                               hipStream_t st);
@@ -408,5 +423,10 @@ hipError_t launch_bset_tol(BatchGmres* gs, int nbox, const double* rel, hipStrea
 hipError_t launch_bstep_end(int* step, const BatchGmres* gs, const int* skip, int nbox, double* pub, int inc,
                             hipStream_t st);
 hipError_t launch_bset_step(int* step, int value, hipStream_t st);
+// out[stride * b + slot] = gs[b].norm2 (collects kRedNorm results for one readback)
+hipError_t launch_bcollect(const BatchGmres* gs, int nbox, double* out, int stride, int slot, hipStream_t st);
+// y = D x with the reference's diagonal preconditioner for every box (rows as launch_brows)
+hipError_t launch_bprecond_diag(const RowBox* boxes, int nbox, int bead_blocks, int rigid_blocks, const double* x,
+                                double* y, hipStream_t st);
 
 }  // namespace cr

@@ -115,6 +115,33 @@ __global__ __launch_bounds__(kB) void k_brigid_rows(const RowBox* __restrict__ b
     y[d.xoff + d.N + d.iso[mem[0]]] = precond ? acc / (double)size : acc;
 }
 
+// The reference's diagonal preconditioner (precondition, cppmodule/ConductorRigid.cu:816-859):
+// bead rows -4 pi, an isolated bead's body row 1/4pi, a rigid body's row 1/size.
+__global__ __launch_bounds__(kB) void k_bprecond_beads(const RowBox* __restrict__ boxes, int nbox,
+                                                       const double* __restrict__ x, double* __restrict__ y)
+{
+    const RowBox& d = boxes[owner<RowBox, &RowBox::blk0>(boxes, nbox, blockIdx.x)];
+    const int i = (blockIdx.x - d.blk0) * kB + threadIdx.x;
+    if (i >= d.N) return;
+    y[d.xoff + i] = -4.0 * kPi * x[d.xoff + i];
+    if (d.rigid_membership[i] == NOT_MEMBER) {
+        const long r = d.xoff + d.N + d.iso[i];
+        y[r] = x[r] / (4.0 * kPi);
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_bprecond_rigid(const RowBox* __restrict__ boxes, int nbox,
+                                                       const double* __restrict__ x, double* __restrict__ y)
+{
+    const RowBox& d = boxes[owner<RowBox, &RowBox::rblk0>(boxes, nbox, blockIdx.x)];
+    const int r = (blockIdx.x - d.rblk0) * kB + threadIdx.x;
+    if (r >= d.N_rigid) return;
+    const unsigned size = d.rigid_size[r];
+    if (size == 0) return;
+    const long row = d.xoff + d.N + d.iso[d.rigid_members[(long)r * d.Nmax]];
+    y[row] = x[row] / (double)size;
+}
+
 // ---- Krylov (single-box forms in k_krylov.hip)
 
 __global__ __launch_bounds__(kB) void k_bdot(const double* __restrict__ V, long ld, int fixed_cols,
@@ -345,6 +372,12 @@ __global__ void k_bset_tol(BatchGmres* __restrict__ gs, int nbox, const double* 
     if (b < nbox) gs[b].tol = rel[b] * sqrt(gs[b].norm2);
 }
 
+__global__ void k_bcollect(const BatchGmres* __restrict__ gs, int nbox, double* __restrict__ out, int stride, int slot)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nbox) out[(long)stride * b + slot] = gs[b].norm2;
+}
+
 __global__ void k_bset_step(int* step, int v)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *step = v;
@@ -456,6 +489,21 @@ hipError_t launch_bstep_end(int* step, const BatchGmres* gs, const int* skip, in
 hipError_t launch_bset_tol(BatchGmres* gs, int nbox, const double* rel, hipStream_t st)
 {
     hipLaunchKernelGGL(k_bset_tol, dim3(cdiv(nbox, 64)), dim3(64), 0, st, gs, nbox, rel);
+    return hipGetLastError();
+}
+
+hipError_t launch_bcollect(const BatchGmres* gs, int nbox, double* out, int stride, int slot, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bcollect, dim3(cdiv(nbox, 64)), dim3(64), 0, st, gs, nbox, out, stride, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_bprecond_diag(const RowBox* boxes, int nbox, int bead_blocks, int rigid_blocks, const double* x,
+                                double* y, hipStream_t st)
+{
+    if (bead_blocks > 0) hipLaunchKernelGGL(k_bprecond_beads, dim3(bead_blocks), dim3(kB), 0, st, boxes, nbox, x, y);
+    if (rigid_blocks > 0)
+        hipLaunchKernelGGL(k_bprecond_rigid, dim3(rigid_blocks), dim3(kB), 0, st, boxes, nbox, x, y);
     return hipGetLastError();
 }
 

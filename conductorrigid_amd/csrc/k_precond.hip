@@ -211,15 +211,13 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// w = A_b^{-1} y per body: bead rows and the body's constraint row.
-__global__ __launch_bounds__(64) void k_block_apply(int N, int nmax, const unsigned* __restrict__ start,
-                                                    const unsigned* __restrict__ beads,
-                                                    const float* __restrict__ ginv, const float* __restrict__ uvec,
-                                                    const double* __restrict__ inv_s, const double* __restrict__ y,
-                                                    double* __restrict__ w)
+// w = A_b^{-1} y for body t: bead rows and the body's constraint row (64 threads).
+__device__ __forceinline__ void block_apply_body(int t, int N, int nmax, const unsigned* __restrict__ start,
+                                                 const unsigned* __restrict__ beads, const float* __restrict__ ginv,
+                                                 const float* __restrict__ uvec, const double* __restrict__ inv_s,
+                                                 const double* __restrict__ y, double* __restrict__ w)
 {
     __shared__ double ys[64];
-    const int t = blockIdx.x;
     const int b0 = start[t];
     const int n = start[t + 1] - b0;
     const int i = threadIdx.x;
@@ -241,6 +239,32 @@ __global__ __launch_bounds__(64) void k_block_apply(int N, int nmax, const unsig
     const double c = (dot + y[N + t]) * inv_s[t];
     if (i < n) w[bead] = -gy + ui * c;
     if (i == 0) w[N + t] = c;
+}
+
+__global__ __launch_bounds__(64) void k_block_apply(int N, int nmax, const unsigned* __restrict__ start,
+                                                    const unsigned* __restrict__ beads,
+                                                    const float* __restrict__ ginv, const float* __restrict__ uvec,
+                                                    const double* __restrict__ inv_s, const double* __restrict__ y,
+                                                    double* __restrict__ w)
+{
+    block_apply_body(blockIdx.x, N, nmax, start, beads, ginv, uvec, inv_s, y, w);
+}
+
+// One workgroup per body of any box (body0 ascending over boxes).
+__global__ __launch_bounds__(64) void k_block_apply_batch(const BlockBox* __restrict__ boxes, int nbox,
+                                                          const double* __restrict__ y, double* __restrict__ w,
+                                                          const int* __restrict__ skip)
+{
+    int lo = 0, hi = nbox - 1;
+    const int k = blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (boxes[mid].body0 <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    const BlockBox& d = boxes[lo];
+    if (skip && skip[lo]) return;
+    block_apply_body(k - d.body0, d.N, d.nmax, d.start, d.beads, d.ginv, d.uvec, d.inv_s, y + d.xoff, w + d.xoff);
 }
 
 inline unsigned blocks(long n, int b) { return (unsigned)((n + b - 1) / b > 0 ? (n + b - 1) / b : 1); }
@@ -300,6 +324,14 @@ hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_s
     if (s.N_isolated <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_block_apply, dim3(s.N_isolated), dim3(64), 0, st, s.N, nmax, body_start, body_beads, ginv,
                        uvec, inv_s, y, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_apply_batch(const BlockBox* boxes, int nbox, int total_bodies, const double* y, double* w,
+                                    const int* skip, hipStream_t st)
+{
+    if (total_bodies <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_block_apply_batch, dim3((unsigned)total_bodies), dim3(64), 0, st, boxes, nbox, y, w, skip);
     return hipGetLastError();
 }
 

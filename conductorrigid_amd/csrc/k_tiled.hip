@@ -448,7 +448,7 @@ __global__ __launch_bounds__(kTiledThreads) void k_spread_tiled_batch(const Tile
                           static_cast<const T*>(d.rec_w), d.q, static_cast<T*>(d.grid), smem_raw);
 }
 
-template <typename T, int PT>
+template <typename T, bool FORCE, int PT>
 __global__ __launch_bounds__(kTiledThreads) void k_contract_tiled_batch(const TiledBox* __restrict__ boxes, int nbox,
                                                                         int total, const int* __restrict__ skip)
 {
@@ -456,9 +456,10 @@ __global__ __launch_bounds__(kTiledThreads) void k_contract_tiled_batch(const Ti
     const int g = xcd_tile(blockIdx.x, total);
     const TiledBox& d = boxes[batch_box(boxes, nbox, g)];
     if (skip && skip[d.box]) return;
-    contract_tile<T, false, PT>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off,
-                                static_cast<const T*>(d.rec_w), nullptr, nullptr, static_cast<const T*>(d.grid),
-                                static_cast<T*>(d.phi), nullptr, smem_raw);
+    contract_tile<T, FORCE, PT>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off,
+                                static_cast<const T*>(d.rec_w), d.pos, FORCE ? d.q : nullptr,
+                                static_cast<const T*>(d.grid), static_cast<T*>(d.phi), FORCE ? d.force : nullptr,
+                                smem_raw);
 }
 
 inline unsigned blocks(long n, int b)
@@ -624,17 +625,24 @@ hipError_t launch_spread_tiled_batch(const TiledBox* boxes, int nbox, int total_
 
 template <typename T>
 hipError_t launch_contract_tiled_batch(const TiledBox* boxes, int nbox, int total_tiles, int P, size_t lds_bytes,
-                                       const int* skip, hipStream_t st)
+                                       const int* skip, hipStream_t st, bool force)
 {
     if (nbox <= 0 || total_tiles <= 0) return hipSuccess;
     if (P > 16) return hipErrorInvalidValue;
 #define CR_CONTRACT_B(PV)                                                                                       \
     do {                                                                                                        \
-        if (lds_bytes > 65536)                                                                                  \
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract_tiled_batch<T, PV>),           \
+        if (lds_bytes > 65536) {                                                                                \
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract_tiled_batch<T, false, PV>),    \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);              \
-        hipLaunchKernelGGL((k_contract_tiled_batch<T, PV>), dim3(total_tiles), dim3(kTiledThreads), lds_bytes,  \
-                           st, boxes, nbox, total_tiles, skip);                                                 \
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract_tiled_batch<T, true, PV>),     \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);              \
+        }                                                                                                       \
+        if (force)                                                                                              \
+            hipLaunchKernelGGL((k_contract_tiled_batch<T, true, PV>), dim3(total_tiles), dim3(kTiledThreads),   \
+                               lds_bytes, st, boxes, nbox, total_tiles, skip);                                  \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_contract_tiled_batch<T, false, PV>), dim3(total_tiles), dim3(kTiledThreads),  \
+                               lds_bytes, st, boxes, nbox, total_tiles, skip);                                  \
     } while (0)
     switch (P) {
         case 4: CR_CONTRACT_B(4); break;
@@ -661,7 +669,7 @@ hipError_t launch_contract_tiled_batch(const TiledBox* boxes, int nbox, int tota
     template hipError_t launch_spread_tiled_batch<T>(const TiledBox*, int, int, int, size_t, const int*,          \
                                                      hipStream_t);                                              \
     template hipError_t launch_contract_tiled_batch<T>(const TiledBox*, int, int, int, size_t, const int*,        \
-                                                       hipStream_t);
+                                                       hipStream_t, bool);
 CR_INST(float)
 CR_INST(double)
 #undef CR_INST
