@@ -92,7 +92,13 @@ hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const B
 // per node) plus per-bead staging tables (stage_elem bytes per value) for `chunk` beads,
 // within kTiledLdsBudget bytes.  The spread accumulates its tile in SpreadAcc.
 constexpr size_t kTiledLdsBudget = 64 * 1024;   // default per-workgroup LDS budget
-constexpr int kTiledThreads = 512;   // workgroup of the binned kernels
+// Workgroups of the binned kernels.  The spread's LDS tile limits a CU to two
+// workgroups, so 1024 threads give it twice the waves to hide LDS latency; the FP32
+// contraction keeps 512 (its smaller tile admits three workgroups per CU), the FP64 one
+// measures faster at 1024.
+constexpr int kSpreadThreads = 1024;
+template <typename T>
+constexpr int contract_threads() { return sizeof(T) == sizeof(double) ? 1024 : 512; }
 constexpr int kTiledChunk = 64;      // beads staged per round (bins hold ~60 at 1M / 256^3)
 using SpreadAcc = double;
 struct TiledLds {

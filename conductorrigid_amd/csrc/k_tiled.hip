@@ -261,6 +261,8 @@ __device__ __forceinline__ void spread_tile(const MeshParams& mp, const TileGeom
     A* ext = reinterpret_cast<A*>(smem_raw);
     Stage<T> s = stage_layout<T>(smem_raw, (size_t)esz * sizeof(A), ch, P);
     for (int k = threadIdx.x; k < esz; k += blockDim.x) ext[k] = (A)0;
+    // the write-out below reads nodes other threads zeroed, also when the bin is empty
+    __syncthreads();
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     const int xs = tc.xs, ys = tc.ys, per = 3 * P;
     const unsigned kb = tstart[tc.t], ke = tstart[tc.t + 1];
@@ -299,7 +301,7 @@ __device__ __forceinline__ void spread_tile(const MeshParams& mp, const TileGeom
 }
 
 template <typename T, typename A, int PT>
-__global__ __launch_bounds__(kTiledThreads) void k_spread_tiled(MeshParams mp, TileGeom tg, int ch,
+__global__ __launch_bounds__(kSpreadThreads) void k_spread_tiled(MeshParams mp, TileGeom tg, int ch,
                                                       const unsigned* __restrict__ tstart,
                                                       const unsigned* __restrict__ sorted,
                                                       const int* __restrict__ rec_off, const T* __restrict__ rec_w,
@@ -399,7 +401,7 @@ __device__ __forceinline__ void contract_tile(const MeshParams& mp, const TileGe
 }
 
 template <typename T, bool FORCE, int PT>
-__global__ __launch_bounds__(kTiledThreads) void k_contract_tiled(MeshParams mp, TileGeom tg, int ch,
+__global__ __launch_bounds__(contract_threads<T>()) void k_contract_tiled(MeshParams mp, TileGeom tg, int ch,
                                                         const unsigned* __restrict__ tstart,
                                                         const unsigned* __restrict__ sorted,
                                                         const int* __restrict__ rec_off, const T* __restrict__ rec_w,
@@ -437,7 +439,7 @@ __global__ void k_fill_batch(const TiledBox* __restrict__ boxes, const int* __re
 }
 
 template <typename T, typename A, int PT>
-__global__ __launch_bounds__(kTiledThreads) void k_spread_tiled_batch(const TiledBox* __restrict__ boxes, int nbox,
+__global__ __launch_bounds__(kSpreadThreads) void k_spread_tiled_batch(const TiledBox* __restrict__ boxes, int nbox,
                                                                       int total, const int* __restrict__ skip)
 {
     extern __shared__ char smem_raw[];
@@ -449,7 +451,7 @@ __global__ __launch_bounds__(kTiledThreads) void k_spread_tiled_batch(const Tile
 }
 
 template <typename T, bool FORCE, int PT>
-__global__ __launch_bounds__(kTiledThreads) void k_contract_tiled_batch(const TiledBox* __restrict__ boxes, int nbox,
+__global__ __launch_bounds__(contract_threads<T>()) void k_contract_tiled_batch(const TiledBox* __restrict__ boxes, int nbox,
                                                                         int total, const int* __restrict__ skip)
 {
     extern __shared__ char smem_raw[];
@@ -528,7 +530,7 @@ hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const B
         if (l.total_bytes > 65536)                                                                             \
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_tiled<T, SpreadAcc, PV>),       \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total_bytes);         \
-        hipLaunchKernelGGL((k_spread_tiled<T, SpreadAcc, PV>), dim3(tg.ntiles), dim3(kTiledThreads),          \
+        hipLaunchKernelGGL((k_spread_tiled<T, SpreadAcc, PV>), dim3(tg.ntiles), dim3(kSpreadThreads),          \
                            l.total_bytes, st, mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, q, \
                            grid);                                                                              \
     } while (0)
@@ -562,11 +564,11 @@ hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total_bytes);          \
         }                                                                                                       \
         if (force)                                                                                              \
-            hipLaunchKernelGGL((k_contract_tiled<T, true, PV>), dim3(tg.ntiles), dim3(kTiledThreads), l.total_bytes, st,  \
+            hipLaunchKernelGGL((k_contract_tiled<T, true, PV>), dim3(tg.ntiles), dim3(contract_threads<T>()), l.total_bytes, st,  \
                                mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, s.pos, q, grid,    \
                                (T*)nullptr, force);                                                             \
         else                                                                                                    \
-            hipLaunchKernelGGL((k_contract_tiled<T, false, PV>), dim3(tg.ntiles), dim3(kTiledThreads), l.total_bytes, st, \
+            hipLaunchKernelGGL((k_contract_tiled<T, false, PV>), dim3(tg.ntiles), dim3(contract_threads<T>()), l.total_bytes, st, \
                                mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, s.pos,             \
                                (const double*)nullptr, grid, phi, (double*)nullptr);                            \
     } while (0)
@@ -606,7 +608,7 @@ hipError_t launch_spread_tiled_batch(const TiledBox* boxes, int nbox, int total_
         if (lds_bytes > 65536)                                                                                  \
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_tiled_batch<T, SpreadAcc, PV>),  \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);              \
-        hipLaunchKernelGGL((k_spread_tiled_batch<T, SpreadAcc, PV>), dim3(total_tiles), dim3(kTiledThreads),    \
+        hipLaunchKernelGGL((k_spread_tiled_batch<T, SpreadAcc, PV>), dim3(total_tiles), dim3(kSpreadThreads),    \
                            lds_bytes, st, boxes, nbox, total_tiles, skip);                                      \
     } while (0)
     switch (P) {
@@ -638,10 +640,10 @@ hipError_t launch_contract_tiled_batch(const TiledBox* boxes, int nbox, int tota
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);              \
         }                                                                                                       \
         if (force)                                                                                              \
-            hipLaunchKernelGGL((k_contract_tiled_batch<T, true, PV>), dim3(total_tiles), dim3(kTiledThreads),   \
+            hipLaunchKernelGGL((k_contract_tiled_batch<T, true, PV>), dim3(total_tiles), dim3(contract_threads<T>()),   \
                                lds_bytes, st, boxes, nbox, total_tiles, skip);                                  \
         else                                                                                                    \
-            hipLaunchKernelGGL((k_contract_tiled_batch<T, false, PV>), dim3(total_tiles), dim3(kTiledThreads),  \
+            hipLaunchKernelGGL((k_contract_tiled_batch<T, false, PV>), dim3(total_tiles), dim3(contract_threads<T>()),  \
                                lds_bytes, st, boxes, nbox, total_tiles, skip);                                  \
     } while (0)
     switch (P) {

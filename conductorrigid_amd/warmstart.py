@@ -247,7 +247,7 @@ __all__ = ["WarmStartMLP", "set_gate", "bd_step", "sync_host", "run_trajectory",
 
 
 def compare_starts(cr, mlp: WarmStartMLP, steps: int, dt: float, field_range=(0.5, 2.0), seed: int = 0,
-                   generator=None):
+                   generator=None, log=None):
     """Continue ``cr``'s trajectory for ``steps`` Brownian steps with a new random |E0| per
     step and solve each step four times: from zero, from the previous step's solution,
     from the network's prediction, and from the previous solution plus the predicted
@@ -275,6 +275,8 @@ def compare_starts(cr, mlp: WarmStartMLP, steps: int, dt: float, field_range=(0.
         iters[mode].append(st["iterations"])
 
     for k in range(steps):
+        if log and k % 100 == 0 and k:
+            log(f"compare step {k}: mean iters " + ", ".join(f"{m} {np.mean(iters[m]):.1f}" for m in modes))
         bd_step(cr, dt, generator=gen)
         q_prev, P_prev = cr.charges.clone(), cr.potentials.clone()
         E = float(rng.uniform(*field_range))
@@ -338,7 +340,7 @@ def main(argv=None) -> int:
     del pairs
     if a.save:
         mlp.save(a.save)
-    iters, ms = compare_starts(cr, mlp, a.steps - a.train, a.dt, seed=a.seed + 1)
+    iters, ms = compare_starts(cr, mlp, a.steps - a.train, a.dt, seed=a.seed + 1, log=log)
     out = {"workload": a.workload, "beads": s.N, "precision": a.precision, "train_steps": a.train,
            "eval_steps": a.steps - a.train, "frames": len(range(0, a.train, a.collect_every)),
            "train_rows": len(range(0, a.train, a.collect_every)) * s.N, "fit_s": t_fit, "trajectory_s": t_traj,
