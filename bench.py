@@ -49,6 +49,20 @@ STAGE_KERNEL = {
 }
 
 
+def measured_traffic(kernel: str) -> float | None:
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 byte-counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "spread_n1m_fp32_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                    return float(line.split(":")[-1])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -190,7 +204,19 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=5)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        import torch
+
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_MiB": args.copy_probe, "bytes_read": 4 * n, "bytes_written": 4 * n}))
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1).", file=sys.stderr)
@@ -259,8 +285,11 @@ def main() -> int:
     log("stages " + ", ".join(f"{k}={v['ms'] * 1e3:.1f}us" for k, v in stages.items()))
     d = stages[dom]
     achieved = d["GBs"] or 0.0
+    traffic = measured_traffic(d["kernel"]) if args.workload == "n1m" else None
     roof = {"bound": "hbm", "kernel": d["kernel"], "stage": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_source": "profiles/spread_n1m_fp32_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)" if traffic
+            else None, "traffic_GBs": traffic / (d["ms"] * 1e-3) / 1e9 if traffic else None,
             "bytes_per_launch": d["bytes"], "avg_launch_ms": d["ms"]}
     sc_bytes = stages["spread"]["bytes"] + stages["contract"]["bytes"]
     sc_ms = stages["mesh_clear"]["ms"] + stages["spread"]["ms"] + stages["contract"]["ms"]
