@@ -170,6 +170,7 @@ class ConductorRigid:
         r = _lib.CrResolved()
         _lib.check(self.lib.cr_get_resolved(self.ctx, C.byref(r)))
         self.params = r.as_dict()
+        self._box = tuple(float(x) for x in self.system.L)
 
     def UpdateField(self, field) -> None:
         self.field = [float(f) for f in field]
@@ -198,6 +199,11 @@ class ConductorRigid:
 
     def computeForces(self, timestep: int = 0, x0: str | None = None, stats: bool = True) -> dict | None:
         mode = self.x0_mode if x0 is None else _X0_MODES[x0]
+        if tuple(float(x) for x in self.system.L) != self._box:
+            # the box changed (e.g. NPT): mesh spacing, Gaussian widths, k-scale and the
+            # neighbour list all depend on it (SetParams, cppmodule/ConductorRigid.cc:116-379)
+            self.SetParams()
+            self.dsys.build_nlist()
         st = _lib.CrSolveStats()
         sv = self.dsys.view()
         code = self.lib.cr_compute_force(self.ctx, C.byref(sv), self.charges.data_ptr(), self.potentials.data_ptr(),
