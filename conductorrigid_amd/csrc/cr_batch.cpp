@@ -58,7 +58,7 @@ struct cr_batch {
     int seg_blocks = 0, bead_blocks = 0, rigid_blocks = 0, total_tiles = 0;
     size_t lds_spread = 0, lds_contract = 0;
     long Gmax = 0;
-    DevBuf V, u, w, x, b, Mb, yraw, partial;
+    DevBuf V, u, w, x, b, Mb, yraw, partial, uf;
     DevBuf d_spread, d_contract, d_rows, d_segs, d_gs, d_skip, d_finished, d_step, d_pub, d_rel, d_blocks;
     int total_bodies = 0;
     std::vector<BlockBox> blocks;
@@ -161,12 +161,15 @@ int batch_matvec(cr_batch* B, const double* in, double* out, int precond, const 
     else
         CR_HIP(launch_contract_tiled_batch<double>(dc, B->nbox, B->total_tiles, P, B->lds_contract, skip, st));
     double* rows_out = blk ? B->yraw.as<double>() : out;
-    if (f32)
-        CR_HIP(launch_brows<float>(B->d_rows.as<RowBox>(), B->nbox, B->bead_blocks, B->rigid_blocks, in, rows_out,
-                                   (precond && !blk) ? 1 : 0, skip, st));
-    else
-        CR_HIP(launch_brows<double>(B->d_rows.as<RowBox>(), B->nbox, B->bead_blocks, B->rigid_blocks, in, rows_out,
-                                    (precond && !blk) ? 1 : 0, skip, st));
+    if (f32) {
+        // the pair sum's gathered charges, rounded to float once per application
+        CR_HIP((launch_convert<double, float>(in, B->uf.as<float>(), B->n_tot, st)));
+        CR_HIP(launch_brows<float>(B->d_rows.as<RowBox>(), B->nbox, B->bead_blocks, B->rigid_blocks, in,
+                                   B->uf.as<float>(), rows_out, (precond && !blk) ? 1 : 0, skip, st));
+    } else {
+        CR_HIP(launch_brows<double>(B->d_rows.as<RowBox>(), B->nbox, B->bead_blocks, B->rigid_blocks, in, in,
+                                    rows_out, (precond && !blk) ? 1 : 0, skip, st));
+    }
     if (blk)
         CR_HIP(launch_block_apply_batch(B->d_blocks.as<BlockBox>(), B->nbox, B->total_bodies, B->yraw.as<double>(), out,
                                         skip, st));
@@ -204,7 +207,7 @@ int batch_step(cr_batch* B, const std::vector<char>* host_skip, hipStream_t st)
 // Everything the captured step depends on by address or launch shape.
 std::vector<const void*> step_signature(const cr_batch* B)
 {
-    std::vector<const void*> s = {B->V.p, B->u.p, B->w.p, B->yraw.p, B->partial.p, B->d_spread.p, B->d_contract.p,
+    std::vector<const void*> s = {B->V.p, B->u.p, B->uf.p, B->w.p, B->yraw.p, B->partial.p, B->d_spread.p, B->d_contract.p,
                                   B->d_rows.p, B->d_segs.p, B->d_gs.p, B->d_skip.p, B->d_step.p, B->d_pub.p,
                                   B->d_blocks.p, B->h_pub};
     const long shape[] = {B->nbox, B->R, B->n_tot, B->ld, B->seg_blocks, B->bead_blocks, B->rigid_blocks,
@@ -474,6 +477,7 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     CR_HIP(B->b.ensure(nv));
     CR_HIP(B->Mb.ensure(nv));
     CR_HIP(B->yraw.ensure(nv));
+    CR_HIP(B->uf.ensure(sizeof(float) * (size_t)std::max(1L, B->ld)));
     CR_HIP(B->partial.ensure(sizeof(double) * 32 * (size_t)std::max(1, blk)));
     CR_HIP(B->d_spread.ensure(sizeof(TiledBox) * nbox));
     CR_HIP(B->d_contract.ensure(sizeof(TiledBox) * nbox));

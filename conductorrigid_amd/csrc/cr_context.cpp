@@ -285,6 +285,24 @@ int spread_impl(cr_context* c, const SysView& v, const double* q, hipStream_t st
     return CR_OK;
 }
 
+// Real-space rows on the assembled operator.  In FP32 bulk the gathered charges are first
+// rounded to float once per application, so the pair sum's gather moves 4-byte operands.
+template <typename T>
+int real_rows(cr_context* c, const SysView& v, const double* x, double* y, int precond, hipStream_t st)
+{
+    const T* qt = nullptr;
+    if constexpr (sizeof(T) == sizeof(double)) {
+        qt = x;
+    } else {
+        CR_HIP(c->qf.ensure(sizeof(float) * (size_t)std::max(1, v.N)));
+        CR_HIP((launch_convert<double, float>(x, c->qf.as<float>(), v.N, st)));
+        qt = c->qf.as<float>();
+    }
+    CR_HIP(launch_real_space_csr<T>(c->rp, v, x, qt, c->phi_recip.as<T>(), c->rs_ldn, c->rs_col.as<unsigned>(),
+                                    c->rs_val.as<T>(), c->rs_len.as<unsigned>(), y, precond, st));
+    return CR_OK;
+}
+
 // y = A x, or y = D A x with precond (PotentialChargeMultiply + sum_charges/set_charge,
 // cppmodule/ConductorRigid.cu:862-1020; D from precondition, :816-859).
 template <typename T>
@@ -294,9 +312,7 @@ int matvec_impl(cr_context* c, const SysView& v, const double* x, double* y, int
     CR_RET(recip_impl<T>(c, st));
     CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
     if (c->csr_ready)
-        CR_HIP(launch_real_space_csr<T>(c->rp, v, x, c->phi_recip.as<T>(), c->rs_ldn, c->rs_col.as<unsigned>(),
-                                        c->rs_val.as<T>(),
-                                        c->rs_len.as<unsigned>(), y, precond, st));
+        CR_RET(real_rows<T>(c, v, x, y, precond, st));
     else
         CR_HIP(launch_real_space<T>(c->rp, v, x, c->phi_recip.as<T>(), table<T>(c, false), y, precond, st));
     CR_HIP(launch_constraint_rows(v, x, y + v.N, precond, st));
@@ -433,8 +449,7 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
             // CGS2: two projections onto V[:,0:i+1]
             CR_HIP(launch_multidot(V, c->ld, i + 1, w, n, partial, st));
             CR_HIP(launch_reduce_partials(partial, nblk, i + 1, g->h1, st));
-            CR_HIP(launch_gs_update(V, c->ld, i + 1, g->h1, w, n, 0, partial, st));
-            CR_HIP(launch_multidot(V, c->ld, i + 1, w, n, partial, st));
+            CR_HIP(launch_gs_update_dot(V, c->ld, i + 1, g->h1, w, n, partial, st));
             CR_HIP(launch_reduce_partials(partial, nblk, i + 1, g->h2, st));
             CR_HIP(launch_gs_update(V, c->ld, i + 1, g->h2, w, n, 2, partial, st));
             CR_HIP(launch_reduce_partials(partial, nblk, 1, &g->wnorm2, st));
@@ -1042,8 +1057,7 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
             CR_HIP(hipEventRecord(ev[7], st));
             CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
             CR_HIP(hipEventRecord(ev[8], st));
-            CR_HIP(launch_real_space_csr<T>(c->rp, v, x, c->phi_recip.as<T>(), c->rs_ldn, c->rs_col.as<unsigned>(),
-                                            c->rs_val.as<T>(), c->rs_len.as<unsigned>(), y, 1, st));
+            CR_RET(real_rows<T>(c, v, x, y, 1, st));
         } else {
             using T = double;
             const AxisPlan* pl = plans<T>(c);
@@ -1065,8 +1079,7 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
             CR_HIP(hipEventRecord(ev[7], st));
             CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
             CR_HIP(hipEventRecord(ev[8], st));
-            CR_HIP(launch_real_space_csr<T>(c->rp, v, x, c->phi_recip.as<T>(), c->rs_ldn, c->rs_col.as<unsigned>(),
-                                            c->rs_val.as<T>(), c->rs_len.as<unsigned>(), y, 1, st));
+            CR_RET(real_rows<T>(c, v, x, y, 1, st));
         }
         CR_HIP(hipEventRecord(ev[9], st));
         CR_HIP(launch_constraint_rows(v, x, y + v.N, 1, st));

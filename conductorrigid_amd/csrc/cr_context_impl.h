@@ -27,6 +27,7 @@ struct cr_context {
     bool csr_ready = false;
     long rs_ldn = 0;
     DevBuf rs_col, rs_val, rs_len;
+    DevBuf qf;  // charges rounded to float for the FP32-bulk pair sum's gather
     // bead binning for the owner-computes spread
     TileGeom tg{};
     DevBuf bead_tile, tile_count, tile_start, tile_cursor, sorted, scan_sums;

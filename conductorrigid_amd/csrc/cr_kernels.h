@@ -202,9 +202,9 @@ template <typename T>
 hipError_t launch_real_csr_build(const RealParams& rp, const SysView& s, const T* table, long ldn, unsigned* col, T* val,
                                  unsigned* row_len, hipStream_t st);
 template <typename T>
-hipError_t launch_real_space_csr(const RealParams& rp, const SysView& s, const double* x, const T* phi_recip,
-                                 long ldn, const unsigned* col, const T* val, const unsigned* row_len, double* y,
-                                 int precond, hipStream_t st);
+hipError_t launch_real_space_csr(const RealParams& rp, const SysView& s, const double* x, const T* qt,
+                                 const T* phi_recip, long ldn, const unsigned* col, const T* val,
+                                 const unsigned* row_len, double* y, int precond, hipStream_t st);
 // *out = max(*out, a[i]) over i < n (out must be initialised)
 hipError_t launch_max_u32(const unsigned* a, int n, unsigned* out, hipStream_t st);
 template <typename T>
@@ -334,6 +334,9 @@ hipError_t launch_reduce_partials(const double* partial, int nblocks, int ncols,
 hipError_t launch_gs_update(const double* V, long ld, int ncols, const double* h, double* w, long n, int mode,
                             double* partial, hipStream_t st);
 // y = a*x + b*y elementwise helpers
+// w -= V[:, 0:ncols] h and partial = the projections of the updated w (layout of launch_multidot)
+hipError_t launch_gs_update_dot(const double* V, long ld, int ncols, const double* h, double* w, long n,
+                                double* partial, hipStream_t st);
 hipError_t launch_axpby(double a, const double* x, double b, double* y, long n, hipStream_t st);
 hipError_t launch_scale_copy(const double* x, const double* s_ptr, double sign, double* y, long n, hipStream_t st);
 // x += V[:,0:k] c  (c on device)
@@ -395,9 +398,10 @@ enum BatchRed { kRedH1 = 0, kRedH2 = 1, kRedWnorm = 2, kRedNorm = 3 };
 constexpr int kBatchBlock = 256;
 // Workgroups a box of n unknowns gets in segmented launches.
 int batch_blocks(long n);
+// xt: x in the row precision T (x itself for double), the real-space rows' gathered operand
 template <typename T>
-hipError_t launch_brows(const RowBox* boxes, int nbox, int bead_blocks, int rigid_blocks, const double* x, double* y,
-                        int precond, const int* skip, hipStream_t st);
+hipError_t launch_brows(const RowBox* boxes, int nbox, int bead_blocks, int rigid_blocks, const double* x,
+                        const T* xt, double* y, int precond, const int* skip, hipStream_t st);
 // partial[k * 32 + c] = sum over block k's range of V[c] . w for c < ncols, with
 // ncols = fixed_cols if > 0 else *step + 1 (device step counter).
 hipError_t launch_bdot(const double* V, long ld, int fixed_cols, const int* step, const double* w,

@@ -74,9 +74,11 @@ __device__ __forceinline__ void seg_range(const BatchSeg& s, int k, long& lo, lo
 // ---- operator rows (PotentialChargeMultiply's real-space and constraint rows,
 // cppmodule/ConductorRigid.cu:469-567, 667-761; the single-box forms are in k_real.hip)
 
+// xt: the batch vector in the row precision T (the gathered operand; x itself when T = double)
 template <typename T>
 __global__ __launch_bounds__(kB) void k_brows(const RowBox* __restrict__ boxes, int nbox, const double* __restrict__ x,
-                                              double* __restrict__ y, int precond, const int* __restrict__ skip)
+                                              const T* __restrict__ xt, double* __restrict__ y, int precond,
+                                              const int* __restrict__ skip)
 {
     const RowBox& d = boxes[owner<RowBox, &RowBox::blk0>(boxes, nbox, blockIdx.x)];
     const int b = (int)(&d - boxes);
@@ -85,11 +87,12 @@ __global__ __launch_bounds__(kB) void k_brows(const RowBox* __restrict__ boxes, 
     if (i >= d.N) return;
     const double* q = x + d.xoff;
     const double* Phi = q + d.N;
+    const T* qt = xt + d.xoff;
     const T* phi = static_cast<const T*>(d.phi_recip);
     const T* val = static_cast<const T*>(d.val);
-    T acc = phi[i] + (T)d.selfterm * (T)q[i];
+    T acc = phi[i] + (T)d.selfterm * qt[i];
     const unsigned len = d.row_len[i];
-    for (unsigned k = 0; k < len; ++k) acc += val[k * d.ldn + i] * (T)q[d.col[k * d.ldn + i]];
+    for (unsigned k = 0; k < len; ++k) acc += val[k * d.ldn + i] * qt[d.col[k * d.ldn + i]];
     const unsigned tag = d.iso[i];
     const double out = Phi[tag] - (double)acc;
     y[d.xoff + i] = precond ? -4.0 * kPi * out : out;
@@ -395,19 +398,19 @@ int batch_blocks(long n)
 }
 
 template <typename T>
-hipError_t launch_brows(const RowBox* boxes, int nbox, int bead_blocks, int rigid_blocks, const double* x, double* y,
-                        int precond, const int* skip, hipStream_t st)
+hipError_t launch_brows(const RowBox* boxes, int nbox, int bead_blocks, int rigid_blocks, const double* x,
+                        const T* xt, double* y, int precond, const int* skip, hipStream_t st)
 {
     if (bead_blocks > 0)
-        hipLaunchKernelGGL(k_brows<T>, dim3(bead_blocks), dim3(kB), 0, st, boxes, nbox, x, y, precond, skip);
+        hipLaunchKernelGGL(k_brows<T>, dim3(bead_blocks), dim3(kB), 0, st, boxes, nbox, x, xt, y, precond, skip);
     if (rigid_blocks > 0)
         hipLaunchKernelGGL(k_brigid_rows, dim3(rigid_blocks), dim3(kB), 0, st, boxes, nbox, x, y, precond, skip);
     return hipGetLastError();
 }
-template hipError_t launch_brows<float>(const RowBox*, int, int, int, const double*, double*, int, const int*,
-                                        hipStream_t);
-template hipError_t launch_brows<double>(const RowBox*, int, int, int, const double*, double*, int, const int*,
-                                         hipStream_t);
+template hipError_t launch_brows<float>(const RowBox*, int, int, int, const double*, const float*, double*, int,
+                                        const int*, hipStream_t);
+template hipError_t launch_brows<double>(const RowBox*, int, int, int, const double*, const double*, double*, int,
+                                         const int*, hipStream_t);
 
 hipError_t launch_bdot(const double* V, long ld, int fixed_cols, const int* step, const double* w,
                        const BatchSeg* segs, int nbox, int total_blocks, double* partial, const int* skip,

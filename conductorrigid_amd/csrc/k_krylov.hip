@@ -108,6 +108,47 @@ __global__ __launch_bounds__(kBlock) void k_gs_update(const double* __restrict__
     }
 }
 
+// w -= V[:, 0:ncols] h, fused with the next projection: partial[b*32 + k] = sum over the
+// block's elements of V[k][i] * w_new[i] (same layout as k_multidot), so CGS2's first
+// update and second projection read the basis once.
+__global__ __launch_bounds__(kBlock) void k_gs_update_dot(const double* __restrict__ V, long ld, int ncols,
+                                                          const double* __restrict__ h, double* __restrict__ w,
+                                                          long n, double* __restrict__ partial)
+{
+    __shared__ double hs[32];
+    if (threadIdx.x < 32) hs[threadIdx.x] = (threadIdx.x < (unsigned)ncols) ? h[threadIdx.x] : 0.0;
+    __syncthreads();
+    double acc[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) acc[k] = 0.0;
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+        double v[32];
+        double wi = w[i];
+#pragma unroll
+        for (int k = 0; k < 32; ++k)
+            if (k < ncols) {
+                v[k] = V[(long)k * ld + i];
+                wi -= hs[k] * v[k];
+            }
+        w[i] = wi;
+#pragma unroll
+        for (int k = 0; k < 32; ++k)
+            if (k < ncols) acc[k] += v[k] * wi;
+    }
+#pragma unroll
+    for (int c0 = 0; c0 < 32; c0 += kChunk) {
+        if (c0 >= ncols) break;
+        double part[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) part[k] = acc[c0 + k];
+        __shared__ double res[kChunk];
+        block_reduce_chunk(part, res);
+        if (threadIdx.x < kChunk && c0 + (int)threadIdx.x < ncols)
+            partial[(long)blockIdx.x * 32 + c0 + threadIdx.x] = res[threadIdx.x];
+        __syncthreads();
+    }
+}
+
 __global__ void k_axpby(double a, const double* __restrict__ x, double b, double* __restrict__ y, long n)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -234,6 +275,14 @@ hipError_t launch_gs_update(const double* V, long ld, int ncols, const double* h
                             double* partial, hipStream_t st)
 {
     hipLaunchKernelGGL(k_gs_update, dim3(vblocks(n)), dim3(kBlock), 0, st, V, ld, ncols, h, w, n, mode, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_gs_update_dot(const double* V, long ld, int ncols, const double* h, double* w, long n,
+                                double* partial, hipStream_t st)
+{
+    if (ncols > 32) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gs_update_dot, dim3(vblocks(n)), dim3(kBlock), 0, st, V, ld, ncols, h, w, n, partial);
     return hipGetLastError();
 }
 

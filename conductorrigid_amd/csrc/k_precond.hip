@@ -241,13 +241,50 @@ __device__ __forceinline__ void block_apply_body(int t, int N, int nmax, const u
     if (i == 0) w[N + t] = c;
 }
 
-__global__ __launch_bounds__(64) void k_block_apply(int N, int nmax, const unsigned* __restrict__ start,
-                                                    const unsigned* __restrict__ beads,
-                                                    const float* __restrict__ ginv, const float* __restrict__ uvec,
-                                                    const double* __restrict__ inv_s, const double* __restrict__ y,
-                                                    double* __restrict__ w)
+// Single system: K = 64 / nmax bodies share a wavefront (lane = body slot * nmax + bead),
+// four wavefronts per workgroup, so small bodies do not leave most lanes idle.
+__global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int K, const unsigned* __restrict__ start,
+                                                     const unsigned* __restrict__ beads,
+                                                     const float* __restrict__ ginv, const float* __restrict__ uvec,
+                                                     const double* __restrict__ inv_s, const double* __restrict__ y,
+                                                     double* __restrict__ w)
 {
-    block_apply_body(blockIdx.x, N, nmax, start, beads, ginv, uvec, inv_s, y, w);
+    __shared__ double ys[256], red[256], cs[256];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = lane / nmax, i = lane - k * nmax;
+    const int t = (blockIdx.x * 4 + wave) * K + k;
+    const bool live = k < K && t < B;
+    unsigned b0 = 0;
+    int n = 0;
+    if (live) {
+        b0 = start[t];
+        n = (int)(start[t + 1] - b0);
+    }
+    const bool bad = live && n > nmax;  // body larger than the workspace: poison the solve
+    const bool act = live && !bad && i < n;
+    const unsigned bead = act ? beads[b0 + i] : 0u;
+    const double yi = act ? y[bead] : 0.0;
+    const double ui = act ? (double)uvec[(long)t * nmax + i] : 0.0;
+    const int base = wave * 64 + k * nmax;  // this body's slots
+    ys[threadIdx.x] = yi;
+    red[threadIdx.x] = ui * yi;
+    __syncthreads();
+    double gy = 0.0;
+    if (act) {
+        const float* row = ginv + ((long)t * nmax + i) * nmax;
+        for (int j = 0; j < n; ++j) gy += (double)row[j] * ys[base + j];
+    }
+    if (live && i == 0) {
+        double dot = 0.0;
+        for (int j = 0; j < n; ++j) dot += red[base + j];
+        cs[base] = bad ? __builtin_nan("") : (dot + y[N + t]) * inv_s[t];
+    }
+    __syncthreads();
+    if (live) {
+        const double c = cs[base];
+        if (act) w[bead] = -gy + ui * c;
+        if (i == 0) w[N + t] = c;
+    }
 }
 
 // One workgroup per body of any box (body0 ascending over boxes).
@@ -322,8 +359,10 @@ hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_s
                               hipStream_t st)
 {
     if (s.N_isolated <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_block_apply, dim3(s.N_isolated), dim3(64), 0, st, s.N, nmax, body_start, body_beads, ginv,
-                       uvec, inv_s, y, w);
+    const int K = nmax <= 64 ? 64 / nmax : 1;
+    const unsigned nb = (unsigned)((s.N_isolated + 4 * K - 1) / (4 * K));
+    hipLaunchKernelGGL(k_block_apply, dim3(nb), dim3(256), 0, st, s.N_isolated, s.N, nmax, K, body_start, body_beads,
+                       ginv, uvec, inv_s, y, w);
     return hipGetLastError();
 }
 

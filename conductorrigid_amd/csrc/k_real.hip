@@ -128,21 +128,22 @@ __global__ __launch_bounds__(256) void k_real_csr_build(RealParams rp, SysView s
     row_len[i] = cnt;
 }
 
-// real_space on the assembled rows: y_i = Phi_{b(i)} - (phi_i + self q_i + sum_j A_ij q_j)
+// real_space on the assembled rows: y_i = Phi_{b(i)} - (phi_i + self q_i + sum_j A_ij q_j).
+// qt: the charges in the row precision T (the gathered operand; x itself when T = double).
 template <typename T>
 __global__ __launch_bounds__(256) void k_real_space_csr(RealParams rp, SysView s, const double* __restrict__ x,
-                                                        const T* __restrict__ phi_recip, long ldn,
-                                                        const unsigned* __restrict__ col, const T* __restrict__ val,
+                                                        const T* __restrict__ qt, const T* __restrict__ phi_recip,
+                                                        long ldn, const unsigned* __restrict__ col,
+                                                        const T* __restrict__ val,
                                                         const unsigned* __restrict__ row_len, double* __restrict__ y,
                                                         int precond)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.N) return;
-    const double* q = x;
     const double* Phi = x + s.N;
-    T acc = phi_recip[i] + (T)rp.selfterm * (T)q[i];
+    T acc = phi_recip[i] + (T)rp.selfterm * qt[i];
     const unsigned len = row_len[i];
-    for (unsigned k = 0; k < len; ++k) acc += val[k * ldn + i] * (T)q[col[k * ldn + i]];
+    for (unsigned k = 0; k < len; ++k) acc += val[k * ldn + i] * qt[col[k * ldn + i]];
     const double out = Phi[s.iso[i]] - (double)acc;
     y[i] = precond ? -4.0 * kPi * out : out;
 }
@@ -323,13 +324,13 @@ hipError_t launch_real_csr_build(const RealParams& rp, const SysView& s, const T
 }
 
 template <typename T>
-hipError_t launch_real_space_csr(const RealParams& rp, const SysView& s, const double* x, const T* phi_recip,
-                                 long ldn, const unsigned* col, const T* val, const unsigned* row_len, double* y,
-                                 int precond, hipStream_t st)
+hipError_t launch_real_space_csr(const RealParams& rp, const SysView& s, const double* x, const T* qt,
+                                 const T* phi_recip, long ldn, const unsigned* col, const T* val,
+                                 const unsigned* row_len, double* y, int precond, hipStream_t st)
 {
     if (s.N <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_real_space_csr<T>, dim3(blocks(s.N, 256)), dim3(256), 0, st, rp, s, x, phi_recip, ldn, col,
-                       val, row_len, y, precond);
+    hipLaunchKernelGGL(k_real_space_csr<T>, dim3(blocks(s.N, 256)), dim3(256), 0, st, rp, s, x, qt, phi_recip, ldn,
+                       col, val, row_len, y, precond);
     return hipGetLastError();
 }
 
@@ -425,12 +426,12 @@ template hipError_t launch_real_csr_build<float>(const RealParams&, const SysVie
                                                  float*, unsigned*, hipStream_t);
 template hipError_t launch_real_csr_build<double>(const RealParams&, const SysView&, const double*, long, unsigned*,
                                                   double*, unsigned*, hipStream_t);
-template hipError_t launch_real_space_csr<float>(const RealParams&, const SysView&, const double*, const float*, long,
-                                                 const unsigned*, const float*, const unsigned*, double*, int,
-                                                 hipStream_t);
-template hipError_t launch_real_space_csr<double>(const RealParams&, const SysView&, const double*, const double*, long,
-                                                  const unsigned*, const double*, const unsigned*, double*, int,
-                                                  hipStream_t);
+template hipError_t launch_real_space_csr<float>(const RealParams&, const SysView&, const double*, const float*,
+                                                 const float*, long, const unsigned*, const float*, const unsigned*,
+                                                 double*, int, hipStream_t);
+template hipError_t launch_real_space_csr<double>(const RealParams&, const SysView&, const double*, const double*,
+                                                  const double*, long, const unsigned*, const double*,
+                                                  const unsigned*, double*, int, hipStream_t);
 hipError_t launch_max_u32(const unsigned* a, int n, unsigned* out, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
