@@ -187,8 +187,7 @@ def bench_batch(args) -> int:
         "graph_vs_eager": res["batch_eager"]["ms_per_eval"] / res["batch_graph"]["ms_per_eval"],
         "modes": res,
         "autotune_s": t_tune,
-        "tiles": [t["tile"] + [t["column"]] for t in tunes],
-        "boxes_with_runs": sum(1 for t in tunes if t["column"] > 1),
+        "tiles": [t["tile"] for t in tunes],
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -309,7 +308,7 @@ def main() -> int:
         import conductorrigid_amd.warmstart as WS
 
         crw = ConductorRigid(system=copy.deepcopy(sysdef), precision="fp32", **common)
-        crw.set_tiles(tune32["tile"], tune32["lds_bytes"], tune32["column"])
+        crw.set_tiles(tune32["tile"], tune32["lds_bytes"])
         mlp = WS.WarmStartMLP(device=0, seed=0)
         t0 = time.perf_counter()
         _, pairs = WS.run_trajectory(crw, args.train_steps, 2e-4, collect=range(0, args.train_steps, 2), mlp=mlp,

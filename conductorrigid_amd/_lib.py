@@ -60,7 +60,7 @@ class CrSystem(C.Structure):
 
 
 class CrTuning(C.Structure):
-    _fields_ = [("tile", C.c_int * 3), ("lds_bytes", C.c_int), ("column", C.c_int), ("spread_contract_ms", C.c_double),
+    _fields_ = [("tile", C.c_int * 3), ("lds_bytes", C.c_int), ("spread_contract_ms", C.c_double),
                 ("candidates", C.c_int)]
 
 
@@ -97,7 +97,7 @@ _SIGS = {
     "cr_bd_step": (C.c_int, [vp, C.POINTER(CrSystem), vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                              C.c_double, C.c_double, vp]),
     "cr_autotune": (C.c_int, [vp, C.POINTER(CrSystem), C.POINTER(CrTuning), vp]),
-    "cr_set_tiles": (C.c_int, [vp, C.POINTER(C.c_int), C.c_int, C.c_int]),
+    "cr_set_tiles": (C.c_int, [vp, C.POINTER(C.c_int), C.c_int]),
     "cr_mlp_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_ulonglong]),
     "cr_mlp_destroy": (None, [vp]),
     "cr_mlp_set_params": (C.c_int, [vp, vp]),

@@ -222,13 +222,11 @@ class ConductorRigid:
         t = _lib.CrTuning()
         sv = self.dsys.view()
         _lib.check(self.lib.cr_autotune(self.ctx, C.byref(sv), C.byref(t), self.stream()))
-        return {"tile": list(t.tile), "lds_bytes": t.lds_bytes, "column": t.column,
-                "spread_contract_ms": t.spread_contract_ms, "candidates": t.candidates}
+        return {"tile": list(t.tile), "lds_bytes": t.lds_bytes, "spread_contract_ms": t.spread_contract_ms,
+                "candidates": t.candidates}
 
-    def set_tiles(self, tile, lds_bytes: int = 64 * 1024, column: int = 1) -> None:
-        """Tile shape, LDS budget and tiles per workgroup run along x (C ABI ``cr_set_tiles``)."""
-        _lib.check(self.lib.cr_set_tiles(self.ctx, (C.c_int * 3)(*[int(v) for v in tile]), int(lds_bytes),
-                                         int(column)))
+    def set_tiles(self, tile, lds_bytes: int = 64 * 1024) -> None:
+        _lib.check(self.lib.cr_set_tiles(self.ctx, (C.c_int * 3)(*[int(v) for v in tile]), int(lds_bytes)))
 
     def OutputData(self, timestep: int) -> str:
         """Text dump in tag order (cppmodule/ConductorRigid.cc:560-622)."""

@@ -383,7 +383,7 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
         tb.rec_off = bv.rec_off;
         tb.rec_w = bv.rec_w;
         tb.phi = c->phi_recip.p;
-        tiles += c->tg.ntiles / std::max(1, c->tg.seg);
+        tiles += c->tg.ntiles;
         Gmax = std::max(Gmax, c->G);
         B->spread_boxes[b] = tb;
         B->spread_boxes[b].ch = ls.chunk;

@@ -737,9 +737,7 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
     const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
     {
         TileGeom& tg = c->tg;
-        tg = TileGeom{};
         tg.valid = 1;
-        tg.seg = 1;
         const int lo = std::max(r.P - 1, 4);
         std::vector<int> opts[3];
         for (int d = 0; d < 3; ++d) {
@@ -1179,7 +1177,7 @@ int cr_bd_step(cr_context* c, const cr_system* s, double* pos, double* rigid_com
 
 // ---- tile autotuner ----
 
-int cr_set_tiles(cr_context* c, const int tile[3], int lds_bytes, int column)
+int cr_set_tiles(cr_context* c, const int tile[3], int lds_bytes)
 {
     if (!c || !c->ready || !tile) return set_error(CR_ERR_STATE, "cr_set_params has not been called");
     const int P = c->mp.P, Nd[3] = {c->mp.Nx, c->mp.Ny, c->mp.Nz};
@@ -1198,8 +1196,6 @@ int cr_set_tiles(cr_context* c, const int tile[3], int lds_bytes, int column)
         tg.nt[d] = Nd[d] / tg.T[d];
         tg.ntiles *= tg.nt[d];
     }
-    tg.seg = column > 0 ? column : 1;
-    if (tg.nt[0] % tg.seg != 0) return set_error(CR_ERR_INVALID, "the run length must divide the tiles along x");
     const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
     if (tiled_lds(c->mp, tg, sizeof(SpreadAcc), Tsz).chunk < 16)
         return set_error(CR_ERR_INVALID, "extended tile does not fit the LDS budget");
@@ -1245,14 +1241,7 @@ int cr_autotune(cr_context* c, const cr_system* s, cr_tuning* out, void* stream)
                         break;
                     }
                 }
-                if (!budget) continue;
-                for (int seg = 1; seg <= 16; seg *= 2) {
-                const int ntx = Nd[0] / tx;
-                // runs must divide the column; longer runs only while at least one
-                // workgroup per CU remains
-                if (ntx % seg != 0) continue;
-                if (seg > 1 && (long)(Nd[0] / tx) * (Nd[1] / ty) * (Nd[2] / tz) / seg < 256) continue;
-                if (cr_set_tiles(c, T3, budget, seg) != CR_OK) continue;
+                if (!budget || cr_set_tiles(c, T3, budget) != CR_OK) continue;
                 CR_RET(prepare(c, s, st, &v));
                 if (!q) {  // any charges will do for timing: the bead records
                     CR_HIP(c->yraw.ensure(sizeof(double) * (size_t)std::max(1, v.N)));
@@ -1281,7 +1270,6 @@ int cr_autotune(cr_context* c, const cr_system* s, cr_tuning* out, void* stream)
                     best = ms / 3.0;
                     best_tg = c->tg;
                 }
-                }
             }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1290,7 +1278,6 @@ int cr_autotune(cr_context* c, const cr_system* s, cr_tuning* out, void* stream)
     if (out) {
         for (int d = 0; d < 3; ++d) out->tile[d] = c->tg.T[d];
         out->lds_bytes = c->tg.lds;
-        out->column = std::max(1, c->tg.seg);
         out->spread_contract_ms = tried ? best : -1.0;
         out->candidates = tried;
     }

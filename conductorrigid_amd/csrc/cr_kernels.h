@@ -60,7 +60,6 @@ struct TileGeom {
     int ntiles;
     int valid;     // 0: no admissible tiling, use the atomic spread
     int lds;       // LDS bytes a binned workgroup may use (extended tile + staging)
-    int seg;       // tiles per workgroup run along x (divides nt[0]; 0 or 1: one tile)
 };
 
 // ---- k_se.hip ----
@@ -119,7 +118,7 @@ struct TiledBox {
     MeshParams mp;
     TileGeom tg;
     int ch;                      // staging chunk within the launch's LDS size
-    int tile0;                   // first global work unit (run of tg.seg tiles) of this box
+    int tile0;                   // first global tile of this box
     int box;                     // index into the skip flags
     long G;                      // mesh nodes
     const unsigned* tile_start;  // [ntiles + 1]

@@ -6,11 +6,11 @@
 //
 // Here beads are binned by the mesh tile holding the first node of their window.
 // Tile edges divide the mesh and are >= P-1 nodes, so a window stays inside the tile
-// extended by P-1 nodes on the + side of each axis.  One workgroup owns one tile (or,
-// as a tuning option, a run of seg tiles along x, below) and works through its bin in
-// chunks: the chunk's bead records are loaded together, and then each wavefront takes
-// one bead at a time, lanes over the (y, z) window plane, walking x -- every operand of
-// the inner loop comes from LDS.
+// extended by P-1 nodes on the + side of each axis.  One workgroup owns one tile and
+// works through its bin in chunks: the chunk's bead records are loaded together, the
+// 3P separable axis weights of every bead are evaluated by all threads into an LDS
+// table, and then each wavefront takes one bead at a time, lanes over the (y, z)
+// window plane, walking x -- every operand of the inner loop comes from LDS.
 //   spread:   window products are added into an LDS copy of the extended tile, which
 //             is then added into the (cleared) mesh with one global atomic per node:
 //             each node receives the sums of the at most 8 tiles whose extension covers
@@ -18,12 +18,6 @@
 //             from per-tile scratch in a second pass, this moves fewer bytes.
 //   contract: the extended tile is read into LDS once; each wavefront reduces its
 //             bead's window across lanes.
-// Runs (tg.seg > 1): the workgroup keeps a circular LDS buffer of Tx + P - 1 mesh planes
-// of one (y, z) tile column; the +x extension of one tile is the start of the next, so
-// each plane is loaded (contraction) or written out (spread) once per run, and spread
-// nodes no other workgroup touches are stored instead of added atomically.  The window
-// addressing then wraps around the buffer, which costs VALU work in the inner loops:
-// the autotuner keeps whichever is faster for the system at hand.
 // Per bead, all node updates stay in LDS.
 #include "se_device.h"
 
@@ -100,61 +94,6 @@ __device__ __forceinline__ TileCoords tile_coords(const MeshParams& mp, const Ti
     c.xs = c.Ey * c.ys;
     c.esz = c.Ex * c.xs;
     return c;
-}
-
-// Work unit of the binned kernels: a run of `seg` consecutive tiles along x in one (y, z)
-// tile column (seg = 1: one tile).  The workgroup keeps a circular LDS buffer of
-// NP = Tx + P - 1 mesh planes, each the tiles' (y, z) extent plus P - 1 nodes.  The +x
-// extension of one tile is the start of the next, so along a run every plane is loaded
-// (contraction) or written out (spread) once instead of once per tile that covers it.
-struct UnitCoords {
-    int sx, ty, tz;        // run index along x, tile column
-    int seg, NP;           // tiles per run, planes in the circular buffer
-    int x_start, y0, z0;   // first mesh plane / row / column of the run
-    int Ey, Ez, ys, ps, esz;  // (y, z) extents, LDS z stride, plane stride, buffer elements
-};
-
-__device__ __forceinline__ UnitCoords unit_coords(const MeshParams& mp, const TileGeom& tg, int u)
-{
-    UnitCoords c;
-    c.seg = tg.seg > 0 ? tg.seg : 1;
-    c.tz = u % tg.nt[2];
-    c.ty = (u / tg.nt[2]) % tg.nt[1];
-    c.sx = u / (tg.nt[2] * tg.nt[1]);
-    c.x_start = c.sx * c.seg * tg.T[0];
-    c.y0 = c.ty * tg.T[1];
-    c.z0 = c.tz * tg.T[2];
-    c.NP = tg.T[0] + mp.P - 1;
-    c.Ey = tg.T[1] + mp.P - 1;
-    c.Ez = tg.T[2] + mp.P - 1;
-    c.ys = ext_ys(c.Ez);
-    c.ps = c.Ey * c.ys;
-    c.esz = c.NP * c.ps;
-    return c;
-}
-
-// Tile i of a run (its bin and origin, for staging).
-__device__ __forceinline__ TileCoords step_tile(const TileGeom& tg, const UnitCoords& uc, int i)
-{
-    TileCoords c;
-    const int tx = uc.sx * uc.seg + i;
-    c.t = (tx * tg.nt[1] + uc.ty) * tg.nt[2] + uc.tz;
-    c.x0 = tx * tg.T[0];
-    c.y0 = uc.y0;
-    c.z0 = uc.z0;
-    c.Ex = uc.NP;
-    c.Ey = uc.Ey;
-    c.Ez = uc.Ez;
-    c.ys = uc.ys;
-    c.xs = uc.ps;
-    c.esz = uc.esz;
-    return c;
-}
-
-// Units of a tiling: runs of seg tiles along x.
-__host__ __device__ __forceinline__ int tiling_units(const TileGeom& tg)
-{
-    return tg.ntiles / (tg.seg > 0 ? tg.seg : 1);
 }
 
 // LDS: [ext Ex*Ey*Ez (accumulator type)][w ch*3P T][u ch*3 T][qs ch T][off ch int][ids ch uint]
@@ -362,96 +301,6 @@ __device__ __forceinline__ void spread_tile(const MeshParams& mp, const TileGeom
 }
 
 template <typename T, typename A, int PT>
-__device__ __forceinline__ void spread_run(const MeshParams& mp, const TileGeom& tg, int u, int ch,
-                                            const unsigned* __restrict__ tstart,
-                                            const unsigned* __restrict__ sorted, const int* __restrict__ rec_off,
-                                            const T* __restrict__ rec_w, const double* __restrict__ q,
-                                            T* __restrict__ grid, char* smem_raw)
-{
-    const UnitCoords uc = unit_coords(mp, tg, u);
-    constexpr int PM = PT > 0 ? PT : 16;
-    const int P = PT > 0 ? PT : mp.P;
-    const int T0 = tg.T[0], T1 = tg.T[1], T2 = tg.T[2];
-    const int NP = uc.NP, ps = uc.ps, ys = uc.ys, per = 3 * P;
-    A* ext = reinterpret_cast<A*>(smem_raw);
-    Stage<T> s = stage_layout<T>(smem_raw, (size_t)uc.esz * sizeof(A), ch, P);
-    for (int k = threadIdx.x; k < uc.esz; k += blockDim.x) ext[k] = (A)0;
-    // the write-outs below read nodes other threads zeroed, also when a bin is empty
-    __syncthreads();
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    const long plane = (long)mp.Ny * mp.Nz;
-    int sb = 0;  // LDS slot of the current tile's first plane
-    for (int i = 0; i < uc.seg; ++i) {
-        const TileCoords tc = step_tile(tg, uc, i);
-        const unsigned kb = tstart[tc.t], ke = tstart[tc.t + 1];
-        for (unsigned k0 = kb; k0 < ke; k0 += ch) {
-            const int cnt = (int)min((unsigned)ch, ke - k0);
-            stage_records<T>(P, sorted, k0, cnt, rec_off, rec_w, q, (T)mp.prefac, s);
-            for (int b = wave; b < cnt; b += nwave) {
-                const int o = s.off[b];
-                const int ox = o & 1023, oy = (o >> 10) & 1023, oz = o >> 20;
-                int s0 = sb + ox;  // slot of the window's first plane; planes from wrap on wrap around
-                s0 -= s0 >= NP ? NP : 0;
-                const int wrap = NP - s0;
-                const T* wi = s.w + b * per;
-                const T qs = s.qs[b];
-                T wx[PM];
-#pragma unroll
-                for (int px = 0; px < PM; ++px) wx[px] = (px < P) ? wi[px] : (T)0;
-                for (int pair = lane; pair < P * P; pair += kWave) {
-                    const int py = pair / P, pz = pair - py * P;
-                    const T c = qs * wi[P + py] * wi[2 * P + pz];
-                    A* col = ext + s0 * ps + (oy + py) * ys + (oz + pz);
-#pragma unroll
-                    for (int px = 0; px < PM; ++px)
-                        if (px < P) atomic_add(col + px * ps - (px >= wrap ? NP * ps : 0), (A)(c * wx[px]));
-                }
-            }
-            __syncthreads();
-        }
-        // this tile's planes are complete: add them into the mesh and clear their slots.
-        // Nodes no other workgroup touches (inside the tile column's own (y, z) range and
-        // not in the run's first P - 1 planes) are stored; the others are added atomically.
-        const int nodes = T0 * uc.Ey * uc.Ez;
-        BoxWalk bw(threadIdx.x, blockDim.x, uc.Ey, uc.Ez);
-        for (int k = threadIdx.x; k < nodes; k += blockDim.x, bw.next()) {
-            int sl = sb + bw.lx;
-            sl -= sl >= NP ? NP : 0;
-            A* e = ext + sl * ps + bw.ly * ys + bw.lz;
-            const T v = (T)*e;
-            *e = (A)0;
-            int gx = tc.x0 + bw.lx, gy = uc.y0 + bw.ly, gz = uc.z0 + bw.lz;
-            gx -= gx >= mp.Nx ? mp.Nx : 0;
-            gy -= gy >= mp.Ny ? mp.Ny : 0;
-            gz -= gz >= mp.Nz ? mp.Nz : 0;
-            T* dst = &grid[(long)gx * plane + (long)gy * mp.Nz + gz];
-            const bool own = bw.ly >= P - 1 && bw.ly < T1 && bw.lz >= P - 1 && bw.lz < T2 && (i > 0 || bw.lx >= P - 1);
-            if (own)
-                *dst = v;
-            else
-                atomic_add(dst, v);
-        }
-        __syncthreads();  // slots cleared before the next tile accumulates into them
-        sb += T0;
-        sb -= sb >= NP ? NP : 0;
-    }
-    // the extension beyond the run: P - 1 planes shared with the next run
-    const int tail = (P - 1) * uc.Ey * uc.Ez;
-    const int xe = uc.x_start + uc.seg * T0;
-    BoxWalk bw(threadIdx.x, blockDim.x, uc.Ey, uc.Ez);
-    for (int k = threadIdx.x; k < tail; k += blockDim.x, bw.next()) {
-        int sl = sb + bw.lx;
-        sl -= sl >= NP ? NP : 0;
-        int gx = xe + bw.lx, gy = uc.y0 + bw.ly, gz = uc.z0 + bw.lz;
-        gx -= gx >= mp.Nx ? mp.Nx : 0;
-        gy -= gy >= mp.Ny ? mp.Ny : 0;
-        gz -= gz >= mp.Nz ? mp.Nz : 0;
-        atomic_add(&grid[(long)gx * plane + (long)gy * mp.Nz + gz], (T)ext[sl * ps + bw.ly * ys + bw.lz]);
-    }
-}
-
-// RUN: runs of tg.seg > 1 tiles per workgroup (circular plane buffer); else one tile.
-template <typename T, typename A, int PT, bool RUN>
 __global__ __launch_bounds__(kSpreadThreads) void k_spread_tiled(MeshParams mp, TileGeom tg, int ch,
                                                       const unsigned* __restrict__ tstart,
                                                       const unsigned* __restrict__ sorted,
@@ -459,11 +308,8 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread_tiled(MeshParams mp, 
                                                       const double* __restrict__ q, T* __restrict__ grid)
 {
     extern __shared__ char smem_raw[];
-    const int u = xcd_tile(blockIdx.x, tiling_units(tg));
-    if constexpr (RUN)
-        spread_run<T, A, PT>(mp, tg, u, ch, tstart, sorted, rec_off, rec_w, q, grid, smem_raw);
-    else
-        spread_tile<T, A, PT>(mp, tg, u, ch, tstart, sorted, rec_off, rec_w, q, grid, smem_raw);
+    spread_tile<T, A, PT>(mp, tg, xcd_tile(blockIdx.x, tg.ntiles), ch, tstart, sorted, rec_off, rec_w, q, grid,
+                          smem_raw);
 }
 
 template <typename T, bool FORCE, int PT>
@@ -555,110 +401,6 @@ __device__ __forceinline__ void contract_tile(const MeshParams& mp, const TileGe
 }
 
 template <typename T, bool FORCE, int PT>
-__device__ __forceinline__ void contract_run(const MeshParams& mp, const TileGeom& tg, int u, int ch,
-                                              const unsigned* __restrict__ tstart,
-                                              const unsigned* __restrict__ sorted, const int* __restrict__ rec_off,
-                                              const T* __restrict__ rec_w, const double4* __restrict__ pos,
-                                              const double* __restrict__ q, const T* __restrict__ grid,
-                                              T* __restrict__ phi, double* __restrict__ force, char* smem_raw)
-{
-    const UnitCoords uc = unit_coords(mp, tg, u);
-    constexpr int PM = PT > 0 ? PT : 16;
-    const int P = PT > 0 ? PT : mp.P;
-    const int T0 = tg.T[0], NP = uc.NP, ps = uc.ps, ys = uc.ys, per = 3 * P;
-    T* ext = reinterpret_cast<T*>(smem_raw);
-    Stage<T> s = stage_layout<T>(smem_raw, (size_t)uc.esz * sizeof(T), ch, P);
-    const long plane = (long)mp.Ny * mp.Nz;
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    const T pc = (T)(mp.quadW * mp.prefac);
-    int sb = 0;  // LDS slot of the current tile's first plane
-    for (int i = 0; i < uc.seg; ++i) {
-        const TileCoords tc = step_tile(tg, uc, i);
-        // planes the tile needs and the buffer lacks: all NP at the run's start, then the T0
-        // planes past the previous tile's extension
-        const int l0 = i == 0 ? 0 : P - 1, nl = i == 0 ? NP : T0;
-        const int nodes = nl * uc.Ey * uc.Ez;
-        BoxWalk bw(threadIdx.x, blockDim.x, uc.Ey, uc.Ez);
-        for (int k = threadIdx.x; k < nodes; k += blockDim.x, bw.next()) {
-            const int lx = l0 + bw.lx;
-            int sl = sb + lx;
-            sl -= sl >= NP ? NP : 0;
-            int gx = tc.x0 + lx, gy = uc.y0 + bw.ly, gz = uc.z0 + bw.lz;
-            gx -= gx >= mp.Nx ? mp.Nx : 0;
-            gy -= gy >= mp.Ny ? mp.Ny : 0;
-            gz -= gz >= mp.Nz ? mp.Nz : 0;
-            ext[sl * ps + bw.ly * ys + bw.lz] = grid[(long)gx * plane + (long)gy * mp.Nz + gz];
-        }
-        const unsigned kb = tstart[tc.t], ke = tstart[tc.t + 1];
-        for (unsigned k0 = kb; k0 < ke; k0 += ch) {
-            const int cnt = (int)min((unsigned)ch, ke - k0);
-            if (FORCE)
-                stage_chunk<T>(mp, tc, sorted, k0, cnt, pos, q, (T)1, s);
-            else
-                stage_records<T>(P, sorted, k0, cnt, rec_off, rec_w, nullptr, (T)1, s);
-            for (int b = wave; b < cnt; b += nwave) {
-                const int o = s.off[b];
-                const int ox = o & 1023, oy = (o >> 10) & 1023, oz = o >> 20;
-                int s0 = sb + ox;
-                s0 -= s0 >= NP ? NP : 0;
-                const int wrap = NP - s0;
-                const T* wi = s.w + b * per;
-                const T ux = FORCE ? s.u[3 * b] : (T)0, uy = FORCE ? s.u[3 * b + 1] : (T)0;
-                const T uz = FORCE ? s.u[3 * b + 2] : (T)0;
-                T wx[PM];
-#pragma unroll
-                for (int px = 0; px < PM; ++px) wx[px] = (px < P) ? wi[px] : (T)0;
-                T acc = 0, fx = 0, fy = 0, fz = 0;
-                for (int pair = lane; pair < P * P; pair += kWave) {
-                    const int py = pair / P, pz = pair - py * P;
-                    const T wyz = wi[P + py] * wi[2 * P + pz];
-                    const T* col = ext + s0 * ps + (oy + py) * ys + (oz + pz);
-                    T g[PM];
-#pragma unroll
-                    for (int px = 0; px < PM; ++px) g[px] = (px < P) ? col[px * ps - (px >= wrap ? NP * ps : 0)] : (T)0;
-                    T sx = 0, sxr = 0;
-#pragma unroll
-                    for (int px = 0; px < PM; ++px) {
-                        const T wg = wx[px] * g[px];
-                        sx += wg;
-                        if (FORCE) sxr += wg * (ux - (T)px);
-                    }
-                    acc += wyz * sx;
-                    if (FORCE) {
-                        fx += wyz * sxr;
-                        fy += wyz * sx * (uy - (T)py);
-                        fz += wyz * sx * (uz - (T)pz);
-                    }
-                }
-                const unsigned bead = s.ids[b];
-                if (!FORCE) {
-                    acc = wave_total(acc);
-                    if (lane == 0) phi[bead] = acc * pc;
-                } else {
-                    fx = wave_total(fx);
-                    fy = wave_total(fy);
-                    fz = wave_total(fz);
-                    if (lane == 0) {
-                        // F = 2 (h^3 prefac) (2 xi^2) q_i sum phi_g e^{...} (r/eta), r = bead - node = (u - p) h
-                        // (cppmodule/ConductorRigid.cu:449)
-                        const double c = 2.0 * (mp.quadW * mp.prefac) * mp.xiterm * (double)s.qs[b];
-                        force[4 * bead + 0] = c * mp.h[0] / mp.eta[0] * (double)fx;
-                        force[4 * bead + 1] = c * mp.h[1] / mp.eta[1] * (double)fy;
-                        force[4 * bead + 2] = c * mp.h[2] / mp.eta[2] * (double)fz;
-                        force[4 * bead + 3] = 0.0;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        // every read of this tile's planes is done before the next loads reuse their slots
-        __syncthreads();
-        sb += T0;
-        sb -= sb >= NP ? NP : 0;
-    }
-}
-
-template <typename T, bool FORCE, int PT, bool RUN>
 __global__ __launch_bounds__(contract_threads<T>()) void k_contract_tiled(MeshParams mp, TileGeom tg, int ch,
                                                         const unsigned* __restrict__ tstart,
                                                         const unsigned* __restrict__ sorted,
@@ -668,12 +410,8 @@ __global__ __launch_bounds__(contract_threads<T>()) void k_contract_tiled(MeshPa
                                                         double* __restrict__ force)
 {
     extern __shared__ char smem_raw[];
-    const int u = xcd_tile(blockIdx.x, tiling_units(tg));
-    if constexpr (RUN)
-        contract_run<T, FORCE, PT>(mp, tg, u, ch, tstart, sorted, rec_off, rec_w, pos, q, grid, phi, force, smem_raw);
-    else
-        contract_tile<T, FORCE, PT>(mp, tg, u, ch, tstart, sorted, rec_off, rec_w, pos, q, grid, phi, force,
-                                    smem_raw);
+    contract_tile<T, FORCE, PT>(mp, tg, xcd_tile(blockIdx.x, tg.ntiles), ch, tstart, sorted, rec_off, rec_w, pos, q,
+                                grid, phi, force, smem_raw);
 }
 
 // Batched launches: one workgroup per tile of any box.  Boxes own consecutive ranges of
@@ -708,12 +446,8 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread_tiled_batch(const Til
     const int g = xcd_tile(blockIdx.x, total);
     const TiledBox& d = boxes[batch_box(boxes, nbox, g)];
     if (skip && skip[d.box]) return;
-    if (d.tg.seg > 1)
-        spread_run<T, A, PT>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off,
-                             static_cast<const T*>(d.rec_w), d.q, static_cast<T*>(d.grid), smem_raw);
-    else
-        spread_tile<T, A, PT>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off,
-                              static_cast<const T*>(d.rec_w), d.q, static_cast<T*>(d.grid), smem_raw);
+    spread_tile<T, A, PT>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off,
+                          static_cast<const T*>(d.rec_w), d.q, static_cast<T*>(d.grid), smem_raw);
 }
 
 template <typename T, bool FORCE, int PT>
@@ -724,16 +458,10 @@ __global__ __launch_bounds__(contract_threads<T>()) void k_contract_tiled_batch(
     const int g = xcd_tile(blockIdx.x, total);
     const TiledBox& d = boxes[batch_box(boxes, nbox, g)];
     if (skip && skip[d.box]) return;
-    if (d.tg.seg > 1)
-        contract_run<T, FORCE, PT>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off,
-                                   static_cast<const T*>(d.rec_w), d.pos, FORCE ? d.q : nullptr,
-                                   static_cast<const T*>(d.grid), static_cast<T*>(d.phi), FORCE ? d.force : nullptr,
-                                   smem_raw);
-    else
-        contract_tile<T, FORCE, PT>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off,
-                                    static_cast<const T*>(d.rec_w), d.pos, FORCE ? d.q : nullptr,
-                                    static_cast<const T*>(d.grid), static_cast<T*>(d.phi),
-                                    FORCE ? d.force : nullptr, smem_raw);
+    contract_tile<T, FORCE, PT>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off,
+                                static_cast<const T*>(d.rec_w), d.pos, FORCE ? d.q : nullptr,
+                                static_cast<const T*>(d.grid), static_cast<T*>(d.phi), FORCE ? d.force : nullptr,
+                                smem_raw);
 }
 
 inline unsigned blocks(long n, int b)
@@ -797,21 +525,14 @@ hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const B
         if (e != hipSuccess) return e;
     }
     const T* rec_w = static_cast<const T*>(bv.rec_w);
-#define CR_SPREAD_PR(PV, RUN)                                                                                  \
+#define CR_SPREAD_P(PV)                                                                                        \
     do {                                                                                                       \
         if (l.total_bytes > 65536)                                                                             \
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_tiled<T, SpreadAcc, PV, RUN>),  \
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_tiled<T, SpreadAcc, PV>),       \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total_bytes);         \
-        hipLaunchKernelGGL((k_spread_tiled<T, SpreadAcc, PV, RUN>), dim3(tiling_units(tg)), dim3(kSpreadThreads), \
+        hipLaunchKernelGGL((k_spread_tiled<T, SpreadAcc, PV>), dim3(tg.ntiles), dim3(kSpreadThreads),          \
                            l.total_bytes, st, mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, q, \
                            grid);                                                                              \
-    } while (0)
-#define CR_SPREAD_P(PV)                      \
-    do {                                     \
-        if (tg.seg > 1)                      \
-            CR_SPREAD_PR(PV, true);          \
-        else                                 \
-            CR_SPREAD_PR(PV, false);         \
     } while (0)
     switch (mp.P) {
         case 4: CR_SPREAD_P(4); break;
@@ -824,7 +545,6 @@ hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const B
         default: CR_SPREAD_P(0); break;
     }
 #undef CR_SPREAD_P
-#undef CR_SPREAD_PR
     return hipGetLastError();
 }
 
@@ -835,30 +555,22 @@ hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const
     const TiledLds l = tiled_lds(mp, tg, sizeof(T), sizeof(T));
     if (l.chunk < 4 || mp.P > 16) return hipErrorInvalidValue;
     const T* rec_w = static_cast<const T*>(bv.rec_w);
-#define CR_CONTRACT_PR(PV, RUN)                                                                                 \
+#define CR_CONTRACT_P(PV)                                                                                       \
     do {                                                                                                        \
         if (l.total_bytes > 65536) {                                                                            \
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract_tiled<T, true, PV, RUN>),       \
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract_tiled<T, true, PV>),            \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total_bytes);          \
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract_tiled<T, false, PV, RUN>),      \
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract_tiled<T, false, PV>),           \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total_bytes);          \
         }                                                                                                       \
         if (force)                                                                                              \
-            hipLaunchKernelGGL((k_contract_tiled<T, true, PV, RUN>), dim3(tiling_units(tg)),                    \
-                               dim3(contract_threads<T>()), l.total_bytes, st, mp, tg, l.chunk, bv.tile_start,  \
-                               bv.sorted, bv.rec_off, rec_w, s.pos, q, grid, (T*)nullptr, force);               \
+            hipLaunchKernelGGL((k_contract_tiled<T, true, PV>), dim3(tg.ntiles), dim3(contract_threads<T>()), l.total_bytes, st,  \
+                               mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, s.pos, q, grid,    \
+                               (T*)nullptr, force);                                                             \
         else                                                                                                    \
-            hipLaunchKernelGGL((k_contract_tiled<T, false, PV, RUN>), dim3(tiling_units(tg)),                   \
-                               dim3(contract_threads<T>()), l.total_bytes, st, mp, tg, l.chunk, bv.tile_start,  \
-                               bv.sorted, bv.rec_off, rec_w, s.pos, (const double*)nullptr, grid, phi,          \
-                               (double*)nullptr);                                                               \
-    } while (0)
-#define CR_CONTRACT_P(PV)                    \
-    do {                                     \
-        if (tg.seg > 1)                      \
-            CR_CONTRACT_PR(PV, true);        \
-        else                                 \
-            CR_CONTRACT_PR(PV, false);       \
+            hipLaunchKernelGGL((k_contract_tiled<T, false, PV>), dim3(tg.ntiles), dim3(contract_threads<T>()), l.total_bytes, st, \
+                               mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, s.pos,             \
+                               (const double*)nullptr, grid, phi, (double*)nullptr);                            \
     } while (0)
     switch (mp.P) {
         case 4: CR_CONTRACT_P(4); break;
@@ -871,7 +583,6 @@ hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const
         default: CR_CONTRACT_P(0); break;
     }
 #undef CR_CONTRACT_P
-#undef CR_CONTRACT_PR
     return hipGetLastError();
 }
 

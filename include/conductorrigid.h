@@ -259,22 +259,18 @@ int cr_bd_step(cr_context* ctx, const cr_system* sys, double* pos, double* rigid
 /* ---- spread/contract tiling ---------------------------------------------
  * No reference counterpart (the reference launches one thread block per bead).  The
  * binned spread and contraction work on mesh tiles whose edges divide the mesh and are
- * at least P - 1.  Each workgroup takes a run of `column` consecutive tiles along x and
- * holds Tx + P - 1 mesh planes of the tiles' (y, z) extent plus P - 1 in LDS, so that
- * the x-extension of one tile is reused by the next.  cr_set_params picks a tiling by
- * rule (runs of one tile); cr_autotune times the spread + contraction of the given
- * system for every admissible tile shape (edges 4..32, each with the smallest LDS budget
- * that holds it) and run length, and keeps the fastest; cr_set_tiles sets one. */
+ * at least P - 1; each workgroup holds its tile extended by P - 1 nodes per axis in LDS.
+ * cr_set_params picks a tiling by rule; cr_autotune times the spread + contraction of
+ * the given system for every admissible tile shape (edges 4..32, each with the smallest
+ * LDS budget that holds it) and keeps the fastest; cr_set_tiles sets one explicitly. */
 typedef struct {
     int tile[3];
     int lds_bytes;
-    int column;                  /* tiles per workgroup run along x */
     double spread_contract_ms;   /* best measured time of one spread + contraction */
-    int candidates;              /* tilings timed */
+    int candidates;              /* tile shapes timed */
 } cr_tuning;
 int cr_autotune(cr_context* ctx, const cr_system* sys, cr_tuning* out, void* stream);
-/* column: tiles per run along x, dividing the number of tiles along x (0 = 1) */
-int cr_set_tiles(cr_context* ctx, const int tile[3], int lds_bytes, int column);
+int cr_set_tiles(cr_context* ctx, const int tile[3], int lds_bytes);
 
 /* ---- batched boxes ------------------------------------------------------
  * No reference counterpart: the reference computes one box per computeForces call
