@@ -188,17 +188,30 @@ __global__ __launch_bounds__(256) void k_real_space_force(RealParams rp, SysView
 
 // sum_charges: y_bottom[tag(r)] = sum of bead charges of rigid body r (FP64), optionally
 // scaled by the preconditioner's 1/size.
-__global__ void k_sum_charges(SysView s, const double* __restrict__ q, double* __restrict__ yb, int precond)
+// K = 64 / W rigid bodies per wavefront with W = min(Nmax, 64) lanes each; lane m of a
+// body sums members m, m + W, ...: the member loads are independent, and the per-body
+// sum is a fixed-order pass over LDS.
+__global__ __launch_bounds__(256) void k_sum_charges(SysView s, int W, const double* __restrict__ q,
+                                                     double* __restrict__ yb, int precond)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= s.N_rigid) return;
-    const unsigned size = s.rigid_size[r];
-    if (size == 0) return;
+    __shared__ double part[256];
+    const int K = 64 / W;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = lane / W, m = lane - k * W;
+    const int r = (blockIdx.x * 4 + wave) * K + k;
+    const bool live = k < K && r < s.N_rigid;
+    const unsigned size = live ? s.rigid_size[r] : 0u;
     const unsigned* mem = s.rigid_members + (long)r * s.Nmax;
     double acc = 0.0;
-    for (unsigned k = 0; k < size; ++k) acc += q[mem[k]];
-    const unsigned tag = s.iso[mem[0]];
-    yb[tag] = precond ? acc / (double)size : acc;
+    for (unsigned j = (unsigned)m; j < size; j += (unsigned)W) acc += q[mem[j]];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (live && m == 0 && size > 0) {
+        const double* p = part + wave * 64 + k * W;
+        double t = 0.0;
+        for (int j = 0; j < W; ++j) t += p[j];
+        yb[s.iso[mem[0]]] = precond ? t / (double)size : t;
+    }
 }
 
 // set_charge: isolated beads carry their own body's charge row.
@@ -346,9 +359,14 @@ hipError_t launch_real_space_force(const RealParams& rp, const SysView& s, const
 
 hipError_t launch_constraint_rows(const SysView& s, const double* q, double* yb, int precond, hipStream_t st)
 {
-    if (s.N_rigid > 0)
-        hipLaunchKernelGGL(k_sum_charges, dim3(blocks(s.N_rigid, 256)), dim3(256), 0, st, s, q, yb, precond);
-    if (s.N > 0) hipLaunchKernelGGL(k_set_charge, dim3(blocks(s.N, 256)), dim3(256), 0, st, s, q, yb, precond);
+    if (s.N_rigid > 0) {
+        const int W = s.Nmax < 64 ? s.Nmax : 64, K = 64 / W;
+        hipLaunchKernelGGL(k_sum_charges, dim3((unsigned)((s.N_rigid + 4 * K - 1) / (4 * K))), dim3(256), 0, st, s, W,
+                           q, yb, precond);
+    }
+    // beads outside rigid bodies (one isolated body each) carry their own charge row
+    if (s.N > 0 && s.N_isolated > s.N_rigid)
+        hipLaunchKernelGGL(k_set_charge, dim3(blocks(s.N, 256)), dim3(256), 0, st, s, q, yb, precond);
     return hipGetLastError();
 }
 
