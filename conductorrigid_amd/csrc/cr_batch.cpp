@@ -330,7 +330,7 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
 
     // per-box preparation (binning, records, ELL rows, objective, preconditioner setup)
     long xoff = 0;
-    int blk = 0, bblk = 0, rblk = 0, tiles = 0, bodies = 0;
+    int blk = 0, bblk = 0, rblk = 0, tiles = 0, bodies = 0, nmax_all = 1;
     size_t lds_s = 0, lds_c = 0;
     long Gmax = 0;
     for (int b = 0; b < nbox; ++b) {
@@ -342,8 +342,7 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
         if ((v.N > 0 && (!q_inout[b] || !force[b])) || (v.N_isolated > 0 && !phi_inout[b]))
             return set_error(CR_ERR_INVALID, "null output array");
         CR_HIP(launch_objective(v, c->mp.L, c->E0, c->b.as<double>(), st));
-        if (block_precond(c)) CR_RET(block_setup(c, v, st));
-        CR_RET(apply_precond(c, v, c->b.as<double>(), c->Mb.as<double>(), st));
+        if (block_precond(c)) CR_RET(block_setup(c, v, st, false));  // inverses: one launch below
         CR_RET(ensure_csr(c, v, st));
 
         BlockBox& bb = B->blocks[b];
@@ -357,7 +356,10 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
         bb.ginv = c->ginv.as<float>();
         bb.uvec = c->uvec.as<float>();
         bb.inv_s = c->inv_s.as<double>();
+        bb.pos = v.pos;
+        for (int d = 0; d < 3; ++d) bb.L[d] = c->mp.L[d];
         bodies += v.N_isolated;
+        nmax_all = std::max(nmax_all, c->block_nmax);
 
         BatchSeg& s = B->segs[b];
         s.xoff = xoff;
@@ -514,16 +516,24 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     CR_HIP(hipMemcpyAsync(B->d_rel.p, rel.data(), sizeof(double) * nbox, hipMemcpyHostToDevice, st));
     CR_HIP(hipMemcpyAsync(B->d_blocks.p, B->blocks.data(), sizeof(BlockBox) * nbox, hipMemcpyHostToDevice, st));
 
-    // objective, preconditioned objective and x0 of every box into the batch vectors
+    // the block preconditioner's per-body inverses of all boxes in one launch
+    if (block_precond(c0))
+        CR_HIP(launch_block_setup_batch(B->d_blocks.as<BlockBox>(), nbox, bodies, nmax_all, st));
+    // objective and x0 of every box into the batch vectors, then the preconditioned objective
     double* x = B->x.as<double>();
     for (int b = 0; b < nbox; ++b) {
         cr_context* c = ctx[b];
         const SysView& v = B->views[b];
         const long o = B->segs[b].xoff, n = B->segs[b].n;
         CR_HIP(hipMemcpyAsync(B->b.as<double>() + o, c->b.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-        CR_HIP(hipMemcpyAsync(B->Mb.as<double>() + o, c->Mb.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
         CR_RET(init_x0(c, v, x0_mode, q_inout[b], phi_inout[b], x + o, st));
     }
+    if (block_precond(c0))
+        CR_HIP(launch_block_apply_batch(B->d_blocks.as<BlockBox>(), nbox, bodies, B->b.as<double>(),
+                                        B->Mb.as<double>(), nullptr, st));
+    else
+        CR_HIP(launch_bprecond_diag(B->d_rows.as<RowBox>(), nbox, bblk, rblk, B->b.as<double>(), B->Mb.as<double>(),
+                                    st));
     CR_HIP(hipStreamSynchronize(st));  // host descriptors and rel tolerances are consumed
 
     // stopping scale per box: ||b|| (diagonal, CUSP's default_monitor) or ||D b|| (block)

@@ -343,7 +343,7 @@ bool block_precond(const cr_context* c) { return c->params.preconditioner == CR_
 
 // Build the per-body inverses of the near-field block preconditioner for the current
 // positions (once per force evaluation).
-int block_setup(cr_context* c, const SysView& v, hipStream_t st)
+int block_setup(cr_context* c, const SysView& v, hipStream_t st, bool invert)
 {
     const int B = v.N_isolated;
     const int nmax = std::max(1, v.Nmax);
@@ -361,7 +361,7 @@ int block_setup(cr_context* c, const SysView& v, hipStream_t st)
     CR_HIP(launch_block_setup(v, c->mp.L, nmax, c->body_count.as<unsigned>(), c->body_start.as<unsigned>(),
                               c->body_beads.as<unsigned>(),
                               c->scan_sums.as<unsigned long long>(), c->ginv.as<float>(), c->uvec.as<float>(),
-                              c->inv_s.as<double>(), st));
+                              c->inv_s.as<double>(), st, invert));
     return CR_OK;
 }
 

@@ -72,7 +72,8 @@ int recip_any(cr_context* c, hipStream_t st);
 const AxisPlan* axis_plans(const cr_context* c);
 int matvec_any(cr_context* c, const SysView& v, const double* x, double* y, int precond, hipStream_t st);
 bool block_precond(const cr_context* c);
-int block_setup(cr_context* c, const SysView& v, hipStream_t st);
+// Group beads by body and (invert) form the per-body inverses of the block preconditioner.
+int block_setup(cr_context* c, const SysView& v, hipStream_t st, bool invert = true);
 int apply_precond(cr_context* c, const SysView& v, const double* in, double* out, hipStream_t st);
 // x (length N + B) from the x0 mode; uses the context's w as scratch for CR_X0_CHARGES.
 int init_x0(cr_context* c, const SysView& v, int x0_mode, const double* q_in, const double* phi_in, double* x,

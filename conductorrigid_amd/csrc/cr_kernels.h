@@ -228,7 +228,8 @@ hipError_t launch_zero_force(int N, double* force, hipStream_t st);
 // Near-field block preconditioner: per isolated body, the inverse of its own saddle block.
 hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, unsigned* body_count,
                               unsigned* body_start, unsigned* body_beads,
-                              unsigned long long* scan_sums, float* ginv, float* uvec, double* inv_s, hipStream_t st);
+                              unsigned long long* scan_sums, float* ginv, float* uvec, double* inv_s, hipStream_t st,
+                              bool invert = true);
 // One box of a batched block-preconditioner application (device array, body0 ascending):
 // the box's per-body inverses and its offset in the batch vectors.
 struct BlockBox {
@@ -239,7 +240,12 @@ struct BlockBox {
     const float* ginv;
     const float* uvec;
     const double* inv_s;
+    const double4* pos;   // bead positions and box (setup only)
+    double L[3];
 };
+// Per-body inverses of every box's bodies in one launch (the boxes' beads already grouped
+// by body); nmax_all = the largest nmax of the batch, at most 64.
+hipError_t launch_block_setup_batch(const BlockBox* boxes, int nbox, int total_bodies, int nmax_all, hipStream_t st);
 hipError_t launch_block_apply_batch(const BlockBox* boxes, int nbox, int total_bodies, const double* y, double* w,
                                     const int* skip, hipStream_t st);
 hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_start, const unsigned* body_beads,

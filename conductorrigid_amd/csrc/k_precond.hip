@@ -139,14 +139,12 @@ __device__ __forceinline__ double lane_bcast(double v, int l)
 // identity, which the elimination leaves untouched.  Four bodies per 256-thread
 // workgroup.
 template <int NB>
-__global__ __launch_bounds__(256) void k_block_setup_wave(int B, int nmax, const unsigned* __restrict__ start,
-                                                          const unsigned* __restrict__ beads,
-                                                          const double4* __restrict__ pos, double Lx, double Ly,
-                                                          double Lz, float* __restrict__ ginv,
-                                                          float* __restrict__ uvec, double* __restrict__ inv_s)
+__device__ __forceinline__ void block_setup_body(int t, int nmax, const unsigned* __restrict__ start,
+                                                 const unsigned* __restrict__ beads,
+                                                 const double4* __restrict__ pos, double Lx, double Ly, double Lz,
+                                                 float* __restrict__ ginv, float* __restrict__ uvec,
+                                                 double* __restrict__ inv_s)
 {
-    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (t >= B) return;  // whole wavefront
     const int i = threadIdx.x & 63;
     const int b0 = start[t];
     const int n = start[t + 1] - b0;
@@ -202,6 +200,36 @@ __global__ __launch_bounds__(256) void k_block_setup_wave(int B, int nmax, const
 #pragma unroll
     for (int j = 0; j < NB; ++j) ssum += (j < n) ? lane_bcast(u, j) : 0.0;
     if (i == 0) inv_s[t] = 1.0 / ssum;
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_block_setup_wave(int B, int nmax, const unsigned* __restrict__ start,
+                                                          const unsigned* __restrict__ beads,
+                                                          const double4* __restrict__ pos, double Lx, double Ly,
+                                                          double Lz, float* __restrict__ ginv,
+                                                          float* __restrict__ uvec, double* __restrict__ inv_s)
+{
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= B) return;  // whole wavefront
+    block_setup_body<NB>(t, nmax, start, beads, pos, Lx, Ly, Lz, ginv, uvec, inv_s);
+}
+
+// All bodies of a batch of boxes in one launch (body0 ascending over boxes).
+template <int NB>
+__global__ __launch_bounds__(256) void k_block_setup_wave_batch(const BlockBox* __restrict__ boxes, int nbox,
+                                                                int total)
+{
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= total) return;  // whole wavefront
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (boxes[mid].body0 <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const BlockBox& d = boxes[lo];
+    block_setup_body<NB>(g - d.body0, d.nmax, d.start, d.beads, d.pos, d.L[0], d.L[1], d.L[2],
+                         const_cast<float*>(d.ginv), const_cast<float*>(d.uvec), const_cast<double*>(d.inv_s));
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -272,7 +300,20 @@ __global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int
     double gy = 0.0;
     if (act) {
         const float* row = ginv + ((long)t * nmax + i) * nmax;
-        for (int j = 0; j < n; ++j) gy += (double)row[j] * ys[base + j];
+        if ((nmax & 3) == 0) {
+            // rows are 16-byte aligned: four entries per load; entries past n are never
+            // written by the setup and are masked
+            const float4* r4 = reinterpret_cast<const float4*>(row);
+            for (int j = 0; j < n; j += 4) {
+                const float4 g = r4[j >> 2];
+                gy += (double)g.x * ys[base + j];
+                if (j + 1 < n) gy += (double)g.y * ys[base + j + 1];
+                if (j + 2 < n) gy += (double)g.z * ys[base + j + 2];
+                if (j + 3 < n) gy += (double)g.w * ys[base + j + 3];
+            }
+        } else {
+            for (int j = 0; j < n; ++j) gy += (double)row[j] * ys[base + j];
+        }
     }
     if (live && i == 0) {
         double dot = 0.0;
@@ -316,7 +357,8 @@ __global__ void k_zero_u32(unsigned* p, long n)
 
 hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, unsigned* body_count,
                               unsigned* body_start, unsigned* body_beads,
-                              unsigned long long* scan_sums, float* ginv, float* uvec, double* inv_s, hipStream_t st)
+                              unsigned long long* scan_sums, float* ginv, float* uvec, double* inv_s, hipStream_t st,
+                              bool invert)
 {
     const int B = s.N_isolated;
     if (B <= 0) return hipSuccess;
@@ -329,6 +371,7 @@ hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, uns
                            body_start, body_beads);
     if (s.N > 0)
         hipLaunchKernelGGL(k_body_fill_single, dim3(blocks(s.N, 256)), dim3(256), 0, st, s, body_start, body_beads);
+    if (!invert) return hipGetLastError();
     const unsigned wb = (unsigned)((B + 3) / 4);
 #define CR_BLOCK_WAVE(NB)                                                                                   \
     hipLaunchKernelGGL(k_block_setup_wave<NB>, dim3(wb), dim3(256), 0, st, B, nmax, body_start, body_beads, \
@@ -363,6 +406,30 @@ hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_s
     const unsigned nb = (unsigned)((s.N_isolated + 4 * K - 1) / (4 * K));
     hipLaunchKernelGGL(k_block_apply, dim3(nb), dim3(256), 0, st, s.N_isolated, s.N, nmax, K, body_start, body_beads,
                        ginv, uvec, inv_s, y, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_setup_batch(const BlockBox* boxes, int nbox, int total_bodies, int nmax_all, hipStream_t st)
+{
+    if (total_bodies <= 0) return hipSuccess;
+    const unsigned wb = (unsigned)((total_bodies + 3) / 4);
+#define CR_BLOCK_WAVE_B(NB) \
+    hipLaunchKernelGGL(k_block_setup_wave_batch<NB>, dim3(wb), dim3(256), 0, st, boxes, nbox, total_bodies)
+    if (nmax_all <= 8)
+        CR_BLOCK_WAVE_B(8);
+    else if (nmax_all <= 16)
+        CR_BLOCK_WAVE_B(16);
+    else if (nmax_all <= 24)
+        CR_BLOCK_WAVE_B(24);
+    else if (nmax_all <= 32)
+        CR_BLOCK_WAVE_B(32);
+    else if (nmax_all <= 40)
+        CR_BLOCK_WAVE_B(40);
+    else if (nmax_all <= 64)
+        CR_BLOCK_WAVE_B(64);
+    else
+        return hipErrorInvalidValue;
+#undef CR_BLOCK_WAVE_B
     return hipGetLastError();
 }
 
