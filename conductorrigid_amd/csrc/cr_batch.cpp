@@ -383,7 +383,7 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
         tb.tile_start = bv.tile_start;
         tb.sorted = bv.sorted;
         tb.rec_off = bv.rec_off;
-        tb.rec_w = bv.rec_w;
+        tb.rec_win = bv.rec_win;
         tb.phi = c->phi_recip.p;
         tiles += c->tg.ntiles;
         Gmax = std::max(Gmax, c->G);

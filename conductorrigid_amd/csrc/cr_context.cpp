@@ -151,14 +151,14 @@ int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out)
                                 c->scan_sums.as<unsigned long long>(), st));
         const size_t nb = (size_t)std::max(1, s->N);
         CR_HIP(c->rec_off.ensure(sizeof(int) * nb));
-        CR_HIP(c->rec_w.ensure(Tsz * 3 * (size_t)c->mp.P * nb));
+        CR_HIP(c->rec_win.ensure(window_record_bytes(c->mp, c->fp32()) * nb));
         if (c->fp32())
             CR_HIP(launch_bead_records<float>(c->mp, c->tg, *out, c->sorted.as<unsigned>(), c->bead_tile.as<unsigned>(),
-                                              c->rec_off.as<int>(), c->rec_w.as<float>(), st));
+                                              c->rec_off.as<int>(), c->rec_win.as<float>(), st));
         else
             CR_HIP(launch_bead_records<double>(c->mp, c->tg, *out, c->sorted.as<unsigned>(),
                                                c->bead_tile.as<unsigned>(), c->rec_off.as<int>(),
-                                               c->rec_w.as<double>(), st));
+                                               c->rec_win.as<double>(), st));
     }
     return CR_OK;
 }
@@ -203,7 +203,7 @@ BinView bin_view(const cr_context* c)
     b.tile_start = c->tile_start.as<unsigned>();
     b.sorted = c->sorted.as<unsigned>();
     b.rec_off = c->rec_off.as<int>();
-    b.rec_w = c->rec_w.p;
+    b.rec_win = c->rec_win.p;
     return b;
 }
 

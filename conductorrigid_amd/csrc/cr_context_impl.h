@@ -31,7 +31,7 @@ struct cr_context {
     // bead binning for the owner-computes spread
     TileGeom tg{};
     DevBuf bead_tile, tile_count, tile_start, tile_cursor, sorted, scan_sums;
-    DevBuf rec_off, rec_w;  // per-bead window records in binned order
+    DevBuf rec_off, rec_win;  // per-bead window records in binned order
     // Krylov workspace
     int R = 10;
     long ld = 0;

@@ -73,17 +73,23 @@ hipError_t launch_bin_beads(const MeshParams& mp, const TileGeom& tg, const SysV
                             unsigned* tile_count, unsigned* tile_start, unsigned* cursor, unsigned* sorted,
                             unsigned long long* scan_sums, hipStream_t st);
 // Binned bead order and per-bead window records (k_bead_records): rec_off packs the
-// window's first node relative to its tile origin (x | y << 10 | z << 20), rec_w holds
-// the 3P separable axis weights (x, then y, then z) in the mesh precision.
+// window's first node relative to its tile origin (x | y << 10 | z << 20).  rec_win
+// holds, in FP32, the bead's position in its window per axis in node units (x, y, z, 0),
+// from which the kernels evaluate the axis weights exp(-c_d (p - u_d)^2); in FP64 the 3P
+// weights themselves (x, then y, then z).
 struct BinView {
     const unsigned* tile_start;   // [ntiles + 1]
     const unsigned* sorted;       // [N] bead index per binned slot
     const int* rec_off;           // [N]
-    const void* rec_w;            // [N * 3P]
+    const void* rec_win;          // window_record_bytes per bead
 };
+inline size_t window_record_bytes(const MeshParams& mp, bool fp32)
+{
+    return fp32 ? 4 * sizeof(float) : 3 * (size_t)mp.P * sizeof(double);
+}
 template <typename T>
 hipError_t launch_bead_records(const MeshParams& mp, const TileGeom& tg, const SysView& s, const unsigned* sorted,
-                               const unsigned* bead_tile, int* rec_off, T* rec_w, hipStream_t st);
+                               const unsigned* bead_tile, int* rec_off, T* rec_win, hipStream_t st);
 // grid = spread of q; clear = false leaves the mesh clear to the caller (it must be zero)
 template <typename T>
 hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const BinView& bv, const double* q, T* grid,
@@ -124,7 +130,7 @@ struct TiledBox {
     const unsigned* tile_start;  // [ntiles + 1]
     const unsigned* sorted;      // [N]
     const int* rec_off;          // [N]
-    const void* rec_w;           // [N * 3P] (T)
+    const void* rec_win;         // window_record_bytes per bead (T)
     const double* q;             // spread input / charges of the force contraction, bead order
     void* grid;                  // real mesh (T)
     void* phi;                   // contraction output (T), bead order

@@ -171,14 +171,17 @@ __device__ __forceinline__ void stage_chunk(const MeshParams& mp, const TileCoor
     __syncthreads();
 }
 
-// Window records of every bead in binned order, built once per bead configuration so
-// that each operator application streams them instead of re-deriving windows.
+// Window records of every bead in binned order, built once per bead configuration: the
+// window's packed start in its tile and, in FP32, the bead's position in the window per
+// axis (node units; 4 values per bead for aligned loads), from which the kernels evaluate
+// the axis weights; in FP64 the 3P axis weights themselves (double exponentials are too
+// slow to re-evaluate in every application).
 template <typename T>
 __global__ __launch_bounds__(256) void k_bead_records(MeshParams mp, TileGeom tg, int N,
                                                       const unsigned* __restrict__ sorted,
                                                       const unsigned* __restrict__ bead_tile,
                                                       const double4* __restrict__ pos, int* __restrict__ rec_off,
-                                                      T* __restrict__ rec_w)
+                                                      T* __restrict__ rec_win)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= N) return;
@@ -187,26 +190,50 @@ __global__ __launch_bounds__(256) void k_bead_records(MeshParams mp, TileGeom tg
     const int tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
     double u[3];
     rec_off[k] = bead_window(mp, pos[bead], tx * tg.T[0], ty * tg.T[1], tz * tg.T[2], u);
-    const int P = mp.P;
-    T* w = rec_w + (long)k * 3 * P;
-    for (int d = 0; d < 3; ++d) {
-        const T cd = axis_coef<T>(mp, d), ud = (T)u[d];
-        for (int p = 0; p < P; ++p) {
-            const T dd = (T)p - ud;
-            w[d * P + p] = gexp<T>(-cd * dd * dd);
+    if constexpr (sizeof(T) == sizeof(double)) {
+        const int P = mp.P;
+        T* w = rec_win + (long)k * 3 * P;
+        for (int d = 0; d < 3; ++d) {
+            const T cd = axis_coef<T>(mp, d), ud = (T)u[d];
+            for (int p = 0; p < P; ++p) {
+                const T dd = (T)p - ud;
+                w[d * P + p] = gexp<T>(-cd * dd * dd);
+            }
         }
+    } else {
+        T* uk = rec_win + 4 * (long)k;
+        uk[0] = (T)u[0];
+        uk[1] = (T)u[1];
+        uk[2] = (T)u[2];
+        uk[3] = (T)0;
     }
 }
 
-// Fill the chunk's LDS tables for beads [k0, k0+cnt) from their window records.
+// Fill the chunk's LDS tables for beads [k0, k0+cnt) from their window records.  FP32:
+// three threads per bead evaluate its 3P axis weights exp(-c_d (p - u_d)^2), one axis
+// each; FP64: the stored weights are copied.
 template <typename T>
-__device__ __forceinline__ void stage_records(int P, const unsigned* __restrict__ sorted, unsigned k0, int cnt,
-                                              const int* __restrict__ rec_off, const T* __restrict__ rec_w,
-                                              const double* __restrict__ q, T qscale, Stage<T>& s)
+__device__ __forceinline__ void stage_records(const MeshParams& mp, int P, const unsigned* __restrict__ sorted,
+                                              unsigned k0, int cnt, const int* __restrict__ rec_off,
+                                              const T* __restrict__ rec_win, const double* __restrict__ q, T qscale,
+                                              Stage<T>& s)
 {
     const int per = 3 * P;
-    const T* src = rec_w + (long)k0 * per;
-    for (int j = threadIdx.x; j < cnt * per; j += blockDim.x) s.w[j] = src[j];
+    if constexpr (sizeof(T) == sizeof(double)) {
+        const T* src = rec_win + (long)k0 * per;
+        for (int j = threadIdx.x; j < cnt * per; j += blockDim.x) s.w[j] = src[j];
+    } else {
+        for (int j = threadIdx.x; j < 3 * cnt; j += blockDim.x) {
+            const int i = j / 3, d = j - 3 * i;
+            const T ud = rec_win[4 * (long)(k0 + i) + d];
+            const T cd = axis_coef<T>(mp, d);
+            T* w = s.w + i * per + d * P;
+            for (int p = 0; p < P; ++p) {
+                const T dd = (T)p - ud;
+                w[p] = gexp<T>(-cd * dd * dd);
+            }
+        }
+    }
     for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
         const unsigned bead = sorted[k0 + i];
         s.off[i] = rec_off[k0 + i];
@@ -251,7 +278,7 @@ template <typename T, typename A, int PT>
 __device__ __forceinline__ void spread_tile(const MeshParams& mp, const TileGeom& tg, int t, int ch,
                                             const unsigned* __restrict__ tstart,
                                             const unsigned* __restrict__ sorted, const int* __restrict__ rec_off,
-                                            const T* __restrict__ rec_w, const double* __restrict__ q,
+                                            const T* __restrict__ rec_win, const double* __restrict__ q,
                                             T* __restrict__ grid, char* smem_raw)
 {
     const TileCoords tc = tile_coords(mp, tg, t);
@@ -268,7 +295,7 @@ __device__ __forceinline__ void spread_tile(const MeshParams& mp, const TileGeom
     const unsigned kb = tstart[tc.t], ke = tstart[tc.t + 1];
     for (unsigned k0 = kb; k0 < ke; k0 += ch) {
         const int cnt = (int)min((unsigned)ch, ke - k0);
-        stage_records<T>(P, sorted, k0, cnt, rec_off, rec_w, q, (T)mp.prefac, s);
+        stage_records<T>(mp, P, sorted, k0, cnt, rec_off, rec_win, q, (T)mp.prefac, s);
         for (int i = wave; i < cnt; i += nwave) {
             const int o = s.off[i];
             const int ox = o & 1023, oy = (o >> 10) & 1023, oz = o >> 20;
@@ -304,11 +331,11 @@ template <typename T, typename A, int PT>
 __global__ __launch_bounds__(kSpreadThreads) void k_spread_tiled(MeshParams mp, TileGeom tg, int ch,
                                                       const unsigned* __restrict__ tstart,
                                                       const unsigned* __restrict__ sorted,
-                                                      const int* __restrict__ rec_off, const T* __restrict__ rec_w,
+                                                      const int* __restrict__ rec_off, const T* __restrict__ rec_win,
                                                       const double* __restrict__ q, T* __restrict__ grid)
 {
     extern __shared__ char smem_raw[];
-    spread_tile<T, A, PT>(mp, tg, xcd_tile(blockIdx.x, tg.ntiles), ch, tstart, sorted, rec_off, rec_w, q, grid,
+    spread_tile<T, A, PT>(mp, tg, xcd_tile(blockIdx.x, tg.ntiles), ch, tstart, sorted, rec_off, rec_win, q, grid,
                           smem_raw);
 }
 
@@ -316,7 +343,7 @@ template <typename T, bool FORCE, int PT>
 __device__ __forceinline__ void contract_tile(const MeshParams& mp, const TileGeom& tg, int t, int ch,
                                               const unsigned* __restrict__ tstart,
                                               const unsigned* __restrict__ sorted, const int* __restrict__ rec_off,
-                                              const T* __restrict__ rec_w, const double4* __restrict__ pos,
+                                              const T* __restrict__ rec_win, const double4* __restrict__ pos,
                                               const double* __restrict__ q, const T* __restrict__ grid,
                                               T* __restrict__ phi, double* __restrict__ force, char* smem_raw)
 {
@@ -345,7 +372,7 @@ __device__ __forceinline__ void contract_tile(const MeshParams& mp, const TileGe
         if (FORCE)
             stage_chunk<T>(mp, tc, sorted, k0, cnt, pos, q, (T)1, s);
         else
-            stage_records<T>(P, sorted, k0, cnt, rec_off, rec_w, nullptr, (T)1, s);
+            stage_records<T>(mp, P, sorted, k0, cnt, rec_off, rec_win, nullptr, (T)1, s);
         for (int i = wave; i < cnt; i += nwave) {
             const int o = s.off[i];
             const int ox = o & 1023, oy = (o >> 10) & 1023, oz = o >> 20;
@@ -404,13 +431,13 @@ template <typename T, bool FORCE, int PT>
 __global__ __launch_bounds__(contract_threads<T>()) void k_contract_tiled(MeshParams mp, TileGeom tg, int ch,
                                                         const unsigned* __restrict__ tstart,
                                                         const unsigned* __restrict__ sorted,
-                                                        const int* __restrict__ rec_off, const T* __restrict__ rec_w,
+                                                        const int* __restrict__ rec_off, const T* __restrict__ rec_win,
                                                         const double4* __restrict__ pos, const double* __restrict__ q,
                                                         const T* __restrict__ grid, T* __restrict__ phi,
                                                         double* __restrict__ force)
 {
     extern __shared__ char smem_raw[];
-    contract_tile<T, FORCE, PT>(mp, tg, xcd_tile(blockIdx.x, tg.ntiles), ch, tstart, sorted, rec_off, rec_w, pos, q,
+    contract_tile<T, FORCE, PT>(mp, tg, xcd_tile(blockIdx.x, tg.ntiles), ch, tstart, sorted, rec_off, rec_win, pos, q,
                                 grid, phi, force, smem_raw);
 }
 
@@ -447,7 +474,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread_tiled_batch(const Til
     const TiledBox& d = boxes[batch_box(boxes, nbox, g)];
     if (skip && skip[d.box]) return;
     spread_tile<T, A, PT>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off,
-                          static_cast<const T*>(d.rec_w), d.q, static_cast<T*>(d.grid), smem_raw);
+                          static_cast<const T*>(d.rec_win), d.q, static_cast<T*>(d.grid), smem_raw);
 }
 
 template <typename T, bool FORCE, int PT>
@@ -459,7 +486,7 @@ __global__ __launch_bounds__(contract_threads<T>()) void k_contract_tiled_batch(
     const TiledBox& d = boxes[batch_box(boxes, nbox, g)];
     if (skip && skip[d.box]) return;
     contract_tile<T, FORCE, PT>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off,
-                                static_cast<const T*>(d.rec_w), d.pos, FORCE ? d.q : nullptr,
+                                static_cast<const T*>(d.rec_win), d.pos, FORCE ? d.q : nullptr,
                                 static_cast<const T*>(d.grid), static_cast<T*>(d.phi), FORCE ? d.force : nullptr,
                                 smem_raw);
 }
@@ -506,11 +533,11 @@ hipError_t launch_bin_beads(const MeshParams& mp, const TileGeom& tg, const SysV
 
 template <typename T>
 hipError_t launch_bead_records(const MeshParams& mp, const TileGeom& tg, const SysView& s, const unsigned* sorted,
-                               const unsigned* bead_tile, int* rec_off, T* rec_w, hipStream_t st)
+                               const unsigned* bead_tile, int* rec_off, T* rec_win, hipStream_t st)
 {
     if (s.N <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_bead_records<T>, dim3(blocks(s.N, 256)), dim3(256), 0, st, mp, tg, s.N, sorted, bead_tile,
-                       s.pos, rec_off, rec_w);
+                       s.pos, rec_off, rec_win);
     return hipGetLastError();
 }
 
@@ -524,14 +551,14 @@ hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const B
         hipError_t e = launch_fill<T>(grid, (T)0, (long)mp.Nx * mp.Ny * mp.Nz, st);
         if (e != hipSuccess) return e;
     }
-    const T* rec_w = static_cast<const T*>(bv.rec_w);
+    const T* rec_win = static_cast<const T*>(bv.rec_win);
 #define CR_SPREAD_P(PV)                                                                                        \
     do {                                                                                                       \
         if (l.total_bytes > 65536)                                                                             \
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_tiled<T, SpreadAcc, PV>),       \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total_bytes);         \
         hipLaunchKernelGGL((k_spread_tiled<T, SpreadAcc, PV>), dim3(tg.ntiles), dim3(kSpreadThreads),          \
-                           l.total_bytes, st, mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, q, \
+                           l.total_bytes, st, mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_win, q, \
                            grid);                                                                              \
     } while (0)
     switch (mp.P) {
@@ -554,7 +581,7 @@ hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const
 {
     const TiledLds l = tiled_lds(mp, tg, sizeof(T), sizeof(T));
     if (l.chunk < 4 || mp.P > 16) return hipErrorInvalidValue;
-    const T* rec_w = static_cast<const T*>(bv.rec_w);
+    const T* rec_win = static_cast<const T*>(bv.rec_win);
 #define CR_CONTRACT_P(PV)                                                                                       \
     do {                                                                                                        \
         if (l.total_bytes > 65536) {                                                                            \
@@ -565,11 +592,11 @@ hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const
         }                                                                                                       \
         if (force)                                                                                              \
             hipLaunchKernelGGL((k_contract_tiled<T, true, PV>), dim3(tg.ntiles), dim3(contract_threads<T>()), l.total_bytes, st,  \
-                               mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, s.pos, q, grid,    \
+                               mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_win, s.pos, q, grid,    \
                                (T*)nullptr, force);                                                             \
         else                                                                                                    \
             hipLaunchKernelGGL((k_contract_tiled<T, false, PV>), dim3(tg.ntiles), dim3(contract_threads<T>()), l.total_bytes, st, \
-                               mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_w, s.pos,             \
+                               mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_win, s.pos,             \
                                (const double*)nullptr, grid, phi, (double*)nullptr);                            \
     } while (0)
     switch (mp.P) {
