@@ -730,6 +730,26 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
         c->sc.e[d] = c->etab[d].as<double>();
     }
     c->sc.invG = 1.0 / (double)c->G;
+    for (int d = 0; d < 3; ++d) c->sc.n[d] = Nd[d];
+    c->sc.radial = nullptr;
+    if (p->L[0] == p->L[1] && p->L[1] == p->L[2]) {
+        // S = j0(|k|)^2 / k^2 * e_x e_y e_z / G with k^2 = kappa^2 m2 (ConductorRigid.cc:258-268)
+        long m2max = 0;
+        for (int d = 0; d < 3; ++d) m2max += (long)(Nd[d] / 2) * (Nd[d] / 2);
+        const double kappa = 2.0 * kPi / p->L[0];
+        std::vector<double> rad((size_t)m2max + 1);
+        rad[0] = 0.0;  // S(0) = 0
+        for (long m2 = 1; m2 <= m2max; ++m2) {
+            const double k2 = kappa * kappa * (double)m2, k = std::sqrt(k2);
+            const double j0 = std::sin(k) / k;
+            rad[(size_t)m2] = j0 * j0 / k2 * c->sc.invG;
+        }
+        if (c->fp32())
+            CR_HIP(upload<float>(c->radtab, rad));
+        else
+            CR_HIP(upload<double>(c->radtab, rad));
+        c->sc.radial = c->radtab.p;
+    }
     // binned spread/contract tiling: per axis the largest divisor of the mesh size in
     // [max(P-1, 4), 16], else the smallest one in (16, 32]; the tile extended by P-1
     // nodes per axis, held in the spread's double accumulator, must fit 3/4 of the

@@ -19,6 +19,7 @@ struct cr_context {
     AxisPlan plan_d[3]{}, plan_f[3]{};
     DevBuf tw_d[3], tw_f[3];
     DevBuf ktab[3], etab[3];
+    DevBuf radtab;  // j0(|k|)^2 / k^2 / G over m2 = |m|^2 (cubic boxes)
     ScaleTables sc{};
     DevBuf tab_d, gtab_d, tab_f, gtab_f;
     DevBuf grid_real, grid_cplx;

@@ -174,6 +174,12 @@ struct ScaleTables {
     const double* k[3];    // wavenumbers per axis
     const double* e[3];    // exp(-(1-eta_d) k_d^2 / (4 xi^2)) per axis
     double invG;
+    // Cubic boxes: k^2 = (2 pi / L)^2 m2 with m2 = mx^2 + my^2 + mz^2 an integer, so the
+    // non-separable factor j0(|k|)^2 / k^2 / G is a table over m2 (mesh precision), read
+    // by the register x pass instead of a square root, sine and two divisions per mode.
+    // Null for other boxes.
+    const void* radial;
+    int n[3];              // mesh size per axis (signed frequency m of index i)
 };
 template <typename T>
 hipError_t launch_fft_x_scale(const AxisPlan& px, int Nx, int Ny, int Zc, T* cplx, const ScaleTables& sc,

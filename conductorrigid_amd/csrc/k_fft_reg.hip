@@ -400,22 +400,41 @@ __device__ __forceinline__ void x_scale_pass(int Ny, int Zc, typename Cx<T>::t* 
     // thread (c, l) holds X[(l + B s) + A k2] in v[s B + k2]: scale by S(kx, ky, kz)
     const double ky = sc.k[1][L.line], ey = sc.e[1][L.line];
     const double kz = ok ? sc.k[2][z] : 0.0, ez = ok ? sc.e[2][z] : 0.0;
+    if (sc.radial) {
+        const T* rad = static_cast<const T*>(sc.radial);
+        const int my = L.line < (sc.n[1] + 1) / 2 ? L.line : L.line - sc.n[1];
+        const int zi = ok ? z : 0;  // padding lanes index the table in range too
+        const int mz = zi < (sc.n[2] + 1) / 2 ? zi : zi - sc.n[2];
+        const int myz = my * my + mz * mz;
+        const double eyz = ey * ez;
 #pragma unroll
-    for (int s = 0; s < m; ++s)
+        for (int s = 0; s < m; ++s)
 #pragma unroll
-        for (int j = 0; j < B; ++j) {
-            const int x = l + B * s + A * j;
-            const T kx = (T)sc.k[0][x], kyt = (T)ky, kzt = (T)kz;
-            const T k2 = kx * kx + kyt * kyt + kzt * kzt;
-            T S = 0;
-            if (k2 > (T)0) {
-                const T kk = sqrt(k2);
-                const T j0 = sin_t<T>(kk) / kk;
-                S = j0 * j0 * (T)(sc.e[0][x] * ey * ez) / k2 * (T)sc.invG;
+            for (int j = 0; j < B; ++j) {
+                const int x = l + B * s + A * j;
+                const int mx = x < (sc.n[0] + 1) / 2 ? x : x - sc.n[0];
+                const T S = ok ? rad[myz + mx * mx] * (T)(sc.e[0][x] * eyz) : (T)0;
+                v[s * B + j].x *= S;
+                v[s * B + j].y *= S;
             }
-            v[s * B + j].x *= S;
-            v[s * B + j].y *= S;
-        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < m; ++s)
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                const int x = l + B * s + A * j;
+                const T kx = (T)sc.k[0][x], kyt = (T)ky, kzt = (T)kz;
+                const T k2 = kx * kx + kyt * kyt + kzt * kzt;
+                T S = 0;
+                if (k2 > (T)0) {
+                    const T kk = sqrt(k2);
+                    const T j0 = sin_t<T>(kk) / kk;
+                    S = j0 * j0 * (T)(sc.e[0][x] * ey * ez) / k2 * (T)sc.invG;
+                }
+                v[s * B + j].x *= S;
+                v[s * B + j].y *= S;
+            }
+    }
     __syncthreads();  // the transpose buffer is reused
     four_step_cols_inv_from_X<A, B, C>(v, lds, tw, c, l);
     if (!ok) return;

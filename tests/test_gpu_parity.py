@@ -217,3 +217,24 @@ def test_register_fft_lengths(mesh, require_gpu):
     _report(f"register recip {mesh} fp64", v3)
     _report(f"register recip {mesh} fp32", v4)
     assert v1 < 1e-13 and v2 < 1e-13 and v3 < 1e-12 and v4 < 1e-5
+
+
+@pytest.mark.parametrize("stretch", [(1.0, 1.0, 1.0), (1.0, 1.05, 1.02)])
+def test_register_recip_box_shapes(stretch, require_gpu):
+    """k-space step of the register x pass on a cubic box (S(k) from the radial m^2 table)
+    and on a stretched box (S(k) evaluated per mode), both against the oracle."""
+    import copy
+
+    s = copy.deepcopy(n1k_system())
+    s.L = np.asarray(s.L, dtype=np.float64) * np.asarray(stretch)
+    mesh = (64, 64, 64)
+    prm = dict(xi=0.929, errortol=1e-6, rc=4.0, mesh=mesh, P=8)
+    rng = np.random.default_rng(11)
+    grid = rng.normal(size=mesh)
+    o = O.Oracle(s, prm["xi"], prm["errortol"], rc=prm["rc"], mesh=mesh, P=prm["P"])
+    og = o.recip_grid(grid)
+    v64 = rel(GpuCase(s, **prm).recip(grid), og)
+    v32 = rel(GpuCase(s, precision="fp32", **prm).recip(grid), og)
+    _report(f"recip box {stretch} fp64", v64)
+    _report(f"recip box {stretch} fp32", v32)
+    assert v64 < 1e-12 and v32 < 1e-5
