@@ -46,6 +46,8 @@ struct cr_context {
     DevBuf bd_force;  // Brownian step: bead forces plus steric repulsion
     GmresState* g_dev = nullptr;
     GmresState* g_host = nullptr;
+    HostPub* hpub = nullptr;            // coherent pinned: residuals the host polls for
+    unsigned long long pub_seq = 0;
     double E0[3] = {0, 0, 0};
     unsigned long generation = 0;  // bumped whenever parameters, tables or tiling change
 
@@ -53,6 +55,7 @@ struct cr_context {
     {
         if (g_dev) (void)hipFree(g_dev);
         if (g_host) (void)hipHostFree(g_host);
+        if (hpub) (void)hipHostFree(hpub);
     }
 
     bool fp32() const { return params.precision == CR_PRECISION_FP32_BULK; }

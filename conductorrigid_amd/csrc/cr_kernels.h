@@ -374,6 +374,14 @@ struct GmresState {
     double resid;
 };
 hipError_t launch_givens(GmresState* g, int i, int R, hipStream_t st);
+// One device scalar published to host memory that the host polls: value, then (after a
+// system-scope fence) the sequence number.  dst must be fine-grained (coherent) pinned
+// host memory.
+struct HostPub {
+    double value;
+    unsigned long long seq;
+};
+hipError_t launch_publish(const double* src, HostPub* dst, unsigned long long seq, hipStream_t st);
 hipError_t launch_cycle_init(GmresState* g, const double* norm2, hipStream_t st);
 hipError_t launch_backsub(GmresState* g, int R, hipStream_t st);
 

@@ -149,6 +149,15 @@ __global__ __launch_bounds__(kBlock) void k_gs_update_dot(const double* __restri
     }
 }
 
+__global__ void k_publish(const double* __restrict__ src, HostPub* dst, unsigned long long seq)
+{
+    if (threadIdx.x != 0) return;
+    const double v = *src;
+    *reinterpret_cast<volatile double*>(&dst->value) = v;
+    __threadfence_system();
+    *reinterpret_cast<volatile unsigned long long*>(&dst->seq) = seq;
+}
+
 __global__ void k_axpby(double a, const double* __restrict__ x, double b, double* __restrict__ y, long n)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -268,6 +277,12 @@ hipError_t launch_reduce_partials(const double* partial, int nblocks, int ncols,
 {
     if (ncols <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_reduce_partials, dim3(ncols), dim3(kBlock), 0, st, partial, nblocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_publish(const double* src, HostPub* dst, unsigned long long seq, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, src, dst, seq);
     return hipGetLastError();
 }
 
