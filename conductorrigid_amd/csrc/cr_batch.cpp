@@ -558,13 +558,20 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     std::vector<char> host_skip(nb, 0);
     const int maxiter = c0->res.max_iter;
     int iter = 0;
+    bool x_zero = x0_mode == CR_X0_ZERO;
     for (;;) {
         // cycle start: w = D (A x) - D b for every unfinished box
-        CR_HIP(hipMemcpyAsync(u, x, sizeof(double) * B->n_tot, hipMemcpyDeviceToDevice, st));
-        for (int b = 0; b < nb; ++b) host_skip[b] = (char)h_fin[b];
-        CR_RET(batch_matvec(B, u, w, 1, finished, &host_skip, st));
-        for (int b = 0; b < nb; ++b)
-            if (!h_fin[b]) ++matvecs[b];
+        if (x_zero) {
+            // x0 = 0 in every box: A x0 = 0 exactly, no operator application needed
+            CR_HIP(launch_fill<double>(w, 0.0, B->n_tot, st));
+            x_zero = false;
+        } else {
+            CR_HIP(hipMemcpyAsync(u, x, sizeof(double) * B->n_tot, hipMemcpyDeviceToDevice, st));
+            for (int b = 0; b < nb; ++b) host_skip[b] = (char)h_fin[b];
+            CR_RET(batch_matvec(B, u, w, 1, finished, &host_skip, st));
+            for (int b = 0; b < nb; ++b)
+                if (!h_fin[b]) ++matvecs[b];
+        }
         CR_HIP(launch_axpby(-1.0, B->Mb.as<double>(), 1.0, w, B->n_tot, st));
         CR_HIP(launch_bdot(w, B->ld, 1, step, w, sg, nb, blk, part, finished, st));
         CR_HIP(launch_breduce(part, sg, nb, 1, step, gs, kRedNorm, finished, st));

@@ -426,7 +426,7 @@ int segments(cr_context* c, const SysView& v, hipStream_t st, unsigned** counts,
 // Restarted, left-preconditioned GMRES(R) following CUSP's krylov::gmres structure
 // (cppmodule/ConductorRigid.cu:1134-1136): the cycle starts from w = D(Ax - b), stops
 // when the Givens estimate |s_{i+1}| <= tol ||b||_2, x += V y at the end of each cycle.
-int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st)
+int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st, bool x_zero)
 {
     const long n = (long)v.N + v.N_isolated;
     const int R = c->R, maxiter = c->res.max_iter;
@@ -449,8 +449,14 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
     int iter = 0, matvecs = 0, restarts = 0, converged = 0;
     double resid = 0;
     for (;;) {
-        CR_RET(matvec_prec(c, v, x, w, st));
-        ++matvecs;
+        if (x_zero) {
+            // x0 = 0: A x0 = 0 exactly, so the first cycle's residual needs no operator
+            CR_HIP(launch_fill<double>(w, 0.0, n, st));
+            x_zero = false;
+        } else {
+            CR_RET(matvec_prec(c, v, x, w, st));
+            ++matvecs;
+        }
         CR_HIP(launch_axpby(-1.0, Mb, 1.0, w, n, st));
         CR_RET(norm2(c, w, n, scal, st));
         CR_HIP(launch_cycle_init(g, scal, st));
@@ -947,7 +953,7 @@ int cr_compute_force(cr_context* c, const cr_system* s, double* q_inout, double*
         return set_error(CR_ERR_INVALID, "unknown x0 mode");
     CR_RET(ensure_csr(c, v, st));
     CR_RET(init_x0(c, v, x0_mode, q_inout, phi_inout, x, st));
-    CR_RET(gmres(c, v, stats, st));
+    CR_RET(gmres(c, v, stats, st, x0_mode == CR_X0_ZERO));
     CR_HIP(hipMemcpyAsync(q_inout, x, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
     CR_HIP(hipMemcpyAsync(phi_inout, x + v.N, sizeof(double) * v.N_isolated, hipMemcpyDeviceToDevice, st));
     if (stats) CR_RET(true_residual(c, v, x, stats, st));
