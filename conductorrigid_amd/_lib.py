@@ -98,6 +98,7 @@ _SIGS = {
                              C.c_double, C.c_double, vp]),
     "cr_autotune": (C.c_int, [vp, C.POINTER(CrSystem), C.POINTER(CrTuning), vp]),
     "cr_set_tiles": (C.c_int, [vp, C.POINTER(C.c_int), C.c_int]),
+    "cr_set_residual_check": (C.c_int, [vp, C.c_int]),
     "cr_mlp_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_ulonglong]),
     "cr_mlp_destroy": (None, [vp]),
     "cr_mlp_set_params": (C.c_int, [vp, vp]),

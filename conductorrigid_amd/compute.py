@@ -225,6 +225,12 @@ class ConductorRigid:
         return {"tile": list(t.tile), "lds_bytes": t.lds_bytes, "spread_contract_ms": t.spread_contract_ms,
                 "candidates": t.candidates}
 
+    def set_residual_check(self, on: bool) -> None:
+        """Recompute the true residual (one extra FP64 operator application) whenever
+        stats are returned (default on; cr_set_residual_check).  Off, ``true_resid`` and
+        ``true_relres`` read -1."""
+        _lib.check(self.lib.cr_set_residual_check(self.ctx, int(bool(on))))
+
     def set_tiles(self, tile, lds_bytes: int = 64 * 1024) -> None:
         _lib.check(self.lib.cr_set_tiles(self.ctx, (C.c_int * 3)(*[int(v) for v in tile]), int(lds_bytes)))
 

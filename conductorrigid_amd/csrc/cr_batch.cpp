@@ -644,7 +644,22 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
         CR_HIP(hipMemcpyAsync(phi_inout[b], x + o + v.N, sizeof(double) * v.N_isolated, hipMemcpyDeviceToDevice, st));
     }
     CR_HIP(hipMemcpyAsync(u, x, sizeof(double) * B->n_tot, hipMemcpyDeviceToDevice, st));
-    if (stats) {
+    bool check = stats != nullptr;
+    for (int b = 0; b < nb && check; ++b) check = ctx[b]->check_residual;
+    if (stats && !check) {
+        for (int b = 0; b < nb; ++b) {
+            cr_solve_stats& S = stats[b];
+            S.iterations = iters[b];
+            S.matvecs = matvecs[b];
+            S.restarts = restarts[b];
+            S.converged = conv[b];
+            S.resid = resid[b];
+            S.tolerance = tol[b];
+            S.true_resid = -1.0;
+            S.true_relres = -1.0;
+        }
+    }
+    if (check) {
         // FP64 judge of every box from one batched operator application r = A x - b:
         // ||D r||, ||r|| and ||b|| per box, read back together
         double* dr = B->yraw.as<double>();

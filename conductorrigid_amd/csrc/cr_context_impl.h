@@ -10,6 +10,7 @@ using namespace cr;
 struct cr_context {
     int device = 0;
     bool ready = false;
+    bool check_residual = true;  // recompute the true residual when stats are requested
     cr_params params{};
     cr_resolved res{};
     MeshParams mp{};

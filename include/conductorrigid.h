@@ -272,6 +272,14 @@ typedef struct {
 int cr_autotune(cr_context* ctx, const cr_system* sys, cr_tuning* out, void* stream);
 int cr_set_tiles(cr_context* ctx, const int tile[3], int lds_bytes);
 
+/* No reference counterpart: the reference never checks the converged state
+ * (cppmodule/ConductorRigid.cu:1142-1147).  With the check on (the default), a force
+ * evaluation that is given a stats struct also recomputes the residual from one FP64
+ * operator application (true_resid, true_relres); off, those two fields are set to -1 and
+ * the evaluation does only the solve and the forces.  Applies to cr_compute_force and, via
+ * each box's context, to cr_batch_compute_force (checked when every box has it on). */
+int cr_set_residual_check(cr_context* ctx, int on);
+
 /* ---- batched boxes ------------------------------------------------------
  * No reference counterpart: the reference computes one box per computeForces call
  * (cppmodule/ConductorRigid.cc:403-557).  A batch solves many independent periodic boxes

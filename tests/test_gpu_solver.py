@@ -101,3 +101,23 @@ def test_100k_properties(require_gpu):
     e = rel(F32[:, :3], F[:, :3])
     print(f"[100k] fp32 iters={st32['iterations']} force err vs fp64 {e:.2e}", flush=True)
     assert e < 1e-5
+
+
+def test_residual_check_switch(require_gpu):
+    """cr_set_residual_check: off, the evaluation skips the true-residual operator
+    application (true_relres reads -1) and returns the same charges and forces."""
+    from helpers import n1k_system
+    from conductorrigid_amd.compute import ConductorRigid
+
+    s = n1k_system()
+    cr = ConductorRigid(system=s, xi=0.929, errortol=1e-6, rc=4.0, mesh=(64, 64, 64), P=8, precision="fp32",
+                        field=(0.0, 0.0, 1.0), gmres_tol=1e-6, preconditioner="block")
+    on = cr.computeForces(0, x0="zero")
+    q_on, f_on = cr.get_charges().copy(), cr.get_forces().copy()
+    cr.set_residual_check(False)
+    off = cr.computeForces(0, x0="zero")
+    print(f"[check] on relres={on['true_relres']:.2e} off relres={off['true_relres']}", flush=True)
+    assert on["true_relres"] >= 0 and off["true_relres"] == -1.0
+    # the spread's float atomics make repeated evaluations equal to rounding, not bitwise
+    assert abs(on["iterations"] - off["iterations"]) <= 1
+    assert rel(cr.get_charges(), q_on) < 1e-5 and rel(cr.get_forces(), f_on) < 1e-5
