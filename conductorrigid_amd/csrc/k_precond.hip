@@ -380,6 +380,8 @@ hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, uns
         CR_BLOCK_WAVE(8);
     } else if (nmax <= 16) {
         CR_BLOCK_WAVE(16);
+    } else if (nmax <= 20) {
+        CR_BLOCK_WAVE(20);  // the common 20-bead body: no padded rows
     } else if (nmax <= 24) {
         CR_BLOCK_WAVE(24);
     } else if (nmax <= 32) {
@@ -419,6 +421,8 @@ hipError_t launch_block_setup_batch(const BlockBox* boxes, int nbox, int total_b
         CR_BLOCK_WAVE_B(8);
     else if (nmax_all <= 16)
         CR_BLOCK_WAVE_B(16);
+    else if (nmax_all <= 20)
+        CR_BLOCK_WAVE_B(20);
     else if (nmax_all <= 24)
         CR_BLOCK_WAVE_B(24);
     else if (nmax_all <= 32)

@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void k_bead_records(MeshParams mp, TileGeom tg
 
 // Fill the chunk's LDS tables for beads [k0, k0+cnt) from their window records.  FP32:
 // three threads per bead evaluate its 3P axis weights exp(-c_d (p - u_d)^2), one axis
-// each; FP64: the stored weights are copied.
+// each, and keep u; FP64: the stored weights are copied.
 template <typename T>
 __device__ __forceinline__ void stage_records(const MeshParams& mp, int P, const unsigned* __restrict__ sorted,
                                               unsigned k0, int cnt, const int* __restrict__ rec_off,
@@ -226,6 +226,7 @@ __device__ __forceinline__ void stage_records(const MeshParams& mp, int P, const
         for (int j = threadIdx.x; j < 3 * cnt; j += blockDim.x) {
             const int i = j / 3, d = j - 3 * i;
             const T ud = rec_win[4 * (long)(k0 + i) + d];
+            s.u[j] = ud;  // the force contraction's (u - p) factors
             const T cd = axis_coef<T>(mp, d);
             T* w = s.w + i * per + d * P;
             for (int p = 0; p < P; ++p) {
@@ -369,10 +370,10 @@ __device__ __forceinline__ void contract_tile(const MeshParams& mp, const TileGe
     const unsigned kb = tstart[tc.t], ke = tstart[tc.t + 1];
     for (unsigned k0 = kb; k0 < ke; k0 += ch) {
         const int cnt = (int)min((unsigned)ch, ke - k0);
-        if (FORCE)
+        if (FORCE && sizeof(T) == sizeof(double))  // FP64 records hold weights, not positions
             stage_chunk<T>(mp, tc, sorted, k0, cnt, pos, q, (T)1, s);
         else
-            stage_records<T>(mp, P, sorted, k0, cnt, rec_off, rec_win, nullptr, (T)1, s);
+            stage_records<T>(mp, P, sorted, k0, cnt, rec_off, rec_win, FORCE ? q : nullptr, (T)1, s);
         for (int i = wave; i < cnt; i += nwave) {
             const int o = s.off[i];
             const int ox = o & 1023, oy = (o >> 10) & 1023, oz = o >> 20;
