@@ -476,12 +476,14 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
             double* vi = V + (long)i * c->ld;
             CR_RET(matvec_prec(c, v, vi, w, st));
             ++matvecs;
-            // CGS2: two projections onto V[:,0:i+1]
+            // CGS2: two projections onto V[:,0:i+1]; the second correction is applied only
+            // when it is not negligible against ||w|| (h2[i+1] = ||w||^2 after the first)
+            const bool wn = i + 1 < 32;
             CR_HIP(launch_multidot(V, c->ld, i + 1, w, n, partial, st));
             CR_HIP(launch_reduce_partials(partial, nblk, i + 1, g->h1, st));
-            CR_HIP(launch_gs_update_dot(V, c->ld, i + 1, g->h1, w, n, partial, st));
-            CR_HIP(launch_reduce_partials(partial, nblk, i + 1, g->h2, st));
-            CR_HIP(launch_gs_update(V, c->ld, i + 1, g->h2, w, n, 2, partial, st));
+            CR_HIP(launch_gs_update_dot(V, c->ld, i + 1, g->h1, w, n, partial, st, wn));
+            CR_HIP(launch_reduce_partials(partial, nblk, i + 1 + (wn ? 1 : 0), g->h2, st));
+            CR_HIP(launch_gs_update(V, c->ld, i + 1, g->h2, w, n, wn ? 3 : 2, partial, st));
             CR_HIP(launch_reduce_partials(partial, nblk, 1, &g->wnorm2, st));
             CR_HIP(launch_givens(g, i, R, st));
             CR_HIP(launch_scale_copy(w, &g->inv_norm, 1.0, V + (long)(i + 1) * c->ld, n, st));

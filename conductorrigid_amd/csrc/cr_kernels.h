@@ -347,13 +347,16 @@ hipError_t launch_multidot(const double* V, long ld, int ncols, const double* w,
                            hipStream_t st);
 // out[k] = sum over blocks of partial[b*32+k], k < ncols (fixed summation order)
 hipError_t launch_reduce_partials(const double* partial, int nblocks, int ncols, double* out, hipStream_t st);
-// w -= V[:,0:ncols] h; then (mode 1) partial dots V^T w, (mode 2) partial ||w||^2, (mode 0) nothing
+// w -= V[:,0:ncols] h; then (mode 1) partial dots V^T w, (mode 2) partial ||w||^2, (mode 0) nothing;
+// mode 3: mode 2 unless ||h[0:ncols]|| is below 1e-13 ||w|| with h[ncols] = ||w||^2, in which
+// case w is left as it is and the partials carry h[ncols] (CGS2's second correction)
 hipError_t launch_gs_update(const double* V, long ld, int ncols, const double* h, double* w, long n, int mode,
                             double* partial, hipStream_t st);
 // y = a*x + b*y elementwise helpers
-// w -= V[:, 0:ncols] h and partial = the projections of the updated w (layout of launch_multidot)
+// w -= V[:, 0:ncols] h and partial = the projections of the updated w (layout of launch_multidot);
+// with_norm (ncols < 32) adds column ncols: the updated w's squared norm
 hipError_t launch_gs_update_dot(const double* V, long ld, int ncols, const double* h, double* w, long n,
-                                double* partial, hipStream_t st);
+                                double* partial, hipStream_t st, bool with_norm = false);
 hipError_t launch_axpby(double a, const double* x, double b, double* y, long n, hipStream_t st);
 hipError_t launch_scale_copy(const double* x, const double* s_ptr, double sign, double* y, long n, hipStream_t st);
 // x += V[:,0:k] c  (c on device)

@@ -84,7 +84,13 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __rest
     }
 }
 
-// w -= V[:, 0:ncols] h ; mode 2 also accumulates ||w||^2 partials into partial[b*32]
+// Relative size below which CGS2's second correction is not applied: ||h2|| <= this *
+// ||w|| changes w by less than that, far below what the GMRES tolerances resolve.
+constexpr double kReorthSkip = 1e-13;
+
+// w -= V[:, 0:ncols] h ; mode 2 also accumulates ||w||^2 partials into partial[b*32].
+// Mode 3 is mode 2 for CGS2's second correction with h[ncols] = ||w||^2 before it: when
+// ||h[0:ncols]|| <= kReorthSkip ||w|| the update is skipped and the partials carry ||w||^2.
 __global__ __launch_bounds__(kBlock) void k_gs_update(const double* __restrict__ V, long ld, int ncols,
                                                       const double* __restrict__ h, double* __restrict__ w, long n,
                                                       int mode, double* __restrict__ partial)
@@ -92,6 +98,15 @@ __global__ __launch_bounds__(kBlock) void k_gs_update(const double* __restrict__
     __shared__ double hs[32];
     if (threadIdx.x < 32) hs[threadIdx.x] = (threadIdx.x < (unsigned)ncols) ? h[threadIdx.x] : 0.0;
     __syncthreads();
+    if (mode == 3) {
+        double h2 = 0.0;
+        for (int k = 0; k < ncols; ++k) h2 += hs[k] * hs[k];
+        const double wn2 = h[ncols];
+        if (h2 <= kReorthSkip * kReorthSkip * wn2) {
+            if (threadIdx.x == 0) partial[(long)blockIdx.x * 32] = blockIdx.x == 0 ? wn2 : 0.0;
+            return;
+        }
+    }
     double acc[kChunk];
 #pragma unroll
     for (int k = 0; k < kChunk; ++k) acc[k] = 0.0;
@@ -101,7 +116,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_update(const double* __restrict__
         w[i] = wi;
         acc[0] += wi * wi;
     }
-    if (mode == 2) {
+    if (mode >= 2) {
         __shared__ double res[kChunk];
         block_reduce_chunk(acc, res);
         if (threadIdx.x == 0) partial[(long)blockIdx.x * 32] = res[0];
@@ -110,10 +125,11 @@ __global__ __launch_bounds__(kBlock) void k_gs_update(const double* __restrict__
 
 // w -= V[:, 0:ncols] h, fused with the next projection: partial[b*32 + k] = sum over the
 // block's elements of V[k][i] * w_new[i] (same layout as k_multidot), so CGS2's first
-// update and second projection read the basis once.
+// update and second projection read the basis once.  with_norm (ncols < 32): column
+// ncols also gets the block's sum of w_new[i]^2.
 __global__ __launch_bounds__(kBlock) void k_gs_update_dot(const double* __restrict__ V, long ld, int ncols,
                                                           const double* __restrict__ h, double* __restrict__ w,
-                                                          long n, double* __restrict__ partial)
+                                                          long n, double* __restrict__ partial, int with_norm)
 {
     __shared__ double hs[32];
     if (threadIdx.x < 32) hs[threadIdx.x] = (threadIdx.x < (unsigned)ncols) ? h[threadIdx.x] : 0.0;
@@ -132,18 +148,21 @@ __global__ __launch_bounds__(kBlock) void k_gs_update_dot(const double* __restri
             }
         w[i] = wi;
 #pragma unroll
-        for (int k = 0; k < 32; ++k)
+        for (int k = 0; k < 32; ++k) {
             if (k < ncols) acc[k] += v[k] * wi;
+            else if (k == ncols && with_norm) acc[k] += wi * wi;
+        }
     }
+    const int nout = ncols + (with_norm ? 1 : 0);
 #pragma unroll
     for (int c0 = 0; c0 < 32; c0 += kChunk) {
-        if (c0 >= ncols) break;
+        if (c0 >= nout) break;
         double part[kChunk];
 #pragma unroll
         for (int k = 0; k < kChunk; ++k) part[k] = acc[c0 + k];
         __shared__ double res[kChunk];
         block_reduce_chunk(part, res);
-        if (threadIdx.x < kChunk && c0 + (int)threadIdx.x < ncols)
+        if (threadIdx.x < kChunk && c0 + (int)threadIdx.x < nout)
             partial[(long)blockIdx.x * 32 + c0 + threadIdx.x] = res[threadIdx.x];
         __syncthreads();
     }
@@ -294,10 +313,11 @@ hipError_t launch_gs_update(const double* V, long ld, int ncols, const double* h
 }
 
 hipError_t launch_gs_update_dot(const double* V, long ld, int ncols, const double* h, double* w, long n,
-                                double* partial, hipStream_t st)
+                                double* partial, hipStream_t st, bool with_norm)
 {
     if (ncols > 32) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_gs_update_dot, dim3(vblocks(n)), dim3(kBlock), 0, st, V, ld, ncols, h, w, n, partial);
+    hipLaunchKernelGGL(k_gs_update_dot, dim3(vblocks(n)), dim3(kBlock), 0, st, V, ld, ncols, h, w, n, partial,
+                       with_norm ? 1 : 0);
     return hipGetLastError();
 }
 
