@@ -60,6 +60,7 @@ struct cr_batch {
     long Gmax = 0;
     DevBuf V, u, w, x, b, Mb, yraw, partial, uf;
     DevBuf d_spread, d_contract, d_rows, d_segs, d_gs, d_skip, d_finished, d_step, d_pub, d_rel, d_blocks;
+    DevBuf d_width;  // ELL width per box
     int total_bodies = 0;
     std::vector<BlockBox> blocks;
     double* h_pub = nullptr;  // pinned: 3 doubles per box
@@ -333,6 +334,11 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     int blk = 0, bblk = 0, rblk = 0, tiles = 0, bodies = 0, nmax_all = 1;
     size_t lds_s = 0, lds_c = 0;
     long Gmax = 0;
+    // binning, records, objective and body grouping of every box, then the ELL widths of
+    // all boxes read back with one synchronisation, then the ELL builds spread over the
+    // side streams (small boxes leave most of the device idle one at a time)
+    CR_HIP(B->d_width.ensure(sizeof(unsigned) * nbox));
+    unsigned* d_width = B->d_width.as<unsigned>();
     for (int b = 0; b < nbox; ++b) {
         cr_context* c = ctx[b];
         SysView& v = B->views[b];
@@ -343,7 +349,24 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
             return set_error(CR_ERR_INVALID, "null output array");
         CR_HIP(launch_objective(v, c->mp.L, c->E0, c->b.as<double>(), st));
         if (block_precond(c)) CR_RET(block_setup(c, v, st, false));  // inverses: one launch below
-        CR_RET(ensure_csr(c, v, st));
+        CR_RET(csr_width(v, d_width + b, st));
+    }
+    {
+        std::vector<unsigned> width(nbox);
+        CR_HIP(hipMemcpyAsync(width.data(), d_width, sizeof(unsigned) * nbox, hipMemcpyDeviceToHost, st));
+        CR_HIP(hipStreamSynchronize(st));
+        CR_HIP(hipEventRecord(B->fork, st));
+        for (int k = 0; k < kSideStreams; ++k) CR_HIP(hipStreamWaitEvent(B->side[k], B->fork, 0));
+        for (int b = 0; b < nbox; ++b) CR_RET(csr_build(ctx[b], B->views[b], width[b], B->side[b % kSideStreams]));
+        for (int k = 0; k < kSideStreams; ++k) {
+            CR_HIP(hipEventRecord(B->join[k], B->side[k]));
+            CR_HIP(hipStreamWaitEvent(st, B->join[k], 0));
+        }
+    }
+    for (int b = 0; b < nbox; ++b) {
+        cr_context* c = ctx[b];
+        SysView& v = B->views[b];
+        const long n = (long)v.N + v.N_isolated;
 
         BlockBox& bb = B->blocks[b];
         bb = BlockBox{};
@@ -699,8 +722,15 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     else
         CR_HIP(launch_contract_tiled_batch<double>(B->d_contract.as<TiledBox>(), nb, B->total_tiles, c0->mp.P,
                                                    B->lds_contract, nullptr, st, true));
+    // per-box pair forces, spread over the side streams
+    CR_HIP(hipEventRecord(B->fork, st));
+    for (int k = 0; k < kSideStreams; ++k) CR_HIP(hipStreamWaitEvent(B->side[k], B->fork, 0));
     for (int b = 0; b < nb; ++b)
-        CR_RET(real_space_force_any(ctx[b], B->views[b], x + B->segs[b].xoff, force[b], st));
+        CR_RET(real_space_force_any(ctx[b], B->views[b], x + B->segs[b].xoff, force[b], B->side[b % kSideStreams]));
+    for (int k = 0; k < kSideStreams; ++k) {
+        CR_HIP(hipEventRecord(B->join[k], B->side[k]));
+        CR_HIP(hipStreamWaitEvent(st, B->join[k], 0));
+    }
     CR_HIP(hipEventRecord(B->done_ev, st));
     CR_HIP(hipStreamWaitEvent(caller, B->done_ev, 0));
     return CR_OK;

@@ -237,16 +237,28 @@ int contract_force_impl(cr_context* c, const SysView& v, const double* q, double
 
 // Assemble the real-space operator for the current positions (valid until the next
 // prepare()).  The pair arrays share the neighbour list's row offsets.
+int csr_width(const SysView& v, unsigned* dwidth, hipStream_t st)
+{
+    // ELL width: the longest neighbour row bounds the longest in-cutoff row
+    CR_HIP(launch_fill<unsigned>(dwidth, 0u, 1, st));
+    if (v.N > 0) CR_HIP(launch_max_u32(v.n_neigh, v.N, dwidth, st));
+    return CR_OK;
+}
+
 int ensure_csr(cr_context* c, const SysView& v, hipStream_t st)
 {
     if (c->csr_ready || v.N <= 0) return CR_OK;
-    // ELL width: the longest neighbour row bounds the longest in-cutoff row
     unsigned* dmax = reinterpret_cast<unsigned*>(c->scal.as<double>() + 32);
-    CR_HIP(launch_fill<unsigned>(dmax, 0u, 1, st));
-    CR_HIP(launch_max_u32(v.n_neigh, v.N, dmax, st));
+    CR_RET(csr_width(v, dmax, st));
     unsigned width = 0;
     CR_HIP(hipMemcpyAsync(&width, dmax, sizeof(unsigned), hipMemcpyDeviceToHost, st));
     CR_HIP(hipStreamSynchronize(st));
+    return csr_build(c, v, width, st);
+}
+
+int csr_build(cr_context* c, const SysView& v, unsigned width, hipStream_t st)
+{
+    if (c->csr_ready || v.N <= 0) return CR_OK;
     c->rs_ldn = ((long)v.N + 63) / 64 * 64;
     const size_t cells = std::max<size_t>(1, (size_t)width * (size_t)c->rs_ldn);
     const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);

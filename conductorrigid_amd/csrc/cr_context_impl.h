@@ -71,6 +71,10 @@ BinView bin_view(const cr_context* c);
 bool contract_tiled_ok(const cr_context* c, size_t elem);
 // Real-space ELL operator for the current positions (valid until the next prepare()).
 int ensure_csr(cr_context* c, const SysView& v, hipStream_t st);
+// The two halves of ensure_csr for callers that prepare many systems: the ELL width into
+// dwidth (device), then, with the width read back, the allocation and the build.
+int csr_width(const SysView& v, unsigned* dwidth, hipStream_t st);
+int csr_build(cr_context* c, const SysView& v, unsigned width, hipStream_t st);
 // grid <- Re IFFT(S FFT(grid)) on the context's mesh, in its precision.
 int recip_any(cr_context* c, hipStream_t st);
 // FFT plans of the context's precision (x, y, z).
