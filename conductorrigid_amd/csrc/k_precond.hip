@@ -271,6 +271,7 @@ __device__ __forceinline__ void block_apply_body(int t, int N, int nmax, const u
 
 // Single system: K = 64 / nmax bodies share a wavefront (lane = body slot * nmax + bead),
 // four wavefronts per workgroup, so small bodies do not leave most lanes idle.
+template <int R4>
 __global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int K, const unsigned* __restrict__ start,
                                                      const unsigned* __restrict__ beads,
                                                      const float* __restrict__ ginv, const float* __restrict__ uvec,
@@ -282,6 +283,14 @@ __global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int
     const int k = lane / nmax, i = lane - k * nmax;
     const int t = (blockIdx.x * 4 + wave) * K + k;
     const bool live = k < K && t < B;
+    // R4 > 0 (nmax = 4 R4): the row of G^-1 does not depend on y, so its loads go out
+    // first and overlap the start -> beads -> y chain
+    float4 g4[R4 > 0 ? R4 : 1];
+    if constexpr (R4 > 0) {
+        const float4* r4 = reinterpret_cast<const float4*>(ginv + ((long)t * nmax + i) * nmax);
+#pragma unroll
+        for (int m = 0; m < R4; ++m) g4[m] = live ? r4[m] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     unsigned b0 = 0;
     int n = 0;
     if (live) {
@@ -298,7 +307,18 @@ __global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int
     red[threadIdx.x] = ui * yi;
     __syncthreads();
     double gy = 0.0;
-    if (act) {
+    if constexpr (R4 > 0) {
+        if (act) {
+#pragma unroll
+            for (int m = 0; m < R4; ++m) {
+                const int j = 4 * m;
+                if (j < n) gy += (double)g4[m].x * ys[base + j];
+                if (j + 1 < n) gy += (double)g4[m].y * ys[base + j + 1];
+                if (j + 2 < n) gy += (double)g4[m].z * ys[base + j + 2];
+                if (j + 3 < n) gy += (double)g4[m].w * ys[base + j + 3];
+            }
+        }
+    } else if (act) {
         const float* row = ginv + ((long)t * nmax + i) * nmax;
         if ((nmax & 3) == 0) {
             // rows are 16-byte aligned: four entries per load; entries past n are never
@@ -406,8 +426,20 @@ hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_s
     if (s.N_isolated <= 0) return hipSuccess;
     const int K = nmax <= 64 ? 64 / nmax : 1;
     const unsigned nb = (unsigned)((s.N_isolated + 4 * K - 1) / (4 * K));
-    hipLaunchKernelGGL(k_block_apply, dim3(nb), dim3(256), 0, st, s.N_isolated, s.N, nmax, K, body_start, body_beads,
-                       ginv, uvec, inv_s, y, w);
+#define CR_BLOCK_APPLY(R4)                                                                                     \
+    hipLaunchKernelGGL(k_block_apply<R4>, dim3(nb), dim3(256), 0, st, s.N_isolated, s.N, nmax, K, body_start,  \
+                       body_beads, ginv, uvec, inv_s, y, w)
+    switch (nmax) {
+        case 8: CR_BLOCK_APPLY(2); break;
+        case 12: CR_BLOCK_APPLY(3); break;
+        case 16: CR_BLOCK_APPLY(4); break;
+        case 20: CR_BLOCK_APPLY(5); break;
+        case 24: CR_BLOCK_APPLY(6); break;
+        case 28: CR_BLOCK_APPLY(7); break;
+        case 32: CR_BLOCK_APPLY(8); break;
+        default: CR_BLOCK_APPLY(0); break;
+    }
+#undef CR_BLOCK_APPLY
     return hipGetLastError();
 }
 
