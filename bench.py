@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 
 # matvec stage -> kernel it times (FP32-bulk, P = 8)
 STAGE_KERNEL = {
-    "mesh_clear": "k_fill<float>",
+    "mesh_clear": "k_fill16",
     "spread": "k_spread_tiled<float, double, 8>",
     "fft_z_r2c": "k_fft_z_r2c<float>",
     "fft_y_fwd": "k_fft_strided<float>",

@@ -8,6 +8,9 @@
 // the wavefront with cross-lane shuffles instead of a shared-memory tree.
 #include "cr_kernels.h"
 
+#include <cstdint>
+#include <cstring>
+
 namespace cr {
 
 namespace {
@@ -149,6 +152,14 @@ __global__ void k_fill(T* p, T v, long n)
     for (; i < n; i += stride) p[i] = v;
 }
 
+// 16 bytes per store (the bit pattern of 4 floats or 2 doubles)
+__global__ void k_fill16(uint4* p, uint4 v, long n16)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) p[i] = v;
+}
+
 template <typename Tin, typename Tout>
 __global__ void k_convert(const Tin* in, Tout* out, long n)
 {
@@ -201,6 +212,21 @@ template <typename T>
 hipError_t launch_fill(T* p, T v, long n, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
+    constexpr long per16 = 16 / (long)sizeof(T);
+    if constexpr (sizeof(T) == 4 || sizeof(T) == 8) {
+        if (reinterpret_cast<uintptr_t>(p) % 16 == 0 && n >= 4096) {
+            // wide stores for the bulk, element stores for the tail
+            unsigned w[4];
+            for (int k = 0; k < 4; ++k) std::memcpy(&w[k], reinterpret_cast<const char*>(&v) + (4 * k) % sizeof(T), 4);
+            const long n16 = n / per16;
+            hipLaunchKernelGGL(k_fill16, dim3(grid_1d(n16, 256)), dim3(256), 0, st, reinterpret_cast<uint4*>(p),
+                               make_uint4(w[0], w[1], w[2], w[3]), n16);
+            const long done = n16 * per16;
+            if (done < n)
+                hipLaunchKernelGGL(k_fill<T>, dim3(1), dim3(256), 0, st, p + done, v, n - done);
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL(k_fill<T>, dim3(grid_1d(n, 256)), dim3(256), 0, st, p, v, n);
     return hipGetLastError();
 }
