@@ -143,11 +143,13 @@ __device__ __forceinline__ void block_setup_body(int t, int nmax, const unsigned
                                                  const unsigned* __restrict__ beads,
                                                  const double4* __restrict__ pos, double Lx, double Ly, double Lz,
                                                  float* __restrict__ ginv, float* __restrict__ uvec,
-                                                 double* __restrict__ inv_s)
+                                                 double* __restrict__ inv_s, int nlo = 0, bool last = true)
 {
     const int i = threadIdx.x & 63;
     const int b0 = start[t];
     const int n = start[t + 1] - b0;
+    if (n <= nlo) return;            // a launch for smaller bodies takes this one
+    if (n > NB && !last) return;     // ... and one for larger bodies this one
     if (n > NB) {
         if (i == 0) inv_s[t] = __builtin_nan("");
         return;
@@ -215,9 +217,10 @@ __global__ __launch_bounds__(256) void k_block_setup_wave(int B, int nmax, const
 }
 
 // All bodies of a batch of boxes in one launch (body0 ascending over boxes).
+// Bodies with nlo < n <= NB; the last launch (NB >= every body) also marks larger ones.
 template <int NB>
 __global__ __launch_bounds__(256) void k_block_setup_wave_batch(const BlockBox* __restrict__ boxes, int nbox,
-                                                                int total)
+                                                                int total, int nlo, int last)
 {
     const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (g >= total) return;  // whole wavefront
@@ -229,7 +232,8 @@ __global__ __launch_bounds__(256) void k_block_setup_wave_batch(const BlockBox* 
     }
     const BlockBox& d = boxes[lo];
     block_setup_body<NB>(g - d.body0, d.nmax, d.start, d.beads, d.pos, d.L[0], d.L[1], d.L[2],
-                         const_cast<float*>(d.ginv), const_cast<float*>(d.uvec), const_cast<double*>(d.inv_s));
+                         const_cast<float*>(d.ginv), const_cast<float*>(d.uvec), const_cast<double*>(d.inv_s), nlo,
+                         last != 0);
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -446,26 +450,30 @@ hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_s
 hipError_t launch_block_setup_batch(const BlockBox* boxes, int nbox, int total_bodies, int nmax_all, hipStream_t st)
 {
     if (total_bodies <= 0) return hipSuccess;
+    if (nmax_all > 64) return hipErrorInvalidValue;
+    // one launch per body-size class up to the largest body: each body is inverted by the
+    // smallest instantiation that holds it (boxes mix body sizes)
     const unsigned wb = (unsigned)((total_bodies + 3) / 4);
-#define CR_BLOCK_WAVE_B(NB) \
-    hipLaunchKernelGGL(k_block_setup_wave_batch<NB>, dim3(wb), dim3(256), 0, st, boxes, nbox, total_bodies)
-    if (nmax_all <= 8)
-        CR_BLOCK_WAVE_B(8);
-    else if (nmax_all <= 16)
-        CR_BLOCK_WAVE_B(16);
-    else if (nmax_all <= 20)
-        CR_BLOCK_WAVE_B(20);
-    else if (nmax_all <= 24)
-        CR_BLOCK_WAVE_B(24);
-    else if (nmax_all <= 32)
-        CR_BLOCK_WAVE_B(32);
-    else if (nmax_all <= 40)
-        CR_BLOCK_WAVE_B(40);
-    else if (nmax_all <= 64)
-        CR_BLOCK_WAVE_B(64);
-    else
-        return hipErrorInvalidValue;
+    const int ladder[] = {8, 16, 20, 24, 32, 40, 64};
+    int nlo = 0;
+    for (int NB : ladder) {
+        const int last = NB >= nmax_all ? 1 : 0;
+#define CR_BLOCK_WAVE_B(NBV)                                                                                    \
+    hipLaunchKernelGGL(k_block_setup_wave_batch<NBV>, dim3(wb), dim3(256), 0, st, boxes, nbox, total_bodies, nlo, \
+                       last)
+        switch (NB) {
+            case 8: CR_BLOCK_WAVE_B(8); break;
+            case 16: CR_BLOCK_WAVE_B(16); break;
+            case 20: CR_BLOCK_WAVE_B(20); break;
+            case 24: CR_BLOCK_WAVE_B(24); break;
+            case 32: CR_BLOCK_WAVE_B(32); break;
+            case 40: CR_BLOCK_WAVE_B(40); break;
+            default: CR_BLOCK_WAVE_B(64); break;
+        }
 #undef CR_BLOCK_WAVE_B
+        if (NB >= nmax_all) break;
+        nlo = NB;
+    }
     return hipGetLastError();
 }
 
