@@ -26,6 +26,8 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <string>
+#include <thread>
 #include <vector>
 
 using namespace cr;
@@ -339,17 +341,48 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     // side streams (small boxes leave most of the device idle one at a time)
     CR_HIP(B->d_width.ensure(sizeof(unsigned) * nbox));
     unsigned* d_width = B->d_width.as<unsigned>();
-    for (int b = 0; b < nbox; ++b) {
-        cr_context* c = ctx[b];
-        SysView& v = B->views[b];
-        CR_RET(prepare(c, &sys[b], st, &v));
-        const long n = (long)v.N + v.N_isolated;
-        if (n > c->n_cap) return set_error(CR_ERR_INVALID, "system larger than the size given to cr_set_params");
-        if ((v.N > 0 && (!q_inout[b] || !force[b])) || (v.N_isolated > 0 && !phi_inout[b]))
-            return set_error(CR_ERR_INVALID, "null output array");
-        CR_HIP(launch_objective(v, c->mp.L, c->E0, c->b.as<double>(), st));
-        if (block_precond(c)) CR_RET(block_setup(c, v, st, false));  // inverses: one launch below
-        CR_RET(csr_width(v, d_width + b, st));
+    {
+        // ~15 small launches per box: issued from one host thread per side stream, so the
+        // launches of different boxes go out in parallel (each box uses only its context)
+        CR_HIP(hipEventRecord(B->fork, st));
+        for (int k = 0; k < kSideStreams; ++k) CR_HIP(hipStreamWaitEvent(B->side[k], B->fork, 0));
+        int rc[kSideStreams];
+        std::string msg[kSideStreams];
+        auto prepare_boxes = [&](int k) {
+            rc[k] = CR_OK;
+            if (hipSetDevice(B->device) != hipSuccess) {
+                rc[k] = set_error(CR_ERR_DEVICE, "hipSetDevice failed");
+                msg[k] = g_error;
+                return;
+            }
+            const hipStream_t sk = B->side[k];
+            for (int b = k; b < nbox && rc[k] == CR_OK; b += kSideStreams) {
+                rc[k] = [&]() -> int {
+                    cr_context* c = ctx[b];
+                    SysView& v = B->views[b];
+                    CR_RET(prepare(c, &sys[b], sk, &v));
+                    const long n = (long)v.N + v.N_isolated;
+                    if (n > c->n_cap)
+                        return set_error(CR_ERR_INVALID, "system larger than the size given to cr_set_params");
+                    if ((v.N > 0 && (!q_inout[b] || !force[b])) || (v.N_isolated > 0 && !phi_inout[b]))
+                        return set_error(CR_ERR_INVALID, "null output array");
+                    CR_HIP(launch_objective(v, c->mp.L, c->E0, c->b.as<double>(), sk));
+                    if (block_precond(c)) CR_RET(block_setup(c, v, sk, false));  // inverses: one launch below
+                    return csr_width(v, d_width + b, sk);
+                }();
+            }
+            if (rc[k] != CR_OK) msg[k] = g_error;
+        };
+        std::vector<std::thread> workers;
+        for (int k = 1; k < kSideStreams; ++k) workers.emplace_back(prepare_boxes, k);
+        prepare_boxes(0);
+        for (auto& t : workers) t.join();
+        for (int k = 0; k < kSideStreams; ++k) {
+            CR_HIP(hipEventRecord(B->join[k], B->side[k]));
+            CR_HIP(hipStreamWaitEvent(st, B->join[k], 0));
+        }
+        for (int k = 0; k < kSideStreams; ++k)
+            if (rc[k] != CR_OK) return set_error(rc[k], msg[k]);
     }
     {
         std::vector<unsigned> width(nbox);
