@@ -286,14 +286,15 @@ __device__ __forceinline__ void spread_tile(const MeshParams& mp, const TileGeom
     constexpr int PM = PT > 0 ? PT : 16;
     const int P = PT > 0 ? PT : mp.P;
     const int esz = tc.esz;
+    const unsigned kb = tstart[tc.t], ke = tstart[tc.t + 1];
+    if (kb == ke) return;  // an empty bin adds nothing to the (cleared) mesh
     A* ext = reinterpret_cast<A*>(smem_raw);
     Stage<T> s = stage_layout<T>(smem_raw, (size_t)esz * sizeof(A), ch, P);
     for (int k = threadIdx.x; k < esz; k += blockDim.x) ext[k] = (A)0;
-    // the write-out below reads nodes other threads zeroed, also when the bin is empty
+    // the write-out below reads nodes other threads zeroed
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     const int xs = tc.xs, ys = tc.ys, per = 3 * P;
-    const unsigned kb = tstart[tc.t], ke = tstart[tc.t + 1];
     for (unsigned k0 = kb; k0 < ke; k0 += ch) {
         const int cnt = (int)min((unsigned)ch, ke - k0);
         stage_records<T>(mp, P, sorted, k0, cnt, rec_off, rec_win, q, (T)mp.prefac, s);
@@ -324,7 +325,9 @@ __device__ __forceinline__ void spread_tile(const MeshParams& mp, const TileGeom
         gx -= gx >= mp.Nx ? mp.Nx : 0;
         gy -= gy >= mp.Ny ? mp.Ny : 0;
         gz -= gz >= mp.Nz ? mp.Nz : 0;
-        atomic_add(&grid[(long)gx * plane + (long)gy * mp.Nz + gz], (T)ext[bw.lx * xs + bw.ly * ys + bw.lz]);
+        // nodes no window of the bin reached stay exactly zero: adding them changes nothing
+        const A v = ext[bw.lx * xs + bw.ly * ys + bw.lz];
+        if (v != (A)0) atomic_add(&grid[(long)gx * plane + (long)gy * mp.Nz + gz], (T)v);
     }
 }
 
