@@ -438,7 +438,7 @@ int segments(cr_context* c, const SysView& v, hipStream_t st, unsigned** counts,
 // Restarted, left-preconditioned GMRES(R) following CUSP's krylov::gmres structure
 // (cppmodule/ConductorRigid.cu:1134-1136): the cycle starts from w = D(Ax - b), stops
 // when the Givens estimate |s_{i+1}| <= tol ||b||_2, x += V y at the end of each cycle.
-int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st, bool x_zero)
+int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st, bool x_zero, const double* ax0)
 {
     const long n = (long)v.N + v.N_isolated;
     const int R = c->R, maxiter = c->res.max_iter;
@@ -465,6 +465,10 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
             // x0 = 0: A x0 = 0 exactly, so the first cycle's residual needs no operator
             CR_HIP(launch_fill<double>(w, 0.0, n, st));
             x_zero = false;
+        } else if (ax0) {
+            // A x0 is known from the initial guess's construction
+            CR_RET(apply_precond(c, v, ax0, w, st));
+            ax0 = nullptr;
         } else {
             CR_RET(matvec_prec(c, v, x, w, st));
             ++matvecs;
@@ -592,9 +596,12 @@ int init_x0(cr_context* c, const SysView& v, int x0_mode, const double* q_in, co
         CR_HIP(hipMemcpyAsync(x, q_in, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
         CR_HIP(launch_project_charges(v, counts, sums, x, st));
         CR_HIP(launch_fill<double>(x + v.N, 0.0, v.N_isolated, st));
-        double* w = c->w.as<double>();
-        CR_RET(matvec_any(c, v, x, w, 0, st));
-        CR_HIP(launch_fit_potentials(v, counts, c->b.as<double>(), w, sums, x + v.N, st));
+        // A [q; 0] into yraw; with the fitted Phi added to the bead rows it is A x0, which
+        // the solver's first cycle then needs no operator application for
+        double* y = c->yraw.as<double>();
+        CR_RET(matvec_any(c, v, x, y, 0, st));
+        CR_HIP(launch_fit_potentials(v, counts, c->b.as<double>(), y, sums, x + v.N, st));
+        CR_HIP(launch_add_body_potentials(v, x + v.N, y, st));
     } else {
         return set_error(CR_ERR_INVALID, "unknown x0 mode");
     }
@@ -967,7 +974,8 @@ int cr_compute_force(cr_context* c, const cr_system* s, double* q_inout, double*
         return set_error(CR_ERR_INVALID, "unknown x0 mode");
     CR_RET(ensure_csr(c, v, st));
     CR_RET(init_x0(c, v, x0_mode, q_inout, phi_inout, x, st));
-    CR_RET(gmres(c, v, stats, st, x0_mode == CR_X0_ZERO));
+    CR_RET(gmres(c, v, stats, st, x0_mode == CR_X0_ZERO,
+                 x0_mode == CR_X0_CHARGES ? c->yraw.as<double>() : nullptr));
     CR_HIP(hipMemcpyAsync(q_inout, x, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
     CR_HIP(hipMemcpyAsync(phi_inout, x + v.N, sizeof(double) * v.N_isolated, hipMemcpyDeviceToDevice, st));
     if (stats && c->check_residual) CR_RET(true_residual(c, v, x, stats, st));

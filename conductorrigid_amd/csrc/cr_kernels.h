@@ -232,6 +232,9 @@ hipError_t launch_body_counts(const SysView& s, unsigned* counts, hipStream_t st
 hipError_t launch_project_charges(const SysView& s, const unsigned* counts, double* sums, double* q, hipStream_t st);
 // Phi_b = mean over the beads of b of (b_i - y_i), the least-squares equipotential of the
 // rows Phi_b - (M q)_i = b_i given y_i = -(M q)_i.  sums: scratch [N_isolated] doubles.
+// y[i] += Phi[body of bead i], i < N: A [q; Phi] from A [q; 0] (the body potential enters
+// only the bead rows, with coefficient 1)
+hipError_t launch_add_body_potentials(const SysView& s, const double* Phi, double* y, hipStream_t st);
 hipError_t launch_fit_potentials(const SysView& s, const unsigned* counts, const double* b, const double* y,
                                  double* sums, double* phi, hipStream_t st);
 hipError_t launch_zero_force(int N, double* force, hipStream_t st);

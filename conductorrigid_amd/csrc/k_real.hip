@@ -165,6 +165,13 @@ __global__ __launch_bounds__(256) void k_real_space_csr(RealParams rp, SysView s
     y[i] = precond ? -4.0 * kPi * out : out;
 }
 
+// y[i] += Phi[body of bead i] for the bead rows: A [q; Phi] = A [q; 0] + that.
+__global__ void k_add_body_potentials(SysView s, const double* __restrict__ Phi, double* __restrict__ y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < s.N) y[i] += Phi[s.iso[i]];
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_real_space_force(RealParams rp, SysView s, const double* __restrict__ q,
                                                           const T* __restrict__ gtable, double E0x, double E0y,
@@ -361,6 +368,13 @@ hipError_t launch_real_space_csr(const RealParams& rp, const SysView& s, const d
     if (s.N <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_real_space_csr<T>, dim3(blocks(s.N, 256)), dim3(256), 0, st, rp, s, x, qt, phi_recip, ldn,
                        col, val, row_len, y, precond);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_body_potentials(const SysView& s, const double* Phi, double* y, hipStream_t st)
+{
+    if (s.N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_add_body_potentials, dim3(blocks(s.N, 256)), dim3(256), 0, st, s, Phi, y);
     return hipGetLastError();
 }
 
