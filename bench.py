@@ -331,6 +331,7 @@ def main() -> int:
         hist = WS.fit(mlp, pairs, epochs=2, batch=16384, lr=2e-3)
         t_train = time.perf_counter() - t0
         del pairs
+        crw.set_residual_check(False)  # timed like the headline: solve + forces
         iters, wms = WS.compare_starts(crw, mlp, args.warm_steps, 2e-4, seed=1)
         warm = {"protocol": f"Brownian trajectory (dt 2e-4, rigid bodies, contact repulsion), |E0| ~ U[0.5, 2] "
                             f"per step; network trained on {len(range(0, args.train_steps, 2))} frames of the "
