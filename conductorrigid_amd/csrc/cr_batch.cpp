@@ -716,7 +716,7 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
         }
     }
     if (check) {
-        // FP64 judge of every box from one batched operator application r = A x - b:
+        // FP64 check of every box from one batched operator application r = A x - b:
         // ||D r||, ||r|| and ||b|| per box, read back together
         double* dr = B->yraw.as<double>();
         CR_RET(batch_matvec(B, u, w, 0, nullptr, nullptr, st));

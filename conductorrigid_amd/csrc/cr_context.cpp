@@ -610,7 +610,7 @@ int init_x0(cr_context* c, const SysView& v, int x0_mode, const double* q_in, co
 
 int true_residual(cr_context* c, const SysView& v, const double* x, cr_solve_stats* stats, hipStream_t st)
 {
-    // FP64 judge of the converged state from one operator application r = A x - b:
+    // FP64 check of the converged state from one operator application r = A x - b:
     // ||D r||_2 and the unpreconditioned ||r|| / ||b||
     const long n = (long)v.N + v.N_isolated;
     double* w = c->w.as<double>();
