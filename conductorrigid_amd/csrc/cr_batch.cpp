@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <string>
 #include <thread>
@@ -374,8 +375,16 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
             if (rc[k] != CR_OK) msg[k] = g_error;
         };
         std::vector<std::thread> workers;
-        for (int k = 1; k < kSideStreams; ++k) workers.emplace_back(prepare_boxes, k);
+        std::vector<int> inline_k;  // side streams whose boxes this thread prepares itself
+        for (int k = 1; k < kSideStreams; ++k) {
+            try {
+                workers.emplace_back(prepare_boxes, k);
+            } catch (const std::exception&) {
+                inline_k.push_back(k);  // no thread available: same work, sequentially
+            }
+        }
         prepare_boxes(0);
+        for (int k : inline_k) prepare_boxes(k);
         for (auto& t : workers) t.join();
         for (int k = 0; k < kSideStreams; ++k) {
             CR_HIP(hipEventRecord(B->join[k], B->side[k]));
