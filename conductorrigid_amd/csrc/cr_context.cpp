@@ -118,8 +118,10 @@ SysView make_view(cr_context* c, const cr_system* s)
     return v;
 }
 
-// Validate the system view, size per-bead buffers and refresh the body-tag map.
-int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out)
+// Validate the system view, size per-bead buffers and refresh the body-tag map; with bin,
+// also sort the beads into the mesh tiles and write their window records (only the mesh
+// operators read those).
+int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out, bool bin)
 {
     if (!c) return set_error(CR_ERR_INVALID, "null context");
     if (!c->ready) return set_error(CR_ERR_STATE, "cr_set_params has not been called");
@@ -138,7 +140,7 @@ int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out)
     CR_HIP(launch_membership(s->N, s->tag, s->body_of_tag, c->iso.as<unsigned>(), st));
     *out = make_view(c, s);
     c->csr_ready = false;  // positions or neighbours may have changed
-    if (c->tg.valid) {
+    if (bin && c->tg.valid) {
         // positions may have changed since the last call: rebin (cheap, O(N))
         const size_t nt = (size_t)c->tg.ntiles;
         CR_HIP(c->bead_tile.ensure(sizeof(unsigned) * (size_t)std::max(1, s->N)));
@@ -1193,7 +1195,7 @@ int cr_mlp_features(cr_context* c, const cr_system* s, unsigned short* X, void* 
 {
     hipStream_t st = (hipStream_t)stream;
     SysView v;
-    CR_RET(prepare(c, s, st, &v));
+    CR_RET(prepare(c, s, st, &v, false));
     if (v.N > 0 && !X) return set_error(CR_ERR_INVALID, "null feature array");
     unsigned* counts = nullptr;
     double* sums = nullptr;
@@ -1206,7 +1208,7 @@ int cr_mlp_targets(cr_context* c, const cr_system* s, const double* q, float* t,
 {
     hipStream_t st = (hipStream_t)stream;
     SysView v;
-    CR_RET(prepare(c, s, st, &v));
+    CR_RET(prepare(c, s, st, &v, false));
     if (v.N > 0 && (!q || !t)) return set_error(CR_ERR_INVALID, "null array");
     unsigned* counts = nullptr;
     double* sums = nullptr;
@@ -1219,7 +1221,7 @@ int cr_mlp_charges(cr_context* c, const cr_system* s, const float* y, double* q,
 {
     hipStream_t st = (hipStream_t)stream;
     SysView v;
-    CR_RET(prepare(c, s, st, &v));
+    CR_RET(prepare(c, s, st, &v, false));
     if (v.N > 0 && (!q || !y)) return set_error(CR_ERR_INVALID, "null array");
     unsigned* counts = nullptr;
     double* sums = nullptr;
@@ -1236,7 +1238,7 @@ int cr_bd_step(cr_context* c, const cr_system* s, double* pos, double* rigid_com
 {
     hipStream_t st = (hipStream_t)stream;
     SysView v;
-    CR_RET(prepare(c, s, st, &v));
+    CR_RET(prepare(c, s, st, &v, false));
     if ((v.N > 0 && (!pos || !force)) || (v.N_rigid > 0 && (!rigid_com || !noise_rigid)))
         return set_error(CR_ERR_INVALID, "null array");
     if (!(dt > 0) || !(kT >= 0) || !(gamma_t > 0) || !(gamma_r > 0) || !(k_rep >= 0))

@@ -65,8 +65,9 @@ struct cr_context {
 namespace cr::impl {
 
 SysView make_view(cr_context* c, const cr_system* s);
-// Validate the system, refresh the body-tag map, rebin beads and rebuild window records.
-int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out);
+// Validate the system and refresh the body-tag map; with bin, also rebin beads and rebuild
+// window records.
+int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out, bool bin = true);
 BinView bin_view(const cr_context* c);
 bool contract_tiled_ok(const cr_context* c, size_t elem);
 // Real-space ELL operator for the current positions (valid until the next prepare()).

@@ -98,79 +98,144 @@ __device__ __forceinline__ float dist2(double dx, double dy, double dz)
 // charge over its bead count [160]; the offset o from its rigid body's centre of mass
 // / 4 and |o| (zero for isolated beads) [161, 165); zero [165, 168); the gate: E0 as
 // bf16 pairs hi [168, 171) and lo = E0 - hi [171, 174); zero [174, 176).
-__global__ __launch_bounds__(256) void k_mlp_features(SysView s, double Lx, double Ly, double Lz, double E0x,
-                                                      double E0y, double E0z, const unsigned* __restrict__ body_count,
-                                                      unsigned short* __restrict__ X)
+// One thread per bead, 64 beads per workgroup.  The K nearest stay sorted in registers
+// (an unrolled insertion: entries after the first one farther than the candidate shift
+// down one slot, so equal distances keep list order).  Neighbours are read four at a
+// time so that their loads are in flight together; they are inserted in list order.
+// Rows are assembled in LDS in two halves of kFeatHalf columns, and each half of the
+// workgroup's 64 consecutive rows leaves as dword stores.
+constexpr int kFeatBlock = 64;
+constexpr int kFeatHalf = KIN / 2;         // 88 columns per half
+constexpr int kFeatPitch = kFeatHalf + 2;  // ushorts per LDS row: 45 dwords, odd, so rows fall on distinct banks
+constexpr int kFeatBatch = 4;
+
+// column col of the row is written when it lies in the half that starts at lo
+__device__ __forceinline__ void put(unsigned short* x, int lo, int col, unsigned short v)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.N) return;
+    if (col >= lo && col < lo + kFeatHalf) x[col - lo] = v;
+}
+
+__global__ __launch_bounds__(kFeatBlock) void k_mlp_features(SysView s, double Lx, double Ly, double Lz, double E0x,
+                                                             double E0y, double E0z,
+                                                             const unsigned* __restrict__ body_count,
+                                                             unsigned short* __restrict__ X)
+{
+    __shared__ unsigned short rows[kFeatBlock * kFeatPitch];
     constexpr int K = kMlpNeigh;
+    const int t = threadIdx.x;
+    const long b0 = (long)blockIdx.x * kFeatBlock;
+    const long i = b0 + t;
+    const bool live = i < s.N;
+    unsigned short* x = rows + t * kFeatPitch;
     float dist[K];
     unsigned idx[K];
-    int cnt = 0;
-    const double4 ri = s.pos[i];
-    const unsigned bi = s.iso[i];
-    const unsigned head = s.head_list[i], nn = s.n_neigh[i];
-    const float r2max = (float)(kMlpRadius * kMlpRadius);
-    for (unsigned k = 0; k < nn; ++k) {
-        const unsigned j = s.nlist[head + k];
-        const double4 rj = s.pos[j];
-        const float d2 = dist2(mi(ri.x - rj.x, Lx), mi(ri.y - rj.y, Ly), mi(ri.z - rj.z, Lz));
-        if (!(d2 < r2max)) continue;
-        // insertion into the sorted list of the K nearest (ties keep the earlier one)
-        int p = cnt < K ? cnt : K;
-        if (p == K && !(d2 < dist[K - 1])) continue;
-        if (p == K) p = K - 1;
-        while (p > 0 && dist[p - 1] > d2) {
-            dist[p] = dist[p - 1];
-            idx[p] = idx[p - 1];
-            --p;
-        }
-        dist[p] = d2;
-        idx[p] = j;
-        if (cnt < K) ++cnt;
-    }
-    unsigned short* x = X + (long)i * KIN;
-    const float inv = 1.0f / (float)kMlpRadius;
+#pragma unroll
     for (int k = 0; k < K; ++k) {
-        float f0 = 0, f1 = 0, f2 = 0, f3 = 0, f4 = 0;
-        if (k < cnt) {
-            const double4 rj = s.pos[idx[k]];
-            f0 = __fmul_rn((float)mi(rj.x - ri.x, Lx), inv);
-            f1 = __fmul_rn((float)mi(rj.y - ri.y, Ly), inv);
-            f2 = __fmul_rn((float)mi(rj.z - ri.z, Lz), inv);
-            f3 = (s.iso[idx[k]] == bi) ? 1.0f : -1.0f;
-            f4 = __fmul_rn(__fsqrt_rn(dist[k]), inv);
+        dist[k] = INFINITY;
+        idx[k] = 0;
+    }
+    int cnt = 0;
+    double4 ri = make_double4(0, 0, 0, 0);
+    unsigned bi = 0;
+    if (live) {
+        ri = s.pos[i];
+        bi = s.iso[i];
+        const unsigned head = s.head_list[i], nn = s.n_neigh[i];
+        const float r2max = (float)(kMlpRadius * kMlpRadius);
+        for (unsigned n0 = 0; n0 < nn; n0 += kFeatBatch) {
+            unsigned jb[kFeatBatch];
+            double4 rb[kFeatBatch];
+#pragma unroll
+            for (int u = 0; u < kFeatBatch; ++u) jb[u] = s.nlist[head + (n0 + u < nn ? n0 + u : n0)];
+#pragma unroll
+            for (int u = 0; u < kFeatBatch; ++u) rb[u] = s.pos[jb[u]];
+#pragma unroll
+            for (int u = 0; u < kFeatBatch; ++u) {
+                if (n0 + u >= nn) break;
+                const double4 rj = rb[u];
+                const float d2 = dist2(mi(ri.x - rj.x, Lx), mi(ri.y - rj.y, Ly), mi(ri.z - rj.z, Lz));
+                // in range, and closer than the current K-th (an empty slot holds +inf)
+                if (!(d2 < r2max) || !(d2 < dist[K - 1])) continue;
+                float cd = d2;
+                unsigned cj = jb[u];
+                bool shift = false;
+#pragma unroll
+                for (int p = 0; p < K; ++p) {
+                    shift = shift || (cd < dist[p]);
+                    const float od = dist[p];
+                    const unsigned oj = idx[p];
+                    dist[p] = shift ? cd : od;
+                    idx[p] = shift ? cj : oj;
+                    cd = shift ? od : cd;
+                    cj = shift ? oj : cj;
+                }
+                cnt += cnt < K;
+            }
         }
-        x[5 * k + 0] = f2bf(f0);
-        x[5 * k + 1] = f2bf(f1);
-        x[5 * k + 2] = f2bf(f2);
-        x[5 * k + 3] = f2bf(f3);
-        x[5 * k + 4] = f2bf(f4);
     }
-    int o = 5 * K;
-    const unsigned nb = body_count[bi];
-    x[o++] = f2bf((float)(s.body_charge[bi] / (double)(nb > 0 ? nb : 1)));
-    float cx = 0, cy = 0, cz = 0;
-    const unsigned r = s.rigid_membership[i];
-    if (r != NOT_MEMBER) {
-        const double4 c = s.rigid_com[r];
-        cx = __fmul_rn((float)mi(ri.x - c.x, Lx), 0.25f);
-        cy = __fmul_rn((float)mi(ri.y - c.y, Ly), 0.25f);
-        cz = __fmul_rn((float)mi(ri.z - c.z, Lz), 0.25f);
+    const long left = (long)s.N - b0;
+    const int nr = left < kFeatBlock ? (int)left : kFeatBlock;
+    constexpr int DW = KIN / 2, HDW = kFeatHalf / 2;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        const int lo = half * kFeatHalf;
+        if (live) {
+            const float inv = 1.0f / (float)kMlpRadius;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (5 * k + 4 < lo || 5 * k >= lo + kFeatHalf) continue;
+                float f0 = 0, f1 = 0, f2 = 0, f3 = 0, f4 = 0;
+                if (k < cnt) {
+                    const double4 rj = s.pos[idx[k]];
+                    f0 = __fmul_rn((float)mi(rj.x - ri.x, Lx), inv);
+                    f1 = __fmul_rn((float)mi(rj.y - ri.y, Ly), inv);
+                    f2 = __fmul_rn((float)mi(rj.z - ri.z, Lz), inv);
+                    f3 = (s.iso[idx[k]] == bi) ? 1.0f : -1.0f;
+                    f4 = __fmul_rn(__fsqrt_rn(dist[k]), inv);
+                }
+                put(x, lo, 5 * k + 0, f2bf(f0));
+                put(x, lo, 5 * k + 1, f2bf(f1));
+                put(x, lo, 5 * k + 2, f2bf(f2));
+                put(x, lo, 5 * k + 3, f2bf(f3));
+                put(x, lo, 5 * k + 4, f2bf(f4));
+            }
+            if (half == 1) {
+                int o = 5 * K;
+                const unsigned nb = body_count[bi];
+                put(x, lo, o++, f2bf((float)(s.body_charge[bi] / (double)(nb > 0 ? nb : 1))));
+                float cx = 0, cy = 0, cz = 0;
+                const unsigned r = s.rigid_membership[i];
+                if (r != NOT_MEMBER) {
+                    const double4 c = s.rigid_com[r];
+                    cx = __fmul_rn((float)mi(ri.x - c.x, Lx), 0.25f);
+                    cy = __fmul_rn((float)mi(ri.y - c.y, Ly), 0.25f);
+                    cz = __fmul_rn((float)mi(ri.z - c.z, Lz), 0.25f);
+                }
+                put(x, lo, o++, f2bf(cx));
+                put(x, lo, o++, f2bf(cy));
+                put(x, lo, o++, f2bf(cz));
+                put(x, lo, o++,
+                    f2bf(__fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(cx, cx), __fmul_rn(cy, cy)), __fmul_rn(cz, cz)))));
+                for (; o < kMlpGate; ++o) put(x, lo, o, 0);
+                const float e[3] = {(float)E0x, (float)E0y, (float)E0z};
+                for (int d = 0; d < 3; ++d) {
+                    const unsigned short hi = f2bf(e[d]);
+                    put(x, lo, kMlpGate + d, hi);
+                    put(x, lo, kMlpGate + 3 + d, f2bf(e[d] - bf2f(hi)));
+                }
+                for (o = kMlpGate + 6; o < KIN; ++o) put(x, lo, o, 0);
+            }
+        }
+        __syncthreads();
+        // this half of rows b0 .. b0 + nr - 1: HDW dwords of each KIN / 2-dword row
+        const unsigned* src = reinterpret_cast<const unsigned*>(rows);
+        unsigned* dst = reinterpret_cast<unsigned*>(X + b0 * KIN) + half * HDW;
+        for (int d = t; d < nr * HDW; d += kFeatBlock) {
+            const int row = d / HDW, col = d - row * HDW;
+            dst[row * DW + col] = src[row * (kFeatPitch / 2) + col];
+        }
+        __syncthreads();
     }
-    x[o++] = f2bf(cx);
-    x[o++] = f2bf(cy);
-    x[o++] = f2bf(cz);
-    x[o++] = f2bf(__fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(cx, cx), __fmul_rn(cy, cy)), __fmul_rn(cz, cz))));
-    for (; o < kMlpGate; ++o) x[o] = 0;
-    const float e[3] = {(float)E0x, (float)E0y, (float)E0z};
-    for (int d = 0; d < 3; ++d) {
-        const unsigned short hi = f2bf(e[d]);
-        x[kMlpGate + d] = hi;
-        x[kMlpGate + 3 + d] = f2bf(e[d] - bf2f(hi));
-    }
-    for (o = kMlpGate + 6; o < KIN; ++o) x[o] = 0;
 }
 
 // ---- forward ----------------------------------------------------------------
@@ -498,8 +563,10 @@ hipError_t launch_mlp_features(const SysView& s, const double L[3], const double
                                unsigned short* X, hipStream_t st)
 {
     if (s.N <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mlp_features, dim3(nblk(s.N, 256)), dim3(256), 0, st, s, L[0], L[1], L[2], E0[0], E0[1],
-                       E0[2], body_count, X);
+    static_assert(KIN % 4 == 0 && kMlpGate >= 5 * kMlpNeigh, "feature rows leave as two halves of dwords");
+    if (reinterpret_cast<uintptr_t>(X) & 3u) return hipErrorInvalidValue;  // rows leave as dword stores
+    hipLaunchKernelGGL(k_mlp_features, dim3(nblk(s.N, kFeatBlock)), dim3(kFeatBlock), 0, st, s, L[0], L[1], L[2],
+                       E0[0], E0[1], E0[2], body_count, X);
     return hipGetLastError();
 }
 
