@@ -239,102 +239,130 @@ __global__ __launch_bounds__(kFeatBlock) void k_mlp_features(SysView s, double L
 }
 
 // ---- forward ----------------------------------------------------------------
-// One wavefront per 16 rows.  TRAIN additionally stores z1, z2 (FP32) and the bf16
-// h1, h2 for the backward pass.
-template <bool TRAIN>
+// One wavefront per T tiles of 16 rows: each weight fragment is loaded once and feeds
+// the T tiles' MFMAs (every row still sees the same products in the same order).  TRAIN
+// additionally stores z1, z2 (FP32) and the bf16 h1, h2 for the backward pass.
+constexpr int kFwdTilesTrain = 2, kFwdTilesInfer = 4;
+template <bool TRAIN, int T>
 __global__ __launch_bounds__(64 * kWaves) void k_mlp_forward(int M, const unsigned short* __restrict__ X,
                                                              MlpDevWeights w, float* __restrict__ y,
                                                              float* __restrict__ z1g, float* __restrict__ z2g,
                                                              unsigned short* __restrict__ h1g,
                                                              unsigned short* __restrict__ h2g)
 {
-    __shared__ unsigned short hs[kWaves][16][H + 8];
+    __shared__ unsigned short hs[kWaves][16 * T][H + 8];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int m0 = (blockIdx.x * kWaves + wave) * 16;
+    const int m0 = (blockIdx.x * kWaves + wave) * 16 * T;
     if (m0 >= M) return;  // whole wavefront leaves together; no block barrier follows
     const int lr = lane & 15, lq = lane >> 4;
-    const int arow = min(m0 + lr, M - 1);  // rows past M repeat the last one; never stored
-    // layer 1: A = trunk columns of the rows, kept in registers; gate columns masked
-    bf16x4 a1[KT1];
+    // layer 1: A = trunk columns of the rows, kept in registers; gate columns masked.
+    // Rows past M repeat the last one and are never stored.
+    bf16x4 a1[T][KT1];
 #pragma unroll
-    for (int kt = 0; kt < KT1; ++kt) {
-        a1[kt] = load4(X + (long)arow * KIN + kt * 16 + 4 * lq);
-        if (kt * 16 + 4 * lq >= TRUNK) a1[kt] = bf16x4{0, 0, 0, 0};
+    for (int t = 0; t < T; ++t) {
+        const int arow = min(m0 + 16 * t + lr, M - 1);
+#pragma unroll
+        for (int kt = 0; kt < KT1; ++kt) {
+            a1[t][kt] = load4(X + (long)arow * KIN + kt * 16 + 4 * lq);
+            if (kt * 16 + 4 * lq >= TRUNK) a1[t][kt] = bf16x4{0, 0, 0, 0};
+        }
     }
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        bf16x4 bw[KT1];
+#pragma unroll
+        for (int kt = 0; kt < KT1; ++kt) bw[kt] = load4(w.W1t + (long)(nt * 16 + lr) * KIN + kt * 16 + 4 * lq);
+        f32x4 acc[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < KT1; ++kt)
-            acc = mfma(a1[kt], load4(w.W1t + (long)(nt * 16 + lr) * KIN + kt * 16 + 4 * lq), acc);
+#pragma unroll
+            for (int t = 0; t < T; ++t) acc[t] = mfma(a1[t][kt], bw[kt], acc[t]);
         const int n = nt * 16 + lr;
         const float b = w.b1[n];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = 4 * lq + r;
-            const float z = acc[r] + b;
-            const unsigned short h = f2bf(silu(z));
-            hs[wave][row][n] = h;
-            if (TRAIN && m0 + row < M) {
-                z1g[(long)(m0 + row) * H + n] = z;
-                h1g[(long)(m0 + row) * H + n] = h;
+        for (int t = 0; t < T; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * t + 4 * lq + r;
+                const float z = acc[t][r] + b;
+                const unsigned short h = f2bf(silu(z));
+                hs[wave][row][n] = h;
+                if (TRAIN && m0 + row < M) {
+                    z1g[(long)(m0 + row) * H + n] = z;
+                    h1g[(long)(m0 + row) * H + n] = h;
+                }
             }
-        }
     }
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
     // layer 2: A = h1 rows from LDS
-    bf16x4 a2[NT];
+    bf16x4 a2[T][NT];
 #pragma unroll
-    for (int kt = 0; kt < NT; ++kt) {
-        bf16x4 v;
+    for (int t = 0; t < T; ++t)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) v[j] = (short)hs[wave][lr][kt * 16 + 4 * lq + j];
-        a2[kt] = v;
-    }
-    float part[4][NO];
+        for (int kt = 0; kt < NT; ++kt) {
+            bf16x4 v;
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+            for (int j = 0; j < 4; ++j) v[j] = (short)hs[wave][16 * t + lr][kt * 16 + 4 * lq + j];
+            a2[t][kt] = v;
+        }
+    float part[T][4][NO];
 #pragma unroll
-        for (int o = 0; o < NO; ++o) part[r][o] = 0.f;
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int o = 0; o < NO; ++o) part[t][r][o] = 0.f;
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        bf16x4 bw[NT];
+#pragma unroll
+        for (int kt = 0; kt < NT; ++kt) bw[kt] = load4(w.W2t + (long)(nt * 16 + lr) * H + kt * 16 + 4 * lq);
+        f32x4 acc[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < NT; ++kt)
-            acc = mfma(a2[kt], load4(w.W2t + (long)(nt * 16 + lr) * H + kt * 16 + 4 * lq), acc);
+#pragma unroll
+            for (int t = 0; t < T; ++t) acc[t] = mfma(a2[t][kt], bw[kt], acc[t]);
         const int n = nt * 16 + lr;
         const float b = w.b2[n];
         float w3[NO];
 #pragma unroll
         for (int o = 0; o < NO; ++o) w3[o] = bf2f(w.W3[n * NO + o]);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = 4 * lq + r;
-            const float z = acc[r] + b;
-            const unsigned short h = f2bf(silu(z));
-            const float hf = bf2f(h);
+        for (int t = 0; t < T; ++t)
 #pragma unroll
-            for (int o = 0; o < NO; ++o) part[r][o] += hf * w3[o];
-            if (TRAIN && m0 + row < M) {
-                z2g[(long)(m0 + row) * H + n] = z;
-                h2g[(long)(m0 + row) * H + n] = h;
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * t + 4 * lq + r;
+                const float z = acc[t][r] + b;
+                const unsigned short h = f2bf(silu(z));
+                const float hf = bf2f(h);
+#pragma unroll
+                for (int o = 0; o < NO; ++o) part[t][r][o] += hf * w3[o];
+                if (TRAIN && m0 + row < M) {
+                    z2g[(long)(m0 + row) * H + n] = z;
+                    h2g[(long)(m0 + row) * H + n] = h;
+                }
             }
-        }
     }
     // layer 3: u = h2 W3 + b3 reduced over the 16 lanes holding one row's columns; y = g . u
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        float u[NO];
+    for (int t = 0; t < T; ++t)
 #pragma unroll
-        for (int o = 0; o < NO; ++o) u[o] = sum16(part[r][o]) + w.b3[o];
-        const int m = m0 + 4 * lq + r;
-        if (lr == 0 && m < M) {
-            float g[NO];
-            row_gate(X + (long)m * KIN, g);
-            y[m] = ((g[0] * u[0] + g[1] * u[1]) + g[2] * u[2]) + g[3] * u[3];
+        for (int r = 0; r < 4; ++r) {
+            float u[NO];
+#pragma unroll
+            for (int o = 0; o < NO; ++o) u[o] = sum16(part[t][r][o]) + w.b3[o];
+            const int m = m0 + 16 * t + 4 * lq + r;
+            if (lr == 0 && m < M) {
+                float g[NO];
+                row_gate(X + (long)m * KIN, g);
+                y[m] = ((g[0] * u[0] + g[1] * u[1]) + g[2] * u[2]) + g[3] * u[3];
+            }
         }
-    }
 }
 
 // ---- backward, per row ------------------------------------------------------
@@ -590,13 +618,13 @@ hipError_t launch_mlp_forward(int M, const unsigned short* X, const MlpDevWeight
                               hipStream_t st)
 {
     if (M <= 0) return hipSuccess;
-    const unsigned blocks = nblk(M, 16 * kWaves);
     if (tb)
-        hipLaunchKernelGGL(k_mlp_forward<true>, dim3(blocks), dim3(64 * kWaves), 0, st, M, X, w, y, tb->z1, tb->z2,
-                           tb->h1, tb->h2);
+        hipLaunchKernelGGL((k_mlp_forward<true, kFwdTilesTrain>), dim3(nblk(M, 16 * kFwdTilesTrain * kWaves)),
+                           dim3(64 * kWaves), 0, st, M, X, w, y, tb->z1, tb->z2, tb->h1, tb->h2);
     else
-        hipLaunchKernelGGL(k_mlp_forward<false>, dim3(blocks), dim3(64 * kWaves), 0, st, M, X, w, y, (float*)nullptr,
-                           (float*)nullptr, (unsigned short*)nullptr, (unsigned short*)nullptr);
+        hipLaunchKernelGGL((k_mlp_forward<false, kFwdTilesInfer>), dim3(nblk(M, 16 * kFwdTilesInfer * kWaves)),
+                           dim3(64 * kWaves), 0, st, M, X, w, y, (float*)nullptr, (float*)nullptr,
+                           (unsigned short*)nullptr, (unsigned short*)nullptr);
     return hipGetLastError();
 }
 
