@@ -127,6 +127,9 @@ __global__ __launch_bounds__(kBlock) void k_gs_update(const double* __restrict__
 // block's elements of V[k][i] * w_new[i] (same layout as k_multidot), so CGS2's first
 // update and second projection read the basis once.  with_norm (ncols < 32): column
 // ncols also gets the block's sum of w_new[i]^2.
+// NC >= ncols + with_norm bounds the columns held per thread, so that a short basis keeps
+// few registers live; every NC gives the same sums in the same order.
+template <int NC>
 __global__ __launch_bounds__(kBlock) void k_gs_update_dot(const double* __restrict__ V, long ld, int ncols,
                                                           const double* __restrict__ h, double* __restrict__ w,
                                                           long n, double* __restrict__ partial, int with_norm)
@@ -134,28 +137,28 @@ __global__ __launch_bounds__(kBlock) void k_gs_update_dot(const double* __restri
     __shared__ double hs[32];
     if (threadIdx.x < 32) hs[threadIdx.x] = (threadIdx.x < (unsigned)ncols) ? h[threadIdx.x] : 0.0;
     __syncthreads();
-    double acc[32];
+    double acc[NC];
 #pragma unroll
-    for (int k = 0; k < 32; ++k) acc[k] = 0.0;
+    for (int k = 0; k < NC; ++k) acc[k] = 0.0;
     for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
-        double v[32];
+        double v[NC];
         double wi = w[i];
 #pragma unroll
-        for (int k = 0; k < 32; ++k)
+        for (int k = 0; k < NC; ++k)
             if (k < ncols) {
                 v[k] = V[(long)k * ld + i];
                 wi -= hs[k] * v[k];
             }
         w[i] = wi;
 #pragma unroll
-        for (int k = 0; k < 32; ++k) {
+        for (int k = 0; k < NC; ++k) {
             if (k < ncols) acc[k] += v[k] * wi;
             else if (k == ncols && with_norm) acc[k] += wi * wi;
         }
     }
     const int nout = ncols + (with_norm ? 1 : 0);
 #pragma unroll
-    for (int c0 = 0; c0 < 32; c0 += kChunk) {
+    for (int c0 = 0; c0 < NC; c0 += kChunk) {
         if (c0 >= nout) break;
         double part[kChunk];
 #pragma unroll
@@ -315,9 +318,20 @@ hipError_t launch_gs_update(const double* V, long ld, int ncols, const double* h
 hipError_t launch_gs_update_dot(const double* V, long ld, int ncols, const double* h, double* w, long n,
                                 double* partial, hipStream_t st, bool with_norm)
 {
-    if (ncols > 32) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_gs_update_dot, dim3(vblocks(n)), dim3(kBlock), 0, st, V, ld, ncols, h, w, n, partial,
-                       with_norm ? 1 : 0);
+    const int nout = ncols + (with_norm ? 1 : 0);
+    if (ncols > 32 || nout > 32) return hipErrorInvalidValue;
+    const int wn = with_norm ? 1 : 0;
+    if (nout <= 8)
+        hipLaunchKernelGGL(k_gs_update_dot<8>, dim3(vblocks(n)), dim3(kBlock), 0, st, V, ld, ncols, h, w, n, partial, wn);
+    else if (nout <= 16)
+        hipLaunchKernelGGL(k_gs_update_dot<16>, dim3(vblocks(n)), dim3(kBlock), 0, st, V, ld, ncols, h, w, n, partial,
+                           wn);
+    else if (nout <= 24)
+        hipLaunchKernelGGL(k_gs_update_dot<24>, dim3(vblocks(n)), dim3(kBlock), 0, st, V, ld, ncols, h, w, n, partial,
+                           wn);
+    else
+        hipLaunchKernelGGL(k_gs_update_dot<32>, dim3(vblocks(n)), dim3(kBlock), 0, st, V, ld, ncols, h, w, n, partial,
+                           wn);
     return hipGetLastError();
 }
 
