@@ -291,12 +291,27 @@ int spread_only(cr_context* c, const SysView& v, const double* q, hipStream_t st
 template <typename T>
 int spread_impl(cr_context* c, const SysView& v, const double* q, hipStream_t st)
 {
+    const bool clear = !c->mesh_zeroed;
+    c->mesh_zeroed = false;
     if (c->tg.valid) {
-        CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, bin_view(c), q, c->grid_real.as<T>(), st));
+        CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, bin_view(c), q, c->grid_real.as<T>(), st, clear));
     } else {
-        CR_HIP(launch_fill<T>(c->grid_real.as<T>(), (T)0, c->G, st));
+        if (clear) CR_HIP(launch_fill<T>(c->grid_real.as<T>(), (T)0, c->G, st));
         CR_HIP(launch_spread<T>(c->mp, v, q, c->grid_real.as<T>(), st));
     }
+    return CR_OK;
+}
+
+// Zero the mesh now for the next operator application, so that the clear runs on the
+// device while the host waits for a residual.  Only gmres() calls this: its next mesh
+// operation is always a spread, and it drops the mark on every exit.
+int preclear_mesh(cr_context* c, hipStream_t st)
+{
+    if (c->fp32())
+        CR_HIP(launch_fill<float>(c->grid_real.as<float>(), 0.0f, c->G, st));
+    else
+        CR_HIP(launch_fill<double>(c->grid_real.as<double>(), 0.0, c->G, st));
+    c->mesh_zeroed = true;
     return CR_OK;
 }
 
@@ -462,6 +477,10 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
 
     int iter = 0, matvecs = 0, restarts = 0, converged = 0;
     double resid = 0;
+    struct MeshMarkReset {
+        cr_context* c;
+        ~MeshMarkReset() { c->mesh_zeroed = false; }
+    } mark_reset{c};
     for (;;) {
         if (x_zero) {
             // x0 = 0: A x0 = 0 exactly, so the first cycle's residual needs no operator
@@ -505,6 +524,8 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
             CR_HIP(launch_reduce_partials(partial, nblk, 1, &g->wnorm2, st));
             CR_HIP(launch_givens(g, i, R, st));
             CR_HIP(launch_scale_copy(w, &g->inv_norm, 1.0, V + (long)(i + 1) * c->ld, n, st));
+            // every continuation (this cycle's next step, or the restart) begins with A x
+            if (iter + 1 <= maxiter) CR_RET(preclear_mesh(c, st));
             CR_RET(read_scalar(c, &g->resid, &resid, st));
             if (!std::isfinite(resid)) return set_error(CR_ERR_NOT_CONVERGED, "non-finite residual in GMRES");
             if (resid <= tolerance) {

@@ -27,6 +27,9 @@ struct cr_context {
     DevBuf phi_recip, iso;
     // real-space operator assembled per force evaluation (CSR over the neighbour list rows)
     bool csr_ready = false;
+    // The mesh was zeroed ahead of a host wait inside one GMRES call; the next spread skips
+    // its own clear.  Set and reset only within gmres().
+    bool mesh_zeroed = false;
     long rs_ldn = 0;
     DevBuf rs_col, rs_val, rs_len;
     DevBuf qf;  // charges rounded to float for the FP32-bulk pair sum's gather
