@@ -21,6 +21,10 @@
 // Per bead, all node updates stay in LDS.
 #include "se_device.h"
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 namespace cr {
 
 namespace {
@@ -545,6 +549,21 @@ hipError_t launch_bead_records(const MeshParams& mp, const TileGeom& tg, const S
     return hipGetLastError();
 }
 
+// Raise a kernel's dynamic-LDS limit above 64 KB once per device and size.  The attribute
+// persists, and setting it costs host time on the launch path of every GMRES step.
+static void allow_lds(const void* fn, size_t bytes)
+{
+    if (bytes <= 65536) return;
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, size_t> raised;  // (device, kernel) -> bytes allowed
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lock(mu);
+    size_t& have = raised[{dev, fn}];
+    if (have >= bytes) return;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess) have = bytes;
+}
+
 template <typename T>
 hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const BinView& bv, const double* q, T* grid,
                                hipStream_t st, bool clear)
@@ -558,9 +577,7 @@ hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const B
     const T* rec_win = static_cast<const T*>(bv.rec_win);
 #define CR_SPREAD_P(PV)                                                                                        \
     do {                                                                                                       \
-        if (l.total_bytes > 65536)                                                                             \
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_tiled<T, SpreadAcc, PV>),       \
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total_bytes);         \
+        allow_lds(reinterpret_cast<const void*>(&k_spread_tiled<T, SpreadAcc, PV>), l.total_bytes);          \
         hipLaunchKernelGGL((k_spread_tiled<T, SpreadAcc, PV>), dim3(tg.ntiles), dim3(kSpreadThreads),          \
                            l.total_bytes, st, mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_win, q, \
                            grid);                                                                              \
@@ -588,12 +605,8 @@ hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const
     const T* rec_win = static_cast<const T*>(bv.rec_win);
 #define CR_CONTRACT_P(PV)                                                                                       \
     do {                                                                                                        \
-        if (l.total_bytes > 65536) {                                                                            \
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract_tiled<T, true, PV>),            \
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total_bytes);          \
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract_tiled<T, false, PV>),           \
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total_bytes);          \
-        }                                                                                                       \
+        allow_lds(reinterpret_cast<const void*>(&k_contract_tiled<T, true, PV>), l.total_bytes);              \
+        allow_lds(reinterpret_cast<const void*>(&k_contract_tiled<T, false, PV>), l.total_bytes);             \
         if (force)                                                                                              \
             hipLaunchKernelGGL((k_contract_tiled<T, true, PV>), dim3(tg.ntiles), dim3(contract_threads<T>()), l.total_bytes, st,  \
                                mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_win, s.pos, q, grid,    \
@@ -636,9 +649,7 @@ hipError_t launch_spread_tiled_batch(const TiledBox* boxes, int nbox, int total_
     if (P > 16) return hipErrorInvalidValue;
 #define CR_SPREAD_B(PV)                                                                                         \
     do {                                                                                                        \
-        if (lds_bytes > 65536)                                                                                  \
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_tiled_batch<T, SpreadAcc, PV>),  \
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);              \
+        allow_lds(reinterpret_cast<const void*>(&k_spread_tiled_batch<T, SpreadAcc, PV>), lds_bytes);         \
         hipLaunchKernelGGL((k_spread_tiled_batch<T, SpreadAcc, PV>), dim3(total_tiles), dim3(kSpreadThreads),    \
                            lds_bytes, st, boxes, nbox, total_tiles, skip);                                      \
     } while (0)
@@ -664,12 +675,8 @@ hipError_t launch_contract_tiled_batch(const TiledBox* boxes, int nbox, int tota
     if (P > 16) return hipErrorInvalidValue;
 #define CR_CONTRACT_B(PV)                                                                                       \
     do {                                                                                                        \
-        if (lds_bytes > 65536) {                                                                                \
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract_tiled_batch<T, false, PV>),    \
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);              \
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract_tiled_batch<T, true, PV>),     \
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);              \
-        }                                                                                                       \
+        allow_lds(reinterpret_cast<const void*>(&k_contract_tiled_batch<T, false, PV>), lds_bytes);           \
+        allow_lds(reinterpret_cast<const void*>(&k_contract_tiled_batch<T, true, PV>), lds_bytes);            \
         if (force)                                                                                              \
             hipLaunchKernelGGL((k_contract_tiled_batch<T, true, PV>), dim3(total_tiles), dim3(contract_threads<T>()),   \
                                lds_bytes, st, boxes, nbox, total_tiles, skip);                                  \
