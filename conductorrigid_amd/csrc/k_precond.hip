@@ -22,6 +22,16 @@ namespace cr {
 
 namespace {
 
+// Entry (i, j) of body t's inverse block.  When nmax is a multiple of 4 the block is stored
+// as nmax/4 panels of 4 columns, each panel row-major over i: the 4 entries a row reads
+// with one float4 load lie next to the next row's, so the lanes of a body (one row each)
+// read each panel as one contiguous run.  Otherwise the block is row-major.
+__device__ __forceinline__ long ginv_at(long t, int nmax, int i, int j)
+{
+    const long base = t * nmax * nmax;
+    return (nmax & 3) == 0 ? base + ((long)(j >> 2) * nmax + i) * 4 + (j & 3) : base + (long)i * nmax + j;
+}
+
 constexpr double kPi = 3.14159265358979323846;
 
 __device__ __forceinline__ double min_image(double d, double L) { return d - L * rint(d / L); }
@@ -107,10 +117,9 @@ __global__ __launch_bounds__(64) void k_block_setup(int nmax, const unsigned* __
     }
     double u = 0.0;
     if (i < n) {
-        float* row = ginv + ((long)t * nmax + i) * nmax;
         for (int j = 0; j < n; ++j) {
             const double v = aug[i * pitch + n + j];
-            row[j] = (float)v;
+            ginv[ginv_at(t, nmax, i, j)] = (float)v;
             u += v;
         }
         uvec[(long)t * nmax + i] = (float)u;
@@ -189,11 +198,10 @@ __device__ __forceinline__ void block_setup_body(int t, int nmax, const unsigned
     }
     double u = 0.0;
     if (i < n) {
-        float* row = ginv + ((long)t * nmax + i) * nmax;
 #pragma unroll
         for (int j = 0; j < NB; ++j)
             if (j < n) {
-                row[j] = (float)a[j];
+                ginv[ginv_at(t, nmax, i, j)] = (float)a[j];
                 u += a[j];
             }
         uvec[(long)t * nmax + i] = (float)u;
@@ -263,8 +271,7 @@ __device__ __forceinline__ void block_apply_body(int t, int N, int nmax, const u
     __syncthreads();
     double gy = 0.0, ui = 0.0;
     if (i < n) {
-        const float* row = ginv + ((long)t * nmax + i) * nmax;
-        for (int j = 0; j < n; ++j) gy += (double)row[j] * ys[j];
+        for (int j = 0; j < n; ++j) gy += (double)ginv[ginv_at(t, nmax, i, j)] * ys[j];
         ui = (double)uvec[(long)t * nmax + i];
     }
     const double dot = wave_sum(ui * yi);
@@ -291,9 +298,9 @@ __global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int
     // first and overlap the start -> beads -> y chain
     float4 g4[R4 > 0 ? R4 : 1];
     if constexpr (R4 > 0) {
-        const float4* r4 = reinterpret_cast<const float4*>(ginv + ((long)t * nmax + i) * nmax);
+        const float4* r4 = reinterpret_cast<const float4*>(ginv + (long)t * nmax * nmax) + i;
 #pragma unroll
-        for (int m = 0; m < R4; ++m) g4[m] = live ? r4[m] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int m = 0; m < R4; ++m) g4[m] = live ? r4[(long)m * nmax] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
     unsigned b0 = 0;
     int n = 0;
@@ -323,20 +330,19 @@ __global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int
             }
         }
     } else if (act) {
-        const float* row = ginv + ((long)t * nmax + i) * nmax;
         if ((nmax & 3) == 0) {
-            // rows are 16-byte aligned: four entries per load; entries past n are never
+            // four entries per load from the 4-column panels; entries past n are never
             // written by the setup and are masked
-            const float4* r4 = reinterpret_cast<const float4*>(row);
+            const float4* r4 = reinterpret_cast<const float4*>(ginv + (long)t * nmax * nmax) + i;
             for (int j = 0; j < n; j += 4) {
-                const float4 g = r4[j >> 2];
+                const float4 g = r4[(long)(j >> 2) * nmax];
                 gy += (double)g.x * ys[base + j];
                 if (j + 1 < n) gy += (double)g.y * ys[base + j + 1];
                 if (j + 2 < n) gy += (double)g.z * ys[base + j + 2];
                 if (j + 3 < n) gy += (double)g.w * ys[base + j + 3];
             }
         } else {
-            for (int j = 0; j < n; ++j) gy += (double)row[j] * ys[base + j];
+            for (int j = 0; j < n; ++j) gy += (double)ginv[ginv_at(t, nmax, i, j)] * ys[base + j];
         }
     }
     if (live && i == 0) {
