@@ -53,10 +53,14 @@ __global__ __launch_bounds__(kBlock) void k_multidot(const double* __restrict__ 
 #pragma unroll
     for (int k = 0; k < kChunk; ++k) acc[k] = 0.0;
     for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+        // all loads first (columns past ncols re-read the last one, cached and unused)
+        double v[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) v[k] = V[(long)min(c0 + k, ncols - 1) * ld + i];
         const double wi = w[i];
 #pragma unroll
         for (int k = 0; k < kChunk; ++k)
-            if (c0 + k < ncols) acc[k] += V[(long)(c0 + k) * ld + i] * wi;
+            if (c0 + k < ncols) acc[k] += v[k] * wi;
     }
     __shared__ double res[kChunk];
     block_reduce_chunk(acc, res);
@@ -141,14 +145,15 @@ __global__ __launch_bounds__(kBlock) void k_gs_update_dot(const double* __restri
 #pragma unroll
     for (int k = 0; k < NC; ++k) acc[k] = 0.0;
     for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+        // all loads first (columns past ncols re-read the last one, which is cached and
+        // never used), so that they are in flight together
         double v[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) v[k] = V[(long)min(k, ncols - 1) * ld + i];
         double wi = w[i];
 #pragma unroll
         for (int k = 0; k < NC; ++k)
-            if (k < ncols) {
-                v[k] = V[(long)k * ld + i];
-                wi -= hs[k] * v[k];
-            }
+            if (k < ncols) wi -= hs[k] * v[k];
         w[i] = wi;
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
@@ -289,6 +294,7 @@ int krylov_blocks(long n) { return (int)vblocks(n); }
 hipError_t launch_multidot(const double* V, long ld, int ncols, const double* w, long n, double* partial,
                            hipStream_t st)
 {
+    if (ncols <= 0 || ncols > 32) return hipErrorInvalidValue;  // the kernel re-reads column ncols - 1
     const unsigned nb = vblocks(n);
     const unsigned nchunk = (unsigned)((ncols + kChunk - 1) / kChunk);
     hipLaunchKernelGGL(k_multidot, dim3(nb, nchunk), dim3(kBlock), 0, st, V, ld, ncols, w, n, partial);
@@ -319,7 +325,7 @@ hipError_t launch_gs_update_dot(const double* V, long ld, int ncols, const doubl
                                 double* partial, hipStream_t st, bool with_norm)
 {
     const int nout = ncols + (with_norm ? 1 : 0);
-    if (ncols > 32 || nout > 32) return hipErrorInvalidValue;
+    if (ncols <= 0 || ncols > 32 || nout > 32) return hipErrorInvalidValue;  // the kernel re-reads column ncols - 1
     const int wn = with_norm ? 1 : 0;
     if (nout <= 8)
         hipLaunchKernelGGL(k_gs_update_dot<8>, dim3(vblocks(n)), dim3(kBlock), 0, st, V, ld, ncols, h, w, n, partial, wn);
