@@ -3,7 +3,7 @@
 Bar: FP64 ("plain method") stages agree with the oracle to 1e-12 relative, the
 full solve at a tight GMRES tolerance to 1e-10 relative (charges, body potentials,
 forces); neighbour lists are compared as exact sets.  The FP32-bulk mode is held
-to the north-star accuracy gate of 1e-5 relative force error against the FP64 run.
+to an accuracy gate of 1e-5 relative force error against the FP64 run.
 """
 import sys
 
