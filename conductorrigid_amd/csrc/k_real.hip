@@ -224,17 +224,25 @@ __global__ __launch_bounds__(256) void k_sum_charges(SysView s, int W, const dou
     const int k = lane / W, m = lane - k * W;
     const int r = (blockIdx.x * 4 + wave) * K + k;
     const bool live = k < K && r < s.N_rigid;
-    const unsigned size = live ? s.rigid_size[r] : 0u;
     const unsigned* mem = s.rigid_members + (long)r * s.Nmax;
+    // member slot m (< W <= Nmax, inside the body's row) is read together with the size;
+    // lane 0 also fetches the body's row index before the barrier
+    const unsigned first = live ? mem[m] : 0u;
+    const unsigned size = live ? s.rigid_size[r] : 0u;
     double acc = 0.0;
-    for (unsigned j = (unsigned)m; j < size; j += (unsigned)W) acc += q[mem[j]];
+    unsigned row = 0;
+    if ((unsigned)m < size) {
+        acc = q[first];
+        if (m == 0) row = s.iso[first];
+    }
+    for (unsigned j = (unsigned)(m + W); j < size; j += (unsigned)W) acc += q[mem[j]];
     part[threadIdx.x] = acc;
     __syncthreads();
     if (live && m == 0 && size > 0) {
         const double* p = part + wave * 64 + k * W;
         double t = 0.0;
         for (int j = 0; j < W; ++j) t += p[j];
-        yb[s.iso[mem[0]]] = precond ? t / (double)size : t;
+        yb[row] = precond ? t / (double)size : t;
     }
 }
 
