@@ -362,21 +362,24 @@ int norm2(cr_context* c, const double* w, long n, double* dst, hipStream_t st)
     return CR_OK;
 }
 
-// A device scalar to the host once the work before it on the stream is done.  The value
-// is published into coherent pinned memory by a one-thread kernel and the host polls for
-// its sequence number, which costs a few microseconds less per GMRES step than a copy and
-// a stream synchronisation.  The poll gives up to hipStreamSynchronize (and its error) if
-// the stream has drained without the value arriving.
-int read_scalar(cr_context* c, const double* dev, double* host, hipStream_t st)
+// A step's residual may be read late only while the geometric trend of the last two
+// predicts it above kLagMargin times the tolerance.
+constexpr double kLagMargin = 10.0;
+
+// Publish a device scalar into mailbox slot `slot` once the work before it on the stream is
+// done; *seq identifies this value for wait_scalar.
+int publish_scalar(cr_context* c, const double* dev, int slot, hipStream_t st, unsigned long long* seq)
 {
-    if (!c->hpub) {
-        CR_HIP(hipMemcpyAsync(host, dev, sizeof(double), hipMemcpyDeviceToHost, st));
-        CR_HIP(hipStreamSynchronize(st));
-        return CR_OK;
-    }
-    const unsigned long long seq = ++c->pub_seq;
-    CR_HIP(launch_publish(dev, c->hpub, seq, st));
-    volatile HostPub* h = c->hpub;
+    *seq = ++c->pub_seq;
+    CR_HIP(launch_publish(dev, c->hpub + slot, *seq, st));
+    return CR_OK;
+}
+
+// Poll slot `slot` until the value published with `seq` arrives.  The poll gives up to
+// hipStreamSynchronize (and its error) if the stream has drained without it.
+int wait_scalar(cr_context* c, int slot, unsigned long long seq, double* host, hipStream_t st)
+{
+    volatile HostPub* h = c->hpub + slot;
     for (unsigned long spin = 1; h->seq != seq; ++spin) {
         if ((spin & 1023) == 0) {
             const hipError_t q = hipStreamQuery(st);
@@ -390,6 +393,22 @@ int read_scalar(cr_context* c, const double* dev, double* host, hipStream_t st)
     std::atomic_thread_fence(std::memory_order_acquire);
     *host = h->value;
     return CR_OK;
+}
+
+// A device scalar to the host once the work before it on the stream is done.  The value
+// is published into coherent pinned memory by a one-thread kernel and the host polls for
+// its sequence number, which costs a few microseconds less per GMRES step than a copy and
+// a stream synchronisation.
+int read_scalar(cr_context* c, const double* dev, double* host, hipStream_t st)
+{
+    if (!c->hpub) {
+        CR_HIP(hipMemcpyAsync(host, dev, sizeof(double), hipMemcpyDeviceToHost, st));
+        CR_HIP(hipStreamSynchronize(st));
+        return CR_OK;
+    }
+    unsigned long long seq = 0;
+    CR_RET(publish_scalar(c, dev, 0, st, &seq));
+    return wait_scalar(c, 0, seq, host, st);
 }
 
 bool block_precond(const cr_context* c) { return c->params.preconditioner == CR_PRECOND_NEARFIELD_BLOCK; }
@@ -507,6 +526,14 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
         ++restarts;
         CR_HIP(launch_scale_copy(w, &g->inv_beta, -1.0, V, n, st));
         int i = -1;
+        // A step's residual is read one step late while the trend of the last two says it is
+        // still far above the tolerance, so that the host does not stall the device between
+        // steps.  If it had converged after all, the step already under way is kept (GMRES
+        // residuals do not increase).
+        bool pending = false;
+        int pend_slot = 0;
+        unsigned long long pend_seq = 0;
+        double r1 = -1.0, r2 = -1.0;  // the last two residuals read this cycle, r1 the newer
         do {
             ++i;
             ++iter;
@@ -524,10 +551,48 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
             CR_HIP(launch_reduce_partials(partial, nblk, 1, &g->wnorm2, st));
             CR_HIP(launch_givens(g, i, R, st));
             CR_HIP(launch_scale_copy(w, &g->inv_norm, 1.0, V + (long)(i + 1) * c->ld, n, st));
-            // every continuation (this cycle's next step, or the restart) begins with A x
-            if (iter + 1 <= maxiter) CR_RET(preclear_mesh(c, st));
-            CR_RET(read_scalar(c, &g->resid, &resid, st));
+            // every continuation (this cycle's next step, or the restart) begins with A x;
+            // its mesh clear is queued behind the residual's publication
+            const bool go_on = iter + 1 <= maxiter;
+            if (!c->hpub) {
+                if (go_on) CR_RET(preclear_mesh(c, st));
+                CR_RET(read_scalar(c, &g->resid, &resid, st));
+                if (!std::isfinite(resid)) return set_error(CR_ERR_NOT_CONVERGED, "non-finite residual in GMRES");
+                if (resid <= tolerance) {
+                    converged = 1;
+                    break;
+                }
+                continue;
+            }
+            const int slot = iter % kPubSlots;
+            unsigned long long seq = 0;
+            CR_RET(publish_scalar(c, &g->resid, slot, st, &seq));
+            if (go_on) CR_RET(preclear_mesh(c, st));
+            if (pending) {
+                double rp = 0;
+                CR_RET(wait_scalar(c, pend_slot, pend_seq, &rp, st));
+                pending = false;
+                if (!std::isfinite(rp)) return set_error(CR_ERR_NOT_CONVERGED, "non-finite residual in GMRES");
+                r2 = r1;
+                r1 = rp;
+                if (rp <= tolerance) {
+                    CR_RET(wait_scalar(c, slot, seq, &resid, st));
+                    if (!std::isfinite(resid)) return set_error(CR_ERR_NOT_CONVERGED, "non-finite residual in GMRES");
+                    converged = 1;
+                    break;
+                }
+            }
+            const bool more = i + 1 < R && iter + 1 <= maxiter;
+            if (more && r2 > 0.0 && r1 > 0.0 && r1 * (r1 / r2) > kLagMargin * tolerance) {
+                pending = true;
+                pend_slot = slot;
+                pend_seq = seq;
+                continue;
+            }
+            CR_RET(wait_scalar(c, slot, seq, &resid, st));
             if (!std::isfinite(resid)) return set_error(CR_ERR_NOT_CONVERGED, "non-finite residual in GMRES");
+            r2 = r1;
+            r1 = resid;
             if (resid <= tolerance) {
                 converged = 1;
                 break;
@@ -673,12 +738,14 @@ int cr_create(cr_context** out, int device)
         return set_error(CR_ERR_DEVICE, "allocation of solver state failed");
     }
     // residual mailbox; without coherent host memory the solver reads back by copy
-    if (hipHostMalloc((void**)&c->hpub, sizeof(HostPub), hipHostMallocCoherent) != hipSuccess) {
+    if (hipHostMalloc((void**)&c->hpub, kPubSlots * sizeof(HostPub), hipHostMallocCoherent) != hipSuccess) {
         c->hpub = nullptr;
         (void)hipGetLastError();
     } else {
-        c->hpub->value = 0.0;
-        c->hpub->seq = 0;
+        for (int k = 0; k < kPubSlots; ++k) {
+            c->hpub[k].value = 0.0;
+            c->hpub[k].seq = 0;
+        }
     }
     *out = c;
     return CR_OK;

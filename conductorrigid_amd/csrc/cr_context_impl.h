@@ -50,7 +50,7 @@ struct cr_context {
     DevBuf bd_force;  // Brownian step: bead forces plus steric repulsion
     GmresState* g_dev = nullptr;
     GmresState* g_host = nullptr;
-    HostPub* hpub = nullptr;            // coherent pinned: residuals the host polls for
+    HostPub* hpub = nullptr;            // coherent pinned, kPubSlots mailboxes: residuals the host polls for
     unsigned long long pub_seq = 0;
     double E0[3] = {0, 0, 0};
     unsigned long generation = 0;  // bumped whenever parameters, tables or tiling change

@@ -387,6 +387,9 @@ struct HostPub {
     double value;
     unsigned long long seq;
 };
+// Mailboxes per context: a GMRES step's residual can be read one step late, while the next
+// step's is being published.
+constexpr int kPubSlots = 2;
 hipError_t launch_publish(const double* src, HostPub* dst, unsigned long long seq, hipStream_t st);
 hipError_t launch_cycle_init(GmresState* g, const double* norm2, hipStream_t st);
 hipError_t launch_backsub(GmresState* g, int R, hipStream_t st);

@@ -266,7 +266,9 @@ __global__ void k_cycle_init(GmresState* g, const double* norm2)
     g->ncols = 0;
 }
 
-// Back substitution H(0:i,0:i) y = s(0:i) (R upper triangular after the rotations).
+// Back substitution H(0:i,0:i) y = s(0:i) (R upper triangular after the rotations).  A zero
+// pivot (a step taken after an exact breakdown: its column and right-hand side are zero)
+// contributes y = 0.
 __global__ void k_backsub(GmresState* g, int R)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -274,7 +276,8 @@ __global__ void k_backsub(GmresState* g, int R)
     const int m = g->ncols;
     for (int k = 0; k < m; ++k) g->y[k] = g->s[k];
     for (int j = m - 1; j >= 0; --j) {
-        g->y[j] /= g->H[j * ld + j];
+        const double d = g->H[j * ld + j];
+        g->y[j] = d != 0.0 ? g->y[j] / d : 0.0;
         for (int k = j - 1; k >= 0; --k) g->y[k] -= g->H[j * ld + k] * g->y[j];
     }
 }
