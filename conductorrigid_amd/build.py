@@ -12,7 +12,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libconductorrigid.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["cr_context.cpp", "cr_batch.cpp", "cr_mlp.cpp", "k_batch.hip", "k_mlp.hip", "k_dynamics.hip", "k_se.hip", "k_tiled.hip", "k_fft.hip", "k_fft_reg.hip", "k_real.hip", "k_nlist.hip", "k_krylov.hip", "k_precond.hip"]
+SOURCES = ["cr_context.cpp", "cr_batch.cpp", "cr_mlp.cpp", "k_batch.hip", "k_mlp.hip", "k_dynamics.hip", "k_se.hip", "k_tiled.hip", "k_spread_fx.hip", "k_fft.hip", "k_fft_reg.hip", "k_real.hip", "k_nlist.hip", "k_krylov.hip", "k_precond.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CR_OFFLOAD_ARCH", "gfx950")
@@ -37,7 +37,7 @@ def build(verbose: bool = True) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-z,defs", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

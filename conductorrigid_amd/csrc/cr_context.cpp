@@ -140,6 +140,8 @@ int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out, boo
     CR_HIP(launch_membership(s->N, s->tag, s->body_of_tag, c->iso.as<unsigned>(), st));
     *out = make_view(c, s);
     c->csr_ready = false;  // positions or neighbours may have changed
+    c->fx_bound = false;  // the fixed-point weight bound follows the binning
+    c->fxp = fx_plan(c->mp, c->tg, c->lds_max);
     if (bin && c->tg.valid) {
         // positions may have changed since the last call: rebin (cheap, O(N))
         const size_t nt = (size_t)c->tg.ntiles;
@@ -184,6 +186,15 @@ const AxisPlan* plans(const cr_context* c)
         return c->plan_f;
 }
 
+// The FFT's first pass reads the fixed-point mesh when the last spread left one.
+template <typename T>
+const FxScale* mesh_fx(cr_context* c)
+{
+    const FxScale* fx = (sizeof(T) == sizeof(float) && c->mesh_fixed) ? c->fxs.as<FxScale>() : nullptr;
+    c->mesh_fixed = false;
+    return fx;
+}
+
 // grid <- Re IFFT(S FFT(grid)) on the context mesh (cppmodule/ConductorRigid.cu:927-933).
 template <typename T>
 int recip_impl(cr_context* c, hipStream_t st)
@@ -192,7 +203,7 @@ int recip_impl(cr_context* c, hipStream_t st)
     const int Nx = c->mp.Nx, Ny = c->mp.Ny;
     T* real = c->grid_real.as<T>();
     T* cp = c->grid_cplx.as<T>();
-    CR_HIP(launch_fft_z_r2c<T>(pl[2], Nx, Ny, real, cp, st));
+    CR_HIP(launch_fft_z_r2c<T>(pl[2], Nx, Ny, real, cp, st, mesh_fx<T>(c)));
     CR_HIP(launch_fft_yx<T>(pl[1], 1, Nx, Ny, c->Nzh, cp, -1, st));
     CR_HIP(launch_fft_x_scale<T>(pl[0], Nx, Ny, c->Nzh, cp, c->sc, st));
     CR_HIP(launch_fft_yx<T>(pl[1], 1, Nx, Ny, c->Nzh, cp, +1, st));
@@ -277,10 +288,40 @@ int csr_build(cr_context* c, const SysView& v, unsigned width, hipStream_t st)
     return CR_OK;
 }
 
-// the spread kernel alone on a mesh the caller has cleared
+// FP32 bulk: the charges x[0:N] rounded to float (c->qf) and, for the fixed-point
+// spread, its scale from their largest magnitude and the weight bound of the current
+// binning (computed first, once per bead configuration: the scale depends on it).
+int charges_f32(cr_context* c, const SysView& v, const double* x, hipStream_t st)
+{
+    if (c->fx_on() && !c->fx_bound) {
+        CR_HIP(launch_spread_fx_bound(c->mp, c->tg, c->fxp, bin_view(c), c->fxs.as<FxScale>(), st));
+        c->fx_bound = true;
+    }
+    CR_HIP(c->qf.ensure(sizeof(float) * (size_t)std::max(1, v.N)));
+    CR_HIP(launch_charges_fx(x, v.N, c->qf.as<float>(), c->fxs.as<FxScale>(), c->mp.prefac, st));
+    return CR_OK;
+}
+
+// Fixed-point spread of c->qf at the scale charges_f32 set.
+int spread_fx(cr_context* c, hipStream_t st, bool clear)
+{
+    if (!c->fx_bound) return set_error(CR_ERR_STATE, "fixed-point spread before its scale");
+    CR_HIP(launch_spread_fx(c->mp, c->tg, c->fxp, bin_view(c), c->qf.as<float>(), c->fxs.as<FxScale>(),
+                            c->grid_real.as<int>(), st, clear));
+    c->mesh_fixed = true;
+    return CR_OK;
+}
+
+// the spread kernel alone on a mesh the caller has cleared (in FP32 bulk with the charge
+// rounding, which the fixed-point spread needs first)
 template <typename T>
 int spread_only(cr_context* c, const SysView& v, const double* q, hipStream_t st)
 {
+    if constexpr (sizeof(T) == sizeof(float)) {
+        CR_RET(charges_f32(c, v, q, st));
+        if (c->fx_on()) return spread_fx(c, st, false);
+    }
+    c->mesh_fixed = false;
     if (c->tg.valid)
         CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, bin_view(c), q, c->grid_real.as<T>(), st, false));
     else
@@ -288,11 +329,19 @@ int spread_only(cr_context* c, const SysView& v, const double* q, hipStream_t st
     return CR_OK;
 }
 
+// q_ready: FP32 bulk only, charges_f32 has already run for q
 template <typename T>
-int spread_impl(cr_context* c, const SysView& v, const double* q, hipStream_t st)
+int spread_impl(cr_context* c, const SysView& v, const double* q, hipStream_t st, bool q_ready = false)
 {
     const bool clear = !c->mesh_zeroed;
     c->mesh_zeroed = false;
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (c->fx_on()) {
+            if (!q_ready) CR_RET(charges_f32(c, v, q, st));
+            return spread_fx(c, st, clear);
+        }
+    }
+    c->mesh_fixed = false;
     if (c->tg.valid) {
         CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, bin_view(c), q, c->grid_real.as<T>(), st, clear));
     } else {
@@ -318,14 +367,14 @@ int preclear_mesh(cr_context* c, hipStream_t st)
 // Real-space rows on the assembled operator.  In FP32 bulk the gathered charges are first
 // rounded to float once per application, so the pair sum's gather moves 4-byte operands.
 template <typename T>
-int real_rows(cr_context* c, const SysView& v, const double* x, double* y, int precond, hipStream_t st)
+int real_rows(cr_context* c, const SysView& v, const double* x, double* y, int precond, hipStream_t st,
+              bool q_ready = false)
 {
     const T* qt = nullptr;
     if constexpr (sizeof(T) == sizeof(double)) {
         qt = x;
     } else {
-        CR_HIP(c->qf.ensure(sizeof(float) * (size_t)std::max(1, v.N)));
-        CR_HIP((launch_convert<double, float>(x, c->qf.as<float>(), v.N, st)));
+        if (!q_ready) CR_RET(charges_f32(c, v, x, st));
         qt = c->qf.as<float>();
     }
     CR_HIP(launch_real_space_csr<T>(c->rp, v, x, qt, c->phi_recip.as<T>(), c->rs_ldn, c->rs_col.as<unsigned>(),
@@ -338,11 +387,13 @@ int real_rows(cr_context* c, const SysView& v, const double* x, double* y, int p
 template <typename T>
 int matvec_impl(cr_context* c, const SysView& v, const double* x, double* y, int precond, hipStream_t st)
 {
-    CR_RET(spread_impl<T>(c, v, x, st));
+    constexpr bool f32 = sizeof(T) == sizeof(float);
+    if (f32) CR_RET(charges_f32(c, v, x, st));  // float charges for the spread and the pair sum
+    CR_RET(spread_impl<T>(c, v, x, st, f32));
     CR_RET(recip_impl<T>(c, st));
     CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
     if (c->csr_ready)
-        CR_RET(real_rows<T>(c, v, x, y, precond, st));
+        CR_RET(real_rows<T>(c, v, x, y, precond, st, f32));
     else
         CR_HIP(launch_real_space<T>(c->rp, v, x, c->phi_recip.as<T>(), table<T>(c, false), y, precond, st));
     CR_HIP(launch_constraint_rows(v, x, y + v.N, precond, st));
@@ -618,7 +669,7 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
 template <typename T>
 int force_pass(cr_context* c, const SysView& v, const double* q, double* force, hipStream_t st)
 {
-    CR_RET(spread_impl<T>(c, v, q, st));
+    CR_RET(spread_impl<T>(c, v, q, st));  // FP32 bulk: rounds the charges itself
     CR_RET(recip_impl<T>(c, st));
     CR_RET(contract_force_impl<T>(c, v, q, force, st));
     CR_HIP(launch_real_space_force<T>(c->rp, v, q, table<T>(c, true), c->E0, force, st));
@@ -736,6 +787,10 @@ int cr_create(cr_context** out, int device)
         hipHostMalloc((void**)&c->g_host, sizeof(GmresState), hipHostMallocDefault) != hipSuccess) {
         delete c;
         return set_error(CR_ERR_DEVICE, "allocation of solver state failed");
+    }
+    if (c->fxs.ensure(sizeof(FxScale)) != hipSuccess || hipMemset(c->fxs.p, 0, sizeof(FxScale)) != hipSuccess) {
+        delete c;
+        return set_error(CR_ERR_DEVICE, "allocation of the spread scale failed");
     }
     // residual mailbox; without coherent host memory the solver reads back by copy
     if (hipHostMalloc((void**)&c->hpub, kPubSlots * sizeof(HostPub), hipHostMallocCoherent) != hipSuccess) {
@@ -975,6 +1030,13 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
             tg.ntiles *= tg.nt[d];
         }
     }
+    {
+        int dev_max = 0;
+        CR_HIP(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+        // a little is left for the kernels' static LDS
+        c->lds_max = dev_max > 1024 ? (size_t)dev_max - 256 : 65536;
+    }
+    c->mesh_fixed = false;
     CR_HIP(c->grid_real.ensure(Tsz * (size_t)c->G));
     CR_HIP(c->grid_cplx.ensure(2 * Tsz * (size_t)c->Gh));
 
@@ -1083,7 +1145,12 @@ int cr_spread(cr_context* c, const cr_system* s, const double* q, double* grid_o
     CR_RET(prepare(c, s, st, &v));
     if (c->fp32()) {
         CR_RET(spread_impl<float>(c, v, q, st));
-        if (grid_out) CR_HIP((launch_convert<float, double>(c->grid_real.as<float>(), grid_out, c->G, st)));
+        if (grid_out) {
+            if (c->mesh_fixed)
+                CR_HIP(launch_decode_fx<double>(c->grid_real.as<int>(), grid_out, c->G, c->fxs.as<FxScale>(), st));
+            else
+                CR_HIP((launch_convert<float, double>(c->grid_real.as<float>(), grid_out, c->G, st)));
+        }
     } else {
         CR_RET(spread_impl<double>(c, v, q, st));
         if (grid_out) CR_HIP((launch_convert<double, double>(c->grid_real.as<double>(), grid_out, c->G, st)));
@@ -1096,7 +1163,10 @@ int cr_recip_grid(cr_context* c, const double* grid_in, double* grid_out, void* 
     if (!c || !c->ready) return set_error(CR_ERR_STATE, "cr_set_params has not been called");
     hipStream_t st = (hipStream_t)stream;
     if (c->fp32()) {
-        if (grid_in) CR_HIP((launch_convert<double, float>(grid_in, c->grid_real.as<float>(), c->G, st)));
+        if (grid_in) {
+            CR_HIP((launch_convert<double, float>(grid_in, c->grid_real.as<float>(), c->G, st)));
+            c->mesh_fixed = false;
+        }
         CR_RET(recip_impl<float>(c, st));
         if (grid_out) CR_HIP((launch_convert<float, double>(c->grid_real.as<float>(), grid_out, c->G, st)));
     } else {
@@ -1211,9 +1281,9 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
             T* cp = c->grid_cplx.as<T>();
             CR_HIP(launch_fill<T>(real, (T)0, c->G, st));
             CR_HIP(hipEventRecord(ev[1], st));
-            CR_RET(spread_only<T>(c, v, x, st));
+            CR_RET(spread_only<T>(c, v, x, st));  // with the charge rounding
             CR_HIP(hipEventRecord(ev[2], st));
-            CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st));
+            CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st, mesh_fx<T>(c)));
             CR_HIP(hipEventRecord(ev[3], st));
             CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, -1, st));
             CR_HIP(hipEventRecord(ev[4], st));
@@ -1225,7 +1295,7 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
             CR_HIP(hipEventRecord(ev[7], st));
             CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
             CR_HIP(hipEventRecord(ev[8], st));
-            CR_RET(real_rows<T>(c, v, x, y, 1, st));
+            CR_RET(real_rows<T>(c, v, x, y, 1, st, true));
         } else {
             using T = double;
             const AxisPlan* pl = plans<T>(c);

@@ -32,7 +32,15 @@ struct cr_context {
     bool mesh_zeroed = false;
     long rs_ldn = 0;
     DevBuf rs_col, rs_val, rs_len;
-    DevBuf qf;  // charges rounded to float for the FP32-bulk pair sum's gather
+    DevBuf qf;  // charges rounded to float for the FP32-bulk kernels
+    // FP32-bulk fixed-point spread (k_spread_fx.hip): its scale state, LDS plan, whether
+    // the weight bound matches the current binning, and whether the mesh holds int32
+    // fixed-point values (the next FFT converts them)
+    DevBuf fxs;
+    FxPlan fxp{};
+    bool fx_bound = false;
+    bool mesh_fixed = false;
+    size_t lds_max = 65536;
     // bead binning for the owner-computes spread
     TileGeom tg{};
     DevBuf bead_tile, tile_count, tile_start, tile_cursor, sorted, scan_sums;
@@ -63,6 +71,7 @@ struct cr_context {
     }
 
     bool fp32() const { return params.precision == CR_PRECISION_FP32_BULK; }
+    bool fx_on() const { return fp32() && tg.valid && fxp.valid; }
 };
 
 namespace cr::impl {

@@ -146,6 +146,41 @@ hipError_t launch_spread_tiled_batch(const TiledBox* boxes, int nbox, int total_
 template <typename T>
 hipError_t launch_contract_tiled_batch(const TiledBox* boxes, int nbox, int total_tiles, int P, size_t lds_bytes,
                                        const int* skip, hipStream_t st, bool force = false);
+// ---- k_spread_fx.hip: FP32-bulk spread in 32-bit fixed point ----
+// The mesh holds int32 node values in units of 1 / (mult / prefac); the first FFT pass
+// (or launch_decode_fx) converts them back to float with inv.
+constexpr int kFxParts = 256;   // workgroups of k_charges_fx (partial maxima)
+struct FxScale {
+    float mult;       // charge -> fixed-point units: 2^30 / (qmax W)
+    float inv;        // fixed-point units -> mesh value: prefac qmax W / 2^30 (NaN for a NaN charge)
+    float qmax;       // max |q| of the last spread's charges
+    unsigned wbits;   // float bits of the largest tile node of sum_b w_b (W = 8 x this)
+    unsigned ticket;  // last-workgroup counter of k_charges_fx, 0 between launches
+    float part[kFxParts];
+};
+constexpr int kFxThreads = 1024;
+constexpr int kFxChunk = 256;    // beads staged per round
+struct FxPlan {
+    int valid;
+    int chunk;
+    size_t lds_bytes;
+};
+// LDS plan for the context's tiling (lds_max: the device's LDS per workgroup); valid == 0
+// when the tile does not fit or P is outside 4..10.
+FxPlan fx_plan(const MeshParams& mp, const TileGeom& tg, size_t lds_max);
+// qf = (float) x [N]; max |qf| and, in the last workgroup, fs->mult and fs->inv from it and
+// fs->wbits.
+hipError_t launch_charges_fx(const double* x, int N, float* qf, FxScale* fs, double prefac, hipStream_t st);
+// grid (int32, G nodes) += fixed-point spread of qf; clear zeroes the mesh first.
+hipError_t launch_spread_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
+                            const float* qf, FxScale* fs, int* grid, hipStream_t st, bool clear);
+// fs->wbits for the binned configuration (once per bead configuration).
+hipError_t launch_spread_fx_bound(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
+                                  FxScale* fs, hipStream_t st);
+// out = in * fs->inv (in and out may alias).
+template <typename T>
+hipError_t launch_decode_fx(const int* in, T* out, long n, const FxScale* fs, hipStream_t st);
+
 // ---- k_se.hip (unbinned fallbacks) ----
 template <typename T>
 hipError_t launch_contract(const MeshParams& mp, const SysView& s, const T* grid, T* phi, hipStream_t st);
@@ -159,8 +194,10 @@ hipError_t launch_convert(const Tin* in, Tout* out, long n, hipStream_t st);
 
 // ---- k_fft.hip ----
 // Real mesh [Nx][Ny][Nz] <-> half spectrum [Nx][Ny][Nz/2+1] (interleaved complex).
+// fx != null: real holds the int32 fixed-point mesh of launch_spread_fx (float meshes only).
 template <typename T>
-hipError_t launch_fft_z_r2c(const AxisPlan& pz, int Nx, int Ny, const T* real, T* cplx, hipStream_t st);
+hipError_t launch_fft_z_r2c(const AxisPlan& pz, int Nx, int Ny, const T* real, T* cplx, hipStream_t st,
+                           const FxScale* fx = nullptr);
 template <typename T>
 hipError_t launch_fft_z_c2r(const AxisPlan& pz, int Nx, int Ny, const T* cplx, T* real, hipStream_t st);
 // C2C along y (axis 1) or x (axis 0) of a complex mesh [Nx][Ny][Zc]; sign -1 / +1.
@@ -188,7 +225,8 @@ hipError_t launch_fft_x_scale(const AxisPlan& px, int Nx, int Ny, int Zc, T* cpl
 // ---- k_fft_reg.hip: register four-step passes for n = R * R, R in {8, 16, 32} ----
 int fftr_radix(int n);  // n when the register-resident passes take length n, else 0
 template <typename T>
-hipError_t launch_fftr_z_r2c(const AxisPlan& pz, long nrows, const T* real, T* cplx, hipStream_t st);
+hipError_t launch_fftr_z_r2c(const AxisPlan& pz, long nrows, const T* real, T* cplx, hipStream_t st,
+                            const FxScale* fx = nullptr);
 template <typename T>
 hipError_t launch_fftr_z_c2r(const AxisPlan& pz, long nrows, const T* cplx, T* real, hipStream_t st);
 template <typename T>

@@ -250,6 +250,7 @@ template hipError_t launch_contract_force<double>(const MeshParams&, const SysVi
 template hipError_t launch_fill<float>(float*, float, long, hipStream_t);
 template hipError_t launch_fill<double>(double*, double, long, hipStream_t);
 template hipError_t launch_fill<unsigned>(unsigned*, unsigned, long, hipStream_t);
+template hipError_t launch_fill<int>(int*, int, long, hipStream_t);
 template hipError_t launch_convert<double, float>(const double*, float*, long, hipStream_t);
 template hipError_t launch_convert<float, double>(const float*, double*, long, hipStream_t);
 template hipError_t launch_convert<double, double>(const double*, double*, long, hipStream_t);

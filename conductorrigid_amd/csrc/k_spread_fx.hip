@@ -1,0 +1,404 @@
+// k_spread_fx.hip -- FP32-bulk spreading (cppmodule/ConductorRigid.cu:144-234) in 32-bit
+// fixed point.
+//
+// The reference adds every window product into the mesh with a float atomic.  Here, as in
+// the binned kernels of k_tiled.hip, beads are binned by the mesh tile holding their
+// window's first node and one workgroup accumulates its tile, extended by P-1 nodes per
+// axis, in LDS.  The accumulator is a 32-bit integer: each product c = S q w_x w_y w_z is
+// rounded to the nearest integer and added with ds_add_u32.  On gfx950 an LDS float atomic
+// add runs an order of magnitude slower than the integer one, and a double accumulator
+// takes twice the LDS; the integer tile is half that size, so two workgroups fit a CU.  The
+// extended tile is then added into a mesh of int32 with global integer atomics, and the
+// FFT's first pass converts the mesh back to float (k_fft_reg.hip, k_fft.hip).
+//
+// Integer sums do not depend on the order of the additions, so the spread is
+// deterministic: every run gives the same mesh bits.
+//
+// Scale.  A mesh node's value is prefac * sum_b q_b w_b(node).  With qmax = max |q_b| and
+// W a bound on sum_b w_b(node) over all nodes, |node| <= prefac qmax W, so the multiplier
+// m = 2^30 / (qmax W) applied to q w keeps every partial and total sum below 2^30 in
+// magnitude (the rounding adds at most 1/2 per product), and int32 cannot overflow.  W is
+// computed once per bead configuration (k_spread_fx<..., ONES = true>): every tile
+// accumulates sum_b w_b for its own beads and reports its largest node; a node is covered
+// by at most 8 tiles (two per axis), so W = 8 x the largest tile node.  qmax is computed for
+// every spread by k_charges_fx, which also rounds the charges to float for the FP32-bulk
+// kernels and, in its last workgroup, writes m and the decode factor prefac qmax W / 2^30.
+#include "cr_kernels.h"
+
+#include <map>
+#include <mutex>
+#include <utility>
+
+namespace cr {
+
+namespace {
+
+constexpr int kFxWave = 64;
+constexpr int kFxChargeThreads = 256;
+
+__device__ __forceinline__ int round_fx(float x)
+{
+    // floor(x + 0.5) in one instruction
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// Workgroups are dealt round-robin to the 8 XCDs: give each XCD a contiguous run of tiles.
+__device__ __forceinline__ int fx_tile(int b, int ntiles)
+{
+    if (ntiles % 8 != 0) return b;
+    return (b % 8) * (ntiles / 8) + b / 8;
+}
+
+// NaN-propagating maximum of |values| (a NaN charge must reach the mesh as NaN)
+__device__ __forceinline__ float nanmax(float m, float a) { return (a > m || a != a) ? a : m; }
+
+__device__ __forceinline__ float wave_nanmax(float m)
+{
+#pragma unroll
+    for (int off = kFxWave / 2; off > 0; off >>= 1) m = nanmax(m, __shfl_xor(m, off, kFxWave));
+    return m;
+}
+
+__global__ __launch_bounds__(kFxChargeThreads) void k_charges_fx(const double* __restrict__ x, int N,
+                                                                 float* __restrict__ qf, FxScale* __restrict__ fs,
+                                                                 double prefac)
+{
+    __shared__ float wmax[kFxChargeThreads / kFxWave];
+    __shared__ int last;
+    float m = 0.0f;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const float v = (float)x[i];
+        qf[i] = v;
+        m = nanmax(m, fabsf(v));
+    }
+    m = wave_nanmax(m);
+    const int lane = threadIdx.x & (kFxWave - 1), wave = threadIdx.x / kFxWave;
+    if (lane == 0) wmax[wave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float bm = wmax[0];
+        for (int k = 1; k < kFxChargeThreads / kFxWave; ++k) bm = nanmax(bm, wmax[k]);
+        fs->part[blockIdx.x] = bm;
+        // release this block's partial, acquire the others' if this is the last block
+        const unsigned ticket = __hip_atomic_fetch_add(&fs->ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = ticket == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last || wave != 0) return;
+    __threadfence();
+    float qm = 0.0f;
+    for (int k = lane; k < (int)gridDim.x; k += kFxWave) qm = nanmax(qm, fs->part[k]);
+    qm = wave_nanmax(qm);
+    if (lane == 0) {
+        const double W = 8.0 * (double)__uint_as_float(fs->wbits);
+        const double qW = (double)qm * W;
+        float mult, inv;
+        if (qm != qm || !(qm <= 3.0e38f)) {  // NaN or infinite charge: decode to NaN
+            mult = 0.0f;
+            inv = __int_as_float(0x7fc00000);
+        } else if (!(qW > 1e-30)) {          // all charges (or all weights) zero
+            mult = 0.0f;
+            inv = 0.0f;
+        } else {
+            mult = (float)(1073741824.0 / qW);
+            inv = (float)(prefac * qW / 1073741824.0);
+        }
+        fs->mult = mult;
+        fs->inv = inv;
+        fs->qmax = qm;
+        fs->ticket = 0u;
+    }
+}
+
+// Node walk over an EX x EY x EZ box in row-major order with carries (no divisions).
+struct FxWalk {
+    int lx, ly, lz, dx, dy, dz, EY, EZ;
+    __device__ __forceinline__ FxWalk(int tid, int stride, int EY_, int EZ_) : EY(EY_), EZ(EZ_)
+    {
+        lz = tid % EZ;
+        const int r = tid / EZ;
+        ly = r % EY;
+        lx = r / EY;
+        dz = stride % EZ;
+        const int q = stride / EZ;
+        dy = q % EY;
+        dx = q / EY;
+    }
+    __device__ __forceinline__ void next()
+    {
+        lz += dz;
+        int c = lz >= EZ;
+        lz -= c ? EZ : 0;
+        ly += dy + c;
+        c = ly >= EY;
+        ly -= c ? EY : 0;
+        lx += dx + c;
+    }
+};
+
+// Per-bead staging stride (floats): the 3 P axis weights, padded to whole float4s.
+__host__ __device__ constexpr int fx_wstride(int P) { return (3 * P + 3) / 4 * 4; }
+
+// TX = 0: tile edges from tg at run time; otherwise compile-time edges (the LDS strides
+// become instruction offsets).  ONES: accumulate sum_b w_b with a per-tile multiplier and
+// report the tile's largest node into fs->wbits instead of writing the mesh.
+template <int PT, int TX, int TY, int TZ, bool ONES>
+__global__ __launch_bounds__(kFxThreads) void k_spread_fx(MeshParams mp, TileGeom tg, int ch,
+                                                          const unsigned* __restrict__ tstart,
+                                                          const unsigned* __restrict__ sorted,
+                                                          const int* __restrict__ rec_off,
+                                                          const float4* __restrict__ rec_u,
+                                                          const float* __restrict__ qf, FxScale* __restrict__ fs,
+                                                          int* __restrict__ grid)
+{
+    constexpr bool CT = TX > 0;
+    constexpr int WS = fx_wstride(PT);
+    const int T0 = CT ? TX : tg.T[0], T1 = CT ? TY : tg.T[1], T2 = CT ? TZ : tg.T[2];
+    const int EX = T0 + PT - 1, EY = T1 + PT - 1, EZ = T2 + PT - 1;
+    const int ys = (EZ + 1) & ~1, xs = EY * ys, esz = EX * xs;
+    extern __shared__ __align__(16) char smem[];
+    __shared__ unsigned tile_max;
+    const int t = fx_tile(blockIdx.x, tg.ntiles);
+    const unsigned kb = tstart[t], ke = tstart[t + 1];
+    if (kb == ke) return;  // an empty bin adds nothing to the (cleared) mesh
+    const int tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
+    const int x0 = tx * T0, y0 = ty * T1, z0 = tz * T2;
+    unsigned* ext = reinterpret_cast<unsigned*>(smem);
+    float* w = reinterpret_cast<float*>(smem + ((size_t)esz * 4 + 15) / 16 * 16);
+    int2* oq = reinterpret_cast<int2*>(w + (size_t)ch * WS);
+    for (int k = threadIdx.x; k < esz / 4; k += kFxThreads) reinterpret_cast<uint4*>(ext)[k] = make_uint4(0, 0, 0, 0);
+    for (int k = (esz / 4) * 4 + threadIdx.x; k < esz; k += kFxThreads) ext[k] = 0u;
+    if (threadIdx.x == 0) tile_max = 0u;
+    __syncthreads();
+    // ONES: per-tile multiplier 2^30 / count keeps the tile's sums of weights <= 1 below 2^30
+    const float mult = ONES ? (float)(1073741824.0 / (double)(ke - kb)) : fs->mult;
+    const float c0 = (float)(mp.xiterm * mp.h[0] * mp.h[0] / mp.eta[0]);
+    const float c1 = (float)(mp.xiterm * mp.h[1] * mp.h[1] / mp.eta[1]);
+    const float c2 = (float)(mp.xiterm * mp.h[2] * mp.h[2] / mp.eta[2]);
+    const int lane = threadIdx.x & (kFxWave - 1), wave = threadIdx.x / kFxWave;
+    constexpr int NW = kFxThreads / kFxWave;
+    for (unsigned k0 = kb; k0 < ke; k0 += ch) {
+        const int cnt = (int)min((unsigned)ch, ke - k0);
+        // three threads per bead: the P weights exp(-c_d (p - u_d)^2) of one axis
+        for (int j = threadIdx.x; j < 3 * cnt; j += kFxThreads) {
+            const int i = j / 3, d = j - 3 * i;
+            const float4 u4 = rec_u[k0 + i];
+            const float ud = d == 0 ? u4.x : (d == 1 ? u4.y : u4.z);
+            const float cd = d == 0 ? c0 : (d == 1 ? c1 : c2);
+            float* wp = w + i * WS + d * PT;
+#pragma unroll
+            for (int p = 0; p < PT; ++p) {
+                const float dd = (float)p - ud;
+                wp[p] = __expf(-cd * dd * dd);
+            }
+        }
+        for (int i = threadIdx.x; i < cnt; i += kFxThreads) {
+            const float qs = ONES ? mult : mult * qf[sorted[k0 + i]];
+            oq[i] = make_int2(rec_off[k0 + i], __float_as_int(qs));
+        }
+        __syncthreads();
+        // one wavefront per bead: lanes over the (y, z) window plane, each walking x
+        for (int i = wave; i < cnt; i += NW) {
+            const int2 r = oq[i];
+            const int ox = r.x & 1023, oy = (r.x >> 10) & 1023, oz = r.x >> 20;
+            const float qs = __int_as_float(r.y);
+            const float* wi = w + i * WS;
+            float wx[PT];
+            if constexpr (PT % 4 == 0) {
+#pragma unroll
+                for (int p = 0; p < PT; p += 4) {
+                    const float4 v4 = *reinterpret_cast<const float4*>(wi + p);
+                    wx[p] = v4.x;
+                    wx[p + 1] = v4.y;
+                    wx[p + 2] = v4.z;
+                    wx[p + 3] = v4.w;
+                }
+            } else {
+#pragma unroll
+                for (int p = 0; p < PT; ++p) wx[p] = wi[p];
+            }
+#pragma unroll
+            for (int pair = lane; pair < PT * PT; pair += kFxWave) {
+                const int py = pair / PT, pz = pair - py * PT;
+                const float c = qs * wi[PT + py] * wi[2 * PT + pz];
+                unsigned* col = ext + ox * xs + (oy + py) * ys + (oz + pz);
+#pragma unroll
+                for (int px = 0; px < PT; ++px) atomicAdd(col + px * xs, (unsigned)round_fx(c * wx[px]));
+            }
+        }
+        __syncthreads();
+    }
+    if constexpr (ONES) {
+        // sums of positive weights: the unsigned maximum is the largest node
+        unsigned m = 0u;
+        for (int k = threadIdx.x; k < esz; k += kFxThreads) m = max(m, ext[k]);
+#pragma unroll
+        for (int off = kFxWave / 2; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, kFxWave));
+        if (lane == 0) atomicMax(&tile_max, m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // each node sums at most (ke - kb) rounded products, each off by <= 1/2
+            const double bound = ((double)tile_max + 0.5 * (double)(ke - kb)) / (double)mult * (1.0 + 1e-5);
+            atomicMax(&fs->wbits, __float_as_uint((float)bound));
+        }
+        return;
+    }
+    const long plane = (long)mp.Ny * mp.Nz;
+    const int nodes = EX * EY * EZ;
+    FxWalk bw(threadIdx.x, kFxThreads, EY, EZ);
+    for (int k = threadIdx.x; k < nodes; k += kFxThreads, bw.next()) {
+        // nodes no window of the bin reached are exactly zero: adding them changes nothing
+        const int v = (int)ext[bw.lx * xs + bw.ly * ys + bw.lz];
+        if (v == 0) continue;
+        int gx = x0 + bw.lx, gy = y0 + bw.ly, gz = z0 + bw.lz;
+        gx -= gx >= mp.Nx ? mp.Nx : 0;
+        gy -= gy >= mp.Ny ? mp.Ny : 0;
+        gz -= gz >= mp.Nz ? mp.Nz : 0;
+        atomicAdd(&grid[(long)gx * plane + (long)gy * mp.Nz + gz], v);
+    }
+}
+
+template <typename T>
+__global__ void k_decode_fx(const int* in, T* out, long n, const FxScale* __restrict__ fs)
+{
+    // in and out may alias (in-place conversion of the mesh)
+    const float inv = fs->inv;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int v = in[i];
+        out[i] = (T)((float)v * inv);
+    }
+}
+
+void allow_lds_fx(const void* fn, size_t bytes)
+{
+    if (bytes <= 65536) return;
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, size_t> raised;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lock(mu);
+    size_t& have = raised[{dev, fn}];
+    if (have >= bytes) return;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess) have = bytes;
+}
+
+template <int PT, int TX, int TY, int TZ, bool ONES>
+hipError_t launch_fx_kernel(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
+                            const float* qf, FxScale* fs, int* grid, hipStream_t st)
+{
+    auto fn = k_spread_fx<PT, TX, TY, TZ, ONES>;
+    allow_lds_fx(reinterpret_cast<const void*>(fn), pl.lds_bytes);
+    hipLaunchKernelGGL(fn, dim3(tg.ntiles), dim3(kFxThreads), pl.lds_bytes, st, mp, tg, pl.chunk, bv.tile_start,
+                       bv.sorted, bv.rec_off, static_cast<const float4*>(bv.rec_win), qf, fs, grid);
+    return hipGetLastError();
+}
+
+template <int PT, bool ONES>
+hipError_t dispatch_shape(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
+                          const float* qf, FxScale* fs, int* grid, hipStream_t st)
+{
+    const int a = tg.T[0], b = tg.T[1], c = tg.T[2];
+#define CR_FX_SHAPE(X, Y, Z) \
+    if (a == X && b == Y && c == Z) return launch_fx_kernel<PT, X, Y, Z, ONES>(mp, tg, pl, bv, qf, fs, grid, st)
+    if constexpr (PT == 8) {
+        CR_FX_SHAPE(16, 16, 16);
+        CR_FX_SHAPE(16, 8, 16);
+        CR_FX_SHAPE(16, 16, 32);
+        CR_FX_SHAPE(32, 16, 16);
+        CR_FX_SHAPE(16, 32, 32);
+        CR_FX_SHAPE(8, 8, 8);
+    } else if constexpr (PT == 9) {
+        CR_FX_SHAPE(16, 16, 16);
+        CR_FX_SHAPE(16, 8, 16);
+        CR_FX_SHAPE(8, 8, 8);
+    }
+#undef CR_FX_SHAPE
+    return launch_fx_kernel<PT, 0, 0, 0, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
+}
+
+template <bool ONES>
+hipError_t dispatch_p(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv, const float* qf,
+                      FxScale* fs, int* grid, hipStream_t st)
+{
+    switch (mp.P) {
+        case 4: return dispatch_shape<4, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
+        case 5: return dispatch_shape<5, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
+        case 6: return dispatch_shape<6, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
+        case 7: return dispatch_shape<7, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
+        case 8: return dispatch_shape<8, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
+        case 9: return dispatch_shape<9, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
+        case 10: return dispatch_shape<10, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+inline unsigned nblk(long n, int b)
+{
+    const long v = (n + b - 1) / b;
+    return (unsigned)(v > 0 ? v : 1);
+}
+
+}  // namespace
+
+FxPlan fx_plan(const MeshParams& mp, const TileGeom& tg, size_t lds_max)
+{
+    FxPlan p{};
+    if (!tg.valid || mp.P < 4 || mp.P > 10) return p;
+    const size_t EX = (size_t)tg.T[0] + mp.P - 1, EY = (size_t)tg.T[1] + mp.P - 1, EZ = (size_t)tg.T[2] + mp.P - 1;
+    const size_t ext = (EX * EY * ((EZ + 1) & ~(size_t)1) * 4 + 15) / 16 * 16;
+    const size_t per = (size_t)fx_wstride(mp.P) * 4 + 8;
+    // two workgroups per CU when the tile leaves room for >= 64 staged beads in half the LDS
+    for (const size_t budget : {lds_max / 2, lds_max}) {
+        if (ext >= budget) continue;
+        long ch = (long)((budget - ext) / per);
+        if (ch > kFxChunk) ch = kFxChunk;
+        ch -= ch % 4;
+        if (ch < (budget == lds_max ? 16 : 64)) continue;
+        p.chunk = (int)ch;
+        p.lds_bytes = ext + (size_t)ch * per;
+        p.valid = 1;
+        return p;
+    }
+    return p;
+}
+
+hipError_t launch_charges_fx(const double* x, int N, float* qf, FxScale* fs, double prefac, hipStream_t st)
+{
+    const unsigned blocks = std::min<unsigned>(kFxParts, nblk(std::max(N, 1), kFxChargeThreads));
+    hipLaunchKernelGGL(k_charges_fx, dim3(blocks), dim3(kFxChargeThreads), 0, st, x, N, qf, fs, prefac);
+    return hipGetLastError();
+}
+
+hipError_t launch_spread_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
+                            const float* qf, FxScale* fs, int* grid, hipStream_t st, bool clear)
+{
+    if (!pl.valid) return hipErrorInvalidValue;
+    if (clear) {
+        hipError_t e = launch_fill<int>(grid, 0, (long)mp.Nx * mp.Ny * mp.Nz, st);
+        if (e != hipSuccess) return e;
+    }
+    return dispatch_p<false>(mp, tg, pl, bv, qf, fs, grid, st);
+}
+
+hipError_t launch_spread_fx_bound(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
+                                  FxScale* fs, hipStream_t st)
+{
+    if (!pl.valid) return hipErrorInvalidValue;
+    hipError_t e = launch_fill<unsigned>(&fs->wbits, 0u, 1, st);
+    if (e != hipSuccess) return e;
+    return dispatch_p<true>(mp, tg, pl, bv, nullptr, fs, nullptr, st);
+}
+
+template <typename T>
+hipError_t launch_decode_fx(const int* in, T* out, long n, const FxScale* fs, hipStream_t st)
+{
+    const unsigned blocks = std::min<unsigned>(4096, nblk(n, 256));
+    hipLaunchKernelGGL(k_decode_fx<T>, dim3(blocks), dim3(256), 0, st, in, out, n, fs);
+    return hipGetLastError();
+}
+template hipError_t launch_decode_fx<float>(const int*, float*, long, const FxScale*, hipStream_t);
+template hipError_t launch_decode_fx<double>(const int*, double*, long, const FxScale*, hipStream_t);
+
+}  // namespace cr
