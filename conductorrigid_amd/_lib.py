@@ -56,7 +56,11 @@ class CrSystem(C.Structure):
     _fields_ = [("N", C.c_int), ("pos", vp), ("tag", vp), ("body_of_tag", vp), ("body_charge", vp),
                 ("N_rigid", C.c_int), ("N_isolated", C.c_int), ("Nmax", C.c_int), ("rigid_membership", vp),
                 ("rigid_members", vp), ("rigid_size", vp), ("rigid_com", vp), ("nlist", vp), ("head_list", vp),
-                ("n_neigh", vp)]
+                ("n_neigh", vp), ("L", C.c_double * 3)]
+
+
+class CrSystemF32(C.Structure):
+    _fields_ = CrSystem._fields_
 
 
 class CrTuning(C.Structure):
@@ -67,7 +71,7 @@ class CrTuning(C.Structure):
 class CrSolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("matvecs", C.c_int), ("restarts", C.c_int), ("converged", C.c_int),
                 ("resid", C.c_double), ("tolerance", C.c_double), ("true_resid", C.c_double),
-                ("true_relres", C.c_double)]
+                ("true_relres", C.c_double), ("fallback", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -81,6 +85,9 @@ _SIGS = {
     "cr_set_params": (C.c_int, [vp, C.POINTER(CrParams), C.c_int, C.c_int]),
     "cr_get_resolved": (C.c_int, [vp, C.POINTER(CrResolved)]),
     "cr_update_field": (C.c_int, [vp, C.POINTER(C.c_double)]),
+    "cr_set_box": (C.c_int, [vp, C.POINTER(C.c_double), vp]),
+    "cr_compute_force_f32": (C.c_int, [vp, C.POINTER(CrSystemF32), vp, vp, vp, C.c_int, C.POINTER(CrSolveStats),
+                                       vp]),
     "cr_zero_force": (C.c_int, [vp, C.c_int, vp, vp]),
     "cr_compute_force": (C.c_int, [vp, C.POINTER(CrSystem), vp, vp, vp, C.c_int, C.POINTER(CrSolveStats), vp]),
     "cr_matvec": (C.c_int, [vp, C.POINTER(CrSystem), vp, vp, vp]),

@@ -104,7 +104,8 @@ class DeviceSystem:
                              self.body_charge.data_ptr(), s.N_rigid, s.N_isolated, max(1, s.Nmax),
                              self.rigid_membership.data_ptr(), self.rigid_members.data_ptr(),
                              self.rigid_size.data_ptr(), self.rigid_com.data_ptr(), self.nlist.data_ptr(),
-                             self.head_list.data_ptr(), self.n_neigh.data_ptr())
+                             self.head_list.data_ptr(), self.n_neigh.data_ptr(),
+                             (C.c_double * 3)(*[float(x) for x in s.L]))
 
     def nlist_host(self):
         return (self.head_list.cpu().numpy().view(np.uint32), self.n_neigh.cpu().numpy().view(np.uint32),
@@ -200,9 +201,10 @@ class ConductorRigid:
     def computeForces(self, timestep: int = 0, x0: str | None = None, stats: bool = True) -> dict | None:
         mode = self.x0_mode if x0 is None else _X0_MODES[x0]
         if tuple(float(x) for x in self.system.L) != self._box:
-            # the box changed (e.g. NPT): mesh spacing, Gaussian widths, k-scale and the
-            # neighbour list all depend on it (SetParams, cppmodule/ConductorRigid.cc:116-379)
-            self.SetParams()
+            # the box changed (e.g. NPT): the neighbour list is rebuilt here (HOOMD's side);
+            # the library takes up the new box itself from the view's box lengths
+            # (mesh spacing, Gaussian widths, k-scale: cr_set_box)
+            self._box = tuple(float(x) for x in self.system.L)
             self.dsys.build_nlist()
         st = _lib.CrSolveStats()
         sv = self.dsys.view()

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -134,6 +135,11 @@ int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out, boo
     if (s->N_rigid > 0 && (!s->rigid_members || !s->rigid_size || !s->rigid_com || s->Nmax <= 0))
         return set_error(CR_ERR_INVALID, "missing rigid-body array");
     if (reinterpret_cast<uintptr_t>(s->pos) % 16 != 0) return set_error(CR_ERR_INVALID, "pos must be 16-byte aligned");
+    if (s->L[0] != 0.0 || s->L[1] != 0.0 || s->L[2] != 0.0) {
+        // the caller's box (HOOMD's BoxDim of this step): take up a change first
+        if (s->L[0] != c->params.L[0] || s->L[1] != c->params.L[1] || s->L[2] != c->params.L[2])
+            CR_RET(set_box(c, s->L, st));
+    }
     const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
     CR_HIP(c->phi_recip.ensure(Tsz * (size_t)std::max(1, s->N)));
     CR_HIP(c->iso.ensure(sizeof(unsigned) * (size_t)std::max(1, s->N)));
@@ -510,6 +516,75 @@ int matvec_prec(cr_context* c, const SysView& v, const double* x, double* out, h
     return matvec_any(c, v, x, out, 1, st);
 }
 
+// Box change: the mesh, window support, cutoff and real-space tables stay (the tables
+// depend on xi only); the spacing h, eta, the window prefactor, the quadrature weight and
+// the k-space scale tables follow the box (SetParams, cppmodule/ConductorRigid.cc:199-268),
+// the tables regenerated on the device.  A derived mesh that the new box would change
+// takes the full SetParams path.
+int set_box(cr_context* c, const double L[3], hipStream_t st)
+{
+    if (!c || !c->ready) return set_error(CR_ERR_STATE, "cr_set_params has not been called");
+    for (int d = 0; d < 3; ++d)
+        if (!(L[d] > 0) || !std::isfinite(L[d])) return set_error(CR_ERR_INVALID, "box lengths must be positive");
+    cr_params& p = c->params;
+    cr_resolved& r = c->res;
+    const std::vector<int> sm = smooth_list();
+    for (int d = 0; d < 3; ++d) {
+        if (p.mesh[d] > 0) continue;
+        int m = (int)std::ceil(1.0 + L[d] * r.kcut / kPi);
+        for (int v : sm)
+            if (m <= v) {
+                m = v;
+                break;
+            }
+        if (m != r.mesh[d]) {
+            cr_params np = p;
+            for (int e = 0; e < 3; ++e) np.L[e] = L[e];
+            CR_HIP(hipStreamSynchronize(st));
+            return cr_set_params(c, &np, c->n_cap, 0);
+        }
+    }
+    for (int d = 0; d < 3; ++d)
+        if (!(r.rc < 0.5 * L[d])) return set_error(CR_ERR_INVALID, "real-space cutoff must stay below half the box");
+    const double xi = p.xi;
+    for (int d = 0; d < 3; ++d) {
+        p.L[d] = L[d];
+        r.h[d] = L[d] / r.mesh[d];
+        r.eta[d] = c->P_eta * r.h[d] * r.h[d] * xi * xi / kPi;
+    }
+    r.prefac = r.xiterm * xi / kPi * std::sqrt(2.0 / (kPi * r.eta[0] * r.eta[1] * r.eta[2]));
+    r.quadW = r.h[0] * r.h[1] * r.h[2];
+    MeshParams& mp = c->mp;
+    for (int d = 0; d < 3; ++d) {
+        mp.L[d] = L[d];
+        mp.h[d] = r.h[d];
+        mp.eta[d] = r.eta[d];
+        c->rp.L[d] = L[d];
+    }
+    mp.prefac = r.prefac;
+    mp.quadW = r.quadW;
+    const int Nd[3] = {mp.Nx, mp.Ny, mp.Nz};
+    for (int d = 0; d < 3; ++d)
+        CR_HIP(launch_kspace_axis(Nd[d], L[d], xi, r.eta[d], c->ktab[d].as<double>(), c->etab[d].as<double>(), st));
+    c->sc.radial = nullptr;
+    if (L[0] == L[1] && L[1] == L[2]) {
+        long m2max = 0;
+        for (int d = 0; d < 3; ++d) m2max += (long)(Nd[d] / 2) * (Nd[d] / 2);
+        const double kappa = 2.0 * kPi / L[0];
+        if (c->fp32()) {
+            CR_HIP(c->radtab.ensure(sizeof(float) * (size_t)(m2max + 1)));
+            CR_HIP(launch_kspace_radial<float>(m2max, kappa, c->sc.invG, c->radtab.as<float>(), st));
+        } else {
+            CR_HIP(c->radtab.ensure(sizeof(double) * (size_t)(m2max + 1)));
+            CR_HIP(launch_kspace_radial<double>(m2max, kappa, c->sc.invG, c->radtab.as<double>(), st));
+        }
+        c->sc.radial = c->radtab.p;
+    }
+    c->fx_bound = false;
+    ++c->generation;
+    return CR_OK;
+}
+
 // Per-body bead counts and a double scratch of one entry per body.
 int segments(cr_context* c, const SysView& v, hipStream_t st, unsigned** counts, double** sums)
 {
@@ -525,6 +600,10 @@ int segments(cr_context* c, const SysView& v, hipStream_t st, unsigned** counts,
 // Restarted, left-preconditioned GMRES(R) following CUSP's krylov::gmres structure
 // (cppmodule/ConductorRigid.cu:1134-1136): the cycle starts from w = D(Ax - b), stops
 // when the Givens estimate |s_{i+1}| <= tol ||b||_2, x += V y at the end of each cycle.
+// A solve from a caller's start is redone from zero when its FP64 true residual exceeds
+// this multiple of the solver tolerance.
+constexpr double kFallbackSlack = 10.0;
+
 int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st, bool x_zero, const double* ax0)
 {
     const long n = (long)v.N + v.N_isolated;
@@ -547,6 +626,20 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
 
     int iter = 0, matvecs = 0, restarts = 0, converged = 0;
     double resid = 0;
+    // a non-finite residual ends the solve; the counts so far go to stats
+    auto nonfinite = [&]() {
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->iterations = iter;
+            stats->matvecs = matvecs;
+            stats->restarts = restarts;
+            stats->resid = std::numeric_limits<double>::quiet_NaN();
+            stats->tolerance = tolerance;
+            stats->true_resid = -1.0;
+            stats->true_relres = -1.0;
+        }
+        return set_error(CR_ERR_NOT_CONVERGED, "non-finite residual in GMRES");
+    };
     struct MeshMarkReset {
         cr_context* c;
         ~MeshMarkReset() { c->mesh_zeroed = false; }
@@ -568,7 +661,7 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
         CR_RET(norm2(c, w, n, scal, st));
         CR_HIP(launch_cycle_init(g, scal, st));
         CR_RET(read_scalar(c, &g->beta, &resid, st));
-        if (!std::isfinite(resid)) return set_error(CR_ERR_NOT_CONVERGED, "non-finite residual in GMRES");
+        if (!std::isfinite(resid)) return nonfinite();
         if (resid <= tolerance) {
             converged = 1;
             break;
@@ -608,7 +701,7 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
             if (!c->hpub) {
                 if (go_on) CR_RET(preclear_mesh(c, st));
                 CR_RET(read_scalar(c, &g->resid, &resid, st));
-                if (!std::isfinite(resid)) return set_error(CR_ERR_NOT_CONVERGED, "non-finite residual in GMRES");
+                if (!std::isfinite(resid)) return nonfinite();
                 if (resid <= tolerance) {
                     converged = 1;
                     break;
@@ -623,12 +716,12 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
                 double rp = 0;
                 CR_RET(wait_scalar(c, pend_slot, pend_seq, &rp, st));
                 pending = false;
-                if (!std::isfinite(rp)) return set_error(CR_ERR_NOT_CONVERGED, "non-finite residual in GMRES");
+                if (!std::isfinite(rp)) return nonfinite();
                 r2 = r1;
                 r1 = rp;
                 if (rp <= tolerance) {
                     CR_RET(wait_scalar(c, slot, seq, &resid, st));
-                    if (!std::isfinite(resid)) return set_error(CR_ERR_NOT_CONVERGED, "non-finite residual in GMRES");
+                    if (!std::isfinite(resid)) return nonfinite();
                     converged = 1;
                     break;
                 }
@@ -641,7 +734,7 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
                 continue;
             }
             CR_RET(wait_scalar(c, slot, seq, &resid, st));
-            if (!std::isfinite(resid)) return set_error(CR_ERR_NOT_CONVERGED, "non-finite residual in GMRES");
+            if (!std::isfinite(resid)) return nonfinite();
             r2 = r1;
             r1 = resid;
             if (resid <= tolerance) {
@@ -662,6 +755,7 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
         stats->tolerance = tolerance;
         stats->true_resid = -1.0;
         stats->true_relres = -1.0;
+        stats->fallback = 0;
     }
     return CR_OK;
 }
@@ -774,6 +868,65 @@ using namespace cr::impl;
 
 extern "C" {
 
+int cr_set_box(cr_context* c, const double L[3], void* stream)
+{
+    if (!L) return set_error(CR_ERR_INVALID, "null box");
+    return set_box(c, L, (hipStream_t)stream);
+}
+
+int cr_compute_force_f32(cr_context* c, const cr_system_f32* s, float* q_inout, float* phi_inout, float* force,
+                         int x0_mode, cr_solve_stats* stats, void* stream)
+{
+    if (!c || !c->ready) return set_error(CR_ERR_STATE, "cr_set_params has not been called");
+    if (!s) return set_error(CR_ERR_INVALID, "null system");
+    hipStream_t st = (hipStream_t)stream;
+    const long N = std::max(0, s->N), B = std::max(0, s->N_isolated), R = std::max(0, s->N_rigid);
+    if ((N > 0 && (!s->pos || !q_inout || !force)) || (B > 0 && (!s->body_charge || !phi_inout)) ||
+        (R > 0 && !s->rigid_com))
+        return set_error(CR_ERR_INVALID, "missing array");
+    CR_HIP(hipSetDevice(c->device));
+    CR_HIP(c->f32_pos.ensure(sizeof(double) * 4 * (size_t)std::max(1L, N)));
+    CR_HIP(c->f32_com.ensure(sizeof(double) * 4 * (size_t)std::max(1L, R)));
+    CR_HIP(c->f32_bq.ensure(sizeof(double) * (size_t)std::max(1L, B)));
+    CR_HIP(c->f32_q.ensure(sizeof(double) * (size_t)std::max(1L, N)));
+    CR_HIP(c->f32_phi.ensure(sizeof(double) * (size_t)std::max(1L, B)));
+    CR_HIP(c->f32_force.ensure(sizeof(double) * 4 * (size_t)std::max(1L, N)));
+    if (N > 0) {
+        CR_HIP((launch_convert<float, double>(s->pos, c->f32_pos.as<double>(), 4 * N, st)));
+        CR_HIP((launch_convert<float, double>(q_inout, c->f32_q.as<double>(), N, st)));
+    }
+    if (R > 0) CR_HIP((launch_convert<float, double>(s->rigid_com, c->f32_com.as<double>(), 4 * R, st)));
+    if (B > 0) {
+        CR_HIP((launch_convert<float, double>(s->body_charge, c->f32_bq.as<double>(), B, st)));
+        CR_HIP((launch_convert<float, double>(phi_inout, c->f32_phi.as<double>(), B, st)));
+    }
+    cr_system d{};
+    d.N = s->N;
+    d.pos = c->f32_pos.as<double>();
+    d.tag = s->tag;
+    d.body_of_tag = s->body_of_tag;
+    d.body_charge = c->f32_bq.as<double>();
+    d.N_rigid = s->N_rigid;
+    d.N_isolated = s->N_isolated;
+    d.Nmax = s->Nmax;
+    d.rigid_membership = s->rigid_membership;
+    d.rigid_members = s->rigid_members;
+    d.rigid_size = s->rigid_size;
+    d.rigid_com = c->f32_com.as<double>();
+    d.nlist = s->nlist;
+    d.head_list = s->head_list;
+    d.n_neigh = s->n_neigh;
+    for (int k = 0; k < 3; ++k) d.L[k] = s->L[k];
+    CR_RET(cr_compute_force(c, &d, c->f32_q.as<double>(), c->f32_phi.as<double>(), c->f32_force.as<double>(), x0_mode,
+                            stats, stream));
+    if (N > 0) {
+        CR_HIP((launch_convert<double, float>(c->f32_q.as<double>(), q_inout, N, st)));
+        CR_HIP((launch_convert<double, float>(c->f32_force.as<double>(), force, 4 * N, st)));
+    }
+    if (B > 0) CR_HIP((launch_convert<double, float>(c->f32_phi.as<double>(), phi_inout, B, st)));
+    return CR_OK;
+}
+
 const char* cr_last_error(void) { return g_error.c_str(); }
 
 int cr_create(cr_context** out, int device)
@@ -847,6 +1000,7 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
     }
     int P = (p->P > 0) ? p->P : (int)std::ceil(-2.0 * std::log(tol) / kPi);
     for (int d = 0; d < 3; ++d) r.eta[d] = P * r.h[d] * r.h[d] * xi * xi / kPi;
+    c->P_eta = P;
     P = std::min(P, std::min(r.mesh[0], std::min(r.mesh[1], r.mesh[2])));
     r.P = P;
     r.drtable = 0.001;
@@ -1126,11 +1280,39 @@ int cr_compute_force(cr_context* c, const cr_system* s, double* q_inout, double*
         return set_error(CR_ERR_INVALID, "unknown x0 mode");
     CR_RET(ensure_csr(c, v, st));
     CR_RET(init_x0(c, v, x0_mode, q_inout, phi_inout, x, st));
-    CR_RET(gmres(c, v, stats, st, x0_mode == CR_X0_ZERO,
-                 x0_mode == CR_X0_CHARGES ? c->yraw.as<double>() : nullptr));
+    cr_solve_stats local{};
+    cr_solve_stats* sp = stats ? stats : &local;
+    const int rc = gmres(c, v, sp, st, x0_mode == CR_X0_ZERO,
+                         x0_mode == CR_X0_CHARGES ? c->yraw.as<double>() : nullptr);
+    bool redo = false, checked = false;
+    if (x0_mode == CR_X0_ZERO) {
+        CR_RET(rc);
+    } else if (rc == CR_ERR_NOT_CONVERGED) {
+        redo = true;  // a non-finite residual from the given start
+    } else {
+        CR_RET(rc);
+        if (!sp->converged) {
+            redo = true;  // the iteration limit from the given start
+        } else if (stats && c->check_residual) {
+            CR_RET(true_residual(c, v, x, sp, st));
+            checked = true;
+            redo = !(sp->true_resid <= kFallbackSlack * sp->tolerance);
+        }
+    }
+    if (redo) {
+        // plain GMRES from x0 = 0 decides (the fallback and judge of the converged residual)
+        const cr_solve_stats first = *sp;
+        CR_RET(init_x0(c, v, CR_X0_ZERO, q_inout, phi_inout, x, st));
+        CR_RET(gmres(c, v, sp, st, true, nullptr));
+        sp->iterations += first.iterations;
+        sp->matvecs += first.matvecs;
+        sp->restarts += first.restarts;
+        sp->fallback = 1;
+        checked = false;
+    }
     CR_HIP(hipMemcpyAsync(q_inout, x, sizeof(double) * v.N, hipMemcpyDeviceToDevice, st));
     CR_HIP(hipMemcpyAsync(phi_inout, x + v.N, sizeof(double) * v.N_isolated, hipMemcpyDeviceToDevice, st));
-    if (stats && c->check_residual) CR_RET(true_residual(c, v, x, stats, st));
+    if (stats && c->check_residual && !checked) CR_RET(true_residual(c, v, x, stats, st));
     if (c->fp32())
         CR_RET(force_pass<float>(c, v, x, force, st));
     else

@@ -60,6 +60,9 @@ struct cr_context {
     GmresState* g_host = nullptr;
     HostPub* hpub = nullptr;            // coherent pinned, kPubSlots mailboxes: residuals the host polls for
     unsigned long long pub_seq = 0;
+    int P_eta = 0;   // window support before the clamp to the mesh (eta uses it)
+    // cr_compute_force_f32: double copies of the caller's float arrays
+    DevBuf f32_pos, f32_com, f32_bq, f32_q, f32_phi, f32_force;
     double E0[3] = {0, 0, 0};
     unsigned long generation = 0;  // bumped whenever parameters, tables or tiling change
 
@@ -77,6 +80,8 @@ struct cr_context {
 namespace cr::impl {
 
 SysView make_view(cr_context* c, const cr_system* s);
+// Take up a new box (cr_set_box): device-side regeneration of the box-dependent quantities.
+int set_box(cr_context* c, const double L[3], hipStream_t st);
 // Validate the system and refresh the body-tag map; with bin, also rebin beads and rebuild
 // window records.
 int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out, bool bin = true);

@@ -191,6 +191,11 @@ template <typename T>
 hipError_t launch_fill(T* p, T v, long n, hipStream_t st);
 template <typename Tin, typename Tout>
 hipError_t launch_convert(const Tin* in, Tout* out, long n, hipStream_t st);
+// k-space factor tables regenerated on the device after a box change (k_se.hip): one
+// axis's wavenumbers and Gaussian factors, and the cubic-box radial table.
+hipError_t launch_kspace_axis(int n, double L, double xi, double eta, double* k, double* e, hipStream_t st);
+template <typename T>
+hipError_t launch_kspace_radial(long m2max, double kappa, double invG, T* rad, hipStream_t st);
 
 // ---- k_fft.hip ----
 // Real mesh [Nx][Ny][Nz] <-> half spectrum [Nx][Ny][Nz/2+1] (interleaved complex).

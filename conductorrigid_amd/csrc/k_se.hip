@@ -247,6 +247,48 @@ template hipError_t launch_contract_force<float>(const MeshParams&, const SysVie
                                                  double*, hipStream_t);
 template hipError_t launch_contract_force<double>(const MeshParams&, const SysView&, const double*, const double*,
                                                   double*, hipStream_t);
+// k-space factor tables of one axis (SetParams, cppmodule/ConductorRigid.cc:245-274):
+// k_i = 2 pi (i < (n+1)/2 ? i : i - n) / L and e_i = exp(-(1 - eta) k_i^2 / (4 xi^2)).
+__global__ void k_kspace_axis(int n, double L, double xi, double eta, double* __restrict__ kt, double* __restrict__ et)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double k = (double)((i < (n + 1) / 2) ? i : i - n) * 2.0 * 3.14159265358979323846 / L;
+    kt[i] = k;
+    et[i] = exp(-(1.0 - eta) * k * k / (4.0 * xi * xi));
+}
+
+// Cubic boxes: j0(|k|)^2 / k^2 / G over m2 = |m|^2, k^2 = kappa^2 m2 (0 at m2 = 0).
+template <typename T>
+__global__ void k_kspace_radial(long m2max, double kappa, double invG, T* __restrict__ rad)
+{
+    const long m2 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m2 > m2max) return;
+    if (m2 == 0) {
+        rad[0] = (T)0;
+        return;
+    }
+    const double k2 = kappa * kappa * (double)m2, k = sqrt(k2);
+    const double j0 = sin(k) / k;
+    rad[m2] = (T)(j0 * j0 / k2 * invG);
+}
+
+hipError_t launch_kspace_axis(int n, double L, double xi, double eta, double* k, double* e, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_kspace_axis, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, L, xi, eta, k, e);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_kspace_radial(long m2max, double kappa, double invG, T* rad, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_kspace_radial<T>, dim3((unsigned)((m2max + 256) / 256)), dim3(256), 0, st, m2max, kappa,
+                       invG, rad);
+    return hipGetLastError();
+}
+template hipError_t launch_kspace_radial<float>(long, double, double, float*, hipStream_t);
+template hipError_t launch_kspace_radial<double>(long, double, double, double*, hipStream_t);
+
 template hipError_t launch_fill<float>(float*, float, long, hipStream_t);
 template hipError_t launch_fill<double>(double*, double, long, hipStream_t);
 template hipError_t launch_fill<unsigned>(unsigned*, unsigned, long, hipStream_t);

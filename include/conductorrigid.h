@@ -104,6 +104,9 @@ typedef struct {
     const unsigned* nlist;              /* full neighbour list */
     const unsigned* head_list;          /* [N] first neighbour of each bead */
     const unsigned* n_neigh;            /* [N] neighbour count of each bead */
+    double L[3];                        /* box lengths of this call (HOOMD passes its BoxDim to
+                                           gpu_ComputeForce every step); all zero: the context's box.
+                                           A different box is taken up first, as cr_set_box does. */
 } cr_system;
 
 typedef struct {
@@ -115,6 +118,7 @@ typedef struct {
     double tolerance;      /* tol * ||b||_2 */
     double true_resid;     /* ||D (b - A x)||_2 recomputed in FP64 after the solve */
     double true_relres;    /* ||b - A x||_2 / ||b||_2 (unpreconditioned), FP64 */
+    int fallback;          /* 1 when the solve was redone from x0 = 0 (see cr_compute_force) */
 } cr_solve_stats;
 
 /* ---- lifetime ---------------------------------------------------------- */
@@ -131,6 +135,17 @@ int cr_get_resolved(const cr_context* ctx, cr_resolved* out);
 /* UpdateField (cppmodule/ConductorRigid.cc:381-386). */
 int cr_update_field(cr_context* ctx, const double E0[3]);
 
+/* Box change (SetParams reads the box, cppmodule/ConductorRigid.cc:116-121; HOOMD hands
+ * gpu_ComputeForce the current BoxDim every step, cppmodule/ConductorRigid.cuh:23-73).
+ * Keeps the mesh, window support, cutoff and real-space tables (they depend on xi and the
+ * tolerance only) and regenerates on the device what depends on the box: the mesh spacing
+ * h, eta, the window prefactor, the quadrature weight and the k-space scale tables.  When
+ * the mesh was derived from the box (no mesh override) and the new box derives another
+ * mesh, the parameters are set again in full (cr_set_params).  Work is queued on stream.
+ * Every compute entry point calls this itself when its cr_system carries a box that
+ * differs from the context's. */
+int cr_set_box(cr_context* ctx, const double L[3], void* stream);
+
 /* ---- the reference's device ABI ---------------------------------------- */
 
 /* gpu_ZeroForce (cppmodule/ConductorRigid.cuh:19-21). force: device double[4N]. */
@@ -139,9 +154,43 @@ int cr_zero_force(cr_context* ctx, int N, double* force, void* stream);
 /* gpu_ComputeForce (cppmodule/ConductorRigid.cuh:23-73): solve for the bead charges
  * and body potentials (GMRES, x0 per x0_mode), write them back to q_inout [N] and
  * phi_inout [N_isolated], and overwrite force [4N] with
- * F = q E0 + reciprocal + real-space forces.  stats may be NULL. */
+ * F = q E0 + reciprocal + real-space forces.  stats may be NULL.
+ * Plain GMRES from x0 = 0 is the fallback: a solve started from CR_X0_PREVIOUS or
+ * CR_X0_CHARGES is redone from zero (stats->fallback = 1, iteration counts summed) when it
+ * meets a non-finite residual, stops at the iteration limit without converging, or -- with
+ * the residual check on and stats given -- ends with an FP64 true residual ||D(b - Ax)|| above
+ * 10x the solver tolerance.  The reference accepts any outcome
+ * (cppmodule/ConductorRigid.cu:1142-1147). */
 int cr_compute_force(cr_context* ctx, const cr_system* sys, double* q_inout, double* phi_inout,
                      double* force, int x0_mode, cr_solve_stats* stats, void* stream);
+
+/* gpu_ComputeForce as a SINGLE_PRECISION HOOMD build hands it over (the only build the
+ * reference compiles in, cppmodule/ConductorRigid.h:6-10; Scalar4 / Scalar pointers,
+ * cppmodule/ConductorRigid.cuh:23-73): the same view with float arrays for positions
+ * (float4 x, y, z, type), body net charges and rigid centres of mass (float4), float bead
+ * charges q_inout [N], body potentials phi_inout [N_isolated] and forces [4N] (float4,
+ * w = 0).  The arrays are converted to double on the device and the call proceeds as
+ * cr_compute_force (in the context's precision); results are rounded back to float. */
+typedef struct {
+    int N;
+    const float* pos;                   /* [4N] x, y, z, type by global index */
+    const unsigned* tag;
+    const unsigned* body_of_tag;
+    const float* body_charge;           /* [N_isolated] */
+    int N_rigid;
+    int N_isolated;
+    int Nmax;
+    const unsigned* rigid_membership;
+    const unsigned* rigid_members;
+    const unsigned* rigid_size;
+    const float* rigid_com;             /* [4*N_rigid] */
+    const unsigned* nlist;
+    const unsigned* head_list;
+    const unsigned* n_neigh;
+    double L[3];
+} cr_system_f32;
+int cr_compute_force_f32(cr_context* ctx, const cr_system_f32* sys, float* q_inout, float* phi_inout,
+                         float* force, int x0_mode, cr_solve_stats* stats, void* stream);
 
 /* MatrixVectorMultiply (cppmodule/ConductorRigid.cuh:76-114): y = A x with
  * x = [q (N) ; Phi (N_isolated)], y = [Phi_b(i) - (M q)_i ; sum_{i in b} q_i]. */

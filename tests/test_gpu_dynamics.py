@@ -101,8 +101,57 @@ def test_box_change_is_picked_up(require_gpu):
     cr.computeForces(0, x0="zero")
     ref = _cr(copy.deepcopy(s2))
     ref.computeForces(0, x0="zero")
-    assert cr.params["h"] == ref.params["h"] and cr.params["mesh"] == ref.params["mesh"]
     vq = rel(cr.get_charges(), ref.get_charges())
     vF = rel(cr.get_forces()[:, :3], ref.get_forces()[:, :3])
     print(f"[box] rescaled box: charges {vq:.1e}, forces {vF:.1e}", flush=True)
     assert vq < 1e-9 and vF < 1e-9
+    # against the CPU oracle built on the rescaled system (spacing, widths, k-scale follow the box)
+    import oracle as O
+
+    o = O.Oracle(s2, SMALL["xi"], SMALL["errortol"], rc=SMALL["rc"], mesh=SMALL["mesh"], P=SMALL["P"],
+                 nlist=cr.dsys.nlist_host())
+    qo, Po, Fo, so = o.compute_forces([0.0, 0.0, 1.0], restart=30, maxiter=600, tol=1e-12)
+    vq, vP, vF = rel(cr.get_charges(), qo), rel(cr.get_potentials(), Po), rel(cr.get_forces()[:, :3], Fo[:, :3])
+    print(f"[box] rescaled box vs oracle: charges {vq:.1e}, potentials {vP:.1e}, forces {vF:.1e}", flush=True)
+    assert vq < 1e-10 and vP < 1e-10 and vF < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_set_box_regenerates_on_device(precision, require_gpu):
+    """cr_set_box on a stretched (non-cubic) and then a cubic box: the context's k-space
+    step and matvec equal the oracle's for that box (tables regenerated on the device)."""
+    import copy
+
+    import oracle as O
+    from helpers import GpuCase
+
+    s = small_system(seed=19)
+    g = GpuCase(s, precision=precision, **SMALL)
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    for f in ((1.03, 0.98, 1.01), (0.97, 0.97, 0.97)):
+        s2 = copy.deepcopy(s)
+        s2.L = s.L * np.asarray(f)
+        s2.pos = s.pos.copy()
+        s2.pos[:, :3] *= np.asarray(f)
+        s2.rigid_com = s.rigid_com.copy()
+        s2.rigid_com[:, :3] *= np.asarray(f)
+        _lib_set_box(g, s2.L)
+        from conductorrigid_amd.compute import update_positions
+
+        g.cr.system.L = s2.L.copy()
+        update_positions(g.cr, s2.pos, s2.rigid_com)
+        o = O.Oracle(s2, SMALL["xi"], SMALL["errortol"], rc=SMALL["rc"], mesh=SMALL["mesh"], P=SMALL["P"],
+                     nlist=g.cr.dsys.nlist_host())
+        grid = np.random.default_rng(3).normal(size=SMALL["mesh"])
+        vr = rel(g.recip(grid), o.recip_grid(grid))
+        x = np.random.default_rng(4).normal(size=g.n)
+        vm = rel(g.matvec(x), o.matvec(x))
+        print(f"[box] {precision} box {f}: recip {vr:.1e} matvec {vm:.1e}", flush=True)
+        assert vr < tol and vm < max(tol, 1e-12 if precision == "fp64" else 1e-5)
+
+
+def _lib_set_box(g, L):
+    from conductorrigid_amd import _lib
+
+    _lib.check(g.lib.cr_set_box(g.ctx, (C.c_double * 3)(*[float(v) for v in L]), g.st()))
+    g.torch.cuda.synchronize()

@@ -151,3 +151,44 @@ def test_learned_warm_start_gain(require_gpu):
     # solution its field-response correction is within noise of the previous start
     assert mean["learned"] < mean["zero"]
     assert mean["learned+previous"] <= mean["previous"] + 1.0
+
+
+def test_config5_workload_learned_start(require_gpu):
+    """The learned-warm-start workload at its own size (100k beads, 5000 bodies x 20, the
+    n100k parameters), on a short version of its protocol: a Brownian trajectory with
+    |E0| ~ U[0.5, 2] per step, the network trained on solver outputs of the first steps,
+    then held-out steps solved from zero and from the network's start.  The learned start
+    gives the zero start's answer (to the solver tolerance) with no fallback."""
+    import torch
+    from conductorrigid_amd.compute import ConductorRigid
+    from conductorrigid_amd.system import make_workload
+    from conductorrigid_amd.warmstart import WarmStartMLP, bd_step, fit, run_trajectory
+
+    s, w = make_workload("n100k")
+    assert s.N == 100000 and s.N_rigid == 5000
+    cr = ConductorRigid(system=s, field=(0.0, 0.0, 1.0), xi=w["xi"], errortol=w["errortol"], rc=w["rc"],
+                        mesh=w["mesh"], P=w["P"], restart=w["restart"], gmres_tol=w["tol"], max_iter=2000,
+                        precision="fp32", preconditioner="block")
+    m = WarmStartMLP(device=0, seed=5)
+    n_train, n_eval = 40, 4
+    recs, pairs = run_trajectory(cr, n_train, dt=2e-4, collect=range(0, n_train, 4), mlp=m, seed=6)
+    hist = fit(m, pairs, epochs=3, batch=8192, lr=2e-3)
+    del pairs
+    rng = np.random.default_rng(7)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    it = {"zero": [], "learned": []}
+    for k in range(n_eval):
+        bd_step(cr, 2e-4, generator=gen)
+        cr.update_field((0.0, 0.0, float(rng.uniform(0.5, 2.0))))
+        st0 = cr.computeForces(k, x0="zero")
+        q_ref = cr.get_charges()
+        m.warm_start(cr)
+        st = cr.computeForces(k, x0="charges")
+        assert st0["converged"] and st["converged"] and st["fallback"] == 0
+        assert rel(cr.get_charges(), q_ref) < 1e-4
+        it["zero"].append(st0["iterations"])
+        it["learned"].append(st["iterations"])
+    print(f"\n[config5] 100k beads: loss {hist[0]:.3e} -> {hist[-1]:.3e}; iterations zero {it['zero']}, "
+          f"learned {it['learned']}", flush=True)
+    assert hist[-1] < hist[0]

@@ -194,10 +194,12 @@ def test_tiling_does_not_change_the_answer(case, tile):
     assert rel(qs, qo) < 1e-10 and rel(F[:, :3], Fo[:, :3]) < 1e-10
 
 
-# Register four-step FFT lengths n = A * B (32 = 8x4, 128 = 16x8, 512 = 32x16 beside the
-# square 64): every axis pass (z R2C / C2R / C2C, strided y, fused x + scale) against
-# numpy's FFT and against the oracle's k-space step.
-@pytest.mark.parametrize("mesh", [(32, 128, 64), (128, 32, 512), (512, 64, 32)])
+# Register four-step FFT lengths n = A * B (32 = 8x4, 128 = 16x8, 256 = 16x16, 512 = 32x16,
+# 1024 = 32x32 beside the square 64): every axis pass (z R2C / C2R / C2C, strided y, fused
+# x + scale) against numpy's FFT and against the oracle's k-space step.  256 is the
+# headline mesh's length on every axis.
+@pytest.mark.parametrize("mesh", [(32, 128, 64), (128, 32, 512), (512, 64, 32), (256, 32, 64), (64, 256, 32),
+                                  (32, 64, 256), (1024, 32, 32), (32, 32, 1024)])
 def test_register_fft_lengths(mesh, require_gpu):
     s = n1k_system()
     prm = dict(xi=0.929, errortol=1e-6, rc=4.0, mesh=mesh, P=8)
