@@ -12,7 +12,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libconductorrigid.so")
 
 CR_OK = 0
+CR_ERR_INVALID = 1
+CR_ERR_DEVICE = 2
 CR_ERR_NOT_CONVERGED = 3
+CR_ERR_STATE = 4
 CR_PRECISION_FP64 = 0
 CR_PRECISION_FP32_BULK = 1
 CR_X0_PREVIOUS = 0
@@ -118,6 +121,8 @@ _SIGS = {
     "cr_mlp_train_step": (C.c_int, [vp, C.c_int, vp, vp, C.c_float, vp, vp]),
     "cr_build_nlist": (C.c_int, [vp, C.c_int, vp, C.POINTER(C.c_double), C.c_double, vp, vp, vp, C.c_long,
                                  C.POINTER(C.c_long), vp]),
+    "cr_build_nlist_noalloc": (C.c_int, [vp, C.c_int, vp, C.POINTER(C.c_double), C.c_double, vp, vp, vp, C.c_long,
+                                         C.POINTER(C.c_long), vp]),
     "cr_batch_create": (C.c_int, [C.POINTER(vp), C.c_int]),
     "cr_batch_destroy": (None, [vp]),
     "cr_batch_compute_force": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.POINTER(CrSystem), C.POINTER(vp),

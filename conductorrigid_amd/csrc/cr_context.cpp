@@ -936,6 +936,9 @@ int cr_create(cr_context** out, int device)
     CR_HIP(hipSetDevice(device));
     cr_context* c = new cr_context();
     c->device = device;
+    // the allocation-free neighbour-list build's static scratch on this device (its
+    // address is looked up once per device; a failure here surfaces at that entry point)
+    if (nlist_scratch_setup(device) != hipSuccess) (void)hipGetLastError();
     if (hipMalloc((void**)&c->g_dev, sizeof(GmresState)) != hipSuccess ||
         hipHostMalloc((void**)&c->g_host, sizeof(GmresState), hipHostMallocDefault) != hipSuccess) {
         delete c;
@@ -1513,6 +1516,26 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     for (int k = 0; k < nst && k < max_stages; ++k) stage_ms[k] = acc[k] / reps;
+    return CR_OK;
+}
+
+int cr_build_nlist_noalloc(cr_context* c, int N, const double* pos, const double L[3], double rlist,
+                           unsigned* head_list, unsigned* n_neigh, unsigned* nlist, long capacity, long* total,
+                           void* stream)
+{
+    if (!c || !L || !total || !(rlist > 0)) return set_error(CR_ERR_INVALID, "bad argument");
+    if (N > 0 && (!pos || !head_list || !n_neigh || (capacity > 0 && !nlist)))
+        return set_error(CR_ERR_INVALID, "null array");
+    for (int d = 0; d < 3; ++d)
+        if (!(L[d] > 0)) return set_error(CR_ERR_INVALID, "bad box");
+    const size_t need = nlist_noalloc_bytes(N, L, rlist), have = nlist_noalloc_capacity();
+    if (need > have)
+        return set_error(CR_ERR_INVALID, "cell-list scratch for N = " + std::to_string(N) + " needs " +
+                                             std::to_string(need) + " bytes; the static scratch holds " +
+                                             std::to_string(have) + " (use cr_build_nlist)");
+    CR_HIP(hipSetDevice(c->device));
+    CR_HIP(build_nlist_noalloc(N, reinterpret_cast<const double4*>(pos), L, rlist, head_list, n_neigh, nlist, capacity,
+                               total, (hipStream_t)stream));
     return CR_OK;
 }
 

@@ -384,6 +384,15 @@ hipError_t exclusive_scan_u32(const unsigned* in, unsigned* out, long n, unsigne
 hipError_t build_nlist(int N, const double4* pos, const double L[3], double rlist, unsigned* head,
                        unsigned* nn, unsigned* nlist, long capacity, long* total, void* scratch_pool,
                        hipStream_t st);
+// The same build on the file-scope static scratch of k_nlist.hip (no allocation):
+// nlist_scratch_setup(device) records that device's scratch address (the device must be
+// current); nlist_noalloc_bytes is what a build needs, nlist_noalloc_capacity what the
+// scratch holds; build_nlist_noalloc returns hipErrorInvalidValue beyond it.
+hipError_t nlist_scratch_setup(int device);
+size_t nlist_noalloc_bytes(int N, const double L[3], double rlist);
+size_t nlist_noalloc_capacity();
+hipError_t build_nlist_noalloc(int N, const double4* pos, const double L[3], double rlist, unsigned* head,
+                               unsigned* nn, unsigned* nlist, long capacity, long* total, hipStream_t st);
 
 // ---- k_krylov.hip ----
 // Partial dot products of w with columns 0..ncols-1 of V (column pitch ld):

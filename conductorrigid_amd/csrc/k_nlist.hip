@@ -11,11 +11,29 @@
 // fill pass over the 27 neighbouring cells, exclusive scans for the offsets.
 #include "cr_kernels.h"
 
+#include <algorithm>
+#include <mutex>
+
 namespace cr {
+
+// Scratch of the allocation-free build (cr_build_nlist_noalloc): file-scope device memory
+// that every GPU the library runs on holds until the process ends.  Host code never uses
+// this name as an address: the address on each device comes from hipGetSymbolAddress
+// (nlist_scratch_setup) and is kept per device.
+constexpr size_t kNlistScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kNlistScratchBytes];
 
 namespace {
 
 constexpr int kScanBlock = 1024;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+__global__ void k_fill_u32(unsigned* __restrict__ p, long n, unsigned v)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
 
 __device__ __forceinline__ double min_image(double d, double L) { return d - L * rint(d / L); }
 
@@ -180,71 +198,155 @@ hipError_t exclusive_scan_u32(const unsigned* in, unsigned* out, long n, unsigne
     return hipGetLastError();
 }
 
+namespace {
+
+// Cell-list working arrays of one build (device).
+struct NlistWork {
+    unsigned *cell, *members, *count, *start, *cursor;
+    unsigned long long* sums;
+};
+
+struct NlistPlan {
+    CellGrid cg;
+    long ncell, nsum;
+};
+
+NlistPlan nlist_plan(int N, const double L[3], double rlist)
+{
+    NlistPlan p;
+    p.cg.brute = 0;
+    p.ncell = 1;
+    for (int d = 0; d < 3; ++d) {
+        p.cg.L[d] = L[d];
+        int n = (int)floor(L[d] / rlist);
+        if (n < 1) n = 1;
+        if (n > 1024) n = 1024;
+        p.cg.nc[d] = n;
+        if (n < 3) p.cg.brute = 1;
+        p.ncell *= n;
+    }
+    p.cg.r2 = rlist * rlist;
+    if (p.cg.brute) {
+        p.cg.nc[0] = p.cg.nc[1] = p.cg.nc[2] = 1;
+        p.ncell = 1;
+    }
+    p.nsum = ((p.ncell > N ? p.ncell : N) + kScanBlock - 1) / kScanBlock + 2;
+    return p;
+}
+
+inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Bytes of the working arrays, each 256-byte aligned, in the order nlist_carve lays them out.
+size_t nlist_work_bytes(int N, const NlistPlan& p)
+{
+    return 2 * align256(sizeof(unsigned) * (size_t)N) + 3 * align256(sizeof(unsigned) * (size_t)p.ncell) +
+           align256(sizeof(unsigned long long) * (size_t)p.nsum);
+}
+
+NlistWork nlist_carve(void* base, int N, const NlistPlan& p)
+{
+    char* b = static_cast<char*>(base);
+    NlistWork w;
+    w.cell = reinterpret_cast<unsigned*>(b);
+    b += align256(sizeof(unsigned) * (size_t)N);
+    w.members = reinterpret_cast<unsigned*>(b);
+    b += align256(sizeof(unsigned) * (size_t)N);
+    w.count = reinterpret_cast<unsigned*>(b);
+    b += align256(sizeof(unsigned) * (size_t)p.ncell);
+    w.start = reinterpret_cast<unsigned*>(b);
+    b += align256(sizeof(unsigned) * (size_t)p.ncell);
+    w.cursor = reinterpret_cast<unsigned*>(b);
+    b += align256(sizeof(unsigned) * (size_t)p.ncell);
+    w.sums = reinterpret_cast<unsigned long long*>(b);
+    return w;
+}
+
+// The build on given working arrays; counters are cleared with a fill kernel.
+hipError_t nlist_run(int N, const double4* pos, const NlistPlan& p, const NlistWork& w, unsigned* head, unsigned* nn,
+                     unsigned* nlist, long capacity, long* total, hipStream_t st)
+{
+    const CellGrid& cg = p.cg;
+    const long ncell = p.ncell;
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(blocks(ncell, 256), 1024u)), dim3(256), 0, st, w.count, ncell, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(blocks(ncell, 256), 1024u)), dim3(256), 0, st, w.cursor, ncell, 0u);
+    hipLaunchKernelGGL(k_cell_of, dim3(blocks(N, 256)), dim3(256), 0, st, N, pos, cg, w.cell, w.count);
+    hipError_t e = exclusive_scan_u32(w.count, w.start, ncell, w.sums, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cell_fill, dim3(blocks(N, 256)), dim3(256), 0, st, N, w.cell, w.start, w.cursor, w.members);
+    hipLaunchKernelGGL(k_cell_sort, dim3(blocks(ncell, 256)), dim3(256), 0, st, ncell, w.start, w.count, w.members);
+    hipLaunchKernelGGL((k_pairs<false>), dim3(blocks(N, 128)), dim3(128), 0, st, N, pos, cg, w.cell, w.start, w.count,
+                       w.members, nn, (const unsigned*)nullptr, (unsigned*)nullptr, 0L);
+    e = exclusive_scan_u32(nn, head, N, w.sums, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_pairs<true>), dim3(blocks(N, 128)), dim3(128), 0, st, N, pos, cg, w.cell, w.start, w.count,
+                       w.members, (unsigned*)nullptr, head, nlist, capacity);
+    unsigned long long tot = 0;
+    const long nb = (N + kScanBlock - 1) / kScanBlock;
+    e = hipMemcpyAsync(&tot, w.sums + nb, sizeof(tot), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    *total = (long)tot;
+    return hipGetLastError();
+}
+
+}  // namespace
+
 hipError_t build_nlist(int N, const double4* pos, const double L[3], double rlist, unsigned* head, unsigned* nn,
                        unsigned* nlist, long capacity, long* total, void* /*scratch_pool*/, hipStream_t st)
 {
     *total = 0;
     if (N <= 0) return hipSuccess;
-    CellGrid cg;
-    cg.brute = 0;
-    long ncell = 1;
-    for (int d = 0; d < 3; ++d) {
-        cg.L[d] = L[d];
-        int n = (int)floor(L[d] / rlist);
-        if (n < 1) n = 1;
-        if (n > 1024) n = 1024;
-        cg.nc[d] = n;
-        if (n < 3) cg.brute = 1;
-        ncell *= n;
-    }
-    cg.r2 = rlist * rlist;
-    if (cg.brute) {
-        cg.nc[0] = cg.nc[1] = cg.nc[2] = 1;
-        ncell = 1;
-    }
-    unsigned *cell = nullptr, *count = nullptr, *start = nullptr, *cursor = nullptr, *members = nullptr;
-    unsigned long long* sums = nullptr;
-    const long nsum = ((ncell > N ? ncell : N) + kScanBlock - 1) / kScanBlock + 2;
-    hipError_t e = hipSuccess;
-#define NL_TRY(x)                 \
-    do {                          \
-        e = (x);                  \
-        if (e != hipSuccess) goto done; \
-    } while (0)
-    NL_TRY(hipMallocAsync((void**)&cell, sizeof(unsigned) * N, st));
-    NL_TRY(hipMallocAsync((void**)&members, sizeof(unsigned) * N, st));
-    NL_TRY(hipMallocAsync((void**)&count, sizeof(unsigned) * ncell, st));
-    NL_TRY(hipMallocAsync((void**)&start, sizeof(unsigned) * ncell, st));
-    NL_TRY(hipMallocAsync((void**)&cursor, sizeof(unsigned) * ncell, st));
-    NL_TRY(hipMallocAsync((void**)&sums, sizeof(unsigned long long) * nsum, st));
-    NL_TRY(hipMemsetAsync(count, 0, sizeof(unsigned) * ncell, st));
-    NL_TRY(hipMemsetAsync(cursor, 0, sizeof(unsigned) * ncell, st));
-    hipLaunchKernelGGL(k_cell_of, dim3(blocks(N, 256)), dim3(256), 0, st, N, pos, cg, cell, count);
-    NL_TRY(exclusive_scan_u32(count, start, ncell, sums, st));
-    hipLaunchKernelGGL(k_cell_fill, dim3(blocks(N, 256)), dim3(256), 0, st, N, cell, start, cursor, members);
-    hipLaunchKernelGGL(k_cell_sort, dim3(blocks(ncell, 256)), dim3(256), 0, st, ncell, start, count, members);
-    hipLaunchKernelGGL((k_pairs<false>), dim3(blocks(N, 128)), dim3(128), 0, st, N, pos, cg, cell, start, count,
-                       members, nn, (const unsigned*)nullptr, (unsigned*)nullptr, 0L);
-    NL_TRY(exclusive_scan_u32(nn, head, N, sums, st));
-    hipLaunchKernelGGL((k_pairs<true>), dim3(blocks(N, 128)), dim3(128), 0, st, N, pos, cg, cell, start, count,
-                       members, (unsigned*)nullptr, head, nlist, capacity);
+    const NlistPlan p = nlist_plan(N, L, rlist);
+    void* base = nullptr;
+    hipError_t e = hipMallocAsync(&base, nlist_work_bytes(N, p), st);
+    if (e != hipSuccess) return e;
+    e = nlist_run(N, pos, p, nlist_carve(base, N, p), head, nn, nlist, capacity, total, st);
+    const hipError_t f = hipFreeAsync(base, st);
+    return e != hipSuccess ? e : f;
+}
+
+hipError_t nlist_scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;  // the device must be current
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+    return hipSuccess;
+}
+
+size_t nlist_noalloc_bytes(int N, const double L[3], double rlist)
+{
+    return N > 0 ? nlist_work_bytes(N, nlist_plan(N, L, rlist)) : 0;
+}
+
+size_t nlist_noalloc_capacity() { return kNlistScratchBytes; }
+
+hipError_t build_nlist_noalloc(int N, const double4* pos, const double L[3], double rlist, unsigned* head,
+                               unsigned* nn, unsigned* nlist, long capacity, long* total, hipStream_t st)
+{
+    *total = 0;
+    if (N <= 0) return hipSuccess;
+    const NlistPlan p = nlist_plan(N, L, rlist);
+    if (nlist_work_bytes(N, p) > kNlistScratchBytes) return hipErrorInvalidValue;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = nlist_scratch_setup(dev);
+    if (e != hipSuccess) return e;
+    void* base = nullptr;
     {
-        unsigned long long tot = 0;
-        const long nb = (N + kScanBlock - 1) / kScanBlock;
-        NL_TRY(hipMemcpyAsync(&tot, sums + nb, sizeof(tot), hipMemcpyDeviceToHost, st));
-        NL_TRY(hipStreamSynchronize(st));
-        *total = (long)tot;
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = g_scratch_ptr[dev];
     }
-done:
-    if (cell) (void)hipFreeAsync(cell, st);
-    if (members) (void)hipFreeAsync(members, st);
-    if (count) (void)hipFreeAsync(count, st);
-    if (start) (void)hipFreeAsync(start, st);
-    if (cursor) (void)hipFreeAsync(cursor, st);
-    if (sums) (void)hipFreeAsync(sums, st);
-#undef NL_TRY
-    if (e == hipSuccess) e = hipGetLastError();
-    return e;
+    return nlist_run(N, pos, p, nlist_carve(base, N, p), head, nn, nlist, capacity, total, st);
 }
 
 }  // namespace cr

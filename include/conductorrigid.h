@@ -253,6 +253,19 @@ int cr_build_nlist(cr_context* ctx, int N, const double* pos, const double L[3],
                    unsigned* head_list, unsigned* n_neigh, unsigned* nlist, long capacity,
                    long* total, void* stream);
 
+/* No reference counterpart: cr_build_nlist without allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, such as a captured Brownian step
+ * that rebuilds the list; tight serving loops).  Same arguments and results as
+ * cr_build_nlist.  The cell-list working arrays (8 N bytes, 12 bytes per cell of edge
+ * >= rlist and the scan sums, each rounded to 256 bytes) live in a 16 MiB static device
+ * array the library holds on every GPU it runs on until the process ends: about 1.6
+ * million beads at rlist = 4.4 and volume fraction 0.3.  Larger inputs are rejected with
+ * CR_ERR_INVALID.  Calls on one GPU share that array: calls that overlap on one GPU race,
+ * and each call clears the counters it needs with kernels (no memset nodes). */
+int cr_build_nlist_noalloc(cr_context* ctx, int N, const double* pos, const double L[3], double rlist,
+                           unsigned* head_list, unsigned* n_neigh, unsigned* nlist, long capacity, long* total,
+                           void* stream);
+
 /* ---- learned warm start -------------------------------------------------
  * No reference counterpart: the reference starts each solve from the previous step's
  * (q, Phi) (cppmodule/ConductorRigid.cu:1125-1127).  A per-bead MLP predicts each bead's
