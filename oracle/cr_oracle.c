@@ -685,13 +685,18 @@ static void gen_rot(double dx, double dy, double* cs, double* sn)
 }
 
 /* Restarted, left-preconditioned GMRES(R) following CUSP krylov::gmres.
- * Returns 1 when converged (estimate <= tol*||b||), 0 at the iteration limit. */
+ * Returns 1 when converged (estimate <= tol*||b||), 0 at the iteration limit.
+ * The preconditioner is applied out of place (into pw): the reference's precondition
+ * kernel (ConductorRigid.cu:816-859) reads a rigid body's potential row after the same
+ * thread has written it, so an in-place call would divide that row by 4 pi n_b instead
+ * of n_b. */
 int cro_gmres(cro_ctx* c, double* x, const double* b, int R, int maxiter, double tol,
               int* iters_out, double* resid_out)
 {
     const int n = c->n;
     double* V = (double*)malloc(sizeof(double) * (long)n * (R + 1));
     double* w = (double*)malloc(sizeof(double) * n);
+    double* pw = (double*)malloc(sizeof(double) * n);
     double* H = (double*)calloc((size_t)(R + 1) * R, sizeof(double));
     double* s = (double*)calloc(R + 1, sizeof(double));
     double* cs = (double*)calloc(R, sizeof(double));
@@ -702,9 +707,9 @@ int cro_gmres(cro_ctx* c, double* x, const double* b, int R, int maxiter, double
 #define HH(r, col) H[(long)(col) * (R + 1) + (r)]
     for (;;) {
         /* initial residual of this cycle: w = M (A x - b), beta = ||w|| */
-        cro_matvec(c, x, w);
-        axpy(n, -1.0, b, w);
-        cro_apply_precond(c, w, w);
+        cro_matvec(c, x, pw);
+        axpy(n, -1.0, b, pw);
+        cro_apply_precond(c, pw, w);
         double beta = sqrt(dot(n, w, w));
         resid = beta;
         if (resid <= tolerance) { converged = 1; break; }
@@ -716,8 +721,8 @@ int cro_gmres(cro_ctx* c, double* x, const double* b, int R, int maxiter, double
         do {
             ++i;
             ++iter;
-            cro_matvec(c, &V[(long)i * n], w);
-            cro_apply_precond(c, w, w);
+            cro_matvec(c, &V[(long)i * n], pw);
+            cro_apply_precond(c, pw, w);
             for (int k = 0; k <= i; ++k) {
                 HH(k, i) = dot(n, w, &V[(long)k * n]);
                 axpy(n, -HH(k, i), &V[(long)k * n], w);
@@ -743,7 +748,7 @@ int cro_gmres(cro_ctx* c, double* x, const double* b, int R, int maxiter, double
 #undef HH
     if (iters_out) *iters_out = iter;
     if (resid_out) *resid_out = resid;
-    free(V); free(w); free(H); free(s); free(cs); free(sn);
+    free(V); free(w); free(pw); free(H); free(s); free(cs); free(sn);
     return converged;
 }
 

@@ -251,6 +251,65 @@ def test_gmres_iteration_limit_and_full_gmres():
     assert info["iters"] == 0 and info["converged"]
 
 
+def _numpy_gmres(o, b, R, tol, maxiter):
+    """Restarted GMRES with left preconditioning and CUSP's stopping rule |s_{i+1}| <=
+    tol ||b||, written independently of the oracle with dense numpy steps (modified
+    Gram-Schmidt, Givens rotations), the preconditioner applied out of place."""
+    n = b.size
+    x = np.zeros(n)
+    it = 0
+    tolerance = tol * np.linalg.norm(b)
+    while True:
+        w = o.precondition(o.matvec(x) - b)
+        beta = np.linalg.norm(w)
+        if beta <= tolerance or it >= maxiter:
+            return x, it, beta
+        V = [-w / beta]
+        H = np.zeros((R + 1, R))
+        sv = np.zeros(R + 1)
+        sv[0] = beta
+        cs, sn = np.zeros(R), np.zeros(R)
+        for i in range(R):
+            it += 1
+            u = o.precondition(o.matvec(V[i]))
+            for k in range(i + 1):
+                H[k, i] = u @ V[k]
+                u = u - H[k, i] * V[k]
+            H[i + 1, i] = np.linalg.norm(u)
+            V.append(u / H[i + 1, i])
+            for k in range(i):
+                t = cs[k] * H[k, i] + sn[k] * H[k + 1, i]
+                H[k + 1, i] = -sn[k] * H[k, i] + cs[k] * H[k + 1, i]
+                H[k, i] = t
+            r = np.hypot(H[i, i], H[i + 1, i])
+            cs[i], sn[i] = H[i, i] / r, H[i + 1, i] / r
+            H[i, i], H[i + 1, i] = r, 0.0
+            sv[i + 1] = -sn[i] * sv[i]
+            sv[i] = cs[i] * sv[i]
+            if abs(sv[i + 1]) <= tolerance or it >= maxiter:
+                break
+        y = np.linalg.solve(H[:i + 1, :i + 1], sv[:i + 1])
+        x = x + sum(y[j] * V[j] for j in range(i + 1))
+        if abs(sv[i + 1]) <= tolerance:
+            return x, it, abs(sv[i + 1])
+
+
+@pytest.mark.parametrize("R", [5, 10])
+def test_gmres_iterations_match_independent_gmres(R):
+    """Same Arnoldi steps and stopping point as an independent restarted GMRES: a system
+    of rigid bodies whose potential rows the diagonal preconditioner divides by the body
+    size (an in-place preconditioner call that re-reads a row it has already scaled, or a
+    wrong restart residual, changes the count)."""
+    s = make_system(8, 8, 0.2, seed=12, charges="alternating", n_isolated_beads=3)
+    o = O.Oracle(s, 0.8, 1e-5, rc=3.5, mesh=(24, 24, 24), P=6)
+    b = o.objective(E0)
+    x, info = o.gmres(b, restart=R, maxiter=400, tol=1e-6)
+    xn, itn, resn = _numpy_gmres(o, b, R, 1e-6, 400)
+    assert info["converged"] and info["iters"] == itn and itn > R
+    assert abs(info["resid"] - resn) <= 1e-4 * resn
+    assert rel(x, xn) < 1e-10
+
+
 # ------------------------------------------------------------- direct Ewald
 @pytest.fixture(scope="module")
 def tiny():
