@@ -148,6 +148,7 @@ int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out, boo
     c->csr_ready = false;  // positions or neighbours may have changed
     c->fx_bound = false;  // the fixed-point weight bound follows the binning
     c->fxp = fx_plan(c->mp, c->tg, c->lds_max);
+    c->fxcp = fx_contract_plan(c->mp, c->tg, c->lds_max);
     if (bin && c->tg.valid) {
         // positions may have changed since the last call: rebin (cheap, O(N))
         const size_t nt = (size_t)c->tg.ntiles;
@@ -236,6 +237,12 @@ bool contract_tiled_ok(const cr_context* c, size_t elem)
 template <typename T>
 int contract_impl(cr_context* c, const SysView& v, T* phi, hipStream_t st)
 {
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (c->fp32() && c->tg.valid && c->fxcp.valid) {
+            CR_HIP(launch_contract_fx(c->mp, c->tg, c->fxcp, bin_view(c), c->grid_real.as<float>(), phi, st));
+            return CR_OK;
+        }
+    }
     if (contract_tiled_ok(c, sizeof(T)))
         CR_HIP(launch_contract_tiled<T>(c->mp, c->tg, v, bin_view(c), nullptr, c->grid_real.as<T>(), phi, nullptr,
                                         st));

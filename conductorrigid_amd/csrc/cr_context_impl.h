@@ -37,7 +37,7 @@ struct cr_context {
     // the weight bound matches the current binning, and whether the mesh holds int32
     // fixed-point values (the next FFT converts them)
     DevBuf fxs;
-    FxPlan fxp{};
+    FxPlan fxp{}, fxcp{};  // spread and potential-contraction LDS plans
     bool fx_bound = false;
     bool mesh_fixed = false;
     size_t lds_max = 65536;

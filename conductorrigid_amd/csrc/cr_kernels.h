@@ -177,6 +177,10 @@ hipError_t launch_spread_fx(const MeshParams& mp, const TileGeom& tg, const FxPl
 // fs->wbits for the binned configuration (once per bead configuration).
 hipError_t launch_spread_fx_bound(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
                                   FxScale* fs, hipStream_t st);
+// FP32-bulk potential contraction on the binned tiles (k_spread_fx.hip); its own LDS plan.
+FxPlan fx_contract_plan(const MeshParams& mp, const TileGeom& tg, size_t lds_max);
+hipError_t launch_contract_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
+                              const float* grid, float* phi, hipStream_t st);
 // out = in * fs->inv (in and out may alias).
 template <typename T>
 hipError_t launch_decode_fx(const int* in, T* out, long n, const FxScale* fs, hipStream_t st);

@@ -24,6 +24,7 @@
 // every spread by k_charges_fx, which also rounds the charges to float for the FP32-bulk
 // kernels and, in its last workgroup, writes m and the decode factor prefac qmax W / 2^30.
 #include "cr_kernels.h"
+#include "se_device.h"
 
 #include <map>
 #include <mutex>
@@ -260,6 +261,120 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx(MeshParams mp, TileGeo
     }
 }
 
+// FP32-bulk potential contraction (cppmodule/ConductorRigid.cu:260-360) on the same
+// binned tiles: phi_i = h^3 prefac sum_window w(node) grid(node).  The extended tile is
+// read into LDS once (float4 rows where the tile does not wrap in z), the chunk's axis
+// weights are staged as in the spread, and each wavefront reduces one bead's window
+// (lanes over the (y, z) plane, each walking x with compile-time LDS offsets).
+template <int PT, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kFxThreads) void k_contract_fx(MeshParams mp, TileGeom tg, int ch,
+                                                            const unsigned* __restrict__ tstart,
+                                                            const unsigned* __restrict__ sorted,
+                                                            const int* __restrict__ rec_off,
+                                                            const float4* __restrict__ rec_u,
+                                                            const float* __restrict__ grid, float* __restrict__ phi)
+{
+    constexpr bool CT = TX > 0;
+    constexpr int WS = fx_wstride(PT);
+    const int T0 = CT ? TX : tg.T[0], T1 = CT ? TY : tg.T[1], T2 = CT ? TZ : tg.T[2];
+    const int EX = T0 + PT - 1, EY = T1 + PT - 1, EZ = T2 + PT - 1;
+    const int ys = (EZ + 3) & ~3, xs = EY * ys, esz = EX * xs;
+    extern __shared__ __align__(16) char smem[];
+    const int t = fx_tile(blockIdx.x, tg.ntiles);
+    const unsigned kb = tstart[t], ke = tstart[t + 1];
+    if (kb == ke) return;
+    const int tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
+    const int x0 = tx * T0, y0 = ty * T1, z0 = tz * T2;
+    float* ext = reinterpret_cast<float*>(smem);
+    float* w = reinterpret_cast<float*>(smem + ((size_t)esz * 4 + 15) / 16 * 16);
+    int2* oi = reinterpret_cast<int2*>(w + (size_t)ch * WS);
+    const long plane = (long)mp.Ny * mp.Nz;
+    if (T2 % 4 == 0 && mp.Nz % 4 == 0 && z0 + EZ <= mp.Nz) {
+        // rows of the tile along z: whole float4s, then the remainder
+        const int nv = EZ / 4, per_row = nv + (EZ - 4 * nv), items = EX * EY * per_row;
+        for (int k = threadIdx.x; k < items; k += kFxThreads) {
+            const int r = k / per_row, c = k - r * per_row;
+            const int ly = r % EY, lx = r / EY;
+            int gx = x0 + lx, gy = y0 + ly;
+            gx -= gx >= mp.Nx ? mp.Nx : 0;
+            gy -= gy >= mp.Ny ? mp.Ny : 0;
+            const float* src = grid + (long)gx * plane + (long)gy * mp.Nz + z0;
+            float* dst = ext + lx * xs + ly * ys;
+            if (c < nv) {
+                *reinterpret_cast<float4*>(dst + 4 * c) = *reinterpret_cast<const float4*>(src + 4 * c);
+            } else {
+                const int z = 4 * nv + (c - nv);
+                dst[z] = src[z];
+            }
+        }
+    } else {
+        const int nodes = EX * EY * EZ;
+        FxWalk bw(threadIdx.x, kFxThreads, EY, EZ);
+        for (int k = threadIdx.x; k < nodes; k += kFxThreads, bw.next()) {
+            int gx = x0 + bw.lx, gy = y0 + bw.ly, gz = z0 + bw.lz;
+            gx -= gx >= mp.Nx ? mp.Nx : 0;
+            gy -= gy >= mp.Ny ? mp.Ny : 0;
+            gz -= gz >= mp.Nz ? mp.Nz : 0;
+            ext[bw.lx * xs + bw.ly * ys + bw.lz] = grid[(long)gx * plane + (long)gy * mp.Nz + gz];
+        }
+    }
+    const float c0 = (float)(mp.xiterm * mp.h[0] * mp.h[0] / mp.eta[0]);
+    const float c1 = (float)(mp.xiterm * mp.h[1] * mp.h[1] / mp.eta[1]);
+    const float c2 = (float)(mp.xiterm * mp.h[2] * mp.h[2] / mp.eta[2]);
+    const float pc = (float)(mp.quadW * mp.prefac);
+    const int lane = threadIdx.x & (kFxWave - 1), wave = threadIdx.x / kFxWave;
+    constexpr int NW = kFxThreads / kFxWave;
+    for (unsigned k0 = kb; k0 < ke; k0 += ch) {
+        const int cnt = (int)min((unsigned)ch, ke - k0);
+        for (int j = threadIdx.x; j < 3 * cnt; j += kFxThreads) {
+            const int i = j / 3, d = j - 3 * i;
+            const float4 u4 = rec_u[k0 + i];
+            const float ud = d == 0 ? u4.x : (d == 1 ? u4.y : u4.z);
+            const float cd = d == 0 ? c0 : (d == 1 ? c1 : c2);
+            float* wp = w + i * WS + d * PT;
+#pragma unroll
+            for (int p = 0; p < PT; ++p) {
+                const float dd = (float)p - ud;
+                wp[p] = __expf(-cd * dd * dd);
+            }
+        }
+        for (int i = threadIdx.x; i < cnt; i += kFxThreads) oi[i] = make_int2(rec_off[k0 + i], (int)sorted[k0 + i]);
+        __syncthreads();  // (the first one also orders the extended tile's stores)
+        for (int i = wave; i < cnt; i += NW) {
+            const int2 r = oi[i];
+            const int ox = r.x & 1023, oy = (r.x >> 10) & 1023, oz = r.x >> 20;
+            const float* wi = w + i * WS;
+            float wx[PT];
+            if constexpr (PT % 4 == 0) {
+#pragma unroll
+                for (int p = 0; p < PT; p += 4) {
+                    const float4 v4 = *reinterpret_cast<const float4*>(wi + p);
+                    wx[p] = v4.x;
+                    wx[p + 1] = v4.y;
+                    wx[p + 2] = v4.z;
+                    wx[p + 3] = v4.w;
+                }
+            } else {
+#pragma unroll
+                for (int p = 0; p < PT; ++p) wx[p] = wi[p];
+            }
+            float acc = 0.0f;
+#pragma unroll
+            for (int pair = lane; pair < PT * PT; pair += kFxWave) {
+                const int py = pair / PT, pz = pair - py * PT;
+                const float* col = ext + ox * xs + (oy + py) * ys + (oz + pz);
+                float sx = 0.0f;
+#pragma unroll
+                for (int px = 0; px < PT; ++px) sx += wx[px] * col[px * xs];
+                acc += wi[PT + py] * wi[2 * PT + pz] * sx;
+            }
+            acc = se::wave_total(acc);
+            if (lane == 0) phi[r.y] = acc * pc;
+        }
+        __syncthreads();
+    }
+}
+
 template <typename T>
 __global__ void k_decode_fx(const int* in, T* out, long n, const FxScale* __restrict__ fs)
 {
@@ -334,6 +449,39 @@ hipError_t dispatch_p(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl
     }
 }
 
+template <int PT, int TX, int TY, int TZ>
+hipError_t launch_contract_kernel(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
+                                  const float* grid, float* phi, hipStream_t st)
+{
+    auto fn = k_contract_fx<PT, TX, TY, TZ>;
+    allow_lds_fx(reinterpret_cast<const void*>(fn), pl.lds_bytes);
+    hipLaunchKernelGGL(fn, dim3(tg.ntiles), dim3(kFxThreads), pl.lds_bytes, st, mp, tg, pl.chunk, bv.tile_start,
+                       bv.sorted, bv.rec_off, static_cast<const float4*>(bv.rec_win), grid, phi);
+    return hipGetLastError();
+}
+
+template <int PT>
+hipError_t contract_shape(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
+                          const float* grid, float* phi, hipStream_t st)
+{
+    const int a = tg.T[0], b = tg.T[1], c = tg.T[2];
+#define CR_FXC_SHAPE(X, Y, Z) \
+    if (a == X && b == Y && c == Z) return launch_contract_kernel<PT, X, Y, Z>(mp, tg, pl, bv, grid, phi, st)
+    if constexpr (PT == 8) {
+        CR_FXC_SHAPE(16, 16, 16);
+        CR_FXC_SHAPE(16, 8, 16);
+        CR_FXC_SHAPE(8, 16, 16);
+        CR_FXC_SHAPE(16, 16, 32);
+        CR_FXC_SHAPE(8, 8, 8);
+    } else if constexpr (PT == 9) {
+        CR_FXC_SHAPE(16, 16, 16);
+        CR_FXC_SHAPE(16, 8, 16);
+        CR_FXC_SHAPE(8, 8, 8);
+    }
+#undef CR_FXC_SHAPE
+    return launch_contract_kernel<PT, 0, 0, 0>(mp, tg, pl, bv, grid, phi, st);
+}
+
 inline unsigned nblk(long n, int b)
 {
     const long v = (n + b - 1) / b;
@@ -362,6 +510,43 @@ FxPlan fx_plan(const MeshParams& mp, const TileGeom& tg, size_t lds_max)
         return p;
     }
     return p;
+}
+
+FxPlan fx_contract_plan(const MeshParams& mp, const TileGeom& tg, size_t lds_max)
+{
+    FxPlan p{};
+    if (!tg.valid || mp.P < 4 || mp.P > 10) return p;
+    const size_t EX = (size_t)tg.T[0] + mp.P - 1, EY = (size_t)tg.T[1] + mp.P - 1, EZ = (size_t)tg.T[2] + mp.P - 1;
+    const size_t ext = (EX * EY * ((EZ + 3) & ~(size_t)3) * 4 + 15) / 16 * 16;
+    const size_t per = (size_t)fx_wstride(mp.P) * 4 + 8;
+    for (const size_t budget : {lds_max / 2, lds_max}) {
+        if (ext >= budget) continue;
+        long ch = (long)((budget - ext) / per);
+        if (ch > kFxChunk) ch = kFxChunk;
+        ch -= ch % 4;
+        if (ch < (budget == lds_max ? 16 : 64)) continue;
+        p.chunk = (int)ch;
+        p.lds_bytes = ext + (size_t)ch * per;
+        p.valid = 1;
+        return p;
+    }
+    return p;
+}
+
+hipError_t launch_contract_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
+                              const float* grid, float* phi, hipStream_t st)
+{
+    if (!pl.valid) return hipErrorInvalidValue;
+    switch (mp.P) {
+        case 4: return contract_shape<4>(mp, tg, pl, bv, grid, phi, st);
+        case 5: return contract_shape<5>(mp, tg, pl, bv, grid, phi, st);
+        case 6: return contract_shape<6>(mp, tg, pl, bv, grid, phi, st);
+        case 7: return contract_shape<7>(mp, tg, pl, bv, grid, phi, st);
+        case 8: return contract_shape<8>(mp, tg, pl, bv, grid, phi, st);
+        case 9: return contract_shape<9>(mp, tg, pl, bv, grid, phi, st);
+        case 10: return contract_shape<10>(mp, tg, pl, bv, grid, phi, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_charges_fx(const double* x, int N, float* qf, FxScale* fs, double prefac, hipStream_t st)

@@ -131,3 +131,29 @@ def test_fx_spread_p9_batch_box(tile, require_gpu):
     vm = rel(g.matvec(x), o.matvec(x))
     _report(f"p9 tile {tile} matvec", vm)
     assert v < 1e-6 and vm < 1e-5
+
+
+@pytest.mark.parametrize("tile", [None, (16, 16, 16), (8, 8, 8), (16, 8, 16), (32, 16, 8)])
+def test_fx_contraction_matches_oracle(tile, require_gpu):
+    """FP32-bulk potential contraction (k_contract_fx): compile-time and run-time tile
+    shapes against the oracle's FP64 contraction of the same mesh (rel <= 1e-6)."""
+    s = n1k_system()
+    g = GpuCase(s, precision="fp32", **N1K)
+    if tile is not None:
+        g.cr.set_tiles(tile, 160 * 1024)
+    o = O.Oracle(s, N1K["xi"], N1K["errortol"], rc=N1K["rc"], mesh=N1K["mesh"], P=N1K["P"])
+    grid = o.spread(np.random.default_rng(29).normal(size=s.N))
+    og = o.recip_grid(grid)
+    gg, phi = g.contract_after_recip(grid)
+    v = rel(phi, o.contract(og))
+    _report(f"contract tile {tile}", v)
+    assert v < 1e-6
+
+
+def test_fx_contraction_odd_p(fx_case):
+    name, s, g, o = fx_case
+    grid = o.spread(np.random.default_rng(30).normal(size=s.N))
+    og = o.recip_grid(grid)
+    v = rel(g.contract_after_recip(grid)[1], o.contract(og))
+    _report(f"{name} contract", v)
+    assert v < 1e-6
