@@ -149,7 +149,7 @@ hipError_t launch_contract_tiled_batch(const TiledBox* boxes, int nbox, int tota
 // ---- k_spread_fx.hip: FP32-bulk spread in 32-bit fixed point ----
 // The mesh holds int32 node values in units of 1 / (mult / prefac); the first FFT pass
 // (or launch_decode_fx) converts them back to float with inv.
-constexpr int kFxParts = 256;   // workgroups of k_charges_fx (partial maxima)
+constexpr int kFxParts = 1024;  // workgroups of k_charges_fx (partial maxima)
 struct FxScale {
     float mult;       // charge -> fixed-point units: 2^30 / (qmax W)
     float inv;        // fixed-point units -> mesh value: prefac qmax W / 2^30 (NaN for a NaN charge)

@@ -39,23 +39,46 @@ __device__ __forceinline__ int tile_of_bead(const MeshParams& mp, const TileGeom
     return ((sx / tg.T[0]) * tg.nt[1] + sy / tg.T[1]) * tg.nt[2] + sz / tg.T[2];
 }
 
-__global__ void k_bin_count(MeshParams mp, TileGeom tg, int N, const double4* __restrict__ pos,
-                            unsigned* __restrict__ bead_tile, unsigned* __restrict__ count)
+// ctr[key] += 1 for every active lane, with one atomic per distinct key of the wavefront
+// (spatially ordered beads mostly share their tile); returns each lane's slot, the
+// counter's old value plus the lane's rank among the lanes of its key.
+__device__ __forceinline__ unsigned wave_count(unsigned* __restrict__ ctr, unsigned key, bool on)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const unsigned t = (unsigned)tile_of_bead(mp, tg, pos[i]);
-    bead_tile[i] = t;
-    atomicAdd(&count[t], 1u);
+    const int lane = threadIdx.x & 63;
+    unsigned long long todo = __ballot(on);
+    unsigned slot = 0;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const unsigned lk = __shfl(key, leader, 64);
+        const unsigned long long same = __ballot(on && key == lk) & todo;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&ctr[lk], (unsigned)__popcll(same));
+        base = __shfl(base, leader, 64);
+        if (on && key == lk) slot = base + (unsigned)__popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+    }
+    return slot;
 }
 
-__global__ void k_bin_fill(int N, const unsigned* __restrict__ bead_tile, const unsigned* __restrict__ start,
-                           unsigned* __restrict__ cursor, unsigned* __restrict__ sorted)
+__global__ __launch_bounds__(256) void k_bin_count(MeshParams mp, TileGeom tg, int N, const double4* __restrict__ pos,
+                                                   unsigned* __restrict__ bead_tile, unsigned* __restrict__ count)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const unsigned t = bead_tile[i];
-    sorted[start[t] + atomicAdd(&cursor[t], 1u)] = (unsigned)i;
+    const bool on = i < N;
+    const unsigned t = on ? (unsigned)tile_of_bead(mp, tg, pos[i]) : 0u;
+    if (on) bead_tile[i] = t;
+    (void)wave_count(count, t, on);
+}
+
+__global__ __launch_bounds__(256) void k_bin_fill(int N, const unsigned* __restrict__ bead_tile,
+                                                  const unsigned* __restrict__ start, unsigned* __restrict__ cursor,
+                                                  unsigned* __restrict__ sorted)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = i < N;
+    const unsigned t = on ? bead_tile[i] : 0u;
+    const unsigned slot = wave_count(cursor, t, on);
+    if (on) sorted[start[t] + slot] = (unsigned)i;
 }
 
 __global__ void k_zero_u32(unsigned* p, long n)
