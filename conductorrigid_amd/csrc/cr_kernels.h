@@ -155,7 +155,6 @@ struct FxScale {
     float inv;        // fixed-point units -> mesh value: prefac qmax W / 2^30 (NaN for a NaN charge)
     float qmax;       // max |q| of the last spread's charges
     unsigned wbits;   // float bits of the largest tile node of sum_b w_b (W = 8 x this)
-    unsigned ticket;  // last-workgroup counter of k_charges_fx, 0 between launches
     float part[kFxParts];
 };
 constexpr int kFxThreads = 1024;
@@ -168,8 +167,7 @@ struct FxPlan {
 // LDS plan for the context's tiling (lds_max: the device's LDS per workgroup); valid == 0
 // when the tile does not fit or P is outside 4..10.
 FxPlan fx_plan(const MeshParams& mp, const TileGeom& tg, size_t lds_max);
-// qf = (float) x [N]; max |qf| and, in the last workgroup, fs->mult and fs->inv from it and
-// fs->wbits.
+// qf = (float) x [N]; max |qf|, then fs->mult and fs->inv from it and fs->wbits.
 hipError_t launch_charges_fx(const double* x, int N, float* qf, FxScale* fs, double prefac, hipStream_t st);
 // grid (int32, G nodes) += fixed-point spread of qf; clear zeroes the mesh first.
 hipError_t launch_spread_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,

@@ -22,7 +22,7 @@
 // accumulates sum_b w_b for its own beads and reports its largest node; a node is covered
 // by at most 8 tiles (two per axis), so W = 8 x the largest tile node.  qmax is computed for
 // every spread by k_charges_fx, which also rounds the charges to float for the FP32-bulk
-// kernels and, in its last workgroup, writes m and the decode factor prefac qmax W / 2^30.
+// kernels; k_fx_scale then writes m and the decode factor prefac qmax W / 2^30.
 #include "cr_kernels.h"
 #include "se_device.h"
 
@@ -63,11 +63,9 @@ __device__ __forceinline__ float wave_nanmax(float m)
 }
 
 __global__ __launch_bounds__(kFxChargeThreads) void k_charges_fx(const double* __restrict__ x, int N,
-                                                                 float* __restrict__ qf, FxScale* __restrict__ fs,
-                                                                 double prefac)
+                                                                 float* __restrict__ qf, FxScale* __restrict__ fs)
 {
     __shared__ float wmax[kFxChargeThreads / kFxWave];
-    __shared__ int last;
     float m = 0.0f;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         const float v = (float)x[i];
@@ -82,35 +80,33 @@ __global__ __launch_bounds__(kFxChargeThreads) void k_charges_fx(const double* _
         float bm = wmax[0];
         for (int k = 1; k < kFxChargeThreads / kFxWave; ++k) bm = nanmax(bm, wmax[k]);
         fs->part[blockIdx.x] = bm;
-        // release this block's partial, acquire the others' if this is the last block
-        const unsigned ticket = __hip_atomic_fetch_add(&fs->ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = ticket == gridDim.x - 1;
     }
-    __syncthreads();
-    if (!last || wave != 0) return;
-    __threadfence();
+}
+
+// One wavefront: qmax from the partial maxima, then the multiplier and decode factor.
+__global__ __launch_bounds__(kFxWave) void k_fx_scale(int nparts, FxScale* __restrict__ fs, double prefac)
+{
+    const int lane = threadIdx.x;
     float qm = 0.0f;
-    for (int k = lane; k < (int)gridDim.x; k += kFxWave) qm = nanmax(qm, fs->part[k]);
+    for (int k = lane; k < nparts; k += kFxWave) qm = nanmax(qm, fs->part[k]);
     qm = wave_nanmax(qm);
-    if (lane == 0) {
-        const double W = 8.0 * (double)__uint_as_float(fs->wbits);
-        const double qW = (double)qm * W;
-        float mult, inv;
-        if (qm != qm || !(qm <= 3.0e38f)) {  // NaN or infinite charge: decode to NaN
-            mult = 0.0f;
-            inv = __int_as_float(0x7fc00000);
-        } else if (!(qW > 1e-30)) {          // all charges (or all weights) zero
-            mult = 0.0f;
-            inv = 0.0f;
-        } else {
-            mult = (float)(1073741824.0 / qW);
-            inv = (float)(prefac * qW / 1073741824.0);
-        }
-        fs->mult = mult;
-        fs->inv = inv;
-        fs->qmax = qm;
-        fs->ticket = 0u;
+    if (lane != 0) return;
+    const double W = 8.0 * (double)__uint_as_float(fs->wbits);
+    const double qW = (double)qm * W;
+    float mult, inv;
+    if (qm != qm || !(qm <= 3.0e38f)) {  // NaN or infinite charge: decode to NaN
+        mult = 0.0f;
+        inv = __int_as_float(0x7fc00000);
+    } else if (!(qW > 1e-30)) {          // all charges (or all weights) zero
+        mult = 0.0f;
+        inv = 0.0f;
+    } else {
+        mult = (float)(1073741824.0 / qW);
+        inv = (float)(prefac * qW / 1073741824.0);
     }
+    fs->mult = mult;
+    fs->inv = inv;
+    fs->qmax = qm;
 }
 
 // Node walk over an EX x EY x EZ box in row-major order with carries (no divisions).
@@ -552,7 +548,8 @@ hipError_t launch_contract_fx(const MeshParams& mp, const TileGeom& tg, const Fx
 hipError_t launch_charges_fx(const double* x, int N, float* qf, FxScale* fs, double prefac, hipStream_t st)
 {
     const unsigned blocks = std::min<unsigned>(kFxParts, nblk(std::max(N, 1), kFxChargeThreads));
-    hipLaunchKernelGGL(k_charges_fx, dim3(blocks), dim3(kFxChargeThreads), 0, st, x, N, qf, fs, prefac);
+    hipLaunchKernelGGL(k_charges_fx, dim3(blocks), dim3(kFxChargeThreads), 0, st, x, N, qf, fs);
+    hipLaunchKernelGGL(k_fx_scale, dim3(1), dim3(kFxWave), 0, st, (int)blocks, fs, prefac);
     return hipGetLastError();
 }
 
