@@ -17,6 +17,7 @@ library through its C ABI.
 from __future__ import annotations
 
 import ctypes as C
+import json
 import math
 import os
 
@@ -237,25 +238,41 @@ class ConductorRigid:
         _lib.check(self.lib.cr_set_tiles(self.ctx, (C.c_int * 3)(*[int(v) for v in tile]), int(lds_bytes)))
 
     def OutputData(self, timestep: int) -> str:
-        """Text dump in tag order (cppmodule/ConductorRigid.cc:560-622)."""
+        """Text dump in tag order (cppmodule/ConductorRigid.cc:560-622); returns the file name."""
         s = self.system
-        rtag = np.empty(s.N, dtype=np.int64)
-        rtag[s.tag.astype(np.int64)] = np.arange(s.N)
-        pos = s.pos[rtag]
-        q = self.charges.cpu().numpy()[:s.N][rtag]
-        F = self.forces.cpu().numpy()[:s.N][rtag]
-        fn = f"{self.fileprefix}.{int(timestep):010d}.txt"
-        with open(fn, "w") as f:
-            f.write("Bead Position\n")
-            for p in pos:
-                f.write(f"{p[0]:.10g}  {p[1]:.10g}  {p[2]:.10g}  \n")
-            f.write("Bead Charge\n")
-            for v in q:
-                f.write(f"{v:.10g}\n")
-            f.write("Bead Force\n")
-            for v in F:
-                f.write(f"{v[0]:.10g}  {v[1]:.10g}  {v[2]:.10g}  \n")
+        fn = output_filename(self.fileprefix, timestep)
+        write_output(fn, s.tag, s.pos, self.charges.cpu().numpy()[:s.N], self.forces.cpu().numpy()[:s.N])
         return fn
+
+    # --- checkpoint ---
+    def save_state(self, path: str) -> None:
+        """Write the solver state that carries across steps (bead charges and body
+        potentials, the previous step's x0), the external field and the parameters to an
+        .npz file.  The reference keeps the charges in HOOMD's particle data but loses the
+        body potentials (cppmodule/ConductorRigid.h:67) across a restart; both are kept."""
+        meta = {"N": int(self.system.N), "N_isolated": int(self.system.N_isolated), "xi": float(self.xi),
+                "errortol": float(self.errortol), "params": {k: v for k, v in self.params.items()
+                                                            if isinstance(v, (int, float, list))}}
+        np.savez(path, charges=self.get_charges(), potentials=self.get_potentials(),
+                 field=np.asarray(self.field, dtype=np.float64), L=np.asarray(self.system.L, dtype=np.float64),
+                 meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+
+    def load_state(self, path: str) -> None:
+        """Restore a state written by save_state (same system size and method parameters);
+        the next computeForces(x0="previous") starts from it."""
+        with np.load(path, allow_pickle=False) as f:
+            meta = json.loads(bytes(f["meta"]).decode())
+            q, phi, field = f["charges"], f["potentials"], f["field"]
+        if meta["N"] != self.system.N or meta["N_isolated"] != self.system.N_isolated:
+            raise ValueError("checkpoint is for a system of another size")
+        for k in ("mesh", "P", "rc"):
+            if meta["params"].get(k) != self.params.get(k):
+                raise ValueError(f"checkpoint parameter {k} differs: {meta['params'].get(k)} vs {self.params.get(k)}")
+        t = self.torch
+        self.charges[:len(q)].copy_(t.from_numpy(np.ascontiguousarray(q, dtype=np.float64)))
+        if len(phi):
+            self.potentials[:len(phi)].copy_(t.from_numpy(np.ascontiguousarray(phi, dtype=np.float64)))
+        self.update_field(tuple(float(v) for v in field))
 
     # --- numpy views ---
     def get_charges(self) -> np.ndarray:
@@ -287,6 +304,36 @@ def update_positions(cr: ConductorRigid, pos: np.ndarray, rigid_com: np.ndarray 
     cr.dsys.build_nlist()
 
 
+
+
+def output_filename(prefix: str, timestep: int) -> str:
+    """``prefix.%010d.txt`` (cppmodule/ConductorRigid.cc:563-567)."""
+    return f"{prefix}.{int(timestep):010d}.txt"
+
+
+def write_output(filename: str, tag, pos, q, F) -> None:
+    """OutputData's text format (cppmodule/ConductorRigid.cc:560-622): sections "Bead
+    Position", "Bead Charge", "Bead Force", each in particle-tag order (global index
+    rtag[t] for tag t), numbers as an ostream with setprecision(10) writes them (printf
+    %.10g), vector components separated and followed by two spaces.  pos and F are
+    indexed by global index, tag[i] is the tag of global index i."""
+    tag = np.asarray(tag, dtype=np.int64)
+    n = tag.size
+    rtag = np.empty(n, dtype=np.int64)
+    rtag[tag] = np.arange(n)
+    pos = np.asarray(pos, dtype=np.float64)[rtag]
+    q = np.asarray(q, dtype=np.float64)[rtag]
+    F = np.asarray(F, dtype=np.float64)[rtag]
+    with open(filename, "w") as f:
+        f.write("Bead Position\n")
+        for p in pos:
+            f.write(f"{p[0]:.10g}  {p[1]:.10g}  {p[2]:.10g}  \n")
+        f.write("Bead Charge\n")
+        for v in q:
+            f.write(f"{v:.10g}\n")
+        f.write("Bead Force\n")
+        for v in F:
+            f.write(f"{v[0]:.10g}  {v[1]:.10g}  {v[2]:.10g}  \n")
 
 
 class BatchConductorRigid:
@@ -350,4 +397,5 @@ def batch_boxes(items, precision: str = "fp32", preconditioner: str = "block", r
     return out
 
 
-__all__ = ["ConductorRigid", "BatchConductorRigid", "DeviceSystem", "batch_boxes", "update_positions", "NOT_MEMBER"]
+__all__ = ["ConductorRigid", "BatchConductorRigid", "DeviceSystem", "batch_boxes", "update_positions", "NOT_MEMBER",
+           "output_filename", "write_output"]
