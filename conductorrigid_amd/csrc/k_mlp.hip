@@ -21,6 +21,7 @@
 // D[4 (l / 16) + r][l % 16], r = 0..3.
 #include "cr_kernels.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace cr {
@@ -44,10 +45,10 @@ __device__ __forceinline__ unsigned short f2bf(float f)
     const unsigned u = __float_as_uint(f);
     return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);  // round to nearest even
 }
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ float dsilu(float z)
 {
-    const float s = 1.0f / (1.0f + __expf(-z));
+    const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-z));
     return s * (1.0f + z * (1.0f - s));
 }
 
@@ -85,7 +86,14 @@ __device__ __forceinline__ void row_gate(const unsigned short* __restrict__ xr, 
 
 // Minimum-image difference and squared distance without contraction into fused
 // multiply-adds, so that the CPU reference reproduces the neighbour order bit for bit.
-__device__ __forceinline__ double mi(double d, double L) { return __dadd_rn(d, -__dmul_rn(L, rint(d / L))); }
+// The image count is rint(d * (1 / L)), which differs from rint(d / L) only where d / L
+// lies within rounding of a half-integer; a pair closer than half the box (every
+// neighbour-list pair, every bead and its body's centre) never does, so the result is
+// bit-identical to d - L rint(d / L) there without a double division per component.
+__device__ __forceinline__ double mi_inv(double d, double L, double invL)
+{
+    return __dadd_rn(d, -__dmul_rn(L, rint(d * invL)));
+}
 __device__ __forceinline__ float dist2(double dx, double dy, double dz)
 {
     return (float)__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
@@ -123,6 +131,7 @@ __global__ __launch_bounds__(kFeatBlock) void k_mlp_features(SysView s, double L
     __shared__ unsigned short rows[kFeatBlock * kFeatPitch];
     constexpr int K = kMlpNeigh;
     const int t = threadIdx.x;
+    const double iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
     const long b0 = (long)blockIdx.x * kFeatBlock;
     const long i = b0 + t;
     const bool live = i < s.N;
@@ -153,7 +162,7 @@ __global__ __launch_bounds__(kFeatBlock) void k_mlp_features(SysView s, double L
             for (int u = 0; u < kFeatBatch; ++u) {
                 if (n0 + u >= nn) break;
                 const double4 rj = rb[u];
-                const float d2 = dist2(mi(ri.x - rj.x, Lx), mi(ri.y - rj.y, Ly), mi(ri.z - rj.z, Lz));
+                const float d2 = dist2(mi_inv(ri.x - rj.x, Lx, iLx), mi_inv(ri.y - rj.y, Ly, iLy), mi_inv(ri.z - rj.z, Lz, iLz));
                 // in range, and closer than the current K-th (an empty slot holds +inf)
                 if (!(d2 < r2max) || !(d2 < dist[K - 1])) continue;
                 float cd = d2;
@@ -187,9 +196,9 @@ __global__ __launch_bounds__(kFeatBlock) void k_mlp_features(SysView s, double L
                 float f0 = 0, f1 = 0, f2 = 0, f3 = 0, f4 = 0;
                 if (k < cnt) {
                     const double4 rj = s.pos[idx[k]];
-                    f0 = __fmul_rn((float)mi(rj.x - ri.x, Lx), inv);
-                    f1 = __fmul_rn((float)mi(rj.y - ri.y, Ly), inv);
-                    f2 = __fmul_rn((float)mi(rj.z - ri.z, Lz), inv);
+                    f0 = __fmul_rn((float)mi_inv(rj.x - ri.x, Lx, iLx), inv);
+                    f1 = __fmul_rn((float)mi_inv(rj.y - ri.y, Ly, iLy), inv);
+                    f2 = __fmul_rn((float)mi_inv(rj.z - ri.z, Lz, iLz), inv);
                     f3 = (s.iso[idx[k]] == bi) ? 1.0f : -1.0f;
                     f4 = __fmul_rn(__fsqrt_rn(dist[k]), inv);
                 }
@@ -207,9 +216,9 @@ __global__ __launch_bounds__(kFeatBlock) void k_mlp_features(SysView s, double L
                 const unsigned r = s.rigid_membership[i];
                 if (r != NOT_MEMBER) {
                     const double4 c = s.rigid_com[r];
-                    cx = __fmul_rn((float)mi(ri.x - c.x, Lx), 0.25f);
-                    cy = __fmul_rn((float)mi(ri.y - c.y, Ly), 0.25f);
-                    cz = __fmul_rn((float)mi(ri.z - c.z, Lz), 0.25f);
+                    cx = __fmul_rn((float)mi_inv(ri.x - c.x, Lx, iLx), 0.25f);
+                    cy = __fmul_rn((float)mi_inv(ri.y - c.y, Ly, iLy), 0.25f);
+                    cz = __fmul_rn((float)mi_inv(ri.z - c.z, Lz, iLz), 0.25f);
                 }
                 put(x, lo, o++, f2bf(cx));
                 put(x, lo, o++, f2bf(cy));
@@ -363,6 +372,178 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp_forward(int M, const unsign
                 y[m] = ((g[0] * u[0] + g[1] * u[1]) + g[2] * u[2]) + g[3] * u[3];
             }
         }
+}
+
+// ---- inference (persistent, gfx950 MFMA 16x16x32, chained layers) -------------
+// Every layer is computed transposed, D[n][m] = sum_k W[n][k] act[m][k], with the weights
+// as the A operand (from LDS) and the activations as the B operand (registers).  In
+// v_mfma_f32_16x16x32_bf16 lane l holds 8 values of A row l % 16 and of B column l % 16
+// (slots j = 0..7 of lane group l / 16; A and B pair the same slots, so any assignment
+// of k to (group, slot) that both operands share gives the full sum), and receives
+// D[4 (l / 16) + r][l % 16], r = 0..3.  A lane therefore leaves a layer holding row
+// m = l % 16 at columns n = 16 nt + 4 (l / 16) + r: for the 32-wide k chunk kc of the next
+// layer, slots 0..3 come from nt = 2 kc and slots 4..7 from nt = 2 kc + 1, i.e.
+//   k(kc, g, j) = 32 kc + 16 (j / 4) + 4 g + j % 4,
+// and the next layer's weights are staged in LDS with their columns in that order, so
+// activations never leave registers.  Layer 3 (4 heads) is one more MFMA chain with rows
+// 4..15 of its A operand zero; its D rows 0..3 are the heads of row m in lanes 0..15.
+// Each workgroup stages the weights once (78 KB) and its wavefronts loop over tiles of
+// 32 rows.  Rounding points as k_mlp_forward: bf16
+// operands and activations, FP32 sums (the bias starts the sum here).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float fl2 __attribute__((ext_vector_type(2)));
+constexpr int kInferWaves = 12;
+constexpr int kInferT = 2;              // 16-row tiles per wavefront pass
+constexpr int kW1Pitch = TRUNK;         // ushorts per LDS row of W1 (336 B; conflict-free 16-byte reads)
+constexpr int kW2Pitch = H + 8;         // ushorts per LDS row of the permuted W2 (272 B)
+constexpr int KC1 = (TRUNK + 31) / 32;  // 6 chunks of 32 columns (trunk 168, zero above)
+constexpr int KC2 = H / 32;             // 4
+constexpr size_t kInferLdsW1 = sizeof(unsigned short) * H * kW1Pitch;
+constexpr size_t kInferLdsW2 = sizeof(unsigned short) * H * kW2Pitch;
+constexpr size_t kInferLdsW3 = sizeof(unsigned short) * NO * H;
+constexpr size_t kInferLds = kInferLdsW1 + kInferLdsW2 + kInferLdsW3 + 2 * sizeof(float) * H;
+static_assert(kInferLds <= 160 * 1024, "inference weights fit one CU's LDS");
+static_assert(NO == 4 && H % 32 == 0 && TRUNK % 8 == 0, "inference tiling");
+
+__device__ __forceinline__ bf16x8 ld8(const unsigned short* p)
+{
+    const uint4 v = *reinterpret_cast<const uint4*>(p);
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ unsigned pack_bf16(float a, float b)
+{
+    const fl2 v = {a, b};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));  // round to nearest even
+}
+
+// column of the permuted weight row that holds original column n (the k order above)
+__device__ __forceinline__ int perm_col(int n)
+{
+    const int kc = n >> 5, w = n & 31, hi = w >> 4, g = (w >> 2) & 3, j = w & 3;
+    return 32 * kc + 8 * g + 4 * hi + j;
+}
+
+// activations of one 16-row tile for chunk kc from the accumulators of nt = 2 kc, 2 kc + 1
+__device__ __forceinline__ bf16x8 act8(f32x4 lo, f32x4 hi)
+{
+    const uint4 u = make_uint4(pack_bf16(silu(lo[0]), silu(lo[1])), pack_bf16(silu(lo[2]), silu(lo[3])),
+                               pack_bf16(silu(hi[0]), silu(hi[1])), pack_bf16(silu(hi[2]), silu(hi[3])));
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(int M, const unsigned short* __restrict__ X,
+                                                                   MlpDevWeights w, float* __restrict__ y)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned short* W1s = reinterpret_cast<unsigned short*>(smem);
+    unsigned short* W2s = reinterpret_cast<unsigned short*>(smem + kInferLdsW1);
+    unsigned short* W3s = reinterpret_cast<unsigned short*>(smem + kInferLdsW1 + kInferLdsW2);
+    float* b1s = reinterpret_cast<float*>(smem + kInferLdsW1 + kInferLdsW2 + kInferLdsW3);
+    float* b2s = b1s + H;
+    // stage: W1 trunk columns as they are; W2 and W3^T with their columns permuted
+    for (int k = threadIdx.x; k < H * (TRUNK / 8); k += blockDim.x) {
+        const int n = k / (TRUNK / 8), c = k - n * (TRUNK / 8);
+        *reinterpret_cast<uint4*>(W1s + n * kW1Pitch + 8 * c) = *reinterpret_cast<const uint4*>(w.W1t + (long)n * KIN + 8 * c);
+    }
+    for (int k = threadIdx.x; k < H * (H / 4); k += blockDim.x) {
+        const int n = k / (H / 4), c = 4 * (k - n * (H / 4));
+        *reinterpret_cast<uint2*>(W2s + n * kW2Pitch + perm_col(c)) = *reinterpret_cast<const uint2*>(w.W2t + (long)n * H + c);
+    }
+    for (int k = threadIdx.x; k < NO * H; k += blockDim.x) {
+        const int o = k / H, n = k - o * H;
+        W3s[o * H + perm_col(n)] = w.W3[n * NO + o];
+    }
+    for (int k = threadIdx.x; k < H; k += blockDim.x) {
+        b1s[k] = w.b1[k];
+        b2s[k] = w.b2[k];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    const bf16x8 zero8 = __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
+    const float4 b3 = make_float4(w.b3[0], w.b3[1], w.b3[2], w.b3[3]);
+    const long ngroups = ((long)M + 16 * kInferT - 1) / (16 * kInferT);
+    for (long gi = (long)blockIdx.x * kInferWaves + wave; gi < ngroups; gi += (long)gridDim.x * kInferWaves) {
+        const long m0 = gi * 16 * kInferT;
+        // an offset the compiler cannot see through, so the weight fragments are read from
+        // LDS on every pass instead of being hoisted out of the loop (336 registers)
+        int wo = 0;
+        asm volatile("" : "+v"(wo));
+        // layer 1, k chunk by k chunk: B = trunk columns of the tile rows (rows past M repeat
+        // the last one and are never stored), all 8 column tiles accumulated at once
+        f32x4 acc1[NT][kInferT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const f32x4 bias = *reinterpret_cast<const f32x4*>(b1s + 16 * nt + 4 * lq);
+#pragma unroll
+            for (int t = 0; t < kInferT; ++t) acc1[nt][t] = bias;
+        }
+#pragma unroll
+        for (int kc = 0; kc < KC1; ++kc) {
+            const int col = kc * 32 + 8 * lq;
+            bf16x8 xb[kInferT];
+#pragma unroll
+            for (int t = 0; t < kInferT; ++t) {
+                const long row = min(m0 + 16 * t + lr, (long)M - 1);
+                xb[t] = col < TRUNK ? ld8(X + row * KIN + col) : zero8;
+            }
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const bf16x8 a = col < TRUNK ? ld8(W1s + wo + (16 * nt + lr) * kW1Pitch + col) : zero8;
+#pragma unroll
+                for (int t = 0; t < kInferT; ++t) acc1[nt][t] = mfma32(a, xb[t], acc1[nt][t]);
+            }
+        }
+        // h1 in B-operand form
+        bf16x8 h1[kInferT][KC2];
+#pragma unroll
+        for (int kc = 0; kc < KC2; ++kc)
+#pragma unroll
+            for (int t = 0; t < kInferT; ++t) h1[t][kc] = act8(acc1[2 * kc][t], acc1[2 * kc + 1][t]);
+        // layer 2 -> h2 chunks, each fed at once to layer 3
+        f32x4 acc3[kInferT];
+#pragma unroll
+        for (int t = 0; t < kInferT; ++t) acc3[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int nt = 0; nt < NT; nt += 2) {
+            f32x4 acc[2][kInferT];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const f32x4 bias = *reinterpret_cast<const f32x4*>(b2s + 16 * (nt + u) + 4 * lq);
+#pragma unroll
+                for (int t = 0; t < kInferT; ++t) acc[u][t] = bias;
+#pragma unroll
+                for (int kc = 0; kc < KC2; ++kc) {
+                    const bf16x8 a = ld8(W2s + wo + (16 * (nt + u) + lr) * kW2Pitch + kc * 32 + 8 * lq);
+#pragma unroll
+                    for (int t = 0; t < kInferT; ++t) acc[u][t] = mfma32(a, h1[t][kc], acc[u][t]);
+                }
+            }
+            const bf16x8 a3 = lr < NO ? ld8(W3s + wo + lr * H + (nt / 2) * 32 + 8 * lq) : zero8;
+#pragma unroll
+            for (int t = 0; t < kInferT; ++t) acc3[t] = mfma32(a3, act8(acc[0][t], acc[1][t]), acc3[t]);
+        }
+        // heads of row m0 + 16 t + lr in lanes 0..15: y = g . (u + b3)
+        if (lq == 0) {
+#pragma unroll
+            for (int t = 0; t < kInferT; ++t) {
+                const long m = m0 + 16 * t + lr;
+                if (m < M) {
+                    float g[NO];
+                    row_gate(X + m * KIN, g);
+                    y[m] = ((g[0] * (acc3[t][0] + b3.x) + g[1] * (acc3[t][1] + b3.y)) + g[2] * (acc3[t][2] + b3.z)) +
+                           g[3] * (acc3[t][3] + b3.w);
+                }
+            }
+        }
+    }
 }
 
 // ---- backward, per row ------------------------------------------------------
@@ -614,6 +795,33 @@ hipError_t launch_mlp_pack(const float* theta, const MlpDevWeights& w, hipStream
     return hipGetLastError();
 }
 
+// Persistent inference: one workgroup of kInferWaves wavefronts per CU (3 per SIMD at the
+// kernel's 138 registers; its LDS holds the weights), capped by the row tiles there are.
+hipError_t launch_mlp_infer(int M, const unsigned short* X, const MlpDevWeights& w, float* y, hipStream_t st)
+{
+    if (M <= 0) return hipSuccess;
+    static int cus[64] = {};
+    static bool lds_set[64] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    if (!cus[dev]) {
+        e = hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+    }
+    if (!lds_set[dev]) {
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mlp_infer), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kInferLds);
+        if (e != hipSuccess) return e;
+        lds_set[dev] = true;
+    }
+    const long groups = ((long)M + 16 * kInferT * kInferWaves - 1) / (16 * kInferT * kInferWaves);
+    const unsigned blocks = (unsigned)std::max(1L, std::min<long>(groups, (long)cus[dev]));
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(64 * kInferWaves), kInferLds, st, M, X, w, y);
+    return hipGetLastError();
+}
+
 hipError_t launch_mlp_forward(int M, const unsigned short* X, const MlpDevWeights& w, float* y, const MlpTrainBufs* tb,
                               hipStream_t st)
 {
@@ -622,9 +830,7 @@ hipError_t launch_mlp_forward(int M, const unsigned short* X, const MlpDevWeight
         hipLaunchKernelGGL((k_mlp_forward<true, kFwdTilesTrain>), dim3(nblk(M, 16 * kFwdTilesTrain * kWaves)),
                            dim3(64 * kWaves), 0, st, M, X, w, y, tb->z1, tb->z2, tb->h1, tb->h2);
     else
-        hipLaunchKernelGGL((k_mlp_forward<false, kFwdTilesInfer>), dim3(nblk(M, 16 * kFwdTilesInfer * kWaves)),
-                           dim3(64 * kWaves), 0, st, M, X, w, y, (float*)nullptr, (float*)nullptr,
-                           (unsigned short*)nullptr, (unsigned short*)nullptr);
+        return launch_mlp_infer(M, X, w, y, st);
     return hipGetLastError();
 }
 
