@@ -41,14 +41,14 @@ HBM_PEAK_GBS = 8000.0
 
 # matvec stage -> kernel it times (FP32-bulk, P = 8)
 STAGE_KERNEL = {
-    "mesh_clear": "k_fill16",
-    "spread": "k_spread_fx<8, 16, 16, 16, false>",
+    "mesh_clear": "k_charges_fx + k_fx_scale",
+    "spread": "k_spread_fx<8, 16, 16, 16, false> + k_fx_gather<4>",
     "fft_z_r2c": "k_fft_z_r2c<float>",
     "fft_y_fwd": "k_fft_strided<float>",
     "fft_x_scale": "k_fft_x_scale<float>",
     "fft_y_inv": "k_fft_strided<float>",
     "fft_z_c2r": "k_fft_z_c2r<float>",
-    "contract": "k_contract_tiled<float, false, 8>",
+    "contract": "k_contract_fx<8, 16, 16, 16>",
     "real_space": "k_real_space_csr<float>",
     "constraint_rows": "k_sum_charges + k_set_charge",
 }
@@ -79,7 +79,10 @@ def algorithmic_bytes(stage: str, N: int, G: int, Gh: int, nn: float, B: int, t:
     algorithmic bytes: its time is overhead of the method."""
     rec = 4 * t
     return {
-        "mesh_clear": 0.0,  # clearing before the atomic accumulation is not algorithmic traffic
+        # FP32 bulk: the fixed-point charge rounding (x in, float charges out); the
+        # fixed-point spread writes every node, so there is no clear.  A clear before an
+        # atomic accumulation (FP64) is not algorithmic traffic.
+        "mesh_clear": 12.0 * N if t == 4 else 0.0,
         "spread": rec * N + t * G,
         "fft_z_r2c": t * G + 2 * t * Gh,
         "fft_y_fwd": 4 * t * Gh,
@@ -241,11 +244,25 @@ FX_SHAPES = {8: {(16, 16, 16), (16, 8, 16), (16, 16, 32), (32, 16, 16), (16, 32,
              9: {(16, 16, 16), (16, 8, 16), (8, 8, 8)}}
 
 
-def spread_kernel_name(P: int, tile) -> str:
+FXC_SHAPES = {8: {(16, 16, 16), (16, 8, 16), (8, 16, 16), (16, 16, 32), (8, 8, 8)},
+              9: {(16, 16, 16), (16, 8, 16), (8, 8, 8)}}
+
+
+def spread_kernel_name(P: int, tile, mesh) -> str:
+    """The FP32-bulk spread stage: the tile kernel, then the gather of its blocks."""
     t = tuple(int(v) for v in tile)
-    if t in FX_SHAPES.get(P, ()):
-        return f"k_spread_fx<{P}, {t[0]}, {t[1]}, {t[2]}, false>"
-    return f"k_spread_fx<{P}, 0, 0, 0, false>"
+    k = f"k_spread_fx<{P}, {t[0]}, {t[1]}, {t[2]}, false>" if t in FX_SHAPES.get(P, ()) else \
+        f"k_spread_fx<{P}, 0, 0, 0, false>"
+    ys = (t[2] + P) & ~1  # block row pitch
+    v = 4 if t[2] % 4 == 0 and ys % 4 == 0 and int(mesh[2]) % 4 == 0 else 1
+    return f"{k} + k_fx_gather<{v}>"
+
+
+def contract_kernel_name(P: int, tile) -> str:
+    t = tuple(int(v) for v in tile)
+    if t in FXC_SHAPES.get(P, ()):
+        return f"k_contract_fx<{P}, {t[0]}, {t[1]}, {t[2]}>"
+    return f"k_contract_fx<{P}, 0, 0, 0>"
 
 
 def direct_ewald_errors() -> dict:
@@ -383,7 +400,8 @@ def main() -> int:
     Gh = mesh[0] * mesh[1] * (mesh[2] // 2 + 1)
     nn = cr32.dsys.n_pairs / sysdef.N
     stage_kernel = dict(STAGE_KERNEL)
-    stage_kernel["spread"] = spread_kernel_name(P_used, tune32["tile"])
+    stage_kernel["spread"] = spread_kernel_name(P_used, tune32["tile"], mesh)
+    stage_kernel["contract"] = contract_kernel_name(P_used, tune32["tile"])
     stages = {}
     for k, name in enumerate(_lib.STAGE_NAMES):
         b = algorithmic_bytes(name, sysdef.N, G, Gh, nn, sysdef.N_isolated, 4)

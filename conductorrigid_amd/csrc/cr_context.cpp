@@ -193,15 +193,6 @@ const AxisPlan* plans(const cr_context* c)
         return c->plan_f;
 }
 
-// The FFT's first pass reads the fixed-point mesh when the last spread left one.
-template <typename T>
-const FxScale* mesh_fx(cr_context* c)
-{
-    const FxScale* fx = (sizeof(T) == sizeof(float) && c->mesh_fixed) ? c->fxs.as<FxScale>() : nullptr;
-    c->mesh_fixed = false;
-    return fx;
-}
-
 // grid <- Re IFFT(S FFT(grid)) on the context mesh (cppmodule/ConductorRigid.cu:927-933).
 template <typename T>
 int recip_impl(cr_context* c, hipStream_t st)
@@ -210,7 +201,7 @@ int recip_impl(cr_context* c, hipStream_t st)
     const int Nx = c->mp.Nx, Ny = c->mp.Ny;
     T* real = c->grid_real.as<T>();
     T* cp = c->grid_cplx.as<T>();
-    CR_HIP(launch_fft_z_r2c<T>(pl[2], Nx, Ny, real, cp, st, mesh_fx<T>(c)));
+    CR_HIP(launch_fft_z_r2c<T>(pl[2], Nx, Ny, real, cp, st));
     CR_HIP(launch_fft_yx<T>(pl[1], 1, Nx, Ny, c->Nzh, cp, -1, st));
     CR_HIP(launch_fft_x_scale<T>(pl[0], Nx, Ny, c->Nzh, cp, c->sc, st));
     CR_HIP(launch_fft_yx<T>(pl[1], 1, Nx, Ny, c->Nzh, cp, +1, st));
@@ -315,13 +306,13 @@ int charges_f32(cr_context* c, const SysView& v, const double* x, hipStream_t st
     return CR_OK;
 }
 
-// Fixed-point spread of c->qf at the scale charges_f32 set.
-int spread_fx(cr_context* c, hipStream_t st, bool clear)
+// Fixed-point spread of c->qf at the scale charges_f32 set; writes every mesh node (float).
+int spread_fx(cr_context* c, hipStream_t st)
 {
     if (!c->fx_bound) return set_error(CR_ERR_STATE, "fixed-point spread before its scale");
+    CR_HIP(c->fxblk.ensure(sizeof(int) * (size_t)c->tg.ntiles * (size_t)fx_block_ints(c->mp, c->tg)));
     CR_HIP(launch_spread_fx(c->mp, c->tg, c->fxp, bin_view(c), c->qf.as<float>(), c->fxs.as<FxScale>(),
-                            c->grid_real.as<int>(), st, clear));
-    c->mesh_fixed = true;
+                            c->fxblk.as<int>(), c->grid_real.as<float>(), st));
     return CR_OK;
 }
 
@@ -332,9 +323,8 @@ int spread_only(cr_context* c, const SysView& v, const double* q, hipStream_t st
 {
     if constexpr (sizeof(T) == sizeof(float)) {
         CR_RET(charges_f32(c, v, q, st));
-        if (c->fx_on()) return spread_fx(c, st, false);
+        if (c->fx_on()) return spread_fx(c, st);
     }
-    c->mesh_fixed = false;
     if (c->tg.valid)
         CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, bin_view(c), q, c->grid_real.as<T>(), st, false));
     else
@@ -351,10 +341,9 @@ int spread_impl(cr_context* c, const SysView& v, const double* q, hipStream_t st
     if constexpr (sizeof(T) == sizeof(float)) {
         if (c->fx_on()) {
             if (!q_ready) CR_RET(charges_f32(c, v, q, st));
-            return spread_fx(c, st, clear);
+            return spread_fx(c, st);  // writes every node: no clear
         }
     }
-    c->mesh_fixed = false;
     if (c->tg.valid) {
         CR_HIP(launch_spread_tiled<T>(c->mp, c->tg, bin_view(c), q, c->grid_real.as<T>(), st, clear));
     } else {
@@ -369,6 +358,10 @@ int spread_impl(cr_context* c, const SysView& v, const double* q, hipStream_t st
 // operation is always a spread, and it drops the mark on every exit.
 int preclear_mesh(cr_context* c, hipStream_t st)
 {
+    if (c->fx_on()) {  // the fixed-point spread writes every node
+        c->mesh_zeroed = true;
+        return CR_OK;
+    }
     if (c->fp32())
         CR_HIP(launch_fill<float>(c->grid_real.as<float>(), 0.0f, c->G, st));
     else
@@ -1200,7 +1193,6 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
         // a little is left for the kernels' static LDS
         c->lds_max = dev_max > 1024 ? (size_t)dev_max - 256 : 65536;
     }
-    c->mesh_fixed = false;
     CR_HIP(c->grid_real.ensure(Tsz * (size_t)c->G));
     CR_HIP(c->grid_cplx.ensure(2 * Tsz * (size_t)c->Gh));
 
@@ -1337,12 +1329,7 @@ int cr_spread(cr_context* c, const cr_system* s, const double* q, double* grid_o
     CR_RET(prepare(c, s, st, &v));
     if (c->fp32()) {
         CR_RET(spread_impl<float>(c, v, q, st));
-        if (grid_out) {
-            if (c->mesh_fixed)
-                CR_HIP(launch_decode_fx<double>(c->grid_real.as<int>(), grid_out, c->G, c->fxs.as<FxScale>(), st));
-            else
-                CR_HIP((launch_convert<float, double>(c->grid_real.as<float>(), grid_out, c->G, st)));
-        }
+        if (grid_out) CR_HIP((launch_convert<float, double>(c->grid_real.as<float>(), grid_out, c->G, st)));
     } else {
         CR_RET(spread_impl<double>(c, v, q, st));
         if (grid_out) CR_HIP((launch_convert<double, double>(c->grid_real.as<double>(), grid_out, c->G, st)));
@@ -1355,10 +1342,7 @@ int cr_recip_grid(cr_context* c, const double* grid_in, double* grid_out, void* 
     if (!c || !c->ready) return set_error(CR_ERR_STATE, "cr_set_params has not been called");
     hipStream_t st = (hipStream_t)stream;
     if (c->fp32()) {
-        if (grid_in) {
-            CR_HIP((launch_convert<double, float>(grid_in, c->grid_real.as<float>(), c->G, st)));
-            c->mesh_fixed = false;
-        }
+        if (grid_in) CR_HIP((launch_convert<double, float>(grid_in, c->grid_real.as<float>(), c->G, st)));
         CR_RET(recip_impl<float>(c, st));
         if (grid_out) CR_HIP((launch_convert<float, double>(c->grid_real.as<float>(), grid_out, c->G, st)));
     } else {
@@ -1464,6 +1448,9 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
     for (auto& e : ev) CR_HIP(hipEventCreate(&e));
     double acc[nst] = {0};
     double* y = c->w.as<double>();
+    // FP32 bulk: the fixed-point scale's window bound is set once per configuration, before
+    // the timed repetitions (as in a solve, where every matvec after the first reuses it)
+    if (c->fp32() && c->fx_on()) CR_RET(charges_f32(c, v, x, st));
     for (int r = 0; r < reps; ++r) {
         CR_HIP(hipEventRecord(ev[0], st));
         if (c->fp32()) {
@@ -1471,11 +1458,20 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
             const AxisPlan* pl = plans<T>(c);
             T* real = c->grid_real.as<T>();
             T* cp = c->grid_cplx.as<T>();
-            CR_HIP(launch_fill<T>(real, (T)0, c->G, st));
+            // the fixed-point spread writes every node and needs no clear; its charge
+            // rounding takes the clear's place, so that the spread stage times the spread
+            // kernels alone
+            if (c->fx_on())
+                CR_RET(charges_f32(c, v, x, st));
+            else
+                CR_HIP(launch_fill<T>(real, (T)0, c->G, st));
             CR_HIP(hipEventRecord(ev[1], st));
-            CR_RET(spread_only<T>(c, v, x, st));  // with the charge rounding
+            if (c->fx_on())
+                CR_RET(spread_fx(c, st));
+            else
+                CR_RET(spread_only<T>(c, v, x, st));
             CR_HIP(hipEventRecord(ev[2], st));
-            CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st, mesh_fx<T>(c)));
+            CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st));
             CR_HIP(hipEventRecord(ev[3], st));
             CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, -1, st));
             CR_HIP(hipEventRecord(ev[4], st));
@@ -1636,6 +1632,43 @@ int cr_set_residual_check(cr_context* c, int on)
     return CR_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// A tile shape is usable when its spread fits: in FP32 bulk the fixed-point spread (4-byte
+// nodes, its own LDS plan), otherwise the binned spread with FP64 accumulators in the
+// budget tg.lds.  The remaining tiled kernels (contractions) check their own fit and fall
+// back to the untiled kernels.
+bool tile_fits(const cr_context* c, const TileGeom& tg)
+{
+    if (c->fp32() && fx_plan(c->mp, tg, c->lds_max).valid) return true;
+    const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
+    return tiled_lds(c->mp, tg, sizeof(SpreadAcc), Tsz).chunk >= 16;
+}
+
+// the smallest LDS budget (in 32 KiB steps from 64 KiB) for a tile's binned kernels with at
+// least 32 staged beads (more only lowers occupancy); 0 when none fits
+int tile_budget(const cr_context* c, const int T3[3], int dev_max)
+{
+    const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
+    const bool fx = c->fp32();
+    for (int b = 64 * 1024; b <= dev_max; b += 32 * 1024) {
+        TileGeom g{};
+        for (int d = 0; d < 3; ++d) g.T[d] = T3[d];
+        g.lds = b;
+        g.valid = 1;
+        // FP32 bulk: the spread and potential contraction have their own plans; the budget
+        // serves the force contraction (4-byte nodes)
+        if (fx ? (fx_plan(c->mp, g, c->lds_max).valid && tiled_lds(c->mp, g, Tsz, Tsz).chunk >= 32)
+               : tiled_lds(c->mp, g, sizeof(SpreadAcc), Tsz).chunk >= 32)
+            return b;
+    }
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
 int cr_set_tiles(cr_context* c, const int tile[3], int lds_bytes)
 {
     if (!c || !c->ready || !tile) return set_error(CR_ERR_STATE, "cr_set_params has not been called");
@@ -1655,9 +1688,7 @@ int cr_set_tiles(cr_context* c, const int tile[3], int lds_bytes)
         tg.nt[d] = Nd[d] / tg.T[d];
         tg.ntiles *= tg.nt[d];
     }
-    const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
-    if (tiled_lds(c->mp, tg, sizeof(SpreadAcc), Tsz).chunk < 16)
-        return set_error(CR_ERR_INVALID, "extended tile does not fit the LDS budget");
+    if (!tile_fits(c, tg)) return set_error(CR_ERR_INVALID, "extended tile does not fit the LDS budget");
     c->tg = tg;
     ++c->generation;
     return CR_OK;
@@ -1670,7 +1701,6 @@ int cr_autotune(cr_context* c, const cr_system* s, cr_tuning* out, void* stream)
     const int P = c->mp.P, Nd[3] = {c->mp.Nx, c->mp.Ny, c->mp.Nz};
     int dev_max = 0;
     CR_HIP(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-    const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
     std::vector<int> opts[3];
     for (int d = 0; d < 3; ++d)
         for (int t = std::max(P - 1, 4); t <= std::min(Nd[d], 32); ++t)
@@ -1687,19 +1717,8 @@ int cr_autotune(cr_context* c, const cr_system* s, cr_tuning* out, void* stream)
     for (int tx : opts[0])
         for (int ty : opts[1])
             for (int tz : opts[2]) {
-                // the smallest budget (in 32 KiB steps from 64 KiB) that holds the tile and
-                // at least 32 staged beads: more LDS than that only lowers occupancy
                 const int T3[3] = {tx, ty, tz};
-                int budget = 0;
-                for (int b = 64 * 1024; b <= dev_max; b += 32 * 1024) {
-                    TileGeom g{};
-                    for (int d = 0; d < 3; ++d) g.T[d] = T3[d];
-                    g.lds = b;
-                    if (tiled_lds(c->mp, g, sizeof(SpreadAcc), Tsz).chunk >= 32) {
-                        budget = b;
-                        break;
-                    }
-                }
+                const int budget = tile_budget(c, T3, dev_max);
                 if (!budget || cr_set_tiles(c, T3, budget) != CR_OK) continue;
                 CR_RET(prepare(c, s, st, &v));
                 if (!q) {  // any charges will do for timing: the bead records

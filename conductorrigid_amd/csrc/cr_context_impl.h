@@ -37,9 +37,9 @@ struct cr_context {
     // the weight bound matches the current binning, and whether the mesh holds int32
     // fixed-point values (the next FFT converts them)
     DevBuf fxs;
+    DevBuf fxblk;  // the fixed-point spread's per-tile integer blocks (tg.ntiles x fx_block_ints)
     FxPlan fxp{}, fxcp{};  // spread and potential-contraction LDS plans
     bool fx_bound = false;
-    bool mesh_fixed = false;
     size_t lds_max = 65536;
     // bead binning for the owner-computes spread
     TileGeom tg{};

@@ -147,8 +147,8 @@ template <typename T>
 hipError_t launch_contract_tiled_batch(const TiledBox* boxes, int nbox, int total_tiles, int P, size_t lds_bytes,
                                        const int* skip, hipStream_t st, bool force = false);
 // ---- k_spread_fx.hip: FP32-bulk spread in 32-bit fixed point ----
-// The mesh holds int32 node values in units of 1 / (mult / prefac); the first FFT pass
-// (or launch_decode_fx) converts them back to float with inv.
+// Node values are summed as int32 in units of 1 / (mult / prefac) and leave as float
+// (int sum x inv).
 constexpr int kFxParts = 1024;  // workgroups of k_charges_fx (partial maxima)
 struct FxScale {
     float mult;       // charge -> fixed-point units: 2^30 / (qmax W)
@@ -169,9 +169,12 @@ struct FxPlan {
 FxPlan fx_plan(const MeshParams& mp, const TileGeom& tg, size_t lds_max);
 // qf = (float) x [N]; max |qf|, then fs->mult and fs->inv from it and fs->wbits.
 hipError_t launch_charges_fx(const double* x, int N, float* qf, FxScale* fs, double prefac, hipStream_t st);
-// grid (int32, G nodes) += fixed-point spread of qf; clear zeroes the mesh first.
+// Scratch ints per tile block of launch_spread_fx (a multiple of 4).
+long fx_block_ints(const MeshParams& mp, const TileGeom& tg);
+// mesh (float, G nodes) = fixed-point spread of qf at fs's scale: the tiles' integer
+// blocks go to the scratch blocks (tg.ntiles x fx_block_ints) and are summed per node.
 hipError_t launch_spread_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                            const float* qf, FxScale* fs, int* grid, hipStream_t st, bool clear);
+                            const float* qf, FxScale* fs, int* blocks, float* mesh, hipStream_t st);
 // fs->wbits for the binned configuration (once per bead configuration).
 hipError_t launch_spread_fx_bound(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
                                   FxScale* fs, hipStream_t st);
@@ -179,9 +182,6 @@ hipError_t launch_spread_fx_bound(const MeshParams& mp, const TileGeom& tg, cons
 FxPlan fx_contract_plan(const MeshParams& mp, const TileGeom& tg, size_t lds_max);
 hipError_t launch_contract_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
                               const float* grid, float* phi, hipStream_t st);
-// out = in * fs->inv (in and out may alias).
-template <typename T>
-hipError_t launch_decode_fx(const int* in, T* out, long n, const FxScale* fs, hipStream_t st);
 
 // ---- k_se.hip (unbinned fallbacks) ----
 template <typename T>
@@ -201,10 +201,8 @@ hipError_t launch_kspace_radial(long m2max, double kappa, double invG, T* rad, h
 
 // ---- k_fft.hip ----
 // Real mesh [Nx][Ny][Nz] <-> half spectrum [Nx][Ny][Nz/2+1] (interleaved complex).
-// fx != null: real holds the int32 fixed-point mesh of launch_spread_fx (float meshes only).
 template <typename T>
-hipError_t launch_fft_z_r2c(const AxisPlan& pz, int Nx, int Ny, const T* real, T* cplx, hipStream_t st,
-                           const FxScale* fx = nullptr);
+hipError_t launch_fft_z_r2c(const AxisPlan& pz, int Nx, int Ny, const T* real, T* cplx, hipStream_t st);
 template <typename T>
 hipError_t launch_fft_z_c2r(const AxisPlan& pz, int Nx, int Ny, const T* cplx, T* real, hipStream_t st);
 // C2C along y (axis 1) or x (axis 0) of a complex mesh [Nx][Ny][Zc]; sign -1 / +1.
@@ -232,8 +230,7 @@ hipError_t launch_fft_x_scale(const AxisPlan& px, int Nx, int Ny, int Zc, T* cpl
 // ---- k_fft_reg.hip: register four-step passes for n = R * R, R in {8, 16, 32} ----
 int fftr_radix(int n);  // n when the register-resident passes take length n, else 0
 template <typename T>
-hipError_t launch_fftr_z_r2c(const AxisPlan& pz, long nrows, const T* real, T* cplx, hipStream_t st,
-                            const FxScale* fx = nullptr);
+hipError_t launch_fftr_z_r2c(const AxisPlan& pz, long nrows, const T* real, T* cplx, hipStream_t st);
 template <typename T>
 hipError_t launch_fftr_z_c2r(const AxisPlan& pz, long nrows, const T* cplx, T* real, hipStream_t st);
 template <typename T>

@@ -373,19 +373,12 @@ bool fits(int n)
 }  // namespace
 
 template <typename T>
-hipError_t launch_fft_z_r2c(const AxisPlan& pz, int Nx, int Ny, const T* real, T* cplx, hipStream_t st,
-                           const FxScale* fx)
+hipError_t launch_fft_z_r2c(const AxisPlan& pz, int Nx, int Ny, const T* real, T* cplx, hipStream_t st)
 {
     using T2 = typename Cplx<T>::type;
     const long nrows = (long)Nx * Ny;
-    if (fx && sizeof(T) != sizeof(float)) return hipErrorInvalidValue;
-    if (fftr_radix(pz.n)) return launch_fftr_z_r2c<T>(pz, nrows, real, cplx, st, fx);
+    if (fftr_radix(pz.n)) return launch_fftr_z_r2c<T>(pz, nrows, real, cplx, st);
     if (!fits<T>(pz.n)) return hipErrorInvalidValue;
-    if (fx) {
-        // the generic passes read float: convert the fixed-point mesh in place first
-        hipError_t e = launch_decode_fx<T>(reinterpret_cast<const int*>(real), const_cast<T*>(real), nrows * pz.n, fx, st);
-        if (e != hipSuccess) return e;
-    }
     const int cols = pick_cols<T>(pz.n);
     hipLaunchKernelGGL(k_fft_z_r2c<T>, dim3((unsigned)((nrows + cols - 1) / cols)), dim3(kThreads),
                        lds_bytes<T>(pz.n, cols), st, pz, nrows, cols, real, (T2*)cplx);
@@ -448,7 +441,7 @@ hipError_t launch_fft_x_scale(const AxisPlan& px, int Nx, int Ny, int Zc, T* cpl
 }
 
 #define CR_FFT_INST(T)                                                                                        \
-    template hipError_t launch_fft_z_r2c<T>(const AxisPlan&, int, int, const T*, T*, hipStream_t, const FxScale*);            \
+    template hipError_t launch_fft_z_r2c<T>(const AxisPlan&, int, int, const T*, T*, hipStream_t);            \
     template hipError_t launch_fft_z_c2r<T>(const AxisPlan&, int, int, const T*, T*, hipStream_t);            \
     template hipError_t launch_fft_z_c2c<T>(const AxisPlan&, int, int, T*, int, hipStream_t);                 \
     template hipError_t launch_fft_yx<T>(const AxisPlan&, int, int, int, int, T*, int, hipStream_t);          \

@@ -192,13 +192,10 @@ __device__ __forceinline__ void four_step_inv_from_X(T2 (&v)[A], T2* buf, const 
     dft<A, +1>(v);
 }
 
-// fx != null: real holds int32 fixed-point node values (k_spread_fx.hip), converted to
-// float with fx->inv as they are loaded.
 template <typename T, int A, int B>
 __global__ __launch_bounds__((B * lanes_per_wg<T, A, B>())) void k_fftr_z_r2c(long nrows, const T* __restrict__ real,
                                                                             typename Cx<T>::t* __restrict__ cplx,
-                                                                            const typename Cx<T>::t* __restrict__ tw,
-                                                                            const FxScale* __restrict__ fx)
+                                                                            const typename Cx<T>::t* __restrict__ tw)
 {
     using T2 = typename Cx<T>::t;
     constexpr int n = A * B, nh = n / 2 + 1, m = A / B;
@@ -207,20 +204,10 @@ __global__ __launch_bounds__((B * lanes_per_wg<T, A, B>())) void k_fftr_z_r2c(lo
     const long row = (long)blockIdx.x * lanes_per_wg<T, A, B>() + p;
     const bool ok = row < nrows;
     T2 v[A];
-    if (fx) {
-        const float inv = fx->inv;
-        const int* fixed = reinterpret_cast<const int*>(real);
 #pragma unroll
-        for (int q = 0; q < A; ++q) {
-            v[q].x = ok ? (T)((float)fixed[row * n + l + B * q] * inv) : (T)0;
-            v[q].y = 0;
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < A; ++q) {
-            v[q].x = ok ? real[row * n + l + B * q] : (T)0;
-            v[q].y = 0;
-        }
+    for (int q = 0; q < A; ++q) {
+        v[q].x = ok ? real[row * n + l + B * q] : (T)0;
+        v[q].y = 0;
     }
     four_step<A, B, -1>(v, lds + p * A * (B + 1), tw, l);
     if (!ok) return;
@@ -491,12 +478,12 @@ __global__ __launch_bounds__((B * cols_per_wg<T, A, B>())) void k_fftr_x_scale_b
 inline unsigned ublk(long n, int per) { return (unsigned)((n + per - 1) / per); }
 
 template <typename T, int A, int B>
-hipError_t z_r2c(long nrows, const T* real, T* cplx, const void* tw, hipStream_t st, const FxScale* fx)
+hipError_t z_r2c(long nrows, const T* real, T* cplx, const void* tw, hipStream_t st)
 {
     using T2 = typename Cx<T>::t;
     constexpr int P = lanes_per_wg<T, A, B>();
     hipLaunchKernelGGL((k_fftr_z_r2c<T, A, B>), dim3(ublk(nrows, P)), dim3(B * P), 0, st, nrows, real, (T2*)cplx,
-                       (const T2*)tw, fx);
+                       (const T2*)tw);
     return hipGetLastError();
 }
 
@@ -586,11 +573,9 @@ int fftr_radix(int n)
     }
 
 template <typename T>
-hipError_t launch_fftr_z_r2c(const AxisPlan& pz, long nrows, const T* real, T* cplx, hipStream_t st,
-                            const FxScale* fx)
+hipError_t launch_fftr_z_r2c(const AxisPlan& pz, long nrows, const T* real, T* cplx, hipStream_t st)
 {
-    if (fx && sizeof(T) != sizeof(float)) return hipErrorInvalidValue;
-#define CALL(A, B) z_r2c<T, A, B>(nrows, real, cplx, pz.tw, st, fx)
+#define CALL(A, B) z_r2c<T, A, B>(nrows, real, cplx, pz.tw, st)
     CR_FFTR_DISPATCH(pz.n, CALL)
 #undef CALL
 }
@@ -637,7 +622,7 @@ hipError_t launch_fftr_x_scale_batch(const AxisPlan& px, int nbox, int Ny, int Z
 }
 
 #define CR_FFTR_INST(T)                                                                                          \
-    template hipError_t launch_fftr_z_r2c<T>(const AxisPlan&, long, const T*, T*, hipStream_t, const FxScale*);                \
+    template hipError_t launch_fftr_z_r2c<T>(const AxisPlan&, long, const T*, T*, hipStream_t);                \
     template hipError_t launch_fftr_z_c2r<T>(const AxisPlan&, long, const T*, T*, hipStream_t);                \
     template hipError_t launch_fftr_z_c2c<T>(const AxisPlan&, long, T*, int, hipStream_t);                     \
     template hipError_t launch_fftr_strided<T>(const AxisPlan&, int, int, int, int, T*, int, hipStream_t);     \

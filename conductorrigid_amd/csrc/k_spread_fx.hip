@@ -8,8 +8,11 @@
 // rounded to the nearest integer and added with ds_add_u32.  On gfx950 an LDS float atomic
 // add runs an order of magnitude slower than the integer one, and a double accumulator
 // takes twice the LDS; the integer tile is half that size, so two workgroups fit a CU.  The
-// extended tile is then added into a mesh of int32 with global integer atomics, and the
-// FFT's first pass converts the mesh back to float (k_fft_reg.hip, k_fft.hip).
+// extended tile then leaves with plain stores as a block of a scratch buffer, and
+// k_fx_gather sums, for every mesh node, the (at most 8) blocks that cover it and writes the
+// float mesh.  Global atomics would add the extended tiles (3x the mesh at 16^3 tiles) at
+// the memory-side atomic rate, about a quarter of the plain-store bandwidth; the blocks
+// cost one store and one load each instead, and the mesh needs no clearing.
 //
 // Integer sums do not depend on the order of the additions, so the spread is
 // deterministic: every run gives the same mesh bits.
@@ -28,6 +31,7 @@
 
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 
 namespace cr {
@@ -148,7 +152,7 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx(MeshParams mp, TileGeo
                                                           const int* __restrict__ rec_off,
                                                           const float4* __restrict__ rec_u,
                                                           const float* __restrict__ qf, FxScale* __restrict__ fs,
-                                                          int* __restrict__ grid)
+                                                          long bstride, int* __restrict__ blocks)
 {
     constexpr bool CT = TX > 0;
     constexpr int WS = fx_wstride(PT);
@@ -159,9 +163,15 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx(MeshParams mp, TileGeo
     __shared__ unsigned tile_max;
     const int t = fx_tile(blockIdx.x, tg.ntiles);
     const unsigned kb = tstart[t], ke = tstart[t + 1];
-    if (kb == ke) return;  // an empty bin adds nothing to the (cleared) mesh
-    const int tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
-    const int x0 = tx * T0, y0 = ty * T1, z0 = tz * T2;
+    if (kb == ke) {
+        // an empty bin: a block of zeros (the gather reads every covering block)
+        if constexpr (!ONES) {
+            int* blk = blocks + (long)t * bstride;
+            for (int k = threadIdx.x; k < (int)(bstride / 4); k += kFxThreads)
+                reinterpret_cast<uint4*>(blk)[k] = make_uint4(0, 0, 0, 0);
+        }
+        return;
+    }
     unsigned* ext = reinterpret_cast<unsigned*>(smem);
     float* w = reinterpret_cast<float*>(smem + ((size_t)esz * 4 + 15) / 16 * 16);
     int2* oq = reinterpret_cast<int2*>(w + (size_t)ch * WS);
@@ -242,18 +252,69 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx(MeshParams mp, TileGeo
         }
         return;
     }
-    const long plane = (long)mp.Ny * mp.Nz;
-    const int nodes = EX * EY * EZ;
-    FxWalk bw(threadIdx.x, kFxThreads, EY, EZ);
-    for (int k = threadIdx.x; k < nodes; k += kFxThreads, bw.next()) {
-        // nodes no window of the bin reached are exactly zero: adding them changes nothing
-        const int v = (int)ext[bw.lx * xs + bw.ly * ys + bw.lz];
-        if (v == 0) continue;
-        int gx = x0 + bw.lx, gy = y0 + bw.ly, gz = z0 + bw.lz;
-        gx -= gx >= mp.Nx ? mp.Nx : 0;
-        gy -= gy >= mp.Ny ? mp.Ny : 0;
-        gz -= gz >= mp.Nz ? mp.Nz : 0;
-        atomicAdd(&grid[(long)gx * plane + (long)gy * mp.Nz + gz], v);
+    // the extended tile leaves as one contiguous block of the scratch (plain 16-byte stores,
+    // rows padded as in LDS); k_fx_gather sums the blocks that cover each mesh node
+    int* blk = blocks + (long)t * bstride;
+    for (int k = threadIdx.x; k < esz / 4; k += kFxThreads)
+        reinterpret_cast<uint4*>(blk)[k] = reinterpret_cast<const uint4*>(ext)[k];
+    for (int k = (esz / 4) * 4 + threadIdx.x; k < esz; k += kFxThreads) blk[k] = (int)ext[k];
+}
+
+// The mesh from the tiles' blocks: a node sums the blocks that cover it -- its own tile's
+// and, within P - 1 nodes of a tile's low face, the previous tile's along that axis (two at
+// most per axis, as P - 1 <= the tile edge) -- and leaves as float, int sum x inv.  Integer
+// sums do not depend on their order, so the mesh is the same as from atomic additions.
+// Every node is written, so the mesh needs no clearing.  V = 4: a thread takes 4
+// consecutive z nodes of one tile (tile edge, block row pitch and Nz multiples of 4; the
+// row padding past EZ holds zeros) with 16-byte loads, all of them issued together.
+template <int V>
+__global__ __launch_bounds__(256) void k_fx_gather(MeshParams mp, TileGeom tg, long bstride,
+                                                   const int* __restrict__ blocks, const FxScale* __restrict__ fs,
+                                                   float* __restrict__ mesh)
+{
+    const int T0 = tg.T[0], T1 = tg.T[1], T2 = tg.T[2];
+    const int EX = T0 + mp.P - 1, EY = T1 + mp.P - 1, EZ = T2 + mp.P - 1;
+    const int ys = (EZ + 1) & ~1;
+    const int nq = mp.Nz / V;
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long rowid = gid / nq;
+    if (rowid >= (long)mp.Nx * mp.Ny) return;
+    const int gz = (int)(gid - rowid * nq) * V;
+    const int gy = (int)(rowid % mp.Ny), gx = (int)(rowid / mp.Ny);
+    const int tx = gx / T0, lx = gx - tx * T0, ty = gy / T1, ly = gy - ty * T1, tz = gz / T2, lz = gz - tz * T2;
+    const int px = tx == 0 ? tg.nt[0] - 1 : tx - 1, py = ty == 0 ? tg.nt[1] - 1 : ty - 1;
+    const int pz = tz == 0 ? tg.nt[2] - 1 : tz - 1;
+    const bool bx = lx + T0 < EX, by = ly + T1 < EY, bz = lz + T2 < EZ;
+    using IV = typename std::conditional<V == 4, int4, int>::type;
+    IV v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int dx = c >> 2, dy = (c >> 1) & 1, dz = c & 1;
+        const bool on = (!dx || bx) && (!dy || by) && (!dz || bz);
+        const long u = ((long)(dx ? px : tx) * tg.nt[1] + (dy ? py : ty)) * tg.nt[2] + (dz ? pz : tz);
+        const long off = u * bstride + ((long)(lx + dx * T0) * EY + (ly + dy * T1)) * ys + lz + dz * T2;
+        if constexpr (V == 4)
+            v[c] = on ? *reinterpret_cast<const int4*>(blocks + off) : make_int4(0, 0, 0, 0);
+        else
+            v[c] = on ? blocks[off] : 0;
+    }
+    const float inv = fs->inv;
+    float* out = mesh + rowid * mp.Nz + gz;
+    if constexpr (V == 4) {
+        int4 a = v[0];
+#pragma unroll
+        for (int c = 1; c < 8; ++c) {
+            a.x += v[c].x;
+            a.y += v[c].y;
+            a.z += v[c].z;
+            a.w += v[c].w;
+        }
+        *reinterpret_cast<float4*>(out) = make_float4((float)a.x * inv, (float)a.y * inv, (float)a.z * inv, (float)a.w * inv);
+    } else {
+        int a = v[0];
+#pragma unroll
+        for (int c = 1; c < 8; ++c) a += v[c];
+        out[0] = (float)a * inv;
     }
 }
 
@@ -371,17 +432,6 @@ __global__ __launch_bounds__(kFxThreads) void k_contract_fx(MeshParams mp, TileG
     }
 }
 
-template <typename T>
-__global__ void k_decode_fx(const int* in, T* out, long n, const FxScale* __restrict__ fs)
-{
-    // in and out may alias (in-place conversion of the mesh)
-    const float inv = fs->inv;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const int v = in[i];
-        out[i] = (T)((float)v * inv);
-    }
-}
-
 void allow_lds_fx(const void* fn, size_t bytes)
 {
     if (bytes <= 65536) return;
@@ -397,22 +447,23 @@ void allow_lds_fx(const void* fn, size_t bytes)
 
 template <int PT, int TX, int TY, int TZ, bool ONES>
 hipError_t launch_fx_kernel(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                            const float* qf, FxScale* fs, int* grid, hipStream_t st)
+                            const float* qf, FxScale* fs, long bstride, int* blocks, hipStream_t st)
 {
     auto fn = k_spread_fx<PT, TX, TY, TZ, ONES>;
     allow_lds_fx(reinterpret_cast<const void*>(fn), pl.lds_bytes);
     hipLaunchKernelGGL(fn, dim3(tg.ntiles), dim3(kFxThreads), pl.lds_bytes, st, mp, tg, pl.chunk, bv.tile_start,
-                       bv.sorted, bv.rec_off, static_cast<const float4*>(bv.rec_win), qf, fs, grid);
+                       bv.sorted, bv.rec_off, static_cast<const float4*>(bv.rec_win), qf, fs, bstride, blocks);
     return hipGetLastError();
 }
 
 template <int PT, bool ONES>
 hipError_t dispatch_shape(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                          const float* qf, FxScale* fs, int* grid, hipStream_t st)
+                          const float* qf, FxScale* fs, long bstride, int* blocks, hipStream_t st)
 {
     const int a = tg.T[0], b = tg.T[1], c = tg.T[2];
 #define CR_FX_SHAPE(X, Y, Z) \
-    if (a == X && b == Y && c == Z) return launch_fx_kernel<PT, X, Y, Z, ONES>(mp, tg, pl, bv, qf, fs, grid, st)
+    if (a == X && b == Y && c == Z)  \
+    return launch_fx_kernel<PT, X, Y, Z, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st)
     if constexpr (PT == 8) {
         CR_FX_SHAPE(16, 16, 16);
         CR_FX_SHAPE(16, 8, 16);
@@ -426,21 +477,21 @@ hipError_t dispatch_shape(const MeshParams& mp, const TileGeom& tg, const FxPlan
         CR_FX_SHAPE(8, 8, 8);
     }
 #undef CR_FX_SHAPE
-    return launch_fx_kernel<PT, 0, 0, 0, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
+    return launch_fx_kernel<PT, 0, 0, 0, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
 }
 
 template <bool ONES>
 hipError_t dispatch_p(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv, const float* qf,
-                      FxScale* fs, int* grid, hipStream_t st)
+                      FxScale* fs, long bstride, int* blocks, hipStream_t st)
 {
     switch (mp.P) {
-        case 4: return dispatch_shape<4, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
-        case 5: return dispatch_shape<5, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
-        case 6: return dispatch_shape<6, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
-        case 7: return dispatch_shape<7, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
-        case 8: return dispatch_shape<8, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
-        case 9: return dispatch_shape<9, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
-        case 10: return dispatch_shape<10, ONES>(mp, tg, pl, bv, qf, fs, grid, st);
+        case 4: return dispatch_shape<4, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
+        case 5: return dispatch_shape<5, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
+        case 6: return dispatch_shape<6, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
+        case 7: return dispatch_shape<7, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
+        case 8: return dispatch_shape<8, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
+        case 9: return dispatch_shape<9, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
+        case 10: return dispatch_shape<10, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -553,15 +604,30 @@ hipError_t launch_charges_fx(const double* x, int N, float* qf, FxScale* fs, dou
     return hipGetLastError();
 }
 
+long fx_block_ints(const MeshParams& mp, const TileGeom& tg)
+{
+    const long EX = tg.T[0] + mp.P - 1, EY = tg.T[1] + mp.P - 1, EZ = tg.T[2] + mp.P - 1;
+    return (EX * EY * ((EZ + 1) & ~1L) + 3) & ~3L;
+}
+
 hipError_t launch_spread_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                            const float* qf, FxScale* fs, int* grid, hipStream_t st, bool clear)
+                            const float* qf, FxScale* fs, int* blocks, float* mesh, hipStream_t st)
 {
     if (!pl.valid) return hipErrorInvalidValue;
-    if (clear) {
-        hipError_t e = launch_fill<int>(grid, 0, (long)mp.Nx * mp.Ny * mp.Nz, st);
-        if (e != hipSuccess) return e;
-    }
-    return dispatch_p<false>(mp, tg, pl, bv, qf, fs, grid, st);
+    for (int d = 0; d < 3; ++d)
+        if (tg.T[d] < mp.P - 1) return hipErrorInvalidValue;  // at most two blocks per axis cover a node
+    const long bstride = fx_block_ints(mp, tg);
+    hipError_t e = dispatch_p<false>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
+    if (e != hipSuccess) return e;
+    const long ys = ((long)tg.T[2] + mp.P) & ~1L;  // block row pitch, (EZ + 1) & ~1
+    const long rows = (long)mp.Nx * mp.Ny;
+    if (tg.T[2] % 4 == 0 && ys % 4 == 0 && mp.Nz % 4 == 0)
+        hipLaunchKernelGGL(k_fx_gather<4>, dim3(nblk(rows * (mp.Nz / 4), 256)), dim3(256), 0, st, mp, tg, bstride, blocks,
+                           fs, mesh);
+    else
+        hipLaunchKernelGGL(k_fx_gather<1>, dim3(nblk(rows * mp.Nz, 256)), dim3(256), 0, st, mp, tg, bstride, blocks, fs,
+                           mesh);
+    return hipGetLastError();
 }
 
 hipError_t launch_spread_fx_bound(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
@@ -570,17 +636,7 @@ hipError_t launch_spread_fx_bound(const MeshParams& mp, const TileGeom& tg, cons
     if (!pl.valid) return hipErrorInvalidValue;
     hipError_t e = launch_fill<unsigned>(&fs->wbits, 0u, 1, st);
     if (e != hipSuccess) return e;
-    return dispatch_p<true>(mp, tg, pl, bv, nullptr, fs, nullptr, st);
+    return dispatch_p<true>(mp, tg, pl, bv, nullptr, fs, 0, nullptr, st);
 }
-
-template <typename T>
-hipError_t launch_decode_fx(const int* in, T* out, long n, const FxScale* fs, hipStream_t st)
-{
-    const unsigned blocks = std::min<unsigned>(4096, nblk(n, 256));
-    hipLaunchKernelGGL(k_decode_fx<T>, dim3(blocks), dim3(256), 0, st, in, out, n, fs);
-    return hipGetLastError();
-}
-template hipError_t launch_decode_fx<float>(const int*, float*, long, const FxScale*, hipStream_t);
-template hipError_t launch_decode_fx<double>(const int*, double*, long, const FxScale*, hipStream_t);
 
 }  // namespace cr

@@ -236,8 +236,9 @@ int cr_real_space_force(cr_context* ctx, const cr_system* sys, const double* q, 
  * Runs each stage of the matvec y = A x `reps` times on `stream`, bracketing every
  * stage with HIP events, and writes the average milliseconds per launch into
  * stage_ms[k] for k < min(max_stages, CR_NUM_STAGES), in this order:
- *   0 mesh clear, 1 spread, 2 FFT z R2C, 3 FFT y forward, 4 FFT x forward+scale+inverse,
- *   5 FFT y inverse, 6 FFT z C2R, 7 contract, 8 real space, 9 constraint rows.
+ *   0 mesh clear (in FP32 bulk with the fixed-point charge rounding), 1 spread, 2 FFT z R2C,
+ *   3 FFT y forward, 4 FFT x forward+scale+inverse, 5 FFT y inverse, 6 FFT z C2R,
+ *   7 contract, 8 real space, 9 constraint rows.
  * No reference counterpart: the reference has no per-kernel timing (HOOMD's
  * Profiler wraps the whole step, cppmodule/ConductorRigid.cc:415-416). */
 #define CR_NUM_STAGES 10
