@@ -1635,33 +1635,27 @@ int cr_set_residual_check(cr_context* c, int on)
 }  // extern "C"
 
 namespace {
-// A tile shape is usable when its spread fits: in FP32 bulk the fixed-point spread (4-byte
-// nodes, its own LDS plan), otherwise the binned spread with FP64 accumulators in the
-// budget tg.lds.  The remaining tiled kernels (contractions) check their own fit and fall
-// back to the untiled kernels.
+// A tile shape is usable when the binned spread with FP64 accumulators fits the budget
+// tg.lds with at least 16 staged beads.  That is the tightest of the binned kernels (the
+// batched boxes run it in both precisions); the FP32-bulk fixed-point kernels, with 4-byte
+// nodes and their own LDS plans, fit every such tile.  Larger FP32 tiles that only the
+// fixed-point spread would fit were measured no faster at N = 1M.
 bool tile_fits(const cr_context* c, const TileGeom& tg)
 {
-    if (c->fp32() && fx_plan(c->mp, tg, c->lds_max).valid) return true;
     const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
     return tiled_lds(c->mp, tg, sizeof(SpreadAcc), Tsz).chunk >= 16;
 }
 
-// the smallest LDS budget (in 32 KiB steps from 64 KiB) for a tile's binned kernels with at
-// least 32 staged beads (more only lowers occupancy); 0 when none fits
+// the smallest LDS budget (in 32 KiB steps from 64 KiB) that holds the tile and at least 32
+// staged beads (more only lowers occupancy); 0 when none does
 int tile_budget(const cr_context* c, const int T3[3], int dev_max)
 {
     const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
-    const bool fx = c->fp32();
     for (int b = 64 * 1024; b <= dev_max; b += 32 * 1024) {
         TileGeom g{};
         for (int d = 0; d < 3; ++d) g.T[d] = T3[d];
         g.lds = b;
-        g.valid = 1;
-        // FP32 bulk: the spread and potential contraction have their own plans; the budget
-        // serves the force contraction (4-byte nodes)
-        if (fx ? (fx_plan(c->mp, g, c->lds_max).valid && tiled_lds(c->mp, g, Tsz, Tsz).chunk >= 32)
-               : tiled_lds(c->mp, g, sizeof(SpreadAcc), Tsz).chunk >= 32)
-            return b;
+        if (tiled_lds(c->mp, g, sizeof(SpreadAcc), Tsz).chunk >= 32) return b;
     }
     return 0;
 }
