@@ -160,7 +160,9 @@ def batch64_summary(steps: int, warmup: int, sequential: bool = True) -> dict:
     items = make_batch(64, seed=4, workers=8)
     N = [s.N for s, _ in items]
     meshes = [p["mesh"][0] for _, p in items]
-    log(f"batch: {len(items)} boxes, {sum(N)} beads (N {min(N)}..{max(N)}), meshes {sorted(set(meshes))} "
+    phis = [p["phi"] for _, p in items]
+    log(f"batch: {len(items)} boxes, {sum(N)} beads (N {min(N)}..{max(N)}, phi {min(phis):.3f}..{max(phis):.3f}), "
+        f"meshes {sorted(set(meshes))} "
         f"({time.perf_counter() - t0:.1f} s)")
     boxes = batch_boxes(items, precision="fp32", preconditioner="block", restart=30, max_iter=1000, gmres_tol=1e-6)
     t0 = time.perf_counter()
@@ -207,7 +209,8 @@ def batch64_summary(steps: int, warmup: int, sequential: bool = True) -> dict:
         log(f"sequential single-box solves: {dts * 1e3:.2f} ms for all boxes")
     dt = res["batch_graph"]["ms_per_eval"] * 1e-3
     return {"workload": f"batch64: {len(items)} boxes, {sum(N)} beads, N log-uniform [2000, 50000], bodies 8/20/40, "
-                        f"phi 0.1-0.3, meshes {min(meshes)}^3-{max(meshes)}^3, P=9, tol 1e-6, restart 30, "
+                        f"phi U[0.1, 0.35] ({min(phis):.3f}-{max(phis):.3f} drawn), meshes {min(meshes)}^3-{max(meshes)}^3, "
+                        f"P=9, tol 1e-6, restart 30, "
                         f"near-field block preconditioner, x0=0",
             "boxes": len(items), "beads": sum(N), "ms_per_eval": dt * 1e3, "batch_evals_per_sec": 1.0 / dt,
             "beads_per_sec": sum(N) / dt,

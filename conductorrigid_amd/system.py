@@ -116,9 +116,49 @@ def _body_means(x: np.ndarray, offsets: np.ndarray, bodies: np.ndarray, count: n
     return (x[rows] * mask[:, :, None]).sum(axis=1) / k[:, None]
 
 
+def _fibonacci_sphere(n: int) -> np.ndarray:
+    """n nearly uniform unit vectors (golden-angle spiral)."""
+    k = np.arange(n) + 0.5
+    z = 1.0 - 2.0 * k / n
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    t = math.pi * (3.0 - math.sqrt(5.0)) * k
+    return np.stack([r * np.cos(t), r * np.sin(t), z], axis=1)
+
+
+def _grow_exhaustive(unwrapped: np.ndarray, placed: np.ndarray, offsets: np.ndarray, count: np.ndarray,
+                     sizes: np.ndarray, L: np.ndarray, tol: float, n_dir: int = 384) -> int:
+    """Grow each unfinished body by up to its missing beads, one bead at a time, at the free
+    touching site (distance 2 to a bead of its own body, >= 2 from every bead) closest to
+    its centre, trying n_dir directions around every one of its beads.  Returns the number
+    of beads placed."""
+    from scipy.spatial import cKDTree
+
+    Lval = float(L[0])
+    dirs = 2.0 * _fibonacci_sphere(n_dir)
+    placed_total = 0
+    for b in np.nonzero(count < sizes)[0]:
+        while count[b] < sizes[b]:
+            idx_placed = np.nonzero(placed)[0]
+            tree = cKDTree(_wrap(unwrapped[idx_placed], L) + 0.5 * L, boxsize=Lval)
+            own = unwrapped[offsets[b]:offsets[b] + count[b]]
+            cand = (own[:, None, :] + dirs[None, :, :]).reshape(-1, 3)
+            dist, _ = tree.query(_wrap(cand, L) + 0.5 * L, k=1)
+            ok = dist >= 2.0 - tol
+            if not np.any(ok):
+                break
+            cand = cand[ok]
+            c = cand[np.argmin(np.linalg.norm(cand - own.mean(axis=0), axis=1))]
+            slot = offsets[b] + count[b]
+            unwrapped[slot] = c
+            placed[slot] = True
+            count[b] += 1
+            placed_total += 1
+    return placed_total
+
+
 def make_system(n_bodies: int, body_sizes, phi: float, seed: int, charges: str = "alternating",
                 n_isolated_beads: int = 0, spatial_sort: bool = True,
-                n_candidates: int = 16) -> SystemDefinition:
+                n_candidates: int = 16, dense_growth: bool = False) -> SystemDefinition:
     """Seeded random system of rigid conductor clusters (plus optional isolated beads).
 
     ``body_sizes`` is an int (all bodies that size) or a sequence to draw from.
@@ -127,7 +167,11 @@ def make_system(n_bodies: int, body_sizes, phi: float, seed: int, charges: str =
     a new bead touches (centre distance 2) a random bead of its own body,
     overlaps no bead of any body (periodic minimum image), and of
     ``n_candidates`` proposals the one closest to the body's centre is kept,
-    which makes compact random aggregates.  ``charges`` is "alternating"
+    which makes compact random aggregates.  When every unfinished body fails its
+    proposals for several rounds, the caged bodies are re-seeded at free spots; with
+    ``dense_growth`` a search over every bead of each unfinished body and 384 directions
+    for a free touching site comes first (this reaches volume fraction 0.35 with 8-, 20-
+    and 40-bead bodies, where re-seeding stalls above ~0.33).  ``charges`` is "alternating"
     (+1/-1 by body), "uniform" (U[-2, 2] shifted to total 0) or "zero".
     """
     from scipy.spatial import cKDTree
@@ -209,6 +253,12 @@ def make_system(n_bodies: int, body_sizes, phi: float, seed: int, charges: str =
             n_recent += chosen.shape[0]
         stall = 0 if chosen_b.size else stall + 1
         if stall > 8:
+            if dense_growth and _grow_exhaustive(unwrapped, placed, offsets, count, sizes, L, tol):
+                # random proposals found no room, a search over every bead of each unfinished
+                # body and a dense set of directions did
+                big_tree, recent, n_recent = None, [], 0
+                stall = 0
+                continue
             # every remaining body is caged in: re-seed each at a random free spot
             restarts += 1
             if restarts > 200:
@@ -351,7 +401,7 @@ def _batch_box(args) -> tuple[SystemDefinition, dict]:
     for k in range(8):
         # dense packings of large clusters can cage the growth: take the next seed
         try:
-            sysdef = make_system(n_bodies, list(body_sizes), phi, seed=seed + k, charges="uniform")
+            sysdef = make_system(n_bodies, list(body_sizes), phi, seed=seed + k, charges="uniform", dense_growth=True)
             break
         except RuntimeError:
             continue
@@ -363,14 +413,13 @@ def _batch_box(args) -> tuple[SystemDefinition, dict]:
 
 
 def make_batch(n_boxes: int = 64, seed: int = 4, n_range=(2000, 50000), body_sizes=(8, 20, 40),
-               phi_range=(0.1, 0.3), errortol: float = 1e-6, workers: int = 1) -> list[tuple[SystemDefinition, dict]]:
+               phi_range=(0.1, 0.35), errortol: float = 1e-6, workers: int = 1) -> list[tuple[SystemDefinition, dict]]:
     """Seeded set of independent periodic boxes of uneven size: N log-uniform in
     n_range, body sizes drawn per body from body_sizes, volume fraction uniform in
     phi_range, net charges U[-2, 2] shifted to a neutral box; Ewald parameters per box
     from batch_box_params.  Returns [(system, params)] per box, the same for any number
-    of worker processes.  The cluster growth of make_system cages above a volume
-    fraction of about 0.33 for mixed 8/20/40-bead bodies, hence the default upper bound
-    of 0.3."""
+    of worker processes.  The boxes grow with make_system's dense_growth, which reaches
+    volume fraction 0.35 with mixed 8/20/40-bead bodies."""
     rng = np.random.default_rng(seed)
     mean_size = float(np.mean(body_sizes))
     jobs = []
