@@ -245,6 +245,13 @@ int contract_impl(cr_context* c, const SysView& v, T* phi, hipStream_t st)
 template <typename T>
 int contract_force_impl(cr_context* c, const SysView& v, const double* q, double* force, hipStream_t st)
 {
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (c->tg.valid && c->fxcp.valid) {
+            CR_HIP(launch_contract_force_fx(c->mp, c->tg, c->fxcp, bin_view(c), c->grid_real.as<float>(), q, force,
+                                            st));
+            return CR_OK;
+        }
+    }
     if (contract_tiled_ok(c, sizeof(T)))
         CR_HIP(launch_contract_tiled<T>(c->mp, c->tg, v, bin_view(c), q, c->grid_real.as<T>(), nullptr, force, st));
     else

@@ -182,6 +182,10 @@ hipError_t launch_spread_fx_bound(const MeshParams& mp, const TileGeom& tg, cons
 FxPlan fx_contract_plan(const MeshParams& mp, const TileGeom& tg, size_t lds_max);
 hipError_t launch_contract_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
                               const float* grid, float* phi, hipStream_t st);
+// FP32-bulk reciprocal force (the gradient of each bead's window) on the same tiles and
+// plan: force [4N] = the reciprocal-space force of the charges q.
+hipError_t launch_contract_force_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
+                                    const float* grid, const double* q, double* force, hipStream_t st);
 
 // ---- k_se.hip (unbinned fallbacks) ----
 template <typename T>

@@ -323,13 +323,14 @@ __global__ __launch_bounds__(256) void k_fx_gather(MeshParams mp, TileGeom tg, l
 // read into LDS once (float4 rows where the tile does not wrap in z), the chunk's axis
 // weights are staged as in the spread, and each wavefront reduces one bead's window
 // (lanes over the (y, z) plane, each walking x with compile-time LDS offsets).
-template <int PT, int TX, int TY, int TZ>
+template <int PT, int TX, int TY, int TZ, bool FORCE>
 __global__ __launch_bounds__(kFxThreads) void k_contract_fx(MeshParams mp, TileGeom tg, int ch,
                                                             const unsigned* __restrict__ tstart,
                                                             const unsigned* __restrict__ sorted,
                                                             const int* __restrict__ rec_off,
                                                             const float4* __restrict__ rec_u,
-                                                            const float* __restrict__ grid, float* __restrict__ phi)
+                                                            const float* __restrict__ grid, float* __restrict__ phi,
+                                                            const double* __restrict__ q, double* __restrict__ force)
 {
     constexpr bool CT = TX > 0;
     constexpr int WS = fx_wstride(PT);
@@ -415,18 +416,52 @@ __global__ __launch_bounds__(kFxThreads) void k_contract_fx(MeshParams mp, TileG
 #pragma unroll
                 for (int p = 0; p < PT; ++p) wx[p] = wi[p];
             }
-            float acc = 0.0f;
+            if constexpr (!FORCE) {
+                float acc = 0.0f;
 #pragma unroll
-            for (int pair = lane; pair < PT * PT; pair += kFxWave) {
-                const int py = pair / PT, pz = pair - py * PT;
-                const float* col = ext + ox * xs + (oy + py) * ys + (oz + pz);
-                float sx = 0.0f;
+                for (int pair = lane; pair < PT * PT; pair += kFxWave) {
+                    const int py = pair / PT, pz = pair - py * PT;
+                    const float* col = ext + ox * xs + (oy + py) * ys + (oz + pz);
+                    float sx = 0.0f;
 #pragma unroll
-                for (int px = 0; px < PT; ++px) sx += wx[px] * col[px * xs];
-                acc += wi[PT + py] * wi[2 * PT + pz] * sx;
+                    for (int px = 0; px < PT; ++px) sx += wx[px] * col[px * xs];
+                    acc += wi[PT + py] * wi[2 * PT + pz] * sx;
+                }
+                acc = se::wave_total(acc);
+                if (lane == 0) phi[r.y] = acc * pc;
+            } else {
+                // gradient of the bead's window: node offsets r = (u - p) h per axis
+                const float4 u4 = rec_u[k0 + i];
+                float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll
+                for (int pair = lane; pair < PT * PT; pair += kFxWave) {
+                    const int py = pair / PT, pz = pair - py * PT;
+                    const float* col = ext + ox * xs + (oy + py) * ys + (oz + pz);
+                    float sx = 0.0f, sxr = 0.0f;
+#pragma unroll
+                    for (int px = 0; px < PT; ++px) {
+                        const float wg = wx[px] * col[px * xs];
+                        sx += wg;
+                        sxr += wg * (u4.x - (float)px);
+                    }
+                    const float wyz = wi[PT + py] * wi[2 * PT + pz];
+                    fx += wyz * sxr;
+                    fy += wyz * sx * (u4.y - (float)py);
+                    fz += wyz * sx * (u4.z - (float)pz);
+                }
+                fx = se::wave_total(fx);
+                fy = se::wave_total(fy);
+                fz = se::wave_total(fz);
+                if (lane == 0) {
+                    // F = 2 (h^3 prefac) (2 xi^2) q_i sum_nodes grid w (r / eta), the reference's
+                    // contract_force (cppmodule/ConductorRigid.cu:449)
+                    const double c = 2.0 * (mp.quadW * mp.prefac) * mp.xiterm * (double)(float)q[r.y];
+                    force[4 * (long)r.y + 0] = c * mp.h[0] / mp.eta[0] * (double)fx;
+                    force[4 * (long)r.y + 1] = c * mp.h[1] / mp.eta[1] * (double)fy;
+                    force[4 * (long)r.y + 2] = c * mp.h[2] / mp.eta[2] * (double)fz;
+                    force[4 * (long)r.y + 3] = 0.0;
+                }
             }
-            acc = se::wave_total(acc);
-            if (lane == 0) phi[r.y] = acc * pc;
         }
         __syncthreads();
     }
@@ -496,24 +531,25 @@ hipError_t dispatch_p(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl
     }
 }
 
-template <int PT, int TX, int TY, int TZ>
+template <int PT, int TX, int TY, int TZ, bool FORCE>
 hipError_t launch_contract_kernel(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                                  const float* grid, float* phi, hipStream_t st)
+                                  const float* grid, float* phi, const double* q, double* force, hipStream_t st)
 {
-    auto fn = k_contract_fx<PT, TX, TY, TZ>;
+    auto fn = k_contract_fx<PT, TX, TY, TZ, FORCE>;
     allow_lds_fx(reinterpret_cast<const void*>(fn), pl.lds_bytes);
     hipLaunchKernelGGL(fn, dim3(tg.ntiles), dim3(kFxThreads), pl.lds_bytes, st, mp, tg, pl.chunk, bv.tile_start,
-                       bv.sorted, bv.rec_off, static_cast<const float4*>(bv.rec_win), grid, phi);
+                       bv.sorted, bv.rec_off, static_cast<const float4*>(bv.rec_win), grid, phi, q, force);
     return hipGetLastError();
 }
 
-template <int PT>
+template <int PT, bool FORCE>
 hipError_t contract_shape(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                          const float* grid, float* phi, hipStream_t st)
+                          const float* grid, float* phi, const double* q, double* force, hipStream_t st)
 {
     const int a = tg.T[0], b = tg.T[1], c = tg.T[2];
 #define CR_FXC_SHAPE(X, Y, Z) \
-    if (a == X && b == Y && c == Z) return launch_contract_kernel<PT, X, Y, Z>(mp, tg, pl, bv, grid, phi, st)
+    if (a == X && b == Y && c == Z)  \
+    return launch_contract_kernel<PT, X, Y, Z, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st)
     if constexpr (PT == 8) {
         CR_FXC_SHAPE(16, 16, 16);
         CR_FXC_SHAPE(16, 8, 16);
@@ -526,7 +562,7 @@ hipError_t contract_shape(const MeshParams& mp, const TileGeom& tg, const FxPlan
         CR_FXC_SHAPE(8, 8, 8);
     }
 #undef CR_FXC_SHAPE
-    return launch_contract_kernel<PT, 0, 0, 0>(mp, tg, pl, bv, grid, phi, st);
+    return launch_contract_kernel<PT, 0, 0, 0, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
 }
 
 inline unsigned nblk(long n, int b)
@@ -580,20 +616,33 @@ FxPlan fx_contract_plan(const MeshParams& mp, const TileGeom& tg, size_t lds_max
     return p;
 }
 
-hipError_t launch_contract_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                              const float* grid, float* phi, hipStream_t st)
+template <bool FORCE>
+hipError_t contract_fx_p(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
+                         const float* grid, float* phi, const double* q, double* force, hipStream_t st)
 {
     if (!pl.valid) return hipErrorInvalidValue;
     switch (mp.P) {
-        case 4: return contract_shape<4>(mp, tg, pl, bv, grid, phi, st);
-        case 5: return contract_shape<5>(mp, tg, pl, bv, grid, phi, st);
-        case 6: return contract_shape<6>(mp, tg, pl, bv, grid, phi, st);
-        case 7: return contract_shape<7>(mp, tg, pl, bv, grid, phi, st);
-        case 8: return contract_shape<8>(mp, tg, pl, bv, grid, phi, st);
-        case 9: return contract_shape<9>(mp, tg, pl, bv, grid, phi, st);
-        case 10: return contract_shape<10>(mp, tg, pl, bv, grid, phi, st);
+        case 4: return contract_shape<4, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
+        case 5: return contract_shape<5, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
+        case 6: return contract_shape<6, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
+        case 7: return contract_shape<7, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
+        case 8: return contract_shape<8, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
+        case 9: return contract_shape<9, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
+        case 10: return contract_shape<10, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_contract_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
+                              const float* grid, float* phi, hipStream_t st)
+{
+    return contract_fx_p<false>(mp, tg, pl, bv, grid, phi, nullptr, nullptr, st);
+}
+
+hipError_t launch_contract_force_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
+                                    const float* grid, const double* q, double* force, hipStream_t st)
+{
+    return contract_fx_p<true>(mp, tg, pl, bv, grid, nullptr, q, force, st);
 }
 
 hipError_t launch_charges_fx(const double* x, int N, float* qf, FxScale* fs, double prefac, hipStream_t st)

@@ -157,3 +157,21 @@ def test_fx_contraction_odd_p(fx_case):
     v = rel(g.contract_after_recip(grid)[1], o.contract(og))
     _report(f"{name} contract", v)
     assert v < 1e-6
+
+
+@pytest.mark.parametrize("tile", [None, (16, 16, 16), (8, 8, 8), (16, 8, 16), (32, 16, 8)])
+def test_fx_force_contraction_matches_oracle(tile, require_gpu):
+    """FP32-bulk reciprocal force (k_contract_fx in force mode): compile-time and run-time
+    tile shapes against the oracle's FP64 contract_force of the same mesh."""
+    s = n1k_system()
+    g = GpuCase(s, precision="fp32", **N1K)
+    if tile is not None:
+        g.cr.set_tiles(tile, 160 * 1024)
+    o = O.Oracle(s, N1K["xi"], N1K["errortol"], rc=N1K["rc"], mesh=N1K["mesh"], P=N1K["P"])
+    q = np.random.default_rng(31).normal(size=s.N)
+    grid = o.spread(q)
+    og = o.recip_grid(grid)
+    F = g.contract_force_after_recip(grid, q)
+    v = rel(F[:, :3], o.contract_force(q, og)[:, :3])
+    _report(f"force contract tile {tile}", v)
+    assert v < 5e-6
