@@ -69,6 +69,13 @@ struct cr_batch {
     double* h_pub = nullptr;  // pinned: 3 doubles per box
     size_t h_pub_n = 0;
     std::vector<TiledBox> spread_boxes, contract_boxes;
+    // the fixed-point path: every box FP32 bulk with valid fixed-point plans
+    bool fx = false, fx_v4 = true;
+    int fx_max_n = 0;
+    long fx_gblocks = 0;
+    size_t lds_fx_s = 0, lds_fx_c = 0;
+    std::vector<FxBox> fx_boxes;
+    DevBuf d_fx;
     std::vector<RowBox> rows;
     std::vector<BatchSeg> segs;
     std::vector<SysView> views;
@@ -93,6 +100,10 @@ struct cr_batch {
 };
 
 namespace {
+
+// A box the batched fixed-point kernels can take: FP32 bulk on binned tiles whose
+// fixed-point spread and contraction plans fit.
+bool batch_fx_capable(const cr_context* c) { return c->fp32() && c->tg.valid && c->fxp.valid && c->fxcp.valid; }
 
 // grid <- Re IFFT(S FFT(grid)) for every box of a mesh group: the z and y passes treat the
 // group as one mesh of K Nx planes; the x pass takes each box's own scale tables.
@@ -125,7 +136,12 @@ int batch_spread_recip(cr_batch* B, const int* skip, const std::vector<char>* ho
     const bool f32 = c0->fp32();
     const int P = c0->mp.P;
     const TiledBox* ds = B->d_spread.as<TiledBox>();
-    if (f32) {
+    if (B->fx) {
+        // fixed-point spread: per-box scale, tile blocks, gather (every node written: no clear)
+        const FxBox* dx = B->d_fx.as<FxBox>();
+        CR_HIP(launch_charges_fx_batch(dx, B->nbox, B->fx_max_n, skip, st));
+        CR_HIP(launch_spread_fx_batch(dx, B->nbox, P, B->total_tiles, B->lds_fx_s, B->fx_gblocks, B->fx_v4, skip, st));
+    } else if (f32) {
         CR_HIP(launch_fill_batch<float>(ds, B->nbox, B->Gmax, skip, st));
         CR_HIP(launch_spread_tiled_batch<float>(ds, B->nbox, B->total_tiles, P, B->lds_spread, skip, st));
     } else {
@@ -160,7 +176,9 @@ int batch_matvec(cr_batch* B, const double* in, double* out, int precond, const 
     const int P = c0->mp.P;
     const TiledBox* dc = B->d_contract.as<TiledBox>();
     CR_RET(batch_spread_recip(B, skip, host_skip, st));
-    if (f32)
+    if (B->fx)
+        CR_HIP(launch_contract_fx_batch(B->d_fx.as<FxBox>(), B->nbox, P, B->total_tiles, B->lds_fx_c, false, skip, st));
+    else if (f32)
         CR_HIP(launch_contract_tiled_batch<float>(dc, B->nbox, B->total_tiles, P, B->lds_contract, skip, st));
     else
         CR_HIP(launch_contract_tiled_batch<double>(dc, B->nbox, B->total_tiles, P, B->lds_contract, skip, st));
@@ -213,9 +231,11 @@ std::vector<const void*> step_signature(const cr_batch* B)
 {
     std::vector<const void*> s = {B->V.p, B->u.p, B->uf.p, B->w.p, B->yraw.p, B->partial.p, B->d_spread.p, B->d_contract.p,
                                   B->d_rows.p, B->d_segs.p, B->d_gs.p, B->d_skip.p, B->d_step.p, B->d_pub.p,
-                                  B->d_blocks.p, B->h_pub};
+                                  B->d_blocks.p, B->h_pub, B->d_fx.p};
     const long shape[] = {B->nbox, B->R, B->n_tot, B->ld, B->seg_blocks, B->bead_blocks, B->rigid_blocks,
-                          B->total_tiles, (long)B->lds_spread, (long)B->lds_contract, B->Gmax, B->total_bodies};
+                          B->total_tiles, (long)B->lds_spread, (long)B->lds_contract, B->Gmax, B->total_bodies,
+                          (long)B->fx, (long)B->fx_v4, (long)B->fx_max_n, B->fx_gblocks, (long)B->lds_fx_s,
+                          (long)B->lds_fx_c};
     for (long v : shape) s.push_back(reinterpret_cast<const void*>(v));
     for (const auto& g : B->groups) {
         s.push_back(g->real.p);
@@ -227,6 +247,9 @@ std::vector<const void*> step_signature(const cr_batch* B)
         s.push_back(c);
         s.push_back(reinterpret_cast<const void*>(c->generation));
         s.push_back(c->phi_recip.p);
+        s.push_back(c->qf.p);
+        s.push_back(c->fxs.p);
+        s.push_back(c->fxblk.p);
     }
     return s;
 }
@@ -368,6 +391,15 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
                     if ((v.N > 0 && (!q_inout[b] || !force[b])) || (v.N_isolated > 0 && !phi_inout[b]))
                         return set_error(CR_ERR_INVALID, "null output array");
                     CR_HIP(launch_objective(v, c->mp.L, c->E0, c->b.as<double>(), sk));
+                    if (batch_fx_capable(c)) {
+                        // the fixed-point scale's window bound for this configuration, and the
+                        // box's charge and block buffers
+                        CR_HIP(c->qf.ensure(sizeof(float) * (size_t)std::max(1, v.N)));
+                        CR_HIP(c->fxblk.ensure(sizeof(int) * (size_t)c->tg.ntiles *
+                                               (size_t)fx_block_ints(c->mp, c->tg)));
+                        CR_HIP(launch_spread_fx_bound(c->mp, c->tg, c->fxp, bin_view(c), c->fxs.as<FxScale>(), sk));
+                        c->fx_bound = true;
+                    }
                     if (block_precond(c)) CR_RET(block_setup(c, v, sk, false));  // inverses: one launch below
                     return csr_width(v, d_width + b, sk);
                 }();
@@ -405,10 +437,48 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
             CR_HIP(hipStreamWaitEvent(st, B->join[k], 0));
         }
     }
+    // the fixed-point path when every box can take it (one gather shape for the launch)
+    B->fx = f32;
+    B->fx_v4 = true;
+    for (int b = 0; b < nbox; ++b) {
+        B->fx = B->fx && batch_fx_capable(ctx[b]);
+        B->fx_v4 = B->fx_v4 && fx_gather_v4(ctx[b]->mp, ctx[b]->tg);
+    }
+    B->fx_boxes.resize(nbox);
+    B->fx_max_n = 0;
+    B->lds_fx_s = B->lds_fx_c = 0;
+    long gblk = 0;
     for (int b = 0; b < nbox; ++b) {
         cr_context* c = ctx[b];
         SysView& v = B->views[b];
         const long n = (long)v.N + v.N_isolated;
+
+        if (B->fx) {
+            FxBox& f = B->fx_boxes[b];
+            f = FxBox{};
+            f.mp = c->mp;
+            f.tg = c->tg;
+            f.box = b;
+            f.tile0 = tiles;
+            f.gblk0 = (int)gblk;
+            f.N = v.N;
+            f.ch = c->fxp.chunk;
+            f.cch = c->fxcp.chunk;
+            f.bstride = fx_block_ints(c->mp, c->tg);
+            const BinView bv = bin_view(c);
+            f.tile_start = bv.tile_start;
+            f.sorted = bv.sorted;
+            f.rec_off = bv.rec_off;
+            f.rec_u = static_cast<const float4*>(bv.rec_win);
+            f.qf = c->qf.as<float>();
+            f.fs = c->fxs.as<FxScale>();
+            f.blocks = c->fxblk.as<int>();
+            f.phi = c->phi_recip.as<float>();
+            gblk += fx_gather_blocks(c->mp, B->fx_v4);
+            B->fx_max_n = std::max(B->fx_max_n, v.N);
+            B->lds_fx_s = std::max(B->lds_fx_s, c->fxp.lds_bytes);
+            B->lds_fx_c = std::max(B->lds_fx_c, c->fxcp.lds_bytes);
+        }
 
         BlockBox& bb = B->blocks[b];
         bb = BlockBox{};
@@ -489,6 +559,7 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     B->total_bodies = bodies;
     B->lds_spread = lds_s;
     B->lds_contract = lds_c;
+    B->fx_gblocks = gblk;
     B->Gmax = Gmax;
 
     // boxes that share a mesh keep their meshes back to back (buffers reused across calls)
@@ -530,6 +601,7 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
                 char* slice = g->real.as<char>() + Tsz * k * (size_t)g->G;
                 B->spread_boxes[b].grid = slice;
                 B->contract_boxes[b].grid = slice;
+                if (B->fx) B->fx_boxes[b].grid = reinterpret_cast<float*>(slice);
             }
             CR_HIP(hipMemcpy(g->sct.p, sct.data(), sizeof(ScaleTables) * K, hipMemcpyHostToDevice));
         }
@@ -557,6 +629,7 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     CR_HIP(B->d_pub.ensure(sizeof(double) * 3 * nbox));
     CR_HIP(B->d_rel.ensure(sizeof(double) * nbox));
     CR_HIP(B->d_blocks.ensure(sizeof(BlockBox) * nbox));
+    CR_HIP(B->d_fx.ensure(sizeof(FxBox) * nbox));
     if (B->h_pub_n < (size_t)(3 * nbox)) {
         if (B->h_pub) (void)hipHostFree(B->h_pub);
         B->h_pub = nullptr;
@@ -570,6 +643,11 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
         B->contract_boxes[b].q = u + B->segs[b].xoff;
         B->contract_boxes[b].pos = B->views[b].pos;
         B->contract_boxes[b].force = force[b];
+        if (B->fx) {
+            B->fx_boxes[b].x = u + B->segs[b].xoff;
+            B->fx_boxes[b].q = u + B->segs[b].xoff;
+            B->fx_boxes[b].force = force[b];
+        }
     }
     std::vector<double> rel(nbox);
     for (int b = 0; b < nbox; ++b) rel[b] = ctx[b]->res.gmres_tol;
@@ -580,6 +658,8 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     CR_HIP(hipMemcpyAsync(B->d_segs.p, B->segs.data(), sizeof(BatchSeg) * nbox, hipMemcpyHostToDevice, st));
     CR_HIP(hipMemcpyAsync(B->d_rel.p, rel.data(), sizeof(double) * nbox, hipMemcpyHostToDevice, st));
     CR_HIP(hipMemcpyAsync(B->d_blocks.p, B->blocks.data(), sizeof(BlockBox) * nbox, hipMemcpyHostToDevice, st));
+    if (B->fx)
+        CR_HIP(hipMemcpyAsync(B->d_fx.p, B->fx_boxes.data(), sizeof(FxBox) * nbox, hipMemcpyHostToDevice, st));
 
     // the block preconditioner's per-body inverses of all boxes in one launch
     if (block_precond(c0))
@@ -758,7 +838,10 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     // force pass: spread of the converged charges, k-space step and reciprocal-space force
     // contraction for all boxes, then the real-space pair forces + q E0 per box
     CR_RET(batch_spread_recip(B, nullptr, nullptr, st));
-    if (c0->fp32())
+    if (B->fx)
+        CR_HIP(launch_contract_fx_batch(B->d_fx.as<FxBox>(), nb, c0->mp.P, B->total_tiles, B->lds_fx_c, true, nullptr,
+                                        st));
+    else if (c0->fp32())
         CR_HIP(launch_contract_tiled_batch<float>(B->d_contract.as<TiledBox>(), nb, B->total_tiles, c0->mp.P,
                                                   B->lds_contract, nullptr, st, true));
     else

@@ -178,6 +178,44 @@ hipError_t launch_spread_fx(const MeshParams& mp, const TileGeom& tg, const FxPl
 // fs->wbits for the binned configuration (once per bead configuration).
 hipError_t launch_spread_fx_bound(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
                                   FxScale* fs, hipStream_t st);
+// Batched boxes on the fixed-point kernels (every box FP32 bulk with valid plans, one P):
+// one descriptor per box, boxes in ascending tile0 and gblk0.
+struct FxBox {
+    MeshParams mp;
+    TileGeom tg;
+    int box;                     // index into the skip flags
+    int tile0;                   // first global tile of this box
+    int gblk0;                   // first gather workgroup of this box
+    int N;                       // beads
+    int ch, cch;                 // staging chunks of the spread and contraction plans
+    long bstride;                // ints per tile block (fx_block_ints)
+    const unsigned* tile_start;  // [ntiles + 1]
+    const unsigned* sorted;      // [N]
+    const int* rec_off;          // [N]
+    const float4* rec_u;         // [N] window records
+    const double* x;             // spread input: the box's charges
+    float* qf;                   // [N] the charges rounded to float
+    FxScale* fs;
+    int* blocks;                 // tile blocks (tg.ntiles x bstride)
+    float* grid;                 // the box's mesh
+    float* phi;                  // potential contraction output [N]
+    const double* q;             // charges of the force contraction
+    double* force;               // force contraction output [4N] (overwritten)
+};
+// whether the gather takes 4 nodes per thread for this mesh and tiling
+bool fx_gather_v4(const MeshParams& mp, const TileGeom& tg);
+// gather workgroups of one box (V = 4 or 1 nodes per thread)
+long fx_gather_blocks(const MeshParams& mp, bool v4);
+// fixed-point scale of every box not skipped: charges of x rounded to float (the same
+// workgroup count for every box, set by the largest, max_n beads), then mult / inv per box
+hipError_t launch_charges_fx_batch(const FxBox* boxes, int nbox, int max_n, const int* skip, hipStream_t st);
+// every box's mesh from its charges: tile kernel over all boxes' tiles, then the gather
+// (gather_blocks workgroups in all)
+hipError_t launch_spread_fx_batch(const FxBox* boxes, int nbox, int P, int total_tiles, size_t lds, long gather_blocks,
+                                  bool v4, const int* skip, hipStream_t st);
+// potential (force = false) or reciprocal-force contraction of every box not skipped
+hipError_t launch_contract_fx_batch(const FxBox* boxes, int nbox, int P, int total_tiles, size_t lds, bool force,
+                                    const int* skip, hipStream_t st);
 // FP32-bulk potential contraction on the binned tiles (k_spread_fx.hip); its own LDS plan.
 FxPlan fx_contract_plan(const MeshParams& mp, const TileGeom& tg, size_t lds_max);
 hipError_t launch_contract_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,

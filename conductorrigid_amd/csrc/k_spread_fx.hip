@@ -66,12 +66,14 @@ __device__ __forceinline__ float wave_nanmax(float m)
     return m;
 }
 
-__global__ __launch_bounds__(kFxChargeThreads) void k_charges_fx(const double* __restrict__ x, int N,
-                                                                 float* __restrict__ qf, FxScale* __restrict__ fs)
+// Block blk of nblk rounds its share of x to float and leaves |q|'s NaN-propagating
+// maximum in fs->part[blk].
+__device__ __forceinline__ void charges_fx_part(const double* __restrict__ x, int N, float* __restrict__ qf,
+                                                FxScale* __restrict__ fs, int blk, int nblk)
 {
     __shared__ float wmax[kFxChargeThreads / kFxWave];
     float m = 0.0f;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    for (long i = (long)blk * blockDim.x + threadIdx.x; i < N; i += (long)nblk * blockDim.x) {
         const float v = (float)x[i];
         qf[i] = v;
         m = nanmax(m, fabsf(v));
@@ -83,12 +85,18 @@ __global__ __launch_bounds__(kFxChargeThreads) void k_charges_fx(const double* _
     if (threadIdx.x == 0) {
         float bm = wmax[0];
         for (int k = 1; k < kFxChargeThreads / kFxWave; ++k) bm = nanmax(bm, wmax[k]);
-        fs->part[blockIdx.x] = bm;
+        fs->part[blk] = bm;
     }
 }
 
+__global__ __launch_bounds__(kFxChargeThreads) void k_charges_fx(const double* __restrict__ x, int N,
+                                                                 float* __restrict__ qf, FxScale* __restrict__ fs)
+{
+    charges_fx_part(x, N, qf, fs, blockIdx.x, gridDim.x);
+}
+
 // One wavefront: qmax from the partial maxima, then the multiplier and decode factor.
-__global__ __launch_bounds__(kFxWave) void k_fx_scale(int nparts, FxScale* __restrict__ fs, double prefac)
+__device__ __forceinline__ void fx_scale_wave(int nparts, FxScale* __restrict__ fs, double prefac)
 {
     const int lane = threadIdx.x;
     float qm = 0.0f;
@@ -111,6 +119,29 @@ __global__ __launch_bounds__(kFxWave) void k_fx_scale(int nparts, FxScale* __res
     fs->mult = mult;
     fs->inv = inv;
     fs->qmax = qm;
+}
+
+__global__ __launch_bounds__(kFxWave) void k_fx_scale(int nparts, FxScale* __restrict__ fs, double prefac)
+{
+    fx_scale_wave(nparts, fs, prefac);
+}
+
+// Batched boxes: blockIdx.y is the box (the same block count for every box; blocks past a
+// small box's beads report a zero partial).
+__global__ __launch_bounds__(kFxChargeThreads) void k_charges_fx_batch(const FxBox* __restrict__ boxes,
+                                                                       const int* __restrict__ skip)
+{
+    const FxBox& d = boxes[blockIdx.y];
+    if (skip && skip[d.box]) return;
+    charges_fx_part(d.x, d.N, d.qf, d.fs, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(kFxWave) void k_fx_scale_batch(const FxBox* __restrict__ boxes, int nparts,
+                                                             const int* __restrict__ skip)
+{
+    const FxBox& d = boxes[blockIdx.x];
+    if (skip && skip[d.box]) return;
+    fx_scale_wave(nparts, d.fs, d.mp.prefac);
 }
 
 // Node walk over an EX x EY x EZ box in row-major order with carries (no divisions).
@@ -146,13 +177,11 @@ __host__ __device__ constexpr int fx_wstride(int P) { return (3 * P + 3) / 4 * 4
 // become instruction offsets).  ONES: accumulate sum_b w_b with a per-tile multiplier and
 // report the tile's largest node into fs->wbits instead of writing the mesh.
 template <int PT, int TX, int TY, int TZ, bool ONES>
-__global__ __launch_bounds__(kFxThreads) void k_spread_fx(MeshParams mp, TileGeom tg, int ch,
-                                                          const unsigned* __restrict__ tstart,
-                                                          const unsigned* __restrict__ sorted,
-                                                          const int* __restrict__ rec_off,
-                                                          const float4* __restrict__ rec_u,
-                                                          const float* __restrict__ qf, FxScale* __restrict__ fs,
-                                                          long bstride, int* __restrict__ blocks)
+__device__ __forceinline__ void spread_fx_tile(const MeshParams& mp, const TileGeom& tg, int t, int ch,
+                                               const unsigned* __restrict__ tstart,
+                                               const unsigned* __restrict__ sorted, const int* __restrict__ rec_off,
+                                               const float4* __restrict__ rec_u, const float* __restrict__ qf,
+                                               FxScale* __restrict__ fs, long bstride, int* __restrict__ blocks)
 {
     constexpr bool CT = TX > 0;
     constexpr int WS = fx_wstride(PT);
@@ -161,7 +190,6 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx(MeshParams mp, TileGeo
     const int ys = (EZ + 1) & ~1, xs = EY * ys, esz = EX * xs;
     extern __shared__ __align__(16) char smem[];
     __shared__ unsigned tile_max;
-    const int t = fx_tile(blockIdx.x, tg.ntiles);
     const unsigned kb = tstart[t], ke = tstart[t + 1];
     if (kb == ke) {
         // an empty bin: a block of zeros (the gather reads every covering block)
@@ -260,6 +288,43 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx(MeshParams mp, TileGeo
     for (int k = (esz / 4) * 4 + threadIdx.x; k < esz; k += kFxThreads) blk[k] = (int)ext[k];
 }
 
+template <int PT, int TX, int TY, int TZ, bool ONES>
+__global__ __launch_bounds__(kFxThreads) void k_spread_fx(MeshParams mp, TileGeom tg, int ch,
+                                                          const unsigned* __restrict__ tstart,
+                                                          const unsigned* __restrict__ sorted,
+                                                          const int* __restrict__ rec_off,
+                                                          const float4* __restrict__ rec_u,
+                                                          const float* __restrict__ qf, FxScale* __restrict__ fs,
+                                                          long bstride, int* __restrict__ blocks)
+{
+    spread_fx_tile<PT, TX, TY, TZ, ONES>(mp, tg, fx_tile(blockIdx.x, tg.ntiles), ch, tstart, sorted, rec_off, rec_u,
+                                         qf, fs, bstride, blocks);
+}
+
+// Batched boxes: tiles of all boxes in one launch; a box owns the global tiles
+// [tile0, tile0 + ntiles) (ascending with the box), found by binary search.
+__device__ __forceinline__ int fx_box_of(const FxBox* __restrict__ boxes, int nbox, int g, bool by_tile)
+{
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((by_tile ? boxes[mid].tile0 : boxes[mid].gblk0) <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int PT>
+__global__ __launch_bounds__(kFxThreads) void k_spread_fx_batch(const FxBox* __restrict__ boxes, int nbox, int total,
+                                                                const int* __restrict__ skip)
+{
+    const int g = fx_tile(blockIdx.x, total);
+    const FxBox& d = boxes[fx_box_of(boxes, nbox, g, true)];
+    if (skip && skip[d.box]) return;
+    spread_fx_tile<PT, 0, 0, 0, false>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off, d.rec_u,
+                                       d.qf, d.fs, d.bstride, d.blocks);
+}
+
 // The mesh from the tiles' blocks: a node sums the blocks that cover it -- its own tile's
 // and, within P - 1 nodes of a tile's low face, the previous tile's along that axis (two at
 // most per axis, as P - 1 <= the tile edge) -- and leaves as float, int sum x inv.  Integer
@@ -268,15 +333,14 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx(MeshParams mp, TileGeo
 // consecutive z nodes of one tile (tile edge, block row pitch and Nz multiples of 4; the
 // row padding past EZ holds zeros) with 16-byte loads, all of them issued together.
 template <int V>
-__global__ __launch_bounds__(256) void k_fx_gather(MeshParams mp, TileGeom tg, long bstride,
-                                                   const int* __restrict__ blocks, const FxScale* __restrict__ fs,
-                                                   float* __restrict__ mesh)
+__device__ __forceinline__ void gather_fx_nodes(const MeshParams& mp, const TileGeom& tg, long bstride,
+                                                const int* __restrict__ blocks, const FxScale* __restrict__ fs,
+                                                float* __restrict__ mesh, long gid)
 {
     const int T0 = tg.T[0], T1 = tg.T[1], T2 = tg.T[2];
     const int EX = T0 + mp.P - 1, EY = T1 + mp.P - 1, EZ = T2 + mp.P - 1;
     const int ys = (EZ + 1) & ~1;
     const int nq = mp.Nz / V;
-    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long rowid = gid / nq;
     if (rowid >= (long)mp.Nx * mp.Ny) return;
     const int gz = (int)(gid - rowid * nq) * V;
@@ -318,19 +382,37 @@ __global__ __launch_bounds__(256) void k_fx_gather(MeshParams mp, TileGeom tg, l
     }
 }
 
+template <int V>
+__global__ __launch_bounds__(256) void k_fx_gather(MeshParams mp, TileGeom tg, long bstride,
+                                                   const int* __restrict__ blocks, const FxScale* __restrict__ fs,
+                                                   float* __restrict__ mesh)
+{
+    gather_fx_nodes<V>(mp, tg, bstride, blocks, fs, mesh, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Batched boxes: a box owns the workgroups [gblk0, gblk0 + its count) of the launch.
+template <int V>
+__global__ __launch_bounds__(256) void k_fx_gather_batch(const FxBox* __restrict__ boxes, int nbox,
+                                                         const int* __restrict__ skip)
+{
+    const FxBox& d = boxes[fx_box_of(boxes, nbox, (int)blockIdx.x, false)];
+    if (skip && skip[d.box]) return;
+    gather_fx_nodes<V>(d.mp, d.tg, d.bstride, d.blocks, d.fs, d.grid,
+                       (long)(blockIdx.x - d.gblk0) * blockDim.x + threadIdx.x);
+}
+
 // FP32-bulk potential contraction (cppmodule/ConductorRigid.cu:260-360) on the same
 // binned tiles: phi_i = h^3 prefac sum_window w(node) grid(node).  The extended tile is
 // read into LDS once (float4 rows where the tile does not wrap in z), the chunk's axis
 // weights are staged as in the spread, and each wavefront reduces one bead's window
 // (lanes over the (y, z) plane, each walking x with compile-time LDS offsets).
 template <int PT, int TX, int TY, int TZ, bool FORCE>
-__global__ __launch_bounds__(kFxThreads) void k_contract_fx(MeshParams mp, TileGeom tg, int ch,
-                                                            const unsigned* __restrict__ tstart,
-                                                            const unsigned* __restrict__ sorted,
-                                                            const int* __restrict__ rec_off,
-                                                            const float4* __restrict__ rec_u,
-                                                            const float* __restrict__ grid, float* __restrict__ phi,
-                                                            const double* __restrict__ q, double* __restrict__ force)
+__device__ __forceinline__ void contract_fx_tile(const MeshParams& mp, const TileGeom& tg, int t, int ch,
+                                                 const unsigned* __restrict__ tstart,
+                                                 const unsigned* __restrict__ sorted, const int* __restrict__ rec_off,
+                                                 const float4* __restrict__ rec_u, const float* __restrict__ grid,
+                                                 float* __restrict__ phi, const double* __restrict__ q,
+                                                 double* __restrict__ force)
 {
     constexpr bool CT = TX > 0;
     constexpr int WS = fx_wstride(PT);
@@ -338,7 +420,6 @@ __global__ __launch_bounds__(kFxThreads) void k_contract_fx(MeshParams mp, TileG
     const int EX = T0 + PT - 1, EY = T1 + PT - 1, EZ = T2 + PT - 1;
     const int ys = (EZ + 3) & ~3, xs = EY * ys, esz = EX * xs;
     extern __shared__ __align__(16) char smem[];
-    const int t = fx_tile(blockIdx.x, tg.ntiles);
     const unsigned kb = tstart[t], ke = tstart[t + 1];
     if (kb == ke) return;
     const int tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
@@ -465,6 +546,30 @@ __global__ __launch_bounds__(kFxThreads) void k_contract_fx(MeshParams mp, TileG
         }
         __syncthreads();
     }
+}
+
+template <int PT, int TX, int TY, int TZ, bool FORCE>
+__global__ __launch_bounds__(kFxThreads) void k_contract_fx(MeshParams mp, TileGeom tg, int ch,
+                                                            const unsigned* __restrict__ tstart,
+                                                            const unsigned* __restrict__ sorted,
+                                                            const int* __restrict__ rec_off,
+                                                            const float4* __restrict__ rec_u,
+                                                            const float* __restrict__ grid, float* __restrict__ phi,
+                                                            const double* __restrict__ q, double* __restrict__ force)
+{
+    contract_fx_tile<PT, TX, TY, TZ, FORCE>(mp, tg, fx_tile(blockIdx.x, tg.ntiles), ch, tstart, sorted, rec_off, rec_u,
+                                            grid, phi, q, force);
+}
+
+template <int PT, bool FORCE>
+__global__ __launch_bounds__(kFxThreads) void k_contract_fx_batch(const FxBox* __restrict__ boxes, int nbox, int total,
+                                                                  const int* __restrict__ skip)
+{
+    const int g = fx_tile(blockIdx.x, total);
+    const FxBox& d = boxes[fx_box_of(boxes, nbox, g, true)];
+    if (skip && skip[d.box]) return;
+    contract_fx_tile<PT, 0, 0, 0, FORCE>(d.mp, d.tg, g - d.tile0, d.cch, d.tile_start, d.sorted, d.rec_off, d.rec_u,
+                                         d.grid, d.phi, d.q, d.force);
 }
 
 void allow_lds_fx(const void* fn, size_t bytes)
@@ -653,6 +758,17 @@ hipError_t launch_charges_fx(const double* x, int N, float* qf, FxScale* fs, dou
     return hipGetLastError();
 }
 
+bool fx_gather_v4(const MeshParams& mp, const TileGeom& tg)
+{
+    const long ys = ((long)tg.T[2] + mp.P) & ~1L;  // block row pitch, (EZ + 1) & ~1
+    return tg.T[2] % 4 == 0 && ys % 4 == 0 && mp.Nz % 4 == 0;
+}
+
+long fx_gather_blocks(const MeshParams& mp, bool v4)
+{
+    return (long)nblk((long)mp.Nx * mp.Ny * (mp.Nz / (v4 ? 4 : 1)), 256);
+}
+
 long fx_block_ints(const MeshParams& mp, const TileGeom& tg)
 {
     const long EX = tg.T[0] + mp.P - 1, EY = tg.T[1] + mp.P - 1, EZ = tg.T[2] + mp.P - 1;
@@ -668,14 +784,12 @@ hipError_t launch_spread_fx(const MeshParams& mp, const TileGeom& tg, const FxPl
     const long bstride = fx_block_ints(mp, tg);
     hipError_t e = dispatch_p<false>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
     if (e != hipSuccess) return e;
-    const long ys = ((long)tg.T[2] + mp.P) & ~1L;  // block row pitch, (EZ + 1) & ~1
-    const long rows = (long)mp.Nx * mp.Ny;
-    if (tg.T[2] % 4 == 0 && ys % 4 == 0 && mp.Nz % 4 == 0)
-        hipLaunchKernelGGL(k_fx_gather<4>, dim3(nblk(rows * (mp.Nz / 4), 256)), dim3(256), 0, st, mp, tg, bstride, blocks,
-                           fs, mesh);
+    const bool v4 = fx_gather_v4(mp, tg);
+    const unsigned gb = (unsigned)fx_gather_blocks(mp, v4);
+    if (v4)
+        hipLaunchKernelGGL(k_fx_gather<4>, dim3(gb), dim3(256), 0, st, mp, tg, bstride, blocks, fs, mesh);
     else
-        hipLaunchKernelGGL(k_fx_gather<1>, dim3(nblk(rows * mp.Nz, 256)), dim3(256), 0, st, mp, tg, bstride, blocks, fs,
-                           mesh);
+        hipLaunchKernelGGL(k_fx_gather<1>, dim3(gb), dim3(256), 0, st, mp, tg, bstride, blocks, fs, mesh);
     return hipGetLastError();
 }
 
@@ -686,6 +800,82 @@ hipError_t launch_spread_fx_bound(const MeshParams& mp, const TileGeom& tg, cons
     hipError_t e = launch_fill<unsigned>(&fs->wbits, 0u, 1, st);
     if (e != hipSuccess) return e;
     return dispatch_p<true>(mp, tg, pl, bv, nullptr, fs, 0, nullptr, st);
+}
+
+hipError_t launch_charges_fx_batch(const FxBox* boxes, int nbox, int max_n, const int* skip, hipStream_t st)
+{
+    if (nbox <= 0) return hipSuccess;
+    const int nparts = (int)std::min<unsigned>(kFxParts, nblk(std::max(max_n, 1), kFxChargeThreads));
+    hipLaunchKernelGGL(k_charges_fx_batch, dim3(nparts, nbox), dim3(kFxChargeThreads), 0, st, boxes, skip);
+    hipLaunchKernelGGL(k_fx_scale_batch, dim3(nbox), dim3(kFxWave), 0, st, boxes, nparts, skip);
+    return hipGetLastError();
+}
+
+namespace {
+template <int PT>
+hipError_t spread_fx_batch_p(const FxBox* boxes, int nbox, int total_tiles, size_t lds, const int* skip,
+                             hipStream_t st)
+{
+    auto fn = k_spread_fx_batch<PT>;
+    allow_lds_fx(reinterpret_cast<const void*>(fn), lds);
+    hipLaunchKernelGGL(fn, dim3(total_tiles), dim3(kFxThreads), lds, st, boxes, nbox, total_tiles, skip);
+    return hipGetLastError();
+}
+
+template <int PT>
+hipError_t contract_fx_batch_p(const FxBox* boxes, int nbox, int total_tiles, size_t lds, bool force,
+                               const int* skip, hipStream_t st)
+{
+    if (force) {
+        auto fn = k_contract_fx_batch<PT, true>;
+        allow_lds_fx(reinterpret_cast<const void*>(fn), lds);
+        hipLaunchKernelGGL(fn, dim3(total_tiles), dim3(kFxThreads), lds, st, boxes, nbox, total_tiles, skip);
+    } else {
+        auto fn = k_contract_fx_batch<PT, false>;
+        allow_lds_fx(reinterpret_cast<const void*>(fn), lds);
+        hipLaunchKernelGGL(fn, dim3(total_tiles), dim3(kFxThreads), lds, st, boxes, nbox, total_tiles, skip);
+    }
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_spread_fx_batch(const FxBox* boxes, int nbox, int P, int total_tiles, size_t lds, long gather_blocks,
+                                  bool v4, const int* skip, hipStream_t st)
+{
+    if (nbox <= 0 || total_tiles <= 0) return hipSuccess;
+    hipError_t e;
+    switch (P) {
+        case 4: e = spread_fx_batch_p<4>(boxes, nbox, total_tiles, lds, skip, st); break;
+        case 5: e = spread_fx_batch_p<5>(boxes, nbox, total_tiles, lds, skip, st); break;
+        case 6: e = spread_fx_batch_p<6>(boxes, nbox, total_tiles, lds, skip, st); break;
+        case 7: e = spread_fx_batch_p<7>(boxes, nbox, total_tiles, lds, skip, st); break;
+        case 8: e = spread_fx_batch_p<8>(boxes, nbox, total_tiles, lds, skip, st); break;
+        case 9: e = spread_fx_batch_p<9>(boxes, nbox, total_tiles, lds, skip, st); break;
+        case 10: e = spread_fx_batch_p<10>(boxes, nbox, total_tiles, lds, skip, st); break;
+        default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+    if (v4)
+        hipLaunchKernelGGL(k_fx_gather_batch<4>, dim3((unsigned)gather_blocks), dim3(256), 0, st, boxes, nbox, skip);
+    else
+        hipLaunchKernelGGL(k_fx_gather_batch<1>, dim3((unsigned)gather_blocks), dim3(256), 0, st, boxes, nbox, skip);
+    return hipGetLastError();
+}
+
+hipError_t launch_contract_fx_batch(const FxBox* boxes, int nbox, int P, int total_tiles, size_t lds, bool force,
+                                    const int* skip, hipStream_t st)
+{
+    if (nbox <= 0 || total_tiles <= 0) return hipSuccess;
+    switch (P) {
+        case 4: return contract_fx_batch_p<4>(boxes, nbox, total_tiles, lds, force, skip, st);
+        case 5: return contract_fx_batch_p<5>(boxes, nbox, total_tiles, lds, force, skip, st);
+        case 6: return contract_fx_batch_p<6>(boxes, nbox, total_tiles, lds, force, skip, st);
+        case 7: return contract_fx_batch_p<7>(boxes, nbox, total_tiles, lds, force, skip, st);
+        case 8: return contract_fx_batch_p<8>(boxes, nbox, total_tiles, lds, force, skip, st);
+        case 9: return contract_fx_batch_p<9>(boxes, nbox, total_tiles, lds, force, skip, st);
+        case 10: return contract_fx_batch_p<10>(boxes, nbox, total_tiles, lds, force, skip, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace cr
