@@ -64,6 +64,8 @@ struct cr_batch {
     DevBuf V, u, w, x, b, Mb, yraw, partial, uf;
     DevBuf d_spread, d_contract, d_rows, d_segs, d_gs, d_skip, d_finished, d_step, d_pub, d_rel, d_blocks;
     DevBuf d_width;  // ELL width per box
+    DevBuf d_pairs, d_fpairs;  // descriptors of the batched ELL build and pair forces
+    std::vector<PairBox> pair_boxes, fpair_boxes;  // their host copies (alive while the copies run)
     int total_bodies = 0;
     std::vector<BlockBox> blocks;
     double* h_pub = nullptr;  // pinned: 3 doubles per box
@@ -429,12 +431,30 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
         std::vector<unsigned> width(nbox);
         CR_HIP(hipMemcpyAsync(width.data(), d_width, sizeof(unsigned) * nbox, hipMemcpyDeviceToHost, st));
         CR_HIP(hipStreamSynchronize(st));
-        CR_HIP(hipEventRecord(B->fork, st));
-        for (int k = 0; k < kSideStreams; ++k) CR_HIP(hipStreamWaitEvent(B->side[k], B->fork, 0));
-        for (int b = 0; b < nbox; ++b) CR_RET(csr_build(ctx[b], B->views[b], width[b], B->side[b % kSideStreams]));
-        for (int k = 0; k < kSideStreams; ++k) {
-            CR_HIP(hipEventRecord(B->join[k], B->side[k]));
-            CR_HIP(hipStreamWaitEvent(st, B->join[k], 0));
+        // the ELL rows of all boxes in one launch (each box's ELL sized by its width)
+        std::vector<PairBox>& pb = B->pair_boxes;
+        pb.clear();
+        std::vector<int> built;
+        int pblk = 0;
+        for (int b = 0; b < nbox; ++b) {
+            cr_context* c = ctx[b];
+            const SysView& v = B->views[b];
+            if (c->csr_ready || v.N <= 0) continue;
+            PairBox d;
+            CR_RET(csr_pair_box(c, v, width[b], &d));
+            d.blk0 = pblk;
+            pblk += (v.N + 255) / 256;
+            pb.push_back(d);
+            built.push_back(b);
+        }
+        if (!pb.empty()) {
+            CR_HIP(B->d_pairs.ensure(sizeof(PairBox) * pb.size()));
+            CR_HIP(hipMemcpyAsync(B->d_pairs.p, pb.data(), sizeof(PairBox) * pb.size(), hipMemcpyHostToDevice, st));
+            if (f32)
+                CR_HIP(launch_real_csr_build_batch<float>(B->d_pairs.as<PairBox>(), (int)pb.size(), pblk, st));
+            else
+                CR_HIP(launch_real_csr_build_batch<double>(B->d_pairs.as<PairBox>(), (int)pb.size(), pblk, st));
+            for (int b : built) ctx[b]->csr_ready = true;
         }
     }
     // the fixed-point path when every box can take it (one gather shape for the launch)
@@ -847,14 +867,27 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     else
         CR_HIP(launch_contract_tiled_batch<double>(B->d_contract.as<TiledBox>(), nb, B->total_tiles, c0->mp.P,
                                                    B->lds_contract, nullptr, st, true));
-    // per-box pair forces, spread over the side streams
-    CR_HIP(hipEventRecord(B->fork, st));
-    for (int k = 0; k < kSideStreams; ++k) CR_HIP(hipStreamWaitEvent(B->side[k], B->fork, 0));
-    for (int b = 0; b < nb; ++b)
-        CR_RET(real_space_force_any(ctx[b], B->views[b], x + B->segs[b].xoff, force[b], B->side[b % kSideStreams]));
-    for (int k = 0; k < kSideStreams; ++k) {
-        CR_HIP(hipEventRecord(B->join[k], B->side[k]));
-        CR_HIP(hipStreamWaitEvent(st, B->join[k], 0));
+    // the pair forces + q E0 of all boxes in one launch
+    {
+        std::vector<PairBox>& pb = B->fpair_boxes;
+        pb.clear();
+        int pblk = 0;
+        for (int b = 0; b < nb; ++b) {
+            if (B->views[b].N <= 0) continue;
+            PairBox d;
+            force_pair_box(ctx[b], B->views[b], x + B->segs[b].xoff, force[b], &d);
+            d.blk0 = pblk;
+            pblk += (B->views[b].N + 255) / 256;
+            pb.push_back(d);
+        }
+        if (!pb.empty()) {
+            CR_HIP(B->d_fpairs.ensure(sizeof(PairBox) * pb.size()));
+            CR_HIP(hipMemcpyAsync(B->d_fpairs.p, pb.data(), sizeof(PairBox) * pb.size(), hipMemcpyHostToDevice, st));
+            if (c0->fp32())
+                CR_HIP(launch_real_space_force_batch<float>(B->d_fpairs.as<PairBox>(), (int)pb.size(), pblk, st));
+            else
+                CR_HIP(launch_real_space_force_batch<double>(B->d_fpairs.as<PairBox>(), (int)pb.size(), pblk, st));
+        }
     }
     CR_HIP(hipEventRecord(B->done_ev, st));
     CR_HIP(hipStreamWaitEvent(caller, B->done_ev, 0));

@@ -280,15 +280,47 @@ int ensure_csr(cr_context* c, const SysView& v, hipStream_t st)
     return csr_build(c, v, width, st);
 }
 
-int csr_build(cr_context* c, const SysView& v, unsigned width, hipStream_t st)
+// the ELL arrays for rows of at most width entries
+int csr_alloc(cr_context* c, const SysView& v, unsigned width)
 {
-    if (c->csr_ready || v.N <= 0) return CR_OK;
     c->rs_ldn = ((long)v.N + 63) / 64 * 64;
     const size_t cells = std::max<size_t>(1, (size_t)width * (size_t)c->rs_ldn);
     const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
     CR_HIP(c->rs_col.ensure(sizeof(unsigned) * cells));
     CR_HIP(c->rs_val.ensure(Tsz * cells));
-    CR_HIP(c->rs_len.ensure(sizeof(unsigned) * (size_t)v.N));
+    CR_HIP(c->rs_len.ensure(sizeof(unsigned) * (size_t)std::max(1, v.N)));
+    return CR_OK;
+}
+
+int csr_pair_box(cr_context* c, const SysView& v, unsigned width, PairBox* pb)
+{
+    CR_RET(csr_alloc(c, v, width));
+    *pb = PairBox{};
+    pb->rp = c->rp;
+    pb->s = v;
+    pb->table = c->fp32() ? static_cast<const void*>(table<float>(c, false)) : static_cast<const void*>(table<double>(c, false));
+    pb->ldn = c->rs_ldn;
+    pb->col = c->rs_col.as<unsigned>();
+    pb->val = c->rs_val.p;
+    pb->row_len = c->rs_len.as<unsigned>();
+    return CR_OK;
+}
+
+void force_pair_box(const cr_context* c, const SysView& v, const double* q, double* force, PairBox* pb)
+{
+    *pb = PairBox{};
+    pb->rp = c->rp;
+    pb->s = v;
+    pb->table = c->fp32() ? static_cast<const void*>(table<float>(c, true)) : static_cast<const void*>(table<double>(c, true));
+    pb->q = q;
+    pb->force = force;
+    for (int d = 0; d < 3; ++d) pb->E0[d] = c->E0[d];
+}
+
+int csr_build(cr_context* c, const SysView& v, unsigned width, hipStream_t st)
+{
+    if (c->csr_ready || v.N <= 0) return CR_OK;
+    CR_RET(csr_alloc(c, v, width));
     if (c->fp32())
         CR_HIP(launch_real_csr_build<float>(c->rp, v, table<float>(c, false), c->rs_ldn, c->rs_col.as<unsigned>(),
                                             c->rs_val.as<float>(), c->rs_len.as<unsigned>(), st));

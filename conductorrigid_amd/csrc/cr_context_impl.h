@@ -93,6 +93,11 @@ int ensure_csr(cr_context* c, const SysView& v, hipStream_t st);
 // dwidth (device), then, with the width read back, the allocation and the build.
 int csr_width(const SysView& v, unsigned* dwidth, hipStream_t st);
 int csr_build(cr_context* c, const SysView& v, unsigned width, hipStream_t st);
+// The batched ELL build's part for one box: its ELL allocated for width and described in
+// *pb (the caller queues the build and then sets csr_ready).
+int csr_pair_box(cr_context* c, const SysView& v, unsigned width, PairBox* pb);
+// The batched pair forces' descriptor of one box (force += pair forces + q E0).
+void force_pair_box(const cr_context* c, const SysView& v, const double* q, double* force, PairBox* pb);
 // grid <- Re IFFT(S FFT(grid)) on the context's mesh, in its precision.
 int recip_any(cr_context* c, hipStream_t st);
 // FFT plans of the context's precision (x, y, z).

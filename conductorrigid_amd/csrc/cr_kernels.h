@@ -52,6 +52,22 @@ struct SysView {
     const unsigned* n_neigh;
 };
 
+// Batched boxes' pair kernels (ELL build, real-space forces): one descriptor per box,
+// boxes in ascending blk0; a box owns the workgroups [blk0, blk0 + ceil(N / 256)).
+struct PairBox {
+    RealParams rp;
+    SysView s;
+    const void* table;   // potential table (ELL build) or gradient table (forces), T
+    long ldn;            // ELL leading dimension
+    unsigned* col;       // ELL column indices
+    void* val;           // ELL values (T)
+    unsigned* row_len;   // ELL row lengths
+    const double* q;     // charges (forces)
+    double* force;       // [4N] forces: += the pair forces and q E0
+    double E0[3];
+    int blk0;
+};
+
 // Mesh tiling for the owner-computes spread: tile edges divide the mesh and are at
 // least P-1 nodes, so a bead's window touches its own tile and the next one per axis.
 struct TileGeom {
@@ -305,6 +321,12 @@ hipError_t launch_max_u32(const unsigned* a, int n, unsigned* out, hipStream_t s
 template <typename T>
 hipError_t launch_real_space_force(const RealParams& rp, const SysView& s, const double* q, const T* gtable,
                                    const double E0[3], double* force, hipStream_t st);
+// The ELL build and the pair forces of all boxes of a batch, one launch each
+// (total_blocks = the sum of ceil(N / 256) over the boxes).
+template <typename T>
+hipError_t launch_real_csr_build_batch(const PairBox* boxes, int nbox, int total_blocks, hipStream_t st);
+template <typename T>
+hipError_t launch_real_space_force_batch(const PairBox* boxes, int nbox, int total_blocks, hipStream_t st);
 hipError_t launch_constraint_rows(const SysView& s, const double* q, double* y_bottom, int precond, hipStream_t st);
 hipError_t launch_precondition(const SysView& s, const double* x, double* y, hipStream_t st);
 hipError_t launch_objective(const SysView& s, const double L[3], const double E0[3], double* b, hipStream_t st);

@@ -82,10 +82,11 @@ struct NlSpan {
     unsigned lo, hi;  // staged entries [lo, hi)
 };
 
-__device__ __forceinline__ NlSpan stage_nlist(const SysView& s, unsigned* sh)
+// blk: the workgroup's block of beads [blk * blockDim.x, ...) in the system s
+__device__ __forceinline__ NlSpan stage_nlist(const SysView& s, unsigned* sh, int blk)
 {
     __shared__ unsigned bounds[2];
-    const int i0 = blockIdx.x * blockDim.x;
+    const int i0 = blk * blockDim.x;
     const int i1 = min(s.N, i0 + (int)blockDim.x) - 1;
     if (threadIdx.x == 0) {
         const unsigned lo = s.head_list[i0], hi = s.head_list[i1] + s.n_neigh[i1];
@@ -106,13 +107,13 @@ __device__ __forceinline__ unsigned nl_at(const SysView& s, const unsigned* sh, 
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_real_csr_build(RealParams rp, SysView s, const T* __restrict__ table,
-                                                        long ldn, unsigned* __restrict__ col, T* __restrict__ val,
-                                                        unsigned* __restrict__ row_len)
+__device__ __forceinline__ void csr_build_rows(const RealParams& rp, const SysView& s, const T* __restrict__ table,
+                                               long ldn, unsigned* __restrict__ col, T* __restrict__ val,
+                                               unsigned* __restrict__ row_len, int blk)
 {
     __shared__ unsigned sh[kNlStage];
-    const NlSpan sp = stage_nlist(s, sh);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const NlSpan sp = stage_nlist(s, sh, blk);
+    const int i = blk * blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     const double4 ri = s.pos[i];
     const unsigned head = s.head_list[i], nn = s.n_neigh[i];
@@ -131,6 +132,34 @@ __global__ __launch_bounds__(256) void k_real_csr_build(RealParams rp, SysView s
         }
     }
     row_len[i] = cnt;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_real_csr_build(RealParams rp, SysView s, const T* __restrict__ table,
+                                                        long ldn, unsigned* __restrict__ col, T* __restrict__ val,
+                                                        unsigned* __restrict__ row_len)
+{
+    csr_build_rows<T>(rp, s, table, ldn, col, val, row_len, blockIdx.x);
+}
+
+// Batched boxes: a box owns the workgroups [blk0, blk0 + ceil(N / 256)) of the launch.
+__device__ __forceinline__ int pair_box_of(const PairBox* __restrict__ boxes, int nbox, int g)
+{
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (boxes[mid].blk0 <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_real_csr_build_batch(const PairBox* __restrict__ boxes, int nbox)
+{
+    const PairBox& d = boxes[pair_box_of(boxes, nbox, blockIdx.x)];
+    csr_build_rows<T>(d.rp, d.s, static_cast<const T*>(d.table), d.ldn, d.col, static_cast<T*>(d.val), d.row_len,
+                      blockIdx.x - d.blk0);
 }
 
 // real_space on the assembled rows: y_i = Phi_{b(i)} - (phi_i + self q_i + sum_j A_ij q_j).
@@ -178,13 +207,13 @@ __global__ void k_add_body_potentials(SysView s, const double* __restrict__ Phi,
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_real_space_force(RealParams rp, SysView s, const double* __restrict__ q,
-                                                          const T* __restrict__ gtable, double E0x, double E0y,
-                                                          double E0z, double* __restrict__ force)
+__device__ __forceinline__ void force_rows(const RealParams& rp, const SysView& s, const double* __restrict__ q,
+                                           const T* __restrict__ gtable, double E0x, double E0y, double E0z,
+                                           double* __restrict__ force, int blk)
 {
     __shared__ unsigned sh[kNlStage];
-    const NlSpan sp = stage_nlist(s, sh);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const NlSpan sp = stage_nlist(s, sh, blk);
+    const int i = blk * blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     const double qi = q[i];
     double Fx = force[4 * i] + qi * E0x;
@@ -213,6 +242,22 @@ __global__ __launch_bounds__(256) void k_real_space_force(RealParams rp, SysView
     force[4 * i + 1] = Fy + (double)fy;
     force[4 * i + 2] = Fz + (double)fz;
     force[4 * i + 3] = 0.0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_real_space_force(RealParams rp, SysView s, const double* __restrict__ q,
+                                                          const T* __restrict__ gtable, double E0x, double E0y,
+                                                          double E0z, double* __restrict__ force)
+{
+    force_rows<T>(rp, s, q, gtable, E0x, E0y, E0z, force, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_real_space_force_batch(const PairBox* __restrict__ boxes, int nbox)
+{
+    const PairBox& d = boxes[pair_box_of(boxes, nbox, blockIdx.x)];
+    force_rows<T>(d.rp, d.s, d.q, static_cast<const T*>(d.table), d.E0[0], d.E0[1], d.E0[2], d.force,
+                  blockIdx.x - d.blk0);
 }
 
 // sum_charges: y_bottom[tag(r)] = sum of bead charges of rigid body r (FP64), optionally
@@ -494,6 +539,26 @@ template hipError_t launch_real_space_csr<float>(const RealParams&, const SysVie
 template hipError_t launch_real_space_csr<double>(const RealParams&, const SysView&, const double*, const double*,
                                                   const double*, long, const unsigned*, const double*,
                                                   const unsigned*, double*, int, hipStream_t);
+template <typename T>
+hipError_t launch_real_csr_build_batch(const PairBox* boxes, int nbox, int total_blocks, hipStream_t st)
+{
+    if (nbox <= 0 || total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_real_csr_build_batch<T>, dim3(total_blocks), dim3(256), 0, st, boxes, nbox);
+    return hipGetLastError();
+}
+template hipError_t launch_real_csr_build_batch<float>(const PairBox*, int, int, hipStream_t);
+template hipError_t launch_real_csr_build_batch<double>(const PairBox*, int, int, hipStream_t);
+
+template <typename T>
+hipError_t launch_real_space_force_batch(const PairBox* boxes, int nbox, int total_blocks, hipStream_t st)
+{
+    if (nbox <= 0 || total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_real_space_force_batch<T>, dim3(total_blocks), dim3(256), 0, st, boxes, nbox);
+    return hipGetLastError();
+}
+template hipError_t launch_real_space_force_batch<float>(const PairBox*, int, int, hipStream_t);
+template hipError_t launch_real_space_force_batch<double>(const PairBox*, int, int, hipStream_t);
+
 hipError_t launch_max_u32(const unsigned* a, int n, unsigned* out, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
