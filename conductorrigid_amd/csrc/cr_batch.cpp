@@ -28,7 +28,6 @@
 #include <exception>
 #include <memory>
 #include <string>
-#include <thread>
 #include <vector>
 
 using namespace cr;
@@ -66,6 +65,8 @@ struct cr_batch {
     DevBuf d_width;  // ELL width per box
     DevBuf d_pairs, d_fpairs;  // descriptors of the batched ELL build and pair forces
     std::vector<PairBox> pair_boxes, fpair_boxes;  // their host copies (alive while the copies run)
+    DevBuf d_prep;                                 // descriptors of the batched preparation
+    std::vector<PrepBox> prep_boxes;
     int total_bodies = 0;
     std::vector<BlockBox> blocks;
     double* h_pub = nullptr;  // pinned: 3 doubles per box
@@ -362,70 +363,71 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     int blk = 0, bblk = 0, rblk = 0, tiles = 0, bodies = 0, nmax_all = 1;
     size_t lds_s = 0, lds_c = 0;
     long Gmax = 0;
-    // binning, records, objective and body grouping of every box, then the ELL widths of
-    // all boxes read back with one synchronisation, then the ELL builds spread over the
-    // side streams (small boxes leave most of the device idle one at a time)
+    // binning, records, objective, body grouping and ELL widths of every box in a handful of
+    // launches over all boxes (each box owns a run of workgroups), then the widths read
+    // back with one synchronisation and the ELL rows of all boxes in one launch
     CR_HIP(B->d_width.ensure(sizeof(unsigned) * nbox));
     unsigned* d_width = B->d_width.as<unsigned>();
     {
-        // ~15 small launches per box: issued from one host thread per side stream, so the
-        // launches of different boxes go out in parallel (each box uses only its context)
-        CR_HIP(hipEventRecord(B->fork, st));
-        for (int k = 0; k < kSideStreams; ++k) CR_HIP(hipStreamWaitEvent(B->side[k], B->fork, 0));
-        int rc[kSideStreams];
-        std::string msg[kSideStreams];
-        auto prepare_boxes = [&](int k) {
-            rc[k] = CR_OK;
-            if (hipSetDevice(B->device) != hipSuccess) {
-                rc[k] = set_error(CR_ERR_DEVICE, "hipSetDevice failed");
-                msg[k] = g_error;
-                return;
+        std::vector<PrepBox>& pb = B->prep_boxes;
+        pb.assign(nbox, PrepBox{});
+        int pblk = 0;
+        for (int b = 0; b < nbox; ++b) {
+            cr_context* c = ctx[b];
+            SysView& v = B->views[b];
+            CR_RET(prepare_host(c, &sys[b], st, &v));
+            const long n = (long)v.N + v.N_isolated;
+            if (n > c->n_cap) return set_error(CR_ERR_INVALID, "system larger than the size given to cr_set_params");
+            if ((v.N > 0 && (!q_inout[b] || !force[b])) || (v.N_isolated > 0 && !phi_inout[b]))
+                return set_error(CR_ERR_INVALID, "null output array");
+            PrepBox& d = pb[b];
+            d.mp = c->mp;
+            d.tg = c->tg;
+            d.s = v;
+            d.blk0 = pblk;
+            d.bin = c->tg.valid ? 1 : 0;
+            d.tag = sys[b].tag;
+            d.body_of_tag = sys[b].body_of_tag;
+            d.iso = c->iso.as<unsigned>();
+            if (d.bin) {
+                d.bead_tile = c->bead_tile.as<unsigned>();
+                d.tile_count = c->tile_count.as<unsigned>();
+                d.tile_start = c->tile_start.as<unsigned>();
+                d.cursor = c->tile_cursor.as<unsigned>();
+                d.sorted = c->sorted.as<unsigned>();
+                d.rec_off = c->rec_off.as<int>();
+                d.rec_win = c->rec_win.p;
             }
-            const hipStream_t sk = B->side[k];
-            for (int b = k; b < nbox && rc[k] == CR_OK; b += kSideStreams) {
-                rc[k] = [&]() -> int {
-                    cr_context* c = ctx[b];
-                    SysView& v = B->views[b];
-                    CR_RET(prepare(c, &sys[b], sk, &v));
-                    const long n = (long)v.N + v.N_isolated;
-                    if (n > c->n_cap)
-                        return set_error(CR_ERR_INVALID, "system larger than the size given to cr_set_params");
-                    if ((v.N > 0 && (!q_inout[b] || !force[b])) || (v.N_isolated > 0 && !phi_inout[b]))
-                        return set_error(CR_ERR_INVALID, "null output array");
-                    CR_HIP(launch_objective(v, c->mp.L, c->E0, c->b.as<double>(), sk));
-                    if (batch_fx_capable(c)) {
-                        // the fixed-point scale's window bound for this configuration, and the
-                        // box's charge and block buffers
-                        CR_HIP(c->qf.ensure(sizeof(float) * (size_t)std::max(1, v.N)));
-                        CR_HIP(c->fxblk.ensure(sizeof(int) * (size_t)c->tg.ntiles *
-                                               (size_t)fx_block_ints(c->mp, c->tg)));
-                        CR_HIP(launch_spread_fx_bound(c->mp, c->tg, c->fxp, bin_view(c), c->fxs.as<FxScale>(), sk));
-                        c->fx_bound = true;
-                    }
-                    if (block_precond(c)) CR_RET(block_setup(c, v, sk, false));  // inverses: one launch below
-                    return csr_width(v, d_width + b, sk);
-                }();
+            d.b = c->b.as<double>();
+            for (int k = 0; k < 3; ++k) d.E0[k] = c->E0[k];
+            if (block_precond(c)) {  // body grouping here, the inverses in one launch below
+                CR_RET(block_alloc(c, v));
+                d.body_count = c->body_count.as<unsigned>();
+                d.body_start = c->body_start.as<unsigned>();
+                d.body_beads = c->body_beads.as<unsigned>();
             }
-            if (rc[k] != CR_OK) msg[k] = g_error;
-        };
-        std::vector<std::thread> workers;
-        std::vector<int> inline_k;  // side streams whose boxes this thread prepares itself
-        for (int k = 1; k < kSideStreams; ++k) {
-            try {
-                workers.emplace_back(prepare_boxes, k);
-            } catch (const std::exception&) {
-                inline_k.push_back(k);  // no thread available: same work, sequentially
+            d.width = d_width + b;
+            if (batch_fx_capable(c)) {
+                // the box's charge and block buffers; its weight bound is zeroed here and
+                // taken over all boxes once the descriptors are up
+                CR_HIP(c->qf.ensure(sizeof(float) * (size_t)std::max(1, v.N)));
+                CR_HIP(c->fxblk.ensure(sizeof(int) * (size_t)c->tg.ntiles * (size_t)fx_block_ints(c->mp, c->tg)));
+                d.wbits = &c->fxs.as<FxScale>()->wbits;
             }
+            long span = std::max<long>({(long)v.N, (long)v.N_rigid * v.Nmax, (long)v.N_isolated + 1, 1L});
+            if (d.bin) span = std::max(span, (long)c->tg.ntiles + 1);
+            pblk += (int)((span + 255) / 256);
         }
-        prepare_boxes(0);
-        for (int k : inline_k) prepare_boxes(k);
-        for (auto& t : workers) t.join();
-        for (int k = 0; k < kSideStreams; ++k) {
-            CR_HIP(hipEventRecord(B->join[k], B->side[k]));
-            CR_HIP(hipStreamWaitEvent(st, B->join[k], 0));
-        }
-        for (int k = 0; k < kSideStreams; ++k)
-            if (rc[k] != CR_OK) return set_error(rc[k], msg[k]);
+        // the objective also goes straight into the batch vector
+        long n_tot = 0;
+        for (int b = 0; b < nbox; ++b) n_tot += (long)B->views[b].N + B->views[b].N_isolated;
+        CR_HIP(B->b.ensure(sizeof(double) * (size_t)std::max(1L, (n_tot + 63) / 64 * 64)));
+        for (long b = 0, o = 0; b < nbox; o += (long)B->views[b].N + B->views[b].N_isolated, ++b)
+            pb[b].bcat = B->b.as<double>() + o;
+        CR_HIP(B->d_prep.ensure(sizeof(PrepBox) * nbox));
+        CR_HIP(hipMemcpyAsync(B->d_prep.p, pb.data(), sizeof(PrepBox) * nbox, hipMemcpyHostToDevice, st));
+        CR_HIP(launch_prep_batch(B->d_prep.as<PrepBox>(), nbox, pblk, f32, st));
+        CR_HIP(launch_objective_batch(B->d_prep.as<PrepBox>(), nbox, pblk, st));
     }
     {
         std::vector<unsigned> width(nbox);
@@ -678,21 +680,23 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
     CR_HIP(hipMemcpyAsync(B->d_segs.p, B->segs.data(), sizeof(BatchSeg) * nbox, hipMemcpyHostToDevice, st));
     CR_HIP(hipMemcpyAsync(B->d_rel.p, rel.data(), sizeof(double) * nbox, hipMemcpyHostToDevice, st));
     CR_HIP(hipMemcpyAsync(B->d_blocks.p, B->blocks.data(), sizeof(BlockBox) * nbox, hipMemcpyHostToDevice, st));
-    if (B->fx)
+    if (B->fx) {
         CR_HIP(hipMemcpyAsync(B->d_fx.p, B->fx_boxes.data(), sizeof(FxBox) * nbox, hipMemcpyHostToDevice, st));
+        // every box's fixed-point weight bound for this configuration (zeroed in the preparation)
+        CR_HIP(launch_spread_fx_bound_batch(B->d_fx.as<FxBox>(), nbox, c0->mp.P, tiles, B->lds_fx_s, st));
+        for (int b = 0; b < nbox; ++b) ctx[b]->fx_bound = true;
+    }
 
     // the block preconditioner's per-body inverses of all boxes in one launch
     if (block_precond(c0))
         CR_HIP(launch_block_setup_batch(B->d_blocks.as<BlockBox>(), nbox, bodies, nmax_all, st));
-    // objective and x0 of every box into the batch vectors, then the preconditioned objective
+    // x0 of every box into the batch vector (the objective is there already), then the preconditioned objective
     double* x = B->x.as<double>();
-    for (int b = 0; b < nbox; ++b) {
-        cr_context* c = ctx[b];
-        const SysView& v = B->views[b];
-        const long o = B->segs[b].xoff, n = B->segs[b].n;
-        CR_HIP(hipMemcpyAsync(B->b.as<double>() + o, c->b.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-        CR_RET(init_x0(c, v, x0_mode, q_inout[b], phi_inout[b], x + o, st));
-    }
+    if (x0_mode == CR_X0_ZERO)
+        CR_HIP(launch_fill<double>(x, 0.0, B->n_tot, st));
+    else
+        for (int b = 0; b < nbox; ++b)
+            CR_RET(init_x0(ctx[b], B->views[b], x0_mode, q_inout[b], phi_inout[b], x + B->segs[b].xoff, st));
     if (block_precond(c0))
         CR_HIP(launch_block_apply_batch(B->d_blocks.as<BlockBox>(), nbox, bodies, B->b.as<double>(),
                                         B->Mb.as<double>(), nullptr, st));

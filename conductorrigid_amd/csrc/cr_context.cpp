@@ -122,7 +122,7 @@ SysView make_view(cr_context* c, const cr_system* s)
 // Validate the system view, size per-bead buffers and refresh the body-tag map; with bin,
 // also sort the beads into the mesh tiles and write their window records (only the mesh
 // operators read those).
-int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out, bool bin)
+int prepare_host(cr_context* c, const cr_system* s, hipStream_t st, SysView* out, bool bin)
 {
     if (!c) return set_error(CR_ERR_INVALID, "null context");
     if (!c->ready) return set_error(CR_ERR_STATE, "cr_set_params has not been called");
@@ -143,7 +143,6 @@ int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out, boo
     const size_t Tsz = c->fp32() ? sizeof(float) : sizeof(double);
     CR_HIP(c->phi_recip.ensure(Tsz * (size_t)std::max(1, s->N)));
     CR_HIP(c->iso.ensure(sizeof(unsigned) * (size_t)std::max(1, s->N)));
-    CR_HIP(launch_membership(s->N, s->tag, s->body_of_tag, c->iso.as<unsigned>(), st));
     *out = make_view(c, s);
     c->csr_ready = false;  // positions or neighbours may have changed
     c->fx_bound = false;  // the fixed-point weight bound follows the binning
@@ -158,12 +157,23 @@ int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out, boo
         CR_HIP(c->tile_start.ensure(sizeof(unsigned) * (nt + 1)));
         CR_HIP(c->tile_cursor.ensure(sizeof(unsigned) * nt));
         CR_HIP(c->scan_sums.ensure(sizeof(unsigned long long) * ((nt + 1) / 1024 + 4)));
-        CR_HIP(launch_bin_beads(c->mp, c->tg, *out, c->bead_tile.as<unsigned>(), c->tile_count.as<unsigned>(),
-                                c->tile_start.as<unsigned>(), c->tile_cursor.as<unsigned>(), c->sorted.as<unsigned>(),
-                                c->scan_sums.as<unsigned long long>(), st));
         const size_t nb = (size_t)std::max(1, s->N);
         CR_HIP(c->rec_off.ensure(sizeof(int) * nb));
         CR_HIP(c->rec_win.ensure(window_record_bytes(c->mp, c->fp32()) * nb));
+    }
+    return CR_OK;
+}
+
+// Validate the system and take up its box (prepare_host), then the membership and, with
+// bin, the beads binned on the context's tiles with their window records.
+int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out, bool bin)
+{
+    CR_RET(prepare_host(c, s, st, out, bin));
+    CR_HIP(launch_membership(s->N, s->tag, s->body_of_tag, c->iso.as<unsigned>(), st));
+    if (bin && c->tg.valid) {
+        CR_HIP(launch_bin_beads(c->mp, c->tg, *out, c->bead_tile.as<unsigned>(), c->tile_count.as<unsigned>(),
+                                c->tile_start.as<unsigned>(), c->tile_cursor.as<unsigned>(), c->sorted.as<unsigned>(),
+                                c->scan_sums.as<unsigned long long>(), st));
         if (c->fp32())
             CR_HIP(launch_bead_records<float>(c->mp, c->tg, *out, c->sorted.as<unsigned>(), c->bead_tile.as<unsigned>(),
                                               c->rec_off.as<int>(), c->rec_win.as<float>(), st));
@@ -511,7 +521,7 @@ bool block_precond(const cr_context* c) { return c->params.preconditioner == CR_
 
 // Build the per-body inverses of the near-field block preconditioner for the current
 // positions (once per force evaluation).
-int block_setup(cr_context* c, const SysView& v, hipStream_t st, bool invert)
+int block_alloc(cr_context* c, const SysView& v)
 {
     const int B = v.N_isolated;
     const int nmax = std::max(1, v.Nmax);
@@ -526,6 +536,13 @@ int block_setup(cr_context* c, const SysView& v, hipStream_t st, bool invert)
     CR_HIP(c->inv_s.ensure(sizeof(double) * b1));
     CR_HIP(c->scan_sums.ensure(sizeof(unsigned long long) * std::max<size_t>((b1 + 1) / 1024 + 4,
                                                                              c->scan_sums.bytes / 8)));
+    return CR_OK;
+}
+
+int block_setup(cr_context* c, const SysView& v, hipStream_t st, bool invert)
+{
+    CR_RET(block_alloc(c, v));
+    const int nmax = c->block_nmax;
     CR_HIP(launch_block_setup(v, c->mp.L, nmax, c->body_count.as<unsigned>(), c->body_start.as<unsigned>(),
                               c->body_beads.as<unsigned>(),
                               c->scan_sums.as<unsigned long long>(), c->ginv.as<float>(), c->uvec.as<float>(),

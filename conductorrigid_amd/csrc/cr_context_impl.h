@@ -85,6 +85,9 @@ int set_box(cr_context* c, const double L[3], hipStream_t st);
 // Validate the system and refresh the body-tag map; with bin, also rebin beads and rebuild
 // window records.
 int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out, bool bin = true);
+// prepare without its launches (checks, box take-up, buffers, plans): the batched boxes
+// run the membership, binning and records of all boxes in a few launches of their own.
+int prepare_host(cr_context* c, const cr_system* s, hipStream_t st, SysView* out, bool bin = true);
 BinView bin_view(const cr_context* c);
 bool contract_tiled_ok(const cr_context* c, size_t elem);
 // Real-space ELL operator for the current positions (valid until the next prepare()).
@@ -105,6 +108,8 @@ const AxisPlan* axis_plans(const cr_context* c);
 int matvec_any(cr_context* c, const SysView& v, const double* x, double* y, int precond, hipStream_t st);
 bool block_precond(const cr_context* c);
 // Group beads by body and (invert) form the per-body inverses of the block preconditioner.
+// the block preconditioner's buffers for v (block_setup without launches)
+int block_alloc(cr_context* c, const SysView& v);
 int block_setup(cr_context* c, const SysView& v, hipStream_t st, bool invert = true);
 int apply_precond(cr_context* c, const SysView& v, const double* in, double* out, hipStream_t st);
 // x (length N + B) from the x0 mode; uses the context's w as scratch for CR_X0_CHARGES.

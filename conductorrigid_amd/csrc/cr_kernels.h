@@ -194,6 +194,40 @@ hipError_t launch_spread_fx(const MeshParams& mp, const TileGeom& tg, const FxPl
 // fs->wbits for the binned configuration (once per bead configuration).
 hipError_t launch_spread_fx_bound(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
                                   FxScale* fs, hipStream_t st);
+// Batched boxes' per-evaluation preparation (membership, binning, window records,
+// objective, body grouping, ELL width, fixed-point bound zeroing): one descriptor per box,
+// boxes in ascending blk0; a box owns the workgroups [blk0, blk0 + nblk) of every phase,
+// nblk = ceil(max(N, N_rigid * Nmax, N_isolated, ntiles + 1) / 256).
+struct PrepBox {
+    MeshParams mp;
+    TileGeom tg;
+    SysView s;                // s.iso is the context's membership array (written here)
+    int blk0;
+    int bin;                  // 1: bin the beads on tg
+    const unsigned* tag;      // the caller's tags and body-of-tag map (membership)
+    const unsigned* body_of_tag;
+    unsigned* iso;
+    unsigned* bead_tile;      // binning (bin = 1)
+    unsigned* tile_count;
+    unsigned* tile_start;
+    unsigned* cursor;
+    unsigned* sorted;
+    int* rec_off;
+    void* rec_win;
+    double* b;                // objective
+    double* bcat;             // the same rows in the batch's concatenated vector
+    double E0[3];
+    unsigned* body_count;     // body grouping (null: none)
+    unsigned* body_start;
+    unsigned* body_beads;
+    unsigned* width;          // max n_neigh (null: none)
+    unsigned* wbits;          // fixed-point weight bound to zero (null: none)
+};
+// The preparation of all boxes: five launches (zero, beads, scans, fills, records), fp32
+// selects the window-record layout; then the objective rows of all boxes in one launch.
+hipError_t launch_prep_batch(const PrepBox* boxes, int nbox, int total_blocks, bool fp32, hipStream_t st);
+hipError_t launch_objective_batch(const PrepBox* boxes, int nbox, int total_blocks, hipStream_t st);
+
 // Batched boxes on the fixed-point kernels (every box FP32 bulk with valid plans, one P):
 // one descriptor per box, boxes in ascending tile0 and gblk0.
 struct FxBox {
@@ -229,6 +263,10 @@ hipError_t launch_charges_fx_batch(const FxBox* boxes, int nbox, int max_n, cons
 // (gather_blocks workgroups in all)
 hipError_t launch_spread_fx_batch(const FxBox* boxes, int nbox, int P, int total_tiles, size_t lds, long gather_blocks,
                                   bool v4, const int* skip, hipStream_t st);
+// every box's fixed-point weight bound (fs->wbits, zeroed beforehand), once per bead
+// configuration
+hipError_t launch_spread_fx_bound_batch(const FxBox* boxes, int nbox, int P, int total_tiles, size_t lds,
+                                        hipStream_t st);
 // potential (force = false) or reciprocal-force contraction of every box not skipped
 hipError_t launch_contract_fx_batch(const FxBox* boxes, int nbox, int P, int total_tiles, size_t lds, bool force,
                                     const int* skip, hipStream_t st);

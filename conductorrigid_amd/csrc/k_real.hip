@@ -324,10 +324,9 @@ __global__ void k_precond_rigid(SysView s, const double* __restrict__ x, double*
 
 // compute_objective with zero_array folded in: rigid beads get -(r_i - R_b).E0 (minimum
 // image), beads outside rigid bodies get 0, rows N.. get the body charges.
-__global__ void k_objective(SysView s, double Lx, double Ly, double Lz, double E0x, double E0y, double E0z,
-                            double* __restrict__ b)
+__device__ __forceinline__ void objective_rows(const SysView& s, double Lx, double Ly, double Lz, double E0x,
+                                               double E0y, double E0z, double* __restrict__ b, long t)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nslot = (long)s.N_rigid * s.Nmax;
     if (t < nslot) {
         const unsigned idx = s.rigid_members[t];
@@ -340,6 +339,27 @@ __global__ void k_objective(SysView s, double Lx, double Ly, double Lz, double E
     }
     if (t < s.N && s.rigid_membership[t] == NOT_MEMBER) b[t] = 0.0;
     if (t < s.N_isolated) b[s.N + t] = s.body_charge[t];
+}
+
+__global__ void k_objective(SysView s, double Lx, double Ly, double Lz, double E0x, double E0y, double E0z,
+                            double* __restrict__ b)
+{
+    objective_rows(s, Lx, Ly, Lz, E0x, E0y, E0z, b, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Batched boxes (PrepBox layout: a box owns the workgroups [blk0, blk0 + nblk))
+__global__ __launch_bounds__(256) void k_objective_batch(const PrepBox* __restrict__ boxes, int nbox)
+{
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (boxes[mid].blk0 <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const PrepBox& d = boxes[lo];
+    const long t = (long)(blockIdx.x - d.blk0) * blockDim.x + threadIdx.x;
+    objective_rows(d.s, d.mp.L[0], d.mp.L[1], d.mp.L[2], d.E0[0], d.E0[1], d.E0[2], d.b, t);
+    if (d.bcat) objective_rows(d.s, d.mp.L[0], d.mp.L[1], d.mp.L[2], d.E0[0], d.E0[1], d.E0[2], d.bcat, t);
 }
 
 __global__ void k_membership(int N, const unsigned* __restrict__ tag, const unsigned* __restrict__ body_of_tag,
@@ -476,6 +496,13 @@ hipError_t launch_objective(const SysView& s, const double L[3], const double E0
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_objective, dim3(blocks(n, 256)), dim3(256), 0, st, s, L[0], L[1], L[2], E0[0], E0[1], E0[2],
                        b);
+    return hipGetLastError();
+}
+
+hipError_t launch_objective_batch(const PrepBox* boxes, int nbox, int total_blocks, hipStream_t st)
+{
+    if (nbox <= 0 || total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_objective_batch, dim3(total_blocks), dim3(256), 0, st, boxes, nbox);
     return hipGetLastError();
 }
 

@@ -314,15 +314,15 @@ __device__ __forceinline__ int fx_box_of(const FxBox* __restrict__ boxes, int nb
     return lo;
 }
 
-template <int PT>
+template <int PT, bool ONES>
 __global__ __launch_bounds__(kFxThreads) void k_spread_fx_batch(const FxBox* __restrict__ boxes, int nbox, int total,
                                                                 const int* __restrict__ skip)
 {
     const int g = fx_tile(blockIdx.x, total);
     const FxBox& d = boxes[fx_box_of(boxes, nbox, g, true)];
     if (skip && skip[d.box]) return;
-    spread_fx_tile<PT, 0, 0, 0, false>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off, d.rec_u,
-                                       d.qf, d.fs, d.bstride, d.blocks);
+    spread_fx_tile<PT, 0, 0, 0, ONES>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off, d.rec_u,
+                                      d.qf, d.fs, d.bstride, d.blocks);
 }
 
 // The mesh from the tiles' blocks: a node sums the blocks that cover it -- its own tile's
@@ -812,11 +812,11 @@ hipError_t launch_charges_fx_batch(const FxBox* boxes, int nbox, int max_n, cons
 }
 
 namespace {
-template <int PT>
+template <int PT, bool ONES>
 hipError_t spread_fx_batch_p(const FxBox* boxes, int nbox, int total_tiles, size_t lds, const int* skip,
                              hipStream_t st)
 {
-    auto fn = k_spread_fx_batch<PT>;
+    auto fn = k_spread_fx_batch<PT, ONES>;
     allow_lds_fx(reinterpret_cast<const void*>(fn), lds);
     hipLaunchKernelGGL(fn, dim3(total_tiles), dim3(kFxThreads), lds, st, boxes, nbox, total_tiles, skip);
     return hipGetLastError();
@@ -845,13 +845,13 @@ hipError_t launch_spread_fx_batch(const FxBox* boxes, int nbox, int P, int total
     if (nbox <= 0 || total_tiles <= 0) return hipSuccess;
     hipError_t e;
     switch (P) {
-        case 4: e = spread_fx_batch_p<4>(boxes, nbox, total_tiles, lds, skip, st); break;
-        case 5: e = spread_fx_batch_p<5>(boxes, nbox, total_tiles, lds, skip, st); break;
-        case 6: e = spread_fx_batch_p<6>(boxes, nbox, total_tiles, lds, skip, st); break;
-        case 7: e = spread_fx_batch_p<7>(boxes, nbox, total_tiles, lds, skip, st); break;
-        case 8: e = spread_fx_batch_p<8>(boxes, nbox, total_tiles, lds, skip, st); break;
-        case 9: e = spread_fx_batch_p<9>(boxes, nbox, total_tiles, lds, skip, st); break;
-        case 10: e = spread_fx_batch_p<10>(boxes, nbox, total_tiles, lds, skip, st); break;
+        case 4: e = spread_fx_batch_p<4, false>(boxes, nbox, total_tiles, lds, skip, st); break;
+        case 5: e = spread_fx_batch_p<5, false>(boxes, nbox, total_tiles, lds, skip, st); break;
+        case 6: e = spread_fx_batch_p<6, false>(boxes, nbox, total_tiles, lds, skip, st); break;
+        case 7: e = spread_fx_batch_p<7, false>(boxes, nbox, total_tiles, lds, skip, st); break;
+        case 8: e = spread_fx_batch_p<8, false>(boxes, nbox, total_tiles, lds, skip, st); break;
+        case 9: e = spread_fx_batch_p<9, false>(boxes, nbox, total_tiles, lds, skip, st); break;
+        case 10: e = spread_fx_batch_p<10, false>(boxes, nbox, total_tiles, lds, skip, st); break;
         default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
@@ -874,6 +874,22 @@ hipError_t launch_contract_fx_batch(const FxBox* boxes, int nbox, int P, int tot
         case 8: return contract_fx_batch_p<8>(boxes, nbox, total_tiles, lds, force, skip, st);
         case 9: return contract_fx_batch_p<9>(boxes, nbox, total_tiles, lds, force, skip, st);
         case 10: return contract_fx_batch_p<10>(boxes, nbox, total_tiles, lds, force, skip, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_spread_fx_bound_batch(const FxBox* boxes, int nbox, int P, int total_tiles, size_t lds,
+                                        hipStream_t st)
+{
+    if (nbox <= 0 || total_tiles <= 0) return hipSuccess;
+    switch (P) {
+        case 4: return spread_fx_batch_p<4, true>(boxes, nbox, total_tiles, lds, nullptr, st);
+        case 5: return spread_fx_batch_p<5, true>(boxes, nbox, total_tiles, lds, nullptr, st);
+        case 6: return spread_fx_batch_p<6, true>(boxes, nbox, total_tiles, lds, nullptr, st);
+        case 7: return spread_fx_batch_p<7, true>(boxes, nbox, total_tiles, lds, nullptr, st);
+        case 8: return spread_fx_batch_p<8, true>(boxes, nbox, total_tiles, lds, nullptr, st);
+        case 9: return spread_fx_batch_p<9, true>(boxes, nbox, total_tiles, lds, nullptr, st);
+        case 10: return spread_fx_batch_p<10, true>(boxes, nbox, total_tiles, lds, nullptr, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -522,6 +522,162 @@ __global__ __launch_bounds__(contract_threads<T>()) void k_contract_tiled_batch(
                                 smem_raw);
 }
 
+// ---- batched per-evaluation preparation (see PrepBox) ------------------------
+__device__ __forceinline__ int prep_box_of(const PrepBox* __restrict__ boxes, int nbox, int g)
+{
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (boxes[mid].blk0 <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// counters of the box's binning and body grouping, its ELL width and weight bound
+__global__ __launch_bounds__(256) void k_prep_zero(const PrepBox* __restrict__ boxes, int nbox)
+{
+    const PrepBox& d = boxes[prep_box_of(boxes, nbox, blockIdx.x)];
+    const long i = (long)(blockIdx.x - d.blk0) * blockDim.x + threadIdx.x;
+    if (d.bin) {
+        if (i <= d.tg.ntiles) d.tile_count[i] = 0u;
+        if (i < d.tg.ntiles) d.cursor[i] = 0u;
+    }
+    if (d.body_count && i <= d.s.N_isolated) d.body_count[i] = 0u;
+    if (i == 0) {
+        if (d.width) *d.width = 0u;
+        if (d.wbits) *d.wbits = 0u;
+    }
+}
+
+// per bead: membership, tile and tile counts, the ELL width; per body: its bead count
+// (a rigid body holds rigid_size beads, a bead outside every rigid body is a body alone)
+__global__ __launch_bounds__(256) void k_prep_beads(const PrepBox* __restrict__ boxes, int nbox)
+{
+    const PrepBox& d = boxes[prep_box_of(boxes, nbox, blockIdx.x)];
+    const SysView& s = d.s;
+    const long i = (long)(blockIdx.x - d.blk0) * blockDim.x + threadIdx.x;
+    const bool bead = i < s.N;
+    if (bead) {
+        const unsigned b = d.body_of_tag[d.tag[i]];
+        d.iso[i] = b;
+        if (d.body_count && s.rigid_membership[i] == NOT_MEMBER) d.body_count[b] = 1u;
+    }
+    const bool on = d.bin && bead;
+    const unsigned t = on ? (unsigned)tile_of_bead(d.mp, d.tg, s.pos[i]) : 0u;
+    if (on) d.bead_tile[i] = t;
+    (void)wave_count(d.tile_count, t, on);
+    if (d.body_count && i < s.N_rigid && s.rigid_size[i] > 0)
+        d.body_count[d.body_of_tag[d.tag[s.rigid_members[i * s.Nmax]]]] = s.rigid_size[i];
+    if (d.width) {
+        unsigned v = bead ? s.n_neigh[i] : 0u;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, off, 64));
+        if ((threadIdx.x & 63) == 0 && v) atomicMax(d.width, v);
+    }
+}
+
+// exclusive scan of n counts into out (out[n] need not exist), one workgroup, chunks of its
+// size with a running carry
+__device__ __forceinline__ void block_exclusive_scan(const unsigned* __restrict__ in, unsigned* __restrict__ out,
+                                                     long n)
+{
+    __shared__ unsigned wsum[16];
+    __shared__ unsigned carry_s;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (threadIdx.x == 0) carry_s = 0u;
+    __syncthreads();
+    for (long base = 0; base < n; base += blockDim.x) {
+        const long k = base + threadIdx.x;
+        const unsigned v = k < n ? in[k] : 0u;
+        unsigned x = v;  // inclusive scan within the wavefront
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned y = (unsigned)__shfl_up((int)x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        unsigned before = carry_s;
+        for (int w = 0; w < wave; ++w) before += wsum[w];
+        if (k < n) out[k] = before + x - v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned tot = 0;
+            for (int w = 0; w < nw; ++w) tot += wsum[w];
+            carry_s += tot;
+        }
+        __syncthreads();
+    }
+}
+
+// blockIdx.x: box; blockIdx.y: 0 the tile counts, 1 the body counts
+__global__ __launch_bounds__(1024) void k_prep_scan(const PrepBox* __restrict__ boxes)
+{
+    const PrepBox& d = boxes[blockIdx.x];
+    if (blockIdx.y == 0) {
+        if (d.bin) block_exclusive_scan(d.tile_count, d.tile_start, (long)d.tg.ntiles + 1);
+    } else if (d.body_count) {
+        block_exclusive_scan(d.body_count, d.body_start, (long)d.s.N_isolated + 1);
+    }
+}
+
+// beads into their tiles' slots; bodies' members in a fixed order (as k_body_fill_*)
+__global__ __launch_bounds__(256) void k_prep_fill(const PrepBox* __restrict__ boxes, int nbox)
+{
+    const PrepBox& d = boxes[prep_box_of(boxes, nbox, blockIdx.x)];
+    const SysView& s = d.s;
+    const long i = (long)(blockIdx.x - d.blk0) * blockDim.x + threadIdx.x;
+    const bool on = d.bin && i < s.N;
+    const unsigned t = on ? d.bead_tile[i] : 0u;
+    const unsigned slot = wave_count(d.cursor, t, on);
+    if (on) d.sorted[d.tile_start[t] + slot] = (unsigned)i;
+    if (!d.body_beads) return;
+    if (i < s.N && s.rigid_membership[i] == NOT_MEMBER) d.body_beads[d.body_start[d.iso[i]]] = (unsigned)i;
+    if (i < (long)s.N_rigid * s.Nmax) {
+        const long r = i / s.Nmax;
+        const int k = (int)(i - r * s.Nmax);
+        if (k < (int)s.rigid_size[r]) {
+            const unsigned* mem = s.rigid_members + r * s.Nmax;
+            d.body_beads[d.body_start[d.iso[mem[0]]] + k] = mem[k];
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_prep_records(const PrepBox* __restrict__ boxes, int nbox)
+{
+    const PrepBox& d = boxes[prep_box_of(boxes, nbox, blockIdx.x)];
+    if (!d.bin) return;
+    const long k = (long)(blockIdx.x - d.blk0) * blockDim.x + threadIdx.x;
+    if (k >= d.s.N) return;
+    const MeshParams& mp = d.mp;
+    const TileGeom& tg = d.tg;
+    const unsigned bead = d.sorted[k];
+    const int t = (int)d.bead_tile[bead];
+    const int tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
+    double u[3];
+    d.rec_off[k] = bead_window(mp, d.s.pos[bead], tx * tg.T[0], ty * tg.T[1], tz * tg.T[2], u);
+    T* rec_win = static_cast<T*>(d.rec_win);
+    if constexpr (sizeof(T) == sizeof(double)) {
+        const int P = mp.P;
+        T* w = rec_win + k * 3 * P;
+        for (int dd = 0; dd < 3; ++dd) {
+            const T cd = axis_coef<T>(mp, dd), ud = (T)u[dd];
+            for (int p = 0; p < P; ++p) {
+                const T e = (T)p - ud;
+                w[dd * P + p] = gexp<T>(-cd * e * e);
+            }
+        }
+    } else {
+        T* uk = rec_win + 4 * k;
+        uk[0] = (T)u[0];
+        uk[1] = (T)u[1];
+        uk[2] = (T)u[2];
+        uk[3] = (T)0;
+    }
+}
+
 inline unsigned blocks(long n, int b)
 {
     long v = (n + b - 1) / b;
@@ -736,5 +892,19 @@ hipError_t launch_contract_tiled_batch(const TiledBox* boxes, int nbox, int tota
 CR_INST(float)
 CR_INST(double)
 #undef CR_INST
+
+hipError_t launch_prep_batch(const PrepBox* boxes, int nbox, int total_blocks, bool fp32, hipStream_t st)
+{
+    if (nbox <= 0 || total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_prep_zero, dim3(total_blocks), dim3(256), 0, st, boxes, nbox);
+    hipLaunchKernelGGL(k_prep_beads, dim3(total_blocks), dim3(256), 0, st, boxes, nbox);
+    hipLaunchKernelGGL(k_prep_scan, dim3(nbox, 2), dim3(1024), 0, st, boxes);
+    hipLaunchKernelGGL(k_prep_fill, dim3(total_blocks), dim3(256), 0, st, boxes, nbox);
+    if (fp32)
+        hipLaunchKernelGGL(k_prep_records<float>, dim3(total_blocks), dim3(256), 0, st, boxes, nbox);
+    else
+        hipLaunchKernelGGL(k_prep_records<double>, dim3(total_blocks), dim3(256), 0, st, boxes, nbox);
+    return hipGetLastError();
+}
 
 }  // namespace cr
