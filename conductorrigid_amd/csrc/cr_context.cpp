@@ -164,24 +164,30 @@ int prepare_host(cr_context* c, const cr_system* s, hipStream_t st, SysView* out
     return CR_OK;
 }
 
+// The beads binned on the context's tiles with their window records (prepare_host sized
+// the buffers).
+int bin_records(cr_context* c, const SysView& v, hipStream_t st)
+{
+    if (!c->tg.valid) return CR_OK;
+    CR_HIP(launch_bin_beads(c->mp, c->tg, v, c->bead_tile.as<unsigned>(), c->tile_count.as<unsigned>(),
+                            c->tile_start.as<unsigned>(), c->tile_cursor.as<unsigned>(), c->sorted.as<unsigned>(),
+                            c->scan_sums.as<unsigned long long>(), st));
+    if (c->fp32())
+        CR_HIP(launch_bead_records<float>(c->mp, c->tg, v, c->sorted.as<unsigned>(), c->bead_tile.as<unsigned>(),
+                                          c->rec_off.as<int>(), c->rec_win.as<float>(), st));
+    else
+        CR_HIP(launch_bead_records<double>(c->mp, c->tg, v, c->sorted.as<unsigned>(), c->bead_tile.as<unsigned>(),
+                                           c->rec_off.as<int>(), c->rec_win.as<double>(), st));
+    return CR_OK;
+}
+
 // Validate the system and take up its box (prepare_host), then the membership and, with
 // bin, the beads binned on the context's tiles with their window records.
 int prepare(cr_context* c, const cr_system* s, hipStream_t st, SysView* out, bool bin)
 {
     CR_RET(prepare_host(c, s, st, out, bin));
     CR_HIP(launch_membership(s->N, s->tag, s->body_of_tag, c->iso.as<unsigned>(), st));
-    if (bin && c->tg.valid) {
-        CR_HIP(launch_bin_beads(c->mp, c->tg, *out, c->bead_tile.as<unsigned>(), c->tile_count.as<unsigned>(),
-                                c->tile_start.as<unsigned>(), c->tile_cursor.as<unsigned>(), c->sorted.as<unsigned>(),
-                                c->scan_sums.as<unsigned long long>(), st));
-        if (c->fp32())
-            CR_HIP(launch_bead_records<float>(c->mp, c->tg, *out, c->sorted.as<unsigned>(), c->bead_tile.as<unsigned>(),
-                                              c->rec_off.as<int>(), c->rec_win.as<float>(), st));
-        else
-            CR_HIP(launch_bead_records<double>(c->mp, c->tg, *out, c->sorted.as<unsigned>(),
-                                               c->bead_tile.as<unsigned>(), c->rec_off.as<int>(),
-                                               c->rec_win.as<double>(), st));
-    }
+    if (bin) CR_RET(bin_records(c, *out, st));
     return CR_OK;
 }
 
@@ -252,20 +258,23 @@ int contract_impl(cr_context* c, const SysView& v, T* phi, hipStream_t st)
     return CR_OK;
 }
 
+// force = the reciprocal-space force of q (+ fadd, 4 doubles per bead, when given)
 template <typename T>
-int contract_force_impl(cr_context* c, const SysView& v, const double* q, double* force, hipStream_t st)
+int contract_force_impl(cr_context* c, const SysView& v, const double* q, double* force, hipStream_t st,
+                        const double* fadd = nullptr)
 {
     if constexpr (sizeof(T) == sizeof(float)) {
         if (c->tg.valid && c->fxcp.valid) {
             CR_HIP(launch_contract_force_fx(c->mp, c->tg, c->fxcp, bin_view(c), c->grid_real.as<float>(), q, force,
-                                            st));
+                                            fadd, st));
             return CR_OK;
         }
     }
     if (contract_tiled_ok(c, sizeof(T)))
-        CR_HIP(launch_contract_tiled<T>(c->mp, c->tg, v, bin_view(c), q, c->grid_real.as<T>(), nullptr, force, st));
+        CR_HIP(launch_contract_tiled<T>(c->mp, c->tg, v, bin_view(c), q, c->grid_real.as<T>(), nullptr, force, st,
+                                        fadd));
     else
-        CR_HIP(launch_contract_force<T>(c->mp, v, q, c->grid_real.as<T>(), force, st));
+        CR_HIP(launch_contract_force<T>(c->mp, v, q, c->grid_real.as<T>(), force, st, fadd));
     return CR_OK;
 }
 
@@ -439,8 +448,11 @@ int real_rows(cr_context* c, const SysView& v, const double* x, double* y, int p
 
 // y = A x, or y = D A x with precond (PotentialChargeMultiply + sum_charges/set_charge,
 // cppmodule/ConductorRigid.cu:862-1020; D from precondition, :816-859).
+// rows = false leaves out the constraint rows y[N:N+B) (the block preconditioner's apply
+// forms them from x itself).
 template <typename T>
-int matvec_impl(cr_context* c, const SysView& v, const double* x, double* y, int precond, hipStream_t st)
+int matvec_impl(cr_context* c, const SysView& v, const double* x, double* y, int precond, hipStream_t st,
+                bool rows = true)
 {
     constexpr bool f32 = sizeof(T) == sizeof(float);
     if (f32) CR_RET(charges_f32(c, v, x, st));  // float charges for the spread and the pair sum
@@ -451,13 +463,14 @@ int matvec_impl(cr_context* c, const SysView& v, const double* x, double* y, int
         CR_RET(real_rows<T>(c, v, x, y, precond, st, f32));
     else
         CR_HIP(launch_real_space<T>(c->rp, v, x, c->phi_recip.as<T>(), table<T>(c, false), y, precond, st));
-    CR_HIP(launch_constraint_rows(v, x, y + v.N, precond, st));
+    if (rows) CR_HIP(launch_constraint_rows(v, x, y + v.N, precond, st));
     return CR_OK;
 }
 
-int matvec_any(cr_context* c, const SysView& v, const double* x, double* y, int precond, hipStream_t st)
+int matvec_any(cr_context* c, const SysView& v, const double* x, double* y, int precond, hipStream_t st, bool rows)
 {
-    return c->fp32() ? matvec_impl<float>(c, v, x, y, precond, st) : matvec_impl<double>(c, v, x, y, precond, st);
+    return c->fp32() ? matvec_impl<float>(c, v, x, y, precond, st, rows)
+                     : matvec_impl<double>(c, v, x, y, precond, st, rows);
 }
 
 // ||w||^2 into dst (device)
@@ -534,8 +547,7 @@ int block_alloc(cr_context* c, const SysView& v)
     CR_HIP(c->ginv.ensure(sizeof(float) * b1 * nmax * nmax));
     CR_HIP(c->uvec.ensure(sizeof(float) * b1 * nmax));
     CR_HIP(c->inv_s.ensure(sizeof(double) * b1));
-    CR_HIP(c->scan_sums.ensure(sizeof(unsigned long long) * std::max<size_t>((b1 + 1) / 1024 + 4,
-                                                                             c->scan_sums.bytes / 8)));
+    CR_HIP(c->block_scan.ensure(sizeof(unsigned long long) * ((b1 + 1) / 1024 + 4)));
     return CR_OK;
 }
 
@@ -545,17 +557,17 @@ int block_setup(cr_context* c, const SysView& v, hipStream_t st, bool invert)
     const int nmax = c->block_nmax;
     CR_HIP(launch_block_setup(v, c->mp.L, nmax, c->body_count.as<unsigned>(), c->body_start.as<unsigned>(),
                               c->body_beads.as<unsigned>(),
-                              c->scan_sums.as<unsigned long long>(), c->ginv.as<float>(), c->uvec.as<float>(),
+                              c->block_scan.as<unsigned long long>(), c->ginv.as<float>(), c->uvec.as<float>(),
                               c->inv_s.as<double>(), st, invert));
     return CR_OK;
 }
 
 // out = D in with the configured preconditioner (diagonal: the reference's; block: near field)
-int apply_precond(cr_context* c, const SysView& v, const double* in, double* out, hipStream_t st)
+int apply_precond(cr_context* c, const SysView& v, const double* in, double* out, hipStream_t st, const double* xq)
 {
     if (block_precond(c)) {
         CR_HIP(launch_block_apply(v, c->block_nmax, c->body_start.as<unsigned>(), c->body_beads.as<unsigned>(),
-                                  c->ginv.as<float>(), c->uvec.as<float>(), c->inv_s.as<double>(), in, out, st));
+                                  c->ginv.as<float>(), c->uvec.as<float>(), c->inv_s.as<double>(), in, out, st, xq));
     } else {
         CR_HIP(launch_precondition(v, in, out, st));
     }
@@ -566,8 +578,9 @@ int apply_precond(cr_context* c, const SysView& v, const double* in, double* out
 int matvec_prec(cr_context* c, const SysView& v, const double* x, double* out, hipStream_t st)
 {
     if (block_precond(c)) {
-        CR_RET(matvec_any(c, v, x, c->yraw.as<double>(), 0, st));
-        return apply_precond(c, v, c->yraw.as<double>(), out, st);
+        // the apply forms the constraint rows from x (one kernel fewer per application)
+        CR_RET(matvec_any(c, v, x, c->yraw.as<double>(), 0, st, false));
+        return apply_precond(c, v, c->yraw.as<double>(), out, st, x);
     }
     return matvec_any(c, v, x, out, 1, st);
 }
@@ -748,13 +761,13 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
             CR_HIP(launch_gs_update_dot(V, c->ld, i + 1, g->h1, w, n, partial, st, wn));
             CR_HIP(launch_reduce_partials(partial, nblk, i + 1 + (wn ? 1 : 0), g->h2, st));
             CR_HIP(launch_gs_update(V, c->ld, i + 1, g->h2, w, n, wn ? 3 : 2, partial, st));
-            CR_HIP(launch_reduce_partials(partial, nblk, 1, &g->wnorm2, st));
-            CR_HIP(launch_givens(g, i, R, st));
-            CR_HIP(launch_scale_copy(w, &g->inv_norm, 1.0, V + (long)(i + 1) * c->ld, n, st));
             // every continuation (this cycle's next step, or the restart) begins with A x;
             // its mesh clear is queued behind the residual's publication
             const bool go_on = iter + 1 <= maxiter;
+            double* vnext = V + (long)(i + 1) * c->ld;
             if (!c->hpub) {
+                // ||w||, V[:, i+1], the rotations (no mailbox: the residual is copied back)
+                CR_HIP(launch_step_tail(partial, nblk, g, i, R, w, vnext, n, nullptr, 0, st));
                 if (go_on) CR_RET(preclear_mesh(c, st));
                 CR_RET(read_scalar(c, &g->resid, &resid, st));
                 if (!std::isfinite(resid)) return nonfinite();
@@ -765,8 +778,9 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
                 continue;
             }
             const int slot = iter % kPubSlots;
-            unsigned long long seq = 0;
-            CR_RET(publish_scalar(c, &g->resid, slot, st, &seq));
+            // ||w||, V[:, i+1], the rotations and the residual's publication in one launch
+            const unsigned long long seq = ++c->pub_seq;
+            CR_HIP(launch_step_tail(partial, nblk, g, i, R, w, vnext, n, c->hpub + slot, seq, st));
             if (go_on) CR_RET(preclear_mesh(c, st));
             if (pending) {
                 double rp = 0;
@@ -816,13 +830,24 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
     return CR_OK;
 }
 
+// The bead forces for charges q: the real-space pair forces and q E0 (into force_rs, on a
+// side stream) beside the spread and k-space step, then the reciprocal force, which adds
+// them per bead.
 template <typename T>
 int force_pass(cr_context* c, const SysView& v, const double* q, double* force, hipStream_t st)
 {
+    if (v.N <= 0) return CR_OK;
+    CR_HIP(c->force_rs.ensure(sizeof(double) * 4 * (size_t)v.N));
+    double* frs = c->force_rs.as<double>();
+    hipStream_t sd = c->side[0];
+    CR_HIP(hipEventRecord(c->ev_fork, st));
+    CR_HIP(hipStreamWaitEvent(sd, c->ev_fork, 0));
+    CR_HIP(launch_real_space_force<T>(c->rp, v, q, table<T>(c, true), c->E0, frs, sd, false));
+    CR_HIP(hipEventRecord(c->ev_join[0], sd));
     CR_RET(spread_impl<T>(c, v, q, st));  // FP32 bulk: rounds the charges itself
     CR_RET(recip_impl<T>(c, st));
-    CR_RET(contract_force_impl<T>(c, v, q, force, st));
-    CR_HIP(launch_real_space_force<T>(c->rp, v, q, table<T>(c, true), c->E0, force, st));
+    CR_HIP(hipStreamWaitEvent(st, c->ev_join[0], 0));
+    CR_RET(contract_force_impl<T>(c, v, q, force, st, frs));
     return CR_OK;
 }
 
@@ -1003,6 +1028,14 @@ int cr_create(cr_context** out, int device)
     if (c->fxs.ensure(sizeof(FxScale)) != hipSuccess || hipMemset(c->fxs.p, 0, sizeof(FxScale)) != hipSuccess) {
         delete c;
         return set_error(CR_ERR_DEVICE, "allocation of the spread scale failed");
+    }
+    bool ok = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; ok && k < cr_context::kSide; ++k)
+        ok = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        delete c;
+        return set_error(CR_ERR_DEVICE, "creation of the side streams failed");
     }
     // residual mailbox; without coherent host memory the solver reads back by copy
     if (hipHostMalloc((void**)&c->hpub, kPubSlots * sizeof(HostPub), hipHostMallocCoherent) != hipSuccess) {
@@ -1324,19 +1357,40 @@ int cr_compute_force(cr_context* c, const cr_system* s, double* q_inout, double*
 {
     hipStream_t st = (hipStream_t)stream;
     SysView v;
-    CR_RET(prepare(c, s, st, &v));
+    CR_RET(prepare_host(c, s, st, &v, true));
     const long n = (long)v.N + v.N_isolated;
     if (n > c->n_cap) return set_error(CR_ERR_INVALID, "system larger than the size given to cr_set_params");
     if ((v.N > 0 && (!q_inout || !force)) || (v.N_isolated > 0 && !phi_inout))
         return set_error(CR_ERR_INVALID, "null output array");
-    double* x = c->x.as<double>();
-    // objective and its preconditioned copy (MatrixVectorSolve, cppmodule/ConductorRigid.cu:1099-1107)
-    CR_HIP(launch_objective(v, c->mp.L, c->E0, c->b.as<double>(), st));
-    if (block_precond(c)) CR_RET(block_setup(c, v, st));
-    CR_RET(apply_precond(c, v, c->b.as<double>(), c->Mb.as<double>(), st));
     if (x0_mode != CR_X0_ZERO && x0_mode != CR_X0_PREVIOUS && x0_mode != CR_X0_CHARGES)
         return set_error(CR_ERR_INVALID, "unknown x0 mode");
-    CR_RET(ensure_csr(c, v, st));
+    double* x = c->x.as<double>();
+    CR_HIP(launch_membership(s->N, s->tag, s->body_of_tag, c->iso.as<unsigned>(), st));
+    // Per-configuration setup in three independent parts, side by side: the block
+    // preconditioner's inverses (side 1), the real-space ELL operator (side 0) and, on the
+    // caller's stream, the binning, the objective and the fixed-point spread's bound.
+    CR_HIP(hipEventRecord(c->ev_fork, st));
+    const bool blk = block_precond(c);
+    if (blk) {
+        CR_HIP(hipStreamWaitEvent(c->side[1], c->ev_fork, 0));
+        CR_RET(block_setup(c, v, c->side[1]));
+        CR_HIP(hipEventRecord(c->ev_join[1], c->side[1]));
+    }
+    CR_RET(bin_records(c, v, st));
+    // objective (MatrixVectorSolve, cppmodule/ConductorRigid.cu:1099-1107)
+    CR_HIP(launch_objective(v, c->mp.L, c->E0, c->b.as<double>(), st));
+    if (c->fx_on() && !c->fx_bound) {
+        CR_HIP(launch_spread_fx_bound(c->mp, c->tg, c->fxp, bin_view(c), c->fxs.as<FxScale>(), st));
+        c->fx_bound = true;
+    }
+    // the ELL width is read back (the host waits for side 0 only, the other parts run on)
+    CR_HIP(hipStreamWaitEvent(c->side[0], c->ev_fork, 0));
+    CR_RET(ensure_csr(c, v, c->side[0]));
+    CR_HIP(hipEventRecord(c->ev_join[0], c->side[0]));
+    CR_HIP(hipStreamWaitEvent(st, c->ev_join[0], 0));
+    if (blk) CR_HIP(hipStreamWaitEvent(st, c->ev_join[1], 0));
+    // the objective's preconditioned copy
+    CR_RET(apply_precond(c, v, c->b.as<double>(), c->Mb.as<double>(), st));
     CR_RET(init_x0(c, v, x0_mode, q_inout, phi_inout, x, st));
     cr_solve_stats local{};
     cr_solve_stats* sp = stats ? stats : &local;

@@ -50,9 +50,18 @@ struct cr_context {
     long ld = 0;
     int n_cap = 0;
     DevBuf V, w, b, Mb, x, partial, scal, yraw;
-    // near-field block preconditioner: beads grouped by body tag, per-body inverses
+    // near-field block preconditioner: beads grouped by body tag, per-body inverses (its
+    // own scan scratch: its setup runs beside the binning, which scans into scan_sums)
     int block_nmax = 0;
-    DevBuf body_count, body_start, body_beads, ginv, uvec, inv_s;
+    DevBuf body_count, body_start, body_beads, ginv, uvec, inv_s, block_scan;
+    // Independent parts of a force evaluation run beside the caller's stream on these
+    // (default priority): the real-space ELL build and the block preconditioner's setup
+    // next to the binning, and the real-space force next to the force pass's mesh work,
+    // which then adds it (force_rs) to the reciprocal force.
+    static constexpr int kSide = 2;
+    hipStream_t side[kSide] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kSide] = {};
+    DevBuf force_rs;
     // per-body segment scratch: bead counts and double sums (warm starts, learned features)
     DevBuf seg_count, seg_sum;
     DevBuf bd_force;  // Brownian step: bead forces plus steric repulsion
@@ -68,6 +77,11 @@ struct cr_context {
 
     ~cr_context()
     {
+        for (int k = 0; k < kSide; ++k) {
+            if (ev_join[k]) (void)hipEventDestroy(ev_join[k]);
+            if (side[k]) (void)hipStreamDestroy(side[k]);
+        }
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (g_dev) (void)hipFree(g_dev);
         if (g_host) (void)hipHostFree(g_host);
         if (hpub) (void)hipHostFree(hpub);
@@ -105,13 +119,17 @@ void force_pair_box(const cr_context* c, const SysView& v, const double* q, doub
 int recip_any(cr_context* c, hipStream_t st);
 // FFT plans of the context's precision (x, y, z).
 const AxisPlan* axis_plans(const cr_context* c);
-int matvec_any(cr_context* c, const SysView& v, const double* x, double* y, int precond, hipStream_t st);
+// rows = false leaves out the constraint rows y[N:N+B)
+int matvec_any(cr_context* c, const SysView& v, const double* x, double* y, int precond, hipStream_t st,
+               bool rows = true);
 bool block_precond(const cr_context* c);
 // Group beads by body and (invert) form the per-body inverses of the block preconditioner.
 // the block preconditioner's buffers for v (block_setup without launches)
 int block_alloc(cr_context* c, const SysView& v);
 int block_setup(cr_context* c, const SysView& v, hipStream_t st, bool invert = true);
-int apply_precond(cr_context* c, const SysView& v, const double* in, double* out, hipStream_t st);
+// xq (block preconditioner only): the constraint rows are formed from xq instead of read from in
+int apply_precond(cr_context* c, const SysView& v, const double* in, double* out, hipStream_t st,
+                  const double* xq = nullptr);
 // x (length N + B) from the x0 mode; uses the context's w as scratch for CR_X0_CHARGES.
 int init_x0(cr_context* c, const SysView& v, int x0_mode, const double* q_in, const double* phi_in, double* x,
             hipStream_t st);

@@ -133,7 +133,8 @@ TiledLds tiled_lds(const MeshParams& mp, const TileGeom& tg, size_t ext_elem, si
 // or reciprocal force (phi unused).
 template <typename T>
 hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const SysView& s, const BinView& bv,
-                                 const double* q, const T* grid, T* phi, double* force, hipStream_t st);
+                                 const double* q, const T* grid, T* phi, double* force, hipStream_t st,
+                                 const double* fadd = nullptr);
 // One box of a batched binned spread / contraction (device array, tile0 ascending): its
 // mesh, tiling and binned bead records, and the arrays it reads and writes.
 struct TiledBox {
@@ -275,16 +276,17 @@ FxPlan fx_contract_plan(const MeshParams& mp, const TileGeom& tg, size_t lds_max
 hipError_t launch_contract_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
                               const float* grid, float* phi, hipStream_t st);
 // FP32-bulk reciprocal force (the gradient of each bead's window) on the same tiles and
-// plan: force [4N] = the reciprocal-space force of the charges q.
+// plan: force [4N] = the reciprocal-space force of the charges q (+ fadd [4N] when given).
 hipError_t launch_contract_force_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                                    const float* grid, const double* q, double* force, hipStream_t st);
+                                    const float* grid, const double* q, double* force, const double* fadd,
+                                    hipStream_t st);
 
 // ---- k_se.hip (unbinned fallbacks) ----
 template <typename T>
 hipError_t launch_contract(const MeshParams& mp, const SysView& s, const T* grid, T* phi, hipStream_t st);
 template <typename T>
 hipError_t launch_contract_force(const MeshParams& mp, const SysView& s, const double* q, const T* grid,
-                                 double* force, hipStream_t st);
+                                 double* force, hipStream_t st, const double* fadd = nullptr);
 template <typename T>
 hipError_t launch_fill(T* p, T v, long n, hipStream_t st);
 template <typename Tin, typename Tout>
@@ -357,8 +359,9 @@ hipError_t launch_real_space_csr(const RealParams& rp, const SysView& s, const d
 // *out = max(*out, a[i]) over i < n (out must be initialised)
 hipError_t launch_max_u32(const unsigned* a, int n, unsigned* out, hipStream_t st);
 template <typename T>
+// force += q E0 + pair forces (accum), or force = q E0 + pair forces
 hipError_t launch_real_space_force(const RealParams& rp, const SysView& s, const double* q, const T* gtable,
-                                   const double E0[3], double* force, hipStream_t st);
+                                   const double E0[3], double* force, hipStream_t st, bool accum = true);
 // The ELL build and the pair forces of all boxes of a batch, one launch each
 // (total_blocks = the sum of ceil(N / 256) over the boxes).
 template <typename T>
@@ -407,9 +410,12 @@ struct BlockBox {
 hipError_t launch_block_setup_batch(const BlockBox* boxes, int nbox, int total_bodies, int nmax_all, hipStream_t st);
 hipError_t launch_block_apply_batch(const BlockBox* boxes, int nbox, int total_bodies, const double* y, double* w,
                                     const int* skip, hipStream_t st);
+// w = D y with the near-field block preconditioner.  xq != nullptr: body t's constraint row
+// is taken as the sum of xq over its beads (the operator's charge row for input xq) instead
+// of y[N + t], which is then never read.
 hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_start, const unsigned* body_beads,
                               const float* ginv, const float* uvec, const double* inv_s, const double* y, double* w,
-                              hipStream_t st);
+                              hipStream_t st, const double* xq = nullptr);
 
 // ---- k_mlp.hip ----
 // Learned warm start: per-bead MLP trunk kMlpTrunk -> kMlpHidden -> kMlpHidden -> kMlpOut
@@ -544,6 +550,11 @@ struct HostPub {
 // step's is being published.
 constexpr int kPubSlots = 2;
 hipError_t launch_publish(const double* src, HostPub* dst, unsigned long long seq, hipStream_t st);
+// The end of Arnoldi step i: ||w||^2 from the last CGS pass's partials (nblocks of them,
+// column 0), vnext = w / ||w||, column i of H and its Givens rotations, then g->resid
+// published to pub (when not null) with sequence number seq.
+hipError_t launch_step_tail(const double* partial, int nblocks, GmresState* g, int i, int R, const double* w,
+                            double* vnext, long n, HostPub* pub, unsigned long long seq, hipStream_t st);
 hipError_t launch_cycle_init(GmresState* g, const double* norm2, hipStream_t st);
 hipError_t launch_backsub(GmresState* g, int R, hipStream_t st);
 

@@ -176,14 +176,6 @@ __global__ __launch_bounds__(kBlock) void k_gs_update_dot(const double* __restri
     }
 }
 
-__global__ void k_publish(const double* __restrict__ src, HostPub* dst, unsigned long long seq)
-{
-    if (threadIdx.x != 0) return;
-    const double v = *src;
-    *reinterpret_cast<volatile double*>(&dst->value) = v;
-    __threadfence_system();
-    *reinterpret_cast<volatile unsigned long long*>(&dst->seq) = seq;
-}
 
 __global__ void k_axpby(double a, const double* __restrict__ x, double b, double* __restrict__ y, long n)
 {
@@ -236,9 +228,8 @@ __device__ __forceinline__ void gen_rot(double dx, double dy, double& cs, double
 }
 
 // Column i of H from the two CGS projections and ||w||, then the rotations.
-__global__ void k_givens(GmresState* g, int i, int R)
+__device__ __forceinline__ void givens_column(GmresState* g, int i, int R)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int ld = R + 1;
     double* H = g->H;
     for (int k = 0; k <= i; ++k) H[i * ld + k] = g->h1[k] + g->h2[k];
@@ -252,6 +243,59 @@ __global__ void k_givens(GmresState* g, int i, int R)
     g->resid = fabs(g->s[i + 1]);
     g->i = i;
     g->ncols = i + 1;
+}
+
+__global__ void k_givens(GmresState* g, int i, int R)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    givens_column(g, i, R);
+}
+
+__device__ __forceinline__ void publish_value(double v, HostPub* dst, unsigned long long seq)
+{
+    *reinterpret_cast<volatile double*>(&dst->value) = v;
+    __threadfence_system();
+    *reinterpret_cast<volatile unsigned long long*>(&dst->seq) = seq;
+}
+
+// The end of an Arnoldi step in one launch: ||w||^2 from the partials of the last CGS pass
+// (every workgroup sums them in k_reduce_partials' order, so all hold the same value),
+// V[:, i+1] = w / ||w|| (as k_scale_copy), and in workgroup 0 column i of H with its
+// Givens rotations (k_givens), then the residual published to the host (pub != nullptr,
+// as k_publish).
+__global__ __launch_bounds__(kBlock) void k_step_tail(const double* __restrict__ partial, int nblocks, GmresState* g,
+                                                      int i, int R, const double* __restrict__ w,
+                                                      double* __restrict__ vnext, long n, HostPub* pub,
+                                                      unsigned long long seq)
+{
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partial[(long)b * 32];
+    __shared__ double red[kBlock / 64];
+    __shared__ double tot;
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < kBlock / 64; ++k) t += red[k];
+        tot = t;
+    }
+    __syncthreads();
+    const double wn2 = tot;
+    const double hn = sqrt(wn2);
+    const double f = (hn != 0.0) ? 1.0 / hn : 0.0;
+    for (long k = (long)blockIdx.x * kBlock + threadIdx.x; k < n; k += (long)gridDim.x * kBlock) vnext[k] = f * w[k];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        g->wnorm2 = wn2;
+        givens_column(g, i, R);
+        if (pub) publish_value(g->resid, pub, seq);
+    }
+}
+
+__global__ void k_publish(const double* __restrict__ src, HostPub* dst, unsigned long long seq)
+{
+    if (threadIdx.x != 0) return;
+    publish_value(*src, dst, seq);
 }
 
 __global__ void k_cycle_init(GmresState* g, const double* norm2)
@@ -308,6 +352,14 @@ hipError_t launch_reduce_partials(const double* partial, int nblocks, int ncols,
 {
     if (ncols <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_reduce_partials, dim3(ncols), dim3(kBlock), 0, st, partial, nblocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_tail(const double* partial, int nblocks, GmresState* g, int i, int R, const double* w,
+                            double* vnext, long n, HostPub* pub, unsigned long long seq, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_step_tail, dim3(vblocks(n)), dim3(kBlock), 0, st, partial, nblocks, g, i, R, w, vnext, n,
+                       pub, seq);
     return hipGetLastError();
 }
 

@@ -296,9 +296,9 @@ __global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int
                                                      const unsigned* __restrict__ beads,
                                                      const float* __restrict__ ginv, const float* __restrict__ uvec,
                                                      const double* __restrict__ inv_s, const double* __restrict__ y,
-                                                     double* __restrict__ w)
+                                                     double* __restrict__ w, const double* __restrict__ xq)
 {
-    __shared__ double ys[256], red[256], cs[256];
+    __shared__ double ys[256], red[256], cs[256], qs[256];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int k = lane / nmax, i = lane - k * nmax;
     const int t = (blockIdx.x * 4 + wave) * K + k;
@@ -325,6 +325,7 @@ __global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int
     const int base = wave * 64 + k * nmax;  // this body's slots
     ys[threadIdx.x] = yi;
     red[threadIdx.x] = ui * yi;
+    if (xq) qs[threadIdx.x] = act ? xq[bead] : 0.0;
     __syncthreads();
     double gy = 0.0;
     if constexpr (R4 > 0) {
@@ -357,7 +358,14 @@ __global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int
     if (live && i == 0) {
         double dot = 0.0;
         for (int j = 0; j < n; ++j) dot += red[base + j];
-        cs[base] = bad ? __builtin_nan("") : (dot + y[N + t]) * inv_s[t];
+        // the body's constraint row: sum_{i in t} q_i (sum_charges / set_charge,
+        // cppmodule/ConductorRigid.cu:667-761), formed here from xq or read from y
+        double yc = 0.0;
+        if (xq)
+            for (int j = 0; j < n; ++j) yc += qs[base + j];
+        else
+            yc = y[N + t];
+        cs[base] = bad ? __builtin_nan("") : (dot + yc) * inv_s[t];
     }
     __syncthreads();
     if (live) {
@@ -442,14 +450,14 @@ hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, uns
 
 hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_start, const unsigned* body_beads,
                               const float* ginv, const float* uvec, const double* inv_s, const double* y, double* w,
-                              hipStream_t st)
+                              hipStream_t st, const double* xq)
 {
     if (s.N_isolated <= 0) return hipSuccess;
     const int K = nmax <= 64 ? 64 / nmax : 1;
     const unsigned nb = (unsigned)((s.N_isolated + 4 * K - 1) / (4 * K));
 #define CR_BLOCK_APPLY(R4)                                                                                     \
     hipLaunchKernelGGL(k_block_apply<R4>, dim3(nb), dim3(256), 0, st, s.N_isolated, s.N, nmax, K, body_start,  \
-                       body_beads, ginv, uvec, inv_s, y, w)
+                       body_beads, ginv, uvec, inv_s, y, w, xq)
     switch (nmax) {
         case 8: CR_BLOCK_APPLY(2); break;
         case 12: CR_BLOCK_APPLY(3); break;

@@ -209,16 +209,17 @@ __global__ void k_add_body_potentials(SysView s, const double* __restrict__ Phi,
 template <typename T>
 __device__ __forceinline__ void force_rows(const RealParams& rp, const SysView& s, const double* __restrict__ q,
                                            const T* __restrict__ gtable, double E0x, double E0y, double E0z,
-                                           double* __restrict__ force, int blk)
+                                           double* __restrict__ force, int blk, bool accum)
 {
     __shared__ unsigned sh[kNlStage];
     const NlSpan sp = stage_nlist(s, sh, blk);
     const int i = blk * blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     const double qi = q[i];
-    double Fx = force[4 * i] + qi * E0x;
-    double Fy = force[4 * i + 1] + qi * E0y;
-    double Fz = force[4 * i + 2] + qi * E0z;
+    // accum: force += (the reciprocal part is already there); otherwise force = this part
+    double Fx = (accum ? force[4 * i] : 0.0) + qi * E0x;
+    double Fy = (accum ? force[4 * i + 1] : 0.0) + qi * E0y;
+    double Fz = (accum ? force[4 * i + 2] : 0.0) + qi * E0z;
     T fx = 0, fy = 0, fz = 0;
     const double4 ri = s.pos[i];
     const unsigned head = s.head_list[i], nn = s.n_neigh[i];
@@ -247,9 +248,9 @@ __device__ __forceinline__ void force_rows(const RealParams& rp, const SysView& 
 template <typename T>
 __global__ __launch_bounds__(256) void k_real_space_force(RealParams rp, SysView s, const double* __restrict__ q,
                                                           const T* __restrict__ gtable, double E0x, double E0y,
-                                                          double E0z, double* __restrict__ force)
+                                                          double E0z, double* __restrict__ force, int accum)
 {
-    force_rows<T>(rp, s, q, gtable, E0x, E0y, E0z, force, blockIdx.x);
+    force_rows<T>(rp, s, q, gtable, E0x, E0y, E0z, force, blockIdx.x, accum != 0);
 }
 
 template <typename T>
@@ -257,7 +258,7 @@ __global__ __launch_bounds__(256) void k_real_space_force_batch(const PairBox* _
 {
     const PairBox& d = boxes[pair_box_of(boxes, nbox, blockIdx.x)];
     force_rows<T>(d.rp, d.s, d.q, static_cast<const T*>(d.table), d.E0[0], d.E0[1], d.E0[2], d.force,
-                  blockIdx.x - d.blk0);
+                  blockIdx.x - d.blk0, true);
 }
 
 // sum_charges: y_bottom[tag(r)] = sum of bead charges of rigid body r (FP64), optionally
@@ -458,11 +459,11 @@ hipError_t launch_add_body_potentials(const SysView& s, const double* Phi, doubl
 
 template <typename T>
 hipError_t launch_real_space_force(const RealParams& rp, const SysView& s, const double* q, const T* gtable,
-                                   const double E0[3], double* force, hipStream_t st)
+                                   const double E0[3], double* force, hipStream_t st, bool accum)
 {
     if (s.N <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_real_space_force<T>, dim3(blocks(s.N, 256)), dim3(256), 0, st, rp, s, q, gtable, E0[0],
-                       E0[1], E0[2], force);
+                       E0[1], E0[2], force, accum ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -593,8 +594,8 @@ hipError_t launch_max_u32(const unsigned* a, int n, unsigned* out, hipStream_t s
     return hipGetLastError();
 }
 template hipError_t launch_real_space_force<float>(const RealParams&, const SysView&, const double*, const float*,
-                                                   const double*, double*, hipStream_t);
+                                                   const double*, double*, hipStream_t, bool);
 template hipError_t launch_real_space_force<double>(const RealParams&, const SysView&, const double*, const double*,
-                                                    const double*, double*, hipStream_t);
+                                                    const double*, double*, hipStream_t, bool);
 
 }  // namespace cr

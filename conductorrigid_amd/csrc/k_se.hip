@@ -90,7 +90,8 @@ __global__ __launch_bounds__(kWave * kBeadsPerBlock) void k_contract(MeshParams 
                                                                     const double* __restrict__ q,
                                                                     const T* __restrict__ grid,
                                                                     T* __restrict__ phi,
-                                                                    double* __restrict__ force)
+                                                                    double* __restrict__ force,
+                                                                    const double* __restrict__ fadd)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int bead = blockIdx.x * kBeadsPerBlock + (threadIdx.x >> 6);
@@ -136,10 +137,14 @@ __global__ __launch_bounds__(kWave * kBeadsPerBlock) void k_contract(MeshParams 
         if (lane == 0) {
             // F = 2 (h^3 prefac) (2 xi^2) q_i sum phi_g e^{...} r/eta  (cppmodule/ConductorRigid.cu:449)
             const double c = 2.0 * (mp.quadW * mp.prefac) * mp.xiterm * q[bead];
-            force[4 * bead + 0] = c * (double)fx;
-            force[4 * bead + 1] = c * (double)fy;
-            force[4 * bead + 2] = c * (double)fz;
-            force[4 * bead + 3] = 0.0;
+            const long o = 4 * (long)bead;
+            // fadd: the real-space force of the same bead, computed beside this pass
+            const double a0 = fadd ? fadd[o] : 0.0, a1 = fadd ? fadd[o + 1] : 0.0;
+            const double a2 = fadd ? fadd[o + 2] : 0.0;
+            force[o + 0] = c * (double)fx + a0;
+            force[o + 1] = c * (double)fy + a1;
+            force[o + 2] = c * (double)fz + a2;
+            force[o + 3] = 0.0;
         }
     }
 }
@@ -193,18 +198,18 @@ hipError_t launch_contract(const MeshParams& mp, const SysView& s, const T* grid
     if (s.N <= 0) return hipSuccess;
     unsigned nb = (unsigned)((s.N + kBeadsPerBlock - 1) / kBeadsPerBlock);
     hipLaunchKernelGGL((k_contract<T, false>), dim3(nb), dim3(kWave * kBeadsPerBlock), 0, st, mp, s.N, s.pos,
-                       (const double*)nullptr, grid, phi, (double*)nullptr);
+                       (const double*)nullptr, grid, phi, (double*)nullptr, (const double*)nullptr);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_contract_force(const MeshParams& mp, const SysView& s, const double* q, const T* grid,
-                                 double* force, hipStream_t st)
+                                 double* force, hipStream_t st, const double* fadd)
 {
     if (s.N <= 0) return hipSuccess;
     unsigned nb = (unsigned)((s.N + kBeadsPerBlock - 1) / kBeadsPerBlock);
     hipLaunchKernelGGL((k_contract<T, true>), dim3(nb), dim3(kWave * kBeadsPerBlock), 0, st, mp, s.N, s.pos, q,
-                       grid, (T*)nullptr, force);
+                       grid, (T*)nullptr, force, fadd);
     return hipGetLastError();
 }
 
@@ -244,9 +249,9 @@ template hipError_t launch_spread<double>(const MeshParams&, const SysView&, con
 template hipError_t launch_contract<float>(const MeshParams&, const SysView&, const float*, float*, hipStream_t);
 template hipError_t launch_contract<double>(const MeshParams&, const SysView&, const double*, double*, hipStream_t);
 template hipError_t launch_contract_force<float>(const MeshParams&, const SysView&, const double*, const float*,
-                                                 double*, hipStream_t);
+                                                 double*, hipStream_t, const double*);
 template hipError_t launch_contract_force<double>(const MeshParams&, const SysView&, const double*, const double*,
-                                                  double*, hipStream_t);
+                                                  double*, hipStream_t, const double*);
 // k-space factor tables of one axis (SetParams, cppmodule/ConductorRigid.cc:245-274):
 // k_i = 2 pi (i < (n+1)/2 ? i : i - n) / L and e_i = exp(-(1 - eta) k_i^2 / (4 xi^2)).
 __global__ void k_kspace_axis(int n, double L, double xi, double eta, double* __restrict__ kt, double* __restrict__ et)

@@ -412,7 +412,7 @@ __device__ __forceinline__ void contract_fx_tile(const MeshParams& mp, const Til
                                                  const unsigned* __restrict__ sorted, const int* __restrict__ rec_off,
                                                  const float4* __restrict__ rec_u, const float* __restrict__ grid,
                                                  float* __restrict__ phi, const double* __restrict__ q,
-                                                 double* __restrict__ force)
+                                                 double* __restrict__ force, const double* __restrict__ fadd)
 {
     constexpr bool CT = TX > 0;
     constexpr int WS = fx_wstride(PT);
@@ -537,10 +537,14 @@ __device__ __forceinline__ void contract_fx_tile(const MeshParams& mp, const Til
                     // F = 2 (h^3 prefac) (2 xi^2) q_i sum_nodes grid w (r / eta), the reference's
                     // contract_force (cppmodule/ConductorRigid.cu:449)
                     const double c = 2.0 * (mp.quadW * mp.prefac) * mp.xiterm * (double)(float)q[r.y];
-                    force[4 * (long)r.y + 0] = c * mp.h[0] / mp.eta[0] * (double)fx;
-                    force[4 * (long)r.y + 1] = c * mp.h[1] / mp.eta[1] * (double)fy;
-                    force[4 * (long)r.y + 2] = c * mp.h[2] / mp.eta[2] * (double)fz;
-                    force[4 * (long)r.y + 3] = 0.0;
+                    const long o = 4 * (long)r.y;
+                    // fadd: the real-space force of the same bead, computed beside this pass
+                    const double a0 = fadd ? fadd[o] : 0.0, a1 = fadd ? fadd[o + 1] : 0.0;
+                    const double a2 = fadd ? fadd[o + 2] : 0.0;
+                    force[o + 0] = c * mp.h[0] / mp.eta[0] * (double)fx + a0;
+                    force[o + 1] = c * mp.h[1] / mp.eta[1] * (double)fy + a1;
+                    force[o + 2] = c * mp.h[2] / mp.eta[2] * (double)fz + a2;
+                    force[o + 3] = 0.0;
                 }
             }
         }
@@ -555,10 +559,11 @@ __global__ __launch_bounds__(kFxThreads) void k_contract_fx(MeshParams mp, TileG
                                                             const int* __restrict__ rec_off,
                                                             const float4* __restrict__ rec_u,
                                                             const float* __restrict__ grid, float* __restrict__ phi,
-                                                            const double* __restrict__ q, double* __restrict__ force)
+                                                            const double* __restrict__ q, double* __restrict__ force,
+                                                            const double* __restrict__ fadd)
 {
     contract_fx_tile<PT, TX, TY, TZ, FORCE>(mp, tg, fx_tile(blockIdx.x, tg.ntiles), ch, tstart, sorted, rec_off, rec_u,
-                                            grid, phi, q, force);
+                                            grid, phi, q, force, fadd);
 }
 
 template <int PT, bool FORCE>
@@ -569,7 +574,7 @@ __global__ __launch_bounds__(kFxThreads) void k_contract_fx_batch(const FxBox* _
     const FxBox& d = boxes[fx_box_of(boxes, nbox, g, true)];
     if (skip && skip[d.box]) return;
     contract_fx_tile<PT, 0, 0, 0, FORCE>(d.mp, d.tg, g - d.tile0, d.cch, d.tile_start, d.sorted, d.rec_off, d.rec_u,
-                                         d.grid, d.phi, d.q, d.force);
+                                         d.grid, d.phi, d.q, d.force, nullptr);
 }
 
 void allow_lds_fx(const void* fn, size_t bytes)
@@ -638,23 +643,25 @@ hipError_t dispatch_p(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl
 
 template <int PT, int TX, int TY, int TZ, bool FORCE>
 hipError_t launch_contract_kernel(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                                  const float* grid, float* phi, const double* q, double* force, hipStream_t st)
+                                  const float* grid, float* phi, const double* q, double* force, const double* fadd,
+                                  hipStream_t st)
 {
     auto fn = k_contract_fx<PT, TX, TY, TZ, FORCE>;
     allow_lds_fx(reinterpret_cast<const void*>(fn), pl.lds_bytes);
     hipLaunchKernelGGL(fn, dim3(tg.ntiles), dim3(kFxThreads), pl.lds_bytes, st, mp, tg, pl.chunk, bv.tile_start,
-                       bv.sorted, bv.rec_off, static_cast<const float4*>(bv.rec_win), grid, phi, q, force);
+                       bv.sorted, bv.rec_off, static_cast<const float4*>(bv.rec_win), grid, phi, q, force, fadd);
     return hipGetLastError();
 }
 
 template <int PT, bool FORCE>
 hipError_t contract_shape(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                          const float* grid, float* phi, const double* q, double* force, hipStream_t st)
+                          const float* grid, float* phi, const double* q, double* force, const double* fadd,
+                          hipStream_t st)
 {
     const int a = tg.T[0], b = tg.T[1], c = tg.T[2];
 #define CR_FXC_SHAPE(X, Y, Z) \
     if (a == X && b == Y && c == Z)  \
-    return launch_contract_kernel<PT, X, Y, Z, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st)
+    return launch_contract_kernel<PT, X, Y, Z, FORCE>(mp, tg, pl, bv, grid, phi, q, force, fadd, st)
     if constexpr (PT == 8) {
         CR_FXC_SHAPE(16, 16, 16);
         CR_FXC_SHAPE(16, 8, 16);
@@ -667,7 +674,7 @@ hipError_t contract_shape(const MeshParams& mp, const TileGeom& tg, const FxPlan
         CR_FXC_SHAPE(8, 8, 8);
     }
 #undef CR_FXC_SHAPE
-    return launch_contract_kernel<PT, 0, 0, 0, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
+    return launch_contract_kernel<PT, 0, 0, 0, FORCE>(mp, tg, pl, bv, grid, phi, q, force, fadd, st);
 }
 
 inline unsigned nblk(long n, int b)
@@ -723,17 +730,18 @@ FxPlan fx_contract_plan(const MeshParams& mp, const TileGeom& tg, size_t lds_max
 
 template <bool FORCE>
 hipError_t contract_fx_p(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                         const float* grid, float* phi, const double* q, double* force, hipStream_t st)
+                         const float* grid, float* phi, const double* q, double* force, const double* fadd,
+                         hipStream_t st)
 {
     if (!pl.valid) return hipErrorInvalidValue;
     switch (mp.P) {
-        case 4: return contract_shape<4, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
-        case 5: return contract_shape<5, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
-        case 6: return contract_shape<6, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
-        case 7: return contract_shape<7, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
-        case 8: return contract_shape<8, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
-        case 9: return contract_shape<9, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
-        case 10: return contract_shape<10, FORCE>(mp, tg, pl, bv, grid, phi, q, force, st);
+        case 4: return contract_shape<4, FORCE>(mp, tg, pl, bv, grid, phi, q, force, fadd, st);
+        case 5: return contract_shape<5, FORCE>(mp, tg, pl, bv, grid, phi, q, force, fadd, st);
+        case 6: return contract_shape<6, FORCE>(mp, tg, pl, bv, grid, phi, q, force, fadd, st);
+        case 7: return contract_shape<7, FORCE>(mp, tg, pl, bv, grid, phi, q, force, fadd, st);
+        case 8: return contract_shape<8, FORCE>(mp, tg, pl, bv, grid, phi, q, force, fadd, st);
+        case 9: return contract_shape<9, FORCE>(mp, tg, pl, bv, grid, phi, q, force, fadd, st);
+        case 10: return contract_shape<10, FORCE>(mp, tg, pl, bv, grid, phi, q, force, fadd, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -741,13 +749,14 @@ hipError_t contract_fx_p(const MeshParams& mp, const TileGeom& tg, const FxPlan&
 hipError_t launch_contract_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
                               const float* grid, float* phi, hipStream_t st)
 {
-    return contract_fx_p<false>(mp, tg, pl, bv, grid, phi, nullptr, nullptr, st);
+    return contract_fx_p<false>(mp, tg, pl, bv, grid, phi, nullptr, nullptr, nullptr, st);
 }
 
 hipError_t launch_contract_force_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                                    const float* grid, const double* q, double* force, hipStream_t st)
+                                    const float* grid, const double* q, double* force, const double* fadd,
+                                    hipStream_t st)
 {
-    return contract_fx_p<true>(mp, tg, pl, bv, grid, nullptr, q, force, st);
+    return contract_fx_p<true>(mp, tg, pl, bv, grid, nullptr, q, force, fadd, st);
 }
 
 hipError_t launch_charges_fx(const double* x, int N, float* qf, FxScale* fs, double prefac, hipStream_t st)

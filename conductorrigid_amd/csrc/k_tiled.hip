@@ -376,7 +376,8 @@ __device__ __forceinline__ void contract_tile(const MeshParams& mp, const TileGe
                                               const unsigned* __restrict__ sorted, const int* __restrict__ rec_off,
                                               const T* __restrict__ rec_win, const double4* __restrict__ pos,
                                               const double* __restrict__ q, const T* __restrict__ grid,
-                                              T* __restrict__ phi, double* __restrict__ force, char* smem_raw)
+                                              T* __restrict__ phi, double* __restrict__ force,
+                                              const double* __restrict__ fadd, char* smem_raw)
 {
     const TileCoords tc = tile_coords(mp, tg, t);
     constexpr int PM = PT > 0 ? PT : 16;
@@ -447,10 +448,14 @@ __device__ __forceinline__ void contract_tile(const MeshParams& mp, const TileGe
                     // F = 2 (h^3 prefac) (2 xi^2) q_i sum phi_g e^{...} (r/eta), r = bead - node = (u - p) h
                     // (cppmodule/ConductorRigid.cu:449)
                     const double c = 2.0 * (mp.quadW * mp.prefac) * mp.xiterm * (double)s.qs[i];
-                    force[4 * bead + 0] = c * mp.h[0] / mp.eta[0] * (double)fx;
-                    force[4 * bead + 1] = c * mp.h[1] / mp.eta[1] * (double)fy;
-                    force[4 * bead + 2] = c * mp.h[2] / mp.eta[2] * (double)fz;
-                    force[4 * bead + 3] = 0.0;
+                    const long o = 4 * (long)bead;
+                    // fadd: the real-space force of the same bead, computed beside this pass
+                    const double a0 = fadd ? fadd[o] : 0.0, a1 = fadd ? fadd[o + 1] : 0.0;
+                    const double a2 = fadd ? fadd[o + 2] : 0.0;
+                    force[o + 0] = c * mp.h[0] / mp.eta[0] * (double)fx + a0;
+                    force[o + 1] = c * mp.h[1] / mp.eta[1] * (double)fy + a1;
+                    force[o + 2] = c * mp.h[2] / mp.eta[2] * (double)fz + a2;
+                    force[o + 3] = 0.0;
                 }
             }
         }
@@ -465,11 +470,11 @@ __global__ __launch_bounds__(contract_threads<T>()) void k_contract_tiled(MeshPa
                                                         const int* __restrict__ rec_off, const T* __restrict__ rec_win,
                                                         const double4* __restrict__ pos, const double* __restrict__ q,
                                                         const T* __restrict__ grid, T* __restrict__ phi,
-                                                        double* __restrict__ force)
+                                                        double* __restrict__ force, const double* __restrict__ fadd)
 {
     extern __shared__ char smem_raw[];
     contract_tile<T, FORCE, PT>(mp, tg, xcd_tile(blockIdx.x, tg.ntiles), ch, tstart, sorted, rec_off, rec_win, pos, q,
-                                grid, phi, force, smem_raw);
+                                grid, phi, force, fadd, smem_raw);
 }
 
 // Batched launches: one workgroup per tile of any box.  Boxes own consecutive ranges of
@@ -519,7 +524,7 @@ __global__ __launch_bounds__(contract_threads<T>()) void k_contract_tiled_batch(
     contract_tile<T, FORCE, PT>(d.mp, d.tg, g - d.tile0, d.ch, d.tile_start, d.sorted, d.rec_off,
                                 static_cast<const T*>(d.rec_win), d.pos, FORCE ? d.q : nullptr,
                                 static_cast<const T*>(d.grid), static_cast<T*>(d.phi), FORCE ? d.force : nullptr,
-                                smem_raw);
+                                nullptr, smem_raw);
 }
 
 // ---- batched per-evaluation preparation (see PrepBox) ------------------------
@@ -777,7 +782,8 @@ hipError_t launch_spread_tiled(const MeshParams& mp, const TileGeom& tg, const B
 
 template <typename T>
 hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const SysView& s, const BinView& bv,
-                                 const double* q, const T* grid, T* phi, double* force, hipStream_t st)
+                                 const double* q, const T* grid, T* phi, double* force, hipStream_t st,
+                                 const double* fadd)
 {
     const TiledLds l = tiled_lds(mp, tg, sizeof(T), sizeof(T));
     if (l.chunk < 4 || mp.P > 16) return hipErrorInvalidValue;
@@ -789,11 +795,11 @@ hipError_t launch_contract_tiled(const MeshParams& mp, const TileGeom& tg, const
         if (force)                                                                                              \
             hipLaunchKernelGGL((k_contract_tiled<T, true, PV>), dim3(tg.ntiles), dim3(contract_threads<T>()), l.total_bytes, st,  \
                                mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_win, s.pos, q, grid,    \
-                               (T*)nullptr, force);                                                             \
+                               (T*)nullptr, force, fadd);                                                       \
         else                                                                                                    \
             hipLaunchKernelGGL((k_contract_tiled<T, false, PV>), dim3(tg.ntiles), dim3(contract_threads<T>()), l.total_bytes, st, \
                                mp, tg, l.chunk, bv.tile_start, bv.sorted, bv.rec_off, rec_win, s.pos,             \
-                               (const double*)nullptr, grid, phi, (double*)nullptr);                            \
+                               (const double*)nullptr, grid, phi, (double*)nullptr, (const double*)nullptr);    \
     } while (0)
     switch (mp.P) {
         case 4: CR_CONTRACT_P(4); break;
@@ -883,7 +889,8 @@ hipError_t launch_contract_tiled_batch(const TiledBox* boxes, int nbox, int tota
     template hipError_t launch_spread_tiled<T>(const MeshParams&, const TileGeom&, const BinView&, const double*, \
                                                T*, hipStream_t, bool);                                          \
     template hipError_t launch_contract_tiled<T>(const MeshParams&, const TileGeom&, const SysView&,              \
-                                                 const BinView&, const double*, const T*, T*, double*, hipStream_t); \
+                                                 const BinView&, const double*, const T*, T*, double*, hipStream_t, \
+                                                 const double*);                                                    \
     template hipError_t launch_fill_batch<T>(const TiledBox*, int, long, const int*, hipStream_t);             \
     template hipError_t launch_spread_tiled_batch<T>(const TiledBox*, int, int, int, size_t, const int*,          \
                                                      hipStream_t);                                              \
