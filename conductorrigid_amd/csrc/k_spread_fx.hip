@@ -288,6 +288,157 @@ __device__ __forceinline__ void spread_fx_tile(const MeshParams& mp, const TileG
     for (int k = (esz / 4) * 4 + threadIdx.x; k < esz; k += kFxThreads) blk[k] = (int)ext[k];
 }
 
+// Persistent forms (compile-time tile shapes).  A workgroup walks `per` tiles: with per > 1
+// and ntiles a multiple of 8 per, workgroup b (dealt to XCD b mod 8) takes the consecutive
+// tiles (b mod 8) ntiles / 8 + (b / 8) per + k, so each XCD still holds a contiguous run.
+__device__ __forceinline__ int fx_run_tile(int b, int k, int per, int nt)
+{
+    if (per == 1) return fx_tile(b, nt);
+    return (b % 8) * (nt / 8) + (b / 8) * per + k;
+}
+
+// A tile's first records, one bead per thread slot: thread j holds the window record of
+// bead j / 3 (axis j % 3 of its weights, j < 3 min(n, npf)) and the offset and bead index
+// of bead j (j < min(n, npf)).
+struct FxPrefetch {
+    float4 u;
+    int ro;
+    unsigned si;
+};
+
+__device__ __forceinline__ FxPrefetch fx_prefetch(unsigned kb, unsigned ke, int npf, const int* __restrict__ rec_off,
+                                                  const unsigned* __restrict__ sorted, const float4* __restrict__ rec_u)
+{
+    FxPrefetch p{make_float4(0.f, 0.f, 0.f, 0.f), 0, 0u};
+    const int m = (int)min(ke - kb, (unsigned)npf), j = threadIdx.x;
+    if (j < 3 * m) p.u = rec_u[kb + j / 3];
+    if (j < m) {
+        p.ro = rec_off[kb + j];
+        p.si = sorted[kb + j];
+    }
+    return p;
+}
+
+// Staged weights of bead slot i along axis d (3 threads per bead): exp(-c_d (p - u_d)^2).
+template <int PT, int WS>
+__device__ __forceinline__ void fx_stage_weights(float* __restrict__ w, int i, int d, const float4& u4, float c0,
+                                                 float c1, float c2)
+{
+    const float ud = d == 0 ? u4.x : (d == 1 ? u4.y : u4.z);
+    const float cd = d == 0 ? c0 : (d == 1 ? c1 : c2);
+    float* wp = w + i * WS + d * PT;
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+        const float dd = (float)p - ud;
+        wp[p] = __expf(-cd * dd * dd);
+    }
+}
+
+// The spread of spread_fx_tile (ONES = false) as a persistent workgroup over `per` tiles.
+// While one tile accumulates, the registers already hold the next tile's records (and, by
+// the write-out, its charges), so that the staging's global-memory round trips overlap the
+// LDS work instead of stalling each tile.  The first `NR` rounds of a tile (NR ch beads,
+// 3 threads per bead for the weights) come from registers, later rounds from memory.
+// Integer sums: the same mesh bits as the per-tile kernel.
+template <int PT, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kFxThreads) void k_spread_fx_run(MeshParams mp, TileGeom tg, int ch, int per,
+                                                              const unsigned* __restrict__ tstart,
+                                                              const unsigned* __restrict__ sorted,
+                                                              const int* __restrict__ rec_off,
+                                                              const float4* __restrict__ rec_u,
+                                                              const float* __restrict__ qf,
+                                                              const FxScale* __restrict__ fs, long bstride,
+                                                              int* __restrict__ blocks)
+{
+    constexpr int WS = fx_wstride(PT);
+    constexpr int EX = TX + PT - 1, EY = TY + PT - 1, EZ = TZ + PT - 1;
+    constexpr int ys = (EZ + 1) & ~1, xs = EY * ys, esz = EX * xs;
+    constexpr int NW = kFxThreads / kFxWave;
+    extern __shared__ __align__(16) char smem[];
+    unsigned* ext = reinterpret_cast<unsigned*>(smem);
+    float* w = reinterpret_cast<float*>(smem + ((size_t)esz * 4 + 15) / 16 * 16);
+    int2* oq = reinterpret_cast<int2*>(w + (size_t)ch * WS);
+    const int j = threadIdx.x, lane = j & (kFxWave - 1), wave = j / kFxWave;
+    const int NR = max(1, (kFxThreads / 3) / ch), npf = NR * ch;
+    const int nt = tg.ntiles;
+    const float mult = fs->mult;
+    const float c0 = (float)(mp.xiterm * mp.h[0] * mp.h[0] / mp.eta[0]);
+    const float c1 = (float)(mp.xiterm * mp.h[1] * mp.h[1] / mp.eta[1]);
+    const float c2 = (float)(mp.xiterm * mp.h[2] * mp.h[2] / mp.eta[2]);
+    int t = fx_run_tile(blockIdx.x, 0, per, nt);
+    unsigned kb = tstart[t], ke = tstart[t + 1];
+    FxPrefetch cur = fx_prefetch(kb, ke, npf, rec_off, sorted, rec_u);
+    float qv = (j < (int)min(ke - kb, (unsigned)npf)) ? qf[cur.si] : 0.0f;
+    for (int k = 0; k < per; ++k) {
+        const bool more = k + 1 < per;
+        const int tn = more ? fx_run_tile(blockIdx.x, k + 1, per, nt) : t;
+        const unsigned kbn = more ? tstart[tn] : 0u, ken = more ? tstart[tn + 1] : 0u;
+        const FxPrefetch nxt = fx_prefetch(kbn, ken, npf, rec_off, sorted, rec_u);
+        for (int q = j; q < esz / 4; q += kFxThreads) reinterpret_cast<uint4*>(ext)[q] = make_uint4(0, 0, 0, 0);
+        if constexpr (esz % 4 != 0)
+            for (int q = (esz / 4) * 4 + j; q < esz; q += kFxThreads) ext[q] = 0u;
+        const unsigned n = ke - kb;
+        int r = 0;
+        for (unsigned r0 = 0; r0 < n; r0 += ch, ++r) {
+            const int cnt = (int)min((unsigned)ch, n - r0);
+            if (r > 0) __syncthreads();  // the previous round's beads are added
+            if (r < NR) {
+                const int jw = j - 3 * r * ch, jo = j - r * ch;
+                if (jw >= 0 && jw < 3 * cnt) fx_stage_weights<PT, WS>(w, jw / 3, jw - 3 * (jw / 3), cur.u, c0, c1, c2);
+                if (jo >= 0 && jo < cnt) oq[jo] = make_int2(cur.ro, __float_as_int(mult * qv));
+            } else {
+                for (int jj = j; jj < 3 * cnt; jj += kFxThreads)
+                    fx_stage_weights<PT, WS>(w, jj / 3, jj - 3 * (jj / 3), rec_u[kb + r0 + jj / 3], c0, c1, c2);
+                for (int jj = j; jj < cnt; jj += kFxThreads)
+                    oq[jj] = make_int2(rec_off[kb + r0 + jj], __float_as_int(mult * qf[sorted[kb + r0 + jj]]));
+            }
+            __syncthreads();
+            for (int i = wave; i < cnt; i += NW) {
+                const int2 rr = oq[i];
+                const int ox = rr.x & 1023, oy = (rr.x >> 10) & 1023, oz = rr.x >> 20;
+                const float qs = __int_as_float(rr.y);
+                const float* wi = w + i * WS;
+                float wx[PT];
+                if constexpr (PT % 4 == 0) {
+#pragma unroll
+                    for (int p = 0; p < PT; p += 4) {
+                        const float4 v4 = *reinterpret_cast<const float4*>(wi + p);
+                        wx[p] = v4.x;
+                        wx[p + 1] = v4.y;
+                        wx[p + 2] = v4.z;
+                        wx[p + 3] = v4.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int p = 0; p < PT; ++p) wx[p] = wi[p];
+                }
+#pragma unroll
+                for (int pair = lane; pair < PT * PT; pair += kFxWave) {
+                    const int py = pair / PT, pz = pair - py * PT;
+                    const float c = qs * wi[PT + py] * wi[2 * PT + pz];
+                    unsigned* col = ext + ox * xs + (oy + py) * ys + (oz + pz);
+#pragma unroll
+                    for (int px = 0; px < PT; ++px) atomicAdd(col + px * xs, (unsigned)round_fx(c * wx[px]));
+                }
+            }
+        }
+        // the next tile's charges (its bead indices have arrived), in flight over the write-out
+        const float qvn = (more && j < (int)min(ken - kbn, (unsigned)npf)) ? qf[nxt.si] : 0.0f;
+        __syncthreads();
+        int* blk = blocks + (long)t * bstride;
+        for (int q = j; q < esz / 4; q += kFxThreads)
+            reinterpret_cast<uint4*>(blk)[q] = reinterpret_cast<const uint4*>(ext)[q];
+        if constexpr (esz % 4 != 0)
+            for (int q = (esz / 4) * 4 + j; q < esz; q += kFxThreads) blk[q] = (int)ext[q];
+        __syncthreads();
+        t = tn;
+        kb = kbn;
+        ke = ken;
+        cur = nxt;
+        qv = qvn;
+    }
+}
+
 template <int PT, int TX, int TY, int TZ, bool ONES>
 __global__ __launch_bounds__(kFxThreads) void k_spread_fx(MeshParams mp, TileGeom tg, int ch,
                                                           const unsigned* __restrict__ tstart,
@@ -590,10 +741,29 @@ void allow_lds_fx(const void* fn, size_t bytes)
     if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess) have = bytes;
 }
 
+// Tiles per persistent workgroup: the largest of 8, 4, 2 that divides the tiles into XCD
+// runs and still gives 512 workgroups (two per CU of an MI355X), else 1.
+constexpr int kFxRunMaxPer = 8;
+inline int fx_run_per(int nt)
+{
+    for (int per = kFxRunMaxPer; per > 1; per /= 2)
+        if (nt % (8 * per) == 0 && nt / per >= 512) return per;
+    return 1;
+}
+
 template <int PT, int TX, int TY, int TZ, bool ONES>
 hipError_t launch_fx_kernel(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
                             const float* qf, FxScale* fs, long bstride, int* blocks, hipStream_t st)
 {
+    if constexpr (!ONES && TX > 0) {
+        const int per = fx_run_per(tg.ntiles);
+        auto fr = k_spread_fx_run<PT, TX, TY, TZ>;
+        allow_lds_fx(reinterpret_cast<const void*>(fr), pl.lds_bytes);
+        hipLaunchKernelGGL(fr, dim3(tg.ntiles / per), dim3(kFxThreads), pl.lds_bytes, st, mp, tg, pl.chunk, per,
+                           bv.tile_start, bv.sorted, bv.rec_off, static_cast<const float4*>(bv.rec_win), qf, fs,
+                           bstride, blocks);
+        return hipGetLastError();
+    }
     auto fn = k_spread_fx<PT, TX, TY, TZ, ONES>;
     allow_lds_fx(reinterpret_cast<const void*>(fn), pl.lds_bytes);
     hipLaunchKernelGGL(fn, dim3(tg.ntiles), dim3(kFxThreads), pl.lds_bytes, st, mp, tg, pl.chunk, bv.tile_start,
