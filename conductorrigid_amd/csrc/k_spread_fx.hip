@@ -173,6 +173,30 @@ struct FxWalk {
 // Per-bead staging stride (floats): the 3 P axis weights, padded to whole float4s.
 __host__ __device__ constexpr int fx_wstride(int P) { return (3 * P + 3) / 4 * 4; }
 
+// A packed window record (x | y << 10 | z << 20, the window's first node in the extended
+// tile) as an element offset in an LDS tile of row pitches xs (x) and ys (y).
+__device__ __forceinline__ int fx_ext_off(int rec, int xs, int ys)
+{
+    return (rec & 1023) * xs + ((rec >> 10) & 1023) * ys + (rec >> 20);
+}
+
+// The bead loops run one wavefront per bead over the wavefront's uniform bead index; the
+// lane's window node pairs (py, pz) = pair / P, pair % P for pair = lane and lane + 64.
+__device__ __forceinline__ int fx_wave() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kFxWave)); }
+
+// Sum over the 64 lanes of a fully active wavefront into lane 63 (DPP row shifts give each
+// row's total in its lane 15, then row broadcasts 15 and 31 carry them up).
+__device__ __forceinline__ float fx_total63(float v)
+{
+    v += se::dpp_shift<0x111>(v);
+    v += se::dpp_shift<0x112>(v);
+    v += se::dpp_shift<0x114>(v);
+    v += se::dpp_shift<0x118>(v);
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xa, 0xf, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x143, 0xc, 0xf, false));
+    return v;
+}
+
 // TX = 0: tile edges from tg at run time; otherwise compile-time edges (the LDS strides
 // become instruction offsets).  ONES: accumulate sum_b w_b with a per-tile multiplier and
 // report the tile's largest node into fs->wbits instead of writing the mesh.
@@ -212,7 +236,7 @@ __device__ __forceinline__ void spread_fx_tile(const MeshParams& mp, const TileG
     const float c0 = (float)(mp.xiterm * mp.h[0] * mp.h[0] / mp.eta[0]);
     const float c1 = (float)(mp.xiterm * mp.h[1] * mp.h[1] / mp.eta[1]);
     const float c2 = (float)(mp.xiterm * mp.h[2] * mp.h[2] / mp.eta[2]);
-    const int lane = threadIdx.x & (kFxWave - 1), wave = threadIdx.x / kFxWave;
+    const int lane = threadIdx.x & (kFxWave - 1), wave = fx_wave();
     constexpr int NW = kFxThreads / kFxWave;
     for (unsigned k0 = kb; k0 < ke; k0 += ch) {
         const int cnt = (int)min((unsigned)ch, ke - k0);
@@ -231,13 +255,12 @@ __device__ __forceinline__ void spread_fx_tile(const MeshParams& mp, const TileG
         }
         for (int i = threadIdx.x; i < cnt; i += kFxThreads) {
             const float qs = ONES ? mult : mult * qf[sorted[k0 + i]];
-            oq[i] = make_int2(rec_off[k0 + i], __float_as_int(qs));
+            oq[i] = make_int2(fx_ext_off(rec_off[k0 + i], xs, ys), __float_as_int(qs));
         }
         __syncthreads();
         // one wavefront per bead: lanes over the (y, z) window plane, each walking x
         for (int i = wave; i < cnt; i += NW) {
             const int2 r = oq[i];
-            const int ox = r.x & 1023, oy = (r.x >> 10) & 1023, oz = r.x >> 20;
             const float qs = __int_as_float(r.y);
             const float* wi = w + i * WS;
             float wx[PT];
@@ -258,7 +281,7 @@ __device__ __forceinline__ void spread_fx_tile(const MeshParams& mp, const TileG
             for (int pair = lane; pair < PT * PT; pair += kFxWave) {
                 const int py = pair / PT, pz = pair - py * PT;
                 const float c = qs * wi[PT + py] * wi[2 * PT + pz];
-                unsigned* col = ext + ox * xs + (oy + py) * ys + (oz + pz);
+                unsigned* col = ext + r.x + py * ys + pz;
 #pragma unroll
                 for (int px = 0; px < PT; ++px) atomicAdd(col + px * xs, (unsigned)round_fx(c * wx[px]));
             }
@@ -358,7 +381,7 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx_run(MeshParams mp, Til
     unsigned* ext = reinterpret_cast<unsigned*>(smem);
     float* w = reinterpret_cast<float*>(smem + ((size_t)esz * 4 + 15) / 16 * 16);
     int2* oq = reinterpret_cast<int2*>(w + (size_t)ch * WS);
-    const int j = threadIdx.x, lane = j & (kFxWave - 1), wave = j / kFxWave;
+    const int j = threadIdx.x, lane = j & (kFxWave - 1), wave = fx_wave();
     const int NR = max(1, (kFxThreads / 3) / ch), npf = NR * ch;
     const int nt = tg.ntiles;
     const float mult = fs->mult;
@@ -385,17 +408,17 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx_run(MeshParams mp, Til
             if (r < NR) {
                 const int jw = j - 3 * r * ch, jo = j - r * ch;
                 if (jw >= 0 && jw < 3 * cnt) fx_stage_weights<PT, WS>(w, jw / 3, jw - 3 * (jw / 3), cur.u, c0, c1, c2);
-                if (jo >= 0 && jo < cnt) oq[jo] = make_int2(cur.ro, __float_as_int(mult * qv));
+                if (jo >= 0 && jo < cnt) oq[jo] = make_int2(fx_ext_off(cur.ro, xs, ys), __float_as_int(mult * qv));
             } else {
                 for (int jj = j; jj < 3 * cnt; jj += kFxThreads)
                     fx_stage_weights<PT, WS>(w, jj / 3, jj - 3 * (jj / 3), rec_u[kb + r0 + jj / 3], c0, c1, c2);
                 for (int jj = j; jj < cnt; jj += kFxThreads)
-                    oq[jj] = make_int2(rec_off[kb + r0 + jj], __float_as_int(mult * qf[sorted[kb + r0 + jj]]));
+                    oq[jj] = make_int2(fx_ext_off(rec_off[kb + r0 + jj], xs, ys),
+                                       __float_as_int(mult * qf[sorted[kb + r0 + jj]]));
             }
             __syncthreads();
             for (int i = wave; i < cnt; i += NW) {
                 const int2 rr = oq[i];
-                const int ox = rr.x & 1023, oy = (rr.x >> 10) & 1023, oz = rr.x >> 20;
                 const float qs = __int_as_float(rr.y);
                 const float* wi = w + i * WS;
                 float wx[PT];
@@ -416,7 +439,7 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx_run(MeshParams mp, Til
                 for (int pair = lane; pair < PT * PT; pair += kFxWave) {
                     const int py = pair / PT, pz = pair - py * PT;
                     const float c = qs * wi[PT + py] * wi[2 * PT + pz];
-                    unsigned* col = ext + ox * xs + (oy + py) * ys + (oz + pz);
+                    unsigned* col = ext + rr.x + py * ys + pz;
 #pragma unroll
                     for (int px = 0; px < PT; ++px) atomicAdd(col + px * xs, (unsigned)round_fx(c * wx[px]));
                 }
@@ -612,7 +635,7 @@ __device__ __forceinline__ void contract_fx_tile(const MeshParams& mp, const Til
     const float c1 = (float)(mp.xiterm * mp.h[1] * mp.h[1] / mp.eta[1]);
     const float c2 = (float)(mp.xiterm * mp.h[2] * mp.h[2] / mp.eta[2]);
     const float pc = (float)(mp.quadW * mp.prefac);
-    const int lane = threadIdx.x & (kFxWave - 1), wave = threadIdx.x / kFxWave;
+    const int lane = threadIdx.x & (kFxWave - 1), wave = fx_wave();
     constexpr int NW = kFxThreads / kFxWave;
     for (unsigned k0 = kb; k0 < ke; k0 += ch) {
         const int cnt = (int)min((unsigned)ch, ke - k0);
@@ -628,12 +651,15 @@ __device__ __forceinline__ void contract_fx_tile(const MeshParams& mp, const Til
                 wp[p] = __expf(-cd * dd * dd);
             }
         }
-        for (int i = threadIdx.x; i < cnt; i += kFxThreads) oi[i] = make_int2(rec_off[k0 + i], (int)sorted[k0 + i]);
+        for (int i = threadIdx.x; i < cnt; i += kFxThreads)
+            oi[i] = make_int2(fx_ext_off(rec_off[k0 + i], xs, ys), (int)sorted[k0 + i]);
         __syncthreads();  // (the first one also orders the extended tile's stores)
+        // A bead's totals land in lane 63, which leaves them in LDS (the potential in its
+        // record's offset slot, the force sums in its first weight slots, both read before);
+        // the chunk's results then go out together.
         for (int i = wave; i < cnt; i += NW) {
             const int2 r = oi[i];
-            const int ox = r.x & 1023, oy = (r.x >> 10) & 1023, oz = r.x >> 20;
-            const float* wi = w + i * WS;
+            float* wi = w + i * WS;
             float wx[PT];
             if constexpr (PT % 4 == 0) {
 #pragma unroll
@@ -653,14 +679,14 @@ __device__ __forceinline__ void contract_fx_tile(const MeshParams& mp, const Til
 #pragma unroll
                 for (int pair = lane; pair < PT * PT; pair += kFxWave) {
                     const int py = pair / PT, pz = pair - py * PT;
-                    const float* col = ext + ox * xs + (oy + py) * ys + (oz + pz);
+                    const float* col = ext + r.x + py * ys + pz;
                     float sx = 0.0f;
 #pragma unroll
                     for (int px = 0; px < PT; ++px) sx += wx[px] * col[px * xs];
                     acc += wi[PT + py] * wi[2 * PT + pz] * sx;
                 }
-                acc = se::wave_total(acc);
-                if (lane == 0) phi[r.y] = acc * pc;
+                acc = fx_total63(acc);
+                if (lane == kFxWave - 1) oi[i].x = __float_as_int(acc);
             } else {
                 // gradient of the bead's window: node offsets r = (u - p) h per axis
                 const float4 u4 = rec_u[k0 + i];
@@ -668,7 +694,7 @@ __device__ __forceinline__ void contract_fx_tile(const MeshParams& mp, const Til
 #pragma unroll
                 for (int pair = lane; pair < PT * PT; pair += kFxWave) {
                     const int py = pair / PT, pz = pair - py * PT;
-                    const float* col = ext + ox * xs + (oy + py) * ys + (oz + pz);
+                    const float* col = ext + r.x + py * ys + pz;
                     float sx = 0.0f, sxr = 0.0f;
 #pragma unroll
                     for (int px = 0; px < PT; ++px) {
@@ -681,22 +707,34 @@ __device__ __forceinline__ void contract_fx_tile(const MeshParams& mp, const Til
                     fy += wyz * sx * (u4.y - (float)py);
                     fz += wyz * sx * (u4.z - (float)pz);
                 }
-                fx = se::wave_total(fx);
-                fy = se::wave_total(fy);
-                fz = se::wave_total(fz);
-                if (lane == 0) {
-                    // F = 2 (h^3 prefac) (2 xi^2) q_i sum_nodes grid w (r / eta), the reference's
-                    // contract_force (cppmodule/ConductorRigid.cu:449)
-                    const double c = 2.0 * (mp.quadW * mp.prefac) * mp.xiterm * (double)(float)q[r.y];
-                    const long o = 4 * (long)r.y;
-                    // fadd: the real-space force of the same bead, computed beside this pass
-                    const double a0 = fadd ? fadd[o] : 0.0, a1 = fadd ? fadd[o + 1] : 0.0;
-                    const double a2 = fadd ? fadd[o + 2] : 0.0;
-                    force[o + 0] = c * mp.h[0] / mp.eta[0] * (double)fx + a0;
-                    force[o + 1] = c * mp.h[1] / mp.eta[1] * (double)fy + a1;
-                    force[o + 2] = c * mp.h[2] / mp.eta[2] * (double)fz + a2;
-                    force[o + 3] = 0.0;
+                fx = fx_total63(fx);
+                fy = fx_total63(fy);
+                fz = fx_total63(fz);
+                if (lane == kFxWave - 1) {
+                    wi[0] = fx;
+                    wi[1] = fy;
+                    wi[2] = fz;
                 }
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < cnt; i += kFxThreads) {
+            const int2 r = oi[i];
+            if constexpr (!FORCE) {
+                phi[r.y] = __int_as_float(r.x) * pc;
+            } else {
+                // F = 2 (h^3 prefac) (2 xi^2) q_i sum_nodes grid w (r / eta), the reference's
+                // contract_force (cppmodule/ConductorRigid.cu:449)
+                const float* wi = w + i * WS;
+                const double c = 2.0 * (mp.quadW * mp.prefac) * mp.xiterm * (double)(float)q[r.y];
+                const long o = 4 * (long)r.y;
+                // fadd: the real-space force of the same bead, computed beside this pass
+                const double a0 = fadd ? fadd[o] : 0.0, a1 = fadd ? fadd[o + 1] : 0.0;
+                const double a2 = fadd ? fadd[o + 2] : 0.0;
+                force[o + 0] = c * mp.h[0] / mp.eta[0] * (double)wi[0] + a0;
+                force[o + 1] = c * mp.h[1] / mp.eta[1] * (double)wi[1] + a1;
+                force[o + 2] = c * mp.h[2] / mp.eta[2] * (double)wi[2] + a2;
+                force[o + 3] = 0.0;
             }
         }
         __syncthreads();
