@@ -40,6 +40,7 @@ namespace {
 
 constexpr int kFxWave = 64;
 constexpr int kFxChargeThreads = 256;
+constexpr int kFxLoadBatch = 5;  // extended-tile items per thread in flight (16^3 tiles, P = 8: 4232 items)
 
 __device__ __forceinline__ int round_fx(float x)
 {
@@ -602,22 +603,47 @@ __device__ __forceinline__ void contract_fx_tile(const MeshParams& mp, const Til
     float* w = reinterpret_cast<float*>(smem + ((size_t)esz * 4 + 15) / 16 * 16);
     int2* oi = reinterpret_cast<int2*>(w + (size_t)ch * WS);
     const long plane = (long)mp.Ny * mp.Nz;
-    if (T2 % 4 == 0 && mp.Nz % 4 == 0 && z0 + EZ <= mp.Nz) {
-        // rows of the tile along z: whole float4s, then the remainder
+    if (T2 % 4 == 0 && mp.Nz % 4 == 0) {
+        // rows of the tile along z: whole float4s, then the remainder nodes.  The tile's z
+        // start and Nz are multiples of 4, so a float4 group wraps past Nz as a whole.  A
+        // thread's items are loaded in batches of kFxLoadBatch before any is stored, so
+        // their memory latencies overlap instead of adding up.
         const int nv = EZ / 4, per_row = nv + (EZ - 4 * nv), items = EX * EY * per_row;
-        for (int k = threadIdx.x; k < items; k += kFxThreads) {
-            const int r = k / per_row, c = k - r * per_row;
-            const int ly = r % EY, lx = r / EY;
-            int gx = x0 + lx, gy = y0 + ly;
-            gx -= gx >= mp.Nx ? mp.Nx : 0;
-            gy -= gy >= mp.Ny ? mp.Ny : 0;
-            const float* src = grid + (long)gx * plane + (long)gy * mp.Nz + z0;
-            float* dst = ext + lx * xs + ly * ys;
-            if (c < nv) {
-                *reinterpret_cast<float4*>(dst + 4 * c) = *reinterpret_cast<const float4*>(src + 4 * c);
-            } else {
-                const int z = 4 * nv + (c - nv);
-                dst[z] = src[z];
+        for (int k0 = threadIdx.x; k0 < items; k0 += kFxLoadBatch * kFxThreads) {
+            float4 v[kFxLoadBatch];
+            int dsto[kFxLoadBatch];
+#pragma unroll
+            for (int u = 0; u < kFxLoadBatch; ++u) {
+                const int k = k0 + u * kFxThreads;
+                v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                dsto[u] = -1;
+                if (k < items) {
+                    const int r = k / per_row, c = k - r * per_row;
+                    const int ly = r % EY, lx = r / EY;
+                    int gx = x0 + lx, gy = y0 + ly;
+                    gx -= gx >= mp.Nx ? mp.Nx : 0;
+                    gy -= gy >= mp.Ny ? mp.Ny : 0;
+                    const float* src = grid + (long)gx * plane + (long)gy * mp.Nz;
+                    if (c < nv) {
+                        int gz = z0 + 4 * c;
+                        gz -= gz >= mp.Nz ? mp.Nz : 0;
+                        v[u] = *reinterpret_cast<const float4*>(src + gz);
+                        dsto[u] = lx * xs + ly * ys + 4 * c;
+                    } else {
+                        const int z = 4 * nv + (c - nv);
+                        int gz = z0 + z;
+                        gz -= gz >= mp.Nz ? mp.Nz : 0;
+                        v[u].x = src[gz];
+                        dsto[u] = -2 - (lx * xs + ly * ys + z);  // a single node
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kFxLoadBatch; ++u) {
+                if (dsto[u] >= 0)
+                    *reinterpret_cast<float4*>(ext + dsto[u]) = v[u];
+                else if (dsto[u] <= -2)
+                    ext[-2 - dsto[u]] = v[u].x;
             }
         }
     } else {
