@@ -96,10 +96,29 @@ __device__ __forceinline__ NlSpan stage_nlist(const SysView& s, unsigned* sh, in
     }
     __syncthreads();
     NlSpan sp{bounds[0], bounds[1]};
-    for (unsigned k = sp.lo + threadIdx.x; k < sp.hi; k += blockDim.x) sh[k - sp.lo] = s.nlist[k];
+    // a thread's loads go out in batches of 8 before their stores, so that their latencies
+    // overlap (one load and wait per entry would serialise ~30 round trips per workgroup)
+    constexpr unsigned kBatch = 8;
+    for (unsigned k0 = sp.lo + threadIdx.x; k0 < sp.hi; k0 += kBatch * blockDim.x) {
+        unsigned v[kBatch];
+#pragma unroll
+        for (unsigned u = 0; u < kBatch; ++u) {
+            const unsigned k = k0 + u * blockDim.x;
+            v[u] = k < sp.hi ? s.nlist[k] : 0u;
+        }
+#pragma unroll
+        for (unsigned u = 0; u < kBatch; ++u) {
+            const unsigned k = k0 + u * blockDim.x;
+            if (k < sp.hi) sh[k - sp.lo] = v[u];
+        }
+    }
     __syncthreads();
     return sp;
 }
+
+// Neighbours of a row are visited in groups of kNbGroup: their indices, then their
+// positions, are loaded before any is used (the row's order of visits is unchanged).
+constexpr unsigned kNbGroup = 4;
 
 __device__ __forceinline__ unsigned nl_at(const SysView& s, const unsigned* sh, const NlSpan& sp, unsigned k)
 {
@@ -118,9 +137,7 @@ __device__ __forceinline__ void csr_build_rows(const RealParams& rp, const SysVi
     const double4 ri = s.pos[i];
     const unsigned head = s.head_list[i], nn = s.n_neigh[i];
     unsigned cnt = 0;
-    for (unsigned jj = 0; jj < nn; ++jj) {
-        const unsigned j = nl_at(s, sh, sp, head + jj);
-        const double4 rj = s.pos[j];
+    auto visit = [&](unsigned j, const double4& rj) {
         const double dx = min_image(ri.x - rj.x, rp.L[0]);
         const double dy = min_image(ri.y - rj.y, rp.L[1]);
         const double dz = min_image(ri.z - rj.z, rp.L[2]);
@@ -130,6 +147,21 @@ __device__ __forceinline__ void csr_build_rows(const RealParams& rp, const SysVi
             val[cnt * ldn + i] = table_lookup<T>(rp, table, sqrt(d2));
             ++cnt;
         }
+    };
+    unsigned jj = 0;
+    for (; jj + kNbGroup <= nn; jj += kNbGroup) {
+        unsigned j[kNbGroup];
+        double4 rj[kNbGroup];
+#pragma unroll
+        for (unsigned u = 0; u < kNbGroup; ++u) j[u] = nl_at(s, sh, sp, head + jj + u);
+#pragma unroll
+        for (unsigned u = 0; u < kNbGroup; ++u) rj[u] = s.pos[j[u]];
+#pragma unroll
+        for (unsigned u = 0; u < kNbGroup; ++u) visit(j[u], rj[u]);
+    }
+    for (; jj < nn; ++jj) {
+        const unsigned j = nl_at(s, sh, sp, head + jj);
+        visit(j, s.pos[j]);
     }
     row_len[i] = cnt;
 }
@@ -223,9 +255,7 @@ __device__ __forceinline__ void force_rows(const RealParams& rp, const SysView& 
     T fx = 0, fy = 0, fz = 0;
     const double4 ri = s.pos[i];
     const unsigned head = s.head_list[i], nn = s.n_neigh[i];
-    for (unsigned jj = 0; jj < nn; ++jj) {
-        const unsigned j = nl_at(s, sh, sp, head + jj);
-        const double4 rj = s.pos[j];
+    auto visit = [&](const double4& rj, double qj) {
         const double dx = min_image(ri.x - rj.x, rp.L[0]);
         const double dy = min_image(ri.y - rj.y, rp.L[1]);
         const double dz = min_image(ri.z - rj.z, rp.L[2]);
@@ -233,11 +263,30 @@ __device__ __forceinline__ void force_rows(const RealParams& rp, const SysView& 
         if (d2 < rp.rc2 && d2 >= rp.rmin2) {
             const double d = sqrt(d2);
             const T A = table_lookup<T>(rp, gtable, d);
-            const T c = -A * (T)(qi * q[j] / d);
+            const T c = -A * (T)(qi * qj / d);
             fx += c * (T)dx;
             fy += c * (T)dy;
             fz += c * (T)dz;
         }
+    };
+    unsigned jj = 0;
+    for (; jj + kNbGroup <= nn; jj += kNbGroup) {
+        unsigned j[kNbGroup];
+        double4 rj[kNbGroup];
+        double qj[kNbGroup];
+#pragma unroll
+        for (unsigned u = 0; u < kNbGroup; ++u) j[u] = nl_at(s, sh, sp, head + jj + u);
+#pragma unroll
+        for (unsigned u = 0; u < kNbGroup; ++u) {
+            rj[u] = s.pos[j[u]];
+            qj[u] = q[j[u]];
+        }
+#pragma unroll
+        for (unsigned u = 0; u < kNbGroup; ++u) visit(rj[u], qj[u]);
+    }
+    for (; jj < nn; ++jj) {
+        const unsigned j = nl_at(s, sh, sp, head + jj);
+        visit(s.pos[j], q[j]);
     }
     force[4 * i] = Fx + (double)fx;
     force[4 * i + 1] = Fy + (double)fy;
