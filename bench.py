@@ -42,13 +42,13 @@ HBM_PEAK_GBS = 8000.0
 # matvec stage -> kernel it times (FP32-bulk, P = 8)
 STAGE_KERNEL = {
     "mesh_clear": "k_charges_fx + k_fx_scale",
-    "spread": "k_spread_fx<8, 16, 16, 16, false> + k_fx_gather<4>",
-    "fft_z_r2c": "k_fft_z_r2c<float>",
-    "fft_y_fwd": "k_fft_strided<float>",
-    "fft_x_scale": "k_fft_x_scale<float>",
-    "fft_y_inv": "k_fft_strided<float>",
-    "fft_z_c2r": "k_fft_z_c2r<float>",
-    "contract": "k_contract_fx<8, 16, 16, 16>",
+    "spread": "k_spread_fx_run<8, 16, 16, 16> + k_fx_gather<4>",
+    "fft_z_r2c": "k_fftr_z_r2c<float, 16, 16>",
+    "fft_y_fwd": "k_fftr_strided<float, 16, 16, -1>",
+    "fft_x_scale": "k_fftr_x_scale<float, 16, 16>",
+    "fft_y_inv": "k_fftr_strided<float, 16, 16, 1>",
+    "fft_z_c2r": "k_fftr_z_c2r<float, 16, 16>",
+    "contract": "k_contract_fx<8, 16, 16, 16, false>",
     "real_space": "k_real_space_csr<float>",
     "constraint_rows": "k_sum_charges + k_set_charge",
 }
@@ -252,9 +252,10 @@ FXC_SHAPES = {8: {(16, 16, 16), (16, 8, 16), (8, 16, 16), (16, 16, 32), (8, 8, 8
 
 
 def spread_kernel_name(P: int, tile, mesh) -> str:
-    """The FP32-bulk spread stage: the tile kernel, then the gather of its blocks."""
+    """The FP32-bulk spread stage: the tile kernel (persistent form for compile-time tile
+    shapes), then the gather of its blocks."""
     t = tuple(int(v) for v in tile)
-    k = f"k_spread_fx<{P}, {t[0]}, {t[1]}, {t[2]}, false>" if t in FX_SHAPES.get(P, ()) else \
+    k = f"k_spread_fx_run<{P}, {t[0]}, {t[1]}, {t[2]}>" if t in FX_SHAPES.get(P, ()) else \
         f"k_spread_fx<{P}, 0, 0, 0, false>"
     ys = (t[2] + P) & ~1  # block row pitch
     v = 4 if t[2] % 4 == 0 and ys % 4 == 0 and int(mesh[2]) % 4 == 0 else 1
@@ -264,8 +265,8 @@ def spread_kernel_name(P: int, tile, mesh) -> str:
 def contract_kernel_name(P: int, tile) -> str:
     t = tuple(int(v) for v in tile)
     if t in FXC_SHAPES.get(P, ()):
-        return f"k_contract_fx<{P}, {t[0]}, {t[1]}, {t[2]}>"
-    return f"k_contract_fx<{P}, 0, 0, 0>"
+        return f"k_contract_fx<{P}, {t[0]}, {t[1]}, {t[2]}, false>"
+    return f"k_contract_fx<{P}, 0, 0, 0, false>"
 
 
 def direct_ewald_errors() -> dict:
@@ -314,6 +315,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-boxes summary")
     ap.add_argument("--no-direct", action="store_true", help="skip the direct-Ewald error leg")
     ap.add_argument("--profile-reps", type=int, default=5)
+    ap.add_argument("--tile", default="", metavar="TX,TY,TZ",
+                    help="FP32-bulk tile shape instead of the autotuner's choice (reproducible profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -377,7 +380,12 @@ def main() -> int:
     nlist_build_ms = float(np.median(nl_ms))
     log(f"params {cr32.params['mesh']} P={cr32.params['P']} rc={cr32.params['rc']} pairs={cr32.dsys.n_pairs} "
         f"nlist build {nlist_build_ms:.2f} ms")
-    tune32 = cr32.autotune()  # setup, untimed: the tile shape for this system
+    if args.tile:  # a fixed tiling (profiling runs); otherwise the autotuner's, untimed setup
+        tile = [int(v) for v in args.tile.split(",")]
+        cr32.set_tiles(tile, 131072)
+        tune32 = {"tile": tile, "lds_bytes": 131072, "fixed": True}
+    else:
+        tune32 = cr32.autotune()
     log(f"autotune fp32: {tune32}")
     # timed evaluations do what the reference's computeForces does (solve + forces); the
     # true-residual check (one extra operator application) runs on an untimed evaluation
