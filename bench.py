@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # matvec stage -> kernel it times (FP32-bulk, P = 8)
 STAGE_KERNEL = {
     "mesh_clear": "k_charges_fx + k_fx_scale",
-    "spread": "k_spread_fx_run<8, 16, 16, 16> + k_fx_gather<4>",
+    "spread": "k_spread_fx_run<8, 16, 16, 16> + k_fx_gather_zc<8, 16, 16, 16>",
     "fft_z_r2c": "k_fftr_z_r2c<float, 16, 16>",
     "fft_y_fwd": "k_fftr_strided<float, 16, 16, -1>",
     "fft_x_scale": "k_fftr_x_scale<float, 16, 16>",
@@ -252,14 +252,14 @@ FXC_SHAPES = {8: {(16, 16, 16), (16, 8, 16), (8, 16, 16), (16, 16, 32), (8, 8, 8
 
 
 def spread_kernel_name(P: int, tile, mesh) -> str:
-    """The FP32-bulk spread stage: the tile kernel (persistent form for compile-time tile
-    shapes), then the gather of its blocks."""
+    """The FP32-bulk spread stage: the tile kernel and the gather of its blocks (for
+    compile-time tile shapes the persistent form, whose blocks carry the z halo)."""
     t = tuple(int(v) for v in tile)
-    k = f"k_spread_fx_run<{P}, {t[0]}, {t[1]}, {t[2]}>" if t in FX_SHAPES.get(P, ()) else \
-        f"k_spread_fx<{P}, 0, 0, 0, false>"
+    if t in FX_SHAPES.get(P, ()):
+        return f"k_spread_fx_run<{P}, {t[0]}, {t[1]}, {t[2]}> + k_fx_gather_zc<{P}, {t[0]}, {t[1]}, {t[2]}>"
     ys = (t[2] + P) & ~1  # block row pitch
     v = 4 if t[2] % 4 == 0 and ys % 4 == 0 and int(mesh[2]) % 4 == 0 else 1
-    return f"{k} + k_fx_gather<{v}>"
+    return f"k_spread_fx<{P}, 0, 0, 0, false> + k_fx_gather<{v}>"
 
 
 def contract_kernel_name(P: int, tile) -> str:

@@ -358,12 +358,30 @@ __device__ __forceinline__ void fx_stage_weights(float* __restrict__ w, int i, i
     }
 }
 
-// The spread of spread_fx_tile (ONES = false) as a persistent workgroup over `per` tiles.
-// While one tile accumulates, the registers already hold the next tile's records (and, by
-// the write-out, its charges), so that the staging's global-memory round trips overlap the
-// LDS work instead of stalling each tile.  The first `NR` rounds of a tile (NR ch beads,
-// 3 threads per bead for the weights) come from registers, later rounds from memory.
-// Integer sums: the same mesh bits as the per-tile kernel.
+// The spread of spread_fx_tile (ONES = false) as a persistent workgroup over `per` tiles,
+// consecutive along z in one (x, y) column of tiles.  While one tile accumulates, the
+// registers already hold the next tile's records (and, by the write-out, its charges), so
+// that the staging's global-memory round trips overlap the LDS work instead of stalling
+// each tile.  The first `NR` rounds of a tile (NR ch beads, 3 threads per bead for the
+// weights) come from registers, later rounds from memory.
+//
+// The +z halo of a tile (its extended nodes z = TZ .. TZ + P - 2) is the low z range of the
+// next tile of the run, so it stays in LDS: after a tile's write-out the halo rows move to
+// the front of the accumulator and the rest is cleared, and the next tile adds onto them.
+// A tile therefore leaves only its TZ z-planes (an EX x EY x TZ block); the last tile of a
+// run also leaves its halo (an EX x EY x HS block per run, HS = P - 1 rounded to 4), which
+// k_fx_gather_zc adds to the first tile of the next run.  Integer sums: the same mesh bits
+// as the per-tile kernel.
+template <int PT, int TX, int TY, int TZ>
+struct FxZc {
+    static constexpr int EX = TX + PT - 1, EY = TY + PT - 1, EZ = TZ + PT - 1;
+    static constexpr int ys = (EZ + 1) & ~1;       // the LDS tile's row pitch
+    static constexpr int H = PT - 1, HS = (H + 3) & ~3;
+    // rows of 16-byte groups, and a halo row that ends inside the zero padding of an LDS row
+    static constexpr bool ok = TZ % 4 == 0 && ys % 4 == 0 && TZ + HS <= ys && HS <= TZ;
+    static constexpr long interior = (long)EX * EY * TZ, halo = (long)EX * EY * HS;
+};
+
 template <int PT, int TX, int TY, int TZ>
 __global__ __launch_bounds__(kFxThreads) void k_spread_fx_run(MeshParams mp, TileGeom tg, int ch, int per,
                                                               const unsigned* __restrict__ tstart,
@@ -371,7 +389,7 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx_run(MeshParams mp, Til
                                                               const int* __restrict__ rec_off,
                                                               const float4* __restrict__ rec_u,
                                                               const float* __restrict__ qf,
-                                                              const FxScale* __restrict__ fs, long bstride,
+                                                              const FxScale* __restrict__ fs,
                                                               int* __restrict__ blocks)
 {
     constexpr int WS = fx_wstride(PT);
@@ -389,18 +407,20 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx_run(MeshParams mp, Til
     const float c0 = (float)(mp.xiterm * mp.h[0] * mp.h[0] / mp.eta[0]);
     const float c1 = (float)(mp.xiterm * mp.h[1] * mp.h[1] / mp.eta[1]);
     const float c2 = (float)(mp.xiterm * mp.h[2] * mp.h[2] / mp.eta[2]);
+    using Zc = FxZc<PT, TX, TY, TZ>;
+    static_assert(Zc::ok, "tile shape without a z-carried halo");
+    constexpr int rows = EX * EY;
+    int* hblocks = blocks + (long)nt * Zc::interior;  // one halo block per run, after the tile blocks
     int t = fx_run_tile(blockIdx.x, 0, per, nt);
     unsigned kb = tstart[t], ke = tstart[t + 1];
     FxPrefetch cur = fx_prefetch(kb, ke, npf, rec_off, sorted, rec_u);
     float qv = (j < (int)min(ke - kb, (unsigned)npf)) ? qf[cur.si] : 0.0f;
+    for (int q = j; q < esz / 4; q += kFxThreads) reinterpret_cast<uint4*>(ext)[q] = make_uint4(0, 0, 0, 0);
     for (int k = 0; k < per; ++k) {
         const bool more = k + 1 < per;
         const int tn = more ? fx_run_tile(blockIdx.x, k + 1, per, nt) : t;
         const unsigned kbn = more ? tstart[tn] : 0u, ken = more ? tstart[tn + 1] : 0u;
         const FxPrefetch nxt = fx_prefetch(kbn, ken, npf, rec_off, sorted, rec_u);
-        for (int q = j; q < esz / 4; q += kFxThreads) reinterpret_cast<uint4*>(ext)[q] = make_uint4(0, 0, 0, 0);
-        if constexpr (esz % 4 != 0)
-            for (int q = (esz / 4) * 4 + j; q < esz; q += kFxThreads) ext[q] = 0u;
         const unsigned n = ke - kb;
         int r = 0;
         for (unsigned r0 = 0; r0 < n; r0 += ch, ++r) {
@@ -449,12 +469,31 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx_run(MeshParams mp, Til
         // the next tile's charges (its bead indices have arrived), in flight over the write-out
         const float qvn = (more && j < (int)min(ken - kbn, (unsigned)npf)) ? qf[nxt.si] : 0.0f;
         __syncthreads();
-        int* blk = blocks + (long)t * bstride;
-        for (int q = j; q < esz / 4; q += kFxThreads)
-            reinterpret_cast<uint4*>(blk)[q] = reinterpret_cast<const uint4*>(ext)[q];
-        if constexpr (esz % 4 != 0)
-            for (int q = (esz / 4) * 4 + j; q < esz; q += kFxThreads) blk[q] = (int)ext[q];
+        // the tile's TZ z-planes (its carried-in halo already added), rows of TZ ints
+        int* blk = blocks + (long)t * Zc::interior;
+        for (int q = j; q < rows * (TZ / 4); q += kFxThreads) {
+            const int row = q / (TZ / 4), c4 = q - row * (TZ / 4);
+            reinterpret_cast<uint4*>(blk + (long)row * TZ)[c4] = reinterpret_cast<const uint4*>(ext + row * ys)[c4];
+        }
+        if (!more) {
+            // the run's last halo, for the first tile of the next run along z
+            int* hb = hblocks + (long)(t / per) * Zc::halo;
+            for (int q = j; q < rows * (Zc::HS / 4); q += kFxThreads) {
+                const int row = q / (Zc::HS / 4), c4 = q - row * (Zc::HS / 4);
+                reinterpret_cast<uint4*>(hb + (long)row * Zc::HS)[c4] =
+                    reinterpret_cast<const uint4*>(ext + row * ys + TZ)[c4];
+            }
+            break;
+        }
         __syncthreads();
+        // carry: each row's halo (and the zero padding after it) to the front, the rest cleared
+        for (int row = j; row < rows; row += kFxThreads) {
+            uint4* r4 = reinterpret_cast<uint4*>(ext + row * ys);
+#pragma unroll
+            for (int c4 = 0; c4 < Zc::HS / 4; ++c4) r4[c4] = r4[TZ / 4 + c4];
+#pragma unroll
+            for (int c4 = Zc::HS / 4; c4 < ys / 4; ++c4) r4[c4] = make_uint4(0, 0, 0, 0);
+        }
         t = tn;
         kb = kbn;
         ke = ken;
@@ -574,6 +613,56 @@ __global__ __launch_bounds__(256) void k_fx_gather_batch(const FxBox* __restrict
     if (skip && skip[d.box]) return;
     gather_fx_nodes<V>(d.mp, d.tg, d.bstride, d.blocks, d.fs, d.grid,
                        (long)(blockIdx.x - d.gblk0) * blockDim.x + threadIdx.x);
+}
+
+// The mesh from k_spread_fx_run's z-carried blocks: a node sums its own tile's block and,
+// within P - 1 nodes of the tile's low x and y faces, the previous tiles' blocks along x and
+// y (at most 4 blocks; a tile's z halo inside a run was added in LDS).  At the first tile of
+// a run (tz a multiple of per), within P - 1 nodes of the low z face, the previous run's halo
+// blocks as well (4 more at most).  A thread takes 4 consecutive z nodes with 16-byte loads;
+// the halo rows' last ints past P - 1 hold zeros.  Integer sums, so the mesh bits are those
+// of k_fx_gather over the per-tile blocks.
+template <int PT, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_fx_gather_zc(MeshParams mp, TileGeom tg, int per_log2,
+                                                      const int* __restrict__ blocks,
+                                                      const FxScale* __restrict__ fs, float* __restrict__ mesh)
+{
+    using Zc = FxZc<PT, TX, TY, TZ>;
+    constexpr int EY = TY + PT - 1;
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nq = mp.Nz >> 2;
+    const long rowid = gid / nq;
+    if (rowid >= (long)mp.Nx * mp.Ny) return;
+    const int gz = (int)(gid - rowid * nq) * 4;
+    const int gy = (int)(rowid % mp.Ny), gx = (int)(rowid / mp.Ny);
+    const int tx = gx / TX, lx = gx - tx * TX, ty = gy / TY, ly = gy - ty * TY, tz = gz / TZ, lz = gz - tz * TZ;
+    const int px = tx == 0 ? tg.nt[0] - 1 : tx - 1, py = ty == 0 ? tg.nt[1] - 1 : ty - 1;
+    const int pz = tz == 0 ? tg.nt[2] - 1 : tz - 1;
+    const bool bx = lx < PT - 1, by = ly < PT - 1;
+    const bool bz = ((tz & ((1 << per_log2) - 1)) == 0) && lz < Zc::HS;
+    const int* hblocks = blocks + (long)tg.ntiles * Zc::interior;
+    int4 v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int dx = c >> 2, dy = (c >> 1) & 1, dz = c & 1;
+        const bool on = (!dx || bx) && (!dy || by) && (!dz || bz);
+        const long u = ((long)(dx ? px : tx) * tg.nt[1] + (dy ? py : ty)) * tg.nt[2] + (dz ? pz : tz);
+        const long row = (long)(lx + dx * TX) * EY + (ly + dy * TY);
+        const int* src = dz ? hblocks + (u >> per_log2) * Zc::halo + row * Zc::HS + lz
+                            : blocks + u * Zc::interior + row * TZ + lz;
+        v[c] = on ? *reinterpret_cast<const int4*>(src) : make_int4(0, 0, 0, 0);
+    }
+    int4 a = v[0];
+#pragma unroll
+    for (int c = 1; c < 8; ++c) {
+        a.x += v[c].x;
+        a.y += v[c].y;
+        a.z += v[c].z;
+        a.w += v[c].w;
+    }
+    const float inv = fs->inv;
+    *reinterpret_cast<float4*>(mesh + rowid * mp.Nz + gz) =
+        make_float4((float)a.x * inv, (float)a.y * inv, (float)a.z * inv, (float)a.w * inv);
 }
 
 // FP32-bulk potential contraction (cppmodule/ConductorRigid.cu:260-360) on the same
@@ -806,28 +895,41 @@ void allow_lds_fx(const void* fn, size_t bytes)
 }
 
 // Tiles per persistent workgroup: the largest of 8, 4, 2 that divides the tiles into XCD
-// runs and still gives 512 workgroups (two per CU of an MI355X), else 1.
+// runs and each column of tiles into whole runs along z, and still gives 512 workgroups (two
+// per CU of an MI355X), else 1.
 constexpr int kFxRunMaxPer = 8;
-inline int fx_run_per(int nt)
+inline int fx_run_per(const TileGeom& tg)
 {
     for (int per = kFxRunMaxPer; per > 1; per /= 2)
-        if (nt % (8 * per) == 0 && nt / per >= 512) return per;
+        if (tg.ntiles % (8 * per) == 0 && tg.nt[2] % per == 0 && tg.ntiles / per >= 512) return per;
     return 1;
 }
 
+// Compile-time tile shapes (ONES = false) spread with k_spread_fx_run and gather with
+// k_fx_gather_zc here, and set *gathered; other shapes leave the per-tile blocks to the
+// caller's k_fx_gather.
 template <int PT, int TX, int TY, int TZ, bool ONES>
 hipError_t launch_fx_kernel(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                            const float* qf, FxScale* fs, long bstride, int* blocks, hipStream_t st)
+                            const float* qf, FxScale* fs, long bstride, int* blocks, float* mesh, bool* gathered,
+                            hipStream_t st)
 {
     if constexpr (!ONES && TX > 0) {
-        const int per = fx_run_per(tg.ntiles);
+        const int per = fx_run_per(tg);
         auto fr = k_spread_fx_run<PT, TX, TY, TZ>;
         allow_lds_fx(reinterpret_cast<const void*>(fr), pl.lds_bytes);
         hipLaunchKernelGGL(fr, dim3(tg.ntiles / per), dim3(kFxThreads), pl.lds_bytes, st, mp, tg, pl.chunk, per,
                            bv.tile_start, bv.sorted, bv.rec_off, static_cast<const float4*>(bv.rec_win), qf, fs,
-                           bstride, blocks);
+                           blocks);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        const unsigned gb = (unsigned)fx_gather_blocks(mp, true);
+        hipLaunchKernelGGL((k_fx_gather_zc<PT, TX, TY, TZ>), dim3(gb), dim3(256), 0, st, mp, tg, __builtin_ctz(per),
+                           blocks, fs, mesh);
+        *gathered = true;
         return hipGetLastError();
     }
+    (void)mesh;
+    (void)gathered;
     auto fn = k_spread_fx<PT, TX, TY, TZ, ONES>;
     allow_lds_fx(reinterpret_cast<const void*>(fn), pl.lds_bytes);
     hipLaunchKernelGGL(fn, dim3(tg.ntiles), dim3(kFxThreads), pl.lds_bytes, st, mp, tg, pl.chunk, bv.tile_start,
@@ -837,12 +939,13 @@ hipError_t launch_fx_kernel(const MeshParams& mp, const TileGeom& tg, const FxPl
 
 template <int PT, bool ONES>
 hipError_t dispatch_shape(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                          const float* qf, FxScale* fs, long bstride, int* blocks, hipStream_t st)
+                          const float* qf, FxScale* fs, long bstride, int* blocks, float* mesh, bool* gathered,
+                          hipStream_t st)
 {
     const int a = tg.T[0], b = tg.T[1], c = tg.T[2];
 #define CR_FX_SHAPE(X, Y, Z) \
     if (a == X && b == Y && c == Z)  \
-    return launch_fx_kernel<PT, X, Y, Z, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st)
+    return launch_fx_kernel<PT, X, Y, Z, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st)
     if constexpr (PT == 8) {
         CR_FX_SHAPE(16, 16, 16);
         CR_FX_SHAPE(16, 8, 16);
@@ -856,21 +959,21 @@ hipError_t dispatch_shape(const MeshParams& mp, const TileGeom& tg, const FxPlan
         CR_FX_SHAPE(8, 8, 8);
     }
 #undef CR_FX_SHAPE
-    return launch_fx_kernel<PT, 0, 0, 0, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
+    return launch_fx_kernel<PT, 0, 0, 0, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
 }
 
 template <bool ONES>
 hipError_t dispatch_p(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv, const float* qf,
-                      FxScale* fs, long bstride, int* blocks, hipStream_t st)
+                      FxScale* fs, long bstride, int* blocks, float* mesh, bool* gathered, hipStream_t st)
 {
     switch (mp.P) {
-        case 4: return dispatch_shape<4, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
-        case 5: return dispatch_shape<5, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
-        case 6: return dispatch_shape<6, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
-        case 7: return dispatch_shape<7, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
-        case 8: return dispatch_shape<8, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
-        case 9: return dispatch_shape<9, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
-        case 10: return dispatch_shape<10, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
+        case 4: return dispatch_shape<4, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
+        case 5: return dispatch_shape<5, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
+        case 6: return dispatch_shape<6, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
+        case 7: return dispatch_shape<7, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
+        case 8: return dispatch_shape<8, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
+        case 9: return dispatch_shape<9, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
+        case 10: return dispatch_shape<10, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1025,8 +1128,9 @@ hipError_t launch_spread_fx(const MeshParams& mp, const TileGeom& tg, const FxPl
     for (int d = 0; d < 3; ++d)
         if (tg.T[d] < mp.P - 1) return hipErrorInvalidValue;  // at most two blocks per axis cover a node
     const long bstride = fx_block_ints(mp, tg);
-    hipError_t e = dispatch_p<false>(mp, tg, pl, bv, qf, fs, bstride, blocks, st);
-    if (e != hipSuccess) return e;
+    bool gathered = false;
+    hipError_t e = dispatch_p<false>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, &gathered, st);
+    if (e != hipSuccess || gathered) return e;
     const bool v4 = fx_gather_v4(mp, tg);
     const unsigned gb = (unsigned)fx_gather_blocks(mp, v4);
     if (v4)
@@ -1042,7 +1146,8 @@ hipError_t launch_spread_fx_bound(const MeshParams& mp, const TileGeom& tg, cons
     if (!pl.valid) return hipErrorInvalidValue;
     hipError_t e = launch_fill<unsigned>(&fs->wbits, 0u, 1, st);
     if (e != hipSuccess) return e;
-    return dispatch_p<true>(mp, tg, pl, bv, nullptr, fs, 0, nullptr, st);
+    bool gathered = false;
+    return dispatch_p<true>(mp, tg, pl, bv, nullptr, fs, 0, nullptr, nullptr, &gathered, st);
 }
 
 hipError_t launch_charges_fx_batch(const FxBox* boxes, int nbox, int max_n, const int* skip, hipStream_t st)
