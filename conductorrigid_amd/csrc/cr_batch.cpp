@@ -118,16 +118,17 @@ int group_recip(MeshGroup& g, const std::vector<cr_context*>& ctx, hipStream_t s
     const int Nx = g.N[0], Ny = g.N[1];
     T* real = g.real.as<T>();
     T* cp = g.cplx.as<T>();
-    CR_HIP(launch_fft_z_r2c<T>(pl[2], K * Nx, Ny, real, cp, st));
-    CR_HIP(launch_fft_yx<T>(pl[1], 1, K * Nx, Ny, g.Zc, cp, -1, st));
+    CR_HIP(launch_fft_z_r2c<T>(pl[2], K * Nx, Ny, real, cp, g.Zc, st));
+    CR_HIP(launch_fft_yx<T>(pl[1], 1, K * Nx, Ny, g.Zc, g.Zc, cp, -1, st));
     if (fftr_radix(Nx)) {
-        CR_HIP(launch_fftr_x_scale_batch<T>(pl[0], K, Ny, g.Zc, cp, g.sct.as<ScaleTables>(), st));
+        CR_HIP(launch_fftr_x_scale_batch<T>(pl[0], K, Ny, g.Zc, g.Zc, cp, g.sct.as<ScaleTables>(), st));
     } else {
         for (int k = 0; k < K; ++k)
-            CR_HIP(launch_fft_x_scale<T>(pl[0], Nx, Ny, g.Zc, cp + 2 * (long)k * g.Gh, ctx[g.boxes[k]]->sc, st));
+            CR_HIP(launch_fft_x_scale<T>(pl[0], Nx, Ny, g.Zc, g.Zc, cp + 2 * (long)k * g.Gh, ctx[g.boxes[k]]->sc,
+                                         st));
     }
-    CR_HIP(launch_fft_yx<T>(pl[1], 1, K * Nx, Ny, g.Zc, cp, +1, st));
-    CR_HIP(launch_fft_z_c2r<T>(pl[2], K * Nx, Ny, cp, real, st));
+    CR_HIP(launch_fft_yx<T>(pl[1], 1, K * Nx, Ny, g.Zc, g.Zc, cp, +1, st));
+    CR_HIP(launch_fft_z_c2r<T>(pl[2], K * Nx, Ny, cp, g.Zc, real, st));
     return CR_OK;
 }
 

@@ -209,6 +209,16 @@ const AxisPlan* plans(const cr_context* c)
         return c->plan_f;
 }
 
+// Row pitch of the half spectrum (complex elements): with the register FFT on every axis,
+// Nz/2+1 rounded up to 16, so that every row starts on a 128-byte line and the strided
+// passes read and write whole lines; the generic passes keep Nz/2+1.
+static int half_spectrum_pitch(const MeshParams& mp)
+{
+    const int nh = mp.Nz / 2 + 1;
+    if (!fftr_radix(mp.Nx) || !fftr_radix(mp.Ny) || !fftr_radix(mp.Nz)) return nh;
+    return (nh + 15) & ~15;
+}
+
 // grid <- Re IFFT(S FFT(grid)) on the context mesh (cppmodule/ConductorRigid.cu:927-933).
 template <typename T>
 int recip_impl(cr_context* c, hipStream_t st)
@@ -217,11 +227,11 @@ int recip_impl(cr_context* c, hipStream_t st)
     const int Nx = c->mp.Nx, Ny = c->mp.Ny;
     T* real = c->grid_real.as<T>();
     T* cp = c->grid_cplx.as<T>();
-    CR_HIP(launch_fft_z_r2c<T>(pl[2], Nx, Ny, real, cp, st));
-    CR_HIP(launch_fft_yx<T>(pl[1], 1, Nx, Ny, c->Nzh, cp, -1, st));
-    CR_HIP(launch_fft_x_scale<T>(pl[0], Nx, Ny, c->Nzh, cp, c->sc, st));
-    CR_HIP(launch_fft_yx<T>(pl[1], 1, Nx, Ny, c->Nzh, cp, +1, st));
-    CR_HIP(launch_fft_z_c2r<T>(pl[2], Nx, Ny, cp, real, st));
+    CR_HIP(launch_fft_z_r2c<T>(pl[2], Nx, Ny, real, cp, c->Nzp, st));
+    CR_HIP(launch_fft_yx<T>(pl[1], 1, Nx, Ny, c->Nzh, c->Nzp, cp, -1, st));
+    CR_HIP(launch_fft_x_scale<T>(pl[0], Nx, Ny, c->Nzh, c->Nzp, cp, c->sc, st));
+    CR_HIP(launch_fft_yx<T>(pl[1], 1, Nx, Ny, c->Nzh, c->Nzp, cp, +1, st));
+    CR_HIP(launch_fft_z_c2r<T>(pl[2], Nx, Ny, cp, c->Nzp, real, st));
     return CR_OK;
 }
 
@@ -548,6 +558,12 @@ int block_alloc(cr_context* c, const SysView& v)
     CR_HIP(c->uvec.ensure(sizeof(float) * b1 * nmax));
     CR_HIP(c->inv_s.ensure(sizeof(double) * b1));
     CR_HIP(c->block_scan.ensure(sizeof(unsigned long long) * ((b1 + 1) / 1024 + 4)));
+    const size_t nbk = (size_t)block_order_buckets(std::max(0, v.N));
+    CR_HIP(c->body_order.ensure(sizeof(unsigned) * b1));
+    CR_HIP(c->order_key.ensure(sizeof(uint2) * b1));
+    CR_HIP(c->order_count.ensure(sizeof(unsigned) * nbk));
+    CR_HIP(c->order_start.ensure(sizeof(unsigned) * nbk));
+    CR_HIP(c->order_scan.ensure(sizeof(unsigned long long) * (nbk / 1024 + 4)));
     return CR_OK;
 }
 
@@ -559,6 +575,11 @@ int block_setup(cr_context* c, const SysView& v, hipStream_t st, bool invert)
                               c->body_beads.as<unsigned>(),
                               c->block_scan.as<unsigned long long>(), c->ginv.as<float>(), c->uvec.as<float>(),
                               c->inv_s.as<double>(), st, invert));
+    if (invert)
+        CR_HIP(launch_block_order(v.N_isolated, v.N, c->body_start.as<unsigned>(), c->body_beads.as<unsigned>(),
+                                  c->order_count.as<unsigned>(), c->order_start.as<unsigned>(),
+                                  c->order_scan.as<unsigned long long>(), c->order_key.as<uint2>(),
+                                  c->body_order.as<unsigned>(), st));
     return CR_OK;
 }
 
@@ -567,7 +588,8 @@ int apply_precond(cr_context* c, const SysView& v, const double* in, double* out
 {
     if (block_precond(c)) {
         CR_HIP(launch_block_apply(v, c->block_nmax, c->body_start.as<unsigned>(), c->body_beads.as<unsigned>(),
-                                  c->ginv.as<float>(), c->uvec.as<float>(), c->inv_s.as<double>(), in, out, st, xq));
+                                  c->ginv.as<float>(), c->uvec.as<float>(), c->inv_s.as<double>(), in, out, st, xq,
+                                  c->body_order.as<unsigned>()));
     } else {
         CR_HIP(launch_precondition(v, in, out, st));
     }
@@ -1153,8 +1175,9 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
 
     // FFT plans and k-space factor tables (ConductorRigid.cc:245-274)
     c->Nzh = mp.Nz / 2 + 1;
+    c->Nzp = half_spectrum_pitch(mp);
     c->G = (long)mp.Nx * mp.Ny * mp.Nz;
-    c->Gh = (long)mp.Nx * mp.Ny * c->Nzh;
+    c->Gh = (long)mp.Nx * mp.Ny * c->Nzp;
     const int Nd[3] = {mp.Nx, mp.Ny, mp.Nz};
     for (int d = 0; d < 3; ++d) {
         std::vector<int> rad = radices(Nd[d]);
@@ -1476,16 +1499,16 @@ int cr_fft_c2c(cr_context* c, const double* in, double* out, int sign, void* str
         float* t = tmp.as<float>();
         CR_HIP((launch_convert<double, float>(in, t, n2, st)));
         CR_HIP(launch_fft_z_c2c<float>(c->plan_f[2], Nx, Ny, t, sign, st));
-        CR_HIP(launch_fft_yx<float>(c->plan_f[1], 1, Nx, Ny, Nz, t, sign, st));
-        CR_HIP(launch_fft_yx<float>(c->plan_f[0], 0, Nx, Ny, Nz, t, sign, st));
+        CR_HIP(launch_fft_yx<float>(c->plan_f[1], 1, Nx, Ny, Nz, Nz, t, sign, st));
+        CR_HIP(launch_fft_yx<float>(c->plan_f[0], 0, Nx, Ny, Nz, Nz, t, sign, st));
         CR_HIP((launch_convert<float, double>(t, out, n2, st)));
     } else {
         CR_HIP(tmp.ensure(sizeof(double) * (size_t)n2));
         double* t = tmp.as<double>();
         CR_HIP((launch_convert<double, double>(in, t, n2, st)));
         CR_HIP(launch_fft_z_c2c<double>(c->plan_d[2], Nx, Ny, t, sign, st));
-        CR_HIP(launch_fft_yx<double>(c->plan_d[1], 1, Nx, Ny, Nz, t, sign, st));
-        CR_HIP(launch_fft_yx<double>(c->plan_d[0], 0, Nx, Ny, Nz, t, sign, st));
+        CR_HIP(launch_fft_yx<double>(c->plan_d[1], 1, Nx, Ny, Nz, Nz, t, sign, st));
+        CR_HIP(launch_fft_yx<double>(c->plan_d[0], 0, Nx, Ny, Nz, Nz, t, sign, st));
         CR_HIP((launch_convert<double, double>(t, out, n2, st)));
     }
     CR_HIP(hipStreamSynchronize(st));
@@ -1581,15 +1604,15 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
             else
                 CR_RET(spread_only<T>(c, v, x, st));
             CR_HIP(hipEventRecord(ev[2], st));
-            CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st));
+            CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, c->Nzp, st));
             CR_HIP(hipEventRecord(ev[3], st));
-            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, -1, st));
+            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, c->Nzp, cp, -1, st));
             CR_HIP(hipEventRecord(ev[4], st));
-            CR_HIP(launch_fft_x_scale<T>(pl[0], c->mp.Nx, c->mp.Ny, c->Nzh, cp, c->sc, st));
+            CR_HIP(launch_fft_x_scale<T>(pl[0], c->mp.Nx, c->mp.Ny, c->Nzh, c->Nzp, cp, c->sc, st));
             CR_HIP(hipEventRecord(ev[5], st));
-            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, +1, st));
+            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, c->Nzp, cp, +1, st));
             CR_HIP(hipEventRecord(ev[6], st));
-            CR_HIP(launch_fft_z_c2r<T>(pl[2], c->mp.Nx, c->mp.Ny, cp, real, st));
+            CR_HIP(launch_fft_z_c2r<T>(pl[2], c->mp.Nx, c->mp.Ny, cp, c->Nzp, real, st));
             CR_HIP(hipEventRecord(ev[7], st));
             CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
             CR_HIP(hipEventRecord(ev[8], st));
@@ -1603,15 +1626,15 @@ int cr_profile_matvec(cr_context* c, const cr_system* s, const double* x, int re
             CR_HIP(hipEventRecord(ev[1], st));
             CR_RET(spread_only<T>(c, v, x, st));
             CR_HIP(hipEventRecord(ev[2], st));
-            CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, st));
+            CR_HIP(launch_fft_z_r2c<T>(pl[2], c->mp.Nx, c->mp.Ny, real, cp, c->Nzp, st));
             CR_HIP(hipEventRecord(ev[3], st));
-            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, -1, st));
+            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, c->Nzp, cp, -1, st));
             CR_HIP(hipEventRecord(ev[4], st));
-            CR_HIP(launch_fft_x_scale<T>(pl[0], c->mp.Nx, c->mp.Ny, c->Nzh, cp, c->sc, st));
+            CR_HIP(launch_fft_x_scale<T>(pl[0], c->mp.Nx, c->mp.Ny, c->Nzh, c->Nzp, cp, c->sc, st));
             CR_HIP(hipEventRecord(ev[5], st));
-            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, cp, +1, st));
+            CR_HIP(launch_fft_yx<T>(pl[1], 1, c->mp.Nx, c->mp.Ny, c->Nzh, c->Nzp, cp, +1, st));
             CR_HIP(hipEventRecord(ev[6], st));
-            CR_HIP(launch_fft_z_c2r<T>(pl[2], c->mp.Nx, c->mp.Ny, cp, real, st));
+            CR_HIP(launch_fft_z_c2r<T>(pl[2], c->mp.Nx, c->mp.Ny, cp, c->Nzp, real, st));
             CR_HIP(hipEventRecord(ev[7], st));
             CR_RET(contract_impl<T>(c, v, c->phi_recip.as<T>(), st));
             CR_HIP(hipEventRecord(ev[8], st));

@@ -15,7 +15,8 @@ struct cr_context {
     cr_resolved res{};
     MeshParams mp{};
     RealParams rp{};
-    int Nzh = 0;
+    int Nzh = 0;  // half-spectrum columns, Nz/2+1
+    int Nzp = 0;  // half-spectrum row pitch (>= Nzh)
     long G = 0, Gh = 0;
     AxisPlan plan_d[3]{}, plan_f[3]{};
     DevBuf tw_d[3], tw_f[3];
@@ -54,6 +55,8 @@ struct cr_context {
     // own scan scratch: its setup runs beside the binning, which scans into scan_sums)
     int block_nmax = 0;
     DevBuf body_count, body_start, body_beads, ginv, uvec, inv_s, block_scan;
+    // the block apply's processing order of the bodies and its counting-sort scratch
+    DevBuf body_order, order_key, order_count, order_start, order_scan;
     // Independent parts of a force evaluation run beside the caller's stream on these
     // (default priority): the real-space ELL build and the block preconditioner's setup
     // next to the binning, and the real-space force next to the force pass's mesh work,

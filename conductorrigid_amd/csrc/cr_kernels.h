@@ -298,14 +298,17 @@ template <typename T>
 hipError_t launch_kspace_radial(long m2max, double kappa, double invG, T* rad, hipStream_t st);
 
 // ---- k_fft.hip ----
-// Real mesh [Nx][Ny][Nz] <-> half spectrum [Nx][Ny][Nz/2+1] (interleaved complex).
+// Real mesh [Nx][Ny][Nz] <-> half spectrum [Nx][Ny][Nz/2+1] (interleaved complex) with rows
+// zp complex elements apart (zp >= Nz/2+1; the padding is never read or written; the
+// generic mixed-radix passes take zp = Nz/2+1 only).
 template <typename T>
-hipError_t launch_fft_z_r2c(const AxisPlan& pz, int Nx, int Ny, const T* real, T* cplx, hipStream_t st);
+hipError_t launch_fft_z_r2c(const AxisPlan& pz, int Nx, int Ny, const T* real, T* cplx, int zp, hipStream_t st);
 template <typename T>
-hipError_t launch_fft_z_c2r(const AxisPlan& pz, int Nx, int Ny, const T* cplx, T* real, hipStream_t st);
-// C2C along y (axis 1) or x (axis 0) of a complex mesh [Nx][Ny][Zc]; sign -1 / +1.
+hipError_t launch_fft_z_c2r(const AxisPlan& pz, int Nx, int Ny, const T* cplx, int zp, T* real, hipStream_t st);
+// C2C along y (axis 1) or x (axis 0) of the Zc columns of a complex mesh [Nx][Ny][Zp]; sign -1 / +1.
 template <typename T>
-hipError_t launch_fft_yx(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, T* cplx, int sign, hipStream_t st);
+hipError_t launch_fft_yx(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, int Zp, T* cplx, int sign,
+                         hipStream_t st);
 // C2C along z of a full complex mesh [Nx][Ny][Nz].
 template <typename T>
 hipError_t launch_fft_z_c2c(const AxisPlan& pz, int Nx, int Ny, T* cplx, int sign, hipStream_t st);
@@ -322,26 +325,28 @@ struct ScaleTables {
     int n[3];              // mesh size per axis (signed frequency m of index i)
 };
 template <typename T>
-hipError_t launch_fft_x_scale(const AxisPlan& px, int Nx, int Ny, int Zc, T* cplx, const ScaleTables& sc,
+hipError_t launch_fft_x_scale(const AxisPlan& px, int Nx, int Ny, int Zc, int Zp, T* cplx, const ScaleTables& sc,
                               hipStream_t st);
 
 // ---- k_fft_reg.hip: register four-step passes for n = R * R, R in {8, 16, 32} ----
 int fftr_radix(int n);  // n when the register-resident passes take length n, else 0
 template <typename T>
-hipError_t launch_fftr_z_r2c(const AxisPlan& pz, long nrows, const T* real, T* cplx, hipStream_t st);
+hipError_t launch_fftr_z_r2c(const AxisPlan& pz, long nrows, int zp, const T* real, T* cplx, hipStream_t st);
 template <typename T>
-hipError_t launch_fftr_z_c2r(const AxisPlan& pz, long nrows, const T* cplx, T* real, hipStream_t st);
+hipError_t launch_fftr_z_c2r(const AxisPlan& pz, long nrows, int zp, const T* cplx, T* real, hipStream_t st);
 template <typename T>
 hipError_t launch_fftr_z_c2c(const AxisPlan& pz, long nrows, T* data, int sign, hipStream_t st);
 template <typename T>
-hipError_t launch_fftr_strided(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, T* data, int sign, hipStream_t st);
+hipError_t launch_fftr_strided(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, int Zp, T* data, int sign,
+                               hipStream_t st);
 template <typename T>
-hipError_t launch_fftr_x_scale(const AxisPlan& px, int Ny, int Zc, T* data, const ScaleTables& sc, hipStream_t st);
-// x pass + scale for nbox meshes stored back to back ([box][Nx][Ny][Zc], Nx = px.n), each
+hipError_t launch_fftr_x_scale(const AxisPlan& px, int Ny, int Zc, int Zp, T* data, const ScaleTables& sc,
+                               hipStream_t st);
+// x pass + scale for nbox meshes stored back to back ([box][Nx][Ny][Zp], Nx = px.n), each
 // with its own scale tables sct[box] (device array); register-path lengths only.
 template <typename T>
-hipError_t launch_fftr_x_scale_batch(const AxisPlan& px, int nbox, int Ny, int Zc, T* data, const ScaleTables* sct,
-                                     hipStream_t st);
+hipError_t launch_fftr_x_scale_batch(const AxisPlan& px, int nbox, int Ny, int Zc, int Zp, T* data,
+                                     const ScaleTables* sct, hipStream_t st);
 
 // ---- k_real.hip ----
 template <typename T>
@@ -413,9 +418,17 @@ hipError_t launch_block_apply_batch(const BlockBox* boxes, int nbox, int total_b
 // w = D y with the near-field block preconditioner.  xq != nullptr: body t's constraint row
 // is taken as the sum of xq over its beads (the operator's charge row for input xq) instead
 // of y[N + t], which is then never read.
+// order: the bodies in processing order (launch_block_order), or null for ascending index.
 hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_start, const unsigned* body_beads,
                               const float* ginv, const float* uvec, const double* inv_s, const double* y, double* w,
-                              hipStream_t st, const double* xq = nullptr);
+                              hipStream_t st, const double* xq = nullptr, const unsigned* order = nullptr);
+// The B bodies in ascending order of their smallest bead index (buckets of 32 indices, any
+// order within a bucket), for the block apply.  Scratch: bucket_count and bucket_start of
+// block_order_buckets(N) entries, scan_sums of block_order_buckets(N) / 1024 + 4, key of B.
+long block_order_buckets(int N);
+hipError_t launch_block_order(int B, int N, const unsigned* body_start, const unsigned* body_beads,
+                              unsigned* bucket_count, unsigned* bucket_start, unsigned long long* scan_sums,
+                              uint2* key, unsigned* order, hipStream_t st);
 
 // ---- k_mlp.hip ----
 // Learned warm start: per-bead MLP trunk kMlpTrunk -> kMlpHidden -> kMlpHidden -> kMlpOut

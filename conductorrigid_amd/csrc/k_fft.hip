@@ -373,12 +373,12 @@ bool fits(int n)
 }  // namespace
 
 template <typename T>
-hipError_t launch_fft_z_r2c(const AxisPlan& pz, int Nx, int Ny, const T* real, T* cplx, hipStream_t st)
+hipError_t launch_fft_z_r2c(const AxisPlan& pz, int Nx, int Ny, const T* real, T* cplx, int zp, hipStream_t st)
 {
     using T2 = typename Cplx<T>::type;
     const long nrows = (long)Nx * Ny;
-    if (fftr_radix(pz.n)) return launch_fftr_z_r2c<T>(pz, nrows, real, cplx, st);
-    if (!fits<T>(pz.n)) return hipErrorInvalidValue;
+    if (fftr_radix(pz.n)) return launch_fftr_z_r2c<T>(pz, nrows, zp, real, cplx, st);
+    if (!fits<T>(pz.n) || zp != pz.n / 2 + 1) return hipErrorInvalidValue;
     const int cols = pick_cols<T>(pz.n);
     hipLaunchKernelGGL(k_fft_z_r2c<T>, dim3((unsigned)((nrows + cols - 1) / cols)), dim3(kThreads),
                        lds_bytes<T>(pz.n, cols), st, pz, nrows, cols, real, (T2*)cplx);
@@ -386,12 +386,12 @@ hipError_t launch_fft_z_r2c(const AxisPlan& pz, int Nx, int Ny, const T* real, T
 }
 
 template <typename T>
-hipError_t launch_fft_z_c2r(const AxisPlan& pz, int Nx, int Ny, const T* cplx, T* real, hipStream_t st)
+hipError_t launch_fft_z_c2r(const AxisPlan& pz, int Nx, int Ny, const T* cplx, int zp, T* real, hipStream_t st)
 {
     using T2 = typename Cplx<T>::type;
     const long nrows = (long)Nx * Ny;
-    if (fftr_radix(pz.n)) return launch_fftr_z_c2r<T>(pz, nrows, cplx, real, st);
-    if (!fits<T>(pz.n)) return hipErrorInvalidValue;
+    if (fftr_radix(pz.n)) return launch_fftr_z_c2r<T>(pz, nrows, zp, cplx, real, st);
+    if (!fits<T>(pz.n) || zp != pz.n / 2 + 1) return hipErrorInvalidValue;
     const int cols = pick_cols<T>(pz.n);
     hipLaunchKernelGGL(k_fft_z_c2r<T>, dim3((unsigned)((nrows + cols - 1) / cols)), dim3(kThreads),
                        lds_bytes<T>(pz.n, cols), st, pz, nrows, cols, (const T2*)cplx, real);
@@ -412,11 +412,12 @@ hipError_t launch_fft_z_c2c(const AxisPlan& pz, int Nx, int Ny, T* cplx, int sig
 }
 
 template <typename T>
-hipError_t launch_fft_yx(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, T* cplx, int sign, hipStream_t st)
+hipError_t launch_fft_yx(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, int Zp, T* cplx, int sign,
+                         hipStream_t st)
 {
     using T2 = typename Cplx<T>::type;
-    if (fftr_radix(pa.n)) return launch_fftr_strided<T>(pa, axis, Nx, Ny, Zc, cplx, sign, st);
-    if (!fits<T>(pa.n)) return hipErrorInvalidValue;
+    if (fftr_radix(pa.n)) return launch_fftr_strided<T>(pa, axis, Nx, Ny, Zc, Zp, cplx, sign, st);
+    if (!fits<T>(pa.n) || Zp != Zc) return hipErrorInvalidValue;
     const int cols = pick_cols<T>(pa.n);
     const int lines = (axis == 1) ? Nx : Ny;
     const int zblocks = (Zc + cols - 1) / cols;
@@ -426,13 +427,13 @@ hipError_t launch_fft_yx(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, T
 }
 
 template <typename T>
-hipError_t launch_fft_x_scale(const AxisPlan& px, int Nx, int Ny, int Zc, T* cplx, const ScaleTables& sc,
+hipError_t launch_fft_x_scale(const AxisPlan& px, int Nx, int Ny, int Zc, int Zp, T* cplx, const ScaleTables& sc,
                               hipStream_t st)
 {
     using T2 = typename Cplx<T>::type;
     (void)Nx;
-    if (fftr_radix(px.n)) return launch_fftr_x_scale<T>(px, Ny, Zc, cplx, sc, st);
-    if (!fits<T>(px.n)) return hipErrorInvalidValue;
+    if (fftr_radix(px.n)) return launch_fftr_x_scale<T>(px, Ny, Zc, Zp, cplx, sc, st);
+    if (!fits<T>(px.n) || Zp != Zc) return hipErrorInvalidValue;
     const int cols = pick_cols<T>(px.n);
     const int zblocks = (Zc + cols - 1) / cols;
     hipLaunchKernelGGL(k_fft_x_scale<T>, dim3((unsigned)(Ny * zblocks)), dim3(kThreads), lds_bytes<T>(px.n, cols), st,
@@ -441,11 +442,12 @@ hipError_t launch_fft_x_scale(const AxisPlan& px, int Nx, int Ny, int Zc, T* cpl
 }
 
 #define CR_FFT_INST(T)                                                                                        \
-    template hipError_t launch_fft_z_r2c<T>(const AxisPlan&, int, int, const T*, T*, hipStream_t);            \
-    template hipError_t launch_fft_z_c2r<T>(const AxisPlan&, int, int, const T*, T*, hipStream_t);            \
+    template hipError_t launch_fft_z_r2c<T>(const AxisPlan&, int, int, const T*, T*, int, hipStream_t);       \
+    template hipError_t launch_fft_z_c2r<T>(const AxisPlan&, int, int, const T*, int, T*, hipStream_t);       \
     template hipError_t launch_fft_z_c2c<T>(const AxisPlan&, int, int, T*, int, hipStream_t);                 \
-    template hipError_t launch_fft_yx<T>(const AxisPlan&, int, int, int, int, T*, int, hipStream_t);          \
-    template hipError_t launch_fft_x_scale<T>(const AxisPlan&, int, int, int, T*, const ScaleTables&, hipStream_t);
+    template hipError_t launch_fft_yx<T>(const AxisPlan&, int, int, int, int, int, T*, int, hipStream_t);     \
+    template hipError_t launch_fft_x_scale<T>(const AxisPlan&, int, int, int, int, T*, const ScaleTables&,    \
+                                              hipStream_t);
 CR_FFT_INST(float)
 CR_FFT_INST(double)
 

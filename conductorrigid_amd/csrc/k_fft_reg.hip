@@ -193,7 +193,8 @@ __device__ __forceinline__ void four_step_inv_from_X(T2 (&v)[A], T2* buf, const 
 }
 
 template <typename T, int A, int B>
-__global__ __launch_bounds__((B * lanes_per_wg<T, A, B>())) void k_fftr_z_r2c(long nrows, const T* __restrict__ real,
+__global__ __launch_bounds__((B * lanes_per_wg<T, A, B>())) void k_fftr_z_r2c(long nrows, int zp,
+                                                                            const T* __restrict__ real,
                                                                             typename Cx<T>::t* __restrict__ cplx,
                                                                             const typename Cx<T>::t* __restrict__ tw)
 {
@@ -216,12 +217,12 @@ __global__ __launch_bounds__((B * lanes_per_wg<T, A, B>())) void k_fftr_z_r2c(lo
 #pragma unroll
         for (int j = 0; j < B; ++j) {
             const int k = l + B * s + A * j;
-            if (k < nh) cplx[row * nh + k] = v[s * B + j];
+            if (k < nh) cplx[row * zp + k] = v[s * B + j];
         }
 }
 
 template <typename T, int A, int B>
-__global__ __launch_bounds__((B * lanes_per_wg<T, A, B>())) void k_fftr_z_c2r(long nrows,
+__global__ __launch_bounds__((B * lanes_per_wg<T, A, B>())) void k_fftr_z_c2r(long nrows, int zp,
                                                                             const typename Cx<T>::t* __restrict__ cplx,
                                                                             T* __restrict__ real,
                                                                             const typename Cx<T>::t* __restrict__ tw)
@@ -243,9 +244,9 @@ __global__ __launch_bounds__((B * lanes_per_wg<T, A, B>())) void k_fftr_z_c2r(lo
             z.y = 0;
             if (ok) {
                 if (k < nh) {
-                    z = cplx[row * nh + k];
+                    z = cplx[row * zp + k];
                 } else {  // Hermitian extension
-                    z = cplx[row * nh + (n - k)];
+                    z = cplx[row * zp + (n - k)];
                     z.y = -z.y;
                 }
             }
@@ -285,28 +286,30 @@ __global__ __launch_bounds__((B * lanes_per_wg<T, A, B>())) void k_fftr_z_c2c(lo
         for (int j = 0; j < B; ++j) data[row * n + l + B * s + A * j] = v[s * B + j];
 }
 
-// ---- strided pencils (y: axis 1, x: axis 0) of [Nx][Ny][Zc] ---------------------
+// ---- strided pencils (y: axis 1, x: axis 0) of [Nx][Ny][Zc], rows Zp apart ----------
 // A workgroup takes C neighbouring z columns of one line; thread t handles column
 // c = t % C (consecutive lanes read consecutive columns) and pencil position t / C.
-// Transpose buffer layout [q][l][c] keeps both phases free of bank conflicts.
+// Transpose buffer layout [q][l][c] keeps both phases free of bank conflicts.  With the
+// row pitch Zp a multiple of 128 bytes, a workgroup's column group is whole cache lines
+// of every row it reads (an odd pitch such as 129 splits each 256-byte run over 3 lines).
 struct Lines {
     long base, stride;
     int z0, line;
 };
 
 template <int C>
-__device__ __forceinline__ Lines lines(int axis, int Ny, int Zc)
+__device__ __forceinline__ Lines lines(int axis, int Ny, int Zc, int Zp)
 {
     Lines L;
     const int zb = (Zc + C - 1) / C;
     L.line = blockIdx.x / zb;
     L.z0 = (blockIdx.x - L.line * zb) * C;
     if (axis == 1) {
-        L.base = (long)L.line * Ny * Zc + L.z0;
-        L.stride = Zc;
+        L.base = (long)L.line * Ny * Zp + L.z0;
+        L.stride = Zp;
     } else {
-        L.base = (long)L.line * Zc + L.z0;
-        L.stride = (long)Ny * Zc;
+        L.base = (long)L.line * Zp + L.z0;
+        L.stride = (long)Ny * Zp;
     }
     return L;
 }
@@ -354,14 +357,14 @@ __device__ __forceinline__ void four_step_cols_inv_from_X(T2 (&v)[A], T2* lds, c
 }
 
 template <typename T, int A, int B, int SIGN>
-__global__ __launch_bounds__((B * cols_per_wg<T, A, B>())) void k_fftr_strided(int axis, int Ny, int Zc,
+__global__ __launch_bounds__((B * cols_per_wg<T, A, B>())) void k_fftr_strided(int axis, int Ny, int Zc, int Zp,
                                                                               typename Cx<T>::t* __restrict__ data,
                                                                               const typename Cx<T>::t* __restrict__ tw)
 {
     using T2 = typename Cx<T>::t;
     constexpr int C = cols_per_wg<T, A, B>(), m = A / B;
     __shared__ T2 lds[C * A * B];
-    const Lines L = lines<C>(axis, Ny, Zc);
+    const Lines L = lines<C>(axis, Ny, Zc, Zp);
     const int c = threadIdx.x % C, l = threadIdx.x / C;
     const bool ok = L.z0 + c < Zc;
     T2 v[A];
@@ -391,13 +394,13 @@ __device__ __forceinline__ double sin_t<double>(double x) { return sin(x); }
 // Forward x FFT, S(k) = j0(|k|)^2 exp(-sum_d (1-eta_d) k_d^2/4xi^2)/|k|^2/G (S(0) = 0;
 // cppmodule/ConductorRigid.cc:253-268), inverse x FFT, on the half spectrum.
 template <typename T, int A, int B>
-__device__ __forceinline__ void x_scale_pass(int Ny, int Zc, typename Cx<T>::t* __restrict__ data,
+__device__ __forceinline__ void x_scale_pass(int Ny, int Zc, int Zp, typename Cx<T>::t* __restrict__ data,
                                              const typename Cx<T>::t* __restrict__ tw, const ScaleTables& sc)
 {
     using T2 = typename Cx<T>::t;
     constexpr int C = cols_per_wg<T, A, B>(), m = A / B;
     __shared__ T2 lds[C * A * B];
-    const Lines L = lines<C>(0, Ny, Zc);
+    const Lines L = lines<C>(0, Ny, Zc, Zp);
     const int c = threadIdx.x % C, l = threadIdx.x / C;
     const int z = L.z0 + c;
     const bool ok = z < Zc;
@@ -457,42 +460,42 @@ __device__ __forceinline__ void x_scale_pass(int Ny, int Zc, typename Cx<T>::t* 
 }
 
 template <typename T, int A, int B>
-__global__ __launch_bounds__((B * cols_per_wg<T, A, B>())) void k_fftr_x_scale(int Ny, int Zc,
+__global__ __launch_bounds__((B * cols_per_wg<T, A, B>())) void k_fftr_x_scale(int Ny, int Zc, int Zp,
                                                                               typename Cx<T>::t* __restrict__ data,
                                                                               const typename Cx<T>::t* __restrict__ tw,
                                                                               ScaleTables sc)
 {
-    x_scale_pass<T, A, B>(Ny, Zc, data, tw, sc);
+    x_scale_pass<T, A, B>(Ny, Zc, Zp, data, tw, sc);
 }
 
 // Boxes of one mesh stored back to back ([box][Nx][Ny][Zc]): blockIdx.y picks the box
 // and its own scale tables.
 template <typename T, int A, int B>
 __global__ __launch_bounds__((B * cols_per_wg<T, A, B>())) void k_fftr_x_scale_batch(
-    int Ny, int Zc, long box_stride, typename Cx<T>::t* __restrict__ data, const typename Cx<T>::t* __restrict__ tw,
-    const ScaleTables* __restrict__ sct)
+    int Ny, int Zc, int Zp, long box_stride, typename Cx<T>::t* __restrict__ data,
+    const typename Cx<T>::t* __restrict__ tw, const ScaleTables* __restrict__ sct)
 {
-    x_scale_pass<T, A, B>(Ny, Zc, data + blockIdx.y * box_stride, tw, sct[blockIdx.y]);
+    x_scale_pass<T, A, B>(Ny, Zc, Zp, data + blockIdx.y * box_stride, tw, sct[blockIdx.y]);
 }
 
 inline unsigned ublk(long n, int per) { return (unsigned)((n + per - 1) / per); }
 
 template <typename T, int A, int B>
-hipError_t z_r2c(long nrows, const T* real, T* cplx, const void* tw, hipStream_t st)
+hipError_t z_r2c(long nrows, int zp, const T* real, T* cplx, const void* tw, hipStream_t st)
 {
     using T2 = typename Cx<T>::t;
     constexpr int P = lanes_per_wg<T, A, B>();
-    hipLaunchKernelGGL((k_fftr_z_r2c<T, A, B>), dim3(ublk(nrows, P)), dim3(B * P), 0, st, nrows, real, (T2*)cplx,
+    hipLaunchKernelGGL((k_fftr_z_r2c<T, A, B>), dim3(ublk(nrows, P)), dim3(B * P), 0, st, nrows, zp, real, (T2*)cplx,
                        (const T2*)tw);
     return hipGetLastError();
 }
 
 template <typename T, int A, int B>
-hipError_t z_c2r(long nrows, const T* cplx, T* real, const void* tw, hipStream_t st)
+hipError_t z_c2r(long nrows, int zp, const T* cplx, T* real, const void* tw, hipStream_t st)
 {
     using T2 = typename Cx<T>::t;
     constexpr int P = lanes_per_wg<T, A, B>();
-    hipLaunchKernelGGL((k_fftr_z_c2r<T, A, B>), dim3(ublk(nrows, P)), dim3(B * P), 0, st, nrows, (const T2*)cplx,
+    hipLaunchKernelGGL((k_fftr_z_c2r<T, A, B>), dim3(ublk(nrows, P)), dim3(B * P), 0, st, nrows, zp, (const T2*)cplx,
                        real, (const T2*)tw);
     return hipGetLastError();
 }
@@ -512,41 +515,42 @@ hipError_t z_c2c(long nrows, T* data, int sign, const void* tw, hipStream_t st)
 }
 
 template <typename T, int A, int B>
-hipError_t strided(int axis, int Nx, int Ny, int Zc, T* data, int sign, const void* tw, hipStream_t st)
+hipError_t strided(int axis, int Nx, int Ny, int Zc, int Zp, T* data, int sign, const void* tw, hipStream_t st)
 {
     using T2 = typename Cx<T>::t;
     constexpr int C = cols_per_wg<T, A, B>();
     const int lines_n = axis == 1 ? Nx : Ny;
     const unsigned blocks = (unsigned)(lines_n * ((Zc + C - 1) / C));
     if (sign < 0)
-        hipLaunchKernelGGL((k_fftr_strided<T, A, B, -1>), dim3(blocks), dim3(B * C), 0, st, axis, Ny, Zc, (T2*)data,
-                           (const T2*)tw);
+        hipLaunchKernelGGL((k_fftr_strided<T, A, B, -1>), dim3(blocks), dim3(B * C), 0, st, axis, Ny, Zc, Zp,
+                           (T2*)data, (const T2*)tw);
     else
-        hipLaunchKernelGGL((k_fftr_strided<T, A, B, +1>), dim3(blocks), dim3(B * C), 0, st, axis, Ny, Zc, (T2*)data,
-                           (const T2*)tw);
+        hipLaunchKernelGGL((k_fftr_strided<T, A, B, +1>), dim3(blocks), dim3(B * C), 0, st, axis, Ny, Zc, Zp,
+                           (T2*)data, (const T2*)tw);
     return hipGetLastError();
 }
 
 template <typename T, int A, int B>
-hipError_t x_scale(int Ny, int Zc, T* data, const void* tw, const ScaleTables& sc, hipStream_t st)
+hipError_t x_scale(int Ny, int Zc, int Zp, T* data, const void* tw, const ScaleTables& sc, hipStream_t st)
 {
     using T2 = typename Cx<T>::t;
     constexpr int C = cols_per_wg<T, A, B>();
     const unsigned blocks = (unsigned)(Ny * ((Zc + C - 1) / C));
-    hipLaunchKernelGGL((k_fftr_x_scale<T, A, B>), dim3(blocks), dim3(B * C), 0, st, Ny, Zc, (T2*)data,
+    hipLaunchKernelGGL((k_fftr_x_scale<T, A, B>), dim3(blocks), dim3(B * C), 0, st, Ny, Zc, Zp, (T2*)data,
                        (const T2*)tw, sc);
     return hipGetLastError();
 }
 
 template <typename T, int A, int B>
-hipError_t x_scale_batch(int nbox, int Ny, int Zc, T* data, const void* tw, const ScaleTables* sct, hipStream_t st)
+hipError_t x_scale_batch(int nbox, int Ny, int Zc, int Zp, T* data, const void* tw, const ScaleTables* sct,
+                         hipStream_t st)
 {
     using T2 = typename Cx<T>::t;
     constexpr int C = cols_per_wg<T, A, B>();
     const unsigned blocks = (unsigned)(Ny * ((Zc + C - 1) / C));
-    const long stride = (long)(A * B) * Ny * Zc;
+    const long stride = (long)(A * B) * Ny * Zp;
     hipLaunchKernelGGL((k_fftr_x_scale_batch<T, A, B>), dim3(blocks, (unsigned)nbox), dim3(B * C), 0, st, Ny, Zc,
-                       stride, (T2*)data, (const T2*)tw, sct);
+                       Zp, stride, (T2*)data, (const T2*)tw, sct);
     return hipGetLastError();
 }
 
@@ -573,17 +577,17 @@ int fftr_radix(int n)
     }
 
 template <typename T>
-hipError_t launch_fftr_z_r2c(const AxisPlan& pz, long nrows, const T* real, T* cplx, hipStream_t st)
+hipError_t launch_fftr_z_r2c(const AxisPlan& pz, long nrows, int zp, const T* real, T* cplx, hipStream_t st)
 {
-#define CALL(A, B) z_r2c<T, A, B>(nrows, real, cplx, pz.tw, st)
+#define CALL(A, B) z_r2c<T, A, B>(nrows, zp, real, cplx, pz.tw, st)
     CR_FFTR_DISPATCH(pz.n, CALL)
 #undef CALL
 }
 
 template <typename T>
-hipError_t launch_fftr_z_c2r(const AxisPlan& pz, long nrows, const T* cplx, T* real, hipStream_t st)
+hipError_t launch_fftr_z_c2r(const AxisPlan& pz, long nrows, int zp, const T* cplx, T* real, hipStream_t st)
 {
-#define CALL(A, B) z_c2r<T, A, B>(nrows, cplx, real, pz.tw, st)
+#define CALL(A, B) z_c2r<T, A, B>(nrows, zp, cplx, real, pz.tw, st)
     CR_FFTR_DISPATCH(pz.n, CALL)
 #undef CALL
 }
@@ -597,37 +601,40 @@ hipError_t launch_fftr_z_c2c(const AxisPlan& pz, long nrows, T* data, int sign, 
 }
 
 template <typename T>
-hipError_t launch_fftr_strided(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, T* data, int sign, hipStream_t st)
+hipError_t launch_fftr_strided(const AxisPlan& pa, int axis, int Nx, int Ny, int Zc, int Zp, T* data, int sign,
+                               hipStream_t st)
 {
-#define CALL(A, B) strided<T, A, B>(axis, Nx, Ny, Zc, data, sign, pa.tw, st)
+#define CALL(A, B) strided<T, A, B>(axis, Nx, Ny, Zc, Zp, data, sign, pa.tw, st)
     CR_FFTR_DISPATCH(pa.n, CALL)
 #undef CALL
 }
 
 template <typename T>
-hipError_t launch_fftr_x_scale(const AxisPlan& px, int Ny, int Zc, T* data, const ScaleTables& sc, hipStream_t st)
+hipError_t launch_fftr_x_scale(const AxisPlan& px, int Ny, int Zc, int Zp, T* data, const ScaleTables& sc,
+                               hipStream_t st)
 {
-#define CALL(A, B) x_scale<T, A, B>(Ny, Zc, data, px.tw, sc, st)
+#define CALL(A, B) x_scale<T, A, B>(Ny, Zc, Zp, data, px.tw, sc, st)
     CR_FFTR_DISPATCH(px.n, CALL)
 #undef CALL
 }
 
 template <typename T>
-hipError_t launch_fftr_x_scale_batch(const AxisPlan& px, int nbox, int Ny, int Zc, T* data, const ScaleTables* sct,
-                                     hipStream_t st)
+hipError_t launch_fftr_x_scale_batch(const AxisPlan& px, int nbox, int Ny, int Zc, int Zp, T* data,
+                                     const ScaleTables* sct, hipStream_t st)
 {
-#define CALL(A, B) x_scale_batch<T, A, B>(nbox, Ny, Zc, data, px.tw, sct, st)
+#define CALL(A, B) x_scale_batch<T, A, B>(nbox, Ny, Zc, Zp, data, px.tw, sct, st)
     CR_FFTR_DISPATCH(px.n, CALL)
 #undef CALL
 }
 
 #define CR_FFTR_INST(T)                                                                                          \
-    template hipError_t launch_fftr_z_r2c<T>(const AxisPlan&, long, const T*, T*, hipStream_t);                \
-    template hipError_t launch_fftr_z_c2r<T>(const AxisPlan&, long, const T*, T*, hipStream_t);                \
+    template hipError_t launch_fftr_z_r2c<T>(const AxisPlan&, long, int, const T*, T*, hipStream_t);           \
+    template hipError_t launch_fftr_z_c2r<T>(const AxisPlan&, long, int, const T*, T*, hipStream_t);           \
     template hipError_t launch_fftr_z_c2c<T>(const AxisPlan&, long, T*, int, hipStream_t);                     \
-    template hipError_t launch_fftr_strided<T>(const AxisPlan&, int, int, int, int, T*, int, hipStream_t);     \
-    template hipError_t launch_fftr_x_scale<T>(const AxisPlan&, int, int, T*, const ScaleTables&, hipStream_t); \
-    template hipError_t launch_fftr_x_scale_batch<T>(const AxisPlan&, int, int, int, T*, const ScaleTables*,     \
+    template hipError_t launch_fftr_strided<T>(const AxisPlan&, int, int, int, int, int, T*, int, hipStream_t); \
+    template hipError_t launch_fftr_x_scale<T>(const AxisPlan&, int, int, int, T*, const ScaleTables&,          \
+                                               hipStream_t);                                                   \
+    template hipError_t launch_fftr_x_scale_batch<T>(const AxisPlan&, int, int, int, int, T*, const ScaleTables*, \
                                                      hipStream_t);
 CR_FFTR_INST(float)
 CR_FFTR_INST(double)

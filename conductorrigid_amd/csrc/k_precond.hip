@@ -289,20 +289,61 @@ __device__ __forceinline__ void block_apply_body(int t, int N, int nmax, const u
     if (i == 0) w[N + t] = c;
 }
 
+// Processing order of the bodies for k_block_apply: ascending smallest bead index, by a
+// counting sort over buckets of 2^kOrderShift bead indices (the bodies of one bucket in any
+// order; a body without beads goes to the last bucket).  Each body's result is independent
+// of the others, so the order changes no bits.  Bead indices in spatial (cell) order make
+// the bodies of one workgroup share the cache lines of y and w that it gathers and scatters.
+constexpr int kOrderShift = 5;
+
+__global__ void k_body_order_key(int B, int N, const unsigned* __restrict__ start, const unsigned* __restrict__ beads,
+                                 unsigned* __restrict__ bucket_count, uint2* __restrict__ key)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const unsigned lo = start[b], hi = start[b + 1];
+    unsigned m = (unsigned)N;
+    for (unsigned k = lo; k < hi; ++k) m = min(m, beads[k]);
+    const unsigned bk = m >> kOrderShift;
+    key[b] = make_uint2(bk, atomicAdd(bucket_count + bk, 1u));
+}
+
+__global__ void k_body_order_place(int B, const uint2* __restrict__ key, const unsigned* __restrict__ bucket_start,
+                                   unsigned* __restrict__ order)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const uint2 kv = key[b];
+    order[bucket_start[kv.x] + kv.y] = (unsigned)b;
+}
+
+// Workgroup b's place in an XCD-local order: consecutive workgroups go to different XCDs
+// (round robin over 8), so XCD x takes the contiguous range of places that follows those
+// of XCDs 0 .. x-1.
+__device__ __forceinline__ int xcd_local_place(int b, int nwg)
+{
+    const int x = b & 7, q = nwg >> 3, r = nwg & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 // Single system: K = 64 / nmax bodies share a wavefront (lane = body slot * nmax + bead),
-// four wavefronts per workgroup, so small bodies do not leave most lanes idle.
+// four wavefronts per workgroup, so small bodies do not leave most lanes idle.  order:
+// the bodies in processing order (k_body_order_*), taken by XCD-local workgroup places;
+// null for ascending body index.
 template <int R4>
 __global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int K, const unsigned* __restrict__ start,
                                                      const unsigned* __restrict__ beads,
                                                      const float* __restrict__ ginv, const float* __restrict__ uvec,
                                                      const double* __restrict__ inv_s, const double* __restrict__ y,
-                                                     double* __restrict__ w, const double* __restrict__ xq)
+                                                     double* __restrict__ w, const double* __restrict__ xq,
+                                                     const unsigned* __restrict__ order)
 {
     __shared__ double ys[256], red[256], cs[256], qs[256];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int k = lane / nmax, i = lane - k * nmax;
-    const int t = (blockIdx.x * 4 + wave) * K + k;
-    const bool live = k < K && t < B;
+    const int slot = (xcd_local_place(blockIdx.x, gridDim.x) * 4 + wave) * K + k;
+    const bool live = k < K && slot < B;
+    const int t = !live ? 0 : (order ? (int)order[slot] : slot);
     // R4 > 0 (nmax = 4 R4): the row of G^-1 does not depend on y, so its loads go out
     // first and overlap the start -> beads -> y chain
     float4 g4[R4 > 0 ? R4 : 1];
@@ -448,16 +489,33 @@ hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, uns
     return hipGetLastError();
 }
 
+hipError_t launch_block_order(int B, int N, const unsigned* body_start, const unsigned* body_beads,
+                              unsigned* bucket_count, unsigned* bucket_start, unsigned long long* scan_sums,
+                              uint2* key, unsigned* order, hipStream_t st)
+{
+    if (B <= 0) return hipSuccess;
+    const long nbk = block_order_buckets(N);
+    hipLaunchKernelGGL(k_zero_u32, dim3(blocks(nbk, 256)), dim3(256), 0, st, bucket_count, nbk);
+    hipLaunchKernelGGL(k_body_order_key, dim3(blocks(B, 256)), dim3(256), 0, st, B, N, body_start, body_beads,
+                       bucket_count, key);
+    hipError_t e = exclusive_scan_u32(bucket_count, bucket_start, nbk, scan_sums, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_body_order_place, dim3(blocks(B, 256)), dim3(256), 0, st, B, key, bucket_start, order);
+    return hipGetLastError();
+}
+
+long block_order_buckets(int N) { return ((long)N >> kOrderShift) + 1; }
+
 hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_start, const unsigned* body_beads,
                               const float* ginv, const float* uvec, const double* inv_s, const double* y, double* w,
-                              hipStream_t st, const double* xq)
+                              hipStream_t st, const double* xq, const unsigned* order)
 {
     if (s.N_isolated <= 0) return hipSuccess;
     const int K = nmax <= 64 ? 64 / nmax : 1;
     const unsigned nb = (unsigned)((s.N_isolated + 4 * K - 1) / (4 * K));
 #define CR_BLOCK_APPLY(R4)                                                                                     \
     hipLaunchKernelGGL(k_block_apply<R4>, dim3(nb), dim3(256), 0, st, s.N_isolated, s.N, nmax, K, body_start,  \
-                       body_beads, ginv, uvec, inv_s, y, w, xq)
+                       body_beads, ginv, uvec, inv_s, y, w, xq, order)
     switch (nmax) {
         case 8: CR_BLOCK_APPLY(2); break;
         case 12: CR_BLOCK_APPLY(3); break;
