@@ -370,7 +370,10 @@ int charges_f32(cr_context* c, const SysView& v, const double* x, hipStream_t st
         c->fx_bound = true;
     }
     CR_HIP(c->qf.ensure(sizeof(float) * (size_t)std::max(1, v.N)));
-    CR_HIP(launch_charges_fx(x, v.N, c->qf.as<float>(), c->fxs.as<FxScale>(), c->mp.prefac, st));
+    if (x && x == c->qf_for)  // the step tail that produced x left its charges and partial maxima
+        CR_HIP(launch_fx_scale(c->qf_parts, c->fxs.as<FxScale>(), c->mp.prefac, st));
+    else
+        CR_HIP(launch_charges_fx(x, v.N, c->qf.as<float>(), c->fxs.as<FxScale>(), c->mp.prefac, st));
     return CR_OK;
 }
 
@@ -733,8 +736,20 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
     };
     struct MeshMarkReset {
         cr_context* c;
-        ~MeshMarkReset() { c->mesh_zeroed = false; }
+        ~MeshMarkReset()
+        {
+            c->mesh_zeroed = false;
+            c->qf_for = nullptr;
+        }
     } mark_reset{c};
+    // FP32 bulk: each step tail also leaves the next application's float charges
+    float* tail_qf = nullptr;
+    if (c->fp32() && v.N > 0 && nblk <= kFxParts) {
+        CR_HIP(c->qf.ensure(sizeof(float) * (size_t)v.N));
+        tail_qf = c->qf.as<float>();
+    }
+    int tail_parts = 0;
+    bool tail_charges = false;
     for (;;) {
         if (x_zero) {
             // x0 = 0: A x0 = 0 exactly, so the first cycle's residual needs no operator
@@ -760,6 +775,7 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
         if (iter >= maxiter) break;
         ++restarts;
         CR_HIP(launch_scale_copy(w, &g->inv_beta, -1.0, V, n, st));
+        tail_charges = false;  // a cycle starts from V[:, 0], not from a step tail's output
         int i = -1;
         // A step's residual is read one step late while the trend of the last two says it is
         // still far above the tolerance, so that the host does not stall the device between
@@ -773,7 +789,14 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
             ++i;
             ++iter;
             double* vi = V + (long)i * c->ld;
-            CR_RET(matvec_prec(c, v, vi, w, st));
+            if (tail_charges) {
+                c->qf_for = vi;
+                c->qf_parts = tail_parts;
+            }
+            const int mv = matvec_prec(c, v, vi, w, st);
+            c->qf_for = nullptr;
+            tail_charges = false;
+            CR_RET(mv);
             ++matvecs;
             // CGS2: two projections onto V[:,0:i+1]; the second correction is applied only
             // when it is not negligible against ||w|| (h2[i+1] = ||w||^2 after the first)
@@ -789,7 +812,9 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
             double* vnext = V + (long)(i + 1) * c->ld;
             if (!c->hpub) {
                 // ||w||, V[:, i+1], the rotations (no mailbox: the residual is copied back)
-                CR_HIP(launch_step_tail(partial, nblk, g, i, R, w, vnext, n, nullptr, 0, st));
+                CR_HIP(launch_step_tail(partial, nblk, g, i, R, w, vnext, n, nullptr, 0, st, tail_qf,
+                                        c->fxs.as<FxScale>(), v.N, &tail_parts));
+                tail_charges = tail_qf != nullptr;
                 if (go_on) CR_RET(preclear_mesh(c, st));
                 CR_RET(read_scalar(c, &g->resid, &resid, st));
                 if (!std::isfinite(resid)) return nonfinite();
@@ -802,7 +827,9 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
             const int slot = iter % kPubSlots;
             // ||w||, V[:, i+1], the rotations and the residual's publication in one launch
             const unsigned long long seq = ++c->pub_seq;
-            CR_HIP(launch_step_tail(partial, nblk, g, i, R, w, vnext, n, c->hpub + slot, seq, st));
+            CR_HIP(launch_step_tail(partial, nblk, g, i, R, w, vnext, n, c->hpub + slot, seq, st, tail_qf,
+                                    c->fxs.as<FxScale>(), v.N, &tail_parts));
+            tail_charges = tail_qf != nullptr;
             if (go_on) CR_RET(preclear_mesh(c, st));
             if (pending) {
                 double rp = 0;

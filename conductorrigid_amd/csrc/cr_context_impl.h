@@ -31,6 +31,10 @@ struct cr_context {
     // The mesh was zeroed ahead of a host wait inside one GMRES call; the next spread skips
     // its own clear.  Set and reset only within gmres().
     bool mesh_zeroed = false;
+    // FP32 charges already in qf (with their partial maxima in fxs) for the operator input
+    // qf_for: set by gmres() for one application, from its step tail
+    const double* qf_for = nullptr;
+    int qf_parts = 0;
     long rs_ldn = 0;
     DevBuf rs_col, rs_val, rs_len;
     DevBuf qf;  // charges rounded to float for the FP32-bulk kernels

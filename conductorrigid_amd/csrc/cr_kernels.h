@@ -186,6 +186,8 @@ struct FxPlan {
 FxPlan fx_plan(const MeshParams& mp, const TileGeom& tg, size_t lds_max);
 // qf = (float) x [N]; max |qf|, then fs->mult and fs->inv from it and fs->wbits.
 hipError_t launch_charges_fx(const double* x, int N, float* qf, FxScale* fs, double prefac, hipStream_t st);
+// The scale alone (fs->mult, fs->inv, fs->qmax) from nparts partial maxima in fs->part.
+hipError_t launch_fx_scale(int nparts, FxScale* fs, double prefac, hipStream_t st);
 // Scratch ints per tile block of launch_spread_fx (a multiple of 4).
 long fx_block_ints(const MeshParams& mp, const TileGeom& tg);
 // mesh (float, G nodes) = fixed-point spread of qf at fs's scale: the tiles' integer
@@ -565,9 +567,12 @@ constexpr int kPubSlots = 2;
 hipError_t launch_publish(const double* src, HostPub* dst, unsigned long long seq, hipStream_t st);
 // The end of Arnoldi step i: ||w||^2 from the last CGS pass's partials (nblocks of them,
 // column 0), vnext = w / ||w||, column i of H and its Givens rotations, then g->resid
-// published to pub (when not null) with sequence number seq.
+// published to pub (when not null) with sequence number seq.  qf != nullptr: also the FP32
+// charges of vnext's first nq entries and their partial maxima in fs->part[0:*nparts), as
+// launch_charges_fx leaves them before its scale (launch_fx_scale completes it).
 hipError_t launch_step_tail(const double* partial, int nblocks, GmresState* g, int i, int R, const double* w,
-                            double* vnext, long n, HostPub* pub, unsigned long long seq, hipStream_t st);
+                            double* vnext, long n, HostPub* pub, unsigned long long seq, hipStream_t st,
+                            float* qf = nullptr, FxScale* fs = nullptr, long nq = 0, int* nparts = nullptr);
 hipError_t launch_cycle_init(GmresState* g, const double* norm2, hipStream_t st);
 hipError_t launch_backsub(GmresState* g, int R, hipStream_t st);
 

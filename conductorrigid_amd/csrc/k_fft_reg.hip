@@ -131,13 +131,15 @@ constexpr int lanes_per_wg()
 }
 
 // Columns per workgroup of the strided passes (C columns x A*B values in the transpose
-// buffer): as many as fit 80 KB of LDS, at most 512 threads.  Wider column groups read
-// longer runs of each row (32 complex floats: 256 bytes); measured at 256^3 against 16
-// columns per workgroup: y passes 43 -> 36 us, x pass 61 -> 57 us.
+// buffer): as many as fit 40 KB (float) or 80 KB (double) of LDS, at most 512 threads, so
+// that a column group is 128 or 256 bytes of each row.  On rows padded to whole 128-byte
+// lines, 16 float columns (one line, twice the workgroups per CU) measured faster than 32
+// at 256^3 (y pass 34.2 -> 32.9 us, x pass 52.2 -> 49.3 us); on the odd pitch Nz/2+1
+// (each group split over partial lines) 32 had been faster than 16.
 template <typename T, int A, int B>
 constexpr int cols_per_wg()
 {
-    constexpr int fit = 81920 / (int)(A * B * 2 * sizeof(T));
+    constexpr int fit = (sizeof(T) == 4 ? 40960 : 81920) / (int)(A * B * 2 * sizeof(T));
     constexpr int cap = 512 / B;
     int p = 1;
     while (p * 2 <= fit && p * 2 <= cap) p *= 2;

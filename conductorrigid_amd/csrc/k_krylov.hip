@@ -258,15 +258,22 @@ __device__ __forceinline__ void publish_value(double v, HostPub* dst, unsigned l
     *reinterpret_cast<volatile unsigned long long*>(&dst->seq) = seq;
 }
 
+// NaN-propagating maximum (a NaN charge must reach the FP32-bulk mesh as NaN)
+__device__ __forceinline__ float nan_max(float m, float a) { return (a > m || a != a) ? a : m; }
+
 // The end of an Arnoldi step in one launch: ||w||^2 from the partials of the last CGS pass
 // (every workgroup sums them in k_reduce_partials' order, so all hold the same value),
 // V[:, i+1] = w / ||w|| (as k_scale_copy), and in workgroup 0 column i of H with its
 // Givens rotations (k_givens), then the residual published to the host (pub != nullptr,
-// as k_publish).
+// as k_publish).  qf != nullptr: also the next operator application's float charges,
+// qf = (float) V[0:nq, i+1], and each workgroup's NaN-propagating max |qf| in
+// fs->part[blockIdx.x] (the float conversion and maximum of k_charges_fx, so the same
+// charges and scale).
 __global__ __launch_bounds__(kBlock) void k_step_tail(const double* __restrict__ partial, int nblocks, GmresState* g,
                                                       int i, int R, const double* __restrict__ w,
                                                       double* __restrict__ vnext, long n, HostPub* pub,
-                                                      unsigned long long seq)
+                                                      unsigned long long seq, float* __restrict__ qf,
+                                                      FxScale* __restrict__ fs, long nq)
 {
     double s = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partial[(long)b * 32];
@@ -284,7 +291,31 @@ __global__ __launch_bounds__(kBlock) void k_step_tail(const double* __restrict__
     const double wn2 = tot;
     const double hn = sqrt(wn2);
     const double f = (hn != 0.0) ? 1.0 / hn : 0.0;
-    for (long k = (long)blockIdx.x * kBlock + threadIdx.x; k < n; k += (long)gridDim.x * kBlock) vnext[k] = f * w[k];
+    if (!qf) {
+        for (long k = (long)blockIdx.x * kBlock + threadIdx.x; k < n; k += (long)gridDim.x * kBlock)
+            vnext[k] = f * w[k];
+    } else {
+        float m = 0.0f;
+        for (long k = (long)blockIdx.x * kBlock + threadIdx.x; k < n; k += (long)gridDim.x * kBlock) {
+            const double v = f * w[k];
+            vnext[k] = v;
+            if (k < nq) {
+                const float q = (float)v;
+                qf[k] = q;
+                m = nan_max(m, fabsf(q));
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_xor(m, off, 64));
+        __shared__ float wm[kBlock / 64];
+        if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float bm = wm[0];
+            for (int k = 1; k < kBlock / 64; ++k) bm = nan_max(bm, wm[k]);
+            fs->part[blockIdx.x] = bm;
+        }
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         g->wnorm2 = wn2;
         givens_column(g, i, R);
@@ -356,10 +387,14 @@ hipError_t launch_reduce_partials(const double* partial, int nblocks, int ncols,
 }
 
 hipError_t launch_step_tail(const double* partial, int nblocks, GmresState* g, int i, int R, const double* w,
-                            double* vnext, long n, HostPub* pub, unsigned long long seq, hipStream_t st)
+                            double* vnext, long n, HostPub* pub, unsigned long long seq, hipStream_t st, float* qf,
+                            FxScale* fs, long nq, int* nparts)
 {
-    hipLaunchKernelGGL(k_step_tail, dim3(vblocks(n)), dim3(kBlock), 0, st, partial, nblocks, g, i, R, w, vnext, n,
-                       pub, seq);
+    const unsigned nb = vblocks(n);
+    if (qf && (!fs || nb > (unsigned)kFxParts || nq > n)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_step_tail, dim3(nb), dim3(kBlock), 0, st, partial, nblocks, g, i, R, w, vnext, n, pub, seq,
+                       qf, fs, nq);
+    if (nparts) *nparts = (int)nb;
     return hipGetLastError();
 }
 

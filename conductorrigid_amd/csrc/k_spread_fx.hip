@@ -1104,6 +1104,13 @@ hipError_t launch_charges_fx(const double* x, int N, float* qf, FxScale* fs, dou
     return hipGetLastError();
 }
 
+hipError_t launch_fx_scale(int nparts, FxScale* fs, double prefac, hipStream_t st)
+{
+    if (nparts < 1 || nparts > kFxParts) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_fx_scale, dim3(1), dim3(kFxWave), 0, st, nparts, fs, prefac);
+    return hipGetLastError();
+}
+
 bool fx_gather_v4(const MeshParams& mp, const TileGeom& tg)
 {
     const long ys = ((long)tg.T[2] + mp.P) & ~1L;  // block row pitch, (EZ + 1) & ~1

@@ -68,23 +68,29 @@ def test_fx_tile_shapes(tile, require_gpu):
     assert v < 1e-6 and _maxrel(got, ref) < 1e-6
 
 
-@pytest.mark.parametrize("mesh", [(128, 64, 64), (128, 128, 64), (128, 128, 128)])
-def test_fx_persistent_runs(mesh, require_gpu):
-    """8^3 tiles on meshes of 1024, 2048 and 4096 tiles: the persistent spread takes runs of
-    2, 4 and 8 tiles along z and carries each tile's z halo in LDS into the next one; the
-    last halo of a run goes to the first tile of the next run (wrapping at the mesh edge)."""
+@pytest.mark.parametrize("mesh,tile", [((128, 64, 64), (8, 8, 8)), ((128, 128, 64), (8, 8, 8)),
+                                       ((128, 128, 128), (8, 8, 8)), ((256, 128, 128), (16, 16, 16))])
+def test_fx_persistent_runs(mesh, tile, require_gpu):
+    """Meshes of 1024-4096 tiles: the persistent spread takes runs of 2, 4 and 8 tiles along
+    z and carries each tile's z halo in LDS into the next tile (the last halo of a run goes
+    to the first tile of the next run, wrapping at the mesh edge); the potential contraction
+    of the same tiling reads the mesh back."""
     s = n1k_system()
     prm = dict(N1K, mesh=mesh)
     g = GpuCase(s, precision="fp32", **prm)
-    g.cr.set_tiles((8, 8, 8), 160 * 1024)
+    g.cr.set_tiles(tile, 160 * 1024)
     o = O.Oracle(s, prm["xi"], prm["errortol"], rc=prm["rc"], mesh=mesh, P=prm["P"])
     q = np.random.default_rng(23).normal(size=s.N)
     ref = o.spread(q)
     got = g.spread(q)
     v = rel(got, ref)
-    _report(f"runs {mesh}", v)
+    _report(f"runs {mesh} {tile}", v)
     assert v < 1e-6 and _maxrel(got, ref) < 1e-6
     assert np.array_equal(got, g.spread(q))
+    _, phi = g.contract_after_recip(ref)
+    vc = rel(phi, o.contract(o.recip_grid(ref)))
+    _report(f"runs {mesh} {tile} contract", vc)
+    assert vc < 1e-6
 
 
 def test_fx_spread_is_deterministic(fx_case):
