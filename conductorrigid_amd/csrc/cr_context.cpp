@@ -360,8 +360,8 @@ int csr_build(cr_context* c, const SysView& v, unsigned width, hipStream_t st)
     return CR_OK;
 }
 
-// FP32 bulk: the charges x[0:N] rounded to float (c->qf) and, for the fixed-point
-// spread, its scale from their largest magnitude and the weight bound of the current
+// FP32 bulk: the charges x[0:N] rounded to float (c->qf) and their partial maxima, from
+// which the fixed-point spread forms its scale with the weight bound of the current
 // binning (computed first, once per bead configuration: the scale depends on it).
 int charges_f32(cr_context* c, const SysView& v, const double* x, hipStream_t st)
 {
@@ -370,10 +370,11 @@ int charges_f32(cr_context* c, const SysView& v, const double* x, hipStream_t st
         c->fx_bound = true;
     }
     CR_HIP(c->qf.ensure(sizeof(float) * (size_t)std::max(1, v.N)));
-    if (x && x == c->qf_for)  // the step tail that produced x left its charges and partial maxima
-        CR_HIP(launch_fx_scale(c->qf_parts, c->fxs.as<FxScale>(), c->mp.prefac, st));
+    // the charges and their partial maxima; the fixed-point spread forms the scale from these
+    if (x && x == c->qf_for)  // the step tail that produced x left them
+        c->fx_parts = c->qf_parts;
     else
-        CR_HIP(launch_charges_fx(x, v.N, c->qf.as<float>(), c->fxs.as<FxScale>(), c->mp.prefac, st));
+        CR_HIP(launch_charges_fx_parts(x, v.N, c->qf.as<float>(), c->fxs.as<FxScale>(), &c->fx_parts, st));
     return CR_OK;
 }
 
@@ -383,7 +384,7 @@ int spread_fx(cr_context* c, hipStream_t st)
     if (!c->fx_bound) return set_error(CR_ERR_STATE, "fixed-point spread before its scale");
     CR_HIP(c->fxblk.ensure(sizeof(int) * (size_t)c->tg.ntiles * (size_t)fx_block_ints(c->mp, c->tg)));
     CR_HIP(launch_spread_fx(c->mp, c->tg, c->fxp, bin_view(c), c->qf.as<float>(), c->fxs.as<FxScale>(),
-                            c->fxblk.as<int>(), c->grid_real.as<float>(), st));
+                            c->fxblk.as<int>(), c->grid_real.as<float>(), st, c->fx_parts));
     return CR_OK;
 }
 

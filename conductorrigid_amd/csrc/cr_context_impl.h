@@ -35,6 +35,9 @@ struct cr_context {
     // qf_for: set by gmres() for one application, from its step tail
     const double* qf_for = nullptr;
     int qf_parts = 0;
+    // partial maxima of the charges in qf (fxs->part), from which the next fixed-point
+    // spread forms its scale
+    int fx_parts = 0;
     long rs_ldn = 0;
     DevBuf rs_col, rs_val, rs_len;
     DevBuf qf;  // charges rounded to float for the FP32-bulk kernels

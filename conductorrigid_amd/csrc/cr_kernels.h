@@ -188,12 +188,17 @@ FxPlan fx_plan(const MeshParams& mp, const TileGeom& tg, size_t lds_max);
 hipError_t launch_charges_fx(const double* x, int N, float* qf, FxScale* fs, double prefac, hipStream_t st);
 // The scale alone (fs->mult, fs->inv, fs->qmax) from nparts partial maxima in fs->part.
 hipError_t launch_fx_scale(int nparts, FxScale* fs, double prefac, hipStream_t st);
+// qf = (float) x [N] and the partial maxima alone (*nparts of them): launch_spread_fx with
+// nparts forms the scale.
+hipError_t launch_charges_fx_parts(const double* x, int N, float* qf, FxScale* fs, int* nparts, hipStream_t st);
 // Scratch ints per tile block of launch_spread_fx (a multiple of 4).
 long fx_block_ints(const MeshParams& mp, const TileGeom& tg);
 // mesh (float, G nodes) = fixed-point spread of qf at fs's scale: the tiles' integer
 // blocks go to the scratch blocks (tg.ntiles x fx_block_ints) and are summed per node.
+// nparts > 0: the scale is first formed from the nparts partial maxima in fs->part (and
+// stored in fs); nparts == 0: fs already holds it.
 hipError_t launch_spread_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                            const float* qf, FxScale* fs, int* blocks, float* mesh, hipStream_t st);
+                            const float* qf, FxScale* fs, int* blocks, float* mesh, hipStream_t st, int nparts = 0);
 // fs->wbits for the binned configuration (once per bead configuration).
 hipError_t launch_spread_fx_bound(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
                                   FxScale* fs, hipStream_t st);

@@ -96,17 +96,11 @@ __global__ __launch_bounds__(kFxChargeThreads) void k_charges_fx(const double* _
     charges_fx_part(x, N, qf, fs, blockIdx.x, gridDim.x);
 }
 
-// One wavefront: qmax from the partial maxima, then the multiplier and decode factor.
-__device__ __forceinline__ void fx_scale_wave(int nparts, FxScale* __restrict__ fs, double prefac)
+// The multiplier and decode factor for charges of NaN-propagating maximum qm.
+__device__ __forceinline__ void fx_scale_values(float qm, unsigned wbits, double prefac, float& mult, float& inv)
 {
-    const int lane = threadIdx.x;
-    float qm = 0.0f;
-    for (int k = lane; k < nparts; k += kFxWave) qm = nanmax(qm, fs->part[k]);
-    qm = wave_nanmax(qm);
-    if (lane != 0) return;
-    const double W = 8.0 * (double)__uint_as_float(fs->wbits);
+    const double W = 8.0 * (double)__uint_as_float(wbits);
     const double qW = (double)qm * W;
-    float mult, inv;
     if (qm != qm || !(qm <= 3.0e38f)) {  // NaN or infinite charge: decode to NaN
         mult = 0.0f;
         inv = __int_as_float(0x7fc00000);
@@ -117,6 +111,18 @@ __device__ __forceinline__ void fx_scale_wave(int nparts, FxScale* __restrict__ 
         mult = (float)(1073741824.0 / qW);
         inv = (float)(prefac * qW / 1073741824.0);
     }
+}
+
+// One wavefront: qmax from the partial maxima, then the multiplier and decode factor.
+__device__ __forceinline__ void fx_scale_wave(int nparts, FxScale* __restrict__ fs, double prefac)
+{
+    const int lane = threadIdx.x;
+    float qm = 0.0f;
+    for (int k = lane; k < nparts; k += kFxWave) qm = nanmax(qm, fs->part[k]);
+    qm = wave_nanmax(qm);
+    if (lane != 0) return;
+    float mult, inv;
+    fx_scale_values(qm, fs->wbits, prefac, mult, inv);
     fs->mult = mult;
     fs->inv = inv;
     fs->qmax = qm;
@@ -389,7 +395,7 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx_run(MeshParams mp, Til
                                                               const int* __restrict__ rec_off,
                                                               const float4* __restrict__ rec_u,
                                                               const float* __restrict__ qf,
-                                                              const FxScale* __restrict__ fs,
+                                                              FxScale* __restrict__ fs, int nparts, double prefac,
                                                               int* __restrict__ blocks)
 {
     constexpr int WS = fx_wstride(PT);
@@ -403,7 +409,6 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx_run(MeshParams mp, Til
     const int j = threadIdx.x, lane = j & (kFxWave - 1), wave = fx_wave();
     const int NR = max(1, (kFxThreads / 3) / ch), npf = NR * ch;
     const int nt = tg.ntiles;
-    const float mult = fs->mult;
     const float c0 = (float)(mp.xiterm * mp.h[0] * mp.h[0] / mp.eta[0]);
     const float c1 = (float)(mp.xiterm * mp.h[1] * mp.h[1] / mp.eta[1]);
     const float c2 = (float)(mp.xiterm * mp.h[2] * mp.h[2] / mp.eta[2]);
@@ -416,6 +421,27 @@ __global__ __launch_bounds__(kFxThreads) void k_spread_fx_run(MeshParams mp, Til
     FxPrefetch cur = fx_prefetch(kb, ke, npf, rec_off, sorted, rec_u);
     float qv = (j < (int)min(ke - kb, (unsigned)npf)) ? qf[cur.si] : 0.0f;
     for (int q = j; q < esz / 4; q += kFxThreads) reinterpret_cast<uint4*>(ext)[q] = make_uint4(0, 0, 0, 0);
+    // the charge scale from the partial maxima (as k_fx_scale; workgroup 0 stores it for the
+    // gather), or, nparts == 0, the one already in fs
+    float mult;
+    if (nparts > 0) {
+        __shared__ float qw[NW];
+        float m = j < nparts ? fs->part[j] : 0.0f;
+        m = wave_nanmax(m);
+        if (lane == 0) qw[wave] = m;
+        __syncthreads();
+        float qm = qw[0];
+        for (int k = 1; k < NW; ++k) qm = nanmax(qm, qw[k]);
+        float inv;
+        fx_scale_values(qm, fs->wbits, prefac, mult, inv);
+        if (blockIdx.x == 0 && j == 0) {
+            fs->mult = mult;
+            fs->inv = inv;
+            fs->qmax = qm;
+        }
+    } else {
+        mult = fs->mult;
+    }
     for (int k = 0; k < per; ++k) {
         const bool more = k + 1 < per;
         const int tn = more ? fx_run_tile(blockIdx.x, k + 1, per, nt) : t;
@@ -910,8 +936,8 @@ inline int fx_run_per(const TileGeom& tg)
 // caller's k_fx_gather.
 template <int PT, int TX, int TY, int TZ, bool ONES>
 hipError_t launch_fx_kernel(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                            const float* qf, FxScale* fs, long bstride, int* blocks, float* mesh, bool* gathered,
-                            hipStream_t st)
+                            const float* qf, FxScale* fs, int nparts, long bstride, int* blocks, float* mesh,
+                            bool* gathered, hipStream_t st)
 {
     if constexpr (!ONES && TX > 0) {
         const int per = fx_run_per(tg);
@@ -919,7 +945,7 @@ hipError_t launch_fx_kernel(const MeshParams& mp, const TileGeom& tg, const FxPl
         allow_lds_fx(reinterpret_cast<const void*>(fr), pl.lds_bytes);
         hipLaunchKernelGGL(fr, dim3(tg.ntiles / per), dim3(kFxThreads), pl.lds_bytes, st, mp, tg, pl.chunk, per,
                            bv.tile_start, bv.sorted, bv.rec_off, static_cast<const float4*>(bv.rec_win), qf, fs,
-                           blocks);
+                           nparts, mp.prefac, blocks);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         const unsigned gb = (unsigned)fx_gather_blocks(mp, true);
@@ -930,6 +956,7 @@ hipError_t launch_fx_kernel(const MeshParams& mp, const TileGeom& tg, const FxPl
     }
     (void)mesh;
     (void)gathered;
+    if (!ONES && nparts > 0) hipLaunchKernelGGL(k_fx_scale, dim3(1), dim3(kFxWave), 0, st, nparts, fs, mp.prefac);
     auto fn = k_spread_fx<PT, TX, TY, TZ, ONES>;
     allow_lds_fx(reinterpret_cast<const void*>(fn), pl.lds_bytes);
     hipLaunchKernelGGL(fn, dim3(tg.ntiles), dim3(kFxThreads), pl.lds_bytes, st, mp, tg, pl.chunk, bv.tile_start,
@@ -939,13 +966,13 @@ hipError_t launch_fx_kernel(const MeshParams& mp, const TileGeom& tg, const FxPl
 
 template <int PT, bool ONES>
 hipError_t dispatch_shape(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                          const float* qf, FxScale* fs, long bstride, int* blocks, float* mesh, bool* gathered,
-                          hipStream_t st)
+                          const float* qf, FxScale* fs, int nparts, long bstride, int* blocks, float* mesh,
+                          bool* gathered, hipStream_t st)
 {
     const int a = tg.T[0], b = tg.T[1], c = tg.T[2];
 #define CR_FX_SHAPE(X, Y, Z) \
     if (a == X && b == Y && c == Z)  \
-    return launch_fx_kernel<PT, X, Y, Z, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st)
+    return launch_fx_kernel<PT, X, Y, Z, ONES>(mp, tg, pl, bv, qf, fs, nparts, bstride, blocks, mesh, gathered, st)
     if constexpr (PT == 8) {
         CR_FX_SHAPE(16, 16, 16);
         CR_FX_SHAPE(16, 8, 16);
@@ -959,21 +986,29 @@ hipError_t dispatch_shape(const MeshParams& mp, const TileGeom& tg, const FxPlan
         CR_FX_SHAPE(8, 8, 8);
     }
 #undef CR_FX_SHAPE
-    return launch_fx_kernel<PT, 0, 0, 0, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
+    return launch_fx_kernel<PT, 0, 0, 0, ONES>(mp, tg, pl, bv, qf, fs, nparts, bstride, blocks, mesh, gathered, st);
 }
 
 template <bool ONES>
 hipError_t dispatch_p(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv, const float* qf,
-                      FxScale* fs, long bstride, int* blocks, float* mesh, bool* gathered, hipStream_t st)
+                      FxScale* fs, int nparts, long bstride, int* blocks, float* mesh, bool* gathered,
+                      hipStream_t st)
 {
     switch (mp.P) {
-        case 4: return dispatch_shape<4, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
-        case 5: return dispatch_shape<5, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
-        case 6: return dispatch_shape<6, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
-        case 7: return dispatch_shape<7, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
-        case 8: return dispatch_shape<8, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
-        case 9: return dispatch_shape<9, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
-        case 10: return dispatch_shape<10, ONES>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, gathered, st);
+        case 4: return dispatch_shape<4, ONES>(mp, tg, pl, bv, qf, fs, nparts, bstride, blocks, mesh, gathered,
+                                                  st);
+        case 5: return dispatch_shape<5, ONES>(mp, tg, pl, bv, qf, fs, nparts, bstride, blocks, mesh, gathered,
+                                                  st);
+        case 6: return dispatch_shape<6, ONES>(mp, tg, pl, bv, qf, fs, nparts, bstride, blocks, mesh, gathered,
+                                                  st);
+        case 7: return dispatch_shape<7, ONES>(mp, tg, pl, bv, qf, fs, nparts, bstride, blocks, mesh, gathered,
+                                                  st);
+        case 8: return dispatch_shape<8, ONES>(mp, tg, pl, bv, qf, fs, nparts, bstride, blocks, mesh, gathered,
+                                                  st);
+        case 9: return dispatch_shape<9, ONES>(mp, tg, pl, bv, qf, fs, nparts, bstride, blocks, mesh, gathered,
+                                                  st);
+        case 10: return dispatch_shape<10, ONES>(mp, tg, pl, bv, qf, fs, nparts, bstride, blocks, mesh, gathered,
+                                                  st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1104,6 +1139,14 @@ hipError_t launch_charges_fx(const double* x, int N, float* qf, FxScale* fs, dou
     return hipGetLastError();
 }
 
+hipError_t launch_charges_fx_parts(const double* x, int N, float* qf, FxScale* fs, int* nparts, hipStream_t st)
+{
+    const unsigned blocks = std::min<unsigned>(kFxParts, nblk(std::max(N, 1), kFxChargeThreads));
+    hipLaunchKernelGGL(k_charges_fx, dim3(blocks), dim3(kFxChargeThreads), 0, st, x, N, qf, fs);
+    *nparts = (int)blocks;
+    return hipGetLastError();
+}
+
 hipError_t launch_fx_scale(int nparts, FxScale* fs, double prefac, hipStream_t st)
 {
     if (nparts < 1 || nparts > kFxParts) return hipErrorInvalidValue;
@@ -1129,14 +1172,15 @@ long fx_block_ints(const MeshParams& mp, const TileGeom& tg)
 }
 
 hipError_t launch_spread_fx(const MeshParams& mp, const TileGeom& tg, const FxPlan& pl, const BinView& bv,
-                            const float* qf, FxScale* fs, int* blocks, float* mesh, hipStream_t st)
+                            const float* qf, FxScale* fs, int* blocks, float* mesh, hipStream_t st, int nparts)
 {
+    if (nparts < 0 || nparts > kFxParts) return hipErrorInvalidValue;
     if (!pl.valid) return hipErrorInvalidValue;
     for (int d = 0; d < 3; ++d)
         if (tg.T[d] < mp.P - 1) return hipErrorInvalidValue;  // at most two blocks per axis cover a node
     const long bstride = fx_block_ints(mp, tg);
     bool gathered = false;
-    hipError_t e = dispatch_p<false>(mp, tg, pl, bv, qf, fs, bstride, blocks, mesh, &gathered, st);
+    hipError_t e = dispatch_p<false>(mp, tg, pl, bv, qf, fs, nparts, bstride, blocks, mesh, &gathered, st);
     if (e != hipSuccess || gathered) return e;
     const bool v4 = fx_gather_v4(mp, tg);
     const unsigned gb = (unsigned)fx_gather_blocks(mp, v4);
@@ -1154,7 +1198,7 @@ hipError_t launch_spread_fx_bound(const MeshParams& mp, const TileGeom& tg, cons
     hipError_t e = launch_fill<unsigned>(&fs->wbits, 0u, 1, st);
     if (e != hipSuccess) return e;
     bool gathered = false;
-    return dispatch_p<true>(mp, tg, pl, bv, nullptr, fs, 0, nullptr, nullptr, &gathered, st);
+    return dispatch_p<true>(mp, tg, pl, bv, nullptr, fs, 0, 0, nullptr, nullptr, &gathered, st);
 }
 
 hipError_t launch_charges_fx_batch(const FxBox* boxes, int nbox, int max_n, const int* skip, hipStream_t st)
