@@ -564,6 +564,8 @@ int block_alloc(cr_context* c, const SysView& v)
     CR_HIP(c->block_scan.ensure(sizeof(unsigned long long) * ((b1 + 1) / 1024 + 4)));
     const size_t nbk = (size_t)block_order_buckets(std::max(0, v.N));
     CR_HIP(c->body_order.ensure(sizeof(unsigned) * b1));
+    CR_HIP(c->slot_count.ensure(sizeof(unsigned) * b1));
+    CR_HIP(c->slot_members.ensure(sizeof(unsigned) * b1 * nmax));
     CR_HIP(c->order_key.ensure(sizeof(uint2) * b1));
     CR_HIP(c->order_count.ensure(sizeof(unsigned) * nbk));
     CR_HIP(c->order_start.ensure(sizeof(unsigned) * nbk));
@@ -571,19 +573,22 @@ int block_alloc(cr_context* c, const SysView& v)
     return CR_OK;
 }
 
+BlockSlots block_slots(cr_context* c)
+{
+    return BlockSlots{c->body_order.as<unsigned>(), c->slot_count.as<unsigned>(), c->slot_members.as<unsigned>(),
+                      c->order_count.as<unsigned>(), c->order_start.as<unsigned>(),
+                      c->order_scan.as<unsigned long long>(), c->order_key.as<uint2>()};
+}
+
 int block_setup(cr_context* c, const SysView& v, hipStream_t st, bool invert)
 {
     CR_RET(block_alloc(c, v));
     const int nmax = c->block_nmax;
+    const BlockSlots slots = block_slots(c);
     CR_HIP(launch_block_setup(v, c->mp.L, nmax, c->body_count.as<unsigned>(), c->body_start.as<unsigned>(),
                               c->body_beads.as<unsigned>(),
                               c->block_scan.as<unsigned long long>(), c->ginv.as<float>(), c->uvec.as<float>(),
-                              c->inv_s.as<double>(), st, invert));
-    if (invert)
-        CR_HIP(launch_block_order(v.N_isolated, v.N, c->body_start.as<unsigned>(), c->body_beads.as<unsigned>(),
-                                  c->order_count.as<unsigned>(), c->order_start.as<unsigned>(),
-                                  c->order_scan.as<unsigned long long>(), c->order_key.as<uint2>(),
-                                  c->body_order.as<unsigned>(), st));
+                              c->inv_s.as<double>(), st, invert, &slots));
     return CR_OK;
 }
 
@@ -591,9 +596,9 @@ int block_setup(cr_context* c, const SysView& v, hipStream_t st, bool invert)
 int apply_precond(cr_context* c, const SysView& v, const double* in, double* out, hipStream_t st, const double* xq)
 {
     if (block_precond(c)) {
-        CR_HIP(launch_block_apply(v, c->block_nmax, c->body_start.as<unsigned>(), c->body_beads.as<unsigned>(),
-                                  c->ginv.as<float>(), c->uvec.as<float>(), c->inv_s.as<double>(), in, out, st, xq,
-                                  c->body_order.as<unsigned>()));
+        const BlockSlots slots = block_slots(c);
+        CR_HIP(launch_block_apply(v, c->block_nmax, slots, c->ginv.as<float>(), c->uvec.as<float>(),
+                                  c->inv_s.as<double>(), in, out, st, xq));
     } else {
         CR_HIP(launch_precondition(v, in, out, st));
     }

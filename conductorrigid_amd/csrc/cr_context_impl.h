@@ -62,8 +62,9 @@ struct cr_context {
     // own scan scratch: its setup runs beside the binning, which scans into scan_sums)
     int block_nmax = 0;
     DevBuf body_count, body_start, body_beads, ginv, uvec, inv_s, block_scan;
-    // the block apply's processing order of the bodies and its counting-sort scratch
-    DevBuf body_order, order_key, order_count, order_start, order_scan;
+    // the block preconditioner's slots (processing order of the bodies, their bead counts
+    // and member lists) and their counting-sort scratch
+    DevBuf body_order, slot_count, slot_members, order_key, order_count, order_start, order_scan;
     // Independent parts of a force evaluation run beside the caller's stream on these
     // (default priority): the real-space ELL build and the block preconditioner's setup
     // next to the binning, and the real-space force next to the force pass's mesh work,

@@ -400,10 +400,27 @@ hipError_t launch_zero_force(int N, double* force, hipStream_t st);
 
 // ---- k_precond.hip ----
 // Near-field block preconditioner: per isolated body, the inverse of its own saddle block.
+// The single-system form keeps its bodies in slot order (ascending smallest bead index, in
+// buckets of 32 indices): order[slot] is the body, count[slot] its bead count and
+// members[slot * nmax + k] its k-th bead; ginv, uvec and inv_s are stored by slot.
+// Scratch: bucket_count and bucket_start of block_order_buckets(N) entries, scan_sums of
+// block_order_buckets(N) / 1024 + 4, key of B.
+struct BlockSlots {
+    unsigned* order;
+    unsigned* count;
+    unsigned* members;
+    unsigned* bucket_count;
+    unsigned* bucket_start;
+    unsigned long long* scan_sums;
+    uint2* key;
+};
+long block_order_buckets(int N);
+// Bodies grouped (body_start, body_beads); with invert, the slots and the per-slot inverses
+// (bodies of at most 64 beads).
 hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, unsigned* body_count,
                               unsigned* body_start, unsigned* body_beads,
                               unsigned long long* scan_sums, float* ginv, float* uvec, double* inv_s, hipStream_t st,
-                              bool invert = true);
+                              bool invert = true, const BlockSlots* slots = nullptr);
 // One box of a batched block-preconditioner application (device array, body0 ascending):
 // the box's per-body inverses and its offset in the batch vectors.
 struct BlockBox {
@@ -422,20 +439,12 @@ struct BlockBox {
 hipError_t launch_block_setup_batch(const BlockBox* boxes, int nbox, int total_bodies, int nmax_all, hipStream_t st);
 hipError_t launch_block_apply_batch(const BlockBox* boxes, int nbox, int total_bodies, const double* y, double* w,
                                     const int* skip, hipStream_t st);
-// w = D y with the near-field block preconditioner.  xq != nullptr: body t's constraint row
-// is taken as the sum of xq over its beads (the operator's charge row for input xq) instead
-// of y[N + t], which is then never read.
-// order: the bodies in processing order (launch_block_order), or null for ascending index.
-hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_start, const unsigned* body_beads,
-                              const float* ginv, const float* uvec, const double* inv_s, const double* y, double* w,
-                              hipStream_t st, const double* xq = nullptr, const unsigned* order = nullptr);
-// The B bodies in ascending order of their smallest bead index (buckets of 32 indices, any
-// order within a bucket), for the block apply.  Scratch: bucket_count and bucket_start of
-// block_order_buckets(N) entries, scan_sums of block_order_buckets(N) / 1024 + 4, key of B.
-long block_order_buckets(int N);
-hipError_t launch_block_order(int B, int N, const unsigned* body_start, const unsigned* body_beads,
-                              unsigned* bucket_count, unsigned* bucket_start, unsigned long long* scan_sums,
-                              uint2* key, unsigned* order, hipStream_t st);
+// w = D y with the near-field block preconditioner (slot-major, from launch_block_setup).
+// xq != nullptr: body t's constraint row is taken as the sum of xq over its beads (the
+// operator's charge row for input xq) instead of y[N + t], which is then never read.
+hipError_t launch_block_apply(const SysView& s, int nmax, const BlockSlots& slots, const float* ginv,
+                              const float* uvec, const double* inv_s, const double* y, double* w, hipStream_t st,
+                              const double* xq = nullptr);
 
 // ---- k_mlp.hip ----
 // Learned warm start: per-bead MLP trunk kMlpTrunk -> kMlpHidden -> kMlpHidden -> kMlpOut

@@ -77,71 +77,6 @@ __global__ void k_body_fill_single(SysView s, const unsigned* __restrict__ start
     if (i < s.N && s.rigid_membership[i] == NOT_MEMBER) beads[start[s.iso[i]]] = (unsigned)i;
 }
 
-// One 64-thread workgroup per body; thread i owns row i of [G | I] in LDS.
-__global__ __launch_bounds__(64) void k_block_setup(int nmax, const unsigned* __restrict__ start,
-                                                    const unsigned* __restrict__ beads,
-                                                    const double4* __restrict__ pos, double Lx, double Ly,
-                                                    double Lz, float* __restrict__ ginv, float* __restrict__ uvec,
-                                                    double* __restrict__ inv_s)
-{
-    extern __shared__ double aug[];
-    __shared__ double red[64];
-    const int t = blockIdx.x;
-    const int b0 = start[t];
-    const int n = start[t + 1] - b0;
-    const int i = threadIdx.x;
-    const int pitch = 2 * nmax;
-    if (n > nmax || n > 64) {  // body larger than the workspace: poison the solve
-        if (i == 0) inv_s[t] = __builtin_nan("");
-        return;
-    }
-    if (i < n) {
-        const double4 pi = pos[beads[b0 + i]];
-        for (int j = 0; j < n; ++j) {
-            double g;
-            if (j == i) {
-                g = shell_shell(0.0);
-            } else {
-                const double4 pj = pos[beads[b0 + j]];
-                const double dx = min_image(pi.x - pj.x, Lx), dy = min_image(pi.y - pj.y, Ly),
-                             dz = min_image(pi.z - pj.z, Lz);
-                g = shell_shell(sqrt(dx * dx + dy * dy + dz * dz));
-            }
-            aug[i * pitch + j] = g;
-            aug[i * pitch + n + j] = (i == j) ? 1.0 : 0.0;
-        }
-    }
-    __syncthreads();
-    for (int k = 0; k < n; ++k) {
-        if (i == k) {
-            const double inv = 1.0 / aug[k * pitch + k];
-            for (int j = 0; j < 2 * n; ++j) aug[k * pitch + j] *= inv;
-        }
-        __syncthreads();
-        if (i < n && i != k) {
-            const double f = aug[i * pitch + k];
-            for (int j = 0; j < 2 * n; ++j) aug[i * pitch + j] -= f * aug[k * pitch + j];
-        }
-        __syncthreads();
-    }
-    double u = 0.0;
-    if (i < n) {
-        for (int j = 0; j < n; ++j) {
-            const double v = aug[i * pitch + n + j];
-            ginv[ginv_at(t, nmax, i, j)] = (float)v;
-            u += v;
-        }
-        uvec[(long)t * nmax + i] = (float)u;
-    }
-    red[i] = u;
-    __syncthreads();
-    if (i == 0) {
-        double s = 0.0;
-        for (int j = 0; j < n; ++j) s += red[j];
-        inv_s[t] = 1.0 / s;
-    }
-}
-
 __device__ __forceinline__ double lane_bcast(double v, int l)
 {
     const long long b = __double_as_longlong(v);
@@ -155,9 +90,9 @@ __device__ __forceinline__ double lane_bcast(double v, int l)
 // symmetric positive definite), the pivot row broadcast element by element with
 // readlane -- no LDS, no barriers.  Rows and columns n..NB-1 are padded with the
 // identity, which the elimination leaves untouched.  Four bodies per 256-thread
-// workgroup.
+// workgroup.  Body t's inverse, u and 1/s go to entry o of ginv, uvec and inv_s.
 template <int NB>
-__device__ __forceinline__ void block_setup_body(int t, int nmax, const unsigned* __restrict__ start,
+__device__ __forceinline__ void block_setup_body(int t, int o, int nmax, const unsigned* __restrict__ start,
                                                  const unsigned* __restrict__ beads,
                                                  const double4* __restrict__ pos, double Lx, double Ly, double Lz,
                                                  float* __restrict__ ginv, float* __restrict__ uvec,
@@ -169,7 +104,7 @@ __device__ __forceinline__ void block_setup_body(int t, int nmax, const unsigned
     if (n <= nlo) return;            // a launch for smaller bodies takes this one
     if (n > NB && !last) return;     // ... and one for larger bodies this one
     if (n > NB) {
-        if (i == 0) inv_s[t] = __builtin_nan("");
+        if (i == 0) inv_s[o] = __builtin_nan("");
         return;
     }
     double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -210,27 +145,29 @@ __device__ __forceinline__ void block_setup_body(int t, int nmax, const unsigned
 #pragma unroll
         for (int j = 0; j < NB; ++j)
             if (j < n) {
-                ginv[ginv_at(t, nmax, i, j)] = (float)a[j];
+                ginv[ginv_at(o, nmax, i, j)] = (float)a[j];
                 u += a[j];
             }
-        uvec[(long)t * nmax + i] = (float)u;
+        uvec[(long)o * nmax + i] = (float)u;
     }
     double ssum = 0.0;
 #pragma unroll
     for (int j = 0; j < NB; ++j) ssum += (j < n) ? lane_bcast(u, j) : 0.0;
-    if (i == 0) inv_s[t] = 1.0 / ssum;
+    if (i == 0) inv_s[o] = 1.0 / ssum;
 }
 
+// Slot s takes body order[s] (the block apply's processing order) and stores it at s.
 template <int NB>
-__global__ __launch_bounds__(256) void k_block_setup_wave(int B, int nmax, const unsigned* __restrict__ start,
+__global__ __launch_bounds__(256) void k_block_setup_wave(int B, int nmax, const unsigned* __restrict__ order,
+                                                          const unsigned* __restrict__ start,
                                                           const unsigned* __restrict__ beads,
                                                           const double4* __restrict__ pos, double Lx, double Ly,
                                                           double Lz, float* __restrict__ ginv,
                                                           float* __restrict__ uvec, double* __restrict__ inv_s)
 {
-    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (t >= B) return;  // whole wavefront
-    block_setup_body<NB>(t, nmax, start, beads, pos, Lx, Ly, Lz, ginv, uvec, inv_s);
+    const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (s >= B) return;  // whole wavefront
+    block_setup_body<NB>((int)order[s], s, nmax, start, beads, pos, Lx, Ly, Lz, ginv, uvec, inv_s);
 }
 
 // All bodies of a batch of boxes in one launch (body0 ascending over boxes).
@@ -248,7 +185,7 @@ __global__ __launch_bounds__(256) void k_block_setup_wave_batch(const BlockBox* 
         else hi = mid - 1;
     }
     const BlockBox& d = boxes[lo];
-    block_setup_body<NB>(g - d.body0, d.nmax, d.start, d.beads, d.pos, d.L[0], d.L[1], d.L[2],
+    block_setup_body<NB>(g - d.body0, g - d.body0, d.nmax, d.start, d.beads, d.pos, d.L[0], d.L[1], d.L[2],
                          const_cast<float*>(d.ginv), const_cast<float*>(d.uvec), const_cast<double*>(d.inv_s), nlo,
                          last != 0);
 }
@@ -308,13 +245,24 @@ __global__ void k_body_order_key(int B, int N, const unsigned* __restrict__ star
     key[b] = make_uint2(bk, atomicAdd(bucket_count + bk, 1u));
 }
 
-__global__ void k_body_order_place(int B, const uint2* __restrict__ key, const unsigned* __restrict__ bucket_start,
-                                   unsigned* __restrict__ order)
+// Body b to its slot: order[slot] = b, count[slot] = its bead count, members[slot * nmax + k]
+// = its k-th bead (kNoBead past the count; a body of more than nmax beads keeps nmax).
+constexpr unsigned kNoBead = 0xffffffffu;
+
+__global__ void k_body_order_place(int B, int nmax, const uint2* __restrict__ key,
+                                   const unsigned* __restrict__ bucket_start, const unsigned* __restrict__ start,
+                                   const unsigned* __restrict__ beads, unsigned* __restrict__ order,
+                                   unsigned* __restrict__ count, unsigned* __restrict__ members)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
     const uint2 kv = key[b];
-    order[bucket_start[kv.x] + kv.y] = (unsigned)b;
+    const unsigned s = bucket_start[kv.x] + kv.y;
+    const unsigned lo = start[b], n = start[b + 1] - lo;
+    order[s] = (unsigned)b;
+    count[s] = n;
+    unsigned* m = members + (long)s * nmax;
+    for (int k = 0; k < nmax; ++k) m[k] = (unsigned)k < n ? beads[lo + k] : kNoBead;
 }
 
 // Workgroup b's place in an XCD-local order: consecutive workgroups go to different XCDs
@@ -327,42 +275,39 @@ __device__ __forceinline__ int xcd_local_place(int b, int nwg)
 }
 
 // Single system: K = 64 / nmax bodies share a wavefront (lane = body slot * nmax + bead),
-// four wavefronts per workgroup, so small bodies do not leave most lanes idle.  order:
-// the bodies in processing order (k_body_order_*), taken by XCD-local workgroup places;
-// null for ascending body index.
+// four wavefronts per workgroup, so small bodies do not leave most lanes idle.  The bodies
+// are taken in slot order (launch_block_order) by XCD-local workgroup places; the inverse
+// blocks, u, 1/s, the bead counts and member lists are stored by slot, so that a
+// workgroup's bodies are contiguous in each and a lane's bead index is one load away.
 template <int R4>
-__global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int K, const unsigned* __restrict__ start,
-                                                     const unsigned* __restrict__ beads,
+__global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int K, const unsigned* __restrict__ order,
+                                                     const unsigned* __restrict__ count,
+                                                     const unsigned* __restrict__ members,
                                                      const float* __restrict__ ginv, const float* __restrict__ uvec,
                                                      const double* __restrict__ inv_s, const double* __restrict__ y,
-                                                     double* __restrict__ w, const double* __restrict__ xq,
-                                                     const unsigned* __restrict__ order)
+                                                     double* __restrict__ w, const double* __restrict__ xq)
 {
     __shared__ double ys[256], red[256], cs[256], qs[256];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int k = lane / nmax, i = lane - k * nmax;
     const int slot = (xcd_local_place(blockIdx.x, gridDim.x) * 4 + wave) * K + k;
     const bool live = k < K && slot < B;
-    const int t = !live ? 0 : (order ? (int)order[slot] : slot);
     // R4 > 0 (nmax = 4 R4): the row of G^-1 does not depend on y, so its loads go out
-    // first and overlap the start -> beads -> y chain
+    // first and overlap the members -> y chain
     float4 g4[R4 > 0 ? R4 : 1];
     if constexpr (R4 > 0) {
-        const float4* r4 = reinterpret_cast<const float4*>(ginv + (long)t * nmax * nmax) + i;
+        const float4* r4 = reinterpret_cast<const float4*>(ginv + (long)slot * nmax * nmax) + i;
 #pragma unroll
         for (int m = 0; m < R4; ++m) g4[m] = live ? r4[(long)m * nmax] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    unsigned b0 = 0;
-    int n = 0;
-    if (live) {
-        b0 = start[t];
-        n = (int)(start[t + 1] - b0);
-    }
+    const unsigned mb = live ? members[(long)slot * nmax + i] : kNoBead;
+    const int n = live ? (int)count[slot] : 0;
+    const int t = live ? (int)order[slot] : 0;  // the body's own index (its constraint row)
     const bool bad = live && n > nmax;  // body larger than the workspace: poison the solve
     const bool act = live && !bad && i < n;
-    const unsigned bead = act ? beads[b0 + i] : 0u;
+    const unsigned bead = act ? mb : 0u;
     const double yi = act ? y[bead] : 0.0;
-    const double ui = act ? (double)uvec[(long)t * nmax + i] : 0.0;
+    const double ui = act ? (double)uvec[(long)slot * nmax + i] : 0.0;
     const int base = wave * 64 + k * nmax;  // this body's slots
     ys[threadIdx.x] = yi;
     red[threadIdx.x] = ui * yi;
@@ -384,7 +329,7 @@ __global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int
         if ((nmax & 3) == 0) {
             // four entries per load from the 4-column panels; entries past n are never
             // written by the setup and are masked
-            const float4* r4 = reinterpret_cast<const float4*>(ginv + (long)t * nmax * nmax) + i;
+            const float4* r4 = reinterpret_cast<const float4*>(ginv + (long)slot * nmax * nmax) + i;
             for (int j = 0; j < n; j += 4) {
                 const float4 g = r4[(long)(j >> 2) * nmax];
                 gy += (double)g.x * ys[base + j];
@@ -393,7 +338,7 @@ __global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int
                 if (j + 3 < n) gy += (double)g.w * ys[base + j + 3];
             }
         } else {
-            for (int j = 0; j < n; ++j) gy += (double)ginv[ginv_at(t, nmax, i, j)] * ys[base + j];
+            for (int j = 0; j < n; ++j) gy += (double)ginv[ginv_at(slot, nmax, i, j)] * ys[base + j];
         }
     }
     if (live && i == 0) {
@@ -406,7 +351,7 @@ __global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int
             for (int j = 0; j < n; ++j) yc += qs[base + j];
         else
             yc = y[N + t];
-        cs[base] = bad ? __builtin_nan("") : (dot + yc) * inv_s[t];
+        cs[base] = bad ? __builtin_nan("") : (dot + yc) * inv_s[slot];
     }
     __syncthreads();
     if (live) {
@@ -446,7 +391,7 @@ __global__ void k_zero_u32(unsigned* p, long n)
 hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, unsigned* body_count,
                               unsigned* body_start, unsigned* body_beads,
                               unsigned long long* scan_sums, float* ginv, float* uvec, double* inv_s, hipStream_t st,
-                              bool invert)
+                              bool invert, const BlockSlots* slots)
 {
     const int B = s.N_isolated;
     if (B <= 0) return hipSuccess;
@@ -462,10 +407,20 @@ hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, uns
     if (s.N > 0)
         hipLaunchKernelGGL(k_body_fill_single, dim3(blocks(s.N, 256)), dim3(256), 0, st, s, body_start, body_beads);
     if (!invert) return hipGetLastError();
+    if (!slots || nmax > 64) return hipErrorInvalidValue;
+    // the apply's slot order first: the inverses are stored by slot
+    const long nbk = block_order_buckets(s.N);
+    hipLaunchKernelGGL(k_zero_u32, dim3(blocks(nbk, 256)), dim3(256), 0, st, slots->bucket_count, nbk);
+    hipLaunchKernelGGL(k_body_order_key, dim3(blocks(B, 256)), dim3(256), 0, st, B, s.N, body_start, body_beads,
+                       slots->bucket_count, slots->key);
+    e = exclusive_scan_u32(slots->bucket_count, slots->bucket_start, nbk, slots->scan_sums, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_body_order_place, dim3(blocks(B, 256)), dim3(256), 0, st, B, nmax, slots->key,
+                       slots->bucket_start, body_start, body_beads, slots->order, slots->count, slots->members);
     const unsigned wb = (unsigned)((B + 3) / 4);
-#define CR_BLOCK_WAVE(NB)                                                                                   \
-    hipLaunchKernelGGL(k_block_setup_wave<NB>, dim3(wb), dim3(256), 0, st, B, nmax, body_start, body_beads, \
-                       s.pos, L[0], L[1], L[2], ginv, uvec, inv_s)
+#define CR_BLOCK_WAVE(NB)                                                                                     \
+    hipLaunchKernelGGL(k_block_setup_wave<NB>, dim3(wb), dim3(256), 0, st, B, nmax, slots->order, body_start, \
+                       body_beads, s.pos, L[0], L[1], L[2], ginv, uvec, inv_s)
     if (nmax <= 8) {
         CR_BLOCK_WAVE(8);
     } else if (nmax <= 16) {
@@ -478,44 +433,26 @@ hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, uns
         CR_BLOCK_WAVE(32);
     } else if (nmax <= 40) {
         CR_BLOCK_WAVE(40);
-    } else if (nmax <= 64) {
-        CR_BLOCK_WAVE(64);
     } else {
-        const size_t lds = sizeof(double) * (size_t)nmax * 2 * nmax;
-        hipLaunchKernelGGL(k_block_setup, dim3(B), dim3(64), lds, st, nmax, body_start, body_beads, s.pos, L[0],
-                           L[1], L[2], ginv, uvec, inv_s);
+        CR_BLOCK_WAVE(64);
     }
 #undef CR_BLOCK_WAVE
     return hipGetLastError();
 }
 
-hipError_t launch_block_order(int B, int N, const unsigned* body_start, const unsigned* body_beads,
-                              unsigned* bucket_count, unsigned* bucket_start, unsigned long long* scan_sums,
-                              uint2* key, unsigned* order, hipStream_t st)
-{
-    if (B <= 0) return hipSuccess;
-    const long nbk = block_order_buckets(N);
-    hipLaunchKernelGGL(k_zero_u32, dim3(blocks(nbk, 256)), dim3(256), 0, st, bucket_count, nbk);
-    hipLaunchKernelGGL(k_body_order_key, dim3(blocks(B, 256)), dim3(256), 0, st, B, N, body_start, body_beads,
-                       bucket_count, key);
-    hipError_t e = exclusive_scan_u32(bucket_count, bucket_start, nbk, scan_sums, st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_body_order_place, dim3(blocks(B, 256)), dim3(256), 0, st, B, key, bucket_start, order);
-    return hipGetLastError();
-}
-
 long block_order_buckets(int N) { return ((long)N >> kOrderShift) + 1; }
 
-hipError_t launch_block_apply(const SysView& s, int nmax, const unsigned* body_start, const unsigned* body_beads,
-                              const float* ginv, const float* uvec, const double* inv_s, const double* y, double* w,
-                              hipStream_t st, const double* xq, const unsigned* order)
+hipError_t launch_block_apply(const SysView& s, int nmax, const BlockSlots& slots, const float* ginv,
+                              const float* uvec, const double* inv_s, const double* y, double* w, hipStream_t st,
+                              const double* xq)
 {
     if (s.N_isolated <= 0) return hipSuccess;
-    const int K = nmax <= 64 ? 64 / nmax : 1;
+    if (nmax > 64) return hipErrorInvalidValue;
+    const int K = 64 / nmax;
     const unsigned nb = (unsigned)((s.N_isolated + 4 * K - 1) / (4 * K));
-#define CR_BLOCK_APPLY(R4)                                                                                     \
-    hipLaunchKernelGGL(k_block_apply<R4>, dim3(nb), dim3(256), 0, st, s.N_isolated, s.N, nmax, K, body_start,  \
-                       body_beads, ginv, uvec, inv_s, y, w, xq, order)
+#define CR_BLOCK_APPLY(R4)                                                                                       \
+    hipLaunchKernelGGL(k_block_apply<R4>, dim3(nb), dim3(256), 0, st, s.N_isolated, s.N, nmax, K, slots.order,   \
+                       slots.count, slots.members, ginv, uvec, inv_s, y, w, xq)
     switch (nmax) {
         case 8: CR_BLOCK_APPLY(2); break;
         case 12: CR_BLOCK_APPLY(3); break;
