@@ -667,18 +667,22 @@ int set_box(cr_context* c, const double L[3], hipStream_t st)
     for (int d = 0; d < 3; ++d)
         CR_HIP(launch_kspace_axis(Nd[d], L[d], xi, r.eta[d], c->ktab[d].as<double>(), c->etab[d].as<double>(), st));
     c->sc.radial = nullptr;
+    c->sc.radial_gauss = 0;
     if (L[0] == L[1] && L[1] == L[2]) {
         long m2max = 0;
         for (int d = 0; d < 3; ++d) m2max += (long)(Nd[d] / 2) * (Nd[d] / 2);
         const double kappa = 2.0 * kPi / L[0];
+        const bool same_eta = r.eta[0] == r.eta[1] && r.eta[1] == r.eta[2];
+        const double gauss = same_eta ? (1.0 - r.eta[0]) * kappa * kappa / (4.0 * xi * xi) : -1.0;
         if (c->fp32()) {
             CR_HIP(c->radtab.ensure(sizeof(float) * (size_t)(m2max + 1)));
-            CR_HIP(launch_kspace_radial<float>(m2max, kappa, c->sc.invG, c->radtab.as<float>(), st));
+            CR_HIP(launch_kspace_radial<float>(m2max, kappa, c->sc.invG, c->radtab.as<float>(), st, gauss));
         } else {
             CR_HIP(c->radtab.ensure(sizeof(double) * (size_t)(m2max + 1)));
-            CR_HIP(launch_kspace_radial<double>(m2max, kappa, c->sc.invG, c->radtab.as<double>(), st));
+            CR_HIP(launch_kspace_radial<double>(m2max, kappa, c->sc.invG, c->radtab.as<double>(), st, gauss));
         }
         c->sc.radial = c->radtab.p;
+        c->sc.radial_gauss = same_eta ? 1 : 0;
     }
     c->fx_bound = false;
     ++c->generation;
@@ -1238,23 +1242,29 @@ int cr_set_params(cr_context* c, const cr_params* p, int n_beads, int n_isolated
     c->sc.invG = 1.0 / (double)c->G;
     for (int d = 0; d < 3; ++d) c->sc.n[d] = Nd[d];
     c->sc.radial = nullptr;
+    c->sc.radial_gauss = 0;
     if (p->L[0] == p->L[1] && p->L[1] == p->L[2]) {
-        // S = j0(|k|)^2 / k^2 * e_x e_y e_z / G with k^2 = kappa^2 m2 (ConductorRigid.cc:258-268)
+        // S = j0(|k|)^2 / k^2 * e_x e_y e_z / G with k^2 = kappa^2 m2 (ConductorRigid.cc:258-268);
+        // with eta the same on every axis e_x e_y e_z = exp(-(1-eta) k^2 / (4 xi^2)) joins the table
         long m2max = 0;
         for (int d = 0; d < 3; ++d) m2max += (long)(Nd[d] / 2) * (Nd[d] / 2);
         const double kappa = 2.0 * kPi / p->L[0];
+        const bool same_eta = r.eta[0] == r.eta[1] && r.eta[1] == r.eta[2];
+        const double gauss = (1.0 - r.eta[0]) * kappa * kappa / (4.0 * xi * xi);
         std::vector<double> rad((size_t)m2max + 1);
         rad[0] = 0.0;  // S(0) = 0
         for (long m2 = 1; m2 <= m2max; ++m2) {
             const double k2 = kappa * kappa * (double)m2, k = std::sqrt(k2);
             const double j0 = std::sin(k) / k;
-            rad[(size_t)m2] = j0 * j0 / k2 * c->sc.invG;
+            const double g = same_eta ? std::exp(-gauss * (double)m2) : 1.0;
+            rad[(size_t)m2] = j0 * j0 / k2 * c->sc.invG * g;
         }
         if (c->fp32())
             CR_HIP(upload<float>(c->radtab, rad));
         else
             CR_HIP(upload<double>(c->radtab, rad));
         c->sc.radial = c->radtab.p;
+        c->sc.radial_gauss = same_eta ? 1 : 0;
     }
     // binned spread/contract tiling: per axis the largest divisor of the mesh size in
     // [max(P-1, 4), 16], else the smallest one in (16, 32]; the tile extended by P-1

@@ -301,8 +301,9 @@ hipError_t launch_convert(const Tin* in, Tout* out, long n, hipStream_t st);
 // k-space factor tables regenerated on the device after a box change (k_se.hip): one
 // axis's wavenumbers and Gaussian factors, and the cubic-box radial table.
 hipError_t launch_kspace_axis(int n, double L, double xi, double eta, double* k, double* e, hipStream_t st);
+// gauss >= 0: each entry also times exp(-gauss m2) (the Gaussian factors of the three axes).
 template <typename T>
-hipError_t launch_kspace_radial(long m2max, double kappa, double invG, T* rad, hipStream_t st);
+hipError_t launch_kspace_radial(long m2max, double kappa, double invG, T* rad, hipStream_t st, double gauss = -1.0);
 
 // ---- k_fft.hip ----
 // Real mesh [Nx][Ny][Nz] <-> half spectrum [Nx][Ny][Nz/2+1] (interleaved complex) with rows
@@ -329,6 +330,9 @@ struct ScaleTables {
     // by the register x pass instead of a square root, sine and two divisions per mode.
     // Null for other boxes.
     const void* radial;
+    // radial also holds the Gaussian factor e_x e_y e_z = exp(-(1-eta) k^2 / (4 xi^2)) (eta
+    // the same on every axis), so that S = radial[m2] alone
+    int radial_gauss;
     int n[3];              // mesh size per axis (signed frequency m of index i)
 };
 template <typename T>

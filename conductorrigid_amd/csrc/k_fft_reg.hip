@@ -394,8 +394,9 @@ template <>
 __device__ __forceinline__ double sin_t<double>(double x) { return sin(x); }
 
 // Forward x FFT, S(k) = j0(|k|)^2 exp(-sum_d (1-eta_d) k_d^2/4xi^2)/|k|^2/G (S(0) = 0;
-// cppmodule/ConductorRigid.cc:253-268), inverse x FFT, on the half spectrum.
-template <typename T, int A, int B>
+// cppmodule/ConductorRigid.cc:253-268), inverse x FFT, on the half spectrum.  RG: the radial
+// table holds all of S (sc.radial_gauss), one table load per mode.
+template <typename T, int A, int B, bool RG = false>
 __device__ __forceinline__ void x_scale_pass(int Ny, int Zc, int Zp, typename Cx<T>::t* __restrict__ data,
                                              const typename Cx<T>::t* __restrict__ tw, const ScaleTables& sc)
 {
@@ -417,9 +418,48 @@ __device__ __forceinline__ void x_scale_pass(int Ny, int Zc, int Zp, typename Cx
     }
     four_step_cols<A, B, C, -1>(v, lds, tw, c, l);
     // thread (c, l) holds X[(l + B s) + A k2] in v[s B + k2]: scale by S(kx, ky, kz)
+    if constexpr (RG) {
+        const T* rad = static_cast<const T*>(sc.radial);
+        const int my = L.line < (sc.n[1] + 1) / 2 ? L.line : L.line - sc.n[1];
+        const int zi = ok ? z : 0;  // padding lanes index the table in range too
+        const int mz = zi < (sc.n[2] + 1) / 2 ? zi : zi - sc.n[2];
+        const int myz = my * my + mz * mz;
+#pragma unroll
+        for (int s = 0; s < m; ++s)
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                const int x = l + B * s + A * j;
+                const int mx = x < (sc.n[0] + 1) / 2 ? x : x - sc.n[0];
+                const T S = ok ? rad[myz + mx * mx] : (T)0;
+                v[s * B + j].x *= S;
+                v[s * B + j].y *= S;
+            }
+        __syncthreads();  // the transpose buffer is reused
+        four_step_cols_inv_from_X<A, B, C>(v, lds, tw, c, l);
+        if (!ok) return;
+#pragma unroll
+        for (int q = 0; q < A; ++q) data[L.base + c + (long)(l + B * q) * L.stride] = v[q];
+        return;
+    }
     const double ky = sc.k[1][L.line], ey = sc.e[1][L.line];
     const double kz = ok ? sc.k[2][z] : 0.0, ez = ok ? sc.e[2][z] : 0.0;
-    if (sc.radial) {
+    if (sc.radial && sc.radial_gauss) {
+        const T* rad = static_cast<const T*>(sc.radial);
+        const int my = L.line < (sc.n[1] + 1) / 2 ? L.line : L.line - sc.n[1];
+        const int zi = ok ? z : 0;
+        const int mz = zi < (sc.n[2] + 1) / 2 ? zi : zi - sc.n[2];
+        const int myz = my * my + mz * mz;
+#pragma unroll
+        for (int s = 0; s < m; ++s)
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                const int x = l + B * s + A * j;
+                const int mx = x < (sc.n[0] + 1) / 2 ? x : x - sc.n[0];
+                const T S = ok ? rad[myz + mx * mx] : (T)0;
+                v[s * B + j].x *= S;
+                v[s * B + j].y *= S;
+            }
+    } else if (sc.radial) {
         const T* rad = static_cast<const T*>(sc.radial);
         const int my = L.line < (sc.n[1] + 1) / 2 ? L.line : L.line - sc.n[1];
         const int zi = ok ? z : 0;  // padding lanes index the table in range too
@@ -461,13 +501,13 @@ __device__ __forceinline__ void x_scale_pass(int Ny, int Zc, int Zp, typename Cx
     for (int q = 0; q < A; ++q) data[L.base + c + (long)(l + B * q) * L.stride] = v[q];
 }
 
-template <typename T, int A, int B>
+template <typename T, int A, int B, bool RG>
 __global__ __launch_bounds__((B * cols_per_wg<T, A, B>())) void k_fftr_x_scale(int Ny, int Zc, int Zp,
                                                                               typename Cx<T>::t* __restrict__ data,
                                                                               const typename Cx<T>::t* __restrict__ tw,
                                                                               ScaleTables sc)
 {
-    x_scale_pass<T, A, B>(Ny, Zc, Zp, data, tw, sc);
+    x_scale_pass<T, A, B, RG>(Ny, Zc, Zp, data, tw, sc);
 }
 
 // Boxes of one mesh stored back to back ([box][Nx][Ny][Zc]): blockIdx.y picks the box
@@ -538,8 +578,12 @@ hipError_t x_scale(int Ny, int Zc, int Zp, T* data, const void* tw, const ScaleT
     using T2 = typename Cx<T>::t;
     constexpr int C = cols_per_wg<T, A, B>();
     const unsigned blocks = (unsigned)(Ny * ((Zc + C - 1) / C));
-    hipLaunchKernelGGL((k_fftr_x_scale<T, A, B>), dim3(blocks), dim3(B * C), 0, st, Ny, Zc, Zp, (T2*)data,
-                       (const T2*)tw, sc);
+    if (sc.radial && sc.radial_gauss)
+        hipLaunchKernelGGL((k_fftr_x_scale<T, A, B, true>), dim3(blocks), dim3(B * C), 0, st, Ny, Zc, Zp, (T2*)data,
+                           (const T2*)tw, sc);
+    else
+        hipLaunchKernelGGL((k_fftr_x_scale<T, A, B, false>), dim3(blocks), dim3(B * C), 0, st, Ny, Zc, Zp,
+                           (T2*)data, (const T2*)tw, sc);
     return hipGetLastError();
 }
 

@@ -263,9 +263,10 @@ __global__ void k_kspace_axis(int n, double L, double xi, double eta, double* __
     et[i] = exp(-(1.0 - eta) * k * k / (4.0 * xi * xi));
 }
 
-// Cubic boxes: j0(|k|)^2 / k^2 / G over m2 = |m|^2, k^2 = kappa^2 m2 (0 at m2 = 0).
+// Cubic boxes: j0(|k|)^2 / k^2 / G over m2 = |m|^2, k^2 = kappa^2 m2 (0 at m2 = 0), times
+// exp(-gauss m2) when gauss >= 0.
 template <typename T>
-__global__ void k_kspace_radial(long m2max, double kappa, double invG, T* __restrict__ rad)
+__global__ void k_kspace_radial(long m2max, double kappa, double invG, T* __restrict__ rad, double gauss)
 {
     const long m2 = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (m2 > m2max) return;
@@ -275,7 +276,8 @@ __global__ void k_kspace_radial(long m2max, double kappa, double invG, T* __rest
     }
     const double k2 = kappa * kappa * (double)m2, k = sqrt(k2);
     const double j0 = sin(k) / k;
-    rad[m2] = (T)(j0 * j0 / k2 * invG);
+    const double g = gauss >= 0.0 ? exp(-gauss * (double)m2) : 1.0;
+    rad[m2] = (T)(j0 * j0 / k2 * invG * g);
 }
 
 hipError_t launch_kspace_axis(int n, double L, double xi, double eta, double* k, double* e, hipStream_t st)
@@ -285,14 +287,14 @@ hipError_t launch_kspace_axis(int n, double L, double xi, double eta, double* k,
 }
 
 template <typename T>
-hipError_t launch_kspace_radial(long m2max, double kappa, double invG, T* rad, hipStream_t st)
+hipError_t launch_kspace_radial(long m2max, double kappa, double invG, T* rad, hipStream_t st, double gauss)
 {
     hipLaunchKernelGGL(k_kspace_radial<T>, dim3((unsigned)((m2max + 256) / 256)), dim3(256), 0, st, m2max, kappa,
-                       invG, rad);
+                       invG, rad, gauss);
     return hipGetLastError();
 }
-template hipError_t launch_kspace_radial<float>(long, double, double, float*, hipStream_t);
-template hipError_t launch_kspace_radial<double>(long, double, double, double*, hipStream_t);
+template hipError_t launch_kspace_radial<float>(long, double, double, float*, hipStream_t, double);
+template hipError_t launch_kspace_radial<double>(long, double, double, double*, hipStream_t, double);
 
 template hipError_t launch_fill<float>(float*, float, long, hipStream_t);
 template hipError_t launch_fill<double>(double*, double, long, hipStream_t);
