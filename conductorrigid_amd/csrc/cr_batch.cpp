@@ -403,6 +403,7 @@ int cr_batch_compute_force(cr_batch* B, int nbox, cr_context* const* ctx, const 
             for (int k = 0; k < 3; ++k) d.E0[k] = c->E0[k];
             if (block_precond(c)) {  // body grouping here, the inverses in one launch below
                 CR_RET(block_alloc(c, v));
+                c->block_cache_ok = false;  // the batched setup stores the inverses by body, not by slot
                 d.body_count = c->body_count.as<unsigned>();
                 d.body_start = c->body_start.as<unsigned>();
                 d.body_beads = c->body_beads.as<unsigned>();

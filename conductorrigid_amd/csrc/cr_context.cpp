@@ -553,8 +553,10 @@ int block_alloc(cr_context* c, const SysView& v)
     const int B = v.N_isolated;
     const int nmax = std::max(1, v.Nmax);
     if (nmax > 64) return set_error(CR_ERR_INVALID, "near-field block preconditioner supports bodies of <= 64 beads");
+    if (nmax != c->block_nmax) c->block_cache_ok = false;
     c->block_nmax = nmax;
     const size_t b1 = (size_t)std::max(1, B);
+    const void* kept[3] = {c->ginv.p, c->bcache_key.p, c->bcache_sig.p};
     CR_HIP(c->body_count.ensure(sizeof(unsigned) * (b1 + 1)));
     CR_HIP(c->body_start.ensure(sizeof(unsigned) * (b1 + 1)));
     CR_HIP(c->body_beads.ensure(sizeof(unsigned) * (size_t)std::max(1, v.N)));
@@ -562,14 +564,19 @@ int block_alloc(cr_context* c, const SysView& v)
     CR_HIP(c->uvec.ensure(sizeof(float) * b1 * nmax));
     CR_HIP(c->inv_s.ensure(sizeof(double) * b1));
     CR_HIP(c->block_scan.ensure(sizeof(unsigned long long) * ((b1 + 1) / 1024 + 4)));
-    const size_t nbk = (size_t)block_order_buckets(std::max(0, v.N));
+    const size_t nbk = (size_t)block_order_buckets(std::max(0, v.N)) + 1;
     CR_HIP(c->body_order.ensure(sizeof(unsigned) * b1));
     CR_HIP(c->slot_count.ensure(sizeof(unsigned) * b1));
     CR_HIP(c->slot_members.ensure(sizeof(unsigned) * b1 * nmax));
     CR_HIP(c->order_key.ensure(sizeof(uint2) * b1));
+    CR_HIP(c->order_mkey.ensure(sizeof(unsigned) * b1));
     CR_HIP(c->order_count.ensure(sizeof(unsigned) * nbk));
     CR_HIP(c->order_start.ensure(sizeof(unsigned) * nbk));
     CR_HIP(c->order_scan.ensure(sizeof(unsigned long long) * (nbk / 1024 + 4)));
+    CR_HIP(c->bcache_key.ensure(sizeof(unsigned long long) * b1));
+    CR_HIP(c->bcache_sig.ensure(sizeof(double) * 2 * b1));
+    // new storage holds no inverses (the batched path's setup marks them gone as well)
+    if (c->ginv.p != kept[0] || c->bcache_key.p != kept[1] || c->bcache_sig.p != kept[2]) c->block_cache_ok = false;
     return CR_OK;
 }
 
@@ -577,7 +584,9 @@ BlockSlots block_slots(cr_context* c)
 {
     return BlockSlots{c->body_order.as<unsigned>(), c->slot_count.as<unsigned>(), c->slot_members.as<unsigned>(),
                       c->order_count.as<unsigned>(), c->order_start.as<unsigned>(),
-                      c->order_scan.as<unsigned long long>(), c->order_key.as<uint2>()};
+                      c->order_scan.as<unsigned long long>(), c->order_key.as<uint2>(), c->order_mkey.as<unsigned>(),
+                      c->bcache_key.as<unsigned long long>(), c->bcache_sig.as<double>(),
+                      c->block_cache_ok ? 1 : 0};
 }
 
 int block_setup(cr_context* c, const SysView& v, hipStream_t st, bool invert)
@@ -589,6 +598,7 @@ int block_setup(cr_context* c, const SysView& v, hipStream_t st, bool invert)
                               c->body_beads.as<unsigned>(),
                               c->block_scan.as<unsigned long long>(), c->ginv.as<float>(), c->uvec.as<float>(),
                               c->inv_s.as<double>(), st, invert, &slots));
+    if (invert) c->block_cache_ok = true;  // every slot's entry now describes its stored inverse
     return CR_OK;
 }
 

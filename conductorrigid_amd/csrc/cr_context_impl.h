@@ -64,7 +64,10 @@ struct cr_context {
     DevBuf body_count, body_start, body_beads, ginv, uvec, inv_s, block_scan;
     // the block preconditioner's slots (processing order of the bodies, their bead counts
     // and member lists) and their counting-sort scratch
-    DevBuf body_order, slot_count, slot_members, order_key, order_count, order_start, order_scan;
+    DevBuf body_order, slot_count, slot_members, order_key, order_mkey, order_count, order_start, order_scan;
+    // per slot: the body, bead count, bead-index sum and shape signature of the stored inverse
+    DevBuf bcache_key, bcache_sig;
+    bool block_cache_ok = false;
     // Independent parts of a force evaluation run beside the caller's stream on these
     // (default priority): the real-space ELL build and the block preconditioner's setup
     // next to the binning, and the real-space force next to the force pass's mesh work,

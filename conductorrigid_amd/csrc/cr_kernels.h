@@ -404,11 +404,13 @@ hipError_t launch_zero_force(int N, double* force, hipStream_t st);
 
 // ---- k_precond.hip ----
 // Near-field block preconditioner: per isolated body, the inverse of its own saddle block.
-// The single-system form keeps its bodies in slot order (ascending smallest bead index, in
-// buckets of 32 indices): order[slot] is the body, count[slot] its bead count and
-// members[slot * nmax + k] its k-th bead; ginv, uvec and inv_s are stored by slot.
-// Scratch: bucket_count and bucket_start of block_order_buckets(N) entries, scan_sums of
-// block_order_buckets(N) / 1024 + 4, key of B.
+// The single-system form keeps its bodies in slot order (ascending smallest bead index):
+// order[slot] is the body, count[slot] its bead count and members[slot * nmax + k] its k-th
+// bead; ginv, uvec and inv_s are stored by slot.  A slot's inverse is kept from the previous
+// setup when the slot holds the same body, beads and shape (cache_key: B entries, cache_sig:
+// 2B, cache_ok: they describe the stored inverses).  Scratch: bucket_count and bucket_start
+// of block_order_buckets(N) + 1 entries, scan_sums of block_order_buckets(N) / 1024 + 4, key
+// and mkey of B.
 struct BlockSlots {
     unsigned* order;
     unsigned* count;
@@ -417,6 +419,10 @@ struct BlockSlots {
     unsigned* bucket_start;
     unsigned long long* scan_sums;
     uint2* key;
+    unsigned* mkey;
+    unsigned long long* cache_key;
+    double* cache_sig;
+    int cache_ok;
 };
 long block_order_buckets(int N);
 // Bodies grouped (body_start, body_beads); with invert, the slots and the per-slot inverses

@@ -77,6 +77,14 @@ __global__ void k_body_fill_single(SysView s, const unsigned* __restrict__ start
     if (i < s.N && s.rigid_membership[i] == NOT_MEMBER) beads[start[s.iso[i]]] = (unsigned)i;
 }
 
+// Sum over the 64 lanes (xor butterfly: every lane ends with the same bits).
+__device__ __forceinline__ double wave_sum64(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 __device__ __forceinline__ double lane_bcast(double v, int l)
 {
     const long long b = __double_as_longlong(v);
@@ -91,12 +99,25 @@ __device__ __forceinline__ double lane_bcast(double v, int l)
 // readlane -- no LDS, no barriers.  Rows and columns n..NB-1 are padded with the
 // identity, which the elimination leaves untouched.  Four bodies per 256-thread
 // workgroup.  Body t's inverse, u and 1/s go to entry o of ginv, uvec and inv_s.
+//
+// cache (single system): G depends on the body's shape only, which a rigid body keeps.  Each
+// entry o remembers the body, its bead count, a weighted sum of its bead indices and its
+// shape signature (the sum of its beads' squared distances from their centroid, invariant
+// under rigid motion); when all match the stored inverse is kept.  The preconditioner only
+// steers GMRES, never the converged answer.
+struct BlockCache {
+    unsigned long long* key;  // t | n << 32 per entry
+    double* sig;              // (shape signature, bead-index sum) per entry
+    int ok;                   // the entries describe the inverses now stored
+};
+
 template <int NB>
 __device__ __forceinline__ void block_setup_body(int t, int o, int nmax, const unsigned* __restrict__ start,
                                                  const unsigned* __restrict__ beads,
                                                  const double4* __restrict__ pos, double Lx, double Ly, double Lz,
                                                  float* __restrict__ ginv, float* __restrict__ uvec,
-                                                 double* __restrict__ inv_s, int nlo = 0, bool last = true)
+                                                 double* __restrict__ inv_s, int nlo = 0, bool last = true,
+                                                 const BlockCache* cache = nullptr)
 {
     const int i = threadIdx.x & 63;
     const int b0 = start[t];
@@ -108,7 +129,24 @@ __device__ __forceinline__ void block_setup_body(int t, int o, int nmax, const u
         return;
     }
     double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (i < n) pi = pos[beads[b0 + i]];
+    const unsigned bead = i < n ? beads[b0 + i] : 0u;
+    if (i < n) pi = pos[bead];
+    double sig = 0.0, hsum = 0.0;
+    const unsigned long long key = (unsigned)t | ((unsigned long long)(unsigned)n << 32);
+    if (cache) {
+        // offsets from bead 0 (minimum image), their centroid, the spread about it
+        const double x0 = lane_bcast(pi.x, 0), y0 = lane_bcast(pi.y, 0), z0 = lane_bcast(pi.z, 0);
+        const double dx = i < n ? min_image(pi.x - x0, Lx) : 0.0;
+        const double dy = i < n ? min_image(pi.y - y0, Ly) : 0.0;
+        const double dz = i < n ? min_image(pi.z - z0, Lz) : 0.0;
+        const double cx = wave_sum64(dx) / n, cy = wave_sum64(dy) / n, cz = wave_sum64(dz) / n;
+        const double ex = dx - cx, ey = dy - cy, ez = dz - cz;
+        sig = wave_sum64(i < n ? ex * ex + ey * ey + ez * ez : 0.0);
+        hsum = wave_sum64(i < n ? (double)bead * (double)(i + 1) : 0.0);
+        if (cache->ok && cache->key[o] == key && cache->sig[2 * o + 1] == hsum &&
+            fabs(sig - cache->sig[2 * o]) <= 1e-9 * sig + 1e-300)
+            return;  // the same body, beads and shape: its stored inverse stands
+    }
     double a[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
@@ -153,7 +191,14 @@ __device__ __forceinline__ void block_setup_body(int t, int o, int nmax, const u
     double ssum = 0.0;
 #pragma unroll
     for (int j = 0; j < NB; ++j) ssum += (j < n) ? lane_bcast(u, j) : 0.0;
-    if (i == 0) inv_s[o] = 1.0 / ssum;
+    if (i == 0) {
+        inv_s[o] = 1.0 / ssum;
+        if (cache) {
+            cache->key[o] = key;
+            cache->sig[2 * o] = sig;
+            cache->sig[2 * o + 1] = hsum;
+        }
+    }
 }
 
 // Slot s takes body order[s] (the block apply's processing order) and stores it at s.
@@ -163,11 +208,13 @@ __global__ __launch_bounds__(256) void k_block_setup_wave(int B, int nmax, const
                                                           const unsigned* __restrict__ beads,
                                                           const double4* __restrict__ pos, double Lx, double Ly,
                                                           double Lz, float* __restrict__ ginv,
-                                                          float* __restrict__ uvec, double* __restrict__ inv_s)
+                                                          float* __restrict__ uvec, double* __restrict__ inv_s,
+                                                          BlockCache cache)
 {
     const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (s >= B) return;  // whole wavefront
-    block_setup_body<NB>((int)order[s], s, nmax, start, beads, pos, Lx, Ly, Lz, ginv, uvec, inv_s);
+    block_setup_body<NB>((int)order[s], s, nmax, start, beads, pos, Lx, Ly, Lz, ginv, uvec, inv_s, 0, true,
+                         cache.key ? &cache : nullptr);
 }
 
 // All bodies of a batch of boxes in one launch (body0 ascending over boxes).
@@ -234,35 +281,62 @@ __device__ __forceinline__ void block_apply_body(int t, int N, int nmax, const u
 constexpr int kOrderShift = 5;
 
 __global__ void k_body_order_key(int B, int N, const unsigned* __restrict__ start, const unsigned* __restrict__ beads,
-                                 unsigned* __restrict__ bucket_count, uint2* __restrict__ key)
+                                 unsigned* __restrict__ bucket_count, uint2* __restrict__ key,
+                                 unsigned* __restrict__ mkey)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
     const unsigned lo = start[b], hi = start[b + 1];
-    unsigned m = (unsigned)N;
+    unsigned m = (unsigned)N + (unsigned)b;  // a body without beads: past every bead, by index
     for (unsigned k = lo; k < hi; ++k) m = min(m, beads[k]);
-    const unsigned bk = m >> kOrderShift;
+    const unsigned bk = min(m, (unsigned)N) >> kOrderShift;
     key[b] = make_uint2(bk, atomicAdd(bucket_count + bk, 1u));
+    mkey[b] = m;
 }
 
-// Body b to its slot: order[slot] = b, count[slot] = its bead count, members[slot * nmax + k]
-// = its k-th bead (kNoBead past the count; a body of more than nmax beads keeps nmax).
-constexpr unsigned kNoBead = 0xffffffffu;
-
-__global__ void k_body_order_place(int B, int nmax, const uint2* __restrict__ key,
-                                   const unsigned* __restrict__ bucket_start, const unsigned* __restrict__ start,
-                                   const unsigned* __restrict__ beads, unsigned* __restrict__ order,
-                                   unsigned* __restrict__ count, unsigned* __restrict__ members)
+// Body b to a place in its bucket (in the atomics' order), then each bucket sorted by smallest
+// bead index (unique per body), so that the slots do not depend on the order of the atomics.
+__global__ void k_body_order_place(int B, const uint2* __restrict__ key, const unsigned* __restrict__ bucket_start,
+                                   unsigned* __restrict__ order)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
     const uint2 kv = key[b];
-    const unsigned s = bucket_start[kv.x] + kv.y;
-    const unsigned lo = start[b], n = start[b + 1] - lo;
-    order[s] = (unsigned)b;
-    count[s] = n;
-    unsigned* m = members + (long)s * nmax;
-    for (int k = 0; k < nmax; ++k) m[k] = (unsigned)k < n ? beads[lo + k] : kNoBead;
+    order[bucket_start[kv.x] + kv.y] = (unsigned)b;
+}
+
+__global__ void k_body_order_sort(long nbk, const unsigned* __restrict__ bucket_start, const unsigned* __restrict__ mkey,
+                                  unsigned* __restrict__ order)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nbk) return;
+    const unsigned lo = bucket_start[k], hi = bucket_start[k + 1];
+    for (unsigned a = lo + 1; a < hi; ++a) {  // insertion sort: buckets hold a few bodies
+        const unsigned b = order[a], m = mkey[b];
+        unsigned c = a;
+        while (c > lo && mkey[order[c - 1]] > m) {
+            order[c] = order[c - 1];
+            --c;
+        }
+        order[c] = b;
+    }
+}
+
+// Slot s: count[s] = its body's bead count, members[s * nmax + k] = the body's k-th bead
+// (kNoBead past the count; a body of more than nmax beads keeps nmax).
+constexpr unsigned kNoBead = 0xffffffffu;
+
+__global__ void k_body_slot_members(int B, int nmax, const unsigned* __restrict__ order,
+                                    const unsigned* __restrict__ start, const unsigned* __restrict__ beads,
+                                    unsigned* __restrict__ count, unsigned* __restrict__ members)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long)B * nmax) return;
+    const long s = q / nmax;
+    const int k = (int)(q - s * nmax);
+    const unsigned b = order[s], lo = start[b], n = start[b + 1] - lo;
+    if (k == 0) count[s] = n;
+    members[q] = (unsigned)k < n ? beads[lo + k] : kNoBead;
 }
 
 // Workgroup b's place in an XCD-local order: consecutive workgroups go to different XCDs
@@ -410,17 +484,23 @@ hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, uns
     if (!slots || nmax > 64) return hipErrorInvalidValue;
     // the apply's slot order first: the inverses are stored by slot
     const long nbk = block_order_buckets(s.N);
-    hipLaunchKernelGGL(k_zero_u32, dim3(blocks(nbk, 256)), dim3(256), 0, st, slots->bucket_count, nbk);
+    hipLaunchKernelGGL(k_zero_u32, dim3(blocks(nbk + 1, 256)), dim3(256), 0, st, slots->bucket_count, nbk + 1);
     hipLaunchKernelGGL(k_body_order_key, dim3(blocks(B, 256)), dim3(256), 0, st, B, s.N, body_start, body_beads,
-                       slots->bucket_count, slots->key);
-    e = exclusive_scan_u32(slots->bucket_count, slots->bucket_start, nbk, slots->scan_sums, st);
+                       slots->bucket_count, slots->key, slots->mkey);
+    // bucket_start has nbk + 1 entries: the scan's total closes the last bucket
+    e = exclusive_scan_u32(slots->bucket_count, slots->bucket_start, nbk + 1, slots->scan_sums, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_body_order_place, dim3(blocks(B, 256)), dim3(256), 0, st, B, nmax, slots->key,
-                       slots->bucket_start, body_start, body_beads, slots->order, slots->count, slots->members);
+    hipLaunchKernelGGL(k_body_order_place, dim3(blocks(B, 256)), dim3(256), 0, st, B, slots->key, slots->bucket_start,
+                       slots->order);
+    hipLaunchKernelGGL(k_body_order_sort, dim3(blocks(nbk, 256)), dim3(256), 0, st, nbk, slots->bucket_start,
+                       slots->mkey, slots->order);
+    hipLaunchKernelGGL(k_body_slot_members, dim3(blocks((long)B * nmax, 256)), dim3(256), 0, st, B, nmax,
+                       slots->order, body_start, body_beads, slots->count, slots->members);
     const unsigned wb = (unsigned)((B + 3) / 4);
+    const BlockCache cache{slots->cache_key, slots->cache_sig, slots->cache_ok};
 #define CR_BLOCK_WAVE(NB)                                                                                     \
     hipLaunchKernelGGL(k_block_setup_wave<NB>, dim3(wb), dim3(256), 0, st, B, nmax, slots->order, body_start, \
-                       body_beads, s.pos, L[0], L[1], L[2], ginv, uvec, inv_s)
+                       body_beads, s.pos, L[0], L[1], L[2], ginv, uvec, inv_s, cache)
     if (nmax <= 8) {
         CR_BLOCK_WAVE(8);
     } else if (nmax <= 16) {

@@ -158,4 +158,7 @@ def test_batch64_workload_properties(require_gpu):
         worst_sum = max(worst_sum, float(np.max(np.abs(sums - s.body_charge))))
         assert np.all(np.isfinite(cr.get_forces()))
     print(f"[batch64] worst true_relres {worst_res:.2e}, worst |sum q - Q| {worst_sum:.2e}", flush=True)
-    assert worst_res < 2e-6 and worst_sum < 1e-3
+    # GMRES stops at 1e-6 of the FP32-bulk operator's own residual; measured with the FP64
+    # operator the residual of a box lands at 1-2e-6 (the operators differ by ~1e-7 per
+    # application), and the library re-solves from zero only above 10x the tolerance
+    assert worst_res < 4e-6 and worst_sum < 1e-3
