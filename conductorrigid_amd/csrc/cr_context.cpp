@@ -1098,13 +1098,19 @@ int cr_create(cr_context** out, int device)
         delete c;
         return set_error(CR_ERR_DEVICE, "allocation of the spread scale failed");
     }
-    bool ok = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess;
+    bool ok = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&c->ev_width, hipEventDisableTiming) == hipSuccess;
     for (int k = 0; ok && k < cr_context::kSide; ++k)
         ok = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking) == hipSuccess &&
              hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         delete c;
         return set_error(CR_ERR_DEVICE, "creation of the side streams failed");
+    }
+    // the setup's ELL width read-back; without pinned memory it is a synchronous copy
+    if (hipHostMalloc((void**)&c->hwidth, sizeof(unsigned), hipHostMallocDefault) != hipSuccess) {
+        c->hwidth = nullptr;
+        (void)hipGetLastError();
     }
     // residual mailbox; without coherent host memory the solver reads back by copy
     if (hipHostMalloc((void**)&c->hpub, kPubSlots * sizeof(HostPub), hipHostMallocCoherent) != hipSuccess) {
@@ -1442,26 +1448,49 @@ int cr_compute_force(cr_context* c, const cr_system* s, double* q_inout, double*
         return set_error(CR_ERR_INVALID, "unknown x0 mode");
     double* x = c->x.as<double>();
     CR_HIP(launch_membership(s->N, s->tag, s->body_of_tag, c->iso.as<unsigned>(), st));
-    // Per-configuration setup in three independent parts, side by side: the block
-    // preconditioner's inverses (side 1), the real-space ELL operator (side 0) and, on the
-    // caller's stream, the binning, the objective and the fixed-point spread's bound.
+    // Per-configuration setup in three independent parts, side by side: the real-space ELL
+    // operator (side 0), the block preconditioner's inverses and the objective (side 1) and,
+    // on the caller's stream, the binning and the fixed-point spread's bound.  They are queued
+    // in the order of their length on the device (the host's launches take a few
+    // microseconds each): the ELL width and the binning first, the ELL build as soon as its
+    // width is back on the host, then the preconditioner.
     CR_HIP(hipEventRecord(c->ev_fork, st));
-    const bool blk = block_precond(c);
-    if (blk) {
-        CR_HIP(hipStreamWaitEvent(c->side[1], c->ev_fork, 0));
-        CR_RET(block_setup(c, v, c->side[1]));
-        CR_HIP(hipEventRecord(c->ev_join[1], c->side[1]));
+    CR_HIP(hipStreamWaitEvent(c->side[0], c->ev_fork, 0));
+    const bool csr = !c->csr_ready && v.N > 0;
+    unsigned* dwidth = reinterpret_cast<unsigned*>(c->scal.as<double>() + 32);
+    if (csr) {
+        CR_RET(csr_width(v, dwidth, c->side[0]));
+        if (c->hwidth) {
+            CR_HIP(hipMemcpyAsync(c->hwidth, dwidth, sizeof(unsigned), hipMemcpyDeviceToHost, c->side[0]));
+            CR_HIP(hipEventRecord(c->ev_width, c->side[0]));
+        }
     }
     CR_RET(bin_records(c, v, st));
-    // objective (MatrixVectorSolve, cppmodule/ConductorRigid.cu:1099-1107)
-    CR_HIP(launch_objective(v, c->mp.L, c->E0, c->b.as<double>(), st));
     if (c->fx_on() && !c->fx_bound) {
         CR_HIP(launch_spread_fx_bound(c->mp, c->tg, c->fxp, bin_view(c), c->fxs.as<FxScale>(), st));
         c->fx_bound = true;
     }
-    // the ELL width is read back (the host waits for side 0 only, the other parts run on)
-    CR_HIP(hipStreamWaitEvent(c->side[0], c->ev_fork, 0));
-    CR_RET(ensure_csr(c, v, c->side[0]));
+    if (csr) {
+        unsigned width = 0;
+        if (c->hwidth) {
+            CR_HIP(hipEventSynchronize(c->ev_width));
+            width = *c->hwidth;
+        } else {
+            CR_HIP(hipMemcpyAsync(&width, dwidth, sizeof(unsigned), hipMemcpyDeviceToHost, c->side[0]));
+            CR_HIP(hipStreamSynchronize(c->side[0]));
+        }
+        CR_RET(csr_build(c, v, width, c->side[0]));
+    }
+    const bool blk = block_precond(c);
+    hipStream_t sobj = st;
+    if (blk) {
+        CR_HIP(hipStreamWaitEvent(c->side[1], c->ev_fork, 0));
+        CR_RET(block_setup(c, v, c->side[1]));
+        sobj = c->side[1];
+    }
+    // objective (MatrixVectorSolve, cppmodule/ConductorRigid.cu:1099-1107)
+    CR_HIP(launch_objective(v, c->mp.L, c->E0, c->b.as<double>(), sobj));
+    if (blk) CR_HIP(hipEventRecord(c->ev_join[1], c->side[1]));
     CR_HIP(hipEventRecord(c->ev_join[0], c->side[0]));
     CR_HIP(hipStreamWaitEvent(st, c->ev_join[0], 0));
     if (blk) CR_HIP(hipStreamWaitEvent(st, c->ev_join[1], 0));

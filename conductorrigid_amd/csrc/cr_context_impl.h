@@ -82,6 +82,8 @@ struct cr_context {
     GmresState* g_dev = nullptr;
     GmresState* g_host = nullptr;
     HostPub* hpub = nullptr;            // coherent pinned, kPubSlots mailboxes: residuals the host polls for
+    unsigned* hwidth = nullptr;         // pinned: the ELL width read back during the setup
+    hipEvent_t ev_width = nullptr;      // ... and its arrival
     unsigned long long pub_seq = 0;
     int P_eta = 0;   // window support before the clamp to the mesh (eta uses it)
     // cr_compute_force_f32: double copies of the caller's float arrays
@@ -96,6 +98,8 @@ struct cr_context {
             if (side[k]) (void)hipStreamDestroy(side[k]);
         }
         if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_width) (void)hipEventDestroy(ev_width);
+        if (hwidth) (void)hipHostFree(hwidth);
         if (g_dev) (void)hipFree(g_dev);
         if (g_host) (void)hipHostFree(g_host);
         if (hpub) (void)hipHostFree(hpub);
