@@ -718,47 +718,26 @@ __device__ __forceinline__ void contract_fx_tile(const MeshParams& mp, const Til
     float* w = reinterpret_cast<float*>(smem + ((size_t)esz * 4 + 15) / 16 * 16);
     int2* oi = reinterpret_cast<int2*>(w + (size_t)ch * WS);
     const long plane = (long)mp.Ny * mp.Nz;
-    if (T2 % 4 == 0 && mp.Nz % 4 == 0) {
-        // rows of the tile along z: whole float4s, then the remainder nodes.  The tile's z
-        // start and Nz are multiples of 4, so a float4 group wraps past Nz as a whole.  A
-        // thread's items are loaded in batches of kFxLoadBatch before any is stored, so
-        // their memory latencies overlap instead of adding up.
-        const int nv = EZ / 4, per_row = nv + (EZ - 4 * nv), items = EX * EY * per_row;
-        for (int k0 = threadIdx.x; k0 < items; k0 += kFxLoadBatch * kFxThreads) {
-            float4 v[kFxLoadBatch];
-            int dsto[kFxLoadBatch];
-#pragma unroll
-            for (int u = 0; u < kFxLoadBatch; ++u) {
-                const int k = k0 + u * kFxThreads;
-                v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-                dsto[u] = -1;
-                if (k < items) {
-                    const int r = k / per_row, c = k - r * per_row;
-                    const int ly = r % EY, lx = r / EY;
-                    int gx = x0 + lx, gy = y0 + ly;
-                    gx -= gx >= mp.Nx ? mp.Nx : 0;
-                    gy -= gy >= mp.Ny ? mp.Ny : 0;
-                    const float* src = grid + (long)gx * plane + (long)gy * mp.Nz;
-                    if (c < nv) {
-                        int gz = z0 + 4 * c;
-                        gz -= gz >= mp.Nz ? mp.Nz : 0;
-                        v[u] = *reinterpret_cast<const float4*>(src + gz);
-                        dsto[u] = lx * xs + ly * ys + 4 * c;
-                    } else {
-                        const int z = 4 * nv + (c - nv);
-                        int gz = z0 + z;
-                        gz -= gz >= mp.Nz ? mp.Nz : 0;
-                        v[u].x = src[gz];
-                        dsto[u] = -2 - (lx * xs + ly * ys + z);  // a single node
-                    }
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < kFxLoadBatch; ++u) {
-                if (dsto[u] >= 0)
-                    *reinterpret_cast<float4*>(ext + dsto[u]) = v[u];
-                else if (dsto[u] <= -2)
-                    ext[-2 - dsto[u]] = v[u].x;
+    const bool dma = T2 % 4 == 0 && mp.Nz % 4 == 0;
+    if (dma) {
+        // rows of the tile along z as ys / 4 float4 groups each (the last group may run past
+        // the extension into the row's padding; the tile's z start and Nz are multiples of 4,
+        // so a group wraps past Nz as a whole), loaded straight into LDS by
+        // global_load_lds_dwordx4: the 64 lanes of a wavefront fill 64 consecutive 16-byte
+        // slots, so item k lands at float4 slot k of the tile without a register round trip
+        const int per_row = ys / 4, items = EX * EY * per_row;
+        const int wbase = threadIdx.x & ~(kFxWave - 1);
+        for (int k0 = 0; k0 < items; k0 += kFxThreads) {
+            const int k = k0 + (int)threadIdx.x;
+            if (k < items) {
+                const int r = k / per_row, c = k - r * per_row;
+                const int ly = r % EY, lx = r / EY;
+                int gx = x0 + lx, gy = y0 + ly, gz = z0 + 4 * c;
+                gx -= gx >= mp.Nx ? mp.Nx : 0;
+                gy -= gy >= mp.Ny ? mp.Ny : 0;
+                gz -= gz >= mp.Nz ? mp.Nz : 0;
+                __builtin_amdgcn_global_load_lds(grid + (long)gx * plane + (long)gy * mp.Nz + gz,
+                                                 ext + 4 * (k0 + wbase), 16, 0, 0);
             }
         }
     } else {
@@ -794,6 +773,7 @@ __device__ __forceinline__ void contract_fx_tile(const MeshParams& mp, const Til
         }
         for (int i = threadIdx.x; i < cnt; i += kFxThreads)
             oi[i] = make_int2(fx_ext_off(rec_off[k0 + i], xs, ys), (int)sorted[k0 + i]);
+        if (dma && k0 == kb) __builtin_amdgcn_s_waitcnt(0);  // this wavefront's tile loads have landed
         __syncthreads();  // (the first one also orders the extended tile's stores)
         // A bead's totals land in lane 63, which leaves them in LDS (the potential in its
         // record's offset slot, the force sums in its first weight slots, both read before);
