@@ -560,7 +560,7 @@ int block_alloc(cr_context* c, const SysView& v)
     CR_HIP(c->body_count.ensure(sizeof(unsigned) * (b1 + 1)));
     CR_HIP(c->body_start.ensure(sizeof(unsigned) * (b1 + 1)));
     CR_HIP(c->body_beads.ensure(sizeof(unsigned) * (size_t)std::max(1, v.N)));
-    CR_HIP(c->ginv.ensure(sizeof(float) * b1 * nmax * nmax));
+    CR_HIP(c->ginv.ensure(sizeof(float) * b1 * (size_t)block_inv_stride(nmax)));
     CR_HIP(c->uvec.ensure(sizeof(float) * b1 * nmax));
     CR_HIP(c->inv_s.ensure(sizeof(double) * b1));
     CR_HIP(c->block_scan.ensure(sizeof(unsigned long long) * ((b1 + 1) / 1024 + 4)));

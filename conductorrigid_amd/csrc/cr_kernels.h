@@ -425,6 +425,9 @@ struct BlockSlots {
     int cache_ok;
 };
 long block_order_buckets(int N);
+// Floats per slot of ginv: a block's inverse is symmetric, so only its lower triangle is
+// stored (row-major, entry (a, b) with a >= b at a (a + 1) / 2 + b), padded to whole float4s.
+__host__ __device__ constexpr long block_inv_stride(int nmax) { return ((long)nmax * (nmax + 1) / 2 + 3) / 4 * 4; }
 // Bodies grouped (body_start, body_beads); with invert, the slots and the per-slot inverses
 // (bodies of at most 64 beads).
 hipError_t launch_block_setup(const SysView& s, const double L[3], int nmax, unsigned* body_count,

@@ -22,14 +22,15 @@ namespace cr {
 
 namespace {
 
-// Entry (i, j) of body t's inverse block.  When nmax is a multiple of 4 the block is stored
-// as nmax/4 panels of 4 columns, each panel row-major over i: the 4 entries a row reads
-// with one float4 load lie next to the next row's, so the lanes of a body (one row each)
-// read each panel as one contiguous run.  Otherwise the block is row-major.
+// Entry (i, j) of body t's inverse block in the packed lower triangle (block_inv_stride).
+__device__ __forceinline__ int tri_at(int i, int j)
+{
+    const int a = i > j ? i : j, b = i > j ? j : i;
+    return a * (a + 1) / 2 + b;
+}
 __device__ __forceinline__ long ginv_at(long t, int nmax, int i, int j)
 {
-    const long base = t * nmax * nmax;
-    return (nmax & 3) == 0 ? base + ((long)(j >> 2) * nmax + i) * 4 + (j & 3) : base + (long)i * nmax + j;
+    return t * block_inv_stride(nmax) + tri_at(i, j);
 }
 
 constexpr double kPi = 3.14159265358979323846;
@@ -183,7 +184,7 @@ __device__ __forceinline__ void block_setup_body(int t, int o, int nmax, const u
 #pragma unroll
         for (int j = 0; j < NB; ++j)
             if (j < n) {
-                ginv[ginv_at(o, nmax, i, j)] = (float)a[j];
+                if (j <= i) ginv[ginv_at(o, nmax, i, j)] = (float)a[j];  // the lower triangle
                 u += a[j];
             }
         uvec[(long)o * nmax + i] = (float)u;
@@ -362,17 +363,27 @@ __global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int
                                                      double* __restrict__ w, const double* __restrict__ xq)
 {
     __shared__ double ys[256], red[256], cs[256], qs[256];
+    // R4 > 0 (nmax = 4 R4): the wavefront's K consecutive slots hold their packed triangles
+    // back to back; those float4s do not depend on y, so they go out first (overlapping the
+    // members -> y chain) and are staged in LDS, where each lane reads its row.
+    constexpr int NM = 4 * (R4 > 0 ? R4 : 1), KZ = 64 / NM, PSZ = (int)block_inv_stride(NM);
+    constexpr int NQ = KZ * PSZ / 4, NL = (NQ + 63) / 64;
+    __shared__ float4 gl[R4 > 0 ? 4 : 1][R4 > 0 ? NQ : 1];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int k = lane / nmax, i = lane - k * nmax;
-    const int slot = (xcd_local_place(blockIdx.x, gridDim.x) * 4 + wave) * K + k;
+    const int slot0 = (xcd_local_place(blockIdx.x, gridDim.x) * 4 + wave) * K;
+    const int slot = slot0 + k;
     const bool live = k < K && slot < B;
-    // R4 > 0 (nmax = 4 R4): the row of G^-1 does not depend on y, so its loads go out
-    // first and overlap the members -> y chain
-    float4 g4[R4 > 0 ? R4 : 1];
+    float4 g4[R4 > 0 ? NL : 1];
     if constexpr (R4 > 0) {
-        const float4* r4 = reinterpret_cast<const float4*>(ginv + (long)slot * nmax * nmax) + i;
+        const int nb = min(K, B - slot0);
+        const int nq = nb > 0 ? nb * (PSZ / 4) : 0;
+        const float4* src = reinterpret_cast<const float4*>(ginv + (long)slot0 * PSZ);
 #pragma unroll
-        for (int m = 0; m < R4; ++m) g4[m] = live ? r4[(long)m * nmax] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int m = 0; m < NL; ++m) {
+            const int q = lane + 64 * m;
+            g4[m] = q < nq ? src[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
     }
     const unsigned mb = live ? members[(long)slot * nmax + i] : kNoBead;
     const int n = live ? (int)count[slot] : 0;
@@ -386,34 +397,23 @@ __global__ __launch_bounds__(256) void k_block_apply(int B, int N, int nmax, int
     ys[threadIdx.x] = yi;
     red[threadIdx.x] = ui * yi;
     if (xq) qs[threadIdx.x] = act ? xq[bead] : 0.0;
+    if constexpr (R4 > 0) {
+#pragma unroll
+        for (int m = 0; m < NL; ++m)
+            if (lane + 64 * m < NQ) gl[wave][lane + 64 * m] = g4[m];
+    }
     __syncthreads();
     double gy = 0.0;
     if constexpr (R4 > 0) {
         if (act) {
+            // entries past n (and slots past B) are zeros or never written: masked
+            const float* gk = reinterpret_cast<const float*>(gl[wave]) + k * PSZ;
 #pragma unroll
-            for (int m = 0; m < R4; ++m) {
-                const int j = 4 * m;
-                if (j < n) gy += (double)g4[m].x * ys[base + j];
-                if (j + 1 < n) gy += (double)g4[m].y * ys[base + j + 1];
-                if (j + 2 < n) gy += (double)g4[m].z * ys[base + j + 2];
-                if (j + 3 < n) gy += (double)g4[m].w * ys[base + j + 3];
-            }
+            for (int j = 0; j < NM; ++j)
+                if (j < n) gy += (double)gk[tri_at(i, j)] * ys[base + j];
         }
     } else if (act) {
-        if ((nmax & 3) == 0) {
-            // four entries per load from the 4-column panels; entries past n are never
-            // written by the setup and are masked
-            const float4* r4 = reinterpret_cast<const float4*>(ginv + (long)slot * nmax * nmax) + i;
-            for (int j = 0; j < n; j += 4) {
-                const float4 g = r4[(long)(j >> 2) * nmax];
-                gy += (double)g.x * ys[base + j];
-                if (j + 1 < n) gy += (double)g.y * ys[base + j + 1];
-                if (j + 2 < n) gy += (double)g.z * ys[base + j + 2];
-                if (j + 3 < n) gy += (double)g.w * ys[base + j + 3];
-            }
-        } else {
-            for (int j = 0; j < n; ++j) gy += (double)ginv[ginv_at(slot, nmax, i, j)] * ys[base + j];
-        }
+        for (int j = 0; j < n; ++j) gy += (double)ginv[ginv_at(slot, nmax, i, j)] * ys[base + j];
     }
     if (live && i == 0) {
         double dot = 0.0;
