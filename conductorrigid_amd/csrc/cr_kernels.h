@@ -545,11 +545,12 @@ hipError_t build_nlist_noalloc(int N, const double4* pos, const double L[3], dou
 
 // ---- k_krylov.hip ----
 // Partial dot products of w with columns 0..ncols-1 of V (column pitch ld):
-// partial[b * 32 + k]; nblocks returned by krylov_blocks(n).
+// partial[k * 1024 + b] for workgroup b (column-major, so that a reduction over the blocks
+// reads one contiguous run); nblocks returned by krylov_blocks(n), at most 1024.
 int krylov_blocks(long n);
 hipError_t launch_multidot(const double* V, long ld, int ncols, const double* w, long n, double* partial,
                            hipStream_t st);
-// out[k] = sum over blocks of partial[b*32+k], k < ncols (fixed summation order)
+// out[k] = sum over blocks of partial[k * 1024 + b], k < ncols (fixed summation order)
 hipError_t launch_reduce_partials(const double* partial, int nblocks, int ncols, double* out, hipStream_t st);
 // w -= V[:,0:ncols] h; then (mode 1) partial dots V^T w, (mode 2) partial ||w||^2, (mode 0) nothing;
 // mode 3: mode 2 unless ||h[0:ncols]|| is below 1e-13 ||w|| with h[ncols] = ||w||^2, in which

@@ -43,7 +43,8 @@ __device__ __forceinline__ void block_reduce_chunk(double (&acc)[kChunk], double
     }
 }
 
-// partial[b*32 + c0 + k] = sum_{i in block b} V[c0+k][i] w[i]
+// partial[(c0 + k) * kMaxBlocks + b] = sum_{i in block b} V[c0+k][i] w[i].  The partials are
+// column-major: a reduction over the blocks of one column reads a contiguous run.
 __global__ __launch_bounds__(kBlock) void k_multidot(const double* __restrict__ V, long ld, int ncols,
                                                      const double* __restrict__ w, long n,
                                                      double* __restrict__ partial)
@@ -65,10 +66,10 @@ __global__ __launch_bounds__(kBlock) void k_multidot(const double* __restrict__ 
     __shared__ double res[kChunk];
     block_reduce_chunk(acc, res);
     if (threadIdx.x < kChunk && c0 + (int)threadIdx.x < ncols)
-        partial[(long)blockIdx.x * 32 + c0 + threadIdx.x] = res[threadIdx.x];
+        partial[(long)(c0 + threadIdx.x) * kMaxBlocks + blockIdx.x] = res[threadIdx.x];
 }
 
-// out[k] = sum_b partial[b*32 + k]: one workgroup per column; each thread sums a
+// out[k] = sum_b partial[k * kMaxBlocks + b]: one workgroup per column; each thread sums a
 // strided subset of the blocks (independent loads in flight), then a fixed-order
 // tree over the workgroup, so the result is deterministic.
 __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __restrict__ partial, int nblocks,
@@ -76,7 +77,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __rest
 {
     const int k = blockIdx.x;
     double s = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partial[(long)b * 32 + k];
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partial[(long)k * kMaxBlocks + b];
     __shared__ double red[kBlock / 64];
     s = wave_sum(s);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __rest
 // ||w|| changes w by less than that, far below what the GMRES tolerances resolve.
 constexpr double kReorthSkip = 1e-13;
 
-// w -= V[:, 0:ncols] h ; mode 2 also accumulates ||w||^2 partials into partial[b*32].
+// w -= V[:, 0:ncols] h ; mode 2 also accumulates ||w||^2 partials into partial[b] (column 0).
 // Mode 3 is mode 2 for CGS2's second correction with h[ncols] = ||w||^2 before it: when
 // ||h[0:ncols]|| <= kReorthSkip ||w|| the update is skipped and the partials carry ||w||^2.
 __global__ __launch_bounds__(kBlock) void k_gs_update(const double* __restrict__ V, long ld, int ncols,
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_update(const double* __restrict__
         for (int k = 0; k < ncols; ++k) h2 += hs[k] * hs[k];
         const double wn2 = h[ncols];
         if (h2 <= kReorthSkip * kReorthSkip * wn2) {
-            if (threadIdx.x == 0) partial[(long)blockIdx.x * 32] = blockIdx.x == 0 ? wn2 : 0.0;
+            if (threadIdx.x == 0) partial[blockIdx.x] = blockIdx.x == 0 ? wn2 : 0.0;
             return;
         }
     }
@@ -123,11 +124,11 @@ __global__ __launch_bounds__(kBlock) void k_gs_update(const double* __restrict__
     if (mode >= 2) {
         __shared__ double res[kChunk];
         block_reduce_chunk(acc, res);
-        if (threadIdx.x == 0) partial[(long)blockIdx.x * 32] = res[0];
+        if (threadIdx.x == 0) partial[blockIdx.x] = res[0];
     }
 }
 
-// w -= V[:, 0:ncols] h, fused with the next projection: partial[b*32 + k] = sum over the
+// w -= V[:, 0:ncols] h, fused with the next projection: partial[k * kMaxBlocks + b] = sum over the
 // block's elements of V[k][i] * w_new[i] (same layout as k_multidot), so CGS2's first
 // update and second projection read the basis once.  with_norm (ncols < 32): column
 // ncols also gets the block's sum of w_new[i]^2.
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_update_dot(const double* __restri
         __shared__ double res[kChunk];
         block_reduce_chunk(part, res);
         if (threadIdx.x < kChunk && c0 + (int)threadIdx.x < nout)
-            partial[(long)blockIdx.x * 32 + c0 + threadIdx.x] = res[threadIdx.x];
+            partial[(long)(c0 + threadIdx.x) * kMaxBlocks + blockIdx.x] = res[threadIdx.x];
         __syncthreads();
     }
 }
@@ -276,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void k_step_tail(const double* __restrict__
                                                       FxScale* __restrict__ fs, long nq)
 {
     double s = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partial[(long)b * 32];
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partial[b];
     __shared__ double red[kBlock / 64];
     __shared__ double tot;
     s = wave_sum(s);
