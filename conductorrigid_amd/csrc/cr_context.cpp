@@ -825,7 +825,10 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
             CR_HIP(launch_reduce_partials(partial, nblk, i + 1, g->h1, st));
             CR_HIP(launch_gs_update_dot(V, c->ld, i + 1, g->h1, w, n, partial, st, wn));
             CR_HIP(launch_reduce_partials(partial, nblk, i + 1 + (wn ? 1 : 0), g->h2, st));
-            CR_HIP(launch_gs_update(V, c->ld, i + 1, g->h2, w, n, wn ? 3 : 2, partial, st));
+            // with ||w||^2 among the projections the step tail applies the second correction
+            // and takes the norm from them; otherwise a separate pass with norm partials
+            if (!wn) CR_HIP(launch_gs_update(V, c->ld, i + 1, g->h2, w, n, 2, partial, st));
+            const double* tail_h2 = wn ? g->h2 : nullptr;
             // every continuation (this cycle's next step, or the restart) begins with A x;
             // its mesh clear is queued behind the residual's publication
             const bool go_on = iter + 1 <= maxiter;
@@ -833,7 +836,7 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
             if (!c->hpub) {
                 // ||w||, V[:, i+1], the rotations (no mailbox: the residual is copied back)
                 CR_HIP(launch_step_tail(partial, nblk, g, i, R, w, vnext, n, nullptr, 0, st, tail_qf,
-                                        c->fxs.as<FxScale>(), v.N, &tail_parts));
+                                        c->fxs.as<FxScale>(), v.N, &tail_parts, V, c->ld, i + 1, tail_h2));
                 tail_charges = tail_qf != nullptr;
                 if (go_on) CR_RET(preclear_mesh(c, st));
                 CR_RET(read_scalar(c, &g->resid, &resid, st));
@@ -848,7 +851,7 @@ int gmres(cr_context* c, const SysView& v, cr_solve_stats* stats, hipStream_t st
             // ||w||, V[:, i+1], the rotations and the residual's publication in one launch
             const unsigned long long seq = ++c->pub_seq;
             CR_HIP(launch_step_tail(partial, nblk, g, i, R, w, vnext, n, c->hpub + slot, seq, st, tail_qf,
-                                    c->fxs.as<FxScale>(), v.N, &tail_parts));
+                                    c->fxs.as<FxScale>(), v.N, &tail_parts, V, c->ld, i + 1, tail_h2));
             tail_charges = tail_qf != nullptr;
             if (go_on) CR_RET(preclear_mesh(c, st));
             if (pending) {

@@ -600,7 +600,10 @@ hipError_t launch_publish(const double* src, HostPub* dst, unsigned long long se
 // launch_charges_fx leaves them before its scale (launch_fx_scale completes it).
 hipError_t launch_step_tail(const double* partial, int nblocks, GmresState* g, int i, int R, const double* w,
                             double* vnext, long n, HostPub* pub, unsigned long long seq, hipStream_t st,
-                            float* qf = nullptr, FxScale* fs = nullptr, long nq = 0, int* nparts = nullptr);
+                            float* qf = nullptr, FxScale* fs = nullptr, long nq = 0, int* nparts = nullptr,
+                            const double* V = nullptr, long ld = 0, int ncols = 0, const double* h2 = nullptr);
+// (h2: the norm and CGS2's second correction w -= V[:, 0:ncols] h2 come from h2[0:ncols]
+// and h2[ncols] = ||w||^2 (partial unused); w itself is left as it is)
 hipError_t launch_cycle_init(GmresState* g, const double* norm2, hipStream_t st);
 hipError_t launch_backsub(GmresState* g, int R, hipStream_t st);
 

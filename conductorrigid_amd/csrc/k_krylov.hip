@@ -274,31 +274,66 @@ __global__ __launch_bounds__(kBlock) void k_step_tail(const double* __restrict__
                                                       int i, int R, const double* __restrict__ w,
                                                       double* __restrict__ vnext, long n, HostPub* pub,
                                                       unsigned long long seq, float* __restrict__ qf,
-                                                      FxScale* __restrict__ fs, long nq)
+                                                      FxScale* __restrict__ fs, long nq,
+                                                      const double* __restrict__ V, long ld, int ncols,
+                                                      const double* __restrict__ h2)
 {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partial[b];
     __shared__ double red[kBlock / 64];
     __shared__ double tot;
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int k = 0; k < kBlock / 64; ++k) t += red[k];
-        tot = t;
+    __shared__ double hs[32];
+    double wn2;
+    bool apply = false;
+    if (h2) {
+        // CGS2's second correction folded in: h2[0:ncols] = V^T w, h2[ncols] = ||w||^2; the
+        // corrected vector w - V h2 has ||w||^2 - ||h2||^2 as its squared norm (V orthonormal),
+        // and the correction is skipped as in k_gs_update's mode 3
+        if (threadIdx.x < 32) hs[threadIdx.x] = (int)threadIdx.x < ncols ? h2[threadIdx.x] : 0.0;
+        __syncthreads();
+        double hsq = 0.0;
+        for (int k = 0; k < ncols; ++k) hsq += hs[k] * hs[k];
+        const double w2 = h2[ncols];
+        apply = hsq > kReorthSkip * kReorthSkip * w2;
+        wn2 = apply ? fmax(w2 - hsq, 0.0) : w2;
+    } else {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partial[b];
+        s = wave_sum(s);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int k = 0; k < kBlock / 64; ++k) t += red[k];
+            tot = t;
+        }
+        __syncthreads();
+        wn2 = tot;
     }
-    __syncthreads();
-    const double wn2 = tot;
     const double hn = sqrt(wn2);
     const double f = (hn != 0.0) ? 1.0 / hn : 0.0;
+    // element k of the (corrected) w, subtracting in k_gs_update's order
+    auto wk = [&](long k) {
+        double v = w[k];
+        if (apply) {
+            int j = 0;
+            for (; j + 4 <= ncols; j += 4) {  // four columns' loads in flight together
+                const double a0 = V[(long)j * ld + k], a1 = V[(long)(j + 1) * ld + k];
+                const double a2 = V[(long)(j + 2) * ld + k], a3 = V[(long)(j + 3) * ld + k];
+                v -= hs[j] * a0;
+                v -= hs[j + 1] * a1;
+                v -= hs[j + 2] * a2;
+                v -= hs[j + 3] * a3;
+            }
+            for (; j < ncols; ++j) v -= hs[j] * V[(long)j * ld + k];
+        }
+        return v;
+    };
     if (!qf) {
         for (long k = (long)blockIdx.x * kBlock + threadIdx.x; k < n; k += (long)gridDim.x * kBlock)
-            vnext[k] = f * w[k];
+            vnext[k] = f * wk(k);
     } else {
         float m = 0.0f;
         for (long k = (long)blockIdx.x * kBlock + threadIdx.x; k < n; k += (long)gridDim.x * kBlock) {
-            const double v = f * w[k];
+            const double v = f * wk(k);
             vnext[k] = v;
             if (k < nq) {
                 const float q = (float)v;
@@ -389,12 +424,14 @@ hipError_t launch_reduce_partials(const double* partial, int nblocks, int ncols,
 
 hipError_t launch_step_tail(const double* partial, int nblocks, GmresState* g, int i, int R, const double* w,
                             double* vnext, long n, HostPub* pub, unsigned long long seq, hipStream_t st, float* qf,
-                            FxScale* fs, long nq, int* nparts)
+                            FxScale* fs, long nq, int* nparts, const double* V, long ld, int ncols,
+                            const double* h2)
 {
     const unsigned nb = vblocks(n);
     if (qf && (!fs || nb > (unsigned)kFxParts || nq > n)) return hipErrorInvalidValue;
+    if (h2 && (ncols < 1 || ncols >= 32 || !V)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_step_tail, dim3(nb), dim3(kBlock), 0, st, partial, nblocks, g, i, R, w, vnext, n, pub, seq,
-                       qf, fs, nq);
+                       qf, fs, nq, V, ld, ncols, h2);
     if (nparts) *nparts = (int)nb;
     return hipGetLastError();
 }
